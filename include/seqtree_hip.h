@@ -1,0 +1,140 @@
+/* seqtree_hip.h -- C-ABI of the MI355X (gfx950) all-vs-all scoring library.
+ *
+ * These entry points replace the compute of the reference's five dense-matrix functions
+ * (include/seqtree/seqtree.hpp:198-242 in seqtree 0.6.0). The C++ API layer above them keeps
+ * the reference's signatures, validation order, exceptions and messages; it encodes the
+ * strings, packs them as CSR and calls in here. Nothing but plain pointers and sizes crosses
+ * this boundary, and no exception escapes it.
+ *
+ * Sequences are passed as CSR: sequence i is bytes[offsets[i] .. offsets[i+1]), offsets has
+ * count+1 entries and offsets[0] == 0. For the gap-block and Gotoh functions the bytes are
+ * symbol codes already produced by the alphabet codec (each < alphabet_size); for Hamming and
+ * Levenshtein they are the raw, case-sensitive characters.
+ *
+ * Output is the dense row-major matrix out[i * n_refs + k] (query i against reference k),
+ * exactly as the reference's std::vector<int32_t> result.
+ *
+ * Every function returns the status code it also stores in *status (status may be NULL):
+ *   ST_OK                 success
+ *   ST_INVALID_ARGUMENT   the reference throws std::invalid_argument here (message kept)
+ *   ST_OVERFLOW           the reference throws std::overflow_error here (message kept)
+ *   ST_DEVICE_ERROR       a HIP runtime error, or no usable gfx950 device
+ *   ST_OUT_OF_MEMORY      a device or pinned host allocation failed
+ *
+ * The functions are reentrant: each call runs on its own stream (host-memory variants) or
+ * on the caller's stream (device-memory variants), with its own stream-ordered scratch.
+ */
+#ifndef SEQTREE_HIP_H
+#define SEQTREE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ST_OK = 0,
+    ST_INVALID_ARGUMENT = 1,
+    ST_OVERFLOW = 2,
+    ST_DEVICE_ERROR = 3,
+    ST_OUT_OF_MEMORY = 4
+};
+
+typedef struct st_status {
+    int32_t code;
+    char message[256];
+} st_status;
+
+/* ---- host-memory entry points (the drop-in path) ------------------------------------ */
+
+/* Single-gap-block penalty matrix.
+ * Replaces the compute of seqtree::gapblock_matrix (src/gapblock.cpp:66-133; per-pair
+ * `cell` at src/gapblock.cpp:39-62). pen is the alphabet_size x alphabet_size penalty table
+ * indexed [query code][ref code] (src/gapblock.cpp:25-31); prior is NULL or the
+ * (prior_width+1)^3 cube indexed [longer length][block length][block start]. The caller has
+ * already validated gap costs, symbols, the prior's size and that every sequence is at most
+ * prior_width long (src/gapblock.cpp:71-102). */
+int32_t st_gapblock_matrix(const uint8_t* q_codes, const uint64_t* q_offsets, uint64_t n_queries,
+                           const uint8_t* r_codes, const uint64_t* r_offsets, uint64_t n_refs,
+                           const int32_t* pen, uint32_t alphabet_size, int32_t gap_open,
+                           int32_t gap_extend, const int32_t* prior, uint32_t prior_width,
+                           int32_t* out, st_status* status);
+
+/* Gotoh affine Needleman-Wunsch (local == 0) or Smith-Waterman (local != 0) similarity
+ * matrix, or with dist != 0 the distance d = s(a,a) + s(b,b) - 2 s(a,b).
+ * Replaces the compute of seqtree::align_score_matrix / align_dist_matrix
+ * (src/pairwise.cpp:273-352; per-pair DP score_encoded at src/pairwise.cpp:76-125). sim is
+ * the signed similarity table indexed [query code][ref code]. Returns ST_OVERFLOW with the
+ * reference's narrow() message (src/pairwise.cpp:42-49) if a result does not fit int32. */
+int32_t st_align_matrix(const uint8_t* q_codes, const uint64_t* q_offsets, uint64_t n_queries,
+                        const uint8_t* r_codes, const uint64_t* r_offsets, uint64_t n_refs,
+                        const int32_t* sim, uint32_t alphabet_size, int32_t local, int32_t dist,
+                        int32_t gap_open, int32_t gap_extend, int32_t* out, st_status* status);
+
+/* Unit-cost Hamming distance over raw bytes. Replaces the compute of seqtree::hamming_matrix
+ * (src/distance.cpp:29-76,97-102). Returns ST_INVALID_ARGUMENT with the reference's message
+ * for the first length-mismatched pair in row-major order. */
+int32_t st_hamming_matrix(const uint8_t* a, const uint64_t* a_offsets, uint64_t n_a,
+                          const uint8_t* b, const uint64_t* b_offsets, uint64_t n_b, int32_t* out,
+                          st_status* status);
+
+/* Unit-cost Levenshtein distance over raw bytes. Replaces the compute of
+ * seqtree::levenshtein_matrix (src/distance.cpp:29-64,78-95,104-109). */
+int32_t st_levenshtein_matrix(const uint8_t* a, const uint64_t* a_offsets, uint64_t n_a,
+                              const uint8_t* b, const uint64_t* b_offsets, uint64_t n_b,
+                              int32_t* out, st_status* status);
+
+/* ---- device-memory entry points ------------------------------------------------------
+ * Same computations with the sequences (CSR bytes and offsets) and the output already in
+ * device memory; the small tables (pen, sim, prior) stay host pointers. The work is enqueued
+ * on `stream` (a hipStream_t, NULL for the null stream). q_max_len / r_max_len must bound
+ * the longest sequence of each side. A call may synchronise `stream` once to read back a few
+ * counters; its scratch is stream-ordered and released on `stream`. These are the same
+ * kernels the host-memory entry points run, without the host<->device copies. */
+int32_t st_gapblock_matrix_device(const uint8_t* q_codes, const uint64_t* q_offsets,
+                                  uint64_t n_queries, uint32_t q_max_len, const uint8_t* r_codes,
+                                  const uint64_t* r_offsets, uint64_t n_refs, uint32_t r_max_len,
+                                  const int32_t* pen, uint32_t alphabet_size, int32_t gap_open,
+                                  int32_t gap_extend, const int32_t* prior, uint32_t prior_width,
+                                  int32_t* out, void* stream, st_status* status);
+
+int32_t st_align_matrix_device(const uint8_t* q_codes, const uint64_t* q_offsets,
+                               uint64_t n_queries, uint32_t q_max_len, const uint8_t* r_codes,
+                               const uint64_t* r_offsets, uint64_t n_refs, uint32_t r_max_len,
+                               const int32_t* sim, uint32_t alphabet_size, int32_t local,
+                               int32_t dist, int32_t gap_open, int32_t gap_extend, int32_t* out,
+                               void* stream, st_status* status);
+
+int32_t st_hamming_matrix_device(const uint8_t* a, const uint64_t* a_offsets, uint64_t n_a,
+                                 uint32_t a_max_len, const uint8_t* b, const uint64_t* b_offsets,
+                                 uint64_t n_b, uint32_t b_max_len, int32_t* out, void* stream,
+                                 st_status* status);
+
+int32_t st_levenshtein_matrix_device(const uint8_t* a, const uint64_t* a_offsets, uint64_t n_a,
+                                     uint32_t a_max_len, const uint8_t* b,
+                                     const uint64_t* b_offsets, uint64_t n_b, uint32_t b_max_len,
+                                     int32_t* out, void* stream, st_status* status);
+
+/* ---- library information ------------------------------------------------------------- */
+
+/* Version string of this library. */
+const char* st_version(void);
+
+/* Number of visible HIP devices (0 when none); never faults on a machine without a GPU. */
+int32_t st_device_count(void);
+
+/* ---- instrumentation (no counterpart in the reference; used by the benchmark) ---------
+ * st_timing_enable(1) makes every kernel launch record a pair of hipEvents on the stream it
+ * runs on. st_timing_collect waits for the recorded events and writes one line per kernel
+ * label, "label launches total_ms", into report (NUL-terminated, truncated to capacity),
+ * then forgets them. It returns the number of labels, or minus the number of unresolved
+ * event pairs. */
+void st_timing_enable(int32_t on);
+int32_t st_timing_collect(char* report, uint64_t capacity);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SEQTREE_HIP_H */

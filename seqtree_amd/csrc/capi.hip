@@ -1,0 +1,812 @@
+// C-ABI entry points of libseqtree_hip.so (declared in include/seqtree_hip.h).
+//
+// Each host-memory entry point copies the CSR inputs to the device, runs the same device
+// path as the *_device entry point on a private stream, and copies the matrix back. Scratch
+// is stream-ordered (hipMallocAsync / hipFreeAsync on the call's stream), so concurrent
+// calls never share buffers. Integer-width choices are made here on the host from the input
+// bounds (a range proof), never by trying and retrying on the device.
+
+#include "st_internal.hpp"
+#include "../../include/seqtree_hip.h"
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace {
+
+using namespace st;
+
+constexpr const char* kVersion = "seqtree_hip 0.1.0 (gfx950)";
+
+void set_status(st_status* st, int code, const char* fmt, ...) {
+    if (!st) return;
+    st->code = code;
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(st->message, sizeof(st->message), fmt, ap);
+    va_end(ap);
+}
+
+int ok(st_status* st) {
+    if (st) {
+        st->code = ST_OK;
+        st->message[0] = 0;
+    }
+    return ST_OK;
+}
+
+struct HipError {
+    int code;
+    std::string msg;
+};
+
+void check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return;
+    const int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+                         ? ST_OUT_OF_MEMORY
+                         : ST_DEVICE_ERROR;
+    throw HipError{code, std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+// Checks once per process that device 0 is reachable; the code object is gfx950 only.
+void require_device() {
+    static std::once_flag once;
+    static int state = 0;  // 1 ok, -1 no device, -2 wrong arch
+    static std::string arch;
+    std::call_once(once, [] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+            state = -1;
+            return;
+        }
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, dev) != hipSuccess) {
+            state = -1;
+            return;
+        }
+        arch = p.gcnArchName;
+        state = arch.rfind("gfx950", 0) == 0 ? 1 : -2;
+    });
+    if (state == -1) throw HipError{ST_DEVICE_ERROR, "seqtree_hip: no HIP device is available"};
+    if (state == -2)
+        throw HipError{ST_DEVICE_ERROR,
+                       "seqtree_hip: device is " + arch + "; this library is built for gfx950"};
+}
+
+bool pools_supported() {
+    static std::once_flag once;
+    static bool yes = false;
+    std::call_once(once, [] {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMemoryPoolsSupported, dev) == hipSuccess)
+            yes = v != 0;
+    });
+    return yes;
+}
+
+// Stream-ordered scratch for one call. Everything allocated here is released on the same
+// stream when the arena goes out of scope (after a stream sync when pools are unavailable).
+class Arena {
+public:
+    explicit Arena(hipStream_t s) : s_(s) {}
+    ~Arena() {
+        if (bufs_.empty()) return;
+        if (pools_supported()) {
+            for (void* p : bufs_) (void)hipFreeAsync(p, s_);
+        } else {
+            (void)hipStreamSynchronize(s_);
+            for (void* p : bufs_) (void)hipFree(p);
+        }
+    }
+    template <class T>
+    T* get(size_t count) {
+        void* p = nullptr;
+        const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+        if (pools_supported())
+            check(hipMallocAsync(&p, bytes, s_), "hipMallocAsync");
+        else
+            check(hipMalloc(&p, bytes), "hipMalloc");
+        bufs_.push_back(p);
+        return static_cast<T*>(p);
+    }
+
+private:
+    hipStream_t s_;
+    std::vector<void*> bufs_;
+};
+
+class OwnedStream {
+public:
+    OwnedStream() { check(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking), "stream"); }
+    ~OwnedStream() {
+        (void)hipStreamSynchronize(s_);
+        (void)hipStreamDestroy(s_);
+    }
+    hipStream_t get() const { return s_; }
+
+private:
+    hipStream_t s_ = nullptr;
+};
+
+template <class T>
+void h2d(T* dst, const T* src, size_t count, hipStream_t s) {
+    if (count) check(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, s), "H2D");
+}
+
+template <class T>
+void d2h(T* dst, const T* src, size_t count, hipStream_t s) {
+    if (count) check(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToHost, s), "D2H");
+}
+
+void sync(hipStream_t s) { check(hipStreamSynchronize(s), "hipStreamSynchronize"); }
+
+// ---- optional launch timing (st_timing_enable / st_timing_collect) ----------------------
+// When enabled, every kernel launch below is bracketed by two hipEvents recorded on the
+// stream the kernel runs on; st_timing_collect resolves them into per-label totals.
+struct Pending {
+    std::string label;
+    hipEvent_t start, stop;
+};
+std::mutex g_timing_mu;
+bool g_timing = false;
+std::vector<Pending> g_pending;
+
+class LaunchTimer {
+public:
+    LaunchTimer(std::string label, hipStream_t s) : s_(s) {
+        std::lock_guard<std::mutex> lk(g_timing_mu);
+        if (!g_timing) return;
+        on_ = hipEventCreate(&a_) == hipSuccess && hipEventCreate(&b_) == hipSuccess &&
+              hipEventRecord(a_, s_) == hipSuccess;
+        label_ = std::move(label);
+    }
+    ~LaunchTimer() {
+        if (!on_) return;
+        (void)hipEventRecord(b_, s_);
+        std::lock_guard<std::mutex> lk(g_timing_mu);
+        g_pending.push_back({label_, a_, b_});
+    }
+
+private:
+    hipStream_t s_;
+    bool on_ = false;
+    hipEvent_t a_{}, b_{};
+    std::string label_;
+};
+
+void launched(const char* what) { check(hipGetLastError(), what); }
+
+uint32_t words_for(uint32_t max_len) { return (max_len + 3) / 4; }
+
+// Packs the reference side (see st_internal.hpp for the layout).
+PackedRefs pack_refs(Arena& ar, const uint8_t* bytes, const uint64_t* off, uint64_t n,
+                     uint32_t max_len, hipStream_t s) {
+    LaunchTimer t("pack_refs", s);
+    PackedRefs p{};
+    p.nchunks = uint32_t((n + kChunk - 1) / kChunk);
+    p.npad = p.nchunks * kChunk;
+    p.nwords = words_for(max_len);
+    uint32_t* words = ar.get<uint32_t>(size_t(p.nwords + 1) * p.npad);
+    uint32_t* len = ar.get<uint32_t>(p.npad);
+    uint32_t* orig = ar.get<uint32_t>(p.npad);
+    launch_pack_refs(bytes, off, n, p.nwords, p.npad, words, len, orig, s);
+    launched("pack_refs");
+    p.words = words;
+    p.len = len;
+    p.orig = orig;
+    return p;
+}
+
+// Buckets queries by length; returns host copies of the class counts (one sync).
+std::vector<uint32_t> classify(Arena& ar, const uint64_t* off, uint64_t n, const ClassBounds& b,
+                               uint32_t** lists, hipStream_t s) {
+    uint32_t* counts = ar.get<uint32_t>(kMaxClasses);
+    *lists = ar.get<uint32_t>(size_t(b.count) * n);
+    launch_fill_u32(counts, kMaxClasses, 0, s);
+    launch_classify(off, n, b, counts, *lists, s);
+    launched("classify");
+    std::vector<uint32_t> h(kMaxClasses);
+    d2h(h.data(), counts, kMaxClasses, s);
+    sync(s);
+    return h;
+}
+
+int64_t iabs64(int64_t v) { return v < 0 ? -v : v; }
+
+// ------------------------------------------------------------------------------------------
+// gap-block
+// ------------------------------------------------------------------------------------------
+void gapblock_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, uint32_t q_max,
+                     const uint8_t* r, const uint64_t* r_off, uint64_t K, uint32_t r_max,
+                     const int32_t* pen_h, uint32_t A, int32_t go, int32_t ge,
+                     const int32_t* prior_h, uint32_t W, int32_t* out, hipStream_t s) {
+    if (N == 0 || K == 0) return;
+    Arena ar(s);
+    int32_t* pen = ar.get<int32_t>(size_t(A) * A);
+    h2d(pen, pen_h, size_t(A) * A, s);
+    const size_t w1 = size_t(W) + 1;
+    int32_t* prior = nullptr;
+    int64_t prior_abs = 0;
+    if (prior_h) {
+        const size_t cube = w1 * w1 * w1;
+        prior = ar.get<int32_t>(cube);
+        h2d(prior, prior_h, cube, s);
+        for (size_t e = 0; e < cube; ++e) prior_abs = std::max(prior_abs, iabs64(prior_h[e]));
+    }
+    int64_t pen_abs = 0;
+    for (size_t e = 0; e < size_t(A) * A; ++e) pen_abs = std::max(pen_abs, iabs64(pen_h[e]));
+
+    GapblockArgs a{};
+    a.q_codes = q;
+    a.q_off = q_off;
+    a.n_queries = N;
+    a.n_refs = K;
+    a.refs = pack_refs(ar, r, r_off, K, r_max, s);
+    a.pen = pen;
+    a.A = A;
+    a.gap_open = go;
+    a.gap_extend = ge;
+    a.prior = prior;
+    a.w1 = uint32_t(w1);
+    a.out = out;
+
+    // No int32 sum can wrap below this bound, so the one-pass minimum is bit-identical to
+    // the reference's suffix/prefix evaluation.
+    const int64_t Lmax = std::max(q_max, r_max);
+    const int64_t worst = 2 * Lmax * pen_abs + prior_abs + int64_t(go) + Lmax * int64_t(ge);
+    const bool exact = worst >= int64_t(INT32_MAX);
+    const bool profile = size_t(q_max) * A * sizeof(int32_t) <= 48 * 1024;
+    {
+        LaunchTimer t(std::string("gapblock") + (profile ? "_profile" : "_table") +
+                          (exact ? "_exact" : ""), s);
+        launch_gapblock(a, profile, q_max, exact, s);
+    }
+    launched("gapblock");
+}
+
+// ------------------------------------------------------------------------------------------
+// Gotoh
+// ------------------------------------------------------------------------------------------
+constexpr int kGotohRows[] = {8, 16, 24, 32, 40, 48, 56, 64};
+constexpr int kGotohClasses = 8;
+
+void gotoh_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, uint32_t q_max,
+                  const uint8_t* r, const uint64_t* r_off, uint64_t K, uint32_t r_max,
+                  const int32_t* sim_h, uint32_t A, int local, int dist, int32_t go, int32_t ge,
+                  int32_t* out, hipStream_t s, bool* overflow_out) {
+    *overflow_out = false;
+    if (N == 0 || K == 0) return;
+    Arena ar(s);
+    std::vector<int32_t> simT_h(size_t(A) * A);
+    int64_t mx = std::max<int64_t>(go, ge);
+    for (uint32_t x = 0; x < A; ++x)
+        for (uint32_t y = 0; y < A; ++y) {
+            simT_h[size_t(y) * A + x] = sim_h[size_t(x) * A + y];
+            mx = std::max(mx, iabs64(sim_h[size_t(x) * A + y]));
+        }
+    int32_t* sim = ar.get<int32_t>(size_t(A) * A);
+    int32_t* simT = ar.get<int32_t>(size_t(A) * A);
+    h2d(sim, sim_h, size_t(A) * A, s);
+    h2d(simT, simT_h.data(), size_t(A) * A, s);
+    int32_t* flag = ar.get<int32_t>(1);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(flag), 1, 0, s);
+
+    GotohArgs a{};
+    a.q_codes = q;
+    a.q_off = q_off;
+    a.r_codes = r;
+    a.r_off = r_off;
+    a.n_queries = N;
+    a.n_refs = K;
+    a.sim = sim;
+    a.simT = simT;
+    a.A = A;
+    a.local = local;
+    a.dist = dist;
+    a.gap_open = go;
+    a.gap_extend = ge;
+    a.out = out;
+    a.overflow = flag;
+
+    // Per-thread int64 scratch for the generic kernels: 6 rows over the shorter sequence.
+    const uint64_t shorter = std::min(q_max, r_max);
+    const uint64_t self_len = std::max(q_max, r_max);
+    auto slots_for = [](uint64_t stride) {
+        const uint64_t per = 6 * stride * sizeof(int64_t);
+        uint64_t slots = (uint64_t(1) << 30) / std::max<uint64_t>(per, 1);
+        slots = std::min<uint64_t>(slots, 65536);
+        slots = std::max<uint64_t>(slots / 256 * 256, 256);
+        return uint32_t(slots);
+    };
+
+    if (dist) {
+        const uint64_t stride = self_len + 1;
+        const uint32_t slots = slots_for(stride);
+        int64_t* scratch = ar.get<int64_t>(size_t(slots) * 6 * stride);
+        int64_t* qs = ar.get<int64_t>(N);
+        int64_t* rs = ar.get<int64_t>(K);
+        LaunchTimer t("gotoh_self", s);
+        launch_gotoh_self(a, q, q_off, N, qs, scratch, slots, stride, s);
+        launch_gotoh_self(a, r, r_off, K, rs, scratch, slots, stride, s);
+        launched("gotoh_self");
+        a.q_self = qs;
+        a.r_self = rs;
+    }
+
+    // Range proof for the int32 register kernel: with R <= 64 stripe rows (dummies
+    // included) every reachable cell is a partial-alignment score bounded by
+    // (64 + r_max) * mx; that bound, the -2^30 sentinel and one gap step below either must
+    // stay inside int32. A distance adds at most 4 * (q_max + r_max) * mx.
+    const int64_t B = (64 + int64_t(r_max)) * mx;
+    bool fast = B + 2 * std::max<int64_t>(go, ge) < (int64_t(1) << 30);
+    if (dist) fast = fast && 4 * (int64_t(q_max) + int64_t(r_max)) * mx < int64_t(INT32_MAX);
+
+    ClassBounds cb{};
+    uint32_t* lists = nullptr;
+    std::vector<uint32_t> counts;
+    if (fast) {
+        cb.count = kGotohClasses + 1;
+        for (int c = 0; c < kGotohClasses; ++c) cb.bound[c] = uint32_t(kGotohRows[c]);
+        cb.bound[kGotohClasses] = 0xFFFFFFFFu;
+    } else {
+        cb.count = 1;
+        cb.bound[0] = 0xFFFFFFFFu;
+    }
+    counts = classify(ar, q_off, N, cb, &lists, s);
+
+    bool generic_ran = false;
+    if (fast) {
+        a.refs = pack_refs(ar, r, r_off, K, r_max, s);
+        for (int c = 0; c < kGotohClasses; ++c) {
+            if (counts[c] == 0) continue;
+            {
+                LaunchTimer t("gotoh_fast_r" + std::to_string(kGotohRows[c]) +
+                                  (local ? "_local" : "_global"), s);
+                launch_gotoh_fast(a, kGotohRows[c], lists + size_t(c) * N, counts[c], s);
+            }
+            launched("gotoh_fast");
+        }
+    }
+    const int gen_class = cb.count - 1;
+    if (counts[gen_class] > 0) {
+        const uint64_t stride = shorter + 1;
+        const uint32_t slots = slots_for(stride);
+        int64_t* scratch = ar.get<int64_t>(size_t(slots) * 6 * stride);
+        {
+            LaunchTimer t("gotoh_generic", s);
+            launch_gotoh_generic(a, lists + size_t(gen_class) * N, counts[gen_class], scratch,
+                                 slots, stride, s);
+        }
+        launched("gotoh_generic");
+        generic_ran = true;
+    }
+    if (generic_ran) {
+        int32_t h = 0;
+        d2h(&h, flag, 1, s);
+        sync(s);
+        *overflow_out = h != 0;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Hamming / Levenshtein
+// ------------------------------------------------------------------------------------------
+void hamming_device(const uint8_t* a_b, const uint64_t* a_off, uint64_t N, uint32_t a_max,
+                    const uint8_t* b_b, const uint64_t* b_off, uint64_t K, uint32_t b_max,
+                    int32_t* out, hipStream_t s) {
+    if (N == 0 || K == 0) return;
+    Arena ar(s);
+    const uint32_t L = std::max(a_max, b_max);  // all lengths are equal by now
+    const uint32_t nw = words_for(L);
+    DistanceArgs d{};
+    d.a_bytes = a_b;
+    d.a_off = a_off;
+    d.b_bytes = b_b;
+    d.b_off = b_off;
+    d.n_a = N;
+    d.n_b = K;
+    d.out = out;
+    d.refs = pack_refs(ar, b_b, b_off, K, L, s);
+    uint32_t* aw = ar.get<uint32_t>(size_t(N) * std::max<uint32_t>(nw, 1));
+    launch_pack_rows(a_b, a_off, N, nw, aw, s);
+    {
+        LaunchTimer t("hamming", s);
+        launch_hamming(d, aw, nw, s);
+    }
+    launched("hamming");
+}
+
+void levenshtein_device(const uint8_t* a_b, const uint64_t* a_off, uint64_t N, uint32_t a_max,
+                        const uint8_t* b_b, const uint64_t* b_off, uint64_t K, uint32_t b_max,
+                        int32_t* out, hipStream_t s) {
+    if (N == 0 || K == 0) return;
+    Arena ar(s);
+    DistanceArgs d{};
+    d.a_bytes = a_b;
+    d.a_off = a_off;
+    d.b_bytes = b_b;
+    d.b_off = b_off;
+    d.n_a = N;
+    d.n_b = K;
+    d.out = out;
+    ClassBounds cb{};
+    cb.count = 5;
+    cb.bound[0] = 64;
+    cb.bound[1] = 128;
+    cb.bound[2] = 192;
+    cb.bound[3] = 256;
+    cb.bound[4] = 0xFFFFFFFFu;
+    uint32_t* lists = nullptr;
+    const std::vector<uint32_t> counts = classify(ar, a_off, N, cb, &lists, s);
+    if (counts[0] + counts[1] + counts[2] + counts[3] > 0) {
+        d.refs = pack_refs(ar, b_b, b_off, K, b_max, s);
+        for (int c = 0; c < 4; ++c) {
+            if (counts[c] == 0) continue;
+            {
+                LaunchTimer t("levenshtein_myers_w" + std::to_string(c + 1), s);
+                launch_levenshtein_myers(d, c + 1, lists + size_t(c) * N, counts[c], s);
+            }
+            launched("levenshtein_myers");
+        }
+    }
+    if (counts[4] > 0) {
+        const uint64_t stride = uint64_t(std::min(a_max, b_max)) + 1;
+        uint64_t slots = (uint64_t(1) << 30) / (2 * stride * sizeof(int32_t));
+        slots = std::max<uint64_t>(std::min<uint64_t>(slots, 65536) / 256 * 256, 256);
+        int32_t* scratch = ar.get<int32_t>(size_t(slots) * 2 * stride);
+        LaunchTimer t("levenshtein_generic", s);
+        launch_levenshtein_generic(d, lists + size_t(4) * N, counts[4], scratch, uint32_t(slots),
+                                   stride, s);
+        launched("levenshtein_generic");
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// host-memory plumbing
+// ------------------------------------------------------------------------------------------
+struct HostSet {
+    const uint8_t* bytes;
+    const uint64_t* off;
+    uint64_t n;
+    uint32_t max_len;
+    uint64_t total;
+};
+
+HostSet host_set(const uint8_t* bytes, const uint64_t* off, uint64_t n) {
+    HostSet h{bytes, off, n, 0, n ? off[n] : 0};
+    for (uint64_t i = 0; i < n; ++i) h.max_len = std::max<uint32_t>(h.max_len, uint32_t(off[i + 1] - off[i]));
+    return h;
+}
+
+struct DeviceSet {
+    const uint8_t* bytes;
+    const uint64_t* off;
+};
+
+DeviceSet upload(Arena& ar, const HostSet& h, hipStream_t s) {
+    uint8_t* b = ar.get<uint8_t>(std::max<uint64_t>(h.total, 1));
+    uint64_t* o = ar.get<uint64_t>(h.n + 1);
+    h2d(b, h.bytes, h.total, s);
+    h2d(o, h.off, h.n + 1, s);
+    return {b, o};
+}
+
+// Runs `body(q_dev_subset, row0, rows, out_dev)` over blocks of query rows whose output fits a
+// device budget, copying each block of the matrix back to `out`.
+template <class Body>
+void run_rows(const HostSet& q, uint64_t K, int32_t* out, hipStream_t s, Arena& ar,
+              const DeviceSet& qd, Body&& body) {
+    const uint64_t budget = uint64_t(8) << 30;  // bytes of int32 output per block
+    const uint64_t rows_per = std::max<uint64_t>(1, budget / std::max<uint64_t>(K * 4, 1));
+    const uint64_t cap = std::min<uint64_t>(rows_per, q.n);
+    int32_t* dout = ar.get<int32_t>(size_t(cap) * K);
+    for (uint64_t r0 = 0; r0 < q.n; r0 += cap) {
+        const uint64_t rows = std::min(cap, q.n - r0);
+        body(qd.off + r0, r0, rows, dout);
+        d2h(out + r0 * K, dout, size_t(rows) * K, s);
+        sync(s);
+    }
+}
+
+template <class F>
+int guarded(st_status* st, F&& f) {
+    try {
+        return f();
+    } catch (const HipError& e) {
+        set_status(st, e.code, "%s", e.msg.c_str());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_status(st, ST_OUT_OF_MEMORY, "seqtree_hip: host allocation failed");
+        return ST_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        set_status(st, ST_DEVICE_ERROR, "seqtree_hip: %s", e.what());
+        return ST_DEVICE_ERROR;
+    }
+}
+
+const char* overflow_message(bool dist) {
+    return dist ? "seqtree: the alignment distance does not fit in 32 bits -- the gap costs are "
+                  "absurdly large for these sequences"
+                : "seqtree: the alignment score does not fit in 32 bits -- the gap costs are "
+                  "absurdly large for these sequences";
+}
+
+int bad_alphabet(st_status* st, uint32_t A) {
+    set_status(st, ST_INVALID_ARGUMENT, "seqtree_hip: alphabet size %u is outside 1..32", A);
+    return ST_INVALID_ARGUMENT;
+}
+
+// First length-mismatched (i, k) in row-major order, or false.
+bool first_mismatch(const HostSet& a, const HostSet& b, uint64_t* i, uint64_t* k) {
+    if (a.n == 0 || b.n == 0) return false;
+    const uint64_t la0 = a.off[1] - a.off[0];
+    for (uint64_t t = 0; t < b.n; ++t)
+        if (b.off[t + 1] - b.off[t] != la0) {
+            *i = 0;
+            *k = t;
+            return true;
+        }
+    for (uint64_t t = 0; t < a.n; ++t)
+        if (a.off[t + 1] - a.off[t] != la0) {
+            *i = t;
+            *k = 0;
+            return true;
+        }
+    return false;
+}
+
+int hamming_mismatch(st_status* st, uint64_t la, uint64_t lb) {
+    set_status(st, ST_INVALID_ARGUMENT,
+               "seqtree: hamming distance is only defined for equal-length sequences (got %llu "
+               "and %llu)",
+               (unsigned long long)la, (unsigned long long)lb);
+    return ST_INVALID_ARGUMENT;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* st_version(void) { return kVersion; }
+
+void st_timing_enable(int32_t on) {
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    g_timing = on != 0;
+}
+
+int32_t st_timing_collect(char* report, uint64_t capacity) {
+    std::vector<Pending> done;
+    {
+        std::lock_guard<std::mutex> lk(g_timing_mu);
+        done.swap(g_pending);
+    }
+    std::map<std::string, std::pair<double, int>> acc;
+    int bad = 0;
+    for (auto& p : done) {
+        float ms = 0.f;
+        if (hipEventSynchronize(p.stop) == hipSuccess &&
+            hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {
+            auto& e = acc[p.label];
+            e.first += ms;
+            e.second += 1;
+        } else {
+            ++bad;
+        }
+        (void)hipEventDestroy(p.start);
+        (void)hipEventDestroy(p.stop);
+    }
+    std::string out;
+    for (auto& kv : acc) {
+        char line[256];
+        std::snprintf(line, sizeof(line), "%s %d %.6f\n", kv.first.c_str(), kv.second.second,
+                      kv.second.first);
+        out += line;
+    }
+    if (report && capacity) {
+        std::strncpy(report, out.c_str(), capacity - 1);
+        report[capacity - 1] = 0;
+    }
+    return bad ? -bad : int32_t(acc.size());
+}
+
+int32_t st_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int32_t st_gapblock_matrix_device(const uint8_t* q_codes, const uint64_t* q_offsets,
+                                  uint64_t n_queries, uint32_t q_max_len, const uint8_t* r_codes,
+                                  const uint64_t* r_offsets, uint64_t n_refs, uint32_t r_max_len,
+                                  const int32_t* pen, uint32_t alphabet_size, int32_t gap_open,
+                                  int32_t gap_extend, const int32_t* prior, uint32_t prior_width,
+                                  int32_t* out, void* stream, st_status* status) {
+    if (alphabet_size == 0 || alphabet_size > kMaxAlphabet) return bad_alphabet(status, alphabet_size);
+    return guarded(status, [&] {
+        require_device();
+        gapblock_device(q_codes, q_offsets, n_queries, q_max_len, r_codes, r_offsets, n_refs,
+                        r_max_len, pen, alphabet_size, gap_open, gap_extend, prior, prior_width,
+                        out, static_cast<hipStream_t>(stream));
+        return ok(status);
+    });
+}
+
+int32_t st_gapblock_matrix(const uint8_t* q_codes, const uint64_t* q_offsets, uint64_t n_queries,
+                           const uint8_t* r_codes, const uint64_t* r_offsets, uint64_t n_refs,
+                           const int32_t* pen, uint32_t alphabet_size, int32_t gap_open,
+                           int32_t gap_extend, const int32_t* prior, uint32_t prior_width,
+                           int32_t* out, st_status* status) {
+    if (alphabet_size == 0 || alphabet_size > kMaxAlphabet) return bad_alphabet(status, alphabet_size);
+    if (n_queries == 0 || n_refs == 0) return ok(status);
+    return guarded(status, [&] {
+        require_device();
+        const HostSet qh = host_set(q_codes, q_offsets, n_queries);
+        const HostSet rh = host_set(r_codes, r_offsets, n_refs);
+        OwnedStream os;
+        const hipStream_t s = os.get();
+        Arena ar(s);
+        const DeviceSet qd = upload(ar, qh, s);
+        const DeviceSet rd = upload(ar, rh, s);
+        run_rows(qh, n_refs, out, s, ar, qd,
+                 [&](const uint64_t* qoff, uint64_t, uint64_t rows, int32_t* dout) {
+                     gapblock_device(qd.bytes, qoff, rows, qh.max_len, rd.bytes, rd.off, n_refs,
+                                     rh.max_len, pen, alphabet_size, gap_open, gap_extend, prior,
+                                     prior_width, dout, s);
+                 });
+        return ok(status);
+    });
+}
+
+int32_t st_align_matrix_device(const uint8_t* q_codes, const uint64_t* q_offsets,
+                               uint64_t n_queries, uint32_t q_max_len, const uint8_t* r_codes,
+                               const uint64_t* r_offsets, uint64_t n_refs, uint32_t r_max_len,
+                               const int32_t* sim, uint32_t alphabet_size, int32_t local,
+                               int32_t dist, int32_t gap_open, int32_t gap_extend, int32_t* out,
+                               void* stream, st_status* status) {
+    if (alphabet_size == 0 || alphabet_size > kMaxAlphabet) return bad_alphabet(status, alphabet_size);
+    return guarded(status, [&] {
+        require_device();
+        bool overflow = false;
+        gotoh_device(q_codes, q_offsets, n_queries, q_max_len, r_codes, r_offsets, n_refs,
+                     r_max_len, sim, alphabet_size, local, dist, gap_open, gap_extend, out,
+                     static_cast<hipStream_t>(stream), &overflow);
+        if (overflow) {
+            set_status(status, ST_OVERFLOW, "%s", overflow_message(dist != 0));
+            return int(ST_OVERFLOW);
+        }
+        return ok(status);
+    });
+}
+
+int32_t st_align_matrix(const uint8_t* q_codes, const uint64_t* q_offsets, uint64_t n_queries,
+                        const uint8_t* r_codes, const uint64_t* r_offsets, uint64_t n_refs,
+                        const int32_t* sim, uint32_t alphabet_size, int32_t local, int32_t dist,
+                        int32_t gap_open, int32_t gap_extend, int32_t* out, st_status* status) {
+    if (alphabet_size == 0 || alphabet_size > kMaxAlphabet) return bad_alphabet(status, alphabet_size);
+    if (n_queries == 0 || n_refs == 0) return ok(status);
+    return guarded(status, [&] {
+        require_device();
+        const HostSet qh = host_set(q_codes, q_offsets, n_queries);
+        const HostSet rh = host_set(r_codes, r_offsets, n_refs);
+        OwnedStream os;
+        const hipStream_t s = os.get();
+        Arena ar(s);
+        const DeviceSet qd = upload(ar, qh, s);
+        const DeviceSet rd = upload(ar, rh, s);
+        bool any_overflow = false;
+        run_rows(qh, n_refs, out, s, ar, qd,
+                 [&](const uint64_t* qoff, uint64_t, uint64_t rows, int32_t* dout) {
+                     bool overflow = false;
+                     gotoh_device(qd.bytes, qoff, rows, qh.max_len, rd.bytes, rd.off, n_refs,
+                                  rh.max_len, sim, alphabet_size, local, dist, gap_open,
+                                  gap_extend, dout, s, &overflow);
+                     any_overflow = any_overflow || overflow;
+                 });
+        if (any_overflow) {
+            set_status(status, ST_OVERFLOW, "%s", overflow_message(dist != 0));
+            return int(ST_OVERFLOW);
+        }
+        return ok(status);
+    });
+}
+
+int32_t st_hamming_matrix_device(const uint8_t* a, const uint64_t* a_offsets, uint64_t n_a,
+                                 uint32_t a_max_len, const uint8_t* b, const uint64_t* b_offsets,
+                                 uint64_t n_b, uint32_t b_max_len, int32_t* out, void* stream,
+                                 st_status* status) {
+    return guarded(status, [&] {
+        require_device();
+        if (n_a == 0 || n_b == 0) return ok(status);
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        Arena ar(s);
+        uint64_t* res = ar.get<uint64_t>(2);
+        launch_fill_u32(reinterpret_cast<uint32_t*>(res), 4, 0xFFFFFFFFu, s);
+        launch_hamming_check(a_offsets, n_a, b_offsets, n_b, res, s);
+        launched("hamming_check");
+        uint64_t h[2];
+        d2h(h, res, 2, s);
+        sync(s);
+        if (h[0] != ~0ull || h[1] != ~0ull) {
+            const uint64_t i = h[0] != ~0ull ? 0 : h[1];
+            const uint64_t k = h[0] != ~0ull ? h[0] : 0;
+            uint64_t ao[2], bo[2];
+            d2h(ao, a_offsets + i, 2, s);
+            d2h(bo, b_offsets + k, 2, s);
+            sync(s);
+            return hamming_mismatch(status, ao[1] - ao[0], bo[1] - bo[0]);
+        }
+        hamming_device(a, a_offsets, n_a, a_max_len, b, b_offsets, n_b, b_max_len, out, s);
+        return ok(status);
+    });
+}
+
+int32_t st_hamming_matrix(const uint8_t* a, const uint64_t* a_offsets, uint64_t n_a,
+                          const uint8_t* b, const uint64_t* b_offsets, uint64_t n_b, int32_t* out,
+                          st_status* status) {
+    const HostSet ah = host_set(a, a_offsets, n_a);
+    const HostSet bh = host_set(b, b_offsets, n_b);
+    uint64_t i = 0, k = 0;
+    if (first_mismatch(ah, bh, &i, &k))
+        return hamming_mismatch(status, a_offsets[i + 1] - a_offsets[i],
+                                b_offsets[k + 1] - b_offsets[k]);
+    if (n_a == 0 || n_b == 0) return ok(status);
+    return guarded(status, [&] {
+        require_device();
+        OwnedStream os;
+        const hipStream_t s = os.get();
+        Arena ar(s);
+        const DeviceSet ad = upload(ar, ah, s);
+        const DeviceSet bd = upload(ar, bh, s);
+        run_rows(ah, n_b, out, s, ar, ad,
+                 [&](const uint64_t* aoff, uint64_t, uint64_t rows, int32_t* dout) {
+                     hamming_device(ad.bytes, aoff, rows, ah.max_len, bd.bytes, bd.off, n_b,
+                                    bh.max_len, dout, s);
+                 });
+        return ok(status);
+    });
+}
+
+int32_t st_levenshtein_matrix_device(const uint8_t* a, const uint64_t* a_offsets, uint64_t n_a,
+                                     uint32_t a_max_len, const uint8_t* b,
+                                     const uint64_t* b_offsets, uint64_t n_b, uint32_t b_max_len,
+                                     int32_t* out, void* stream, st_status* status) {
+    return guarded(status, [&] {
+        require_device();
+        levenshtein_device(a, a_offsets, n_a, a_max_len, b, b_offsets, n_b, b_max_len, out,
+                           static_cast<hipStream_t>(stream));
+        return ok(status);
+    });
+}
+
+int32_t st_levenshtein_matrix(const uint8_t* a, const uint64_t* a_offsets, uint64_t n_a,
+                              const uint8_t* b, const uint64_t* b_offsets, uint64_t n_b,
+                              int32_t* out, st_status* status) {
+    if (n_a == 0 || n_b == 0) return ok(status);
+    return guarded(status, [&] {
+        require_device();
+        const HostSet ah = host_set(a, a_offsets, n_a);
+        const HostSet bh = host_set(b, b_offsets, n_b);
+        OwnedStream os;
+        const hipStream_t s = os.get();
+        Arena ar(s);
+        const DeviceSet ad = upload(ar, ah, s);
+        const DeviceSet bd = upload(ar, bh, s);
+        run_rows(ah, n_b, out, s, ar, ad,
+                 [&](const uint64_t* aoff, uint64_t, uint64_t rows, int32_t* dout) {
+                     levenshtein_device(ad.bytes, aoff, rows, ah.max_len, bd.bytes, bd.off, n_b,
+                                        bh.max_len, dout, s);
+                 });
+        return ok(status);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,238 @@
+// The reference's scoring functions, with their validation order, exception types and
+// messages, in front of the GPU C-ABI (seqtree_hip.h). Host work is limited to validation,
+// encoding and CSR packing; all scoring happens on the device.
+//
+// Reference behaviour followed here:
+//   gapblock_matrix     src/gapblock.cpp:12-22 (encode), 66-103 (checks, order, early return)
+//   align_*_matrix      src/pairwise.cpp:51-68 (encode, check), 273-286, 338-352
+//   align_score         src/pairwise.cpp:131-139
+//   hamming/levenshtein src/distance.cpp:68-109
+#include "seqtree.hpp"
+#include "seqtree_hip.h"
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+namespace seqtree {
+
+namespace {
+
+// Strings packed back to back with N+1 offsets, the form the C-ABI takes.
+struct Csr {
+    std::vector<uint8_t> bytes;
+    std::vector<uint64_t> off;
+    uint32_t max_len = 0;
+};
+
+template <class Enc>
+Csr pack(const std::vector<std::string>& v, Enc&& encode_char) {
+    Csr c;
+    size_t total = 0;
+    for (const auto& s : v) total += s.size();
+    c.bytes.resize(total);
+    c.off.resize(v.size() + 1);
+    size_t p = 0;
+    for (size_t i = 0; i < v.size(); ++i) {
+        c.off[i] = p;
+        const std::string& s = v[i];
+        for (size_t t = 0; t < s.size(); ++t) c.bytes[p + t] = encode_char(s[t]);
+        p += s.size();
+        c.max_len = std::max<uint32_t>(c.max_len, uint32_t(s.size()));
+    }
+    c.off[v.size()] = p;
+    return c;
+}
+
+Csr pack_raw(const std::vector<std::string>& v) {
+    return pack(v, [](char ch) { return static_cast<uint8_t>(ch); });
+}
+
+// Encoding for gap-block: "symbol 'x' is not in the alphabet" (gapblock.cpp:12-22).
+Csr pack_gapblock(const std::vector<std::string>& v, const Codec& codec) {
+    return pack(v, [&](char ch) {
+        const uint8_t code = codec.encode(ch);
+        if (code == Codec::kInvalid)
+            throw std::invalid_argument(std::string("symbol '") + ch + "' is not in the alphabet");
+        return code;
+    });
+}
+
+// Encoding for the Gotoh paths: "invalid symbol 'x' in sequence" (pairwise.cpp:51-60).
+Csr pack_pairwise(const std::vector<std::string>& v, const Codec& codec) {
+    return pack(v, [&](char ch) {
+        const uint8_t code = codec.encode(ch);
+        if (code == Codec::kInvalid)
+            throw std::invalid_argument(std::string("invalid symbol '") + ch + "' in sequence");
+        return code;
+    });
+}
+
+void raise(int code, const st_status& st) {
+    switch (code) {
+        case ST_OK: return;
+        case ST_INVALID_ARGUMENT: throw std::invalid_argument(st.message);
+        case ST_OVERFLOW: throw std::overflow_error(st.message);
+        case ST_OUT_OF_MEMORY: throw std::bad_alloc();
+        default: throw std::runtime_error(st.message);
+    }
+}
+
+void check_pairwise(const SubstitutionMatrix& mat, const Codec& codec, int32_t open,
+                    int32_t extend) {
+    if (open < 0 || extend < 0)
+        throw std::invalid_argument("seqtree: gap_open and gap_extend must be >= 0 (they are "
+                                    "magnitudes, and are subtracted)");
+    if (mat.size() != codec.size())
+        throw std::invalid_argument("matrix size does not match the alphabet");
+}
+
+std::vector<int32_t> pairwise_matrix(const std::vector<std::string>& queries,
+                                     const std::vector<std::string>& refs,
+                                     const SubstitutionMatrix& mat, Alphabet alphabet,
+                                     AlignMode mode, int32_t go, int32_t ge, bool dist) {
+    const Codec codec(alphabet);
+    check_pairwise(mat, codec, go, ge);
+    const size_t N = queries.size(), K = refs.size();
+    std::vector<int32_t> out(N * K);
+    if (N == 0 || K == 0) return out;
+    const Csr q = pack_pairwise(queries, codec);
+    const Csr r = pack_pairwise(refs, codec);
+    st_status st{};
+    const int rc = st_align_matrix(q.bytes.data(), q.off.data(), N, r.bytes.data(), r.off.data(),
+                                   K, mat.similarities().data(), mat.size(),
+                                   mode == AlignMode::Local, dist, go, ge, out.data(), &st);
+    raise(rc, st);
+    return out;
+}
+
+}  // namespace
+
+int32_t align_score(std::string_view query, std::string_view ref, const SubstitutionMatrix& mat,
+                    Alphabet alphabet, AlignMode mode, int32_t gap_open, int32_t gap_extend) {
+    const Codec codec(alphabet);
+    check_pairwise(mat, codec, gap_open, gap_extend);
+    const Csr q = pack_pairwise({std::string(query)}, codec);
+    const Csr r = pack_pairwise({std::string(ref)}, codec);
+    int32_t out = 0;
+    st_status st{};
+    const int rc = st_align_matrix(q.bytes.data(), q.off.data(), 1, r.bytes.data(), r.off.data(), 1,
+                                   mat.similarities().data(), mat.size(),
+                                   mode == AlignMode::Local, 0, gap_open, gap_extend, &out, &st);
+    raise(rc, st);
+    return out;
+}
+
+std::vector<int32_t> align_score_matrix(const std::vector<std::string>& queries,
+                                        const std::vector<std::string>& refs,
+                                        const SubstitutionMatrix& mat, Alphabet alphabet,
+                                        AlignMode mode, int32_t gap_open, int32_t gap_extend,
+                                        int /*threads*/) {
+    return pairwise_matrix(queries, refs, mat, alphabet, mode, gap_open, gap_extend, false);
+}
+
+std::vector<int32_t> align_dist_matrix(const std::vector<std::string>& queries,
+                                       const std::vector<std::string>& refs,
+                                       const SubstitutionMatrix& mat, Alphabet alphabet,
+                                       AlignMode mode, int32_t gap_open, int32_t gap_extend,
+                                       int /*threads*/) {
+    return pairwise_matrix(queries, refs, mat, alphabet, mode, gap_open, gap_extend, true);
+}
+
+std::vector<int32_t> gapblock_matrix(const std::vector<std::string>& queries,
+                                     const std::vector<std::string>& refs, Alphabet alphabet,
+                                     const SubstitutionMatrix* matrix, int32_t gap_open,
+                                     int32_t gap_extend, const std::vector<int32_t>& prior,
+                                     uint32_t prior_width, int /*threads*/) {
+    if (gap_open < 0 || gap_extend < 0)
+        throw std::invalid_argument("gap_open and gap_extend must be >= 0");
+    const Codec codec(alphabet);
+    const uint8_t A = codec.size();
+    if (matrix && matrix->size() != A)
+        throw std::invalid_argument("matrix size does not match the alphabet");
+
+    const size_t N = queries.size(), K = refs.size();
+    std::vector<int32_t> out(N * K);
+    if (N == 0 || K == 0) return out;
+
+    std::vector<int32_t> pen(size_t(A) * A);
+    for (uint32_t a = 0; a < A; ++a)
+        for (uint32_t b = 0; b < A; ++b)
+            pen[a * A + b] = matrix ? matrix->penalty(uint8_t(a), uint8_t(b)) : int32_t(a != b);
+
+    const Csr q = pack_gapblock(queries, codec);
+    const Csr r = pack_gapblock(refs, codec);
+    const uint32_t longest = std::max(q.max_len, r.max_len);
+
+    const size_t W1 = size_t(prior_width) + 1;
+    if (!prior.empty()) {
+        if (prior.size() != W1 * W1 * W1)
+            throw std::invalid_argument("prior table must have (prior_width + 1)^3 entries");
+        if (longest > prior_width)
+            throw std::invalid_argument("a sequence is longer than the prior table's width");
+    }
+
+    st_status st{};
+    const int rc = st_gapblock_matrix(q.bytes.data(), q.off.data(), N, r.bytes.data(),
+                                      r.off.data(), K, pen.data(), A, gap_open, gap_extend,
+                                      prior.empty() ? nullptr : prior.data(), prior_width,
+                                      out.data(), &st);
+    raise(rc, st);
+    return out;
+}
+
+int hamming(std::string_view a, std::string_view b) {
+    if (a.size() != b.size())
+        throw std::invalid_argument("seqtree: hamming distance is only defined for equal-length "
+                                    "sequences (got " + std::to_string(a.size()) + " and " +
+                                    std::to_string(b.size()) + ")");
+    const Csr x = pack_raw({std::string(a)});
+    const Csr y = pack_raw({std::string(b)});
+    int32_t out = 0;
+    st_status st{};
+    raise(st_hamming_matrix(x.bytes.data(), x.off.data(), 1, y.bytes.data(), y.off.data(), 1, &out,
+                            &st),
+          st);
+    return out;
+}
+
+int levenshtein(std::string_view a, std::string_view b) {
+    const Csr x = pack_raw({std::string(a)});
+    const Csr y = pack_raw({std::string(b)});
+    int32_t out = 0;
+    st_status st{};
+    raise(st_levenshtein_matrix(x.bytes.data(), x.off.data(), 1, y.bytes.data(), y.off.data(), 1,
+                                &out, &st),
+          st);
+    return out;
+}
+
+std::vector<int32_t> hamming_matrix(const std::vector<std::string>& a,
+                                    const std::vector<std::string>& b, int /*threads*/) {
+    const size_t N = a.size(), K = b.size();
+    std::vector<int32_t> out(N * K);
+    if (N == 0 || K == 0) return out;
+    const Csr x = pack_raw(a);
+    const Csr y = pack_raw(b);
+    st_status st{};
+    raise(st_hamming_matrix(x.bytes.data(), x.off.data(), N, y.bytes.data(), y.off.data(), K,
+                            out.data(), &st),
+          st);
+    return out;
+}
+
+std::vector<int32_t> levenshtein_matrix(const std::vector<std::string>& a,
+                                        const std::vector<std::string>& b, int /*threads*/) {
+    const size_t N = a.size(), K = b.size();
+    std::vector<int32_t> out(N * K);
+    if (N == 0 || K == 0) return out;
+    const Csr x = pack_raw(a);
+    const Csr y = pack_raw(b);
+    st_status st{};
+    raise(st_levenshtein_matrix(x.bytes.data(), x.off.data(), N, y.bytes.data(), y.off.data(), K,
+                                out.data(), &st),
+          st);
+    return out;
+}
+
+}  // namespace seqtree

@@ -1,0 +1,260 @@
+// pybind11 module `_core`: the reference's hot-path bindings (src/_bindings.cpp:34-40
+// parse_alphabet, 203-233 ScoreMatrix and gapblock_matrix, 240-315 SubstitutionMatrix,
+// 346-350 alphabet helpers, 470-615 ScoreMatrix buffer protocol and the matrix functions)
+// with the same names, argument names, defaults and error types, over the GPU C++ API.
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <algorithm>
+#include <optional>
+#include <string>
+#include <vector>
+
+#include "seqtree.hpp"
+#include "seqtree_hip.h"
+
+namespace py = pybind11;
+using namespace seqtree;
+
+namespace {
+
+std::string lower(std::string s) {
+    std::transform(s.begin(), s.end(), s.begin(), [](unsigned char c) { return std::tolower(c); });
+    return s;
+}
+
+Alphabet parse_alphabet(const std::string& a) {
+    const std::string l = lower(a);
+    if (l == "aa" || l == "amino" || l == "protein") return Alphabet::AminoAcid;
+    if (l == "nt" || l == "dna" || l == "nucleotide") return Alphabet::Nucleotide;
+    if (l == "nt_iupac" || l == "iupac") return Alphabet::NucleotideIUPAC;
+    throw py::value_error("unknown alphabet '" + a + "' (use 'aa', 'nt', or 'iupac')");
+}
+
+AlignMode parse_align_mode(const std::string& m) {
+    const std::string l = lower(m);
+    if (l == "global" || l == "nw" || l == "needleman-wunsch") return AlignMode::Global;
+    if (l == "local" || l == "sw" || l == "smith-waterman") return AlignMode::Local;
+    throw py::value_error("unknown mode '" + m + "' (use 'global' or 'local')");
+}
+
+std::string alphabet_symbols(Alphabet a) {
+    const Codec c(a);
+    std::string s;
+    for (uint8_t i = 0; i < c.size(); ++i) s += c.decode(i);
+    return s;
+}
+
+// Dense (rows, cols) int32 block lent to Python through the buffer protocol.
+struct ScoreMatrix {
+    std::vector<int32_t> data;
+    size_t rows = 0, cols = 0;
+};
+
+// Index of a single amino-acid character in codec order, checked against the matrix size.
+uint8_t residue(const SubstitutionMatrix& m, const std::string& a, const char* what) {
+    if (a.size() != 1)
+        throw py::value_error(std::string(what) + "() takes two single amino-acid characters");
+    static const std::string aa = alphabet_symbols(Alphabet::AminoAcid);
+    const auto i = aa.find(a[0]);
+    if (i == std::string::npos) throw py::value_error("unknown amino acid; expected one of " + aa);
+    if (i >= m.size()) throw py::value_error("residue out of range for this matrix's alphabet");
+    return uint8_t(i);
+}
+
+template <class F>
+ScoreMatrix make_matrix(size_t rows, size_t cols, F&& compute) {
+    ScoreMatrix out;
+    out.rows = rows;
+    out.cols = cols;
+    {
+        py::gil_scoped_release release;
+        out.data = compute();
+    }
+    return out;
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_core, m) {
+    m.doc() = "seqtree all-vs-all scoring on AMD Instinct MI355X (gfx950 HIP kernels)";
+
+    py::class_<SubstitutionMatrix>(m, "SubstitutionMatrix",
+                                   "Substitution matrix with a signed similarity view and a "
+                                   "non-negative Gram penalty view (penalty(a,a) == 0).")
+        .def_static("blosum62", &SubstitutionMatrix::blosum62)
+        .def_static("blosum45", &SubstitutionMatrix::blosum45)
+        .def_static("blosum80", &SubstitutionMatrix::blosum80)
+        .def_static("pam250", &SubstitutionMatrix::pam250)
+        .def_static("pam100", &SubstitutionMatrix::pam100)
+        .def_static("structural", &SubstitutionMatrix::structural)
+        .def_static("unit", &SubstitutionMatrix::unit, py::arg("size"))
+        .def_static(
+            "from_similarity",
+            [](const std::vector<std::vector<int32_t>>& grid) {
+                const size_t n = grid.size();
+                if (n == 0 || n > 32) throw py::value_error("matrix size must be 1..32");
+                std::vector<int32_t> flat;
+                flat.reserve(n * n);
+                for (const auto& row : grid) {
+                    if (row.size() != n) throw py::value_error("similarity matrix must be square");
+                    flat.insert(flat.end(), row.begin(), row.end());
+                }
+                return SubstitutionMatrix::from_similarity(uint8_t(n), flat.data());
+            },
+            py::arg("grid"),
+            "Build from a square similarity grid in the alphabet's symbol order; penalties are "
+            "s[a,a] + s[b,b] - 2*s[a,b] clamped at 0.")
+        .def("size", &SubstitutionMatrix::size)
+        .def(
+            "similarity",
+            [](const SubstitutionMatrix& self, const std::string& a, const std::string& b) {
+                return self.similarity(residue(self, a, "similarity"),
+                                       residue(self, b, "similarity"));
+            },
+            py::arg("a"), py::arg("b"), "Raw signed similarity of two amino acids.")
+        .def("scale", &SubstitutionMatrix::scale,
+             "Median penalty over all mismatched symbol pairs (the matrix's natural unit).")
+        .def(
+            "penalty",
+            [](const SubstitutionMatrix& self, const std::string& a, const std::string& b) {
+                return self.penalty(residue(self, a, "penalty"), residue(self, b, "penalty"));
+            },
+            py::arg("a"), py::arg("b"), "Gram-distance penalty of two amino acids.")
+        .def("__repr__", [](const SubstitutionMatrix& s) {
+            return "SubstitutionMatrix(size=" + std::to_string(s.size()) + ")";
+        });
+
+    m.def("alphabet_symbols",
+          [](const std::string& a) { return alphabet_symbols(parse_alphabet(a)); },
+          py::arg("alphabet") = "aa", "Symbols of an alphabet in codec (matrix) order.");
+    m.def("amino_acids", [] { return alphabet_symbols(Alphabet::AminoAcid); },
+          "The 24 amino-acid symbols in codec (matrix) order.");
+
+    py::class_<ScoreMatrix>(m, "ScoreMatrix", py::buffer_protocol(),
+                            "A read-only (n_queries, n_refs) int32 matrix, row-major. "
+                            "numpy.asarray(sm) and memoryview(sm) wrap it without copying.")
+        .def_buffer([](ScoreMatrix& s) {
+            return py::buffer_info(s.data.data(), sizeof(int32_t),
+                                   py::format_descriptor<int32_t>::format(), 2, {s.rows, s.cols},
+                                   {sizeof(int32_t) * s.cols, sizeof(int32_t)});
+        })
+        .def_property_readonly("shape",
+                               [](const ScoreMatrix& s) { return py::make_tuple(s.rows, s.cols); })
+        .def("__len__", [](const ScoreMatrix& s) { return s.rows; })
+        .def(
+            "row",
+            [](const ScoreMatrix& s, size_t i) {
+                if (i >= s.rows) throw py::index_error("row out of range");
+                return std::vector<int32_t>(s.data.begin() + i * s.cols,
+                                            s.data.begin() + (i + 1) * s.cols);
+            },
+            py::arg("i"), "Row i as a list, one value per reference.")
+        .def("__getitem__",
+             [](const ScoreMatrix& s, std::pair<size_t, size_t> ik) {
+                 if (ik.first >= s.rows || ik.second >= s.cols)
+                     throw py::index_error("index out of range");
+                 return s.data[ik.first * s.cols + ik.second];
+             })
+        .def("__repr__", [](const ScoreMatrix& s) {
+            return "ScoreMatrix(" + std::to_string(s.rows) + ", " + std::to_string(s.cols) + ")";
+        });
+
+    m.def(
+        "align_score",
+        [](const std::string& q, const std::string& r, const SubstitutionMatrix& mat,
+           const std::string& mode, int32_t gap_open, int32_t gap_extend,
+           const std::string& alphabet) {
+            const Alphabet a = parse_alphabet(alphabet);
+            const AlignMode md = parse_align_mode(mode);
+            py::gil_scoped_release release;
+            return align_score(q, r, mat, a, md, gap_open, gap_extend);
+        },
+        py::arg("query"), py::arg("ref"), py::arg("matrix"), py::arg("mode") = "global",
+        py::arg("gap_open") = 11, py::arg("gap_extend") = 1, py::arg("alphabet") = "aa",
+        "Optimal Gotoh similarity score ('global' Needleman-Wunsch, 'local' Smith-Waterman).");
+
+    m.def(
+        "align_score_matrix",
+        [](const std::vector<std::string>& q, const std::vector<std::string>& r,
+           const SubstitutionMatrix& mat, const std::string& mode, int32_t gap_open,
+           int32_t gap_extend, const std::string& alphabet, int threads) {
+            const Alphabet a = parse_alphabet(alphabet);
+            const AlignMode md = parse_align_mode(mode);
+            return make_matrix(q.size(), r.size(), [&] {
+                return align_score_matrix(q, r, mat, a, md, gap_open, gap_extend, threads);
+            });
+        },
+        py::arg("queries"), py::arg("refs"), py::arg("matrix"), py::arg("mode") = "global",
+        py::arg("gap_open") = 11, py::arg("gap_extend") = 1, py::arg("alphabet") = "aa",
+        py::arg("threads") = 0, "Dense (n_queries, n_refs) similarity matrix, GIL released.");
+
+    m.def(
+        "align_dist_matrix",
+        [](const std::vector<std::string>& q, const std::vector<std::string>& r,
+           const SubstitutionMatrix& mat, const std::string& mode, int32_t gap_open,
+           int32_t gap_extend, const std::string& alphabet, int threads) {
+            const Alphabet a = parse_alphabet(alphabet);
+            const AlignMode md = parse_align_mode(mode);
+            return make_matrix(q.size(), r.size(), [&] {
+                return align_dist_matrix(q, r, mat, a, md, gap_open, gap_extend, threads);
+            });
+        },
+        py::arg("queries"), py::arg("refs"), py::arg("matrix"), py::arg("mode") = "global",
+        py::arg("gap_open") = 11, py::arg("gap_extend") = 1, py::arg("alphabet") = "aa",
+        py::arg("threads") = 0,
+        "Dense distance matrix d = s(a,a) + s(b,b) - 2*s(a,b), GIL released.");
+
+    m.def(
+        "gapblock_matrix",
+        [](const std::vector<std::string>& queries, const std::vector<std::string>& refs,
+           const std::string& alphabet, const std::optional<SubstitutionMatrix>& matrix,
+           int32_t gap_open, int32_t gap_extend, const std::vector<int32_t>& prior,
+           uint32_t prior_width, int threads) {
+            const Alphabet a = parse_alphabet(alphabet);
+            return make_matrix(queries.size(), refs.size(), [&] {
+                return gapblock_matrix(queries, refs, a, matrix ? &*matrix : nullptr, gap_open,
+                                       gap_extend, prior, prior_width, threads);
+            });
+        },
+        py::arg("queries"), py::arg("refs"), py::arg("alphabet") = "aa",
+        py::arg("matrix") = std::nullopt, py::arg("gap_open") = 1, py::arg("gap_extend") = 1,
+        py::arg("prior") = std::vector<int32_t>{}, py::arg("prior_width") = 0,
+        py::arg("threads") = 0,
+        "Exhaustive single-gap-block penalties for every (query, ref) pair, GIL released.");
+
+    m.def(
+        "hamming",
+        [](const std::string& a, const std::string& b) {
+            py::gil_scoped_release release;
+            return hamming(a, b);
+        },
+        py::arg("a"), py::arg("b"),
+        "Hamming distance of two equal-length sequences (ValueError otherwise), byte-wise.");
+    m.def(
+        "levenshtein",
+        [](const std::string& a, const std::string& b) {
+            py::gil_scoped_release release;
+            return levenshtein(a, b);
+        },
+        py::arg("a"), py::arg("b"), "Unit-cost edit distance, byte-wise and case-sensitive.");
+
+    m.def(
+        "hamming_matrix",
+        [](const std::vector<std::string>& a, const std::vector<std::string>& b, int threads) {
+            return make_matrix(a.size(), b.size(), [&] { return hamming_matrix(a, b, threads); });
+        },
+        py::arg("a"), py::arg("b"), py::arg("threads") = 0,
+        "Dense (len(a), len(b)) int32 Hamming matrix, GIL released.");
+    m.def(
+        "levenshtein_matrix",
+        [](const std::vector<std::string>& a, const std::vector<std::string>& b, int threads) {
+            return make_matrix(a.size(), b.size(),
+                               [&] { return levenshtein_matrix(a, b, threads); });
+        },
+        py::arg("a"), py::arg("b"), py::arg("threads") = 0,
+        "Dense (len(a), len(b)) int32 Levenshtein matrix, GIL released.");
+
+    m.def("version", [] { return std::string(st_version()); }, "Native library version.");
+    m.def("device_count", [] { return st_device_count(); }, "Visible HIP devices.");
+}

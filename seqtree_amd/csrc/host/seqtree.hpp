@@ -1,0 +1,103 @@
+// C++ API of the MI355X scoring path: the same declarations, argument meaning, validation
+// order and exceptions as the reference's include/seqtree/seqtree.hpp (Codec :15-31,
+// SubstitutionMatrix :43-75, AlignMode :186, the scalar and matrix functions :188-242) and
+// include/seqtree/types.hpp:9 (Alphabet), restricted to the all-vs-all scoring path.
+//
+// The dense-matrix functions run on the GPU through the C-ABI in seqtree_hip.h. Their
+// `threads` argument is accepted for signature compatibility and ignored: the host does only
+// validation and encoding, and the device does the scoring.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <string_view>
+#include <vector>
+
+namespace seqtree {
+
+enum class Alphabet : uint8_t { AminoAcid, Nucleotide, NucleotideIUPAC };
+
+// Maps characters to small integer codes (< 32) and back, case-insensitively on input.
+// Characters outside the alphabet encode to kInvalid.
+class Codec {
+public:
+    static constexpr uint8_t kInvalid = 0xFF;
+    explicit Codec(Alphabet);
+    uint8_t size() const { return size_; }
+    Alphabet alphabet() const { return alphabet_; }
+    uint8_t encode(char c) const { return enc_[static_cast<uint8_t>(c)]; }
+    char decode(uint8_t code) const { return dec_[code]; }
+
+private:
+    Alphabet alphabet_;
+    uint8_t size_ = 0;
+    uint8_t enc_[256];
+    char dec_[32];
+};
+
+// A substitution matrix in two views: the signed similarity (maximised by Gotoh) and the
+// non-negative Gram penalty pen(a,b) = max(0, s(a,a) + s(b,b) - 2 s(a,b)) (minimised by the
+// gap-block score). unit(A) is the identity: penalty 0/1, similarity 1/0.
+class SubstitutionMatrix {
+public:
+    static SubstitutionMatrix unit(uint8_t size);
+    static SubstitutionMatrix blosum62();
+    static SubstitutionMatrix blosum45();
+    static SubstitutionMatrix blosum80();
+    static SubstitutionMatrix pam250();
+    static SubstitutionMatrix pam100();
+    static SubstitutionMatrix structural();
+    static SubstitutionMatrix from_similarity(uint8_t size, const int32_t* sim);
+
+    uint8_t size() const { return size_; }
+    int32_t penalty(uint8_t a, uint8_t b) const { return pen_[a * size_ + b]; }
+    int32_t similarity(uint8_t a, uint8_t b) const { return sim_[a * size_ + b]; }
+    // Median penalty over the mismatched symbol pairs.
+    int32_t scale() const;
+
+    const std::vector<int32_t>& penalties() const { return pen_; }
+    const std::vector<int32_t>& similarities() const { return sim_; }
+
+private:
+    uint8_t size_ = 0;
+    std::vector<int32_t> pen_;
+    std::vector<int32_t> sim_;
+};
+
+enum class AlignMode { Global, Local };
+
+// Optimal Gotoh score of one pair (int64 DP semantics, narrowed to int32 or
+// std::overflow_error).
+int32_t align_score(std::string_view query, std::string_view ref, const SubstitutionMatrix&,
+                    Alphabet, AlignMode, int32_t gap_open, int32_t gap_extend);
+
+// Dense N*K similarity matrix, row-major.
+std::vector<int32_t> align_score_matrix(const std::vector<std::string>& queries,
+                                        const std::vector<std::string>& refs,
+                                        const SubstitutionMatrix&, Alphabet, AlignMode,
+                                        int32_t gap_open, int32_t gap_extend, int threads = 0);
+
+// Dense N*K distance matrix d(a,b) = s(a,a) + s(b,b) - 2*s(a,b).
+std::vector<int32_t> align_dist_matrix(const std::vector<std::string>& queries,
+                                       const std::vector<std::string>& refs,
+                                       const SubstitutionMatrix&, Alphabet, AlignMode,
+                                       int32_t gap_open, int32_t gap_extend, int threads = 0);
+
+// Dense N*K single-gap-block penalty matrix, row-major. `prior` is the gap prior flattened to
+// [m][d][i] with stride prior_width + 1, or empty; `matrix` may be null for unit cost.
+std::vector<int32_t> gapblock_matrix(const std::vector<std::string>& queries,
+                                     const std::vector<std::string>& refs, Alphabet,
+                                     const SubstitutionMatrix* matrix, int32_t gap_open,
+                                     int32_t gap_extend, const std::vector<int32_t>& prior,
+                                     uint32_t prior_width, int threads = 0);
+
+// Unit-cost distances on raw, case-sensitive bytes.
+int hamming(std::string_view a, std::string_view b);
+int levenshtein(std::string_view a, std::string_view b);
+
+std::vector<int32_t> hamming_matrix(const std::vector<std::string>& a,
+                                    const std::vector<std::string>& b, int threads = 0);
+std::vector<int32_t> levenshtein_matrix(const std::vector<std::string>& a,
+                                        const std::vector<std::string>& b, int threads = 0);
+
+}  // namespace seqtree

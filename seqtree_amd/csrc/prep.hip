@@ -1,0 +1,110 @@
+// Per-call preparation kernels: reference packing (chunk-local length sort + transposed
+// word layout) and query bucketing by length.
+
+#include "st_internal.hpp"
+
+namespace st {
+namespace {
+
+constexpr uint32_t kPackThreads = 512;  // kChunk / 2: one compare-exchange per thread per stage
+
+// One workgroup per chunk of kChunk references. Sorts the chunk's slots by (length, index)
+// with a bitonic network in LDS, then writes len/orig and the residues of each slot as
+// little-endian words, word-major, so scoring waves read 64 consecutive slots per load.
+__global__ __launch_bounds__(kPackThreads) void pack_refs_kernel(
+    const uint8_t* __restrict__ bytes, const uint64_t* __restrict__ off, uint64_t n,
+    uint32_t nwords, uint32_t npad, uint32_t* __restrict__ words, uint32_t* __restrict__ len_out,
+    uint32_t* __restrict__ orig_out) {
+    __shared__ uint64_t keys[kChunk];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t base = uint64_t(blockIdx.x) * kChunk;
+
+    for (uint32_t t = tid; t < kChunk; t += kPackThreads) {
+        const uint64_t k = base + t;
+        // padding slots sort last: their key length is 2^32 - 1, above any real length
+        const uint64_t L = k < n ? (off[k + 1] - off[k]) : 0xFFFFFFFFull;
+        keys[t] = (L << 11) | t;
+    }
+    __syncthreads();
+
+    for (uint32_t size = 2; size <= kChunk; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            const uint32_t t = tid;
+            const uint32_t i = 2 * stride * (t / stride) + (t % stride);
+            const uint32_t j = i + stride;
+            const bool ascending = (i & size) == 0;
+            const uint64_t a = keys[i], b = keys[j];
+            if ((a > b) == ascending) {
+                keys[i] = b;
+                keys[j] = a;
+            }
+            __syncthreads();
+        }
+    }
+
+    for (uint32_t s = tid; s < kChunk; s += kPackThreads) {
+        const uint64_t key = keys[s];
+        const uint32_t t = uint32_t(key & 2047u);
+        const uint64_t k = base + t;
+        const bool valid = k < n;
+        const uint32_t L = valid ? uint32_t(key >> 11) : 0u;
+        const uint64_t slot = base + s;
+        len_out[slot] = L;
+        orig_out[slot] = valid ? uint32_t(k) : kNoSlot;
+        const uint8_t* src = bytes + (valid ? off[k] : 0);
+        for (uint32_t w = 0; w <= nwords; ++w) {
+            uint32_t word = 0;
+            const uint32_t p = 4 * w;
+            if (p < L) {
+                const uint32_t left = L - p;
+                word = src[p];
+                if (left > 1) word |= uint32_t(src[p + 1]) << 8;
+                if (left > 2) word |= uint32_t(src[p + 2]) << 16;
+                if (left > 3) word |= uint32_t(src[p + 3]) << 24;
+            }
+            words[uint64_t(w) * npad + slot] = word;
+        }
+    }
+}
+
+__global__ void classify_kernel(const uint64_t* __restrict__ off, uint64_t n, ClassBounds b,
+                                uint32_t* __restrict__ counts, uint32_t* __restrict__ lists) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t m = off[i + 1] - off[i];
+    int c = 0;
+    while (c < b.count - 1 && m > b.bound[c]) ++c;
+    const uint32_t pos = atomicAdd(&counts[c], 1u);
+    lists[uint64_t(c) * n + pos] = uint32_t(i);
+}
+
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+void launch_pack_refs(const uint8_t* bytes, const uint64_t* off, uint64_t n, uint32_t nwords,
+                      uint32_t npad, uint32_t* words, uint32_t* len, uint32_t* orig,
+                      hipStream_t s) {
+    const uint32_t chunks = npad / kChunk;
+    if (chunks == 0) return;
+    hipLaunchKernelGGL(pack_refs_kernel, dim3(chunks), dim3(kPackThreads), 0, s, bytes, off, n,
+                       nwords, npad, words, len, orig);
+}
+
+void launch_classify(const uint64_t* off, uint64_t n, const ClassBounds& b, uint32_t* counts,
+                     uint32_t* lists, hipStream_t s) {
+    if (n == 0) return;
+    const uint32_t blocks = uint32_t((n + 255) / 256);
+    hipLaunchKernelGGL(classify_kernel, dim3(blocks), dim3(256), 0, s, off, n, b, counts, lists);
+}
+
+void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
+    if (n == 0) return;
+    const uint32_t blocks = uint32_t((n + 255) / 256);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks), dim3(256), 0, s, p, n, v);
+}
+
+}  // namespace st

@@ -1,0 +1,137 @@
+// Internal declarations shared by the HIP translation units of libseqtree_hip.so.
+//
+// Data layout in HBM (one call):
+//   * query side: the caller's CSR (codes + uint64 offsets), read directly; a workgroup owns
+//     one query, so its residues are read as uniform (scalar / broadcast) values.
+//   * reference side: packed once per call by pack_refs into
+//       words[(nwords + 1) * npad]   word w (residues 4w..4w+3, little-endian bytes) of slot s
+//                                    at words[w * npad + s]: a wave reading word w of 64
+//                                    consecutive slots issues one 256-byte coalesced load;
+//                                    row nwords is an all-zero pad row;
+//       len[npad], orig[npad]        slot length and original column index (kNoSlot = pad).
+//     Slots are grouped in chunks of kChunk consecutive references; inside a chunk they are
+//     sorted by length so that the 64 lanes of a wave see (almost) equal lengths and the DP
+//     loops stay convergent. A scoring workgroup owns (query, chunk), stages its kChunk
+//     results in LDS by original column and writes them back as one coalesced row segment.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace st {
+
+constexpr uint32_t kChunk = 1024;        // reference slots per scoring workgroup
+constexpr uint32_t kThreads = 256;       // threads per scoring workgroup (4 wave64s)
+constexpr uint32_t kSlotsPerThread = kChunk / kThreads;
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+constexpr uint32_t kMaxAlphabet = 32;    // codec sizes are <= 32 (seqtree.hpp:15-16)
+
+struct PackedRefs {
+    const uint32_t* words;
+    const uint32_t* len;
+    const uint32_t* orig;
+    uint32_t npad;
+    uint32_t nwords;
+    uint32_t nchunks;
+};
+
+// ---- prep.hip --------------------------------------------------------------------------
+void launch_pack_refs(const uint8_t* bytes, const uint64_t* off, uint64_t n, uint32_t nwords,
+                      uint32_t npad, uint32_t* words, uint32_t* len, uint32_t* orig,
+                      hipStream_t s);
+
+// Buckets the n queries by length: class c receives every query with
+// length <= bounds[c] (and > bounds[c-1]); the last bound must be 0xFFFFFFFF.
+constexpr int kMaxClasses = 16;
+struct ClassBounds {
+    uint32_t bound[kMaxClasses];
+    int count;
+};
+void launch_classify(const uint64_t* off, uint64_t n, const ClassBounds& b, uint32_t* counts,
+                     uint32_t* lists, hipStream_t s);
+
+void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+
+// ---- gapblock.hip ----------------------------------------------------------------------
+struct GapblockArgs {
+    const uint8_t* q_codes;
+    const uint64_t* q_off;
+    uint64_t n_queries;
+    uint64_t n_refs;
+    PackedRefs refs;
+    const int32_t* pen;    // device, A*A
+    uint32_t A;
+    int32_t gap_open, gap_extend;
+    const int32_t* prior;  // device cube or nullptr
+    uint32_t w1;           // prior_width + 1
+    int32_t* out;
+};
+// use_profile: stage a per-query profile (q_max_len * A int32) in LDS.
+// exact_wrap: evaluate the candidates in the reference's order with wrapping int32
+// arithmetic (needed only when an int32 sum may overflow).
+void launch_gapblock(const GapblockArgs& a, bool use_profile, uint32_t q_max_len, bool exact_wrap,
+                     hipStream_t s);
+
+// ---- gotoh.hip -------------------------------------------------------------------------
+struct GotohArgs {
+    const uint8_t* q_codes;
+    const uint64_t* q_off;
+    const uint8_t* r_codes;  // CSR, for the generic kernels
+    const uint64_t* r_off;
+    uint64_t n_queries;
+    uint64_t n_refs;
+    PackedRefs refs;
+    const int32_t* sim;   // device, A*A   [query code][ref code]
+    const int32_t* simT;  // device, A*A   transposed
+    uint32_t A;
+    int32_t local;
+    int32_t dist;
+    int32_t gap_open, gap_extend;
+    const int64_t* q_self;  // dist only
+    const int64_t* r_self;
+    int32_t* out;
+    int32_t* overflow;      // device flag, set to 1 when a result does not fit int32
+};
+// Register-stripe kernel for queries of length <= rows (rows in {8,16,...,64}); the query
+// indices come from `list` (count entries). Needs every value to fit the int32 range proof.
+void launch_gotoh_fast(const GotohArgs& a, int rows, const uint32_t* list, uint32_t count,
+                       hipStream_t s);
+// int64 per-thread DP for arbitrary lengths and gap costs: every (list[t], k) pair, k < n_refs.
+// scratch holds `slots` regions of 6 * (stride) int64.
+void launch_gotoh_generic(const GotohArgs& a, const uint32_t* list, uint32_t count,
+                          int64_t* scratch, uint32_t slots, uint64_t stride, hipStream_t s);
+// self scores s(x, x) for every sequence of a CSR set, into int64 res.
+void launch_gotoh_self(const GotohArgs& a, const uint8_t* codes, const uint64_t* off, uint64_t n,
+                       int64_t* res, int64_t* scratch, uint32_t slots, uint64_t stride,
+                       hipStream_t s);
+
+// ---- distance.hip ----------------------------------------------------------------------
+struct DistanceArgs {
+    const uint8_t* a_bytes;
+    const uint64_t* a_off;
+    const uint8_t* b_bytes;  // CSR, for the generic kernel
+    const uint64_t* b_off;
+    uint64_t n_a;
+    uint64_t n_b;
+    PackedRefs refs;
+    int32_t* out;
+};
+// Queries packed row-major as a_words[i * nwords + w] (equal lengths, zero padded).
+void launch_pack_rows(const uint8_t* bytes, const uint64_t* off, uint64_t n, uint32_t nwords,
+                      uint32_t* words, hipStream_t s);
+void launch_hamming(const DistanceArgs& a, const uint32_t* a_words, uint32_t nwords,
+                    hipStream_t s);
+// Locates the first length-mismatched pair in row-major order; res = {i, k} or {~0, ~0}.
+void launch_hamming_check(const uint64_t* a_off, uint64_t n_a, const uint64_t* b_off,
+                          uint64_t n_b, uint64_t* res, hipStream_t s);
+// Myers bit-parallel Levenshtein for queries of length <= 64 * words (words in 1..4).
+void launch_levenshtein_myers(const DistanceArgs& a, int words, const uint32_t* list,
+                              uint32_t count, hipStream_t s);
+// Two-row DP, one pair per thread, for arbitrary lengths.
+void launch_levenshtein_generic(const DistanceArgs& a, const uint32_t* list, uint32_t count,
+                                int32_t* scratch, uint32_t slots, uint64_t stride,
+                                hipStream_t s);
+
+}  // namespace st

@@ -1,0 +1,145 @@
+"""GPU parity for the gap-block score matrix (reference: src/gapblock.cpp, the matrix contract
+of tests/python/test_gapblock.py:479-557). The checker is the CPU oracle."""
+from __future__ import annotations
+
+import random
+
+import numpy as np
+import pytest
+
+from conftest import AA, cdr3_like, rand_seq
+
+pytestmark = pytest.mark.gpu
+
+
+def _pen(st, matrix, alphabet="aa"):
+    A = len(st.alphabet_symbols(alphabet))
+    if matrix is None:
+        return (np.arange(A)[:, None] != np.arange(A)[None, :]).astype(np.int32)
+    syms = st.alphabet_symbols(alphabet)
+    return np.array([[matrix.penalty(a, b) for b in syms] for a in syms], np.int32)
+
+
+def _check(st, oracle_lib, q, r, matrix=None, go=28, ge=1, prior=None, alphabet="aa"):
+    from seqtree_amd.gapblock import _prior_cube
+
+    got = np.asarray(st.score_matrix(q, r, matrix, go, ge, prior, alphabet=alphabet))
+    width = max((len(s) for s in q + r), default=0)
+    cube = _prior_cube(prior, width) if prior is not None else None
+    want = oracle_lib.gapblock_matrix(q, r, _pen(st, matrix, alphabet), go, ge, cube, width,
+                                      alphabet=alphabet, threads=8)
+    assert got.shape == want.shape
+    bad = np.argwhere(got != want)
+    assert bad.size == 0, f"{len(bad)} cells differ, first {bad[:3].tolist()}"
+    return got
+
+
+def _priors(st):
+    g = st.gapblock
+    return {
+        "none": None,
+        "central": g.central_prior(21),
+        "frame": g.frame_prior(21, 5),
+        "positions": g.positions_prior((3, 4, -4, -3)),
+        "profile": g.profile_prior(21, lambda j, m: 1.0 - abs(2 * j + 1 - m) / m),
+    }
+
+
+@pytest.mark.parametrize("name", ["none", "central", "frame", "positions", "profile"])
+def test_matrix_equals_scalar_scorer_and_oracle(gpu, oracle_lib, name):
+    st = gpu
+    prior = _priors(st)[name]
+    rng = random.Random(7)
+    seqs = [rand_seq(rng, 4, 20) for _ in range(40)]
+    m = st.SubstitutionMatrix.blosum62()
+    got = _check(st, oracle_lib, seqs, seqs, m, 28, 1, prior)
+    pen = {(a, b): m.penalty(a, b) for a in AA for b in AA}
+    for i, q in enumerate(seqs):
+        for k, r in enumerate(seqs):
+            assert got[i, k] == st.gapblock_score(q, r, m, 28, 1, prior, _pen=pen)[0]
+    # a valid ball: zero diagonal, non-negative, symmetric
+    assert (np.diag(got) == 0).all() and (got >= 0).all() and (got == got.T).all()
+
+
+@pytest.mark.parametrize("matrix", ["blosum62", "blosum45", "blosum80", "pam250", "pam100",
+                                    "structural", None])
+@pytest.mark.parametrize("go,ge", [(1, 1), (28, 1), (0, 3), (2, 5), (0, 0)])
+def test_random_ragged_sets(gpu, oracle_lib, matrix, go, ge):
+    st = gpu
+    rng = random.Random(hash((matrix, go, ge)) & 0xFFFF)
+    q = [rand_seq(rng, 0, 33) for _ in range(37)]
+    r = [rand_seq(rng, 0, 33) for _ in range(2100)]  # three chunks, the last one partial
+    mat = getattr(st.SubstitutionMatrix, matrix)() if matrix else None
+    _check(st, oracle_lib, q, r, mat, go, ge, st.gapblock.central_prior(9))
+    _check(st, oracle_lib, q, r, mat, go, ge, None)
+
+
+def test_unit_cost_and_empty_inputs(gpu, oracle_lib):
+    st = gpu
+    sm = st.score_matrix(["ACGT", "ACTT"], ["ACGT"], None, gap_open=1)
+    assert sm[0, 0] == 0 and sm[1, 0] == 1
+    assert st.score_matrix([], ["ACGT"]).shape == (0, 1)
+    assert st.score_matrix(["ACGT"], []).shape == (1, 0)
+    _check(st, oracle_lib, ["", "A", "AC", ""], ["", "ACGT", "T"], None, 1, 1, alphabet="nt")
+
+
+def test_nucleotide_alphabets(gpu, oracle_lib):
+    st = gpu
+    rng = random.Random(3)
+    q = [rand_seq(rng, 5, 30, "ACGT") for _ in range(50)]
+    r = [rand_seq(rng, 5, 30, "acgt") for _ in range(300)]
+    _check(st, oracle_lib, q, r, None, 3, 1, st.gapblock.central_prior(2), alphabet="nt")
+    qi = [rand_seq(rng, 5, 30, "ACGTRYSWKMBDHVN") for _ in range(40)]
+    _check(st, oracle_lib, qi, qi, st.SubstitutionMatrix.unit(15), 2, 1, alphabet="iupac")
+
+
+def test_wrapping_prior_takes_the_exact_path(gpu, oracle_lib):
+    """A prior near INT32_MAX makes the int32 candidate sums wrap; the kernel must then pick
+    the same minimum as the reference's int32 evaluation order."""
+    st = gpu
+    rng = random.Random(5)
+    seqs = [rand_seq(rng, 3, 14) for _ in range(60)]
+    prior = lambda i, d, m: (2**31 - 1) - 7 * i  # noqa: E731
+    _check(st, oracle_lib, seqs, seqs, st.SubstitutionMatrix.blosum62(), 28, 1, prior)
+
+
+def test_long_queries_use_the_table_path(gpu, oracle_lib):
+    """Queries past 512 residues no longer fit an LDS profile; the A x A table path runs."""
+    st = gpu
+    rng = random.Random(9)
+    q = [rand_seq(rng, 500, 700) for _ in range(6)]
+    r = [rand_seq(rng, 480, 720) for _ in range(90)] + q[:2]
+    _check(st, oracle_lib, q, r, st.SubstitutionMatrix.blosum62(), 28, 1, None)
+
+
+def test_custom_asymmetric_matrix_operand_order(gpu, oracle_lib):
+    """pen is indexed [query code][ref code]; an asymmetric table exposes a swapped operand."""
+    st = gpu
+    rng = np.random.default_rng(11)
+    grid = rng.integers(-6, 12, size=(24, 24)).tolist()
+    m = st.SubstitutionMatrix.from_similarity(grid)
+    rr = random.Random(11)
+    q = [rand_seq(rr, 2, 25) for _ in range(30)]
+    r = [rand_seq(rr, 2, 25) for _ in range(500)]
+    _check(st, oracle_lib, q, r, m, 17, 2, st.gapblock.frame_prior(3, 4))
+
+
+def test_cdr3_workload_shape(gpu, oracle_lib):
+    """1,000 x 10,000 CDR3-like strings, BLOSUM62, gap_open = 2 * scale, central prior."""
+    st = gpu
+    rng = random.Random(2024)
+    q = cdr3_like(rng, 1000, 10, 20)
+    r = cdr3_like(rng, 10000, 10, 20)
+    m = st.SubstitutionMatrix.blosum62()
+    _check(st, oracle_lib, q, r, m, 2 * m.scale(), 1, st.gapblock.central_prior(21))
+
+
+def test_errors_and_buffer(gpu):
+    st = gpu
+    m = st.SubstitutionMatrix.blosum62()
+    sm = st.score_matrix(["CASSL", "CASL"], ["CASSL"], m, 28)
+    mv = memoryview(sm)
+    assert mv.format == "i" and mv.shape == (2, 1) and mv.c_contiguous
+    assert mv[0, 0] == 0 and mv[1, 0] == sm[1, 0] > 0
+    with pytest.raises(ValueError, match="negative"):
+        st.score_matrix(["CAS"], ["CA"], m, 28, gap_prior=lambda i, d, n: -1)

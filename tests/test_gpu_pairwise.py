@@ -1,0 +1,173 @@
+"""GPU parity for Gotoh NW/SW score and distance matrices (reference: src/pairwise.cpp and
+tests/python/test_pairwise.py). The checker is the CPU oracle; known answers come from the
+reference's tests and docstrings."""
+from __future__ import annotations
+
+import random
+
+import numpy as np
+import pytest
+
+from conftest import cdr3_like, rand_seq
+
+pytestmark = pytest.mark.gpu
+
+SETTINGS = [
+    ("global", 11, 1), ("global", 12, 1), ("global", 5, 5), ("global", 1, 1),
+    ("global", 0, 0), ("global", 100, 5), ("global", 1, 3), ("global", 0, 5),
+    ("local", 1, 10), ("global", 10**8, 1), ("global", 2**30, 1),
+    ("local", 11, 1), ("local", 5, 5), ("local", 1, 1), ("local", 12, 1),
+]
+
+
+def _sim(st, m, alphabet="aa"):
+    syms = st.alphabet_symbols(alphabet)
+    if m.size() != len(syms):
+        raise ValueError
+    return np.array([[m.similarity(a, b) for b in syms] for a in syms], np.int32)
+
+
+def _check(st, oracle_lib, q, r, m, mode, go, ge, dist=False, alphabet="aa", sim=None):
+    fn = st.pairwise.dist_matrix if dist else st.pairwise.score_matrix
+    code, want = oracle_lib.align_matrix(q, r, _sim(st, m, alphabet) if sim is None else sim,
+                                         mode == "local", dist, go, ge, alphabet=alphabet,
+                                         threads=8)
+    if code == 2:
+        with pytest.raises(OverflowError, match="distance" if dist else "score"):
+            fn(q, r, m, mode=mode, gap_open=go, gap_extend=ge, alphabet=alphabet)
+        return None
+    got = np.asarray(fn(q, r, m, mode=mode, gap_open=go, gap_extend=ge, alphabet=alphabet))
+    bad = np.argwhere(got != want)
+    assert bad.size == 0, f"{len(bad)} cells differ, first {bad[:3].tolist()}"
+    return got
+
+
+@pytest.mark.parametrize("mode,go,ge", SETTINGS)
+@pytest.mark.parametrize("name", ["blosum62", "blosum45", "blosum80"])
+def test_settings_grid(gpu, oracle_lib, name, mode, go, ge):
+    st = gpu
+    m = getattr(st.SubstitutionMatrix, name)()
+    rng = random.Random(hash((name, mode, go, ge)) & 0xFFFF)
+    q = [rand_seq(rng, 0, 26) for _ in range(23)] + ["A", "W" * 3]
+    r = [rand_seq(rng, 0, 26) for _ in range(1100)] + ["", "KKAAA"]
+    _check(st, oracle_lib, q, r, m, mode, go, ge)
+
+
+@pytest.mark.parametrize("mode", ["global", "local"])
+@pytest.mark.parametrize("name", ["blosum62", "pam250", "pam100", "structural"])
+def test_distance_matrices(gpu, oracle_lib, name, mode):
+    st = gpu
+    m = getattr(st.SubstitutionMatrix, name)()
+    rng = random.Random(hash((name, mode)) & 0xFFFF)
+    seqs = cdr3_like(rng, 40, 8, 25)
+    d = _check(st, oracle_lib, seqs, seqs, m, mode, 11, 1, dist=True)
+    assert (np.diag(d) == 0).all() and (d >= 0).all() and (d == d.T).all()
+    q = [rand_seq(rng, 0, 30) for _ in range(17)]
+    r = [rand_seq(rng, 0, 30) for _ in range(1500)]
+    _check(st, oracle_lib, q, r, m, mode, 11, 1, dist=True)
+
+
+def test_known_answers(gpu):
+    st = gpu
+    m = st.SubstitutionMatrix.blosum62()
+    score = st.pairwise.score
+    for L in (1, 2, 3, 5):
+        assert score("AAA", "AAA" + "A" * L, m, gap_open=11, gap_extend=1) == 12 - (11 + L - 1)
+        if L <= 3:
+            assert score("AAA", "AAA" + "A" * L, m, gap_open=5, gap_extend=5) == 12 - 5 * L
+    assert score("AAA", "KKAAA", m, gap_open=11, gap_extend=1) == 0
+    assert score("WWW", "KKK", m, mode="local") == 0
+    assert score("WWWAAAWWW", "KKKAAAKKK", m, mode="local") == 12
+    assert score("WWWAAAWWW", "KKKAAAKKK", m, mode="global") < 0
+    s = "CASSLGQAYEQYF"
+    assert score(s, s, m, mode="local") == score(s, s, m, mode="global")
+    assert score("CASSLGQAYEQYF", "CASSPGQAYEQF", m) == 45
+    assert score("CASSLGQAYEQYF", "CASSPGQAYEQF", m, gap_open=12) == 44
+    assert score("AAA", "AAA", m) == 12 and score("AAA", "AAAAA", m) == 0
+    assert score("", "", m) == 0
+    assert score("", "AAA", m) == -(11 + 2)
+    assert score("", "AAA", m, mode="local") == 0
+
+
+def test_huge_gap_costs_take_the_int64_path(gpu):
+    st = gpu
+    m = st.SubstitutionMatrix.blosum62()
+    q, r = "CASSLGQAYEQYF", "CASSPGQAYEQF"
+    assert st.pairwise.score(q, r, m, gap_open=10**8) == -99_999_944
+    assert st.pairwise.score(q, r, m, gap_open=2**30) == -1_073_741_768
+    assert st.pairwise.score(q, r, m, gap_open=2**31 - 1) == -2_147_483_591
+    with pytest.raises(OverflowError, match="32 bits"):
+        st.pairwise.score("A" * 400, "W" * 600, m, gap_open=2**31 - 1, gap_extend=2**31 - 1)
+
+
+def test_overflow_raises_with_the_right_quantity(gpu):
+    st = gpu
+    m = st.SubstitutionMatrix.blosum62()
+    with pytest.raises(OverflowError):
+        st.pairwise.score_matrix(["A" * 100], ["C"], m, gap_open=10**8, gap_extend=10**8)
+    with pytest.raises(OverflowError):
+        st.pairwise.dist_matrix(["A" * 100], ["C"], m, gap_open=10**8, gap_extend=10**8)
+    with pytest.raises(OverflowError, match="distance"):
+        st.pairwise.dist_matrix(["AAA"], ["C"], m, gap_open=2**31 - 1, gap_extend=2**31 - 1)
+
+
+def test_long_queries_and_references(gpu, oracle_lib):
+    """Queries past 64 residues run the int64 per-thread DP; long references stay on the
+    register kernel."""
+    st = gpu
+    m = st.SubstitutionMatrix.blosum62()
+    rng = random.Random(17)
+    q = [rand_seq(rng, 0, 64) for _ in range(10)] + [rand_seq(rng, 60, 130) for _ in range(6)]
+    r = [rand_seq(rng, 0, 200) for _ in range(300)]
+    for mode in ("global", "local"):
+        _check(st, oracle_lib, q, r, m, mode, 12, 1)
+        _check(st, oracle_lib, q, r, m, mode, 12, 1, dist=True)
+
+
+def test_asymmetric_custom_matrix(gpu, oracle_lib):
+    """sim is indexed [query code][ref code]; the transposed int64 path must use sim^T."""
+    st = gpu
+    rng = np.random.default_rng(5)
+    grid = rng.integers(-7, 9, size=(24, 24)).astype(np.int32)
+    m = st.SubstitutionMatrix.from_similarity(grid.tolist())
+    rr = random.Random(5)
+    q = [rand_seq(rr, 0, 80) for _ in range(20)]
+    r = [rand_seq(rr, 0, 40) for _ in range(200)] + [rand_seq(rr, 70, 90) for _ in range(20)]
+    for mode in ("global", "local"):
+        for go, ge in ((11, 1), (3, 7), (10**7, 3)):
+            _check(st, oracle_lib, q, r, m, mode, go, ge, sim=grid)
+            _check(st, oracle_lib, q, r, m, mode, go, ge, dist=True, sim=grid)
+
+
+def test_nucleotide_unit_matrix(gpu, oracle_lib):
+    st = gpu
+    m = st.SubstitutionMatrix.unit(4)
+    rng = random.Random(21)
+    q = [rand_seq(rng, 0, 40, "ACGT") for _ in range(30)]
+    r = [rand_seq(rng, 0, 40, "ACGT") for _ in range(400)]
+    sim = np.eye(4, dtype=np.int32)
+    for mode in ("global", "local"):
+        _check(st, oracle_lib, q, r, m, mode, 2, 1, alphabet="nt", sim=sim)
+
+
+def test_cdr3_workload_slice(gpu, oracle_lib):
+    """A 200-row slice of the 20,000 x 20,000 CDR3-like workload (lengths 8-25)."""
+    st = gpu
+    m = st.SubstitutionMatrix.blosum62()
+    rng = random.Random(99)
+    q = cdr3_like(rng, 200, 8, 25)
+    r = cdr3_like(rng, 20000, 8, 25)
+    for mode in ("global", "local"):
+        _check(st, oracle_lib, q, r, m, mode, 11, 1)
+
+
+def test_bad_arguments(gpu):
+    st = gpu
+    m = st.SubstitutionMatrix.blosum62()
+    with pytest.raises(ValueError, match="gap_open"):
+        st.pairwise.score("AA", "AA", m, gap_open=-1)
+    with pytest.raises(ValueError, match="mode"):
+        st.pairwise.score("AA", "AA", m, mode="semiglobal")
+    with pytest.raises(ValueError, match="alphabet|symbol"):
+        st.pairwise.score("AA#", "AA", m)
+    assert st.pairwise.score_matrix([], ["A"], m).shape == (0, 1)
