@@ -210,10 +210,10 @@ def main() -> int:
     # is sum(m) * sum(n) over the queries of its length class, times K_MIN ops per cell
     label, (launches, k_ms) = max(kernels.items(), key=lambda kv: kv[1][1])
     roofline = None
-    if label.startswith("gotoh_fast_r"):
-        rows = int(label.split("_")[2][1:])
-        mode = label.split("_")[3]
-        in_class = (q_len <= rows) & (q_len > rows - 8)
+    if label.startswith("gotoh_i"):
+        _, cell, rtag, mode = label.split("_")
+        rows = int(rtag[1:])
+        in_class = (q_len <= rows) & (q_len > rows - 4)
         cells = float(q_len[in_class].sum()) * float(r_len.sum())
         avg_s = k_ms / launches / 1e3
         achieved = cells * K_MIN[mode] / avg_s / 1e12

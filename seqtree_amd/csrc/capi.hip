@@ -276,8 +276,9 @@ void gapblock_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, uint32
 // ------------------------------------------------------------------------------------------
 // Gotoh
 // ------------------------------------------------------------------------------------------
-constexpr int kGotohRows[] = {8, 16, 24, 32, 40, 48, 56, 64};
-constexpr int kGotohClasses = 8;
+// Register-kernel row classes: a query of length m runs in the class R = roundup(m, 4).
+constexpr int kGotohClasses = 16;
+constexpr int gotoh_rows(int c) { return 4 * (c + 1); }
 
 void gotoh_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, uint32_t q_max,
                   const uint8_t* r, const uint64_t* r_off, uint64_t K, uint32_t r_max,
@@ -342,12 +343,15 @@ void gotoh_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, uint32_t 
         a.r_self = rs;
     }
 
-    // Range proof for the int32 register kernel: with R <= 64 stripe rows (dummies
-    // included) every reachable cell is a partial-alignment score bounded by
-    // (64 + r_max) * mx; that bound, the -2^30 sentinel and one gap step below either must
-    // stay inside int32. A distance adds at most 4 * (q_max + r_max) * mx.
+    // Range proof for the register kernels: with R <= 64 stripe rows (dummies included)
+    // every reachable cell is a partial-alignment score bounded by B = (64 + r_max) * mx;
+    // B, the sentinel (-2^30 in int32, -2^14 in int16) and one gap step below either must
+    // stay inside the cell type. A distance adds at most 4 * (q_max + r_max) * mx, formed
+    // in int64 and stored as int32.
     const int64_t B = (64 + int64_t(r_max)) * mx;
-    bool fast = B + 2 * std::max<int64_t>(go, ge) < (int64_t(1) << 30);
+    const int64_t G = std::max<int64_t>(go, ge);
+    bool fast = B + 2 * G < (int64_t(1) << 30);
+    const bool packed16 = B + 2 * G < (int64_t(1) << 14);
     if (dist) fast = fast && 4 * (int64_t(q_max) + int64_t(r_max)) * mx < int64_t(INT32_MAX);
 
     ClassBounds cb{};
@@ -355,7 +359,7 @@ void gotoh_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, uint32_t 
     std::vector<uint32_t> counts;
     if (fast) {
         cb.count = kGotohClasses + 1;
-        for (int c = 0; c < kGotohClasses; ++c) cb.bound[c] = uint32_t(kGotohRows[c]);
+        for (int c = 0; c < kGotohClasses; ++c) cb.bound[c] = uint32_t(gotoh_rows(c));
         cb.bound[kGotohClasses] = 0xFFFFFFFFu;
     } else {
         cb.count = 1;
@@ -369,9 +373,10 @@ void gotoh_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, uint32_t 
         for (int c = 0; c < kGotohClasses; ++c) {
             if (counts[c] == 0) continue;
             {
-                LaunchTimer t("gotoh_fast_r" + std::to_string(kGotohRows[c]) +
-                                  (local ? "_local" : "_global"), s);
-                launch_gotoh_fast(a, kGotohRows[c], lists + size_t(c) * N, counts[c], s);
+                LaunchTimer t(std::string(packed16 ? "gotoh_i16x2_r" : "gotoh_i32_r") +
+                                  std::to_string(gotoh_rows(c)) + (local ? "_local" : "_global"),
+                              s);
+                launch_gotoh_fast(a, gotoh_rows(c), packed16, lists + size_t(c) * N, counts[c], s);
             }
             launched("gotoh_fast");
         }

@@ -8,16 +8,19 @@
 // Global: X[i][0] = -gap_run(i), Y[0][j] = -gap_run(j), result max(M,X,Y)[m][n].
 // Local: M boundary 0, gap states unreachable on the boundary, result max over all M (>= 0).
 //
-// Fast kernel: workgroup = (query, reference chunk); each lane owns one reference and
-// sweeps its columns j, keeping a stripe of R query rows in registers (R >= m, rows past m
-// are dummies whose profile entries can never win). With U = max(M, Y) and V = max(M, X)
-// the cell update is 10 VALU ops (11 local):
+// Register kernel: workgroup = (query, reference chunk); each lane owns one reference and
+// sweeps its columns j, keeping a stripe of R query rows in registers (R >= m, R a multiple
+// of 4; rows past m are dummies whose profile entries can never win). With U = max(M, Y) and
+// V = max(M, X) the cell update is 10 VALU ops (11 local):
 //   M = Hdiag + s;  X = max(U_up - go, X_up - ge);  Y = max(V_left - go, Y_left - ge)
 //   U = max(M, Y);  V = max(M, X);  H = max(U, X)
 // The substitution score comes from a query profile in LDS, prof[row][code] with a fixed
 // 32-entry row stride, so the lookup is one ds_read with an immediate row offset.
-// It runs in int32 when a host-side range proof shows every reachable value, the -2^30
-// sentinel and one gap step below it stay inside int32; otherwise the int64 generic kernel
+// When a host-side range proof shows every reachable value, the -2^14 sentinel and one gap
+// step below it fit int16, two queries share a workgroup and ride in the two halves of each
+// register: the same 10 instructions, as v_pk_* ops, advance two cells, and one profile read
+// (both queries' scores packed in one dword) feeds both. With the proof only for int32 the
+// kernel runs one query in int32 (-2^30 sentinel); otherwise the int64 generic kernel
 // (a straight two-row DP per thread) computes exactly what the reference's int64 DP does,
 // including the narrow() overflow check.
 
@@ -27,32 +30,79 @@ namespace st {
 namespace {
 
 constexpr int32_t kNeg32 = -(1 << 30);
+constexpr int16_t kNeg16 = -(1 << 14);
 
-template <int R, bool LOCAL>
-__global__ __launch_bounds__(kThreads) void gotoh_fast_kernel(GotohArgs a,
-                                                              const uint32_t* __restrict__ list,
-                                                              uint64_t base) {
-    __shared__ int32_t prof[R * 32];
-    __shared__ int32_t stage[kChunk];
+typedef short v2i16 __attribute__((ext_vector_type(2)));
+
+// Cell arithmetic of the register kernel. I32: one query per workgroup, int32 cells.
+// I16x2: two queries per workgroup packed into the 16-bit halves of every register, so one
+// v_pk_* instruction advances two DP cells (same reference column, two query rows).
+struct I32 {
+    using T = int32_t;
+    static constexpr int Q = 1;
+    static constexpr int32_t kNeg = kNeg32;
+    __device__ static T splat(int32_t v) { return v; }
+    __device__ static T vmax(T a, T b) { return max(a, b); }
+    __device__ static T load(uint32_t w) { return int32_t(w); }
+    __device__ static uint32_t pack(const int32_t* v) { return uint32_t(v[0]); }
+    __device__ static int32_t half(T v, int) { return v; }
+};
+
+struct I16x2 {
+    using T = v2i16;
+    static constexpr int Q = 2;
+    static constexpr int32_t kNeg = kNeg16;
+    __device__ static T splat(int32_t v) { return T{int16_t(v), int16_t(v)}; }
+    __device__ static T vmax(T a, T b) { return __builtin_elementwise_max(a, b); }
+    __device__ static T load(uint32_t w) { return __builtin_bit_cast(T, w); }
+    __device__ static uint32_t pack(const int32_t* v) {
+        return (uint32_t(uint16_t(int16_t(v[0])))) | (uint32_t(uint16_t(int16_t(v[1]))) << 16);
+    }
+    __device__ static int32_t half(T v, int h) { return h == 0 ? int32_t(v.x) : int32_t(v.y); }
+};
+
+// Register-stripe Gotoh: workgroup = (C::Q queries of one length class, reference chunk).
+template <class C, int R, bool LOCAL>
+__global__ __launch_bounds__(kThreads) void gotoh_reg_kernel(GotohArgs a,
+                                                             const uint32_t* __restrict__ list,
+                                                             uint32_t count, uint64_t base) {
+    using T = typename C::T;
+    constexpr int Q = C::Q;
+    __shared__ uint32_t prof[R * 32];
+    __shared__ int32_t stage[Q][kChunk];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t nch = a.refs.nchunks;
     const uint32_t c = blockIdx.x % nch;
-    const uint32_t qi = list[base + blockIdx.x / nch];
-    const uint64_t q0 = a.q_off[qi];
-    const uint32_t m = uint32_t(a.q_off[qi + 1] - q0);
-    const uint8_t* q = a.q_codes + q0;
-    const uint32_t A = a.A;
+    const uint64_t g = base + blockIdx.x / nch;
 
+    uint32_t qi[Q], m[Q];
+    bool valid[Q];
+    const uint8_t* q[Q];
+#pragma unroll
+    for (int h = 0; h < Q; ++h) {
+        const uint64_t idx = g * Q + h;
+        valid[h] = idx < count;
+        qi[h] = list[valid[h] ? idx : g * Q];
+        const uint64_t q0 = a.q_off[qi[h]];
+        m[h] = uint32_t(a.q_off[qi[h] + 1] - q0);
+        q[h] = a.q_codes + q0;
+    }
+    const uint32_t A = a.A;
+    const int32_t dummy = LOCAL ? C::kNeg : 0;  // rows past a query's end can never win
     for (uint32_t e = tid; e < uint32_t(R) * 32; e += kThreads) {
         const uint32_t r = e >> 5, b = e & 31u;
-        prof[e] = (r < m && b < A) ? a.sim[uint32_t(q[r]) * A + b] : (LOCAL ? kNeg32 : 0);
+        int32_t v[Q];
+#pragma unroll
+        for (int h = 0; h < Q; ++h)
+            v[h] = (r < m[h] && b < A) ? a.sim[uint32_t(q[h][r]) * A + b] : dummy;
+        prof[e] = C::pack(v);
     }
     __syncthreads();
 
-    const int32_t go = a.gap_open, ge = a.gap_extend;
+    const T go = C::splat(a.gap_open), ge = C::splat(a.gap_extend);
+    const T neg = C::splat(C::kNeg), zero = C::splat(0);
     const uint32_t npad = a.refs.npad;
-    const int64_t qself = a.dist ? a.q_self[qi] : 0;
 
     for (uint32_t it = 0; it < kSlotsPerThread; ++it) {
         const uint32_t slot = c * kChunk + it * kThreads + tid;
@@ -60,71 +110,78 @@ __global__ __launch_bounds__(kThreads) void gotoh_fast_kernel(GotohArgs a,
         if (k == kNoSlot) continue;
         const uint32_t n = a.refs.len[slot];
 
-        int32_t V[R], Y[R], H[R];
+        T V[R], Y[R], H[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int32_t g = LOCAL ? 0 : -(go + r * ge);  // -gap_run(r + 1)
-            V[r] = g;
-            Y[r] = kNeg32;
-            H[r] = g;
+            const T gr = LOCAL ? zero : C::splat(-(a.gap_open + r * a.gap_extend));
+            V[r] = gr;
+            Y[r] = neg;
+            H[r] = gr;
         }
-        int32_t hrow0 = 0;
-        int32_t best = 0;
+        T hrow0 = zero;
+        T best = zero;
         const uint32_t* wp = a.refs.words + slot;
         uint32_t word = 0;
         for (uint32_t j = 0; j < n; ++j) {
             if ((j & 3u) == 0) word = wp[uint64_t(j >> 2) * npad];
             const uint32_t rc = (word >> (8 * (j & 3u))) & 0xFFu;
-            const int32_t* pcol = prof + rc;
-            int32_t uup, hd;
-            int32_t xup = kNeg32;
+            const uint32_t* pcol = prof + rc;
+            T uup, hd;
+            T xup = neg;
             if (LOCAL) {
-                uup = 0;
-                hd = 0;
+                uup = zero;
+                hd = zero;
             } else {
-                const int32_t g = -(go + int32_t(j) * ge);  // row 0 of column j+1
-                uup = g;
+                const T g0 = C::splat(-(a.gap_open + int32_t(j) * a.gap_extend));
+                uup = g0;
                 hd = hrow0;
-                hrow0 = g;
+                hrow0 = g0;
             }
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const int32_t s = pcol[r * 32];
-                int32_t mv = hd + s;
-                if (LOCAL) mv = max(mv, 0);
-                const int32_t xv = max(uup - go, xup - ge);
-                const int32_t yv = max(V[r] - go, Y[r] - ge);
+                const T s = C::load(pcol[r * 32]);
+                T mv = hd + s;
+                if (LOCAL) mv = C::vmax(mv, zero);
+                const T xv = C::vmax(uup - go, xup - ge);
+                const T yv = C::vmax(V[r] - go, Y[r] - ge);
                 hd = H[r];
-                uup = max(mv, yv);
+                uup = C::vmax(mv, yv);
                 xup = xv;
-                V[r] = max(mv, xv);
+                V[r] = C::vmax(mv, xv);
                 Y[r] = yv;
-                H[r] = max(uup, xv);
-                if (LOCAL) best = max(best, mv);
+                H[r] = C::vmax(uup, xv);
+                if (LOCAL) best = C::vmax(best, mv);
             }
         }
 
-        int32_t res;
-        if (LOCAL) {
-            res = best;
-        } else if (m == 0) {
-            res = n == 0 ? 0 : -(go + int32_t(n - 1) * ge);
-        } else {
-            res = H[0];
 #pragma unroll
-            for (int r = 1; r < R; ++r)
-                if (uint32_t(r) == m - 1) res = H[r];
+        for (int h = 0; h < Q; ++h) {
+            int32_t res;
+            if (LOCAL) {
+                res = C::half(best, h);
+            } else if (m[h] == 0) {
+                res = n == 0 ? 0 : -(a.gap_open + int32_t(n - 1) * a.gap_extend);
+            } else {
+                res = C::half(H[0], h);
+#pragma unroll
+                for (int r = 1; r < R; ++r)
+                    if (uint32_t(r) == m[h] - 1) res = C::half(H[r], h);
+            }
+            if (a.dist) res = int32_t(a.q_self[qi[h]] + a.r_self[k] - 2 * int64_t(res));
+            stage[h][k - c * kChunk] = res;
         }
-        if (a.dist) res = int32_t(qself + a.r_self[k] - 2 * int64_t(res));
-        stage[k - c * kChunk] = res;
     }
     __syncthreads();
 
     const uint64_t col0 = uint64_t(c) * kChunk;
     const uint64_t lim64 = a.n_refs - col0;
     const uint32_t lim = lim64 < kChunk ? uint32_t(lim64) : kChunk;
-    int32_t* row = a.out + uint64_t(qi) * a.n_refs + col0;
-    for (uint32_t t = tid; t < lim; t += kThreads) row[t] = stage[t];
+#pragma unroll
+    for (int h = 0; h < Q; ++h) {
+        if (!valid[h]) continue;
+        int32_t* row = a.out + uint64_t(qi[h]) * a.n_refs + col0;
+        for (uint32_t t = tid; t < lim; t += kThreads) row[t] = stage[h][t];
+    }
 }
 
 // Exact int64 restatement of the reference DP for one pair; buf holds 6 * (min(m,n)+1).
@@ -225,36 +282,52 @@ __global__ __launch_bounds__(256) void gotoh_self_kernel(GotohArgs a,
     }
 }
 
-template <int R>
+template <class C, int R>
 void launch_rows(const GotohArgs& a, const uint32_t* list, uint32_t count, hipStream_t s) {
     const uint64_t nch = a.refs.nchunks;
-    const uint64_t max_q = (uint64_t(1) << 30) / nch;
-    for (uint64_t q0 = 0; q0 < count; q0 += max_q) {
-        const uint64_t nq = count - q0 < max_q ? count - q0 : max_q;
+    const uint64_t groups = (uint64_t(count) + C::Q - 1) / C::Q;
+    const uint64_t max_g = (uint64_t(1) << 30) / nch;
+    for (uint64_t g0 = 0; g0 < groups; g0 += max_g) {
+        const uint64_t ng = groups - g0 < max_g ? groups - g0 : max_g;
         if (a.local)
-            hipLaunchKernelGGL((gotoh_fast_kernel<R, true>), dim3(uint32_t(nq * nch)),
-                               dim3(kThreads), 0, s, a, list, q0);
+            hipLaunchKernelGGL((gotoh_reg_kernel<C, R, true>), dim3(uint32_t(ng * nch)),
+                               dim3(kThreads), 0, s, a, list, count, g0);
         else
-            hipLaunchKernelGGL((gotoh_fast_kernel<R, false>), dim3(uint32_t(nq * nch)),
-                               dim3(kThreads), 0, s, a, list, q0);
+            hipLaunchKernelGGL((gotoh_reg_kernel<C, R, false>), dim3(uint32_t(ng * nch)),
+                               dim3(kThreads), 0, s, a, list, count, g0);
+    }
+}
+
+template <class C>
+void launch_cell(const GotohArgs& a, int rows, const uint32_t* list, uint32_t count,
+                 hipStream_t s) {
+    switch (rows) {
+        case 4: launch_rows<C, 4>(a, list, count, s); break;
+        case 8: launch_rows<C, 8>(a, list, count, s); break;
+        case 12: launch_rows<C, 12>(a, list, count, s); break;
+        case 16: launch_rows<C, 16>(a, list, count, s); break;
+        case 20: launch_rows<C, 20>(a, list, count, s); break;
+        case 24: launch_rows<C, 24>(a, list, count, s); break;
+        case 28: launch_rows<C, 28>(a, list, count, s); break;
+        case 32: launch_rows<C, 32>(a, list, count, s); break;
+        case 36: launch_rows<C, 36>(a, list, count, s); break;
+        case 40: launch_rows<C, 40>(a, list, count, s); break;
+        case 44: launch_rows<C, 44>(a, list, count, s); break;
+        case 48: launch_rows<C, 48>(a, list, count, s); break;
+        case 52: launch_rows<C, 52>(a, list, count, s); break;
+        case 56: launch_rows<C, 56>(a, list, count, s); break;
+        case 60: launch_rows<C, 60>(a, list, count, s); break;
+        default: launch_rows<C, 64>(a, list, count, s); break;
     }
 }
 
 }  // namespace
 
-void launch_gotoh_fast(const GotohArgs& a, int rows, const uint32_t* list, uint32_t count,
-                       hipStream_t s) {
+void launch_gotoh_fast(const GotohArgs& a, int rows, bool packed16, const uint32_t* list,
+                       uint32_t count, hipStream_t s) {
     if (count == 0 || a.refs.nchunks == 0) return;
-    switch (rows) {
-        case 8: launch_rows<8>(a, list, count, s); break;
-        case 16: launch_rows<16>(a, list, count, s); break;
-        case 24: launch_rows<24>(a, list, count, s); break;
-        case 32: launch_rows<32>(a, list, count, s); break;
-        case 40: launch_rows<40>(a, list, count, s); break;
-        case 48: launch_rows<48>(a, list, count, s); break;
-        case 56: launch_rows<56>(a, list, count, s); break;
-        default: launch_rows<64>(a, list, count, s); break;
-    }
+    if (packed16) launch_cell<I16x2>(a, rows, list, count, s);
+    else launch_cell<I32>(a, rows, list, count, s);
 }
 
 void launch_gotoh_generic(const GotohArgs& a, const uint32_t* list, uint32_t count,

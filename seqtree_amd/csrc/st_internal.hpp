@@ -44,7 +44,7 @@ void launch_pack_refs(const uint8_t* bytes, const uint64_t* off, uint64_t n, uin
 
 // Buckets the n queries by length: class c receives every query with
 // length <= bounds[c] (and > bounds[c-1]); the last bound must be 0xFFFFFFFF.
-constexpr int kMaxClasses = 16;
+constexpr int kMaxClasses = 24;
 struct ClassBounds {
     uint32_t bound[kMaxClasses];
     int count;
@@ -94,10 +94,11 @@ struct GotohArgs {
     int32_t* out;
     int32_t* overflow;      // device flag, set to 1 when a result does not fit int32
 };
-// Register-stripe kernel for queries of length <= rows (rows in {8,16,...,64}); the query
-// indices come from `list` (count entries). Needs every value to fit the int32 range proof.
-void launch_gotoh_fast(const GotohArgs& a, int rows, const uint32_t* list, uint32_t count,
-                       hipStream_t s);
+// Register-stripe kernel for queries of length <= rows (rows a multiple of 4, <= 64); the
+// query indices come from `list` (count entries). packed16: two queries per workgroup in
+// int16 halves (needs the int16 range proof), else one query in int32 (int32 proof).
+void launch_gotoh_fast(const GotohArgs& a, int rows, bool packed16, const uint32_t* list,
+                       uint32_t count, hipStream_t s);
 // int64 per-thread DP for arbitrary lengths and gap costs: every (list[t], k) pair, k < n_refs.
 // scratch holds `slots` regions of 6 * (stride) int64.
 void launch_gotoh_generic(const GotohArgs& a, const uint32_t* list, uint32_t count,

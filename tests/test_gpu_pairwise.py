@@ -17,6 +17,8 @@ SETTINGS = [
     ("global", 0, 0), ("global", 100, 5), ("global", 1, 3), ("global", 0, 5),
     ("local", 1, 10), ("global", 10**8, 1), ("global", 2**30, 1),
     ("local", 11, 1), ("local", 5, 5), ("local", 1, 1), ("local", 12, 1),
+    # past the int16 range proof but inside int32: the one-query int32 register kernel
+    ("global", 1000, 3), ("local", 2000, 7),
 ]
 
 
