@@ -46,10 +46,14 @@ LEN_LO, LEN_HI = 8, 25
 GAP_OPEN, GAP_EXTEND = 11, 1
 SEED = 2
 
-# Minimal VALU instructions per DP cell of the register kernel (see the header of
-# seqtree_amd/csrc/gotoh.hip): M, X, Y, U, V, H updates; local adds the 0 floor.
-K_MIN = {"global": 10, "local": 11}
-# int32 VALU issue peak of MI355X: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz.
+# VALU lane-instructions per DP cell of the register kernel's cell update (see
+# seqtree_amd/csrc/gotoh.hip, "H form"): M = Hg_diag + s', X = max(Hg_up, X_up - ge),
+# Y = max(Hg_left, Y_left - ge), Hg = max(M, X, Y) - go -> 8 instructions (7 with v_max3 in
+# int32), local adds the 0 floor and the running best (+2). A packed int16 instruction
+# advances two cells, so the i16x2 kernels spend half as many per cell.
+K_MIN = {("i16x2", "global"): 4.0, ("i16x2", "local"): 5.0,
+         ("i32", "global"): 7.0, ("i32", "local"): 9.0}
+# VALU lane-instruction issue peak of MI355X: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz.
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 
 
@@ -134,6 +138,21 @@ def cpu_baseline(qs, rs, sim, rows: int) -> dict:
                       f"global + local ({pairs} pairs, {dt:.1f} s wall)"}
 
 
+def copy_calibration() -> int:
+    """Three copies of a 1 GiB int32 tensor: 1 GiB read + 1 GiB written per dispatch."""
+    import torch
+
+    n = 256 * 1024 * 1024
+    x = torch.ones(n, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_dispatch": 4 * n, "dispatches": 3}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,7 +160,12 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-rows", type=int, default=1000,
                     help="query rows of the CPU baseline sample (0 skips it)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (scale for rocprofv3 --pmc "
+                         "FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the all-vs-all matrix shards by query rows with no collective, see DESIGN.md)",
@@ -216,12 +240,12 @@ def main() -> int:
         in_class = (q_len <= rows) & (q_len > rows - 4)
         cells = float(q_len[in_class].sum()) * float(r_len.sum())
         avg_s = k_ms / launches / 1e3
-        achieved = cells * K_MIN[mode] / avg_s / 1e12
+        achieved = cells * K_MIN[(cell, mode)] / avg_s / 1e12
         roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2),
                     "unit": "Tops/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
                     "kernel": label, "launches": launches,
                     "avg_launch_ms": round(k_ms / launches, 4),
-                    "cells_per_launch": cells, "ops_per_cell": K_MIN[mode]}
+                    "cells_per_launch": cells, "ops_per_cell": K_MIN[(cell, mode)]}
 
     cpu = None
     if args.cpu_rows > 0:

@@ -351,7 +351,8 @@ void gotoh_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, uint32_t 
     const int64_t B = (64 + int64_t(r_max)) * mx;
     const int64_t G = std::max<int64_t>(go, ge);
     bool fast = B + 2 * G < (int64_t(1) << 30);
-    const bool packed16 = B + 2 * G < (int64_t(1) << 14);
+    // the packed cell uses the H form only, which needs gap_open >= gap_extend
+    const bool packed16 = B + 2 * G < (int64_t(1) << 14) && go >= ge;
     if (dist) fast = fast && 4 * (int64_t(q_max) + int64_t(r_max)) * mx < int64_t(INT32_MAX);
 
     ClassBounds cb{};

@@ -62,7 +62,15 @@ struct I16x2 {
 };
 
 // Register-stripe Gotoh: workgroup = (C::Q queries of one length class, reference chunk).
-template <class C, int R, bool LOCAL>
+//
+// HFORM (used when gap_open >= gap_extend): because X - go <= X - ge and Y - go <= Y - ge,
+//   X[i][j] = max(H[i-1][j] - go, X[i-1][j] - ge),  Y[i][j] = max(H[i][j-1] - go, Y[i][j-1] - ge)
+// with H = max(M, X, Y). Keeping Hg = H - go per cell and folding go into the profile
+// (prof = s + go, so M = Hg_diag + prof) leaves 8 instructions per cell (10 local):
+//   M = Hgd + s';  X = max(Hg_up, X_up - ge);  Y = max(Hg_left, Y_left - ge);
+//   Hg = max(M, X, Y) - go
+// and two registers per stripe row (Hg, Y). Otherwise the U/V form above is used.
+template <class C, int R, bool LOCAL, bool HFORM>
 __global__ __launch_bounds__(kThreads) void gotoh_reg_kernel(GotohArgs a,
                                                              const uint32_t* __restrict__ list,
                                                              uint32_t count, uint64_t base) {
@@ -89,13 +97,14 @@ __global__ __launch_bounds__(kThreads) void gotoh_reg_kernel(GotohArgs a,
         q[h] = a.q_codes + q0;
     }
     const uint32_t A = a.A;
-    const int32_t dummy = LOCAL ? C::kNeg : 0;  // rows past a query's end can never win
+    const int32_t shift = HFORM ? a.gap_open : 0;  // profile holds s + go in the H form
+    const int32_t dummy = LOCAL ? C::kNeg : shift;  // rows past a query's end never win
     for (uint32_t e = tid; e < uint32_t(R) * 32; e += kThreads) {
         const uint32_t r = e >> 5, b = e & 31u;
         int32_t v[Q];
 #pragma unroll
         for (int h = 0; h < Q; ++h)
-            v[h] = (r < m[h] && b < A) ? a.sim[uint32_t(q[h][r]) * A + b] : dummy;
+            v[h] = (r < m[h] && b < A) ? a.sim[uint32_t(q[h][r]) * A + b] + shift : dummy;
         prof[e] = C::pack(v);
     }
     __syncthreads();
@@ -110,46 +119,62 @@ __global__ __launch_bounds__(kThreads) void gotoh_reg_kernel(GotohArgs a,
         if (k == kNoSlot) continue;
         const uint32_t n = a.refs.len[slot];
 
-        T V[R], Y[R], H[R];
+        // HFORM: Hs = H - go, Y.  U/V form: V = max(M, X), Y, Hs = H.
+        T V[HFORM ? 1 : R], Y[R], Hs[R];
+        {
+            // column 0: H[i][0] = -gap_run(i) (global) or 0 (local), Y = -inf, X = H
+            T gr = LOCAL ? zero : C::splat(-a.gap_open);
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const T gr = LOCAL ? zero : C::splat(-(a.gap_open + r * a.gap_extend));
-            V[r] = gr;
-            Y[r] = neg;
-            H[r] = gr;
+            for (int r = 0; r < R; ++r) {
+                Hs[r] = HFORM ? gr - go : gr;
+                if constexpr (!HFORM) V[r] = gr;
+                Y[r] = neg;
+                if (!LOCAL) gr = gr - ge;
+            }
         }
-        T hrow0 = zero;
+        T hrow0 = HFORM ? zero - go : zero;  // H[0][j] (minus go in the H form), j = 0
         T best = zero;
         const uint32_t* wp = a.refs.words + slot;
         uint32_t word = 0;
+        T g0 = C::splat(-a.gap_open);  // -gap_run(j + 1) on row 0
         for (uint32_t j = 0; j < n; ++j) {
             if ((j & 3u) == 0) word = wp[uint64_t(j >> 2) * npad];
             const uint32_t rc = (word >> (8 * (j & 3u))) & 0xFFu;
             const uint32_t* pcol = prof + rc;
-            T uup, hd;
+            T up, hd;  // H form: Hg_up ; U/V form: U_up
             T xup = neg;
             if (LOCAL) {
-                uup = zero;
-                hd = zero;
+                up = HFORM ? zero - go : zero;
+                hd = HFORM ? zero - go : zero;
             } else {
-                const T g0 = C::splat(-(a.gap_open + int32_t(j) * a.gap_extend));
-                uup = g0;
+                up = HFORM ? g0 - go : g0;
                 hd = hrow0;
-                hrow0 = g0;
+                hrow0 = up;
+                g0 = g0 - ge;
             }
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const T s = C::load(pcol[r * 32]);
                 T mv = hd + s;
                 if (LOCAL) mv = C::vmax(mv, zero);
-                const T xv = C::vmax(uup - go, xup - ge);
-                const T yv = C::vmax(V[r] - go, Y[r] - ge);
-                hd = H[r];
-                uup = C::vmax(mv, yv);
-                xup = xv;
-                V[r] = C::vmax(mv, xv);
-                Y[r] = yv;
-                H[r] = C::vmax(uup, xv);
+                if constexpr (HFORM) {
+                    const T xv = C::vmax(up, xup - ge);
+                    const T yv = C::vmax(Hs[r], Y[r] - ge);
+                    hd = Hs[r];
+                    Hs[r] = C::vmax(C::vmax(mv, xv), yv) - go;
+                    up = Hs[r];
+                    xup = xv;
+                    Y[r] = yv;
+                } else {
+                    const T xv = C::vmax(up - go, xup - ge);
+                    const T yv = C::vmax(V[r] - go, Y[r] - ge);
+                    hd = Hs[r];
+                    up = C::vmax(mv, yv);
+                    xup = xv;
+                    V[r] = C::vmax(mv, xv);
+                    Y[r] = yv;
+                    Hs[r] = C::vmax(up, xv);
+                }
                 if (LOCAL) best = C::vmax(best, mv);
             }
         }
@@ -162,10 +187,11 @@ __global__ __launch_bounds__(kThreads) void gotoh_reg_kernel(GotohArgs a,
             } else if (m[h] == 0) {
                 res = n == 0 ? 0 : -(a.gap_open + int32_t(n - 1) * a.gap_extend);
             } else {
-                res = C::half(H[0], h);
+                res = C::half(Hs[0], h);
 #pragma unroll
                 for (int r = 1; r < R; ++r)
-                    if (uint32_t(r) == m[h] - 1) res = C::half(H[r], h);
+                    if (uint32_t(r) == m[h] - 1) res = C::half(Hs[r], h);
+                if (HFORM) res += a.gap_open;
             }
             if (a.dist) res = int32_t(a.q_self[qi[h]] + a.r_self[k] - 2 * int64_t(res));
             stage[h][k - c * kChunk] = res;
@@ -282,20 +308,33 @@ __global__ __launch_bounds__(256) void gotoh_self_kernel(GotohArgs a,
     }
 }
 
-template <class C, int R>
-void launch_rows(const GotohArgs& a, const uint32_t* list, uint32_t count, hipStream_t s) {
+template <class C, int R, bool HFORM>
+void launch_form(const GotohArgs& a, const uint32_t* list, uint32_t count, hipStream_t s) {
     const uint64_t nch = a.refs.nchunks;
     const uint64_t groups = (uint64_t(count) + C::Q - 1) / C::Q;
     const uint64_t max_g = (uint64_t(1) << 30) / nch;
     for (uint64_t g0 = 0; g0 < groups; g0 += max_g) {
         const uint64_t ng = groups - g0 < max_g ? groups - g0 : max_g;
         if (a.local)
-            hipLaunchKernelGGL((gotoh_reg_kernel<C, R, true>), dim3(uint32_t(ng * nch)),
+            hipLaunchKernelGGL((gotoh_reg_kernel<C, R, true, HFORM>), dim3(uint32_t(ng * nch)),
                                dim3(kThreads), 0, s, a, list, count, g0);
         else
-            hipLaunchKernelGGL((gotoh_reg_kernel<C, R, false>), dim3(uint32_t(ng * nch)),
+            hipLaunchKernelGGL((gotoh_reg_kernel<C, R, false, HFORM>), dim3(uint32_t(ng * nch)),
                                dim3(kThreads), 0, s, a, list, count, g0);
     }
+}
+
+// The packed int16 cell always has gap_open >= gap_extend (the host routes the other case
+// to int32), so only the int32 cell instantiates the U/V form.
+template <class C, int R>
+void launch_rows(const GotohArgs& a, const uint32_t* list, uint32_t count, hipStream_t s) {
+    if constexpr (C::Q == 1) {
+        if (a.gap_open < a.gap_extend) {
+            launch_form<C, R, false>(a, list, count, s);
+            return;
+        }
+    }
+    launch_form<C, R, true>(a, list, count, s);
 }
 
 template <class C>

@@ -13,8 +13,10 @@ pytestmark = pytest.mark.gpu
 
 
 def _pen(st, matrix, alphabet="aa"):
+    """Penalty grid in codec order; unit matrices (any alphabet) are the 0/1 identity cost."""
     A = len(st.alphabet_symbols(alphabet))
-    if matrix is None:
+    if matrix is None or alphabet != "aa":
+        assert matrix is None or matrix.size() == A
         return (np.arange(A)[:, None] != np.arange(A)[None, :]).astype(np.int32)
     syms = st.alphabet_symbols(alphabet)
     return np.array([[matrix.penalty(a, b) for b in syms] for a in syms], np.int32)
