@@ -1,20 +1,27 @@
 #!/usr/bin/env python
-"""All-vs-all Gotoh scoring throughput on one MI355X.
+"""All-vs-all scoring throughput on one MI355X.
 
-Workload (one "step"): the 20,000 x 20,000 all-vs-all Gotoh affine score matrix of synthetic
-CDR3-like amino-acid strings ('C' + body + 'F', total length uniform in 8..25, body residues
-i.i.d. at Robinson & Robinson background frequencies, numpy PCG64 seed 2), BLOSUM62, gap_open 11,
-gap_extend 1, computed twice -- global (Needleman-Wunsch) and local (Smith-Waterman) -- into
-two device-resident int32 matrices. Inputs (codec-encoded CSR strings) are already in HBM when
-the timed region starts; each step runs the full device path (reference packing, query
-bucketing, scoring kernels) through the C-ABI entry point st_align_matrix_device on one HIP
-stream.
+Default workload (one "step", the headline line): the 20,000 x 20,000 all-vs-all Gotoh affine
+score matrix of synthetic CDR3-like amino-acid strings ('C' + body + 'F', total length uniform
+in 8..25, body residues i.i.d. at Robinson & Robinson background frequencies, numpy PCG64 seed
+2), BLOSUM62, gap_open 11, gap_extend 1, computed twice -- global (Needleman-Wunsch) and local
+(Smith-Waterman) -- into two device-resident int32 matrices. Inputs (codec-encoded CSR
+strings) are already in HBM when the timed region starts; each step runs the full device path
+(reference packing, query bucketing, scoring kernels) through the C-ABI entry point
+st_align_matrix_device on one HIP stream.
 
-Prints one JSON line: pairs/s (value), GCUPS, the roofline of the dominant kernel measured
-with HIP events on its own stream, and the reference's multithreaded CPU path timed on a
-bounded sample on this host.
+--workload selects the other kernels of the path on their own configurations:
+  gapblock     1,000 x 10,000 CDR3-like (lengths 10..20), BLOSUM62, gap_open 2*scale = 28,
+               gap_extend 1, central prior lambda 21
+  hamming      50,000 x 50,000 random ACGT UMIs of length 12
+  levenshtein  20,000 x 20,000 ACGT reads of length 40..60, half derived from the other half
+               by 1-3 random substitutions / insertions / deletions
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 1]
+Prints one JSON line: pairs/s (value), cell updates per second, the roofline of the dominant
+kernel measured with HIP events on the stream it runs on, and the reference's multithreaded CPU
+path timed on a bounded sample on this host.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--workload gotoh]
 """
 from __future__ import annotations
 
@@ -69,17 +76,40 @@ def cdr3_set(rng: np.random.Generator, n: int, lo: int, hi: int) -> list[np.ndar
     return out
 
 
+def dna_set(rng: np.random.Generator, n: int, lo: int, hi: int) -> list[np.ndarray]:
+    """Raw ACGT byte strings with length uniform in [lo, hi]."""
+    acgt = np.frombuffer(b"ACGT", np.uint8)
+    return [acgt[rng.integers(0, 4, size=int(L))] for L in rng.integers(lo, hi + 1, size=n)]
+
+
+def mutate(rng: np.random.Generator, s: np.ndarray) -> np.ndarray:
+    """1-3 random substitutions, insertions or deletions."""
+    acgt = np.frombuffer(b"ACGT", np.uint8)
+    t = list(s)
+    for _ in range(int(rng.integers(1, 4))):
+        op, p = int(rng.integers(0, 3)), int(rng.integers(0, len(t)))
+        if op == 0:
+            t[p] = acgt[rng.integers(0, 4)]
+        elif op == 1:
+            t.insert(p, acgt[rng.integers(0, 4)])
+        elif len(t) > 1:
+            del t[p]
+    return np.array(t, np.uint8)
+
+
 def csr(seqs: list[np.ndarray]) -> tuple[np.ndarray, np.ndarray]:
     off = np.zeros(len(seqs) + 1, np.uint64)
     off[1:] = np.cumsum([len(s) for s in seqs])
     return np.concatenate(seqs).astype(np.uint8), off
 
 
-def blosum62_sim() -> np.ndarray:
+def matrix_tables(name: str) -> tuple[np.ndarray, np.ndarray, int]:
     import seqtree_amd as st
 
-    m = st.SubstitutionMatrix.blosum62()
-    return np.array([[m.similarity(a, b) for b in AA_ORDER] for a in AA_ORDER], np.int32)
+    m = getattr(st.SubstitutionMatrix, name)()
+    sim = np.array([[m.similarity(a, b) for b in AA_ORDER] for a in AA_ORDER], np.int32)
+    pen = np.array([[m.penalty(a, b) for b in AA_ORDER] for a in AA_ORDER], np.int32)
+    return sim, pen, m.scale()
 
 
 class Lib:
@@ -90,6 +120,13 @@ class Lib:
         L.st_align_matrix_device.restype = i32
         L.st_align_matrix_device.argtypes = [vp, vp, u64, u32, vp, vp, u64, u32, vp, u32, i32,
                                              i32, i32, i32, vp, vp, vp]
+        L.st_gapblock_matrix_device.restype = i32
+        L.st_gapblock_matrix_device.argtypes = [vp, vp, u64, u32, vp, vp, u64, u32, vp, u32, i32,
+                                                i32, vp, u32, vp, vp, vp]
+        for name in ("st_hamming_matrix_device", "st_levenshtein_matrix_device"):
+            fn = getattr(L, name)
+            fn.restype = i32
+            fn.argtypes = [vp, vp, u64, u32, vp, vp, u64, u32, vp, vp, vp]
         L.st_timing_enable.argtypes = [i32]
         L.st_timing_collect.restype = i32
         L.st_timing_collect.argtypes = [ctypes.c_char_p, u64]
@@ -110,32 +147,267 @@ class Status(ctypes.Structure):
     _fields_ = [("code", ctypes.c_int32), ("message", ctypes.c_char * 256)]
 
 
-def cpu_baseline(qs, rs, sim, rows: int) -> dict:
-    """The reference's own align_score_matrix (oracle/_ref, compiled -O3 from its sources) on
-    the first `rows` queries against all references, global and local, on every host core."""
+def check(rc: int, st: Status, what: str) -> None:
+    if rc != 0:
+        raise RuntimeError(f"{what}: {rc} {st.message.decode()}")
+
+
+class DeviceSet:
+    """A CSR string set resident in HBM (torch tensors only hold the memory)."""
+
+    def __init__(self, torch, dev, seqs: list[np.ndarray]):
+        b, o = csr(seqs)
+        self.n = len(seqs)
+        self.lens = np.diff(o.astype(np.int64))
+        self.max_len = int(self.lens.max())
+        self.bytes = torch.from_numpy(b).to(dev)
+        self.off = torch.from_numpy(o.view(np.int64)).to(dev)
+
+
+def time_steps(torch, lib: Lib, stream, step, steps: int, warmup: int):
+    """Untimed warmup, then `steps` steps bracketed by HIP events on `stream`; per-kernel
+    launch times come from the library's own events on the same stream."""
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    lib.lib.st_timing_enable(1)
+    lib.timing()  # drop anything recorded before the timed region
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    lib.lib.st_timing_enable(0)
+    return ev0.elapsed_time(ev1) / steps, lib.timing()
+
+
+def traffic_for(label: str):
+    """HBM bytes per dispatch measured with rocprofv3 --pmc (profiles/traffic.json), or None."""
+    path = ROOT / "profiles" / "traffic.json"
+    if not path.exists():
+        return None
+    return json.loads(path.read_text()).get(label)
+
+
+def dominant(kernels):
+    label, (launches, ms) = max(kernels.items(), key=lambda kv: kv[1][1])
+    return label, launches, ms / launches / 1e3
+
+
+def cpu_reference(kind: str, q, r, rows: int, **kw) -> dict:
+    """The reference's own C++ (oracle/_ref, compiled -O3 from its sources) on the first `rows`
+    queries against every reference, on every host core; the plain-C port where the compiled
+    reference is absent."""
     import oracle
 
     threads = oracle.host_threads()
-    q = ["".join(AA_ORDER[c] for c in s) for s in qs[:rows]]
-    r = ["".join(AA_ORDER[c] for c in s) for s in rs]
-    if oracle.reference_available():
-        ref = oracle.Reference()
-        kind = "reference"
-        run = lambda local: ref.align_matrix(q, r, "blosum62", local, False, GAP_OPEN,  # noqa
-                                             GAP_EXTEND, threads=threads)
-    else:
-        orc = oracle.Oracle()
-        kind = "port"
-        run = lambda local: orc.align_matrix(q, r, sim, local, False, GAP_OPEN,  # noqa
-                                             GAP_EXTEND, threads=threads)[1]
+    qs = q[:rows]
+    have_ref = oracle.reference_available()
+    impl = oracle.Reference() if have_ref else oracle.Oracle()
     t0 = time.perf_counter()
-    for local in (False, True):
-        run(local)
+    if kind == "gotoh":
+        for local in (False, True):
+            if have_ref:
+                impl.align_matrix(qs, r, "blosum62", local, False, GAP_OPEN, GAP_EXTEND,
+                                  threads=threads)
+            else:
+                impl.align_matrix(qs, r, kw["sim"], local, False, GAP_OPEN, GAP_EXTEND,
+                                  threads=threads)
+        pairs = 2 * len(qs) * len(r)
+    elif kind == "gapblock":
+        if have_ref:
+            impl.gapblock_matrix(qs, r, "blosum62", kw["go"], 1, kw["cube"], kw["width"],
+                                 threads=threads)
+        else:
+            impl.gapblock_matrix(qs, r, kw["pen"], kw["go"], 1, kw["cube"], kw["width"],
+                                 threads=threads)
+        pairs = len(qs) * len(r)
+    else:
+        impl.distance_matrix(kind, qs, r, threads=threads)
+        pairs = len(qs) * len(r)
     dt = time.perf_counter() - t0
-    pairs = 2 * len(q) * len(r)
-    return {"value": pairs / dt, "unit": "pairs/s", "cores": threads, "kind": kind,
-            "sample": f"first {len(q)} queries x all {len(r)} references of the same workload, "
-                      f"global + local ({pairs} pairs, {dt:.1f} s wall)"}
+    return {"value": pairs / dt, "unit": "pairs/s", "cores": threads,
+            "kind": "reference" if have_ref else "port",
+            "sample": f"first {len(qs)} queries x all {len(r)} references of the same workload"
+                      f"{' (global + local)' if kind == 'gotoh' else ''}: {pairs} pairs in "
+                      f"{dt:.2f} s wall"}
+
+
+def base_line(metric, value, steps, warmup, ms_step, config, kernels, roofline, cpu, **extra):
+    line = {"metric": metric, "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": ms_step, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic"}
+    line.update(extra)
+    line.update({"config": config,
+                 "kernels_ms": {k: round(v[1] / steps, 4) for k, v in sorted(kernels.items())},
+                 "roofline": roofline, "cpu_baseline": cpu})
+    return line
+
+
+def valu_roofline(label, launches, avg_s, ops_per_launch, **info):
+    achieved = ops_per_launch / avg_s / 1e12
+    return {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2),
+            "unit": "Tops/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
+            "traffic": traffic_for(label), "kernel": label, "launches": launches, "avg_launch_ms": round(avg_s * 1e3, 4),
+            **info}
+
+
+# ------------------------------------------------------------------------------ workloads
+
+def run_gotoh(torch, dev, lib, stream, args) -> dict:
+    rng = np.random.default_rng(SEED)
+    qs = cdr3_set(rng, N_QUERIES, LEN_LO, LEN_HI)
+    rs = cdr3_set(rng, N_REFS, LEN_LO, LEN_HI)
+    Q, R = DeviceSet(torch, dev, qs), DeviceSet(torch, dev, rs)
+    sim, _, _ = matrix_tables("blosum62")
+    outs = {mode: torch.empty((Q.n, R.n), dtype=torch.int32, device=dev)
+            for mode in ("global", "local")}
+    st = Status()
+
+    def step():
+        for mode in ("global", "local"):
+            rc = lib.lib.st_align_matrix_device(
+                Q.bytes.data_ptr(), Q.off.data_ptr(), Q.n, Q.max_len, R.bytes.data_ptr(),
+                R.off.data_ptr(), R.n, R.max_len, sim.ctypes.data, len(AA_ORDER),
+                int(mode == "local"), 0, GAP_OPEN, GAP_EXTEND, outs[mode].data_ptr(),
+                ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st))
+            check(rc, st, "st_align_matrix_device")
+
+    ms_step, kernels = time_steps(torch, lib, stream, step, args.steps, args.warmup)
+    pairs_per_step = 2 * Q.n * R.n
+    cells_per_mode = float(Q.lens.sum()) * float(R.lens.sum())
+    # dominant kernel: its algorithmic work per launch is sum(m) * sum(n) over the queries of
+    # its length class, times the per-cell instruction count of its cell update
+    label, launches, avg_s = dominant(kernels)
+    roofline = None
+    if label.startswith("gotoh_i"):
+        _, cell, rtag, mode = label.split("_")
+        rows = int(rtag[1:])
+        in_class = (Q.lens <= rows) & (Q.lens > rows - 4)
+        cells = float(Q.lens[in_class].sum()) * float(R.lens.sum())
+        roofline = valu_roofline(label, launches, avg_s, cells * K_MIN[(cell, mode)],
+                                 cells_per_launch=cells, ops_per_cell=K_MIN[(cell, mode)])
+    cpu = None
+    if args.cpu_rows > 0:
+        to_str = lambda v: "".join(AA_ORDER[c] for c in v)  # noqa: E731
+        cpu = cpu_reference("gotoh", [to_str(s) for s in qs], [to_str(s) for s in rs],
+                            args.cpu_rows, sim=sim)
+    return base_line(
+        "all-vs-all pairs/s (Gotoh NW+SW score_matrix)", pairs_per_step / (ms_step / 1e3),
+        args.steps, args.warmup, ms_step,
+        {"workload": "gotoh_score_matrix_cdr3_20000x20000_blosum62_go11_ge1_global+local",
+         "queries": Q.n, "refs": R.n, "lengths": [LEN_LO, LEN_HI], "seed": SEED,
+         "pairs_per_step": pairs_per_step, "cells_per_step": 2 * cells_per_mode,
+         "parallelism": "single GPU"},
+        kernels, roofline, cpu, gcups=2 * cells_per_mode / (ms_step / 1e3) / 1e9)
+
+
+def run_gapblock(torch, dev, lib, stream, args) -> dict:
+    from seqtree_amd.gapblock import _prior_cube, central_prior
+
+    rng = np.random.default_rng(SEED + 1)
+    qs = cdr3_set(rng, 1000, 10, 20)
+    rs = cdr3_set(rng, 10000, 10, 20)
+    Q, R = DeviceSet(torch, dev, qs), DeviceSet(torch, dev, rs)
+    _, pen, scale = matrix_tables("blosum62")
+    go, lam = 2 * scale, int(1.5 * scale)
+    width = max(Q.max_len, R.max_len)
+    cube = np.array(_prior_cube(central_prior(lam), width), np.int32)
+    out = torch.empty((Q.n, R.n), dtype=torch.int32, device=dev)
+    st = Status()
+
+    def step():
+        rc = lib.lib.st_gapblock_matrix_device(
+            Q.bytes.data_ptr(), Q.off.data_ptr(), Q.n, Q.max_len, R.bytes.data_ptr(),
+            R.off.data_ptr(), R.n, R.max_len, pen.ctypes.data, len(AA_ORDER), go, 1,
+            cube.ctypes.data, width, out.data_ptr(), ctypes.c_void_p(stream.cuda_stream),
+            ctypes.byref(st))
+        check(rc, st, "st_gapblock_matrix_device")
+
+    ms_step, kernels = time_steps(torch, lib, stream, step, args.steps, args.warmup)
+    pairs = Q.n * R.n
+    # algorithmic lookups (SURVEY units): L per pair at d == 0, 2L at d > 0; the one-pass
+    # minimum spends 5 integer ops per block position when d > 0 (two prefix sums, their
+    # difference, the prior, the min) and 1 at d == 0
+    L = np.minimum(Q.lens[:, None], R.lens[None, :])
+    d = np.abs(Q.lens[:, None] - R.lens[None, :])
+    lookups = float(np.where(d > 0, 2 * L, L).sum())
+    ops = float(np.where(d > 0, 5 * L, L).sum())
+    label, launches, avg_s = dominant(kernels)
+    roofline = valu_roofline(label, launches, avg_s, ops, lookups_per_launch=lookups,
+                             ops_per_position=5) if label.startswith("gapblock") else None
+    cpu = None
+    if args.cpu_rows > 0:
+        to_str = lambda v: "".join(AA_ORDER[c] for c in v)  # noqa: E731
+        cpu = cpu_reference("gapblock", [to_str(s) for s in qs], [to_str(s) for s in rs],
+                            min(args.cpu_rows, Q.n), pen=pen, go=go, cube=cube.tolist(),
+                            width=width)
+    return base_line(
+        "all-vs-all pairs/s (gap-block score_matrix)", pairs / (ms_step / 1e3), args.steps,
+        args.warmup, ms_step,
+        {"workload": "gapblock_score_matrix_cdr3_1000x10000_blosum62_go28_ge1_central21",
+         "queries": Q.n, "refs": R.n, "lengths": [10, 20], "seed": SEED + 1,
+         "pairs_per_step": pairs, "lookups_per_step": lookups, "parallelism": "single GPU"},
+        kernels, roofline, cpu, glookups=lookups / (ms_step / 1e3) / 1e9)
+
+
+def run_distance(torch, dev, lib, stream, args, which: str) -> dict:
+    rng = np.random.default_rng(SEED + 2)
+    if which == "hamming":
+        a = dna_set(rng, 50000, 12, 12)
+        b = dna_set(rng, 50000, 12, 12)
+        fn = lib.lib.st_hamming_matrix_device
+    else:
+        base = dna_set(rng, 10000, 40, 60)
+        reads = base + [mutate(rng, s) for s in base]
+        order = rng.permutation(len(reads))
+        a = b = [reads[i] for i in order]
+        fn = lib.lib.st_levenshtein_matrix_device
+    A_, B_ = DeviceSet(torch, dev, a), DeviceSet(torch, dev, b)
+    out = torch.empty((A_.n, B_.n), dtype=torch.int32, device=dev)
+    st = Status()
+
+    def step():
+        rc = fn(A_.bytes.data_ptr(), A_.off.data_ptr(), A_.n, A_.max_len, B_.bytes.data_ptr(),
+                B_.off.data_ptr(), B_.n, B_.max_len, out.data_ptr(),
+                ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st))
+        check(rc, st, which)
+
+    ms_step, kernels = time_steps(torch, lib, stream, step, args.steps, args.warmup)
+    pairs = A_.n * B_.n
+    label, launches, avg_s = dominant(kernels)
+    extra = {}
+    if which == "hamming":
+        # HBM-write bound: the int32 matrix is the algorithmic traffic (4 B per pair)
+        achieved = 4.0 * pairs / avg_s / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0,
+                    "unit": "GB/s", "frac": round(achieved / 8000.0, 4),
+                    "traffic": traffic_for(label),
+                    "kernel": label, "launches": launches, "avg_launch_ms": round(avg_s * 1e3, 4),
+                    "bytes_per_launch": 4.0 * pairs}
+        workload = "hamming_matrix_acgt_umi12_50000x50000"
+    else:
+        cells = float(A_.lens.sum()) * float(B_.lens.sum())
+        # Myers: 17 64-bit word operations per text column, 2 VALU lane-ops each
+        cols = float(A_.n) * float(B_.lens.sum())
+        roofline = valu_roofline(label, launches, avg_s, cols * 34, columns_per_launch=cols,
+                                 ops_per_column=34) if label.startswith("levenshtein_myers") \
+            else None
+        extra["gcups"] = cells / (ms_step / 1e3) / 1e9
+        workload = "levenshtein_matrix_acgt_reads40-60_20000x20000"
+    cpu = None
+    if args.cpu_rows > 0:
+        rows = 500 if which == "hamming" else 200
+        cpu = cpu_reference(which, [bytes(x) for x in a], [bytes(x) for x in b],
+                            min(rows, args.cpu_rows))
+    return base_line(
+        f"all-vs-all pairs/s ({which}_matrix)", pairs / (ms_step / 1e3), args.steps,
+        args.warmup, ms_step,
+        {"workload": workload, "queries": A_.n, "refs": B_.n, "seed": SEED + 2,
+         "pairs_per_step": pairs, "parallelism": "single GPU"},
+        kernels, roofline, cpu, **extra)
 
 
 def copy_calibration() -> int:
@@ -160,6 +432,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-rows", type=int, default=1000,
                     help="query rows of the CPU baseline sample (0 skips it)")
+    ap.add_argument("--workload", default="gotoh",
+                    choices=["gotoh", "gapblock", "hamming", "levenshtein"])
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scale for rocprofv3 --pmc "
                          "FETCH_SIZE / WRITE_SIZE readings)")
@@ -178,101 +452,14 @@ def main() -> int:
         print("bench.py: no GPU visible", file=sys.stderr)
         return 1
     dev = torch.device("cuda:0")
-    rng = np.random.default_rng(SEED)
-    qs = cdr3_set(rng, N_QUERIES, LEN_LO, LEN_HI)
-    rs = cdr3_set(rng, N_REFS, LEN_LO, LEN_HI)
-    qb, qo = csr(qs)
-    rb, ro = csr(rs)
-    q_len = np.diff(qo.astype(np.int64))
-    r_len = np.diff(ro.astype(np.int64))
-    sim = blosum62_sim()
-    A = len(AA_ORDER)
-
-    tq_b = torch.from_numpy(qb).to(dev)
-    tq_o = torch.from_numpy(qo.view(np.int64)).to(dev)
-    tr_b = torch.from_numpy(rb).to(dev)
-    tr_o = torch.from_numpy(ro.view(np.int64)).to(dev)
-    outs = {mode: torch.empty((N_QUERIES, N_REFS), dtype=torch.int32, device=dev)
-            for mode in ("global", "local")}
     lib = Lib()
     stream = torch.cuda.Stream(device=dev)
-    st = Status()
-
-    def step():
-        for mode in ("global", "local"):
-            rc = lib.lib.st_align_matrix_device(
-                tq_b.data_ptr(), tq_o.data_ptr(), N_QUERIES, int(q_len.max()), tr_b.data_ptr(),
-                tr_o.data_ptr(), N_REFS, int(r_len.max()), sim.ctypes.data, A,
-                int(mode == "local"), 0, GAP_OPEN, GAP_EXTEND, outs[mode].data_ptr(),
-                ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st))
-            if rc != 0:
-                raise RuntimeError(f"st_align_matrix_device: {rc} {st.message.decode()}")
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    lib.lib.st_timing_enable(1)
-    lib.timing()  # drop anything recorded before the timed region
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        step()
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    lib.lib.st_timing_enable(0)
-    ms_total = ev0.elapsed_time(ev1)
-    ms_step = ms_total / args.steps
-    kernels = lib.timing()
-
-    pairs_per_step = 2 * N_QUERIES * N_REFS
-    cells_per_mode = float(q_len.sum()) * float(r_len.sum())
-    value = pairs_per_step / (ms_step / 1e3)
-    gcups = 2 * cells_per_mode / (ms_step / 1e3) / 1e9
-
-    # dominant kernel: the label with the largest total time; its algorithmic work per launch
-    # is sum(m) * sum(n) over the queries of its length class, times K_MIN ops per cell
-    label, (launches, k_ms) = max(kernels.items(), key=lambda kv: kv[1][1])
-    roofline = None
-    if label.startswith("gotoh_i"):
-        _, cell, rtag, mode = label.split("_")
-        rows = int(rtag[1:])
-        in_class = (q_len <= rows) & (q_len > rows - 4)
-        cells = float(q_len[in_class].sum()) * float(r_len.sum())
-        avg_s = k_ms / launches / 1e3
-        achieved = cells * K_MIN[(cell, mode)] / avg_s / 1e12
-        roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2),
-                    "unit": "Tops/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
-                    "kernel": label, "launches": launches,
-                    "avg_launch_ms": round(k_ms / launches, 4),
-                    "cells_per_launch": cells, "ops_per_cell": K_MIN[(cell, mode)]}
-
-    cpu = None
-    if args.cpu_rows > 0:
-        cpu = cpu_baseline(qs, rs, sim, args.cpu_rows)
-
-    line = {
-        "metric": "all-vs-all pairs/s (Gotoh NW+SW score_matrix)",
-        "value": value,
-        "unit": "pairs/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int32",
-        "data": "synthetic",
-        "gcups": gcups,
-        "config": {"workload": "gotoh_score_matrix_cdr3_20000x20000_blosum62_go11_ge1_global+local",
-                   "queries": N_QUERIES, "refs": N_REFS, "lengths": [LEN_LO, LEN_HI],
-                   "seed": SEED, "pairs_per_step": pairs_per_step,
-                   "cells_per_step": 2 * cells_per_mode, "parallelism": "single GPU"},
-        "kernels_ms": {k: round(v[1] / args.steps, 4) for k, v in sorted(kernels.items())},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-    }
+    if args.workload == "gotoh":
+        line = run_gotoh(torch, dev, lib, stream, args)
+    elif args.workload == "gapblock":
+        line = run_gapblock(torch, dev, lib, stream, args)
+    else:
+        line = run_distance(torch, dev, lib, stream, args, args.workload)
     print(json.dumps(line))
     return 0
 

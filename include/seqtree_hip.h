@@ -85,6 +85,30 @@ int32_t st_levenshtein_matrix(const uint8_t* a, const uint64_t* a_offsets, uint6
                               const uint8_t* b, const uint64_t* b_offsets, uint64_t n_b,
                               int32_t* out, st_status* status);
 
+/* Thresholded (sparse) gap-block output: every (query, reference) pair whose score is
+ * <= cutoff, as parallel arrays hit_query / hit_ref / hit_score in unspecified order.
+ * *hit_count receives the total number of such pairs; only the first `capacity` are stored,
+ * so a count above capacity tells the caller to retry with more room. This is the exhaustive
+ * counterpart of the reference's GapBlockIndex.search (python/seqtree/gapblock.py:567-673)
+ * without its d_max limit; it has no direct C++ counterpart in the reference. */
+int32_t st_gapblock_threshold(const uint8_t* q_codes, const uint64_t* q_offsets,
+                              uint64_t n_queries, const uint8_t* r_codes,
+                              const uint64_t* r_offsets, uint64_t n_refs, const int32_t* pen,
+                              uint32_t alphabet_size, int32_t gap_open, int32_t gap_extend,
+                              const int32_t* prior, uint32_t prior_width, int32_t cutoff,
+                              uint64_t capacity, uint32_t* hit_query, uint32_t* hit_ref,
+                              int32_t* hit_score, uint64_t* hit_count, st_status* status);
+
+/* Gap-block scores of an explicit list of pairs: out[t] = score(query pair_query[t],
+ * reference pair_ref[t]) with the same semantics as st_gapblock_matrix (the per-pair `cell`
+ * of src/gapblock.cpp:39-62). Indices past either set return ST_INVALID_ARGUMENT. */
+int32_t st_gapblock_pairs(const uint8_t* q_codes, const uint64_t* q_offsets, uint64_t n_queries,
+                          const uint8_t* r_codes, const uint64_t* r_offsets, uint64_t n_refs,
+                          const int32_t* pen, uint32_t alphabet_size, int32_t gap_open,
+                          int32_t gap_extend, const int32_t* prior, uint32_t prior_width,
+                          const uint32_t* pair_query, const uint32_t* pair_ref, uint64_t n_pairs,
+                          int32_t* out, st_status* status);
+
 /* ---- device-memory entry points ------------------------------------------------------
  * Same computations with the sequences (CSR bytes and offsets) and the output already in
  * device memory; the small tables (pen, sim, prior) stay host pointers. The work is enqueued
@@ -98,6 +122,25 @@ int32_t st_gapblock_matrix_device(const uint8_t* q_codes, const uint64_t* q_offs
                                   const int32_t* pen, uint32_t alphabet_size, int32_t gap_open,
                                   int32_t gap_extend, const int32_t* prior, uint32_t prior_width,
                                   int32_t* out, void* stream, st_status* status);
+
+int32_t st_gapblock_threshold_device(const uint8_t* q_codes, const uint64_t* q_offsets,
+                                     uint64_t n_queries, uint32_t q_max_len,
+                                     const uint8_t* r_codes, const uint64_t* r_offsets,
+                                     uint64_t n_refs, uint32_t r_max_len, const int32_t* pen,
+                                     uint32_t alphabet_size, int32_t gap_open, int32_t gap_extend,
+                                     const int32_t* prior, uint32_t prior_width, int32_t cutoff,
+                                     uint64_t capacity, uint32_t* hit_query, uint32_t* hit_ref,
+                                     int32_t* hit_score, uint64_t* hit_count, void* stream,
+                                     st_status* status);
+
+int32_t st_gapblock_pairs_device(const uint8_t* q_codes, const uint64_t* q_offsets,
+                                 uint64_t n_queries, uint32_t q_max_len, const uint8_t* r_codes,
+                                 const uint64_t* r_offsets, uint64_t n_refs, uint32_t r_max_len,
+                                 const int32_t* pen, uint32_t alphabet_size, int32_t gap_open,
+                                 int32_t gap_extend, const int32_t* prior, uint32_t prior_width,
+                                 const uint32_t* pair_query, const uint32_t* pair_ref,
+                                 uint64_t n_pairs, int32_t* out, void* stream,
+                                 st_status* status);
 
 int32_t st_align_matrix_device(const uint8_t* q_codes, const uint64_t* q_offsets,
                                uint64_t n_queries, uint32_t q_max_len, const uint8_t* r_codes,

@@ -225,12 +225,19 @@ int64_t iabs64(int64_t v) { return v < 0 ? -v : v; }
 // ------------------------------------------------------------------------------------------
 // gap-block
 // ------------------------------------------------------------------------------------------
-void gapblock_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, uint32_t q_max,
-                     const uint8_t* r, const uint64_t* r_off, uint64_t K, uint32_t r_max,
-                     const int32_t* pen_h, uint32_t A, int32_t go, int32_t ge,
-                     const int32_t* prior_h, uint32_t W, int32_t* out, hipStream_t s) {
-    if (N == 0 || K == 0) return;
-    Arena ar(s);
+// Uploads the tables, decides the evaluation variant and (for the matrix and threshold modes)
+// packs the references. Returns the launch arguments; `exact` / `profile` select the variant.
+struct GapblockPlan {
+    GapblockArgs a;
+    bool exact;
+    bool profile;
+};
+
+GapblockPlan gapblock_plan(Arena& ar, const uint8_t* q, const uint64_t* q_off, uint64_t N,
+                           uint32_t q_max, const uint8_t* r, const uint64_t* r_off, uint64_t K,
+                           uint32_t r_max, const int32_t* pen_h, uint32_t A, int32_t go,
+                           int32_t ge, const int32_t* prior_h, uint32_t W, bool pack,
+                           hipStream_t s) {
     int32_t* pen = ar.get<int32_t>(size_t(A) * A);
     h2d(pen, pen_h, size_t(A) * A, s);
     const size_t w1 = size_t(W) + 1;
@@ -245,32 +252,78 @@ void gapblock_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, uint32
     int64_t pen_abs = 0;
     for (size_t e = 0; e < size_t(A) * A; ++e) pen_abs = std::max(pen_abs, iabs64(pen_h[e]));
 
-    GapblockArgs a{};
+    GapblockPlan p{};
+    GapblockArgs& a = p.a;
     a.q_codes = q;
     a.q_off = q_off;
     a.n_queries = N;
     a.n_refs = K;
-    a.refs = pack_refs(ar, r, r_off, K, r_max, s);
+    if (pack) a.refs = pack_refs(ar, r, r_off, K, r_max, s);
     a.pen = pen;
     a.A = A;
     a.gap_open = go;
     a.gap_extend = ge;
     a.prior = prior;
     a.w1 = uint32_t(w1);
-    a.out = out;
-
     // No int32 sum can wrap below this bound, so the one-pass minimum is bit-identical to
     // the reference's suffix/prefix evaluation.
     const int64_t Lmax = std::max(q_max, r_max);
     const int64_t worst = 2 * Lmax * pen_abs + prior_abs + int64_t(go) + Lmax * int64_t(ge);
-    const bool exact = worst >= int64_t(INT32_MAX);
-    const bool profile = size_t(q_max) * A * sizeof(int32_t) <= 48 * 1024;
-    {
-        LaunchTimer t(std::string("gapblock") + (profile ? "_profile" : "_table") +
-                          (exact ? "_exact" : ""), s);
-        launch_gapblock(a, profile, q_max, exact, s);
-    }
+    p.exact = worst >= int64_t(INT32_MAX);
+    p.profile = size_t(q_max) * A * sizeof(int32_t) <= 48 * 1024;
+    return p;
+}
+
+void gapblock_run(const GapblockPlan& p, uint32_t q_max, hipStream_t s) {
+    LaunchTimer t(std::string(p.a.hits.count ? "gapblock_threshold" : "gapblock") +
+                      (p.profile ? "_profile" : "_table") + (p.exact ? "_exact" : ""),
+                  s);
+    launch_gapblock(p.a, p.profile, q_max, p.exact, s);
+}
+
+void gapblock_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, uint32_t q_max,
+                     const uint8_t* r, const uint64_t* r_off, uint64_t K, uint32_t r_max,
+                     const int32_t* pen_h, uint32_t A, int32_t go, int32_t ge,
+                     const int32_t* prior_h, uint32_t W, int32_t* out, hipStream_t s) {
+    if (N == 0 || K == 0) return;
+    Arena ar(s);
+    GapblockPlan p = gapblock_plan(ar, q, q_off, N, q_max, r, r_off, K, r_max, pen_h, A, go, ge,
+                                   prior_h, W, true, s);
+    p.a.out = out;
+    gapblock_run(p, q_max, s);
     launched("gapblock");
+}
+
+void gapblock_threshold_device(const uint8_t* q, const uint64_t* q_off, uint64_t N,
+                               uint32_t q_max, const uint8_t* r, const uint64_t* r_off,
+                               uint64_t K, uint32_t r_max, const int32_t* pen_h, uint32_t A,
+                               int32_t go, int32_t ge, const int32_t* prior_h, uint32_t W,
+                               const GapblockHits& hits, hipStream_t s) {
+    launch_fill_u32(reinterpret_cast<uint32_t*>(hits.count), 2, 0u, s);
+    if (N == 0 || K == 0) return;
+    Arena ar(s);
+    GapblockPlan p = gapblock_plan(ar, q, q_off, N, q_max, r, r_off, K, r_max, pen_h, A, go, ge,
+                                   prior_h, W, true, s);
+    p.a.hits = hits;
+    gapblock_run(p, q_max, s);
+    launched("gapblock_threshold");
+}
+
+void gapblock_pairs_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, uint32_t q_max,
+                           const uint8_t* r, const uint64_t* r_off, uint64_t K, uint32_t r_max,
+                           const int32_t* pen_h, uint32_t A, int32_t go, int32_t ge,
+                           const int32_t* prior_h, uint32_t W, const uint32_t* pq,
+                           const uint32_t* pr, uint64_t n_pairs, int32_t* out, hipStream_t s) {
+    if (n_pairs == 0) return;
+    Arena ar(s);
+    GapblockPlan p = gapblock_plan(ar, q, q_off, N, q_max, r, r_off, K, r_max, pen_h, A, go, ge,
+                                   prior_h, W, false, s);
+    p.a.out = out;
+    {
+        LaunchTimer t(std::string("gapblock_pairs") + (p.exact ? "_exact" : ""), s);
+        launch_gapblock_pairs(p.a, r, r_off, pq, pr, n_pairs, p.exact, s);
+    }
+    launched("gapblock_pairs");
 }
 
 // ------------------------------------------------------------------------------------------
@@ -668,6 +721,122 @@ int32_t st_gapblock_matrix(const uint8_t* q_codes, const uint64_t* q_offsets, ui
                                      rh.max_len, pen, alphabet_size, gap_open, gap_extend, prior,
                                      prior_width, dout, s);
                  });
+        return ok(status);
+    });
+}
+
+int32_t st_gapblock_threshold_device(const uint8_t* q_codes, const uint64_t* q_offsets,
+                                     uint64_t n_queries, uint32_t q_max_len,
+                                     const uint8_t* r_codes, const uint64_t* r_offsets,
+                                     uint64_t n_refs, uint32_t r_max_len, const int32_t* pen,
+                                     uint32_t alphabet_size, int32_t gap_open, int32_t gap_extend,
+                                     const int32_t* prior, uint32_t prior_width, int32_t cutoff,
+                                     uint64_t capacity, uint32_t* hit_query, uint32_t* hit_ref,
+                                     int32_t* hit_score, uint64_t* hit_count, void* stream,
+                                     st_status* status) {
+    if (alphabet_size == 0 || alphabet_size > kMaxAlphabet) return bad_alphabet(status, alphabet_size);
+    return guarded(status, [&] {
+        require_device();
+        GapblockHits h{cutoff, capacity, hit_query, hit_ref, hit_score,
+                       reinterpret_cast<unsigned long long*>(hit_count)};
+        gapblock_threshold_device(q_codes, q_offsets, n_queries, q_max_len, r_codes, r_offsets,
+                                  n_refs, r_max_len, pen, alphabet_size, gap_open, gap_extend,
+                                  prior, prior_width, h, static_cast<hipStream_t>(stream));
+        return ok(status);
+    });
+}
+
+int32_t st_gapblock_threshold(const uint8_t* q_codes, const uint64_t* q_offsets,
+                              uint64_t n_queries, const uint8_t* r_codes,
+                              const uint64_t* r_offsets, uint64_t n_refs, const int32_t* pen,
+                              uint32_t alphabet_size, int32_t gap_open, int32_t gap_extend,
+                              const int32_t* prior, uint32_t prior_width, int32_t cutoff,
+                              uint64_t capacity, uint32_t* hit_query, uint32_t* hit_ref,
+                              int32_t* hit_score, uint64_t* hit_count, st_status* status) {
+    if (alphabet_size == 0 || alphabet_size > kMaxAlphabet) return bad_alphabet(status, alphabet_size);
+    *hit_count = 0;
+    if (n_queries == 0 || n_refs == 0) return ok(status);
+    return guarded(status, [&] {
+        require_device();
+        const HostSet qh = host_set(q_codes, q_offsets, n_queries);
+        const HostSet rh = host_set(r_codes, r_offsets, n_refs);
+        OwnedStream os;
+        const hipStream_t s = os.get();
+        Arena ar(s);
+        const DeviceSet qd = upload(ar, qh, s);
+        const DeviceSet rd = upload(ar, rh, s);
+        const uint64_t cap = std::max<uint64_t>(capacity, 1);
+        GapblockHits h{cutoff, capacity, ar.get<uint32_t>(cap), ar.get<uint32_t>(cap),
+                       ar.get<int32_t>(cap), ar.get<unsigned long long>(1)};
+        gapblock_threshold_device(qd.bytes, qd.off, n_queries, qh.max_len, rd.bytes, rd.off,
+                                  n_refs, rh.max_len, pen, alphabet_size, gap_open, gap_extend,
+                                  prior, prior_width, h, s);
+        unsigned long long count = 0;
+        d2h(&count, h.count, 1, s);
+        sync(s);
+        const uint64_t kept = std::min<uint64_t>(count, capacity);
+        d2h(hit_query, h.query, kept, s);
+        d2h(hit_ref, h.ref, kept, s);
+        d2h(hit_score, h.score, kept, s);
+        sync(s);
+        *hit_count = count;
+        return ok(status);
+    });
+}
+
+int32_t st_gapblock_pairs_device(const uint8_t* q_codes, const uint64_t* q_offsets,
+                                 uint64_t n_queries, uint32_t q_max_len, const uint8_t* r_codes,
+                                 const uint64_t* r_offsets, uint64_t n_refs, uint32_t r_max_len,
+                                 const int32_t* pen, uint32_t alphabet_size, int32_t gap_open,
+                                 int32_t gap_extend, const int32_t* prior, uint32_t prior_width,
+                                 const uint32_t* pair_query, const uint32_t* pair_ref,
+                                 uint64_t n_pairs, int32_t* out, void* stream,
+                                 st_status* status) {
+    if (alphabet_size == 0 || alphabet_size > kMaxAlphabet) return bad_alphabet(status, alphabet_size);
+    return guarded(status, [&] {
+        require_device();
+        gapblock_pairs_device(q_codes, q_offsets, n_queries, q_max_len, r_codes, r_offsets,
+                              n_refs, r_max_len, pen, alphabet_size, gap_open, gap_extend, prior,
+                              prior_width, pair_query, pair_ref, n_pairs, out,
+                              static_cast<hipStream_t>(stream));
+        return ok(status);
+    });
+}
+
+int32_t st_gapblock_pairs(const uint8_t* q_codes, const uint64_t* q_offsets, uint64_t n_queries,
+                          const uint8_t* r_codes, const uint64_t* r_offsets, uint64_t n_refs,
+                          const int32_t* pen, uint32_t alphabet_size, int32_t gap_open,
+                          int32_t gap_extend, const int32_t* prior, uint32_t prior_width,
+                          const uint32_t* pair_query, const uint32_t* pair_ref, uint64_t n_pairs,
+                          int32_t* out, st_status* status) {
+    if (alphabet_size == 0 || alphabet_size > kMaxAlphabet) return bad_alphabet(status, alphabet_size);
+    if (n_pairs == 0) return ok(status);
+    for (uint64_t t = 0; t < n_pairs; ++t)
+        if (pair_query[t] >= n_queries || pair_ref[t] >= n_refs) {
+            set_status(status, ST_INVALID_ARGUMENT,
+                       "seqtree_hip: pair %llu indexes past the query or reference set",
+                       (unsigned long long)t);
+            return int(ST_INVALID_ARGUMENT);
+        }
+    return guarded(status, [&] {
+        require_device();
+        const HostSet qh = host_set(q_codes, q_offsets, n_queries);
+        const HostSet rh = host_set(r_codes, r_offsets, n_refs);
+        OwnedStream os;
+        const hipStream_t s = os.get();
+        Arena ar(s);
+        const DeviceSet qd = upload(ar, qh, s);
+        const DeviceSet rd = upload(ar, rh, s);
+        uint32_t* pq = ar.get<uint32_t>(n_pairs);
+        uint32_t* pr = ar.get<uint32_t>(n_pairs);
+        int32_t* dout = ar.get<int32_t>(n_pairs);
+        h2d(pq, pair_query, n_pairs, s);
+        h2d(pr, pair_ref, n_pairs, s);
+        gapblock_pairs_device(qd.bytes, qd.off, n_queries, qh.max_len, rd.bytes, rd.off, n_refs,
+                              rh.max_len, pen, alphabet_size, gap_open, gap_extend, prior,
+                              prior_width, pq, pr, n_pairs, dout, s);
+        d2h(out, dout, n_pairs, s);
+        sync(s);
         return ok(status);
     });
 }

@@ -139,31 +139,42 @@ std::vector<int32_t> align_dist_matrix(const std::vector<std::string>& queries,
     return pairwise_matrix(queries, refs, mat, alphabet, mode, gap_open, gap_extend, true);
 }
 
-std::vector<int32_t> gapblock_matrix(const std::vector<std::string>& queries,
-                                     const std::vector<std::string>& refs, Alphabet alphabet,
-                                     const SubstitutionMatrix* matrix, int32_t gap_open,
-                                     int32_t gap_extend, const std::vector<int32_t>& prior,
-                                     uint32_t prior_width, int /*threads*/) {
+namespace {
+
+// Validation and encoding shared by the gap-block entry points, in the reference's order
+// (gapblock.cpp:71-102): gap costs, matrix size, [empty -> nothing to do], query symbols,
+// ref symbols, prior size, prior width.
+struct GapblockInput {
+    Csr q, r;
+    std::vector<int32_t> pen;
+    uint8_t A = 0;
+    bool empty = false;
+};
+
+GapblockInput gapblock_input(const std::vector<std::string>& queries,
+                             const std::vector<std::string>& refs, Alphabet alphabet,
+                             const SubstitutionMatrix* matrix, int32_t gap_open,
+                             int32_t gap_extend, const std::vector<int32_t>& prior,
+                             uint32_t prior_width) {
     if (gap_open < 0 || gap_extend < 0)
         throw std::invalid_argument("gap_open and gap_extend must be >= 0");
     const Codec codec(alphabet);
-    const uint8_t A = codec.size();
+    GapblockInput in;
+    in.A = codec.size();
+    const uint8_t A = in.A;
     if (matrix && matrix->size() != A)
         throw std::invalid_argument("matrix size does not match the alphabet");
-
-    const size_t N = queries.size(), K = refs.size();
-    std::vector<int32_t> out(N * K);
-    if (N == 0 || K == 0) return out;
-
-    std::vector<int32_t> pen(size_t(A) * A);
+    if (queries.empty() || refs.empty()) {
+        in.empty = true;
+        return in;
+    }
+    in.pen.resize(size_t(A) * A);
     for (uint32_t a = 0; a < A; ++a)
         for (uint32_t b = 0; b < A; ++b)
-            pen[a * A + b] = matrix ? matrix->penalty(uint8_t(a), uint8_t(b)) : int32_t(a != b);
-
-    const Csr q = pack_gapblock(queries, codec);
-    const Csr r = pack_gapblock(refs, codec);
-    const uint32_t longest = std::max(q.max_len, r.max_len);
-
+            in.pen[a * A + b] = matrix ? matrix->penalty(uint8_t(a), uint8_t(b)) : int32_t(a != b);
+    in.q = pack_gapblock(queries, codec);
+    in.r = pack_gapblock(refs, codec);
+    const uint32_t longest = std::max(in.q.max_len, in.r.max_len);
     const size_t W1 = size_t(prior_width) + 1;
     if (!prior.empty()) {
         if (prior.size() != W1 * W1 * W1)
@@ -171,6 +182,97 @@ std::vector<int32_t> gapblock_matrix(const std::vector<std::string>& queries,
         if (longest > prior_width)
             throw std::invalid_argument("a sequence is longer than the prior table's width");
     }
+    return in;
+}
+
+}  // namespace
+
+GapblockHits gapblock_threshold(const std::vector<std::string>& queries,
+                                const std::vector<std::string>& refs, Alphabet alphabet,
+                                const SubstitutionMatrix* matrix, int32_t gap_open,
+                                int32_t gap_extend, const std::vector<int32_t>& prior,
+                                uint32_t prior_width, int32_t cutoff) {
+    const GapblockInput in = gapblock_input(queries, refs, alphabet, matrix, gap_open, gap_extend,
+                                            prior, prior_width);
+    GapblockHits hits;
+    if (in.empty) return hits;
+    const size_t N = queries.size(), K = refs.size();
+    uint64_t capacity = std::min<uint64_t>(uint64_t(N) * K, uint64_t(1) << 22);
+    for (;;) {
+        hits.query.resize(capacity);
+        hits.ref.resize(capacity);
+        hits.score.resize(capacity);
+        uint64_t count = 0;
+        st_status st{};
+        raise(st_gapblock_threshold(in.q.bytes.data(), in.q.off.data(), N, in.r.bytes.data(),
+                                    in.r.off.data(), K, in.pen.data(), in.A, gap_open,
+                                    gap_extend, prior.empty() ? nullptr : prior.data(),
+                                    prior_width, cutoff, capacity, hits.query.data(),
+                                    hits.ref.data(), hits.score.data(), &count, &st),
+              st);
+        if (count <= capacity) {
+            hits.query.resize(count);
+            hits.ref.resize(count);
+            hits.score.resize(count);
+            break;
+        }
+        capacity = count;  // the first pass counted every hit; the second stores them all
+    }
+    // row-major order, independent of the device's append order
+    std::vector<uint64_t> order(hits.query.size());
+    for (size_t t = 0; t < order.size(); ++t) order[t] = t;
+    std::sort(order.begin(), order.end(), [&](uint64_t x, uint64_t y) {
+        return hits.query[x] != hits.query[y] ? hits.query[x] < hits.query[y]
+                                              : hits.ref[x] < hits.ref[y];
+    });
+    GapblockHits sorted;
+    sorted.query.reserve(order.size());
+    sorted.ref.reserve(order.size());
+    sorted.score.reserve(order.size());
+    for (uint64_t t : order) {
+        sorted.query.push_back(hits.query[t]);
+        sorted.ref.push_back(hits.ref[t]);
+        sorted.score.push_back(hits.score[t]);
+    }
+    return sorted;
+}
+
+std::vector<int32_t> gapblock_pairs(const std::vector<std::string>& queries,
+                                    const std::vector<std::string>& refs, Alphabet alphabet,
+                                    const SubstitutionMatrix* matrix, int32_t gap_open,
+                                    int32_t gap_extend, const std::vector<int32_t>& prior,
+                                    uint32_t prior_width, const std::vector<uint32_t>& pair_query,
+                                    const std::vector<uint32_t>& pair_ref) {
+    if (pair_query.size() != pair_ref.size())
+        throw std::invalid_argument("pair_query and pair_ref must have the same length");
+    const GapblockInput in = gapblock_input(queries, refs, alphabet, matrix, gap_open, gap_extend,
+                                            prior, prior_width);
+    std::vector<int32_t> out(pair_query.size());
+    if (out.empty()) return out;
+    if (in.empty) throw std::invalid_argument("pairs index an empty query or reference set");
+    st_status st{};
+    raise(st_gapblock_pairs(in.q.bytes.data(), in.q.off.data(), queries.size(), in.r.bytes.data(),
+                            in.r.off.data(), refs.size(), in.pen.data(), in.A, gap_open,
+                            gap_extend, prior.empty() ? nullptr : prior.data(), prior_width,
+                            pair_query.data(), pair_ref.data(), out.size(), out.data(), &st),
+          st);
+    return out;
+}
+
+std::vector<int32_t> gapblock_matrix(const std::vector<std::string>& queries,
+                                     const std::vector<std::string>& refs, Alphabet alphabet,
+                                     const SubstitutionMatrix* matrix, int32_t gap_open,
+                                     int32_t gap_extend, const std::vector<int32_t>& prior,
+                                     uint32_t prior_width, int /*threads*/) {
+    const GapblockInput in = gapblock_input(queries, refs, alphabet, matrix, gap_open, gap_extend,
+                                            prior, prior_width);
+    const size_t N = queries.size(), K = refs.size();
+    std::vector<int32_t> out(N * K);
+    if (in.empty) return out;
+    const Csr& q = in.q;
+    const Csr& r = in.r;
+    const std::vector<int32_t>& pen = in.pen;
+    const uint8_t A = in.A;
 
     st_status st{};
     const int rc = st_gapblock_matrix(q.bytes.data(), q.off.data(), N, r.bytes.data(),

@@ -2,6 +2,7 @@
 // parse_alphabet, 203-233 ScoreMatrix and gapblock_matrix, 240-315 SubstitutionMatrix,
 // 346-350 alphabet helpers, 470-615 ScoreMatrix buffer protocol and the matrix functions)
 // with the same names, argument names, defaults and error types, over the GPU C++ API.
+#include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
@@ -222,6 +223,57 @@ PYBIND11_MODULE(_core, m) {
         py::arg("prior") = std::vector<int32_t>{}, py::arg("prior_width") = 0,
         py::arg("threads") = 0,
         "Exhaustive single-gap-block penalties for every (query, ref) pair, GIL released.");
+
+    m.def(
+        "gapblock_threshold",
+        [](const std::vector<std::string>& queries, const std::vector<std::string>& refs,
+           int32_t cutoff, const std::string& alphabet,
+           const std::optional<SubstitutionMatrix>& matrix, int32_t gap_open, int32_t gap_extend,
+           const std::vector<int32_t>& prior, uint32_t prior_width) {
+            const Alphabet a = parse_alphabet(alphabet);
+            GapblockHits h;
+            {
+                py::gil_scoped_release release;
+                h = gapblock_threshold(queries, refs, a, matrix ? &*matrix : nullptr, gap_open,
+                                       gap_extend, prior, prior_width, cutoff);
+            }
+            auto arr = [](const auto& v) {
+                using T = typename std::decay_t<decltype(v)>::value_type;
+                py::array_t<T> out(v.size());
+                std::copy(v.begin(), v.end(), out.mutable_data());
+                return out;
+            };
+            return py::make_tuple(arr(h.query), arr(h.ref), arr(h.score));
+        },
+        py::arg("queries"), py::arg("refs"), py::arg("cutoff"), py::arg("alphabet") = "aa",
+        py::arg("matrix") = std::nullopt, py::arg("gap_open") = 1, py::arg("gap_extend") = 1,
+        py::arg("prior") = std::vector<int32_t>{}, py::arg("prior_width") = 0,
+        "Every (query, ref) pair with gap-block score <= cutoff as three arrays (query index, "
+        "ref index, score), sorted by (query, ref). GIL released.");
+
+    m.def(
+        "gapblock_pairs",
+        [](const std::vector<std::string>& queries, const std::vector<std::string>& refs,
+           const std::vector<uint32_t>& pair_query, const std::vector<uint32_t>& pair_ref,
+           const std::string& alphabet, const std::optional<SubstitutionMatrix>& matrix,
+           int32_t gap_open, int32_t gap_extend, const std::vector<int32_t>& prior,
+           uint32_t prior_width) {
+            const Alphabet a = parse_alphabet(alphabet);
+            std::vector<int32_t> v;
+            {
+                py::gil_scoped_release release;
+                v = gapblock_pairs(queries, refs, a, matrix ? &*matrix : nullptr, gap_open,
+                                   gap_extend, prior, prior_width, pair_query, pair_ref);
+            }
+            py::array_t<int32_t> out(v.size());
+            std::copy(v.begin(), v.end(), out.mutable_data());
+            return out;
+        },
+        py::arg("queries"), py::arg("refs"), py::arg("pair_query"), py::arg("pair_ref"),
+        py::arg("alphabet") = "aa", py::arg("matrix") = std::nullopt, py::arg("gap_open") = 1,
+        py::arg("gap_extend") = 1, py::arg("prior") = std::vector<int32_t>{},
+        py::arg("prior_width") = 0,
+        "Gap-block scores of explicit (query index, ref index) pairs, as an int32 array.");
 
     m.def(
         "hamming",

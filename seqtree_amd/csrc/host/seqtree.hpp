@@ -91,6 +91,27 @@ std::vector<int32_t> gapblock_matrix(const std::vector<std::string>& queries,
                                      int32_t gap_extend, const std::vector<int32_t>& prior,
                                      uint32_t prior_width, int threads = 0);
 
+// Every (query, ref) pair whose gap-block score is <= cutoff, sorted by (query, ref). The
+// exhaustive counterpart of the reference's GapBlockIndex.search without a d_max bound.
+struct GapblockHits {
+    std::vector<uint32_t> query;
+    std::vector<uint32_t> ref;
+    std::vector<int32_t> score;
+};
+GapblockHits gapblock_threshold(const std::vector<std::string>& queries,
+                                const std::vector<std::string>& refs, Alphabet,
+                                const SubstitutionMatrix* matrix, int32_t gap_open,
+                                int32_t gap_extend, const std::vector<int32_t>& prior,
+                                uint32_t prior_width, int32_t cutoff);
+
+// Gap-block scores of explicit (query index, ref index) pairs.
+std::vector<int32_t> gapblock_pairs(const std::vector<std::string>& queries,
+                                    const std::vector<std::string>& refs, Alphabet,
+                                    const SubstitutionMatrix* matrix, int32_t gap_open,
+                                    int32_t gap_extend, const std::vector<int32_t>& prior,
+                                    uint32_t prior_width, const std::vector<uint32_t>& pair_query,
+                                    const std::vector<uint32_t>& pair_ref);
+
 // Unit-cost distances on raw, case-sensitive bytes.
 int hamming(std::string_view a, std::string_view b);
 int levenshtein(std::string_view a, std::string_view b);

@@ -55,6 +55,16 @@ void launch_classify(const uint64_t* off, uint64_t n, const ClassBounds& b, uint
 void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 
 // ---- gapblock.hip ----------------------------------------------------------------------
+// Thresholded output: every pair with score <= cutoff is appended as (query, ref, score);
+// count is a device counter that keeps counting past capacity (entries past it are dropped).
+struct GapblockHits {
+    int32_t cutoff;
+    uint64_t capacity;
+    uint32_t* query;
+    uint32_t* ref;
+    int32_t* score;
+    unsigned long long* count;  // nullptr: dense output
+};
 struct GapblockArgs {
     const uint8_t* q_codes;
     const uint64_t* q_off;
@@ -66,13 +76,18 @@ struct GapblockArgs {
     int32_t gap_open, gap_extend;
     const int32_t* prior;  // device cube or nullptr
     uint32_t w1;           // prior_width + 1
-    int32_t* out;
+    int32_t* out;          // dense N x K output (or one score per pair in pair-list mode)
+    GapblockHits hits;     // hits.count != nullptr selects the thresholded output
 };
 // use_profile: stage a per-query profile (q_max_len * A int32) in LDS.
 // exact_wrap: evaluate the candidates in the reference's order with wrapping int32
 // arithmetic (needed only when an int32 sum may overflow).
 void launch_gapblock(const GapblockArgs& a, bool use_profile, uint32_t q_max_len, bool exact_wrap,
                      hipStream_t s);
+// Scores an explicit list of (query, ref) pairs into a.out[t]; references come as CSR.
+void launch_gapblock_pairs(const GapblockArgs& a, const uint8_t* r_codes, const uint64_t* r_off,
+                           const uint32_t* pair_query, const uint32_t* pair_ref, uint64_t n_pairs,
+                           bool exact_wrap, hipStream_t s);
 
 // ---- gotoh.hip -------------------------------------------------------------------------
 struct GotohArgs {

@@ -145,3 +145,56 @@ def test_errors_and_buffer(gpu):
     assert mv[0, 0] == 0 and mv[1, 0] == sm[1, 0] > 0
     with pytest.raises(ValueError, match="negative"):
         st.score_matrix(["CAS"], ["CA"], m, 28, gap_prior=lambda i, d, n: -1)
+
+
+# ---------------------------------------------------------------------------- sparse output
+
+@pytest.mark.parametrize("cutoff", [-1, 0, 40, 90, 10**9])
+def test_threshold_equals_the_dense_matrix(gpu, oracle_lib, cutoff):
+    st = gpu
+    from seqtree_amd.gapblock import _prior_cube
+
+    rng = random.Random(cutoff & 0xFFFF)
+    q = cdr3_like(rng, 60, 6, 30)
+    r = cdr3_like(rng, 2500, 6, 30)
+    m = st.SubstitutionMatrix.blosum62()
+    prior = st.central_prior(21)
+    qi, ri, sc = st.gapblock.score_threshold(q, r, cutoff, m, 28, 1, prior)
+    w = max(len(s) for s in q + r)
+    dense = oracle_lib.gapblock_matrix(q, r, _pen(st, m), 28, 1, _prior_cube(prior, w), w,
+                                       threads=8)
+    want = np.argwhere(dense <= cutoff)
+    assert qi.dtype == np.uint32 and sc.dtype == np.int32
+    assert np.array_equal(np.stack([qi, ri], 1), want)  # sorted row-major
+    assert np.array_equal(sc, dense[want[:, 0], want[:, 1]])
+
+
+def test_threshold_grows_past_the_first_capacity(gpu):
+    """More hits than the first pass keeps (4 Mi): the count drives a second, complete pass."""
+    st = gpu
+    rng = random.Random(1)
+    q = cdr3_like(rng, 2100, 8, 12)
+    r = cdr3_like(rng, 2100, 8, 12)
+    qi, ri, sc = st.gapblock.score_threshold(q, r, 10**9, None, 1, 1)
+    assert len(qi) == 2100 * 2100
+    dense = np.asarray(st.score_matrix(q, r, None, 1, 1))
+    assert np.array_equal(sc, dense.ravel())
+
+
+def test_pairs_equal_the_dense_matrix(gpu, oracle_lib):
+    st = gpu
+    from seqtree_amd.gapblock import _prior_cube
+
+    rng = random.Random(4)
+    q = [rand_seq(rng, 0, 40) for _ in range(50)]
+    r = [rand_seq(rng, 0, 40) for _ in range(70)]
+    m = st.SubstitutionMatrix.pam250()
+    w = max(len(s) for s in q + r)
+    npr = np.random.default_rng(4)
+    pq = npr.integers(0, len(q), 5000).astype(np.uint32)
+    pr = npr.integers(0, len(r), 5000).astype(np.uint32)
+    for prior in (None, st.central_prior(9), lambda i, d, n: (2**31 - 1) - 3 * i):
+        got = st.gapblock.score_pairs(q, r, pq, pr, m, 17, 2, prior)
+        cube = _prior_cube(prior, w) if prior else None
+        dense = oracle_lib.gapblock_matrix(q, r, _pen(st, m), 17, 2, cube, w, threads=8)
+        assert np.array_equal(got, dense[pq, pr])

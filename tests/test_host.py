@@ -299,3 +299,16 @@ def test_compute_without_a_device_fails_loudly(st):
         st.score_matrix(["CAS"], ["CA"], st.SubstitutionMatrix.blosum62(), 28)
     with pytest.raises(RuntimeError, match="no HIP device"):
         st.distance.levenshtein_matrix(["A"], ["B"])
+
+
+def test_pair_list_validation(st):
+    m = st.SubstitutionMatrix.blosum62()
+    with pytest.raises(ValueError, match="indexes past"):
+        st.gapblock.score_pairs(["CAS"], ["CA"], [0, 1], [0, 0], m, 28)
+    with pytest.raises(ValueError, match="same length"):
+        st.gapblock.score_pairs(["CAS"], ["CA"], [0, 0], [0], m, 28)
+    with pytest.raises(ValueError, match="symbol '#'"):
+        st.gapblock.score_threshold(["CA#"], ["CA"], 10, m, 28)
+    assert len(st.gapblock.score_pairs(["CAS"], ["CA"], [], [], m, 28)) == 0
+    qi, ri, sc = st.gapblock.score_threshold([], ["CA"], 10, m, 28)
+    assert len(qi) == len(ri) == len(sc) == 0
