@@ -159,6 +159,48 @@ int32_t st_levenshtein_matrix_device(const uint8_t* a, const uint64_t* a_offsets
                                      const uint64_t* b_offsets, uint64_t n_b, uint32_t b_max_len,
                                      int32_t* out, void* stream, st_status* status);
 
+/* ---- ragged batch of gap-block jobs ---------------------------------------------------
+ * Many independent gap-block matrices of uneven sizes in one persistent launch. No
+ * counterpart in the reference (whose gapblock_matrix scores one job per call): each job is
+ * what one st_gapblock_matrix_device call would compute, with its sequences and output in
+ * device memory and its tables (pen, prior) on the host.
+ *
+ * st_gapblock_batch_plan_create uploads the tables, allocates the packed-reference buffers,
+ * chooses the tile shape (autotune != 0 times the candidate shapes on this batch once per
+ * process and workload signature; 0 takes the default or a memoised choice) and captures the
+ * whole batch -- reference packing, work-counter reset and the persistent scoring kernel --
+ * in a hipGraph. st_gapblock_batch_plan_run replays that graph on `stream`; the caller's
+ * sequence buffers must stay valid and unchanged in size for the plan's lifetime. */
+typedef struct st_gapblock_job {
+    const uint8_t* q_codes;
+    const uint64_t* q_offsets;
+    uint64_t n_queries;
+    uint32_t q_max_len;
+    const uint8_t* r_codes;
+    const uint64_t* r_offsets;
+    uint64_t n_refs;
+    uint32_t r_max_len;
+    const int32_t* pen;
+    uint32_t alphabet_size;
+    int32_t gap_open;
+    int32_t gap_extend;
+    const int32_t* prior;
+    uint32_t prior_width;
+    int32_t* out;
+} st_gapblock_job;
+
+typedef struct st_batch_plan st_batch_plan;
+
+int32_t st_gapblock_batch_plan_create(const st_gapblock_job* jobs, uint32_t n_jobs,
+                                      int32_t autotune, void* stream, st_batch_plan** plan,
+                                      st_status* status);
+int32_t st_gapblock_batch_plan_run(st_batch_plan* plan, void* stream, st_status* status);
+/* The chosen tile shape and the number of work units of the plan. */
+int32_t st_gapblock_batch_plan_tiles(const st_batch_plan* plan, uint32_t* queries_per_unit,
+                                     uint32_t* threads, uint32_t* workgroups,
+                                     uint64_t* work_units);
+void st_gapblock_batch_plan_destroy(st_batch_plan* plan);
+
 /* ---- library information ------------------------------------------------------------- */
 
 /* Version string of this library. */

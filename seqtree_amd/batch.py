@@ -1,0 +1,183 @@
+"""Ragged batches of gap-block jobs in one persistent, graph-captured launch.
+
+Each job is an ordinary gap-block score matrix (:func:`seqtree_amd.gapblock.score_matrix`
+semantics: its own queries, references, matrix, gap costs and prior); the batch scores all of
+them with one persistent kernel whose tile shape is chosen by a small autotuner, and replays
+the captured hipGraph on every :meth:`GapblockBatch.run`. The reference has no batch API; each
+job's result equals what its ``score_matrix`` would return.
+
+Device memory for the encoded sequences and the output matrices is held in PyTorch tensors
+(PyTorch is used only as an allocator and stream provider); the kernels and the plan live in
+``libseqtree_hip.so`` behind the C-ABI of ``include/seqtree_hip.h``.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from pathlib import Path
+from typing import Sequence
+
+import numpy as np
+
+from . import _core
+from .gapblock import GapPrior, _default_gap_open, _prior_cube
+
+_LIB = Path(__file__).resolve().parent / "libseqtree_hip.so"
+
+
+class _Status(ctypes.Structure):
+    _fields_ = [("code", ctypes.c_int32), ("message", ctypes.c_char * 256)]
+
+
+class _Job(ctypes.Structure):
+    _fields_ = [
+        ("q_codes", ctypes.c_void_p), ("q_offsets", ctypes.c_void_p),
+        ("n_queries", ctypes.c_uint64), ("q_max_len", ctypes.c_uint32),
+        ("r_codes", ctypes.c_void_p), ("r_offsets", ctypes.c_void_p),
+        ("n_refs", ctypes.c_uint64), ("r_max_len", ctypes.c_uint32),
+        ("pen", ctypes.c_void_p), ("alphabet_size", ctypes.c_uint32),
+        ("gap_open", ctypes.c_int32), ("gap_extend", ctypes.c_int32),
+        ("prior", ctypes.c_void_p), ("prior_width", ctypes.c_uint32),
+        ("out", ctypes.c_void_p),
+    ]
+
+
+def _lib():
+    lib = ctypes.CDLL(str(_LIB))
+    vp = ctypes.c_void_p
+    lib.st_gapblock_batch_plan_create.restype = ctypes.c_int32
+    lib.st_gapblock_batch_plan_create.argtypes = [ctypes.POINTER(_Job), ctypes.c_uint32,
+                                                  ctypes.c_int32, vp, ctypes.POINTER(vp),
+                                                  ctypes.POINTER(_Status)]
+    lib.st_gapblock_batch_plan_run.restype = ctypes.c_int32
+    lib.st_gapblock_batch_plan_run.argtypes = [vp, vp, ctypes.POINTER(_Status)]
+    lib.st_gapblock_batch_plan_tiles.restype = ctypes.c_int32
+    lib.st_gapblock_batch_plan_tiles.argtypes = [vp] + [ctypes.POINTER(ctypes.c_uint32)] * 3 + [
+        ctypes.POINTER(ctypes.c_uint64)]
+    lib.st_gapblock_batch_plan_destroy.argtypes = [vp]
+    return lib
+
+
+@dataclass
+class GapblockJob:
+    """One gap-block score matrix: the arguments of :func:`gapblock.score_matrix`."""
+
+    queries: Sequence[str]
+    refs: Sequence[str]
+    matrix: _core.SubstitutionMatrix | None = None
+    gap_open: int | None = None
+    gap_extend: int = 1
+    gap_prior: GapPrior | None = None
+    alphabet: str = "aa"
+
+
+def _encode(seqs: Sequence[str], symbols: str) -> tuple[np.ndarray, np.ndarray]:
+    lut = np.full(256, 255, np.uint8)
+    for code, ch in enumerate(symbols):
+        lut[ord(ch)] = code
+        lut[ord(ch.lower())] = code
+    raw = [s.encode("latin-1") for s in seqs]
+    off = np.zeros(len(raw) + 1, np.uint64)
+    off[1:] = np.cumsum([len(b) for b in raw])
+    data = lut[np.frombuffer(b"".join(raw) or b"\0", np.uint8)]
+    if (data[: int(off[-1])] == 255).any():
+        bad = next(ch for s in seqs for ch in s if lut[ord(ch)] == 255)
+        raise ValueError(f"symbol '{bad}' is not in the alphabet")
+    return data, off
+
+
+class GapblockBatch:
+    """A captured batch of gap-block jobs on the current GPU.
+
+    ``autotune=True`` times the candidate tile shapes on this batch (once per process and
+    workload shape) before capturing the graph. ``run()`` replays it on ``stream`` (a
+    ``torch.cuda.Stream``, default the current one); ``results()`` copies the matrices back.
+    """
+
+    def __init__(self, jobs: Sequence[GapblockJob], autotune: bool = False, stream=None):
+        import torch
+
+        self._torch = torch
+        self._lib = _lib()
+        dev = torch.device("cuda", torch.cuda.current_device())
+        self._stream = stream or torch.cuda.current_stream(dev)
+        self._keep = []  # host tables and device tensors referenced by the plan
+        self.shapes = []
+        self.outputs = []
+        descs = (_Job * len(jobs))()
+        for t, job in enumerate(jobs):
+            symbols = _core.alphabet_symbols(job.alphabet)
+            A = len(symbols)
+            m = job.matrix
+            if m is not None and m.size() != A:
+                raise ValueError("matrix size does not match the alphabet")
+            go = _default_gap_open(m) if job.gap_open is None else job.gap_open
+            if go < 0 or job.gap_extend < 0:
+                raise ValueError("gap_open and gap_extend must be >= 0")
+            pen = np.array([[m.penalty(a, b) if m is not None else int(a != b)
+                             for b in symbols] for a in symbols], np.int32) \
+                if (m is None or job.alphabet == "aa") else \
+                (1 - np.eye(A, dtype=np.int32))
+            qb, qo = _encode(job.queries, symbols)
+            rb, ro = _encode(job.refs, symbols)
+            width = max([len(s) for s in list(job.queries) + list(job.refs)] + [0])
+            cube = (np.array(_prior_cube(job.gap_prior, width), np.int32)
+                    if job.gap_prior is not None else None)
+            tq_b, tq_o = torch.from_numpy(qb).to(dev), torch.from_numpy(qo.view(np.int64)).to(dev)
+            tr_b, tr_o = torch.from_numpy(rb).to(dev), torch.from_numpy(ro.view(np.int64)).to(dev)
+            out = torch.empty((len(job.queries), len(job.refs)), dtype=torch.int32, device=dev)
+            self._keep += [pen, cube, tq_b, tq_o, tr_b, tr_o]
+            self.outputs.append(out)
+            self.shapes.append((len(job.queries), len(job.refs)))
+            d = descs[t]
+            d.q_codes, d.q_offsets = tq_b.data_ptr(), tq_o.data_ptr()
+            d.n_queries = len(job.queries)
+            d.q_max_len = max([len(s) for s in job.queries] + [0])
+            d.r_codes, d.r_offsets = tr_b.data_ptr(), tr_o.data_ptr()
+            d.n_refs = len(job.refs)
+            d.r_max_len = max([len(s) for s in job.refs] + [0])
+            d.pen = pen.ctypes.data
+            d.alphabet_size = A
+            d.gap_open, d.gap_extend = go, job.gap_extend
+            d.prior = cube.ctypes.data if cube is not None else None
+            d.prior_width = width
+            d.out = out.data_ptr()
+        st = _Status()
+        plan = ctypes.c_void_p()
+        rc = self._lib.st_gapblock_batch_plan_create(descs, len(jobs), int(autotune),
+                                                     ctypes.c_void_p(self._stream.cuda_stream),
+                                                     ctypes.byref(plan), ctypes.byref(st))
+        if rc != 0:
+            raise RuntimeError(f"st_gapblock_batch_plan_create: {st.message.decode()}")
+        self._plan = plan
+
+    def tiles(self) -> dict:
+        q, th, wg, units = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
+        self._lib.st_gapblock_batch_plan_tiles(self._plan, ctypes.byref(q), ctypes.byref(th),
+                                               ctypes.byref(wg), ctypes.byref(units))
+        return {"queries_per_unit": q.value, "threads": th.value, "workgroups": wg.value,
+                "work_units": units.value}
+
+    def run(self, stream=None) -> None:
+        st = _Status()
+        s = stream or self._stream
+        rc = self._lib.st_gapblock_batch_plan_run(self._plan, ctypes.c_void_p(s.cuda_stream),
+                                                  ctypes.byref(st))
+        if rc != 0:
+            raise RuntimeError(f"st_gapblock_batch_plan_run: {st.message.decode()}")
+
+    def results(self) -> list[np.ndarray]:
+        self._torch.cuda.synchronize()
+        return [o.cpu().numpy() for o in self.outputs]
+
+    def close(self) -> None:
+        if getattr(self, "_plan", None):
+            self._torch.cuda.synchronize()
+            self._lib.st_gapblock_batch_plan_destroy(self._plan)
+            self._plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 -- interpreter shutdown
+            pass

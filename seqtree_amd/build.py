@@ -27,7 +27,8 @@ CSRC = HERE / "csrc"
 HOST = CSRC / "host"
 BUILD = HERE / "_build"
 
-HIP_SOURCES = ["prep.hip", "gapblock.hip", "gotoh.hip", "distance.hip", "capi.hip"]
+HIP_SOURCES = ["prep.hip", "gapblock.hip", "batch.hip", "gotoh.hip", "distance.hip",
+               "capi.hip"]
 HOST_SOURCES = ["codec.cpp", "substitution_matrix.cpp", "matrix_data.cpp", "api.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

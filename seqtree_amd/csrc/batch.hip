@@ -1,0 +1,138 @@
+// Ragged multi-job gap-block scoring in one persistent launch.
+//
+// A batch is many independent gap-block jobs (each its own query set, reference set,
+// penalty table, gap costs and prior) of uneven sizes. Every job's query rows are cut into
+// work units of `queries_per_unit` queries; a unit is the full row block of those queries
+// against all of the job's references. Units are numbered job by job, heaviest jobs
+// (largest reference sets) first, and handed out through one device atomic counter to a
+// fixed grid of workgroups that loop until the counter runs past the last unit — so the
+// launch shape does not depend on the batch, and the whole batch (reference packing,
+// counter reset, scoring) can be captured once in a hipGraph and replayed.
+//
+// Inside a unit the workgroup builds each query's penalty profile in LDS and sweeps the
+// job's chunk-sorted references exactly as the single-job kernel does (same per-pair
+// routine, gapblock_core.hpp), staging each chunk's results in LDS for one coalesced row
+// write.
+
+#include "gapblock_core.hpp"
+
+namespace st {
+namespace {
+
+using gb::Look;
+using gb::PackedWords;
+using gb::pair_score;
+using gb::pair_score_uniform;
+
+template <bool PROFILE, bool EXACT, uint32_t THREADS>
+__device__ void run_unit(const BatchJob& J, uint64_t q_begin, uint32_t qt, int32_t* lds,
+                         int32_t* stage) {
+    constexpr uint32_t kSlots = kChunk / THREADS;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t A = J.A;
+    const uint64_t q_end = q_begin + qt < J.n_queries ? q_begin + qt : J.n_queries;
+    if constexpr (!PROFILE) {
+        for (uint32_t e = tid; e < A * A; e += THREADS) lds[e] = J.pen[e];
+        __syncthreads();
+    }
+    for (uint64_t i = q_begin; i < q_end; ++i) {
+        const uint64_t q0 = J.q_off[i];
+        const uint32_t m = uint32_t(J.q_off[i + 1] - q0);
+        const uint8_t* q = J.q_codes + q0;
+        int32_t* nprof = lds + m * A;  // PROFILE: the negated profile follows the profile
+        if constexpr (PROFILE) {
+            __syncthreads();  // the previous query's profile is no longer read
+            for (uint32_t e = tid; e < m * A; e += THREADS) {
+                const uint32_t t = e / A;
+                const int32_t v = J.pen[uint32_t(q[t]) * A + (e - t * A)];
+                lds[e] = v;
+                nprof[e] = int32_t(0u - uint32_t(v));
+            }
+            __syncthreads();
+        }
+        const Look<PROFILE> look{lds, q, A};
+        for (uint32_t c = 0; c < J.refs.nchunks; ++c) {
+#pragma unroll
+            for (uint32_t it = 0; it < kSlots; ++it) {
+                const uint32_t slot = c * kChunk + it * THREADS + tid;
+                const uint32_t k = J.refs.orig[slot];
+                const uint32_t n = J.refs.len[slot];
+                const PackedWords words{J.refs.words + slot, J.refs.npad};
+                const uint32_t n0 = __builtin_amdgcn_readfirstlane(n);
+                int32_t score = 0;
+                if (PROFILE && !EXACT && __all(k == kNoSlot || n == n0) &&
+                    (m < n0 ? m : n0) <= gb::kFastMaxL) {
+                    score = pair_score_uniform(lds, nprof, A, words, m, n0, J.prior, J.w1,
+                                               J.gap_open, J.gap_extend);
+                } else if (k != kNoSlot) {
+                    score = pair_score<PROFILE, EXACT>(look, words, m, n, J.prior, J.w1,
+                                                       J.gap_open, J.gap_extend);
+                }
+                if (k != kNoSlot) stage[k - c * kChunk] = score;
+            }
+            __syncthreads();
+            const uint64_t col0 = uint64_t(c) * kChunk;
+            const uint64_t lim64 = J.n_refs - col0;
+            const uint32_t lim = lim64 < kChunk ? uint32_t(lim64) : kChunk;
+            int32_t* row = J.out + i * J.n_refs + col0;
+            for (uint32_t t = tid; t < lim; t += THREADS) row[t] = stage[t];
+            __syncthreads();
+        }
+    }
+}
+
+template <uint32_t THREADS>
+__global__ __launch_bounds__(THREADS) void gapblock_batch_kernel(
+    const BatchJob* __restrict__ jobs, uint32_t n_jobs, uint64_t total_units, uint32_t qt,
+    unsigned long long* __restrict__ next_unit) {
+    extern __shared__ int32_t lds[];
+    __shared__ int32_t stage[kChunk];
+    __shared__ unsigned long long unit_sh;
+    __shared__ uint32_t job_sh;
+
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const unsigned long long u = atomicAdd(next_unit, 1ull);
+            unit_sh = u;
+            uint32_t lo = 0, hi = n_jobs;  // last job whose first unit is <= u
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) / 2;
+                if (jobs[mid].unit_begin <= u) lo = mid;
+                else hi = mid;
+            }
+            job_sh = lo;
+        }
+        __syncthreads();
+        const uint64_t u = unit_sh;
+        const uint32_t j = job_sh;
+        __syncthreads();  // every thread has read the unit before thread 0 takes the next
+        if (u >= total_units) break;
+        const BatchJob& J = jobs[j];
+        const uint64_t q_begin = (u - J.unit_begin) * qt;
+        if (J.profile) {
+            if (J.exact) run_unit<true, true, THREADS>(J, q_begin, qt, lds, stage);
+            else run_unit<true, false, THREADS>(J, q_begin, qt, lds, stage);
+        } else {
+            if (J.exact) run_unit<false, true, THREADS>(J, q_begin, qt, lds, stage);
+            else run_unit<false, false, THREADS>(J, q_begin, qt, lds, stage);
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+void launch_gapblock_batch(const BatchJob* jobs, uint32_t n_jobs, uint64_t total_units,
+                           const BatchTiles& tiles, size_t lds_bytes,
+                           unsigned long long* next_unit, hipStream_t s) {
+    if (total_units == 0) return;
+    const dim3 grid(tiles.workgroups);
+    if (tiles.threads == 512)
+        hipLaunchKernelGGL(gapblock_batch_kernel<512>, grid, dim3(512), lds_bytes, s, jobs, n_jobs,
+                           total_units, tiles.queries_per_unit, next_unit);
+    else
+        hipLaunchKernelGGL(gapblock_batch_kernel<256>, grid, dim3(256), lds_bytes, s, jobs, n_jobs,
+                           total_units, tiles.queries_per_unit, next_unit);
+}
+
+}  // namespace st

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -270,7 +271,7 @@ GapblockPlan gapblock_plan(Arena& ar, const uint8_t* q, const uint64_t* q_off, u
     const int64_t Lmax = std::max(q_max, r_max);
     const int64_t worst = 2 * Lmax * pen_abs + prior_abs + int64_t(go) + Lmax * int64_t(ge);
     p.exact = worst >= int64_t(INT32_MAX);
-    p.profile = size_t(q_max) * A * sizeof(int32_t) <= 48 * 1024;
+    p.profile = 2 * size_t(q_max) * A * sizeof(int32_t) <= 48 * 1024;  // profile + negation
     return p;
 }
 
@@ -324,6 +325,96 @@ void gapblock_pairs_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, 
         launch_gapblock_pairs(p.a, r, r_off, pq, pr, n_pairs, p.exact, s);
     }
     launched("gapblock_pairs");
+}
+
+// ------------------------------------------------------------------------------------------
+// ragged multi-job gap-block batch (st_gapblock_batch_*)
+// ------------------------------------------------------------------------------------------
+struct DeviceBuffer {
+    void* p = nullptr;
+    ~DeviceBuffer() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+struct BatchPlanImpl {
+    std::vector<st_gapblock_job> jobs;        // caller's descriptors (device sequences/outputs)
+    std::vector<std::unique_ptr<DeviceBuffer>> owned;
+    std::vector<BatchJob> dev_jobs;           // in unit order
+    std::vector<uint32_t> order;              // dev_jobs[t] is jobs[order[t]]
+    std::vector<PackedRefs> refs;             // per caller job
+    struct PackArgs {
+        const uint8_t* bytes;
+        const uint64_t* off;
+        uint64_t n;
+        uint32_t nwords, npad;
+        uint32_t *words, *len, *orig;
+    };
+    std::vector<PackArgs> packs;
+    BatchJob* d_jobs = nullptr;
+    unsigned long long* d_next = nullptr;
+    uint64_t total_units = 0;
+    size_t lds_bytes = 0;
+    BatchTiles tiles{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int cus = 256;
+
+    ~BatchPlanImpl() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+    template <class T>
+    T* alloc(size_t count) {
+        auto b = std::make_unique<DeviceBuffer>();
+        check(hipMalloc(&b->p, std::max<size_t>(count * sizeof(T), 256)), "hipMalloc");
+        T* out = static_cast<T*>(b->p);
+        owned.push_back(std::move(b));
+        return out;
+    }
+    // Orders units heaviest job first and uploads the descriptors for this tile shape.
+    void layout(uint32_t qt, hipStream_t s) {
+        uint64_t u = 0;
+        for (auto& j : dev_jobs) {
+            j.unit_begin = u;
+            u += (j.n_queries + qt - 1) / qt;
+        }
+        total_units = u;
+        h2d(d_jobs, dev_jobs.data(), dev_jobs.size(), s);
+    }
+    // The whole batch as stream work: pack every job's references, reset the unit counter
+    // (a fill kernel: no memset nodes in a captured graph), run the persistent kernel.
+    void enqueue(hipStream_t s) {
+        for (const auto& p : packs)
+            launch_pack_refs(p.bytes, p.off, p.n, p.nwords, p.npad, p.words, p.len, p.orig, s);
+        launch_fill_u32(reinterpret_cast<uint32_t*>(d_next), 2, 0u, s);
+        launch_gapblock_batch(d_jobs, uint32_t(dev_jobs.size()), total_units, tiles, lds_bytes,
+                              d_next, s);
+    }
+};
+
+// Tile shapes the autotuner tries (queries per unit, threads per workgroup, workgroups/CU).
+constexpr uint32_t kTuneQpu[] = {1, 4, 16};
+constexpr uint32_t kTuneThreads[] = {256, 512};
+constexpr uint32_t kTuneWgPerCu[] = {4, 8};
+
+// Process-wide memo of tuned shapes, keyed by a workload signature.
+std::mutex g_tune_mu;
+std::map<std::string, BatchTiles> g_tuned;
+
+std::string batch_signature(const std::vector<st_gapblock_job>& jobs) {
+    // job count, total queries/refs (log2 buckets) and the widest lengths
+    uint64_t nq = 0, nr = 0;
+    uint32_t lq = 0, lr = 0;
+    for (const auto& j : jobs) {
+        nq += j.n_queries;
+        nr += j.n_refs;
+        lq = std::max(lq, j.q_max_len);
+        lr = std::max(lr, j.r_max_len);
+    }
+    auto lg = [](uint64_t v) { int b = 0; while (v >>= 1) ++b; return b; };
+    return std::to_string(jobs.size()) + ":" + std::to_string(lg(nq)) + ":" +
+           std::to_string(lg(nr)) + ":" + std::to_string(lq) + ":" + std::to_string(lr);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -840,6 +931,171 @@ int32_t st_gapblock_pairs(const uint8_t* q_codes, const uint64_t* q_offsets, uin
         return ok(status);
     });
 }
+
+struct st_batch_plan {
+    BatchPlanImpl impl;
+};
+
+int32_t st_gapblock_batch_plan_create(const st_gapblock_job* jobs, uint32_t n_jobs,
+                                      int32_t autotune, void* stream, st_batch_plan** plan,
+                                      st_status* status) {
+    *plan = nullptr;
+    for (uint32_t t = 0; t < n_jobs; ++t)
+        if (jobs[t].alphabet_size == 0 || jobs[t].alphabet_size > kMaxAlphabet)
+            return bad_alphabet(status, jobs[t].alphabet_size);
+    return guarded(status, [&] {
+        require_device();
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        auto P = std::make_unique<st_batch_plan>();
+        BatchPlanImpl& B = P->impl;
+        B.jobs.assign(jobs, jobs + n_jobs);
+        int dev = 0;
+        hipDeviceProp_t prop;
+        check(hipGetDevice(&dev), "hipGetDevice");
+        check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+        B.cus = prop.multiProcessorCount;
+
+        // heaviest jobs first: a unit costs queries_per_unit x n_refs pairs
+        B.order.resize(n_jobs);
+        for (uint32_t t = 0; t < n_jobs; ++t) B.order[t] = t;
+        std::stable_sort(B.order.begin(), B.order.end(), [&](uint32_t x, uint32_t y) {
+            return jobs[x].n_refs > jobs[y].n_refs;
+        });
+        size_t lds = sizeof(int32_t);
+        for (uint32_t t : B.order) {
+            const st_gapblock_job& j = jobs[t];
+            if (j.n_queries == 0 || j.n_refs == 0) continue;
+            const uint32_t A = j.alphabet_size;
+            int32_t* pen = B.alloc<int32_t>(size_t(A) * A);
+            h2d(pen, j.pen, size_t(A) * A, s);
+            const size_t w1 = size_t(j.prior_width) + 1;
+            int32_t* prior = nullptr;
+            int64_t prior_abs = 0, pen_abs = 0;
+            if (j.prior) {
+                prior = B.alloc<int32_t>(w1 * w1 * w1);
+                h2d(prior, j.prior, w1 * w1 * w1, s);
+                for (size_t e = 0; e < w1 * w1 * w1; ++e)
+                    prior_abs = std::max(prior_abs, iabs64(j.prior[e]));
+            }
+            for (size_t e = 0; e < size_t(A) * A; ++e) pen_abs = std::max(pen_abs, iabs64(j.pen[e]));
+            PackedRefs pr{};
+            pr.nchunks = uint32_t((j.n_refs + kChunk - 1) / kChunk);
+            pr.npad = pr.nchunks * kChunk;
+            pr.nwords = words_for(j.r_max_len);
+            uint32_t* words = B.alloc<uint32_t>(size_t(pr.nwords + 1) * pr.npad);
+            uint32_t* len = B.alloc<uint32_t>(pr.npad);
+            uint32_t* orig = B.alloc<uint32_t>(pr.npad);
+            pr.words = words;
+            pr.len = len;
+            pr.orig = orig;
+            B.packs.push_back({j.r_codes, j.r_offsets, j.n_refs, pr.nwords, pr.npad, words, len,
+                               orig});
+            BatchJob d{};
+            d.q_codes = j.q_codes;
+            d.q_off = j.q_offsets;
+            d.n_queries = j.n_queries;
+            d.n_refs = j.n_refs;
+            d.refs = pr;
+            d.pen = pen;
+            d.A = A;
+            d.gap_open = j.gap_open;
+            d.gap_extend = j.gap_extend;
+            d.prior = prior;
+            d.w1 = uint32_t(w1);
+            d.out = j.out;
+            const int64_t Lmax = std::max(j.q_max_len, j.r_max_len);
+            d.exact = 2 * Lmax * pen_abs + prior_abs + int64_t(j.gap_open) +
+                          Lmax * int64_t(j.gap_extend) >= int64_t(INT32_MAX);
+            d.profile = 2 * size_t(j.q_max_len) * A * sizeof(int32_t) <= 48 * 1024;
+            lds = std::max(lds, d.profile ? 2 * size_t(j.q_max_len) * A * sizeof(int32_t)
+                                          : size_t(A) * A * sizeof(int32_t));
+            B.dev_jobs.push_back(d);
+        }
+        B.lds_bytes = lds;
+        B.d_jobs = B.alloc<BatchJob>(std::max<size_t>(B.dev_jobs.size(), 1));
+        B.d_next = B.alloc<unsigned long long>(1);
+
+        // tile shape: memoised per workload signature, else tuned or the default
+        const std::string sig = batch_signature(B.jobs);
+        bool known = false;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tuned.find(sig);
+            if (it != g_tuned.end()) {
+                B.tiles = it->second;
+                known = true;
+            }
+        }
+        if (!known) {
+            B.tiles = {4, 256, uint32_t(B.cus) * 8};
+            if (autotune && !B.dev_jobs.empty()) {
+                hipEvent_t e0, e1;
+                check(hipEventCreate(&e0), "event");
+                check(hipEventCreate(&e1), "event");
+                float best = 1e30f;
+                BatchTiles best_t = B.tiles;
+                for (uint32_t qpu : kTuneQpu)
+                    for (uint32_t th : kTuneThreads)
+                        for (uint32_t wpc : kTuneWgPerCu) {
+                            B.tiles = {qpu, th, uint32_t(B.cus) * wpc};
+                            B.layout(qpu, s);
+                            B.enqueue(s);  // warm the candidate once
+                            check(hipEventRecord(e0, s), "event");
+                            B.enqueue(s);
+                            check(hipEventRecord(e1, s), "event");
+                            check(hipEventSynchronize(e1), "event");
+                            float ms = 0;
+                            check(hipEventElapsedTime(&ms, e0, e1), "event");
+                            if (ms < best) {
+                                best = ms;
+                                best_t = B.tiles;
+                            }
+                        }
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                B.tiles = best_t;
+                std::lock_guard<std::mutex> lk(g_tune_mu);
+                g_tuned[sig] = B.tiles;
+            }
+        }
+        B.layout(B.tiles.queries_per_unit, s);
+        sync(s);
+
+        // capture the batch once; every run replays the graph
+        hipStream_t cs = nullptr;
+        check(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "stream");
+        check(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "begin capture");
+        B.enqueue(cs);
+        const hipError_t ce = hipStreamEndCapture(cs, &B.graph);
+        (void)hipStreamDestroy(cs);
+        check(ce, "end capture");
+        check(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0), "graph instantiate");
+        *plan = P.release();
+        return ok(status);
+    });
+}
+
+int32_t st_gapblock_batch_plan_run(st_batch_plan* plan, void* stream, st_status* status) {
+    return guarded(status, [&] {
+        if (!plan) throw HipError{ST_INVALID_ARGUMENT, "seqtree_hip: null batch plan"};
+        LaunchTimer t("gapblock_batch_graph", static_cast<hipStream_t>(stream));
+        check(hipGraphLaunch(plan->impl.exec, static_cast<hipStream_t>(stream)), "graph launch");
+        return ok(status);
+    });
+}
+
+int32_t st_gapblock_batch_plan_tiles(const st_batch_plan* plan, uint32_t* queries_per_unit,
+                                     uint32_t* threads, uint32_t* workgroups,
+                                     uint64_t* work_units) {
+    if (!plan) return ST_INVALID_ARGUMENT;
+    *queries_per_unit = plan->impl.tiles.queries_per_unit;
+    *threads = plan->impl.tiles.threads;
+    *workgroups = plan->impl.tiles.workgroups;
+    *work_units = plan->impl.total_units;
+    return ST_OK;
+}
+
+void st_gapblock_batch_plan_destroy(st_batch_plan* plan) { delete plan; }
 
 int32_t st_align_matrix_device(const uint8_t* q_codes, const uint64_t* q_offsets,
                                uint64_t n_queries, uint32_t q_max_len, const uint8_t* r_codes,

@@ -16,121 +16,16 @@
 // order with wrapping arithmetic after a first pass for Sd, so the chosen minimum is the one
 // the reference's int32 code produces.
 
-#include "st_internal.hpp"
-
-#include <climits>
+#include "gapblock_core.hpp"
 
 namespace st {
 namespace {
 
-template <bool PROFILE>
-struct Look {
-    const int32_t* lds;
-    const uint8_t* q;
-    uint32_t A;
-    __device__ __forceinline__ int32_t operator()(uint32_t t, uint32_t rc) const {
-        if constexpr (PROFILE) {
-            return lds[t * A + rc];
-        } else {
-            return lds[uint32_t(q[t]) * A + rc];
-        }
-    }
-};
-
-// Reference residues as little-endian words: residues 4w..4w+3 of one sequence.
-struct PackedWords {  // the chunk-packed layout, one slot
-    const uint32_t* wp;
-    uint32_t npad;
-    __device__ __forceinline__ uint32_t operator()(uint32_t w) const {
-        return wp[uint64_t(w) * npad];
-    }
-};
-
-struct CsrWords {  // straight from the CSR bytes (pair-list mode)
-    const uint8_t* r;
-    uint32_t n;
-    __device__ __forceinline__ uint32_t operator()(uint32_t w) const {
-        uint32_t word = 0;
-        const uint32_t p = 4 * w;
-#pragma unroll
-        for (uint32_t b = 0; b < 4; ++b)
-            if (p + b < n) word |= uint32_t(r[p + b]) << (8 * b);
-        return word;
-    }
-};
-
-// Visits block-start positions t = 0..L-1 in order, handing the two penalties to f.
-template <bool PROFILE, class Words, class F>
-__device__ __forceinline__ void walk(const Look<PROFILE>& look, const Words& words, uint32_t L,
-                                     uint32_t d, bool q_longer, F&& f) {
-    for (uint32_t w = 0; 4 * w < L; ++w) {
-        const uint32_t x = words(w);
-        uint32_t y = 0;
-        if (!q_longer) {
-            const uint32_t t0 = 4 * w + d;
-            const uint32_t lo = words(t0 >> 2);
-            const uint32_t hi = words((t0 >> 2) + 1);
-            y = __builtin_amdgcn_alignbyte(hi, lo, t0 & 3u);
-        }
-#pragma unroll
-        for (uint32_t kk = 0; kk < 4; ++kk) {
-            const uint32_t t = 4 * w + kk;
-            if (t < L) {
-                const uint32_t rc = (x >> (8 * kk)) & 0xFFu;
-                const int32_t p0 = look(t, rc);
-                const int32_t pd =
-                    q_longer ? look(t + d, rc) : look(t, (y >> (8 * kk)) & 0xFFu);
-                f(t, p0, pd);
-            }
-        }
-    }
-}
-
-// Gap-block score of one pair (query of length m behind `look`, reference of length n behind
-// `words`). prior: the (w1)^3 cube or nullptr.
-template <bool PROFILE, bool EXACT, class Words>
-__device__ __forceinline__ int32_t pair_score(const Look<PROFILE>& look, const Words& words,
-                                              uint32_t m, uint32_t n, const int32_t* prior,
-                                              uint32_t w1, int32_t go, int32_t ge) {
-    const uint32_t L = m < n ? m : n;
-    const uint32_t d = m > n ? m - n : n - m;
-    const bool q_longer = m >= n;
-    const int32_t* prow =
-        (prior && d) ? prior + (uint64_t(q_longer ? m : n) * w1 + d) * w1 : nullptr;
-    if constexpr (!EXACT) {
-        uint32_t P0 = 0, Pd = 0;
-        int32_t best = prow ? prow[0] : 0;
-        walk<PROFILE>(look, words, L, d, q_longer, [&](uint32_t t, int32_t p0, int32_t pd) {
-            P0 += uint32_t(p0);
-            Pd += uint32_t(pd);
-            if (d) {
-                const int32_t cand = int32_t(P0 - Pd + uint32_t(prow ? prow[t + 1] : 0));
-                best = cand < best ? cand : best;
-            }
-        });
-        return d == 0 ? int32_t(Pd)
-                      : int32_t(Pd + uint32_t(best) + uint32_t(go) + (d - 1) * uint32_t(ge));
-    } else {
-        uint32_t Sd = 0;
-        walk<PROFILE>(look, words, L, d, q_longer,
-                      [&](uint32_t, int32_t, int32_t pd) { Sd += uint32_t(pd); });
-        if (d == 0) return int32_t(Sd);
-        uint32_t P0 = 0, Pd = 0;
-        int32_t best = INT_MAX;
-        // candidate i = t is formed before position t is consumed, i = L after the loop
-        walk<PROFILE>(look, words, L, d, q_longer, [&](uint32_t t, int32_t p0, int32_t pd) {
-            const uint32_t pr = prow ? uint32_t(prow[t]) : 0u;
-            const int32_t cand = int32_t((P0 + (Sd - Pd)) + pr);
-            best = cand < best ? cand : best;
-            P0 += uint32_t(p0);
-            Pd += uint32_t(pd);
-        });
-        const uint32_t pr = prow ? uint32_t(prow[L]) : 0u;
-        const int32_t cand = int32_t((P0 + (Sd - Pd)) + pr);
-        best = cand < best ? cand : best;
-        return int32_t(uint32_t(best) + uint32_t(go) + (d - 1) * uint32_t(ge));
-    }
-}
+using gb::CsrWords;
+using gb::Look;
+using gb::PackedWords;
+using gb::pair_score;
+using gb::pair_score_uniform;
 
 // Appends (query, ref, score) for every lane with hit set: one atomic per wave.
 __device__ __forceinline__ void append_hit(bool hit, uint32_t qi, uint32_t k, int32_t score,
@@ -166,10 +61,14 @@ __global__ __launch_bounds__(kThreads) void gapblock_kernel(GapblockArgs a, uint
     const uint8_t* q = a.q_codes + q0;
     const uint32_t A = a.A;
 
+    // PROFILE: prof[t][b] = pen[q[t]][b] followed by its negation (the fast path's -pd)
+    int32_t* nprof = lds + m * A;
     if constexpr (PROFILE) {
         for (uint32_t e = tid; e < m * A; e += kThreads) {
             const uint32_t t = e / A;
-            lds[e] = a.pen[uint32_t(q[t]) * A + (e - t * A)];
+            const int32_t v = a.pen[uint32_t(q[t]) * A + (e - t * A)];
+            lds[e] = v;
+            nprof[e] = int32_t(0u - uint32_t(v));
         }
     } else {
         for (uint32_t e = tid; e < A * A; e += kThreads) lds[e] = a.pen[e];
@@ -180,11 +79,19 @@ __global__ __launch_bounds__(kThreads) void gapblock_kernel(GapblockArgs a, uint
     for (uint32_t it = 0; it < kSlotsPerThread; ++it) {
         const uint32_t slot = c * kChunk + it * kThreads + tid;
         const uint32_t k = a.refs.orig[slot];
+        const uint32_t n = a.refs.len[slot];
+        const PackedWords words{a.refs.words + slot, a.refs.npad};
         int32_t score = 0;
-        if (k != kNoSlot)
-            score = pair_score<PROFILE, EXACT>(look, PackedWords{a.refs.words + slot, a.refs.npad},
-                                               m, a.refs.len[slot], a.prior, a.w1, a.gap_open,
+        // one reference length across the wave (padding lanes excepted): uniform fast path
+        const uint32_t n0 = __builtin_amdgcn_readfirstlane(n);
+        const bool uniform = __all(k == kNoSlot || n == n0);
+        if (PROFILE && !EXACT && uniform && (m < n0 ? m : n0) <= gb::kFastMaxL) {
+            score = pair_score_uniform(lds, nprof, A, words, m, n0, a.prior, a.w1, a.gap_open,
+                                       a.gap_extend);
+        } else if (k != kNoSlot) {
+            score = pair_score<PROFILE, EXACT>(look, words, m, n, a.prior, a.w1, a.gap_open,
                                                a.gap_extend);
+        }
         if constexpr (THRESH) {
             append_hit(k != kNoSlot && score <= a.hits.cutoff, uint32_t(i), k, score, a.hits);
         } else {
@@ -250,7 +157,8 @@ void launch_mode(const GapblockArgs& a, bool use_profile, size_t lds_bytes, bool
 void launch_gapblock(const GapblockArgs& a, bool use_profile, uint32_t q_max_len, bool exact_wrap,
                      hipStream_t s) {
     if (a.n_queries == 0 || a.refs.nchunks == 0) return;
-    const size_t lds = use_profile ? size_t(q_max_len) * a.A * sizeof(int32_t)
+    // profile mode holds the profile and its negation
+    const size_t lds = use_profile ? 2 * size_t(q_max_len) * a.A * sizeof(int32_t)
                                    : size_t(a.A) * a.A * sizeof(int32_t);
     const size_t lds_bytes = lds ? lds : sizeof(int32_t);
     if (a.hits.count) launch_mode<true>(a, use_profile, lds_bytes, exact_wrap, s);

@@ -89,6 +89,35 @@ void launch_gapblock_pairs(const GapblockArgs& a, const uint8_t* r_codes, const 
                            const uint32_t* pair_query, const uint32_t* pair_ref, uint64_t n_pairs,
                            bool exact_wrap, hipStream_t s);
 
+// ---- batch.hip -------------------------------------------------------------------------
+// One job of a ragged gap-block batch, as the persistent kernel reads it.
+struct BatchJob {
+    const uint8_t* q_codes;
+    const uint64_t* q_off;
+    uint64_t n_queries;
+    uint64_t n_refs;
+    PackedRefs refs;
+    const int32_t* pen;    // device, A*A
+    uint32_t A;
+    int32_t gap_open, gap_extend;
+    const int32_t* prior;  // device cube or nullptr
+    uint32_t w1;
+    int32_t* out;          // device, n_queries * n_refs
+    uint32_t profile;      // LDS profile (+ negation) rather than the A x A table
+    uint32_t exact;        // wrapping-order evaluation
+    uint64_t unit_begin;   // index of the job's first work unit
+};
+// Tile shape of the persistent launch: queries per work unit, threads per workgroup
+// (256 or 512) and the number of resident workgroups.
+struct BatchTiles {
+    uint32_t queries_per_unit;
+    uint32_t threads;
+    uint32_t workgroups;
+};
+void launch_gapblock_batch(const BatchJob* jobs, uint32_t n_jobs, uint64_t total_units,
+                           const BatchTiles& tiles, size_t lds_bytes,
+                           unsigned long long* next_unit, hipStream_t s);
+
 // ---- gotoh.hip -------------------------------------------------------------------------
 struct GotohArgs {
     const uint8_t* q_codes;
