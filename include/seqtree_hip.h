@@ -209,6 +209,15 @@ const char* st_version(void);
 /* Number of visible HIP devices (0 when none); never faults on a machine without a GPU. */
 int32_t st_device_count(void);
 
+/* ---- device memory (no counterpart in the reference) ------------------------------------
+ * Minimal allocation and synchronous copies so a caller without its own GPU allocator can
+ * use the device-memory entry points. kind: 1 host->device, 2 device->host, 3 device->device.
+ * st_synchronize waits for all work on the device. */
+int32_t st_device_alloc(uint64_t bytes, void** ptr, st_status* status);
+int32_t st_device_free(void* ptr);
+int32_t st_memcpy(void* dst, const void* src, uint64_t bytes, int32_t kind, st_status* status);
+int32_t st_synchronize(st_status* status);
+
 /* ---- instrumentation (no counterpart in the reference; used by the benchmark) ---------
  * st_timing_enable(1) makes every kernel launch record a pair of hipEvents on the stream it
  * runs on. st_timing_collect waits for the recorded events and writes one line per kernel

@@ -6,9 +6,10 @@ them with one persistent kernel whose tile shape is chosen by a small autotuner,
 the captured hipGraph on every :meth:`GapblockBatch.run`. The reference has no batch API; each
 job's result equals what its ``score_matrix`` would return.
 
-Device memory for the encoded sequences and the output matrices is held in PyTorch tensors
-(PyTorch is used only as an allocator and stream provider); the kernels and the plan live in
-``libseqtree_hip.so`` behind the C-ABI of ``include/seqtree_hip.h``.
+Device memory for the encoded sequences and the output matrices comes from the library's
+own allocator (``seqtree_amd._device``); the kernels and the plan live in
+``libseqtree_hip.so`` behind the C-ABI of ``include/seqtree_hip.h``. Runs use the device's
+null stream.
 """
 from __future__ import annotations
 
@@ -20,6 +21,7 @@ from typing import Sequence
 import numpy as np
 
 from . import _core
+from ._device import DeviceArray, synchronize
 from .gapblock import GapPrior, _default_gap_open, _prior_cube
 
 _LIB = Path(__file__).resolve().parent / "libseqtree_hip.so"
@@ -87,21 +89,17 @@ def _encode(seqs: Sequence[str], symbols: str) -> tuple[np.ndarray, np.ndarray]:
 
 
 class GapblockBatch:
-    """A captured batch of gap-block jobs on the current GPU.
+    """A captured batch of gap-block jobs on the GPU.
 
     ``autotune=True`` times the candidate tile shapes on this batch (once per process and
-    workload shape) before capturing the graph. ``run()`` replays it on ``stream`` (a
-    ``torch.cuda.Stream``, default the current one); ``results()`` copies the matrices back.
+    workload shape) before capturing the graph. ``run()`` replays it on ``stream`` (a raw
+    ``hipStream_t`` handle, default the null stream); ``results()`` copies the matrices back.
     """
 
-    def __init__(self, jobs: Sequence[GapblockJob], autotune: bool = False, stream=None):
-        import torch
-
-        self._torch = torch
+    def __init__(self, jobs: Sequence[GapblockJob], autotune: bool = False, stream: int = 0):
         self._lib = _lib()
-        dev = torch.device("cuda", torch.cuda.current_device())
-        self._stream = stream or torch.cuda.current_stream(dev)
-        self._keep = []  # host tables and device tensors referenced by the plan
+        self._stream = stream
+        self._keep = []  # host tables and device buffers referenced by the plan
         self.shapes = []
         self.outputs = []
         descs = (_Job * len(jobs))()
@@ -123,17 +121,17 @@ class GapblockBatch:
             width = max([len(s) for s in list(job.queries) + list(job.refs)] + [0])
             cube = (np.array(_prior_cube(job.gap_prior, width), np.int32)
                     if job.gap_prior is not None else None)
-            tq_b, tq_o = torch.from_numpy(qb).to(dev), torch.from_numpy(qo.view(np.int64)).to(dev)
-            tr_b, tr_o = torch.from_numpy(rb).to(dev), torch.from_numpy(ro.view(np.int64)).to(dev)
-            out = torch.empty((len(job.queries), len(job.refs)), dtype=torch.int32, device=dev)
+            tq_b, tq_o = DeviceArray.from_numpy(qb), DeviceArray.from_numpy(qo)
+            tr_b, tr_o = DeviceArray.from_numpy(rb), DeviceArray.from_numpy(ro)
+            out = DeviceArray((len(job.queries), len(job.refs)), np.int32)
             self._keep += [pen, cube, tq_b, tq_o, tr_b, tr_o]
             self.outputs.append(out)
             self.shapes.append((len(job.queries), len(job.refs)))
             d = descs[t]
-            d.q_codes, d.q_offsets = tq_b.data_ptr(), tq_o.data_ptr()
+            d.q_codes, d.q_offsets = tq_b.ptr, tq_o.ptr
             d.n_queries = len(job.queries)
             d.q_max_len = max([len(s) for s in job.queries] + [0])
-            d.r_codes, d.r_offsets = tr_b.data_ptr(), tr_o.data_ptr()
+            d.r_codes, d.r_offsets = tr_b.ptr, tr_o.ptr
             d.n_refs = len(job.refs)
             d.r_max_len = max([len(s) for s in job.refs] + [0])
             d.pen = pen.ctypes.data
@@ -141,11 +139,11 @@ class GapblockBatch:
             d.gap_open, d.gap_extend = go, job.gap_extend
             d.prior = cube.ctypes.data if cube is not None else None
             d.prior_width = width
-            d.out = out.data_ptr()
+            d.out = out.ptr
         st = _Status()
         plan = ctypes.c_void_p()
         rc = self._lib.st_gapblock_batch_plan_create(descs, len(jobs), int(autotune),
-                                                     ctypes.c_void_p(self._stream.cuda_stream),
+                                                     ctypes.c_void_p(self._stream),
                                                      ctypes.byref(plan), ctypes.byref(st))
         if rc != 0:
             raise RuntimeError(f"st_gapblock_batch_plan_create: {st.message.decode()}")
@@ -158,21 +156,21 @@ class GapblockBatch:
         return {"queries_per_unit": q.value, "threads": th.value, "workgroups": wg.value,
                 "work_units": units.value}
 
-    def run(self, stream=None) -> None:
+    def run(self, stream: int | None = None) -> None:
         st = _Status()
-        s = stream or self._stream
-        rc = self._lib.st_gapblock_batch_plan_run(self._plan, ctypes.c_void_p(s.cuda_stream),
+        s = self._stream if stream is None else stream
+        rc = self._lib.st_gapblock_batch_plan_run(self._plan, ctypes.c_void_p(s),
                                                   ctypes.byref(st))
         if rc != 0:
             raise RuntimeError(f"st_gapblock_batch_plan_run: {st.message.decode()}")
 
     def results(self) -> list[np.ndarray]:
-        self._torch.cuda.synchronize()
-        return [o.cpu().numpy() for o in self.outputs]
+        synchronize()
+        return [o.numpy() for o in self.outputs]
 
     def close(self) -> None:
         if getattr(self, "_plan", None):
-            self._torch.cuda.synchronize()
+            synchronize()
             self._lib.st_gapblock_batch_plan_destroy(self._plan)
             self._plan = None
 

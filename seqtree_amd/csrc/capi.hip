@@ -728,6 +728,34 @@ extern "C" {
 
 const char* st_version(void) { return kVersion; }
 
+int32_t st_device_alloc(uint64_t bytes, void** ptr, st_status* status) {
+    *ptr = nullptr;
+    return guarded(status, [&] {
+        require_device();
+        check(hipMalloc(ptr, std::max<uint64_t>(bytes, 1)), "hipMalloc");
+        return ok(status);
+    });
+}
+
+int32_t st_device_free(void* ptr) { return ptr && hipFree(ptr) != hipSuccess ? ST_DEVICE_ERROR : ST_OK; }
+
+int32_t st_memcpy(void* dst, const void* src, uint64_t bytes, int32_t kind, st_status* status) {
+    return guarded(status, [&] {
+        const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice
+                              : kind == 2 ? hipMemcpyDeviceToHost
+                                          : hipMemcpyDeviceToDevice;
+        if (bytes) check(hipMemcpy(dst, src, bytes, k), "hipMemcpy");
+        return ok(status);
+    });
+}
+
+int32_t st_synchronize(st_status* status) {
+    return guarded(status, [&] {
+        check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        return ok(status);
+    });
+}
+
 void st_timing_enable(int32_t on) {
     std::lock_guard<std::mutex> lk(g_timing_mu);
     g_timing = on != 0;

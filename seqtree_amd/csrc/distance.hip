@@ -77,6 +77,46 @@ __global__ __launch_bounds__(256) void hamming_kernel(DistanceArgs a,
     }
 }
 
+// Same computation, four consecutive references per thread: each query row is written with
+// one 16-byte nontemporal store per thread (1 KiB per wave instruction), which the HBM write
+// stream prefers to 4-byte stores. Requires n_b % 4 == 0 so every row start is 16-byte aligned.
+template <int NW>
+__global__ __launch_bounds__(256) void hamming4_kernel(DistanceArgs a,
+                                                       const uint32_t* __restrict__ aw,
+                                                       uint32_t nwords, uint32_t nbx) {
+    const uint32_t bx = blockIdx.x % nbx;
+    const uint64_t i0 = uint64_t(blockIdx.x / nbx) * kHamQueries;
+    const uint64_t k0 = (uint64_t(bx) * 256 + threadIdx.x) * 4;
+    if (k0 >= a.n_b) return;
+    const uint32_t npad = a.refs.npad;
+    uint32_t rw[4][NW];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int w = 0; w < NW; ++w)
+            rw[c][w] = uint32_t(w) < nwords ? a.refs.words[uint64_t(w) * npad + k0 + c] : 0u;
+    const uint64_t iend = i0 + kHamQueries < a.n_a ? i0 + kHamQueries : a.n_a;
+    for (uint64_t i = i0; i < iend; ++i) {
+        const uint32_t* qa = aw + i * nwords;
+        uint32_t acc[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            if (uint32_t(w) < nwords) {
+                const uint32_t qw = qa[w];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const uint32_t x = qw ^ rw[c][w];
+                    const uint32_t t = ((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x;
+                    acc[c] += __popc(t & 0x80808080u);
+                }
+            }
+        }
+        typedef int v4i __attribute__((ext_vector_type(4)));
+        const v4i v = {int(acc[0]), int(acc[1]), int(acc[2]), int(acc[3])};
+        __builtin_nontemporal_store(v, reinterpret_cast<v4i*>(a.out + i * a.n_b + k0));
+    }
+}
+
 __global__ void hamming_check_kernel(const uint64_t* __restrict__ a_off, uint64_t n_a,
                                      const uint64_t* __restrict__ b_off, uint64_t n_b,
                                      unsigned long long* __restrict__ res) {
@@ -244,7 +284,8 @@ void launch_pack_rows(const uint8_t* bytes, const uint64_t* off, uint64_t n, uin
 void launch_hamming(const DistanceArgs& a, const uint32_t* a_words, uint32_t nwords,
                     hipStream_t s) {
     if (a.n_a == 0 || a.n_b == 0) return;
-    const uint32_t nbx = uint32_t((a.n_b + 255) / 256);
+    const bool vec4 = a.n_b % 4 == 0 && nwords <= 4;
+    const uint32_t nbx = uint32_t((a.n_b + (vec4 ? 1023 : 255)) / (vec4 ? 1024 : 256));
     const uint64_t nby = (a.n_a + kHamQueries - 1) / kHamQueries;
     // one launch covers up to 2^30 workgroups; split the query blocks beyond that
     const uint64_t max_by = (uint64_t(1) << 30) / nbx;
@@ -255,7 +296,9 @@ void launch_hamming(const DistanceArgs& a, const uint32_t* a_words, uint32_t nwo
         sub.n_a = a.n_a - by0 * kHamQueries;
         const uint32_t* aw = a_words + by0 * kHamQueries * nwords;
         const dim3 grid(uint32_t(cnt * nbx));
-        if (nwords <= 4)
+        if (vec4)
+            hipLaunchKernelGGL(hamming4_kernel<4>, grid, dim3(256), 0, s, sub, aw, nwords, nbx);
+        else if (nwords <= 4)
             hipLaunchKernelGGL(hamming_kernel<4>, grid, dim3(256), 0, s, sub, aw, nwords, nbx);
         else if (nwords <= 16)
             hipLaunchKernelGGL(hamming_kernel<16>, grid, dim3(256), 0, s, sub, aw, nwords, nbx);

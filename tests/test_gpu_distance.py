@@ -37,6 +37,17 @@ def test_hamming_lengths(gpu, oracle_lib, L):
     _check(st, oracle_lib, "hamming", a, b)
 
 
+@pytest.mark.parametrize("nb", [1, 3, 4, 1301, 1302, 1303, 1304])
+def test_hamming_ref_counts(gpu, oracle_lib, nb):
+    """Reference counts on and off the four-column store width, short and long rows."""
+    st = gpu
+    rng = random.Random(nb)
+    for L in (12, 40):
+        a = [rand_seq(rng, L, L, "ACGT") for _ in range(37)]
+        b = [rand_seq(rng, L, L, "ACGT") for _ in range(nb)]
+        _check(st, oracle_lib, "hamming", a, b)
+
+
 def test_hamming_raw_bytes_and_case(gpu, oracle_lib):
     st = gpu
     rng = random.Random(4)
