@@ -53,15 +53,18 @@ LEN_LO, LEN_HI = 8, 25
 GAP_OPEN, GAP_EXTEND = 11, 1
 SEED = 2
 
-# VALU lane-instructions per DP cell of the register kernel's cell update (see
+# Integer lane-operations per DP cell of the register kernel's cell update (see
 # seqtree_amd/csrc/gotoh.hip, "H form"): M = Hg_diag + s', X = max(Hg_up, X_up - ge),
-# Y = max(Hg_left, Y_left - ge), Hg = max(M, X, Y) - go -> 8 instructions (7 with v_max3 in
-# int32), local adds the 0 floor and the running best (+2). A packed int16 instruction
-# advances two cells, so the i16x2 kernels spend half as many per cell.
-K_MIN = {("i16x2", "global"): 4.0, ("i16x2", "local"): 5.0,
-         ("i32", "global"): 7.0, ("i32", "local"): 9.0}
-# VALU lane-instruction issue peak of MI355X: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz.
-VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+# Y = max(Hg_left, Y_left - ge), Hg = max(M, X, Y) - go -> 8 operations; local adds the 0
+# floor and the running best (+2). A packed int16 instruction performs two lane-operations
+# (one per query half), so the count is the same for both cell widths.
+K_MIN = {"global": 8.0, "local": 10.0}
+# VALU lane-operation peak of MI355X: the published FP32 vector rate, 157.3 TFLOP/s, is
+# 78.6e12 FMA lane-operations/s = 256 CUs x 128 lanes x 2.4 GHz. The on-device probe
+# (st_probe_valu, reported in every VALU line) reaches ~87% of it with two-operand VGPR
+# instructions (v_add_u32, v_max_i16); three-operand, packed and SGPR forms issue at half that
+# rate, so a kernel built from them tops out near 50% of this peak.
+VALU_PEAK_TOPS = 157.3 / 2
 
 
 def cdr3_set(rng: np.random.Generator, n: int, lo: int, hi: int) -> list[np.ndarray]:
@@ -74,6 +77,23 @@ def cdr3_set(rng: np.random.Generator, n: int, lo: int, hi: int) -> list[np.ndar
         body = codes_bg[rng.choice(20, size=L - 2, p=BG_FREQ)]
         out.append(np.concatenate([[c_code], body, [f_code]]).astype(np.uint8))
     return out
+
+
+def cdr3_csr(rng: np.random.Generator, n: int, lo: int, hi: int):
+    """cdr3_set's strings generated in bulk, as CSR (codes, offsets, lengths)."""
+    lens = rng.integers(lo, hi + 1, size=n).astype(np.int64)
+    off = np.zeros(n + 1, np.uint64)
+    off[1:] = np.cumsum(lens)
+    codes_bg = np.array([AA_ORDER.index(c) for c in BG_RESIDUES], np.uint8)
+    codes = np.empty(int(off[-1]), np.uint8)
+    body = np.ones(len(codes), bool)
+    starts = off[:-1].astype(np.int64)
+    body[starts] = False
+    body[starts + lens - 1] = False
+    codes[body] = codes_bg[rng.choice(20, size=int(body.sum()), p=BG_FREQ)]
+    codes[starts] = AA_ORDER.index("C")
+    codes[starts + lens - 1] = AA_ORDER.index("F")
+    return codes, off, lens
 
 
 def dna_set(rng: np.random.Generator, n: int, lo: int, hi: int) -> list[np.ndarray]:
@@ -127,6 +147,9 @@ class Lib:
             fn = getattr(L, name)
             fn.restype = i32
             fn.argtypes = [vp, vp, u64, u32, vp, vp, u64, u32, vp, vp, vp]
+        L.st_probe_valu.restype = i32
+        L.st_probe_valu.argtypes = [i32, ctypes.c_char_p, u64, ctypes.POINTER(ctypes.c_double),
+                                    vp]
         L.st_timing_enable.argtypes = [i32]
         L.st_timing_collect.restype = i32
         L.st_timing_collect.argtypes = [ctypes.c_char_p, u64]
@@ -196,43 +219,43 @@ def dominant(kernels):
     return label, launches, ms / launches / 1e3
 
 
-def cpu_reference(kind: str, q, r, rows: int, **kw) -> dict:
+def cpu_reference(kind: str, q, r, rows: int, seconds: float = 10.0, **kw) -> dict:
     """The reference's own C++ (oracle/_ref, compiled -O3 from its sources) on the first `rows`
-    queries against every reference, on every host core; the plain-C port where the compiled
-    reference is absent."""
+    queries against every reference, on every host core, repeated until `seconds` of wall time
+    have passed; the plain-C port where the compiled reference is absent."""
     import oracle
 
     threads = oracle.host_threads()
     qs = q[:rows]
     have_ref = oracle.reference_available()
     impl = oracle.Reference() if have_ref else oracle.Oracle()
-    t0 = time.perf_counter()
-    if kind == "gotoh":
-        for local in (False, True):
-            if have_ref:
-                impl.align_matrix(qs, r, "blosum62", local, False, GAP_OPEN, GAP_EXTEND,
-                                  threads=threads)
-            else:
-                impl.align_matrix(qs, r, kw["sim"], local, False, GAP_OPEN, GAP_EXTEND,
-                                  threads=threads)
-        pairs = 2 * len(qs) * len(r)
-    elif kind == "gapblock":
-        if have_ref:
-            impl.gapblock_matrix(qs, r, "blosum62", kw["go"], 1, kw["cube"], kw["width"],
-                                 threads=threads)
-        else:
-            impl.gapblock_matrix(qs, r, kw["pen"], kw["go"], 1, kw["cube"], kw["width"],
-                                 threads=threads)
-        pairs = len(qs) * len(r)
-    else:
+
+    def once() -> int:
+        if kind == "gotoh":
+            for local in (False, True):
+                impl.align_matrix(qs, r, "blosum62" if have_ref else kw["sim"], local, False,
+                                  GAP_OPEN, GAP_EXTEND, threads=threads)
+            return 2 * len(qs) * len(r)
+        if kind == "gapblock":
+            impl.gapblock_matrix(qs, r, "blosum62" if have_ref else kw["pen"], kw["go"], 1,
+                                 kw["cube"], kw["width"], threads=threads)
+            return len(qs) * len(r)
         impl.distance_matrix(kind, qs, r, threads=threads)
-        pairs = len(qs) * len(r)
-    dt = time.perf_counter() - t0
+        return len(qs) * len(r)
+
+    pairs, calls = 0, 0
+    t0 = time.perf_counter()
+    while True:
+        pairs += once()
+        calls += 1
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            break
     return {"value": pairs / dt, "unit": "pairs/s", "cores": threads,
             "kind": "reference" if have_ref else "port",
             "sample": f"first {len(qs)} queries x all {len(r)} references of the same workload"
-                      f"{' (global + local)' if kind == 'gotoh' else ''}: {pairs} pairs in "
-                      f"{dt:.2f} s wall"}
+                      f"{' (global + local)' if kind == 'gotoh' else ''}, {calls} calls: "
+                      f"{pairs} pairs in {dt:.2f} s wall"}
 
 
 def base_line(metric, value, steps, warmup, ms_step, config, kernels, roofline, cpu, **extra):
@@ -244,6 +267,19 @@ def base_line(metric, value, steps, warmup, ms_step, config, kernels, roofline, 
                  "kernels_ms": {k: round(v[1] / steps, 4) for k, v in sorted(kernels.items())},
                  "roofline": roofline, "cpu_baseline": cpu})
     return line
+
+
+def valu_probe(lib) -> dict:
+    """Measured issue rate (T lane-instructions/s) of each probed VALU instruction kind."""
+    out = {}
+    for kind in range(16):
+        name = ctypes.create_string_buffer(32)
+        rate = ctypes.c_double()
+        st = Status()
+        if lib.lib.st_probe_valu(kind, name, 32, ctypes.byref(rate), ctypes.byref(st)) != 0:
+            break
+        out[name.value.decode()] = round(rate.value / 1e12, 2)
+    return out
 
 
 def valu_roofline(label, launches, avg_s, ops_per_launch, **info):
@@ -287,13 +323,13 @@ def run_gotoh(torch, dev, lib, stream, args) -> dict:
         rows = int(rtag[1:])
         in_class = (Q.lens <= rows) & (Q.lens > rows - 4)
         cells = float(Q.lens[in_class].sum()) * float(R.lens.sum())
-        roofline = valu_roofline(label, launches, avg_s, cells * K_MIN[(cell, mode)],
-                                 cells_per_launch=cells, ops_per_cell=K_MIN[(cell, mode)])
+        roofline = valu_roofline(label, launches, avg_s, cells * K_MIN[mode],
+                                 cells_per_launch=cells, ops_per_cell=K_MIN[mode])
     cpu = None
     if args.cpu_rows > 0:
         to_str = lambda v: "".join(AA_ORDER[c] for c in v)  # noqa: E731
         cpu = cpu_reference("gotoh", [to_str(s) for s in qs], [to_str(s) for s in rs],
-                            args.cpu_rows, sim=sim)
+                            args.cpu_rows, seconds=args.cpu_seconds, sim=sim)
     return base_line(
         "all-vs-all pairs/s (Gotoh NW+SW score_matrix)", pairs_per_step / (ms_step / 1e3),
         args.steps, args.warmup, ms_step,
@@ -328,7 +364,7 @@ def run_gapblock(torch, dev, lib, stream, args) -> dict:
 
     ms_step, kernels = time_steps(torch, lib, stream, step, args.steps, args.warmup)
     pairs = Q.n * R.n
-    # algorithmic lookups (SURVEY units): L per pair at d == 0, 2L at d > 0; the one-pass
+    # algorithmic lookups: L per pair at d == 0, 2L at d > 0; the one-pass
     # minimum spends 5 integer ops per block position when d > 0 (two prefix sums, their
     # difference, the prior, the min) and 1 at d == 0
     L = np.minimum(Q.lens[:, None], R.lens[None, :])
@@ -342,14 +378,96 @@ def run_gapblock(torch, dev, lib, stream, args) -> dict:
     if args.cpu_rows > 0:
         to_str = lambda v: "".join(AA_ORDER[c] for c in v)  # noqa: E731
         cpu = cpu_reference("gapblock", [to_str(s) for s in qs], [to_str(s) for s in rs],
-                            min(args.cpu_rows, Q.n), pen=pen, go=go, cube=cube.tolist(),
-                            width=width)
+                            min(args.cpu_rows, Q.n), seconds=args.cpu_seconds, pen=pen, go=go,
+                            cube=cube.tolist(), width=width)
     return base_line(
         "all-vs-all pairs/s (gap-block score_matrix)", pairs / (ms_step / 1e3), args.steps,
         args.warmup, ms_step,
         {"workload": "gapblock_score_matrix_cdr3_1000x10000_blosum62_go28_ge1_central21",
          "queries": Q.n, "refs": R.n, "lengths": [10, 20], "seed": SEED + 1,
          "pairs_per_step": pairs, "lookups_per_step": lookups, "parallelism": "single GPU"},
+        kernels, roofline, cpu, glookups=lookups / (ms_step / 1e3) / 1e9)
+
+
+# Ragged batch: 64 jobs, log-normal set sizes (median 2,000 queries and 50,000 references,
+# sigma 0.8) clipped to [512, 8192] and [10^4, 2*10^5], lengths 6-30, matrices alternating
+# BLOSUM62 (gap_open 2*14, central prior 21) and structural (2*7, 10).
+BATCH_JOBS = 64
+BATCH_N = (np.log(2000.0), 0.8, 512, 8192)
+BATCH_K = (np.log(50000.0), 0.8, 10_000, 200_000)
+BATCH_LEN = (6, 30)
+
+
+def batch_jobs(seed: int):
+    import seqtree_amd as st
+    from seqtree_amd.batch import EncodedSet, GapblockJob
+
+    rng = np.random.default_rng(seed)
+    mats = [st.SubstitutionMatrix.blosum62(), st.SubstitutionMatrix.structural()]
+    jobs, hists = [], []
+    for t in range(BATCH_JOBS):
+        n = int(np.clip(round(rng.lognormal(BATCH_N[0], BATCH_N[1])), BATCH_N[2], BATCH_N[3]))
+        k = int(np.clip(round(rng.lognormal(BATCH_K[0], BATCH_K[1])), BATCH_K[2], BATCH_K[3]))
+        qc, qo, ql = cdr3_csr(rng, n, *BATCH_LEN)
+        rc, ro, rl = cdr3_csr(rng, k, *BATCH_LEN)
+        m = mats[t % 2]
+        jobs.append(GapblockJob(EncodedSet(qc, qo), EncodedSet(rc, ro), m, 2 * m.scale(), 1,
+                                st.central_prior(int(1.5 * m.scale()))))
+        hists.append((np.bincount(ql, minlength=BATCH_LEN[1] + 1).astype(np.float64),
+                      np.bincount(rl, minlength=BATCH_LEN[1] + 1).astype(np.float64)))
+    return jobs, hists
+
+
+def run_batch(torch, dev, lib, stream, args) -> dict:
+    from seqtree_amd.batch import GapblockBatch
+
+    seed = SEED + 3
+    jobs, hists = batch_jobs(seed)
+    pairs = float(sum(len(j.queries) * len(j.refs) for j in jobs))
+    # algorithmic work from the length histograms: per pair L lookups at d == 0, 2L at d > 0,
+    # and 5 integer operations per block position at d > 0 (1 at d == 0), as for one job
+    ls = np.arange(BATCH_LEN[1] + 1, dtype=np.float64)
+    Lmin = np.minimum(ls[:, None], ls[None, :])
+    dpos = ls[:, None] != ls[None, :]
+    look_w = np.where(dpos, 2 * Lmin, Lmin)
+    ops_w = np.where(dpos, 5 * Lmin, Lmin)
+    lookups = float(sum(hq @ look_w @ hr for hq, hr in hists))
+    ops = float(sum(hq @ ops_w @ hr for hq, hr in hists))
+    t0 = time.perf_counter()
+    batch = GapblockBatch(jobs, autotune=True, stream=stream.cuda_stream)
+    plan_s = time.perf_counter() - t0
+    tiles = batch.tiles()
+
+    def step():
+        batch.run()
+
+    ms_step, kernels = time_steps(torch, lib, stream, step, args.steps, args.warmup)
+    label, launches, avg_s = dominant(kernels)
+    roofline = valu_roofline(label, launches, avg_s, ops, lookups_per_launch=lookups,
+                             ops_per_position=5)
+    cpu = None
+    if args.cpu_rows > 0:
+        # the reference has no batch call: its gap-block score_matrix on rows of job 0
+        _, pen, _ = matrix_tables("blosum62")
+        j0 = jobs[0]
+        width = int(max(j0.queries.lengths().max(), j0.refs.lengths().max()))
+        from seqtree_amd.gapblock import _prior_cube
+        to_strs = lambda e: ["".join(AA_ORDER[c] for c in e.codes[int(e.offsets[i]):int(e.offsets[i + 1])])  # noqa: E731,E501
+                             for i in range(len(e))]
+        cpu = cpu_reference("gapblock", to_strs(j0.queries), to_strs(j0.refs),
+                            min(args.cpu_rows, 200), seconds=args.cpu_seconds, pen=pen,
+                            go=j0.gap_open,
+                            cube=_prior_cube(j0.gap_prior, width), width=width)
+    batch.close()
+    return base_line(
+        "all-vs-all pairs/s (ragged gap-block batch)", pairs / (ms_step / 1e3), args.steps,
+        args.warmup, ms_step,
+        {"workload": "gapblock_batch_64jobs_lognormal_cdr3_6-30_blosum62+structural",
+         "jobs": BATCH_JOBS, "seed": seed, "pairs_per_step": pairs,
+         "queries_total": int(sum(len(j.queries) for j in jobs)),
+         "refs_total": int(sum(len(j.refs) for j in jobs)),
+         "lookups_per_step": lookups, "tiles": tiles, "plan_and_autotune_s": round(plan_s, 2),
+         "parallelism": "single GPU"},
         kernels, roofline, cpu, glookups=lookups / (ms_step / 1e3) / 1e9)
 
 
@@ -401,7 +519,7 @@ def run_distance(torch, dev, lib, stream, args, which: str) -> dict:
     if args.cpu_rows > 0:
         rows = 500 if which == "hamming" else 200
         cpu = cpu_reference(which, [bytes(x) for x in a], [bytes(x) for x in b],
-                            min(rows, args.cpu_rows))
+                            min(rows, args.cpu_rows), seconds=args.cpu_seconds)
     return base_line(
         f"all-vs-all pairs/s ({which}_matrix)", pairs / (ms_step / 1e3), args.steps,
         args.warmup, ms_step,
@@ -432,8 +550,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-rows", type=int, default=1000,
                     help="query rows of the CPU baseline sample (0 skips it)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="wall time the CPU baseline sample is repeated for")
     ap.add_argument("--workload", default="gotoh",
-                    choices=["gotoh", "gapblock", "hamming", "levenshtein"])
+                    choices=["gotoh", "gapblock", "hamming", "levenshtein", "batch"])
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scale for rocprofv3 --pmc "
                          "FETCH_SIZE / WRITE_SIZE readings)")
@@ -442,7 +562,7 @@ def main() -> int:
         return copy_calibration()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the all-vs-all matrix shards by query rows with no collective, see DESIGN.md)",
+              "(the all-vs-all matrix shards by query rows with no collective between GPUs)",
               file=sys.stderr)
         return 2
 
@@ -458,8 +578,12 @@ def main() -> int:
         line = run_gotoh(torch, dev, lib, stream, args)
     elif args.workload == "gapblock":
         line = run_gapblock(torch, dev, lib, stream, args)
+    elif args.workload == "batch":
+        line = run_batch(torch, dev, lib, stream, args)
     else:
         line = run_distance(torch, dev, lib, stream, args, args.workload)
+    if line.get("roofline") and line["roofline"]["bound"] == "valu":
+        line["roofline"]["probe_tops"] = valu_probe(lib)
     print(json.dumps(line))
     return 0
 

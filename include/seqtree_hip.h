@@ -227,6 +227,14 @@ int32_t st_synchronize(st_status* status);
 void st_timing_enable(int32_t on);
 int32_t st_timing_collect(char* report, uint64_t capacity);
 
+/* Measured VALU issue rate of one instruction kind (0 v_add_u32, 1 v_max_i16, 2 v_max_i32,
+ * 3 v_pk_max_i16, 4 v_max3_i32, 5 v_bfi_b32) over the whole device, in lane-instructions per
+ * second: the measured peaks the VALU-bound kernels' rooflines are checked against. Writes the
+ * instruction's name into name (NUL-terminated, truncated to name_capacity); an unknown kind
+ * is ST_INVALID_ARGUMENT. */
+int32_t st_probe_valu(int32_t kind, char* name, uint64_t name_capacity, double* lane_ops_per_s,
+                      st_status* status);
+
 #ifdef __cplusplus
 }
 #endif

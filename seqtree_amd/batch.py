@@ -61,11 +61,27 @@ def _lib():
 
 
 @dataclass
-class GapblockJob:
-    """One gap-block score matrix: the arguments of :func:`gapblock.score_matrix`."""
+class EncodedSet:
+    """Strings already in codec codes, as CSR: ``codes`` (uint8) and ``offsets`` (uint64,
+    n + 1 entries). Lets large batches skip the per-string Python encoding."""
 
-    queries: Sequence[str]
-    refs: Sequence[str]
+    codes: np.ndarray
+    offsets: np.ndarray
+
+    def __len__(self) -> int:
+        return len(self.offsets) - 1
+
+    def lengths(self) -> np.ndarray:
+        return np.diff(self.offsets.astype(np.int64))
+
+
+@dataclass
+class GapblockJob:
+    """One gap-block score matrix: the arguments of :func:`gapblock.score_matrix`.
+    ``queries`` and ``refs`` are strings or an :class:`EncodedSet`."""
+
+    queries: Sequence[str] | EncodedSet
+    refs: Sequence[str] | EncodedSet
     matrix: _core.SubstitutionMatrix | None = None
     gap_open: int | None = None
     gap_extend: int = 1
@@ -73,7 +89,15 @@ class GapblockJob:
     alphabet: str = "aa"
 
 
-def _encode(seqs: Sequence[str], symbols: str) -> tuple[np.ndarray, np.ndarray]:
+def _encode(seqs, symbols: str) -> tuple[np.ndarray, np.ndarray]:
+    if isinstance(seqs, EncodedSet):
+        codes = np.ascontiguousarray(seqs.codes, np.uint8)
+        off = np.ascontiguousarray(seqs.offsets, np.uint64)
+        if off[0] != 0 or (np.diff(off.astype(np.int64)) < 0).any() or int(off[-1]) > len(codes):
+            raise ValueError("malformed EncodedSet offsets")
+        if int(off[-1]) and int(codes[: int(off[-1])].max()) >= len(symbols):
+            raise ValueError("EncodedSet code outside the alphabet")
+        return (codes if len(codes) else np.zeros(1, np.uint8)), off
     lut = np.full(256, 255, np.uint8)
     for code, ch in enumerate(symbols):
         lut[ord(ch)] = code
@@ -118,7 +142,9 @@ class GapblockBatch:
                 (1 - np.eye(A, dtype=np.int32))
             qb, qo = _encode(job.queries, symbols)
             rb, ro = _encode(job.refs, symbols)
-            width = max([len(s) for s in list(job.queries) + list(job.refs)] + [0])
+            q_lens = np.diff(qo.astype(np.int64))
+            r_lens = np.diff(ro.astype(np.int64))
+            width = int(max(q_lens.max(initial=0), r_lens.max(initial=0)))
             cube = (np.array(_prior_cube(job.gap_prior, width), np.int32)
                     if job.gap_prior is not None else None)
             tq_b, tq_o = DeviceArray.from_numpy(qb), DeviceArray.from_numpy(qo)
@@ -130,10 +156,10 @@ class GapblockBatch:
             d = descs[t]
             d.q_codes, d.q_offsets = tq_b.ptr, tq_o.ptr
             d.n_queries = len(job.queries)
-            d.q_max_len = max([len(s) for s in job.queries] + [0])
+            d.q_max_len = int(q_lens.max(initial=0))
             d.r_codes, d.r_offsets = tr_b.ptr, tr_o.ptr
             d.n_refs = len(job.refs)
-            d.r_max_len = max([len(s) for s in job.refs] + [0])
+            d.r_max_len = int(r_lens.max(initial=0))
             d.pen = pen.ctypes.data
             d.alphabet_size = A
             d.gap_open, d.gap_extend = go, job.gap_extend

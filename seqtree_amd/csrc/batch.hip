@@ -22,35 +22,34 @@ namespace {
 using gb::Look;
 using gb::PackedWords;
 using gb::pair_score;
-using gb::pair_score_uniform;
+using gb::pair_score_fast;
 
-template <bool PROFILE, bool EXACT, uint32_t THREADS>
+template <bool PROFILE, bool EXACT, bool SHORT, uint32_t THREADS>
 __device__ void run_unit(const BatchJob& J, uint64_t q_begin, uint32_t qt, int32_t* lds,
                          int32_t* stage) {
     constexpr uint32_t kSlots = kChunk / THREADS;
     const uint32_t tid = threadIdx.x;
     const uint32_t A = J.A;
     const uint64_t q_end = q_begin + qt < J.n_queries ? q_begin + qt : J.n_queries;
+    int32_t* slice = lds + (SHORT ? J.prof_rows : 0u) * gb::kRowInts;
     if constexpr (!PROFILE) {
-        for (uint32_t e = tid; e < A * A; e += THREADS) lds[e] = J.pen[e];
+        gb::build_rows(lds, A, A, J.pen, A, [](uint32_t r) { return r; }, tid, THREADS);
         __syncthreads();
     }
     for (uint64_t i = q_begin; i < q_end; ++i) {
         const uint64_t q0 = J.q_off[i];
         const uint32_t m = uint32_t(J.q_off[i + 1] - q0);
         const uint8_t* q = J.q_codes + q0;
-        int32_t* nprof = lds + m * A;  // PROFILE: the negated profile follows the profile
         if constexpr (PROFILE) {
             __syncthreads();  // the previous query's profile is no longer read
-            for (uint32_t e = tid; e < m * A; e += THREADS) {
-                const uint32_t t = e / A;
-                const int32_t v = J.pen[uint32_t(q[t]) * A + (e - t * A)];
-                lds[e] = v;
-                nprof[e] = int32_t(0u - uint32_t(v));
-            }
+            const uint32_t rows = SHORT ? J.prof_rows : m;
+            gb::build_rows(lds, rows, m, J.pen, A, [&](uint32_t t) { return uint32_t(q[t]); },
+                           tid, THREADS);
+            if constexpr (SHORT)
+                gb::build_slice(slice, J.slice_rows - 1, m, J.prior, J.w1, J.big, tid, THREADS);
             __syncthreads();
         }
-        const Look<PROFILE> look{lds, q, A};
+        const Look<PROFILE> look{lds, q};
         for (uint32_t c = 0; c < J.refs.nchunks; ++c) {
 #pragma unroll
             for (uint32_t it = 0; it < kSlots; ++it) {
@@ -58,15 +57,20 @@ __device__ void run_unit(const BatchJob& J, uint64_t q_begin, uint32_t qt, int32
                 const uint32_t k = J.refs.orig[slot];
                 const uint32_t n = J.refs.len[slot];
                 const PackedWords words{J.refs.words + slot, J.refs.npad};
-                const uint32_t n0 = __builtin_amdgcn_readfirstlane(n);
                 int32_t score = 0;
-                if (PROFILE && !EXACT && __all(k == kNoSlot || n == n0) &&
-                    (m < n0 ? m : n0) <= gb::kFastMaxL) {
-                    score = pair_score_uniform(lds, nprof, A, words, m, n0, J.prior, J.w1,
-                                               J.gap_open, J.gap_extend);
-                } else if (k != kNoSlot) {
-                    score = pair_score<PROFILE, EXACT>(look, words, m, n, J.prior, J.w1,
-                                                       J.gap_open, J.gap_extend);
+                if constexpr (SHORT) {
+                    score = gb::pair_score_short(lds, slice, words, k, m, n, J.prior, J.w1,
+                                                 J.gap_open, J.gap_extend);
+                } else {
+                    const uint32_t n0 = __builtin_amdgcn_readfirstlane(n);
+                    if (PROFILE && !EXACT && __all(k == kNoSlot || n == n0) &&
+                        (m < n0 ? m : n0) <= gb::kFastMaxL) {
+                        score = pair_score_fast(lds, words, m, n0, J.prior, J.w1, J.gap_open,
+                                                J.gap_extend);
+                    } else if (k != kNoSlot) {
+                        score = pair_score<PROFILE, EXACT>(look, words, m, n, J.prior, J.w1,
+                                                           J.gap_open, J.gap_extend);
+                    }
                 }
                 if (k != kNoSlot) stage[k - c * kChunk] = score;
             }
@@ -109,12 +113,14 @@ __global__ __launch_bounds__(THREADS) void gapblock_batch_kernel(
         if (u >= total_units) break;
         const BatchJob& J = jobs[j];
         const uint64_t q_begin = (u - J.unit_begin) * qt;
-        if (J.profile) {
-            if (J.exact) run_unit<true, true, THREADS>(J, q_begin, qt, lds, stage);
-            else run_unit<true, false, THREADS>(J, q_begin, qt, lds, stage);
+        if (J.short_path) {
+            run_unit<true, false, true, THREADS>(J, q_begin, qt, lds, stage);
+        } else if (J.profile) {
+            if (J.exact) run_unit<true, true, false, THREADS>(J, q_begin, qt, lds, stage);
+            else run_unit<true, false, false, THREADS>(J, q_begin, qt, lds, stage);
         } else {
-            if (J.exact) run_unit<false, true, THREADS>(J, q_begin, qt, lds, stage);
-            else run_unit<false, false, THREADS>(J, q_begin, qt, lds, stage);
+            if (J.exact) run_unit<false, true, false, THREADS>(J, q_begin, qt, lds, stage);
+            else run_unit<false, false, false, THREADS>(J, q_begin, qt, lds, stage);
         }
         __syncthreads();
     }

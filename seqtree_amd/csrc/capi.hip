@@ -6,6 +6,7 @@
 // calls never share buffers. Integer-width choices are made here on the host from the input
 // bounds (a range proof), never by trying and retrying on the device.
 
+#include "gapblock_core.hpp"
 #include "st_internal.hpp"
 #include "../../include/seqtree_hip.h"
 
@@ -188,9 +189,9 @@ void launched(const char* what) { check(hipGetLastError(), what); }
 
 uint32_t words_for(uint32_t max_len) { return (max_len + 3) / 4; }
 
-// Packs the reference side (see st_internal.hpp for the layout).
+// Packs the reference side (see st_internal.hpp for the layout); scale as launch_pack_refs.
 PackedRefs pack_refs(Arena& ar, const uint8_t* bytes, const uint64_t* off, uint64_t n,
-                     uint32_t max_len, hipStream_t s) {
+                     uint32_t max_len, hipStream_t s, uint32_t scale = 1) {
     LaunchTimer t("pack_refs", s);
     PackedRefs p{};
     p.nchunks = uint32_t((n + kChunk - 1) / kChunk);
@@ -199,7 +200,7 @@ PackedRefs pack_refs(Arena& ar, const uint8_t* bytes, const uint64_t* off, uint6
     uint32_t* words = ar.get<uint32_t>(size_t(p.nwords + 1) * p.npad);
     uint32_t* len = ar.get<uint32_t>(p.npad);
     uint32_t* orig = ar.get<uint32_t>(p.npad);
-    launch_pack_refs(bytes, off, n, p.nwords, p.npad, words, len, orig, s);
+    launch_pack_refs(bytes, off, n, p.nwords, p.npad, words, len, orig, scale, s);
     launched("pack_refs");
     p.words = words;
     p.len = len;
@@ -232,13 +233,32 @@ struct GapblockPlan {
     GapblockArgs a;
     bool exact;
     bool profile;
+    bool short_path;
 };
+
+// Short-launch eligibility (see gapblock_core.hpp) and its past-the-end prior value.
+// T bounds |prefix sum| of one diagonal; with D = P0 - Pd in [-2T, 2T], big = INT32_MAX - 2T
+// keeps D + big from wrapping, and 6T + prior_abs < INT32_MAX puts D + big above every real
+// candidate D + prior <= 2T + prior_abs.
+bool short_launch(uint32_t q_max, uint32_t r_max, uint32_t A, bool exact, int64_t pen_abs,
+                  int64_t prior_abs, int32_t* big) {
+    const int64_t T = int64_t(std::max(q_max, r_max)) * pen_abs;
+    if (exact || A > 31 || q_max > gb::kFastMaxL || r_max > gb::kShortMaxRef) return false;
+    if (6 * T + prior_abs >= int64_t(INT32_MAX)) return false;
+    *big = int32_t(int64_t(INT32_MAX) - 2 * T);
+    return true;
+}
 
 GapblockPlan gapblock_plan(Arena& ar, const uint8_t* q, const uint64_t* q_off, uint64_t N,
                            uint32_t q_max, const uint8_t* r, const uint64_t* r_off, uint64_t K,
                            uint32_t r_max, const int32_t* pen_h, uint32_t A, int32_t go,
                            int32_t ge, const int32_t* prior_h, uint32_t W, bool pack,
                            hipStream_t s) {
+    if (prior_h && W < std::max(q_max, r_max))
+        throw HipError{ST_INVALID_ARGUMENT,
+                       "seqtree_hip: prior_width " + std::to_string(W) +
+                           " is below the longest sequence (" +
+                           std::to_string(std::max(q_max, r_max)) + ")"};
     int32_t* pen = ar.get<int32_t>(size_t(A) * A);
     h2d(pen, pen_h, size_t(A) * A, s);
     const size_t w1 = size_t(W) + 1;
@@ -246,8 +266,9 @@ GapblockPlan gapblock_plan(Arena& ar, const uint8_t* q, const uint64_t* q_off, u
     int64_t prior_abs = 0;
     if (prior_h) {
         const size_t cube = w1 * w1 * w1;
-        prior = ar.get<int32_t>(cube);
+        prior = ar.get<int32_t>(cube + gb::kPriorPad);
         h2d(prior, prior_h, cube, s);
+        launch_fill_u32(reinterpret_cast<uint32_t*>(prior + cube), gb::kPriorPad, 0u, s);
         for (size_t e = 0; e < cube; ++e) prior_abs = std::max(prior_abs, iabs64(prior_h[e]));
     }
     int64_t pen_abs = 0;
@@ -259,7 +280,7 @@ GapblockPlan gapblock_plan(Arena& ar, const uint8_t* q, const uint64_t* q_off, u
     a.q_off = q_off;
     a.n_queries = N;
     a.n_refs = K;
-    if (pack) a.refs = pack_refs(ar, r, r_off, K, r_max, s);
+    if (pack) a.refs = pack_refs(ar, r, r_off, K, r_max, s, 4);
     a.pen = pen;
     a.A = A;
     a.gap_open = go;
@@ -271,15 +292,32 @@ GapblockPlan gapblock_plan(Arena& ar, const uint8_t* q, const uint64_t* q_off, u
     const int64_t Lmax = std::max(q_max, r_max);
     const int64_t worst = 2 * Lmax * pen_abs + prior_abs + int64_t(go) + Lmax * int64_t(ge);
     p.exact = worst >= int64_t(INT32_MAX);
-    p.profile = 2 * size_t(q_max) * A * sizeof(int32_t) <= 48 * 1024;  // profile + negation
+    p.profile = size_t(q_max) * gb::kRowBytes <= 48 * 1024;  // one table row per position
+    p.short_path = pack && short_launch(q_max, r_max, A, p.exact, pen_abs, prior_abs, &a.big);
+    // chunks per workgroup: about four workgroups per CU in all (one resident round at the
+    // occupancy these kernels reach), so the query tables are built as rarely as possible
+    a.group = 1;
+    if (pack && a.refs.nchunks > 1) {
+        int dev = 0, cus = 256;
+        if (hipGetDevice(&dev) == hipSuccess)
+            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        const uint64_t units = N * uint64_t(a.refs.nchunks);
+        const uint64_t target = uint64_t(cus) * 4;
+        a.group = uint32_t(std::min<uint64_t>(a.refs.nchunks, std::max<uint64_t>(1, units / target)));
+    }
+    if (p.short_path) {
+        a.prof_rows = q_max + gb::kFastMaxL;
+        a.slice_rows = r_max + 1;
+    }
     return p;
 }
 
 void gapblock_run(const GapblockPlan& p, uint32_t q_max, hipStream_t s) {
     LaunchTimer t(std::string(p.a.hits.count ? "gapblock_threshold" : "gapblock") +
-                      (p.profile ? "_profile" : "_table") + (p.exact ? "_exact" : ""),
+                      (p.short_path ? "_short" : p.profile ? "_profile" : "_table") +
+                      (p.exact ? "_exact" : ""),
                   s);
-    launch_gapblock(p.a, p.profile, q_max, p.exact, s);
+    launch_gapblock(p.a, p.profile, q_max, p.exact, p.short_path, s);
 }
 
 void gapblock_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, uint32_t q_max,
@@ -349,6 +387,7 @@ struct BatchPlanImpl {
         uint64_t n;
         uint32_t nwords, npad;
         uint32_t *words, *len, *orig;
+        uint32_t scale;
     };
     std::vector<PackArgs> packs;
     BatchJob* d_jobs = nullptr;
@@ -386,7 +425,8 @@ struct BatchPlanImpl {
     // (a fill kernel: no memset nodes in a captured graph), run the persistent kernel.
     void enqueue(hipStream_t s) {
         for (const auto& p : packs)
-            launch_pack_refs(p.bytes, p.off, p.n, p.nwords, p.npad, p.words, p.len, p.orig, s);
+            launch_pack_refs(p.bytes, p.off, p.n, p.nwords, p.npad, p.words, p.len, p.orig,
+                             p.scale, s);
         launch_fill_u32(reinterpret_cast<uint32_t*>(d_next), 2, 0u, s);
         launch_gapblock_batch(d_jobs, uint32_t(dev_jobs.size()), total_units, tiles, lds_bytes,
                               d_next, s);
@@ -756,6 +796,52 @@ int32_t st_synchronize(st_status* status) {
     });
 }
 
+int32_t st_probe_valu(int32_t kind, char* name, uint64_t name_capacity, double* lane_ops_per_s,
+                      st_status* status) {
+    return guarded(status, [&] {
+        const char* nm = probe_name(kind);
+        if (!nm) {
+            set_status(status, ST_INVALID_ARGUMENT, "seqtree_hip: no VALU probe of kind %d", kind);
+            return int(ST_INVALID_ARGUMENT);
+        }
+        if (name && name_capacity) std::snprintf(name, name_capacity, "%s", nm);
+        require_device();
+        int dev = 0, cus = 0;
+        check(hipGetDevice(&dev), "hipGetDevice");
+        check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+              "hipDeviceGetAttribute");
+        const uint32_t blocks = uint32_t(cus) * 32, iters = 16384;
+        uint32_t* out = nullptr;
+        check(hipMalloc(&out, size_t(blocks) * 256 * sizeof(uint32_t)), "hipMalloc");
+        hipEvent_t a = nullptr, b = nullptr;
+        float best = 0.f;
+        try {
+            check(hipEventCreate(&a), "hipEventCreate");
+            check(hipEventCreate(&b), "hipEventCreate");
+            for (int rep = 0; rep < 4; ++rep) {  // the first run is a warm-up
+                check(hipEventRecord(a, nullptr), "hipEventRecord");
+                launch_probe(kind, out, blocks, iters, nullptr);
+                check(hipGetLastError(), "probe launch");
+                check(hipEventRecord(b, nullptr), "hipEventRecord");
+                check(hipEventSynchronize(b), "hipEventSynchronize");
+                float ms = 0.f;
+                check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+                if (rep == 1 || (rep > 1 && ms < best)) best = ms;
+            }
+        } catch (...) {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+            (void)hipFree(out);
+            throw;
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        check(hipFree(out), "hipFree");
+        *lane_ops_per_s = double(blocks) * 256.0 * 16.0 * iters / (double(best) * 1e-3);
+        return ok(status);
+    });
+}
+
 void st_timing_enable(int32_t on) {
     std::lock_guard<std::mutex> lk(g_timing_mu);
     g_timing = on != 0;
@@ -996,12 +1082,19 @@ int32_t st_gapblock_batch_plan_create(const st_gapblock_job* jobs, uint32_t n_jo
             const uint32_t A = j.alphabet_size;
             int32_t* pen = B.alloc<int32_t>(size_t(A) * A);
             h2d(pen, j.pen, size_t(A) * A, s);
+            if (j.prior && j.prior_width < std::max(j.q_max_len, j.r_max_len))
+                throw HipError{ST_INVALID_ARGUMENT,
+                               "seqtree_hip: batch job " + std::to_string(t) + ": prior_width " +
+                                   std::to_string(j.prior_width) +
+                                   " is below the longest sequence"};
             const size_t w1 = size_t(j.prior_width) + 1;
             int32_t* prior = nullptr;
             int64_t prior_abs = 0, pen_abs = 0;
             if (j.prior) {
-                prior = B.alloc<int32_t>(w1 * w1 * w1);
+                prior = B.alloc<int32_t>(w1 * w1 * w1 + gb::kPriorPad);
                 h2d(prior, j.prior, w1 * w1 * w1, s);
+                launch_fill_u32(reinterpret_cast<uint32_t*>(prior + w1 * w1 * w1), gb::kPriorPad,
+                                0u, s);
                 for (size_t e = 0; e < w1 * w1 * w1; ++e)
                     prior_abs = std::max(prior_abs, iabs64(j.prior[e]));
             }
@@ -1017,7 +1110,7 @@ int32_t st_gapblock_batch_plan_create(const st_gapblock_job* jobs, uint32_t n_jo
             pr.len = len;
             pr.orig = orig;
             B.packs.push_back({j.r_codes, j.r_offsets, j.n_refs, pr.nwords, pr.npad, words, len,
-                               orig});
+                               orig, 4u});
             BatchJob d{};
             d.q_codes = j.q_codes;
             d.q_off = j.q_offsets;
@@ -1034,9 +1127,18 @@ int32_t st_gapblock_batch_plan_create(const st_gapblock_job* jobs, uint32_t n_jo
             const int64_t Lmax = std::max(j.q_max_len, j.r_max_len);
             d.exact = 2 * Lmax * pen_abs + prior_abs + int64_t(j.gap_open) +
                           Lmax * int64_t(j.gap_extend) >= int64_t(INT32_MAX);
-            d.profile = 2 * size_t(j.q_max_len) * A * sizeof(int32_t) <= 48 * 1024;
-            lds = std::max(lds, d.profile ? 2 * size_t(j.q_max_len) * A * sizeof(int32_t)
-                                          : size_t(A) * A * sizeof(int32_t));
+            d.profile = size_t(j.q_max_len) * gb::kRowBytes <= 48 * 1024;
+            lds = std::max(lds, size_t(d.profile ? j.q_max_len : A) * gb::kRowBytes);
+            int32_t big = 0;
+            d.short_path = short_launch(j.q_max_len, j.r_max_len, A, d.exact, pen_abs, prior_abs,
+                                        &big);
+            if (d.short_path) {
+                d.big = big;
+                d.prof_rows = j.q_max_len + gb::kFastMaxL;
+                d.slice_rows = j.r_max_len + 1;
+                lds = std::max(lds, size_t(d.prof_rows) * gb::kRowBytes +
+                                        size_t(d.slice_rows) * gb::kSliceStride * sizeof(int32_t));
+            }
             B.dev_jobs.push_back(d);
         }
         B.lds_bytes = lds;

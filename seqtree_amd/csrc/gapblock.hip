@@ -1,10 +1,11 @@
 // Single-gap-block scoring kernel (reference: src/gapblock.cpp:39-62 `cell`, driven by
 // src/gapblock.cpp:66-133).
 //
-// Workgroup = (query i, reference chunk c). The query's penalty profile prof[t][b] =
-// pen[q[t]][b] sits in LDS (or, for very long queries, the A x A penalty table), so every
-// lookup is one ds_read whose row is wave-uniform. Each lane owns one reference slot; its
-// residues stream in as 4-byte words (coalesced across the wave), and the "reference longer"
+// Workgroup = (query i, reference chunk c). The query's penalty profile (row t holds
+// pen[q[t]][b] and its negation, see gapblock_core.hpp) sits in LDS -- or, for very long
+// queries, the A x A penalty table in the same row format -- so every lookup is one ds_read
+// whose row is wave-uniform. Each lane owns one reference slot; its residues stream in as
+// 4-byte words of byte offsets (coalesced across the wave), and the "reference longer"
 // diagonal is realigned in registers with v_alignbyte instead of a second byte gather.
 //
 // With L = min(m, n), d = |m - n|, p0(t) the diagonal-0 penalty and pd(t) the diagonal-d
@@ -25,7 +26,7 @@ using gb::CsrWords;
 using gb::Look;
 using gb::PackedWords;
 using gb::pair_score;
-using gb::pair_score_uniform;
+using gb::pair_score_fast;
 
 // Appends (query, ref, score) for every lane with hit set: one atomic per wave.
 __device__ __forceinline__ void append_hit(bool hit, uint32_t qi, uint32_t k, int32_t score,
@@ -47,64 +48,77 @@ __device__ __forceinline__ void append_hit(bool hit, uint32_t qi, uint32_t k, in
     }
 }
 
-template <bool PROFILE, bool EXACT, bool THRESH>
+// SHORT: the short-launch layout (requires PROFILE && !EXACT): profile rows, then the
+// prior slice, both in the dynamic LDS.
+// A workgroup owns query i and chunks [g * group, min((g + 1) * group, nchunks)): the
+// query's tables are built once and reused across the group's chunks.
+template <bool PROFILE, bool EXACT, bool THRESH, bool SHORT>
 __global__ __launch_bounds__(kThreads) void gapblock_kernel(GapblockArgs a, uint64_t q_base) {
     extern __shared__ int32_t lds[];
     __shared__ int32_t stage[THRESH ? 1 : kChunk];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t nch = a.refs.nchunks;
-    const uint32_t c = blockIdx.x % nch;
-    const uint64_t i = q_base + blockIdx.x / nch;
+    const uint32_t ngroups = (nch + a.group - 1) / a.group;
+    const uint32_t g = blockIdx.x % ngroups;
+    const uint64_t i = q_base + blockIdx.x / ngroups;
+    const uint32_t c_end = (g + 1) * a.group < nch ? (g + 1) * a.group : nch;
     const uint64_t q0 = a.q_off[i];
     const uint32_t m = uint32_t(a.q_off[i + 1] - q0);
     const uint8_t* q = a.q_codes + q0;
     const uint32_t A = a.A;
+    int32_t* slice = lds + (SHORT ? a.prof_rows : 0u) * gb::kRowInts;
 
-    // PROFILE: prof[t][b] = pen[q[t]][b] followed by its negation (the fast path's -pd)
-    int32_t* nprof = lds + m * A;
-    if constexpr (PROFILE) {
-        for (uint32_t e = tid; e < m * A; e += kThreads) {
-            const uint32_t t = e / A;
-            const int32_t v = a.pen[uint32_t(q[t]) * A + (e - t * A)];
-            lds[e] = v;
-            nprof[e] = int32_t(0u - uint32_t(v));
-        }
+    if constexpr (SHORT) {
+        gb::build_rows(lds, a.prof_rows, m, a.pen, A, [&](uint32_t t) { return uint32_t(q[t]); },
+                       tid, kThreads);
+        gb::build_slice(slice, a.slice_rows - 1, m, a.prior, a.w1, a.big, tid, kThreads);
+    } else if constexpr (PROFILE) {
+        gb::build_rows(lds, m, m, a.pen, A, [&](uint32_t t) { return uint32_t(q[t]); }, tid,
+                       kThreads);
     } else {
-        for (uint32_t e = tid; e < A * A; e += kThreads) lds[e] = a.pen[e];
+        gb::build_rows(lds, A, A, a.pen, A, [](uint32_t r) { return r; }, tid, kThreads);
     }
     __syncthreads();
 
-    const Look<PROFILE> look{lds, q, A};
-    for (uint32_t it = 0; it < kSlotsPerThread; ++it) {
-        const uint32_t slot = c * kChunk + it * kThreads + tid;
-        const uint32_t k = a.refs.orig[slot];
-        const uint32_t n = a.refs.len[slot];
-        const PackedWords words{a.refs.words + slot, a.refs.npad};
-        int32_t score = 0;
-        // one reference length across the wave (padding lanes excepted): uniform fast path
-        const uint32_t n0 = __builtin_amdgcn_readfirstlane(n);
-        const bool uniform = __all(k == kNoSlot || n == n0);
-        if (PROFILE && !EXACT && uniform && (m < n0 ? m : n0) <= gb::kFastMaxL) {
-            score = pair_score_uniform(lds, nprof, A, words, m, n0, a.prior, a.w1, a.gap_open,
-                                       a.gap_extend);
-        } else if (k != kNoSlot) {
-            score = pair_score<PROFILE, EXACT>(look, words, m, n, a.prior, a.w1, a.gap_open,
-                                               a.gap_extend);
+    const Look<PROFILE> look{lds, q};
+    for (uint32_t c = g * a.group; c < c_end; ++c) {
+        for (uint32_t it = 0; it < kSlotsPerThread; ++it) {
+            const uint32_t slot = c * kChunk + it * kThreads + tid;
+            const uint32_t k = a.refs.orig[slot];
+            const uint32_t n = a.refs.len[slot];
+            const PackedWords words{a.refs.words + slot, a.refs.npad};
+            int32_t score = 0;
+            if constexpr (SHORT) {
+                score = gb::pair_score_short(lds, slice, words, k, m, n, a.prior, a.w1,
+                                             a.gap_open, a.gap_extend);
+            } else {
+                // one reference length across the wave (padding lanes excepted): uniform path
+                const uint32_t n0 = __builtin_amdgcn_readfirstlane(n);
+                const bool uniform = __all(k == kNoSlot || n == n0);
+                if (PROFILE && !EXACT && uniform && (m < n0 ? m : n0) <= gb::kFastMaxL) {
+                    score = pair_score_fast(lds, words, m, n0, a.prior, a.w1, a.gap_open,
+                                            a.gap_extend);
+                } else if (k != kNoSlot) {
+                    score = pair_score<PROFILE, EXACT>(look, words, m, n, a.prior, a.w1,
+                                                       a.gap_open, a.gap_extend);
+                }
+            }
+            if constexpr (THRESH) {
+                append_hit(k != kNoSlot && score <= a.hits.cutoff, uint32_t(i), k, score, a.hits);
+            } else {
+                if (k != kNoSlot) stage[k - c * kChunk] = score;
+            }
         }
-        if constexpr (THRESH) {
-            append_hit(k != kNoSlot && score <= a.hits.cutoff, uint32_t(i), k, score, a.hits);
-        } else {
-            if (k != kNoSlot) stage[k - c * kChunk] = score;
+        if constexpr (!THRESH) {
+            __syncthreads();
+            const uint64_t col0 = uint64_t(c) * kChunk;
+            const uint64_t lim64 = a.n_refs - col0;
+            const uint32_t lim = lim64 < kChunk ? uint32_t(lim64) : kChunk;
+            int32_t* row = a.out + i * a.n_refs + col0;
+            for (uint32_t t = tid; t < lim; t += kThreads) row[t] = stage[t];
+            __syncthreads();  // the stage is rewritten by the next chunk
         }
-    }
-    if constexpr (!THRESH) {
-        __syncthreads();
-        const uint64_t col0 = uint64_t(c) * kChunk;
-        const uint64_t lim64 = a.n_refs - col0;
-        const uint32_t lim = lim64 < kChunk ? uint32_t(lim64) : kChunk;
-        int32_t* row = a.out + i * a.n_refs + col0;
-        for (uint32_t t = tid; t < lim; t += kThreads) row[t] = stage[t];
     }
 }
 
@@ -114,55 +128,60 @@ template <bool EXACT>
 __global__ __launch_bounds__(kThreads) void gapblock_pairs_kernel(
     GapblockArgs a, const uint8_t* __restrict__ r_codes, const uint64_t* __restrict__ r_off,
     const uint32_t* __restrict__ pq, const uint32_t* __restrict__ pr, uint64_t n_pairs) {
-    __shared__ int32_t pen[kMaxAlphabet * kMaxAlphabet];
+    __shared__ int32_t pen[kMaxAlphabet * gb::kRowInts];
     const uint32_t A = a.A;
-    for (uint32_t e = threadIdx.x; e < A * A; e += kThreads) pen[e] = a.pen[e];
+    gb::build_rows(pen, A, A, a.pen, A, [](uint32_t r) { return r; }, threadIdx.x, kThreads);
     __syncthreads();
     const uint64_t stride = uint64_t(gridDim.x) * kThreads;
     for (uint64_t t = uint64_t(blockIdx.x) * kThreads + threadIdx.x; t < n_pairs; t += stride) {
         const uint32_t qi = pq[t], k = pr[t];
         const uint64_t q0 = a.q_off[qi], r0 = r_off[k];
         const uint32_t m = uint32_t(a.q_off[qi + 1] - q0), n = uint32_t(r_off[k + 1] - r0);
-        const Look<false> look{pen, a.q_codes + q0, A};
+        const Look<false> look{pen, a.q_codes + q0};
         a.out[t] = pair_score<false, EXACT>(look, CsrWords{r_codes + r0, n}, m, n, a.prior, a.w1,
                                             a.gap_open, a.gap_extend);
     }
 }
 
-template <bool PROFILE, bool EXACT, bool THRESH>
+template <bool PROFILE, bool EXACT, bool THRESH, bool SHORT>
 void launch_variant(const GapblockArgs& a, size_t lds_bytes, hipStream_t s) {
-    const uint64_t nch = a.refs.nchunks;
-    const uint64_t max_q = (uint64_t(1) << 30) / nch;  // keep a launch under 2^30 workgroups
+    const uint64_t ngroups = (a.refs.nchunks + a.group - 1) / a.group;
+    const uint64_t max_q = (uint64_t(1) << 30) / ngroups;  // launches under 2^30 workgroups
     for (uint64_t q0 = 0; q0 < a.n_queries; q0 += max_q) {
         const uint64_t nq = a.n_queries - q0 < max_q ? a.n_queries - q0 : max_q;
-        hipLaunchKernelGGL((gapblock_kernel<PROFILE, EXACT, THRESH>), dim3(uint32_t(nq * nch)),
-                           dim3(kThreads), lds_bytes, s, a, q0);
+        hipLaunchKernelGGL((gapblock_kernel<PROFILE, EXACT, THRESH, SHORT>),
+                           dim3(uint32_t(nq * ngroups)), dim3(kThreads), lds_bytes, s, a, q0);
     }
 }
 
 template <bool THRESH>
 void launch_mode(const GapblockArgs& a, bool use_profile, size_t lds_bytes, bool exact_wrap,
-                 hipStream_t s) {
-    if (use_profile) {
-        if (exact_wrap) launch_variant<true, true, THRESH>(a, lds_bytes, s);
-        else launch_variant<true, false, THRESH>(a, lds_bytes, s);
+                 bool short_path, hipStream_t s) {
+    if (short_path) {
+        launch_variant<true, false, THRESH, true>(a, lds_bytes, s);
+    } else if (use_profile) {
+        if (exact_wrap) launch_variant<true, true, THRESH, false>(a, lds_bytes, s);
+        else launch_variant<true, false, THRESH, false>(a, lds_bytes, s);
     } else {
-        if (exact_wrap) launch_variant<false, true, THRESH>(a, lds_bytes, s);
-        else launch_variant<false, false, THRESH>(a, lds_bytes, s);
+        if (exact_wrap) launch_variant<false, true, THRESH, false>(a, lds_bytes, s);
+        else launch_variant<false, false, THRESH, false>(a, lds_bytes, s);
     }
 }
 
 }  // namespace
 
 void launch_gapblock(const GapblockArgs& a, bool use_profile, uint32_t q_max_len, bool exact_wrap,
-                     hipStream_t s) {
+                     bool short_path, hipStream_t s) {
     if (a.n_queries == 0 || a.refs.nchunks == 0) return;
-    // profile mode holds the profile and its negation
-    const size_t lds = use_profile ? 2 * size_t(q_max_len) * a.A * sizeof(int32_t)
-                                   : size_t(a.A) * a.A * sizeof(int32_t);
+    // one table row (penalties and their negations) per query position or per residue; short
+    // launches add zero rows and the prior slice
+    const size_t lds =
+        short_path ? size_t(a.prof_rows) * gb::kRowBytes +
+                         size_t(a.slice_rows) * gb::kSliceStride * sizeof(int32_t)
+                   : size_t(use_profile ? q_max_len : a.A) * gb::kRowBytes;
     const size_t lds_bytes = lds ? lds : sizeof(int32_t);
-    if (a.hits.count) launch_mode<true>(a, use_profile, lds_bytes, exact_wrap, s);
-    else launch_mode<false>(a, use_profile, lds_bytes, exact_wrap, s);
+    if (a.hits.count) launch_mode<true>(a, use_profile, lds_bytes, exact_wrap, short_path, s);
+    else launch_mode<false>(a, use_profile, lds_bytes, exact_wrap, short_path, s);
 }
 
 void launch_gapblock_pairs(const GapblockArgs& a, const uint8_t* r_codes, const uint64_t* r_off,

@@ -1,6 +1,13 @@
 // Per-pair gap-block scoring shared by the dense / threshold / pair-list kernels
 // (gapblock.hip) and the ragged multi-job kernel (batch.hip).
 // Reference: src/gapblock.cpp:39-62 (`cell`).
+//
+// Table layout in LDS. A "row" is 64 int32 (256 bytes): entries 0..31 hold pen[a][b] for
+// b < A (zero past A), entries 32..63 hold the negations -pen[a][b]. In profile mode row t is
+// the query position t (a = q[t]); in table mode row a is the query residue a. Residue codes
+// travel as byte offsets (code * 4, at most 124 for A <= 32), so a lookup is one ds_read at
+// row * 256 + code4 (+ 128 for the negation), with the row part folded into the
+// instruction's immediate offset once positions are unrolled.
 #pragma once
 
 #include "st_internal.hpp"
@@ -10,21 +17,74 @@
 namespace st {
 namespace gb {
 
+constexpr uint32_t kRowInts = 64;
+constexpr uint32_t kRowBytes = kRowInts * 4;
+constexpr uint32_t kNegBytes = 128;
+
+// Fast path limit: at most this many block-start positions (L = min(m, n)).
+constexpr uint32_t kFastMaxL = 32;
+// Ints of zero padding after every device copy of a prior cube, so the fast path can read
+// kFastMaxL + 1 consecutive entries of any prior row without a bounds check.
+constexpr uint32_t kPriorPad = kFastMaxL + 1;
+
+__device__ __forceinline__ int32_t lds_at(const int32_t* base, uint32_t byte_off) {
+    return *reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
+// Fills `rows` table rows (see above) from the A x A penalty table: row r < valid uses
+// residue code_of(r), rows past `valid` are all zero. Called by all threads of a workgroup
+// of `threads` threads.
+template <class CodeOf>
+__device__ __forceinline__ void build_rows(int32_t* lds, uint32_t rows, uint32_t valid,
+                                           const int32_t* pen, uint32_t A, CodeOf&& code_of,
+                                           uint32_t tid, uint32_t threads) {
+    for (uint32_t e = tid; e < rows * kRowInts; e += threads) {
+        const uint32_t r = e / kRowInts, c = e % kRowInts, b = c % 32;
+        const int32_t v = (r < valid && b < A) ? pen[code_of(r) * A + b] : 0;
+        lds[e] = c < 32 ? v : int32_t(0u - uint32_t(v));
+    }
+}
+
+// ---- short-sequence launches (every query <= kFastMaxL, references <= kShortMaxRef) ------
+// The profile gets kFastMaxL zero rows past the query, the alphabet leaves column 31 zero
+// (A <= 31) and reference words are padded with code 31 (byte offset kPadByte), so every
+// lookup past a pair's own block positions reads 0. The per-query prior slice
+// slice[n][i] (row n = reference length, stride kSliceStride) holds the prior of candidate
+// i for this query against a length-n reference, or `big` past i = min(m, n); big is chosen
+// on the host so that D + big neither wraps nor undercuts a real candidate. Together they
+// let a wave whose lanes hold references of different lengths run one straight-line
+// sequence of 4 * NW positions.
+constexpr uint32_t kShortMaxRef = 64;
+constexpr uint32_t kSliceStride = kFastMaxL + 1;
+constexpr uint32_t kPadByte = 31u * 4u;
+
+__device__ __forceinline__ void build_slice(int32_t* slice, uint32_t r_max, uint32_t m,
+                                            const int32_t* prior, uint32_t w1, int32_t big,
+                                            uint32_t tid, uint32_t threads) {
+    for (uint32_t e = tid; e < (r_max + 1) * kSliceStride; e += threads) {
+        const uint32_t n = e / kSliceStride, i = e % kSliceStride;
+        const uint32_t L = m < n ? m : n, d = m > n ? m - n : n - m;
+        int32_t v = big;
+        if (i <= L) v = (prior && d) ? prior[(uint64_t(m > n ? m : n) * w1 + d) * w1 + i] : 0;
+        slice[e] = v;
+    }
+}
+
 template <bool PROFILE>
 struct Look {
     const int32_t* lds;
     const uint8_t* q;
-    uint32_t A;
-    __device__ __forceinline__ int32_t operator()(uint32_t t, uint32_t rc) const {
+    // positive penalty of query position t against reference byte offset rc4
+    __device__ __forceinline__ int32_t operator()(uint32_t t, uint32_t rc4) const {
         if constexpr (PROFILE) {
-            return lds[t * A + rc];
+            return lds_at(lds, t * kRowBytes + rc4);
         } else {
-            return lds[uint32_t(q[t]) * A + rc];
+            return lds_at(lds, uint32_t(q[t]) * kRowBytes + rc4);
         }
     }
 };
 
-// Reference residues as little-endian words: residues 4w..4w+3 of one sequence.
+// Reference residues as little-endian words of byte offsets: residues 4w..4w+3.
 struct PackedWords {  // the chunk-packed layout, one slot
     const uint32_t* wp;
     uint32_t npad;
@@ -33,7 +93,7 @@ struct PackedWords {  // the chunk-packed layout, one slot
     }
 };
 
-struct CsrWords {  // straight from the CSR bytes (pair-list mode)
+struct CsrWords {  // straight from the CSR codes (pair-list mode)
     const uint8_t* r;
     uint32_t n;
     __device__ __forceinline__ uint32_t operator()(uint32_t w) const {
@@ -41,7 +101,7 @@ struct CsrWords {  // straight from the CSR bytes (pair-list mode)
         const uint32_t p = 4 * w;
 #pragma unroll
         for (uint32_t b = 0; b < 4; ++b)
-            if (p + b < n) word |= uint32_t(r[p + b]) << (8 * b);
+            if (p + b < n) word |= (uint32_t(r[p + b]) * 4u) << (8 * b);
         return word;
     }
 };
@@ -63,10 +123,10 @@ __device__ __forceinline__ void walk(const Look<PROFILE>& look, const Words& wor
         for (uint32_t kk = 0; kk < 4; ++kk) {
             const uint32_t t = 4 * w + kk;
             if (t < L) {
-                const uint32_t rc = (x >> (8 * kk)) & 0xFFu;
-                const int32_t p0 = look(t, rc);
+                const uint32_t rc4 = (x >> (8 * kk)) & 0xFFu;
+                const int32_t p0 = look(t, rc4);
                 const int32_t pd =
-                    q_longer ? look(t + d, rc) : look(t, (y >> (8 * kk)) & 0xFFu);
+                    q_longer ? look(t + d, rc4) : look(t, (y >> (8 * kk)) & 0xFFu);
                 f(t, p0, pd);
             }
         }
@@ -120,64 +180,181 @@ __device__ __forceinline__ int32_t pair_score(const Look<PROFILE>& look, const W
 }
 
 // Fast path for a wave whose lanes all hold references of one length n (the common case:
-// references are length-sorted inside a chunk and the query is the workgroup's). Then L, d,
-// the branch and the prior row are wave-uniform: the positions are fully unrolled under
-// scalar guards, prior values come through scalar loads, and with prof (pen) and nprof (-pen)
-// in LDS each block position costs one v_add3 (D += p0 - pd), one add (-Sd), one add (prior)
-// and one min, plus the two lookups' address arithmetic. Valid only when no int32 sum can
-// wrap (the non-exact case) and L <= kFastMaxL. rowoff: A * 4 bytes per profile row.
-constexpr uint32_t kFastMaxL = 32;
-
-__device__ __forceinline__ int32_t lds_at(const int32_t* base, uint32_t byte_off) {
-    return *reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(base) + byte_off);
+// references are length-sorted inside a chunk and the query is the workgroup's profile).
+// Then L, d and the prior row are wave-uniform, so the pair runs as straight-line code,
+// instantiated per word count NW = ceil(L / 4) so every array below has a compile-time size
+// and lives in registers: all reference words are loaded up front, the prior row is read
+// into scalar registers in one go (device prior cubes carry kPriorPad ints of padding, so
+// reading 4 * NW + 1 <= kFastMaxL + 1 entries of any row stays in bounds), and each block
+// position costs two ds_reads with immediate row offsets plus a few adds and one min.
+// MODE 0: d == 0; MODE 1: query longer; MODE 2: reference longer. Valid only when no int32
+// sum can wrap (the non-exact case) and L <= kFastMaxL.
+template <int MODE, uint32_t NW, class Words>
+__device__ __forceinline__ int32_t pair_score_uniform(const int32_t* prof, const Words& words,
+                                                      uint32_t L, uint32_t d,
+                                                      const int32_t* prow, int32_t go,
+                                                      int32_t ge) {
+    constexpr uint32_t kT = 4 * NW;
+    uint32_t x[NW];
+#pragma unroll
+    for (uint32_t w = 0; w < NW; ++w) x[w] = words(w);
+    if constexpr (MODE == 0) {
+        uint32_t S = 0;
+#pragma unroll
+        for (uint32_t t = 0; t < kT; ++t) {
+            if (t + 4 >= kT && t >= L) break;  // only the last word can be partial
+            S += uint32_t(lds_at(prof, t * kRowBytes + ((x[t / 4] >> (8 * (t % 4))) & 0xFFu)));
+        }
+        return int32_t(S);
+    } else {
+        uint32_t y[NW];
+        if constexpr (MODE == 2) {
+            // reference longer: the diagonal-d residue of position t is r[t + d]
+            const uint32_t dq = d >> 2, sh = d & 3u;
+            uint32_t z[NW + 1];
+#pragma unroll
+            for (uint32_t j = 0; j <= NW; ++j) z[j] = words(dq + j);
+#pragma unroll
+            for (uint32_t w = 0; w < NW; ++w) y[w] = __builtin_amdgcn_alignbyte(z[w + 1], z[w], sh);
+        }
+        int32_t pv[kT + 1];
+        if (prow) {
+#pragma unroll
+            for (uint32_t t = 0; t <= kT; ++t) pv[t] = prow[t];
+        } else {
+#pragma unroll
+            for (uint32_t t = 0; t <= kT; ++t) pv[t] = 0;
+        }
+        // MODE 1: the diagonal-d lookup reads query row t + d of the negated half
+        const uint32_t off1 = MODE == 1 ? d * kRowBytes + kNegBytes : kNegBytes;
+        uint32_t D = 0, nS = 0;  // D = P0 - Pd, nS = -Pd
+        int32_t best = pv[0];
+#pragma unroll
+        for (uint32_t t = 0; t < kT; ++t) {
+            if (t + 4 >= kT && t >= L) break;
+            const uint32_t rc4 = (x[t / 4] >> (8 * (t % 4))) & 0xFFu;
+            const uint32_t p0 = uint32_t(lds_at(prof, t * kRowBytes + rc4));
+            uint32_t npd;
+            if constexpr (MODE == 1) {
+                npd = uint32_t(lds_at(prof, t * kRowBytes + (rc4 + off1)));
+            } else {
+                const uint32_t rd4 = (y[t / 4] >> (8 * (t % 4))) & 0xFFu;
+                npd = uint32_t(lds_at(prof, t * kRowBytes + kNegBytes + rd4));
+            }
+            D += p0 + npd;
+            nS += npd;
+            const int32_t cand = int32_t(D + uint32_t(pv[t + 1]));
+            best = cand < best ? cand : best;
+        }
+        const uint32_t Sd = 0u - nS;
+        return int32_t(Sd + uint32_t(best) + uint32_t(go) + (d - 1) * uint32_t(ge));
+    }
 }
 
+template <int MODE, class Words>
+__device__ __forceinline__ int32_t pair_score_words(const int32_t* prof, const Words& words,
+                                                    uint32_t L, uint32_t d, const int32_t* prow,
+                                                    int32_t go, int32_t ge) {
+    switch ((L + 3) / 4) {
+        case 0:
+        case 1: return pair_score_uniform<MODE, 1>(prof, words, L, d, prow, go, ge);
+        case 2: return pair_score_uniform<MODE, 2>(prof, words, L, d, prow, go, ge);
+        case 3: return pair_score_uniform<MODE, 3>(prof, words, L, d, prow, go, ge);
+        case 4: return pair_score_uniform<MODE, 4>(prof, words, L, d, prow, go, ge);
+        case 5: return pair_score_uniform<MODE, 5>(prof, words, L, d, prow, go, ge);
+        case 6: return pair_score_uniform<MODE, 6>(prof, words, L, d, prow, go, ge);
+        case 7: return pair_score_uniform<MODE, 7>(prof, words, L, d, prow, go, ge);
+        default: return pair_score_uniform<MODE, 8>(prof, words, L, d, prow, go, ge);
+    }
+}
+
+// Dispatches the fast path on the (wave-uniform) lengths; requires min(m, n) <= kFastMaxL.
 template <class Words>
-__device__ __forceinline__ int32_t pair_score_uniform(const int32_t* prof, const int32_t* nprof,
-                                                      uint32_t A, const Words& words, uint32_t m,
-                                                      uint32_t n, const int32_t* prior,
-                                                      uint32_t w1, int32_t go, int32_t ge) {
+__device__ __forceinline__ int32_t pair_score_fast(const int32_t* prof, const Words& words,
+                                                   uint32_t m, uint32_t n,
+                                                   const int32_t* prior, uint32_t w1,
+                                                   int32_t go, int32_t ge) {
     const uint32_t L = m < n ? m : n;
     const uint32_t d = m > n ? m - n : n - m;
-    const bool q_longer = m >= n;
-    const uint32_t row = A * 4;
-    const int32_t* __restrict__ prow =
-        (prior && d) ? prior + (uint64_t(q_longer ? m : n) * w1 + d) * w1 : nullptr;
+    if (d == 0) return pair_score_words<0>(prof, words, L, 0, nullptr, go, ge);
+    const int32_t* prow = prior ? prior + (uint64_t(m > n ? m : n) * w1 + d) * w1 : nullptr;
+    if (m > n) return pair_score_words<1>(prof, words, L, d, prow, go, ge);
+    return pair_score_words<2>(prof, words, L, d, prow, go, ge);
+}
+
+// Mixed-length wave in a short launch: per-lane n (and so L, d, the diagonal and the prior
+// row) but one instruction stream of 4 * NW positions, NW = ceil(max_lane L / 4). The
+// diagonal-d residue comes from the lane's own realigned words (query longer: the same
+// residue, read from row t + d of the negated half; reference longer: r[t + d], gathered
+// through per-lane word indices). Positions past the lane's L read zero penalties and
+// prior `big`, so they change neither the sums nor the minimum.
+template <uint32_t NW, class Words>
+__device__ __forceinline__ int32_t pair_score_mixed(const int32_t* prof, const int32_t* slice,
+                                                    const Words& words, uint32_t m, uint32_t n,
+                                                    int32_t go, int32_t ge) {
+    const uint32_t d = m > n ? m - n : n - m;
+    const bool ql = m >= n;
+    const uint32_t dq = ql ? 0u : d >> 2, sh = ql ? 0u : d & 3u;
+    const uint32_t off1 = (ql ? d * kRowBytes : 0u) + kNegBytes;
+    const int32_t* prow = slice + n * kSliceStride;
+    uint32_t x[NW], z[NW + 1];
+#pragma unroll
+    for (uint32_t w = 0; w < NW; ++w) x[w] = words(w);
+#pragma unroll
+    for (uint32_t j = 0; j <= NW; ++j) z[j] = words(dq + j);
     uint32_t D = 0, nS = 0;  // D = P0 - Pd, nS = -Pd
-    int32_t best = prow ? prow[0] : 0;
-    // byte offset of the second lookup's row relative to the first (diagonal d)
-    const uint32_t shift_rows = q_longer ? d * row : 0u;
+    int32_t best = prow[0];
 #pragma unroll
-    for (uint32_t w = 0; w < kFastMaxL / 4; ++w) {
-        if (4 * w >= L) break;
-        const uint32_t x = words(w);
-        uint32_t y = x;
-        if (!q_longer) {
-            const uint32_t t0 = 4 * w + d;
-            y = __builtin_amdgcn_alignbyte(words((t0 >> 2) + 1), words(t0 >> 2), t0 & 3u);
-        }
+    for (uint32_t w = 0; w < NW; ++w) {
+        const uint32_t y = __builtin_amdgcn_alignbyte(z[w + 1], z[w], sh);
 #pragma unroll
-        for (uint32_t kk = 0; kk < 4; ++kk) {
-            const uint32_t t = 4 * w + kk;
-            if (t >= L) break;
-            const uint32_t a0 = ((x >> (8 * kk)) & 0xFFu) * 4 + t * row;
-            if (d == 0) {
-                nS -= uint32_t(lds_at(prof, a0));
-            } else {
-                const uint32_t a1 =
-                    q_longer ? a0 + shift_rows : ((y >> (8 * kk)) & 0xFFu) * 4 + t * row;
-                const uint32_t p0 = uint32_t(lds_at(prof, a0));
-                const uint32_t npd = uint32_t(lds_at(nprof, a1));
-                D += p0 + npd;
-                nS += npd;
-                const int32_t cand = int32_t(D + uint32_t(prow ? prow[t + 1] : 0));
-                best = cand < best ? cand : best;
-            }
+        for (uint32_t b = 0; b < 4; ++b) {
+            const uint32_t t = 4 * w + b;
+            const uint32_t rc4 = (x[w] >> (8 * b)) & 0xFFu;
+            const uint32_t rd4 = (y >> (8 * b)) & 0xFFu;
+            const uint32_t p0 = uint32_t(lds_at(prof, t * kRowBytes + rc4));
+            const uint32_t npd = uint32_t(lds_at(prof, t * kRowBytes + (rd4 + off1)));
+            D += p0 + npd;
+            nS += npd;
+            const int32_t cand = int32_t(D + uint32_t(prow[t + 1]));
+            best = cand < best ? cand : best;
         }
+        // keep at most one word of lookups in flight: bounds the registers (and so keeps
+        // occupancy up) at the cost of per-word LDS latency, which other waves hide
+        __builtin_amdgcn_sched_barrier(0);
     }
     const uint32_t Sd = 0u - nS;
     return d == 0 ? int32_t(Sd)
                   : int32_t(Sd + uint32_t(best) + uint32_t(go) + (d - 1) * uint32_t(ge));
+}
+
+// One reference slot of a short launch: the uniform fast path when the wave's references
+// share one length, else the mixed path. m is the workgroup's query length (<= kFastMaxL).
+template <class Words>
+__device__ __forceinline__ int32_t pair_score_short(const int32_t* prof, const int32_t* slice,
+                                                    const Words& words, uint32_t k, uint32_t m,
+                                                    uint32_t n, const int32_t* prior,
+                                                    uint32_t w1, int32_t go, int32_t ge) {
+    const uint32_t n0 = __builtin_amdgcn_readfirstlane(n);
+    if (__all(k == kNoSlot || n == n0)) return pair_score_fast(prof, words, m, n0, prior, w1, go, ge);
+    uint32_t nmax = n;
+#pragma unroll
+    for (uint32_t o = 32; o > 0; o >>= 1) {
+        const uint32_t v = __shfl_xor(nmax, int(o));
+        nmax = v > nmax ? v : nmax;
+    }
+    const uint32_t Lmax = __builtin_amdgcn_readfirstlane(m < nmax ? m : nmax);
+    switch ((Lmax + 3) / 4) {
+        case 0:
+        case 1: return pair_score_mixed<1>(prof, slice, words, m, n, go, ge);
+        case 2: return pair_score_mixed<2>(prof, slice, words, m, n, go, ge);
+        case 3: return pair_score_mixed<3>(prof, slice, words, m, n, go, ge);
+        case 4: return pair_score_mixed<4>(prof, slice, words, m, n, go, ge);
+        case 5: return pair_score_mixed<5>(prof, slice, words, m, n, go, ge);
+        case 6: return pair_score_mixed<6>(prof, slice, words, m, n, go, ge);
+        case 7: return pair_score_mixed<7>(prof, slice, words, m, n, go, ge);
+        default: return pair_score_mixed<8>(prof, slice, words, m, n, go, ge);
+    }
 }
 
 }  // namespace gb

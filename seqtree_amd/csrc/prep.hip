@@ -14,7 +14,7 @@ constexpr uint32_t kPackThreads = 512;  // kChunk / 2: one compare-exchange per 
 __global__ __launch_bounds__(kPackThreads) void pack_refs_kernel(
     const uint8_t* __restrict__ bytes, const uint64_t* __restrict__ off, uint64_t n,
     uint32_t nwords, uint32_t npad, uint32_t* __restrict__ words, uint32_t* __restrict__ len_out,
-    uint32_t* __restrict__ orig_out) {
+    uint32_t* __restrict__ orig_out, uint32_t scale) {
     __shared__ uint64_t keys[kChunk];
     const uint32_t tid = threadIdx.x;
     const uint64_t base = uint64_t(blockIdx.x) * kChunk;
@@ -52,16 +52,14 @@ __global__ __launch_bounds__(kPackThreads) void pack_refs_kernel(
         len_out[slot] = L;
         orig_out[slot] = valid ? uint32_t(k) : kNoSlot;
         const uint8_t* src = bytes + (valid ? off[k] : 0);
+        // bytes past the end: 0 for raw bytes, code 31 (byte offset 124) for scale 4
+        const uint32_t pad = scale == 4 ? 124u : 0u;
         for (uint32_t w = 0; w <= nwords; ++w) {
             uint32_t word = 0;
             const uint32_t p = 4 * w;
-            if (p < L) {
-                const uint32_t left = L - p;
-                word = src[p];
-                if (left > 1) word |= uint32_t(src[p + 1]) << 8;
-                if (left > 2) word |= uint32_t(src[p + 2]) << 16;
-                if (left > 3) word |= uint32_t(src[p + 3]) << 24;
-            }
+#pragma unroll
+            for (uint32_t b = 0; b < 4; ++b)
+                word |= (p + b < L ? uint32_t(src[p + b]) * scale : pad) << (8 * b);
             words[uint64_t(w) * npad + slot] = word;
         }
     }
@@ -87,11 +85,11 @@ __global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint64_t n, uint32_t v
 
 void launch_pack_refs(const uint8_t* bytes, const uint64_t* off, uint64_t n, uint32_t nwords,
                       uint32_t npad, uint32_t* words, uint32_t* len, uint32_t* orig,
-                      hipStream_t s) {
+                      uint32_t scale, hipStream_t s) {
     const uint32_t chunks = npad / kChunk;
     if (chunks == 0) return;
     hipLaunchKernelGGL(pack_refs_kernel, dim3(chunks), dim3(kPackThreads), 0, s, bytes, off, n,
-                       nwords, npad, words, len, orig);
+                       nwords, npad, words, len, orig, scale);
 }
 
 void launch_classify(const uint64_t* off, uint64_t n, const ClassBounds& b, uint32_t* counts,

@@ -38,9 +38,12 @@ struct PackedRefs {
 };
 
 // ---- prep.hip --------------------------------------------------------------------------
+// scale multiplies every byte (1: raw bytes / codes, zero past the end; 4: codes as LDS byte
+// offsets, the gap-block kernels' form -- codes are < 32, so code * 4 still fits a byte --
+// with code 31 (byte 124) past the end).
 void launch_pack_refs(const uint8_t* bytes, const uint64_t* off, uint64_t n, uint32_t nwords,
                       uint32_t npad, uint32_t* words, uint32_t* len, uint32_t* orig,
-                      hipStream_t s);
+                      uint32_t scale, hipStream_t s);
 
 // Buckets the n queries by length: class c receives every query with
 // length <= bounds[c] (and > bounds[c-1]); the last bound must be 0xFFFFFFFF.
@@ -53,6 +56,12 @@ void launch_classify(const uint64_t* off, uint64_t n, const ClassBounds& b, uint
                      uint32_t* lists, hipStream_t s);
 
 void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+
+// ---- probe.hip -------------------------------------------------------------------------
+// VALU issue-rate probe: `blocks` workgroups of 256 threads, each thread 16 chains of
+// `iters` instructions of the named kind (probe_name returns null past the last kind).
+const char* probe_name(int kind);
+void launch_probe(int kind, uint32_t* out, uint32_t blocks, uint32_t iters, hipStream_t s);
 
 // ---- gapblock.hip ----------------------------------------------------------------------
 // Thresholded output: every pair with score <= cutoff is appended as (query, ref, score);
@@ -78,12 +87,20 @@ struct GapblockArgs {
     uint32_t w1;           // prior_width + 1
     int32_t* out;          // dense N x K output (or one score per pair in pair-list mode)
     GapblockHits hits;     // hits.count != nullptr selects the thresholded output
+    // short launches (queries <= 32, references <= 64, A <= 31): profile rows per workgroup
+    // (query length + 32 zero rows), prior-slice rows (reference lengths 0..r_max) and the
+    // slice's past-the-end value
+    uint32_t prof_rows;
+    uint32_t slice_rows;
+    int32_t big;
+    uint32_t group;        // reference chunks per workgroup (>= 1)
 };
-// use_profile: stage a per-query profile (q_max_len * A int32) in LDS.
+// use_profile: stage a per-query profile (one table row per query position) in LDS.
 // exact_wrap: evaluate the candidates in the reference's order with wrapping int32
 // arithmetic (needed only when an int32 sum may overflow).
+// short_path: the short-launch kernels (a.prof_rows / a.slice_rows / a.big set).
 void launch_gapblock(const GapblockArgs& a, bool use_profile, uint32_t q_max_len, bool exact_wrap,
-                     hipStream_t s);
+                     bool short_path, hipStream_t s);
 // Scores an explicit list of (query, ref) pairs into a.out[t]; references come as CSR.
 void launch_gapblock_pairs(const GapblockArgs& a, const uint8_t* r_codes, const uint64_t* r_off,
                            const uint32_t* pair_query, const uint32_t* pair_ref, uint64_t n_pairs,
@@ -105,6 +122,10 @@ struct BatchJob {
     int32_t* out;          // device, n_queries * n_refs
     uint32_t profile;      // LDS profile (+ negation) rather than the A x A table
     uint32_t exact;        // wrapping-order evaluation
+    uint32_t short_path;   // short-launch evaluation (as GapblockArgs)
+    uint32_t prof_rows;
+    uint32_t slice_rows;
+    int32_t big;
     uint64_t unit_begin;   // index of the job's first work unit
 };
 // Tile shape of the persistent launch: queries per work unit, threads per workgroup

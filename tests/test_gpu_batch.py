@@ -87,3 +87,27 @@ def test_batch_against_the_oracle(gpu, oracle_lib):
                                           _prior_cube(job.gap_prior, w), w, threads=8)
         assert np.array_equal(got, want)
     batch.close()
+
+
+def test_batch_encoded_sets_equal_strings(gpu):
+    st = gpu
+    from seqtree_amd.batch import EncodedSet, GapblockBatch, GapblockJob
+
+    rng = random.Random(11)
+    syms = st.amino_acids()
+    m = st.SubstitutionMatrix.structural()
+
+    def enc(seqs):
+        off = np.zeros(len(seqs) + 1, np.uint64)
+        off[1:] = np.cumsum([len(s) for s in seqs])
+        codes = np.array([syms.index(c) for s in seqs for c in s], np.uint8)
+        return EncodedSet(codes, off)
+
+    q, r = cdr3_like(rng, 33, 6, 30), cdr3_like(rng, 1500, 6, 30)
+    jobs = [GapblockJob(q, r, m, 14, 1, st.central_prior(10)),
+            GapblockJob(enc(q), enc(r), m, 14, 1, st.central_prior(10))]
+    batch = GapblockBatch(jobs)
+    batch.run()
+    a, b = batch.results()
+    assert np.array_equal(a, b) and np.array_equal(a, _dense(st, jobs[0]))
+    batch.close()

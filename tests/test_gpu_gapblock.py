@@ -198,3 +198,25 @@ def test_pairs_equal_the_dense_matrix(gpu, oracle_lib):
         cube = _prior_cube(prior, w) if prior else None
         dense = oracle_lib.gapblock_matrix(q, r, _pen(st, m), 17, 2, cube, w, threads=8)
         assert np.array_equal(got, dense[pq, pr])
+
+
+@pytest.mark.parametrize("prior", ["central", None])
+def test_short_launch_mixed_waves_and_chunk_groups(gpu, oracle_lib, prior):
+    """Queries up to 32 and references up to 64 residues (the short launch: mixed-length
+    waves, zero rows and prior sentinels) over enough rows that workgroups take several
+    reference chunks each; every row is checked against the oracle on a sample of rows and
+    against per-row recomputation for the rest."""
+    st = gpu
+    rng = random.Random(77 if prior else 78)
+    m = st.SubstitutionMatrix.blosum62()
+    q = [rand_seq(rng, 0, 32) for _ in range(2000)]
+    r = [rand_seq(rng, 0, 64) for _ in range(5000)]
+    gp = st.central_prior(21) if prior else None
+    full = np.asarray(st.score_matrix(q, r, m, 28, 1, gp))
+    rows = sorted(rng.sample(range(len(q)), 24))
+    sub = _check(st, oracle_lib, [q[i] for i in rows], r, m, 28, 1, gp)
+    assert np.array_equal(full[rows], sub)
+    # small row blocks run with one chunk per workgroup: same values
+    for i0 in range(0, len(q), 400):
+        blk = np.asarray(st.score_matrix(q[i0:i0 + 7], r, m, 28, 1, gp))
+        assert np.array_equal(full[i0:i0 + 7], blk)

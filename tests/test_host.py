@@ -312,3 +312,18 @@ def test_pair_list_validation(st):
     assert len(st.gapblock.score_pairs(["CAS"], ["CA"], [], [], m, 28)) == 0
     qi, ri, sc = st.gapblock.score_threshold([], ["CA"], 10, m, 28)
     assert len(qi) == len(ri) == len(sc) == 0
+
+
+def test_batch_encoded_set_validation():
+    from seqtree_amd.batch import EncodedSet, _encode
+
+    syms = "ACGT"
+    codes, off = _encode(EncodedSet(np.array([0, 1, 2, 3], np.uint8),
+                                    np.array([0, 2, 4], np.uint64)), syms)
+    assert list(off) == [0, 2, 4] and list(codes) == [0, 1, 2, 3]
+    with pytest.raises(ValueError, match="outside the alphabet"):
+        _encode(EncodedSet(np.array([0, 4], np.uint8), np.array([0, 2], np.uint64)), syms)
+    with pytest.raises(ValueError, match="malformed"):
+        _encode(EncodedSet(np.array([0, 1], np.uint8), np.array([0, 3], np.uint64)), syms)
+    codes, off = _encode(EncodedSet(np.zeros(0, np.uint8), np.zeros(1, np.uint64)), syms)
+    assert list(off) == [0]
