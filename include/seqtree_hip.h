@@ -123,6 +123,9 @@ int32_t st_gapblock_matrix_device(const uint8_t* q_codes, const uint64_t* q_offs
                                   int32_t gap_extend, const int32_t* prior, uint32_t prior_width,
                                   int32_t* out, void* stream, st_status* status);
 
+/* The device form of st_gapblock_threshold: hit_query / hit_ref / hit_score AND hit_count are
+ * device memory. *hit_count is a device counter (zeroed by the call, then counted into by the
+ * kernels); read it back once the stream's work is done. */
 int32_t st_gapblock_threshold_device(const uint8_t* q_codes, const uint64_t* q_offsets,
                                      uint64_t n_queries, uint32_t q_max_len,
                                      const uint8_t* r_codes, const uint64_t* r_offsets,
@@ -208,6 +211,52 @@ const char* st_version(void);
 
 /* Number of visible HIP devices (0 when none); never faults on a machine without a GPU. */
 int32_t st_device_count(void);
+
+/* ---- learned prefilter (no counterpart in the reference) -------------------------------
+ * A 1D-conv encoder (embedding 32, three width-5 convolutions of 128 channels with ReLU,
+ * mean pool, 128-d output; BF16 operands, FP32 accumulation) whose squared embedding
+ * distance |e(a) - e(b)|^2 = s(a,a) + s(b,b) - 2 s(a,b) screens (query, reference) pairs
+ * before exact gap-block rescoring. Sequences are codec codes as CSR in device memory, at most
+ * 64 residues each. All pointers below are device pointers unless noted; the calls are
+ * enqueued on `stream` (NULL: the null stream) and the ones returning host values synchronise
+ * it. A model is used by one thread at a time. */
+typedef struct st_prefilter st_prefilter;
+uint64_t st_prefilter_param_count(void);
+int32_t st_prefilter_create(uint64_t seed, st_prefilter** model, st_status* status);
+void st_prefilter_destroy(st_prefilter* model);
+/* FP32 parameters to / from host memory (count = st_prefilter_param_count()). */
+int32_t st_prefilter_get_params(const st_prefilter* model, float* params, uint64_t count,
+                                st_status* status);
+int32_t st_prefilter_set_params(st_prefilter* model, const float* params, uint64_t count,
+                                st_status* status);
+/* The gradient of the last train step (host copy), for checking the backward pass. */
+int32_t st_prefilter_get_grads(const st_prefilter* model, float* grads, uint64_t count,
+                               st_status* status);
+/* One Adam step on the pair loss sum_t weight[t] * (|e(a_t) - e(b_t)|^2 - target[t])^2 over
+ * the n_seqs sequences given (pair_a / pair_b index them). *loss_out (host) receives the mean
+ * loss before the step. */
+int32_t st_prefilter_train_step(st_prefilter* model, const uint8_t* codes,
+                                const uint64_t* offsets, uint32_t n_seqs, uint32_t max_len,
+                                const uint32_t* pair_a, const uint32_t* pair_b,
+                                const float* target, const float* weight, uint32_t n_pairs,
+                                float lr, float* loss_out, void* stream, st_status* status);
+/* Embeddings (n x 128 BF16, row-major) and their squared norms (n floats). */
+int32_t st_prefilter_encode(const st_prefilter* model, const uint8_t* codes,
+                            const uint64_t* offsets, uint32_t n, uint32_t max_len, void* emb,
+                            float* norm, void* stream, st_status* status);
+/* Every (query, reference) with |q|^2 + |r|^2 - 2 q.r <= tau (BF16 MFMA, FP32 accumulation),
+ * in no particular order; the first `capacity` go to hit_query / hit_ref and *hit_count (host)
+ * receives the total, which may exceed capacity (call again with more room). */
+int32_t st_prefilter_screen(const void* q_emb, const float* q_norm, uint32_t n_queries,
+                            const void* r_emb, const float* r_norm, uint32_t n_refs, float tau,
+                            uint64_t capacity, uint32_t* hit_query, uint32_t* hit_ref,
+                            uint64_t* hit_count, void* stream, st_status* status);
+
+/* Indices t < n with values[t] <= cutoff (device int32 values), in no particular order; the
+ * first `capacity` go to index (device) and *count (host) receives the total. Used to keep the
+ * exactly rescored prefilter candidates that are real hits. */
+int32_t st_select_le(const int32_t* values, uint64_t n, int32_t cutoff, uint64_t capacity,
+                     uint32_t* index, uint64_t* count, void* stream, st_status* status);
 
 /* ---- device memory (no counterpart in the reference) ------------------------------------
  * Minimal allocation and synchronous copies so a caller without its own GPU allocator can

@@ -11,6 +11,8 @@
 #include "../../include/seqtree_hip.h"
 
 #include <algorithm>
+#include <cmath>
+#include <random>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -762,6 +764,68 @@ int hamming_mismatch(st_status* st, uint64_t la, uint64_t lb) {
     return ST_INVALID_ARGUMENT;
 }
 
+
+// ------------------------------------------------------------------------------------------
+// learned prefilter (st_prefilter_*)
+// ------------------------------------------------------------------------------------------
+// Activation buffers: R rows (a multiple of the 128-row GEMM tile) plus kGuardRows zero rows
+// on either side, so the five-row convolution window of every row stays in bounds.
+constexpr uint32_t kGuardRows = 2;
+
+struct PfActs {
+    __bf16* base = nullptr;  // allocation start
+    __bf16* rows = nullptr;  // row 0
+};
+
+PfActs pf_acts(Arena& ar, uint64_t R, uint32_t C, hipStream_t s) {
+    PfActs a;
+    const uint64_t total = (R + 2 * kGuardRows) * C;
+    a.base = ar.get<__bf16>(total);
+    a.rows = a.base + uint64_t(kGuardRows) * C;
+    // zero the guard rows (bf16 zero is all-zero bits; two elements per 32-bit word)
+    launch_fill_u32(reinterpret_cast<uint32_t*>(a.base), uint64_t(kGuardRows) * C / 2, 0u, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(a.rows + R * C), uint64_t(kGuardRows) * C / 2, 0u,
+                    s);
+    return a;
+}
+
+uint64_t pf_rows(uint32_t n, uint32_t slot) { return (uint64_t(n) * slot + 127) / 128 * 128; }
+
+// Encoder forward over one block of sequences; H3 is left in `ha`.
+void pf_forward(const __bf16* B, const float* P, const uint8_t* codes, const uint64_t* off,
+                uint32_t n, uint32_t slot, uint64_t R, const PfActs& x0, const PfActs& h1,
+                const PfActs& h2, const PfActs& h3, uint8_t* valid, hipStream_t s) {
+    LaunchTimer t("prefilter_encode", s);
+    pf::launch_embed(codes, off, n, slot, R, B + pf::kOffEmbB, x0.rows, valid, s);
+    pf::launch_conv(1, 0, x0.rows, B + pf::kOffWt1, P + pf::kOffB1, nullptr, valid, h1.rows, R, s);
+    pf::launch_conv(2, 0, h1.rows, B + pf::kOffWt2, P + pf::kOffB2, nullptr, valid, h2.rows, R, s);
+    pf::launch_conv(3, 0, h2.rows, B + pf::kOffWt3, P + pf::kOffB3, nullptr, valid, h3.rows, R, s);
+    launched("prefilter forward");
+}
+
+struct PrefilterImpl {
+    float* P = nullptr;   // FP32 master parameters
+    float* G = nullptr;   // gradients
+    float* M = nullptr;   // Adam moments
+    float* V = nullptr;
+    __bf16* B = nullptr;  // BF16 operand copies
+    uint64_t step = 0;
+
+    ~PrefilterImpl() {
+        for (void* p : {static_cast<void*>(P), static_cast<void*>(G), static_cast<void*>(M),
+                        static_cast<void*>(V), static_cast<void*>(B)})
+            if (p) (void)hipFree(p);
+    }
+};
+
+void pf_check_set(uint32_t n, uint32_t max_len) {
+    if (max_len > pf::kMaxLen)
+        throw HipError{ST_INVALID_ARGUMENT, "seqtree_hip: the prefilter encoder takes sequences "
+                                            "of at most " + std::to_string(pf::kMaxLen) +
+                                                " residues (got " + std::to_string(max_len) + ")"};
+    if (n == 0) throw HipError{ST_INVALID_ARGUMENT, "seqtree_hip: empty sequence set"};
+}
+
 }  // namespace
 
 extern "C" {
@@ -838,6 +902,222 @@ int32_t st_probe_valu(int32_t kind, char* name, uint64_t name_capacity, double* 
         (void)hipEventDestroy(b);
         check(hipFree(out), "hipFree");
         *lane_ops_per_s = double(blocks) * 256.0 * 16.0 * iters / (double(best) * 1e-3);
+        return ok(status);
+    });
+}
+
+
+struct st_prefilter {
+    PrefilterImpl impl;
+};
+
+uint64_t st_prefilter_param_count(void) { return pf::kParams; }
+
+int32_t st_prefilter_create(uint64_t seed, st_prefilter** model, st_status* status) {
+    *model = nullptr;
+    return guarded(status, [&] {
+        require_device();
+        auto M = std::make_unique<st_prefilter>();
+        PrefilterImpl& m = M->impl;
+        for (float** p : {&m.P, &m.G, &m.M, &m.V})
+            check(hipMalloc(reinterpret_cast<void**>(p), pf::kParams * sizeof(float)), "hipMalloc");
+        check(hipMalloc(reinterpret_cast<void**>(&m.B), pf::kBf16Params * sizeof(__bf16)),
+              "hipMalloc");
+        // He-normal convolution weights, N(0, 1/2) embedding, zero biases
+        std::vector<float> h(pf::kParams, 0.f);
+        std::mt19937_64 rng(seed);
+        std::normal_distribution<float> nd(0.f, 1.f);
+        for (uint32_t i = 0; i < pf::kEmbRows * pf::kEmbDim; ++i) h[pf::kOffEmb + i] = 0.5f * nd(rng);
+        const uint32_t offs[3] = {pf::kOffW1, pf::kOffW2, pf::kOffW3};
+        const uint32_t cin[3] = {pf::kEmbDim, pf::kChannels, pf::kChannels};
+        for (int l = 0; l < 3; ++l) {
+            const float sd = std::sqrt(2.f / float(5 * cin[l]));
+            for (uint32_t i = 0; i < 5 * cin[l] * pf::kChannels; ++i) h[offs[l] + i] = sd * nd(rng);
+        }
+        check(hipMemcpy(m.P, h.data(), pf::kParams * sizeof(float), hipMemcpyHostToDevice), "H2D");
+        check(hipMemset(m.M, 0, pf::kParams * sizeof(float)), "hipMemset");
+        check(hipMemset(m.V, 0, pf::kParams * sizeof(float)), "hipMemset");
+        pf::launch_pack_weights(m.P, m.B, nullptr);
+        check(hipDeviceSynchronize(), "prefilter init");
+        *model = M.release();
+        return ok(status);
+    });
+}
+
+void st_prefilter_destroy(st_prefilter* model) {
+    if (model) (void)hipDeviceSynchronize();
+    delete model;
+}
+
+int32_t st_prefilter_get_params(const st_prefilter* model, float* params, uint64_t count,
+                                st_status* status) {
+    return guarded(status, [&] {
+        if (!model || count != pf::kParams)
+            throw HipError{ST_INVALID_ARGUMENT, "seqtree_hip: parameter count mismatch"};
+        check(hipDeviceSynchronize(), "sync");
+        check(hipMemcpy(params, model->impl.P, count * sizeof(float), hipMemcpyDeviceToHost), "D2H");
+        return ok(status);
+    });
+}
+
+int32_t st_prefilter_set_params(st_prefilter* model, const float* params, uint64_t count,
+                                st_status* status) {
+    return guarded(status, [&] {
+        if (!model || count != pf::kParams)
+            throw HipError{ST_INVALID_ARGUMENT, "seqtree_hip: parameter count mismatch"};
+        check(hipDeviceSynchronize(), "sync");
+        check(hipMemcpy(model->impl.P, params, count * sizeof(float), hipMemcpyHostToDevice), "H2D");
+        pf::launch_pack_weights(model->impl.P, model->impl.B, nullptr);
+        check(hipDeviceSynchronize(), "prefilter pack");
+        return ok(status);
+    });
+}
+
+int32_t st_prefilter_get_grads(const st_prefilter* model, float* grads, uint64_t count,
+                               st_status* status) {
+    return guarded(status, [&] {
+        if (!model || count != pf::kParams)
+            throw HipError{ST_INVALID_ARGUMENT, "seqtree_hip: parameter count mismatch"};
+        check(hipDeviceSynchronize(), "sync");
+        check(hipMemcpy(grads, model->impl.G, count * sizeof(float), hipMemcpyDeviceToHost), "D2H");
+        return ok(status);
+    });
+}
+
+int32_t st_select_le(const int32_t* values, uint64_t n, int32_t cutoff, uint64_t capacity,
+                     uint32_t* index, uint64_t* count, void* stream, st_status* status) {
+    return guarded(status, [&] {
+        require_device();
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        Arena ar(s);
+        unsigned long long* c = ar.get<unsigned long long>(1);
+        launch_fill_u32(reinterpret_cast<uint32_t*>(c), 2, 0u, s);
+        {
+            LaunchTimer t("select_le", s);
+            pf::launch_select_le(values, n, cutoff, index, capacity, c, s);
+        }
+        launched("select_le");
+        unsigned long long h = 0;
+        d2h(&h, c, 1, s);
+        sync(s);
+        *count = h;
+        return ok(status);
+    });
+}
+
+int32_t st_prefilter_train_step(st_prefilter* model, const uint8_t* codes,
+                                const uint64_t* offsets, uint32_t n_seqs, uint32_t max_len,
+                                const uint32_t* pair_a, const uint32_t* pair_b,
+                                const float* target, const float* weight, uint32_t n_pairs,
+                                float lr, float* loss_out, void* stream, st_status* status) {
+    return guarded(status, [&] {
+        if (!model) throw HipError{ST_INVALID_ARGUMENT, "seqtree_hip: null prefilter model"};
+        pf_check_set(n_seqs, max_len);
+        require_device();
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        PrefilterImpl& m = model->impl;
+        const uint32_t slot = max_len + 2 * kGuardRows;
+        const uint64_t R = pf_rows(n_seqs, slot);
+        Arena ar(s);
+        const PfActs x0 = pf_acts(ar, R, pf::kEmbDim, s), h1 = pf_acts(ar, R, pf::kChannels, s),
+                     h2 = pf_acts(ar, R, pf::kChannels, s), h3 = pf_acts(ar, R, pf::kChannels, s),
+                     ga = pf_acts(ar, R, pf::kChannels, s), gb = pf_acts(ar, R, pf::kChannels, s),
+                     dx0 = pf_acts(ar, R, pf::kEmbDim, s);
+        uint8_t* valid = ar.get<uint8_t>(R);
+        float* ef = ar.get<float>(uint64_t(n_seqs) * pf::kChannels);
+        float* dE = ar.get<float>(uint64_t(n_seqs) * pf::kChannels);
+        float* loss = ar.get<float>(1);
+        launch_fill_u32(reinterpret_cast<uint32_t*>(dE), uint64_t(n_seqs) * pf::kChannels, 0u, s);
+        launch_fill_u32(reinterpret_cast<uint32_t*>(loss), 1, 0u, s);
+        launch_fill_u32(reinterpret_cast<uint32_t*>(m.G), pf::kParams, 0u, s);
+
+        pf_forward(m.B, m.P, codes, offsets, n_seqs, slot, R, x0, h1, h2, h3, valid, s);
+        {
+            LaunchTimer t("prefilter_train", s);
+            pf::launch_pool(h3.rows, offsets, n_seqs, slot, ef, nullptr, nullptr, s);
+            pf::launch_pair_loss(ef, pair_a, pair_b, target, weight, n_pairs, dE, loss, s);
+            // backward: layer 3, 2, 1, embedding
+            pf::launch_pool_bwd(h3.rows, dE, offsets, n_seqs, slot, R, ga.rows, s);
+            pf::launch_dw(3, h2.rows, ga.rows, R, m.G + pf::kOffW3, m.G + pf::kOffB3, s);
+            pf::launch_conv(3, 1, ga.rows, m.B + pf::kOffWf3, nullptr, h2.rows, valid, gb.rows, R,
+                            s);
+            pf::launch_dw(2, h1.rows, gb.rows, R, m.G + pf::kOffW2, m.G + pf::kOffB2, s);
+            pf::launch_conv(2, 1, gb.rows, m.B + pf::kOffWf2, nullptr, h1.rows, valid, ga.rows, R,
+                            s);
+            pf::launch_dw(1, x0.rows, ga.rows, R, m.G + pf::kOffW1, m.G + pf::kOffB1, s);
+            pf::launch_conv(1, 2, ga.rows, m.B + pf::kOffWf1, nullptr, nullptr, valid, dx0.rows, R,
+                            s);
+            pf::launch_embed_bwd(dx0.rows, codes, offsets, n_seqs, slot, m.G + pf::kOffEmb, s);
+            m.step += 1;
+            const float b1 = 0.9f, b2 = 0.999f;
+            const float c1 = 1.f - std::pow(b1, float(m.step)), c2 = 1.f - std::pow(b2, float(m.step));
+            pf::launch_adam(m.P, m.G, m.M, m.V, pf::kParams, lr, b1, b2, c1, c2, 1e-8f, s);
+            pf::launch_pack_weights(m.P, m.B, s);
+        }
+        launched("prefilter train");
+        float h = 0.f;
+        d2h(&h, loss, 1, s);
+        sync(s);
+        if (loss_out) *loss_out = n_pairs ? h / float(n_pairs) : 0.f;
+        return ok(status);
+    });
+}
+
+int32_t st_prefilter_encode(const st_prefilter* model, const uint8_t* codes,
+                            const uint64_t* offsets, uint32_t n, uint32_t max_len, void* emb,
+                            float* norm, void* stream, st_status* status) {
+    return guarded(status, [&] {
+        if (!model) throw HipError{ST_INVALID_ARGUMENT, "seqtree_hip: null prefilter model"};
+        pf_check_set(n, max_len);
+        require_device();
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        const PrefilterImpl& m = model->impl;
+        const uint32_t slot = max_len + 2 * kGuardRows;
+        // blocks of sequences sized to ~256 MB per 128-channel activation buffer
+        const uint32_t blk = uint32_t(std::min<uint64_t>(n, std::max<uint64_t>(1024, (uint64_t(1) << 20) / slot)));
+        const uint64_t Rb = pf_rows(blk, slot);
+        Arena ar(s);
+        const PfActs x0 = pf_acts(ar, Rb, pf::kEmbDim, s), ha = pf_acts(ar, Rb, pf::kChannels, s),
+                     hb = pf_acts(ar, Rb, pf::kChannels, s);
+        uint8_t* valid = ar.get<uint8_t>(Rb);
+        __bf16* out = static_cast<__bf16*>(emb);
+        for (uint32_t s0 = 0; s0 < n; s0 += blk) {
+            const uint32_t cnt = std::min(blk, n - s0);
+            const uint64_t R = pf_rows(cnt, slot);
+            pf_forward(m.B, m.P, codes, offsets + s0, cnt, slot, R, x0, ha, hb, ha, valid, s);
+            LaunchTimer t("prefilter_pool", s);
+            pf::launch_pool(ha.rows, offsets + s0, cnt, slot, nullptr,
+                            out + uint64_t(s0) * pf::kChannels, norm + s0, s);
+        }
+        launched("prefilter encode");
+        return ok(status);
+    });
+}
+
+int32_t st_prefilter_screen(const void* q_emb, const float* q_norm, uint32_t n_queries,
+                            const void* r_emb, const float* r_norm, uint32_t n_refs, float tau,
+                            uint64_t capacity, uint32_t* hit_query, uint32_t* hit_ref,
+                            uint64_t* hit_count, void* stream, st_status* status) {
+    return guarded(status, [&] {
+        require_device();
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        int dev = 0, cus = 256;
+        check(hipGetDevice(&dev), "hipGetDevice");
+        check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "attr");
+        Arena ar(s);
+        unsigned long long* count = ar.get<unsigned long long>(1);
+        launch_fill_u32(reinterpret_cast<uint32_t*>(count), 2, 0u, s);
+        {
+            LaunchTimer t("prefilter_screen", s);
+            pf::launch_gram(static_cast<const __bf16*>(q_emb), q_norm, n_queries,
+                            static_cast<const __bf16*>(r_emb), r_norm, n_refs, tau, hit_query,
+                            hit_ref, capacity, count, uint32_t(cus), s);
+        }
+        launched("prefilter screen");
+        unsigned long long h = 0;
+        d2h(&h, count, 1, s);
+        sync(s);
+        *hit_count = h;
         return ok(status);
     });
 }

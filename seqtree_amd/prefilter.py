@@ -1,0 +1,404 @@
+"""Learned prefilter for thresholded gap-block search on the GPU.
+
+A small 1D-conv encoder (embedding 32 -> three width-5 convolutions of 128 channels with
+ReLU -> mean pool -> 128-d) is trained so that the squared embedding distance
+``|e(a) - e(b)|^2 = s(a,a) + s(b,b) - 2 s(a,b)`` tracks the exact gap-block score of (a, b),
+on pairs whose labels come from the exact gap-block kernel itself. A search then
+
+1. embeds the queries and the references (BF16 MFMA kernels, FP32 accumulation),
+2. screens every (query, reference) pair with one BF16 MFMA Gram GEMM whose epilogue keeps
+   the pairs with estimated distance <= tau (the cutoff plus a calibrated margin),
+3. rescores the survivors with the exact gap-block pair kernel and keeps those <= cutoff.
+
+Every reported score is exact; what the screen can cost is recall, which the exhaustive exact
+pass (``st_gapblock_threshold_device``) measures. The reference has no prefilter: its
+``GapBlockIndex.search`` (gapblock.py:567-673) answers the same question with a
+deletion-variant trie.
+
+All kernels and the model live in ``libseqtree_hip.so`` (``st_prefilter_*`` in
+``include/seqtree_hip.h``); this module holds the data pipeline, the training loop and the
+margin calibration.
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _device
+from ._device import DeviceArray, Status, check
+from .gapblock import _prior_cube, central_prior
+from .synthetic import AA_ORDER
+
+EMB = 128
+# Distances are learned in units of this many gap-block score points.
+SCORE_UNIT = 100.0
+
+
+def _bind():
+    L = _device.lib()
+    if getattr(L, "_pf_bound", False):
+        return L
+    vp, u32, u64, i32, f32 = (ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32,
+                              ctypes.c_float)
+    S = ctypes.POINTER(Status)
+    L.st_prefilter_param_count.restype = u64
+    L.st_prefilter_create.restype = i32
+    L.st_prefilter_create.argtypes = [u64, ctypes.POINTER(vp), S]
+    L.st_prefilter_destroy.argtypes = [vp]
+    for name in ("st_prefilter_get_params", "st_prefilter_set_params", "st_prefilter_get_grads"):
+        fn = getattr(L, name)
+        fn.restype = i32
+        fn.argtypes = [vp, vp, u64, S]
+    L.st_prefilter_train_step.restype = i32
+    L.st_prefilter_train_step.argtypes = [vp, vp, vp, u32, u32, vp, vp, vp, vp, u32, f32,
+                                          ctypes.POINTER(f32), vp, S]
+    L.st_prefilter_encode.restype = i32
+    L.st_prefilter_encode.argtypes = [vp, vp, vp, u32, u32, vp, vp, vp, S]
+    L.st_prefilter_screen.restype = i32
+    L.st_prefilter_screen.argtypes = [vp, vp, u32, vp, vp, u32, f32, u64, vp, vp,
+                                      ctypes.POINTER(u64), vp, S]
+    L.st_select_le.restype = i32
+    L.st_select_le.argtypes = [vp, u64, i32, u64, vp, ctypes.POINTER(u64), vp, S]
+    L.st_gapblock_pairs_device.restype = i32
+    L.st_gapblock_pairs_device.argtypes = [vp, vp, u64, u32, vp, vp, u64, u32, vp, u32, i32, i32,
+                                           vp, u32, vp, vp, u64, vp, vp, S]
+    L.st_gapblock_threshold_device.restype = i32
+    L.st_gapblock_threshold_device.argtypes = [vp, vp, u64, u32, vp, vp, u64, u32, vp, u32, i32,
+                                               i32, vp, u32, i32, u64, vp, vp, vp, vp, vp, S]
+    L._pf_bound = True
+    return L
+
+
+@dataclass
+class DeviceSet:
+    """A CSR set of codec codes resident on the GPU."""
+
+    codes: DeviceArray
+    offsets: DeviceArray
+    n: int
+    max_len: int
+
+    @classmethod
+    def upload(cls, codes: np.ndarray, offsets: np.ndarray) -> "DeviceSet":
+        lens = np.diff(offsets.astype(np.int64))
+        c = codes if len(codes) else np.zeros(1, np.uint8)
+        return cls(DeviceArray.from_numpy(np.ascontiguousarray(c, np.uint8)),
+                   DeviceArray.from_numpy(np.ascontiguousarray(offsets, np.uint64)),
+                   len(offsets) - 1, int(lens.max(initial=0)))
+
+
+@dataclass
+class GapblockParams:
+    """Exact gap-block scoring parameters (pen grid in codec order, gaps, prior cube)."""
+
+    pen: np.ndarray
+    gap_open: int
+    gap_extend: int
+    cube: np.ndarray | None
+    width: int
+
+    @classmethod
+    def blosum62_central(cls, width: int) -> "GapblockParams":
+        """BLOSUM62, gap_open = 2 * scale = 28, gap_extend 1, central prior 1.5 * scale = 21."""
+        from . import _core
+
+        m = _core.SubstitutionMatrix.blosum62()
+        pen = np.array([[m.penalty(a, b) for b in AA_ORDER] for a in AA_ORDER], np.int32)
+        lam = int(1.5 * m.scale())
+        cube = np.array(_prior_cube(central_prior(lam), width), np.int32)
+        return cls(pen, 2 * m.scale(), 1, cube, width)
+
+
+def exact_pairs(q: DeviceSet, r: DeviceSet, pa: DeviceArray, pb: DeviceArray, n_pairs: int,
+                gp: GapblockParams, out: DeviceArray | None = None) -> DeviceArray:
+    """Exact gap-block scores of pairs (q[pa[t]], r[pb[t]]) on the GPU (device int32)."""
+    L = _bind()
+    out = out if out is not None else DeviceArray(max(n_pairs, 1), np.int32)
+    st = Status()
+    rc = L.st_gapblock_pairs_device(
+        q.codes.ptr, q.offsets.ptr, q.n, q.max_len, r.codes.ptr, r.offsets.ptr, r.n, r.max_len,
+        gp.pen.ctypes.data, len(AA_ORDER), gp.gap_open, gp.gap_extend,
+        gp.cube.ctypes.data if gp.cube is not None else None, gp.width, pa.ptr, pb.ptr, n_pairs,
+        out.ptr, None, ctypes.byref(st))
+    check(rc, st, "st_gapblock_pairs_device")
+    return out
+
+
+def exact_threshold(q: DeviceSet, r: DeviceSet, cutoff: int, gp: GapblockParams,
+                    capacity: int = 1 << 22):
+    """Exhaustive exact search: every pair with score <= cutoff, as host arrays
+    (query, ref, score). Retries with more room when the first capacity is short."""
+    L = _bind()
+    count = DeviceArray(1, np.uint64)  # the device entry point counts in device memory
+    while True:
+        hq, hr = DeviceArray(capacity, np.uint32), DeviceArray(capacity, np.uint32)
+        hs = DeviceArray(capacity, np.int32)
+        st = Status()
+        rc = L.st_gapblock_threshold_device(
+            q.codes.ptr, q.offsets.ptr, q.n, q.max_len, r.codes.ptr, r.offsets.ptr, r.n,
+            r.max_len, gp.pen.ctypes.data, len(AA_ORDER), gp.gap_open, gp.gap_extend,
+            gp.cube.ctypes.data if gp.cube is not None else None, gp.width, cutoff, capacity,
+            hq.ptr, hr.ptr, hs.ptr, count.ptr, None, ctypes.byref(st))
+        check(rc, st, "st_gapblock_threshold_device")
+        k = int(count.numpy()[0])
+        if k <= capacity:
+            return hq.numpy()[:k], hr.numpy()[:k], hs.numpy()[:k]
+        capacity = k
+
+
+class Prefilter:
+    """The encoder model on the GPU (FP32 master parameters, BF16 operand copies)."""
+
+    def __init__(self, seed: int = 0):
+        self._L = _bind()
+        self._h = ctypes.c_void_p()
+        st = Status()
+        check(self._L.st_prefilter_create(seed, ctypes.byref(self._h), ctypes.byref(st)), st,
+              "st_prefilter_create")
+        self.n_params = int(self._L.st_prefilter_param_count())
+
+    def close(self):
+        if self._h:
+            self._L.st_prefilter_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 -- interpreter shutdown
+            pass
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.n_params, np.float32)
+        st = Status()
+        check(self._L.st_prefilter_get_params(self._h, out.ctypes.data, out.size,
+                                              ctypes.byref(st)), st, "get_params")
+        return out
+
+    def grads(self) -> np.ndarray:
+        out = np.empty(self.n_params, np.float32)
+        st = Status()
+        check(self._L.st_prefilter_get_grads(self._h, out.ctypes.data, out.size,
+                                             ctypes.byref(st)), st, "get_grads")
+        return out
+
+    def load(self, params: np.ndarray) -> None:
+        p = np.ascontiguousarray(params, np.float32)
+        st = Status()
+        check(self._L.st_prefilter_set_params(self._h, p.ctypes.data, p.size, ctypes.byref(st)),
+              st, "set_params")
+
+    def train_step(self, seqs: DeviceSet, pa: DeviceArray, pb: DeviceArray, y: DeviceArray,
+                   w: DeviceArray, n_pairs: int, lr: float, max_len: int | None = None) -> float:
+        loss = ctypes.c_float()
+        st = Status()
+        rc = self._L.st_prefilter_train_step(
+            self._h, seqs.codes.ptr, seqs.offsets.ptr, seqs.n, max_len or seqs.max_len, pa.ptr,
+            pb.ptr, y.ptr, w.ptr, n_pairs, lr, ctypes.byref(loss), None, ctypes.byref(st))
+        check(rc, st, "st_prefilter_train_step")
+        return float(loss.value)
+
+    def encode(self, seqs: DeviceSet, max_len: int | None = None):
+        """(embeddings n x 128 BF16 as a uint16 device array, squared norms float32)."""
+        emb = DeviceArray((seqs.n, EMB), np.uint16)
+        norm = DeviceArray(seqs.n, np.float32)
+        st = Status()
+        rc = self._L.st_prefilter_encode(self._h, seqs.codes.ptr, seqs.offsets.ptr, seqs.n,
+                                         max_len or seqs.max_len, emb.ptr, norm.ptr, None,
+                                         ctypes.byref(st))
+        check(rc, st, "st_prefilter_encode")
+        return emb, norm
+
+
+def screen(q_emb, q_norm, nq: int, r_emb, r_norm, nr: int, tau: float, capacity: int):
+    """Device (query, ref) index arrays of the pairs with estimated distance <= tau, and
+    their count (grows the buffers and repeats when the capacity is short)."""
+    L = _bind()
+    while True:
+        hq, hr = DeviceArray(capacity, np.uint32), DeviceArray(capacity, np.uint32)
+        count = ctypes.c_uint64()
+        st = Status()
+        rc = L.st_prefilter_screen(q_emb.ptr, q_norm.ptr, nq, r_emb.ptr, r_norm.ptr, nr, tau,
+                                   capacity, hq.ptr, hr.ptr, ctypes.byref(count), None,
+                                   ctypes.byref(st))
+        check(rc, st, "st_prefilter_screen")
+        if count.value <= capacity:
+            return hq, hr, int(count.value)
+        capacity = int(count.value)
+
+
+def select_le(values: DeviceArray, n: int, cutoff: int) -> np.ndarray:
+    """Host indices t < n with values[t] <= cutoff (sorted)."""
+    L = _bind()
+    cap = 1 << 16
+    while True:
+        idx = DeviceArray(cap, np.uint32)
+        count = ctypes.c_uint64()
+        st = Status()
+        check(L.st_select_le(values.ptr, n, cutoff, cap, idx.ptr, ctypes.byref(count), None,
+                             ctypes.byref(st)), st, "st_select_le")
+        if count.value <= cap:
+            return np.sort(idx.numpy()[:count.value])
+        cap = int(count.value)
+
+
+# ------------------------------------------------------------------------------ training
+@dataclass
+class TrainingData:
+    seqs_codes: np.ndarray
+    seqs_off: np.ndarray
+    pair_a: np.ndarray
+    pair_b: np.ndarray
+    score: np.ndarray  # exact gap-block scores
+
+
+def label_pairs(codes: np.ndarray, off: np.ndarray, pa: np.ndarray, pb: np.ndarray,
+                gp: GapblockParams) -> np.ndarray:
+    """Exact gap-block scores of (s[pa[t]], s[pb[t]]) within one set, on the GPU."""
+    ds = DeviceSet.upload(codes, off)
+    da = DeviceArray.from_numpy(pa.astype(np.uint32))
+    db = DeviceArray.from_numpy(pb.astype(np.uint32))
+    return exact_pairs(ds, ds, da, db, len(pa), gp).numpy()[:len(pa)]
+
+
+def sample_pairs(rng: np.random.Generator, fam: np.ndarray, n_pairs: int, near_frac: float):
+    """near_frac of the pairs inside a family (distinct members), the rest uniform."""
+    n = len(fam)
+    n_near = int(n_pairs * near_frac)
+    starts = np.searchsorted(fam, fam)  # first member of each sequence's family
+    sizes = np.bincount(fam)[fam]
+    a = rng.integers(0, n, size=n_near)
+    off = rng.integers(1, np.maximum(sizes[a], 2), size=n_near)
+    b = starts[a] + (a - starts[a] + off) % np.maximum(sizes[a], 1)
+    ra = rng.integers(0, n, size=n_pairs - n_near)
+    rb = rng.integers(0, n, size=n_pairs - n_near)
+    return np.concatenate([a, ra]).astype(np.int64), np.concatenate([b, rb]).astype(np.int64)
+
+
+def pair_weights(score: np.ndarray) -> np.ndarray:
+    """Relative squared error: (d - y)^2 / (y + 0.1)^2 in distance units, so close pairs --
+    the ones a cutoff search keeps -- dominate the fit."""
+    y = score / SCORE_UNIT
+    return (1.0 / (y + 0.1) ** 2).astype(np.float32)
+
+
+def train(model: Prefilter, data: TrainingData, epochs: int, batch_pairs: int = 4096,
+          lr: float = 2e-3, seed: int = 0, max_len: int | None = None, log=None) -> list[float]:
+    """Adam over the labelled pairs; each step embeds the distinct sequences of its batch."""
+    rng = np.random.default_rng(seed)
+    lens = np.diff(data.seqs_off.astype(np.int64))
+    max_len = max_len or int(lens.max())
+    y_all = (data.score / SCORE_UNIT).astype(np.float32)
+    w_all = pair_weights(data.score)
+    n = len(data.pair_a)
+    steps = (n + batch_pairs - 1) // batch_pairs
+    total = epochs * steps
+    # device staging buffers sized for the largest batch
+    cap_seqs = 2 * batch_pairs
+    d_codes = DeviceArray(cap_seqs * max_len, np.uint8)
+    d_off = DeviceArray(cap_seqs + 1, np.uint64)
+    d_pa, d_pb = DeviceArray(batch_pairs, np.uint32), DeviceArray(batch_pairs, np.uint32)
+    d_y, d_w = DeviceArray(batch_pairs, np.float32), DeviceArray(batch_pairs, np.float32)
+    losses = []
+    k = 0
+    for ep in range(epochs):
+        order = rng.permutation(n)
+        ep_loss = 0.0
+        for s in range(steps):
+            sel = order[s * batch_pairs:(s + 1) * batch_pairs]
+            a, b = data.pair_a[sel], data.pair_b[sel]
+            uniq, inv = np.unique(np.concatenate([a, b]), return_inverse=True)
+            ia, ib = inv[:len(sel)], inv[len(sel):]
+            sl = [data.seqs_codes[int(data.seqs_off[u]):int(data.seqs_off[u + 1])] for u in uniq]
+            off = np.zeros(len(uniq) + 1, np.uint64)
+            off[1:] = np.cumsum([len(x) for x in sl])
+            codes = np.concatenate(sl).astype(np.uint8)
+            _upload(d_codes, codes)
+            _upload(d_off, off)
+            for dst, src in ((d_pa, ia.astype(np.uint32)), (d_pb, ib.astype(np.uint32)),
+                             (d_y, y_all[sel]), (d_w, w_all[sel])):
+                _upload(dst, src)
+            batch = DeviceSet(d_codes, d_off, len(uniq), max_len)
+            frac = k / max(total - 1, 1)
+            step_lr = lr * (0.05 + 0.95 * 0.5 * (1 + np.cos(np.pi * frac)))  # cosine decay
+            ep_loss += model.train_step(batch, d_pa, d_pb, d_y, d_w, len(sel), step_lr, max_len)
+            k += 1
+        losses.append(ep_loss / steps)
+        if log:
+            log(f"epoch {ep + 1}/{epochs}: mean weighted loss {losses[-1]:.4f}")
+    return losses
+
+
+def _upload(dst: DeviceArray, src: np.ndarray) -> None:
+    src = np.ascontiguousarray(src)
+    assert src.nbytes <= dst.nbytes
+    st = Status()
+    check(_device.lib().st_memcpy(dst.ptr, src.ctypes.data, src.nbytes, 1, ctypes.byref(st)), st,
+          "st_memcpy")
+
+
+def embed_distances(model: Prefilter, codes, off, pa, pb, max_len: int) -> np.ndarray:
+    """Model distances (BF16 embeddings, FP32 arithmetic) of pairs within one set, in score
+    units."""
+    ds = DeviceSet.upload(codes, off)
+    emb, norm = model.encode(ds, max_len)
+    e = _bf16_to_f32(emb.numpy())
+    d = ((e[pa] - e[pb]) ** 2).sum(axis=1)
+    return d * SCORE_UNIT
+
+
+def _bf16_to_f32(u16: np.ndarray) -> np.ndarray:
+    return (u16.astype(np.uint32) << 16).view(np.float32)
+
+
+def calibrate_tau(model_dist: np.ndarray, exact: np.ndarray, cutoff: int,
+                  quantile: float = 0.999) -> float:
+    """Screening threshold (score units) that keeps `quantile` of the held-out pairs whose
+    exact score is <= cutoff: the cutoff itself or the quantile of their model distance,
+    whichever is larger."""
+    close = model_dist[exact <= cutoff]
+    if len(close) == 0:
+        return float(cutoff)
+    return float(max(cutoff, np.quantile(close, quantile)))
+
+
+@dataclass
+class SearchResult:
+    query: np.ndarray
+    ref: np.ndarray
+    score: np.ndarray
+    candidates: int
+    seconds: dict
+
+
+def search(model: Prefilter, q: DeviceSet, r: DeviceSet, cutoff: int, tau_score: float,
+           gp: GapblockParams, max_len: int, capacity: int = 1 << 26) -> SearchResult:
+    """Embed, screen at tau (score units), rescore the candidates exactly, keep <= cutoff."""
+    t = {}
+    t0 = time.perf_counter()
+    qe, qn = model.encode(q, max_len)
+    re_, rn = model.encode(r, max_len)
+    _device.synchronize()
+    t["encode"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    cq, cr, nc = screen(qe, qn, q.n, re_, rn, r.n, tau_score / SCORE_UNIT, capacity)
+    t["screen"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    scores = exact_pairs(q, r, cq, cr, nc, gp)
+    keep = select_le(scores, nc, cutoff)
+    t["rescore"] = time.perf_counter() - t0
+    hq = cq.numpy()[:nc][keep] if nc else np.zeros(0, np.uint32)
+    hr = cr.numpy()[:nc][keep] if nc else np.zeros(0, np.uint32)
+    hs = scores.numpy()[:nc][keep] if nc else np.zeros(0, np.int32)
+    return SearchResult(hq, hr, hs, nc, t)
+
+
+def recall(found_q, found_r, exact_q, exact_r) -> float:
+    """Fraction of the exact hits (exact_q, exact_r) that the prefilter found."""
+    if len(exact_q) == 0:
+        return 1.0
+    key = lambda a, b: (a.astype(np.int64) << 32) | b.astype(np.int64)  # noqa: E731
+    return float(np.isin(key(exact_q, exact_r), key(found_q, found_r)).mean())

@@ -1,0 +1,236 @@
+"""GPU checks of the learned prefilter's kernels (no counterpart in the reference).
+
+The screen is checked exactly against numpy on BF16-representable data; the encoder forward
+and the training gradients against a float64 numpy model of the same network (with the
+GPU's BF16 rounding points on the forward activations); the end-to-end search against the
+exhaustive exact pass (every reported pair must be an exact hit with its exact score)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+C, E = 128, 32
+
+
+def bf16(x):
+    """Round float32 values to bfloat16 (nearest even), returned as float32."""
+    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return u.astype(np.uint32).view(np.float32)
+
+
+def to_bf16_bits(x):
+    return (bf16(x).view(np.uint32) >> 16).astype(np.uint16)
+
+
+def unpack(p):
+    """Parameter vector -> (emb [32][32], [(W [5 cin][128], b [128])] * 3)."""
+    emb = p[:32 * E].reshape(32, E)
+    o = 32 * E
+    layers = []
+    for cin in (E, C, C):
+        W = p[o:o + 5 * cin * C].reshape(5 * cin, C)
+        o += 5 * cin * C
+        b = p[o:o + C]
+        o += C
+        layers.append((W, b))
+    assert o == len(p)
+    return emb, layers
+
+
+def conv(X, W, b):
+    """X [L][cin] -> [L][128], zero padding 2 each side, W rows k * cin + ci."""
+    L, cin = X.shape
+    Xp = np.vstack([np.zeros((2, cin)), X, np.zeros((2, cin))])
+    cols = np.stack([Xp[p:p + 5].reshape(-1) for p in range(L)]) if L else np.zeros((0, 5 * cin))
+    return cols @ W + b, cols
+
+
+def forward(params, seqs, round_acts=True):
+    emb, layers = unpack(params.astype(np.float64))
+    r = bf16 if round_acts else (lambda v: v)
+    embr = r(emb)
+    out, caches = [], []
+    for s in seqs:
+        X = embr[s]
+        cache = [X]
+        for W, b in layers:
+            Y, _ = conv(X, r(W), b)
+            X = r(np.maximum(Y, 0.0))
+            cache.append(X)
+        out.append(X.mean(axis=0) if len(s) else np.zeros(C))
+        caches.append(cache)
+    return np.array(out), caches
+
+
+def backward(params, seqs, pa, pb, y, w):
+    """float64 gradient of sum w (|e_a - e_b|^2 - y)^2 through the same network."""
+    emb, layers = unpack(params.astype(np.float64))
+    e, caches = forward(params, seqs)
+    g_emb = np.zeros_like(emb)
+    g_layers = [(np.zeros_like(W), np.zeros_like(b)) for W, b in layers]
+    dE = np.zeros_like(e)
+    for a, bb, yt, wt in zip(pa, pb, y, w):
+        diff = e[a] - e[bb]
+        d = (diff ** 2).sum()
+        g = 4 * wt * (d - yt) * diff
+        dE[a] += g
+        dE[bb] -= g
+    for s, cache, de in zip(seqs, caches, dE):
+        L = len(s)
+        dH = np.tile(de / L, (L, 1))
+        for li in (2, 1, 0):
+            W, b = layers[li]
+            H = cache[li + 1]
+            dY = dH * (H > 0)
+            _, cols = conv(cache[li], bf16(W), b)
+            g_layers[li][0][...] += cols.T @ dY
+            g_layers[li][1][...] += dY.sum(axis=0)
+            cin = W.shape[0] // 5
+            dcols = dY @ bf16(W).T  # [L][5 cin]
+            dX = np.zeros((L + 4, cin))
+            for p in range(L):
+                dX[p:p + 5] += dcols[p].reshape(5, cin)
+            dH = dX[2:L + 2]
+        np.add.at(g_emb, s, dH)
+    flat = [g_emb.reshape(-1)]
+    for gW, gb in g_layers:
+        flat += [gW.reshape(-1), gb]
+    return np.concatenate(flat)
+
+
+def rand_params(rng, n):
+    p = np.zeros(n, np.float32)
+    emb_n = 32 * E
+    p[:emb_n] = rng.normal(0, 0.5, emb_n)
+    o = emb_n
+    for cin in (E, C, C):
+        k = 5 * cin * C
+        p[o:o + k] = rng.normal(0, np.sqrt(2.0 / (5 * cin)), k)
+        o += k
+        p[o:o + C] = rng.normal(0, 0.05, C)
+        o += C
+    return p
+
+
+def rand_set(rng, n, lo, hi, A=20):
+    lens = rng.integers(lo, hi + 1, size=n)
+    seqs = [rng.integers(0, A, size=int(L)).astype(np.uint8) for L in lens]
+    off = np.zeros(n + 1, np.uint64)
+    off[1:] = np.cumsum(lens)
+    codes = np.concatenate(seqs) if n else np.zeros(0, np.uint8)
+    return seqs, codes, off
+
+
+def test_screen_matches_numpy(gpu):
+    from seqtree_amd._device import DeviceArray
+    from seqtree_amd.prefilter import screen
+
+    rng = np.random.default_rng(3)
+    nq, nr = 300, 2077
+    Q = rng.integers(-8, 9, size=(nq, C)) / 8.0
+    R = Q[rng.integers(0, nq, size=nr)] + rng.integers(-2, 3, size=(nr, C)) / 8.0
+    Q, R = Q.astype(np.float32), R.astype(np.float32)
+    qn = (Q.astype(np.float64) ** 2).sum(1)
+    rn = (R.astype(np.float64) ** 2).sum(1)
+    d = qn[:, None] + rn[None, :] - 2 * (Q.astype(np.float64) @ R.astype(np.float64).T)
+    tau = float(np.quantile(d, 0.002) // (1 / 64) * (1 / 64))
+    want = np.argwhere(d <= tau)
+    dq = DeviceArray.from_numpy(to_bf16_bits(Q))
+    dr = DeviceArray.from_numpy(to_bf16_bits(R))
+    hq, hr, n = screen(dq, DeviceArray.from_numpy(qn.astype(np.float32)), nq, dr,
+                       DeviceArray.from_numpy(rn.astype(np.float32)), nr, tau, 64)
+    got = np.stack([hq.numpy()[:n], hr.numpy()[:n]], 1).astype(np.int64)
+    got = got[np.lexsort((got[:, 1], got[:, 0]))]
+    assert n == len(want) and np.array_equal(got, want)
+
+
+def test_encode_matches_numpy_model(gpu):
+    from seqtree_amd.prefilter import DeviceSet, Prefilter, _bf16_to_f32
+
+    rng = np.random.default_rng(5)
+    model = Prefilter(seed=1)
+    params = rand_params(rng, model.n_params)
+    model.load(params)
+    assert np.array_equal(model.params(), params)
+    seqs, codes, off = rand_set(rng, 301, 1, 20)
+    emb, norm = model.encode(DeviceSet.upload(codes, off), 20)
+    got = _bf16_to_f32(emb.numpy())
+    want, _ = forward(params, seqs)
+    assert np.allclose(got, want, rtol=3e-2, atol=3e-2 * np.abs(want).max())
+    assert np.allclose(norm.numpy(), (got.astype(np.float64) ** 2).sum(1), rtol=1e-4)
+    model.close()
+
+
+def test_training_gradients_match_numpy_model(gpu):
+    from seqtree_amd._device import DeviceArray
+    from seqtree_amd.prefilter import DeviceSet, Prefilter
+
+    rng = np.random.default_rng(7)
+    model = Prefilter(seed=2)
+    params = rand_params(rng, model.n_params)
+    model.load(params)
+    seqs, codes, off = rand_set(rng, 41, 3, 12)
+    npairs = 70
+    pa = rng.integers(0, 41, size=npairs)
+    pb = rng.integers(0, 41, size=npairs)
+    y = rng.uniform(0, 2, size=npairs).astype(np.float32)
+    w = rng.uniform(0.5, 1.5, size=npairs).astype(np.float32)
+    loss = model.train_step(DeviceSet.upload(codes, off),
+                            DeviceArray.from_numpy(pa.astype(np.uint32)),
+                            DeviceArray.from_numpy(pb.astype(np.uint32)),
+                            DeviceArray.from_numpy(y), DeviceArray.from_numpy(w), npairs, 0.0, 12)
+    e, _ = forward(params, seqs)
+    d = ((e[pa] - e[pb]) ** 2).sum(1)
+    assert np.isclose(loss, float((w * (d - y) ** 2).mean()), rtol=3e-2)
+    got = model.grads().astype(np.float64)
+    want = backward(params, seqs, pa, pb, y, w)
+    # per parameter block: same direction and size up to BF16 gradient rounding
+    o = 0
+    for name, size in (("emb", 32 * E), ("W1", 5 * E * C), ("b1", C), ("W2", 5 * C * C),
+                       ("b2", C), ("W3", 5 * C * C), ("b3", C)):
+        g, h = got[o:o + size], want[o:o + size]
+        o += size
+        rel = np.linalg.norm(g - h) / max(np.linalg.norm(h), 1e-12)
+        assert rel < 0.05, (name, rel)
+    assert np.array_equal(model.params(), params)  # lr = 0 leaves the parameters alone
+    model.close()
+
+
+def test_select_le(gpu):
+    from seqtree_amd._device import DeviceArray
+    from seqtree_amd.prefilter import select_le
+
+    rng = np.random.default_rng(9)
+    v = rng.integers(-100, 100, size=200_003).astype(np.int32)
+    got = select_le(DeviceArray.from_numpy(v), len(v), -90)
+    assert np.array_equal(got, np.flatnonzero(v <= -90))
+
+
+def test_small_search_reports_only_exact_hits(gpu):
+    from seqtree_amd import synthetic
+    from seqtree_amd.prefilter import (DeviceSet, GapblockParams, Prefilter, TrainingData,
+                                       exact_threshold, label_pairs, recall, sample_pairs,
+                                       search, train)
+
+    rng = np.random.default_rng(11)
+    codes, off, fam = synthetic.families(rng, 400, 4, 10, 20)
+    gp = GapblockParams.blosum62_central(20)
+    pa, pb = sample_pairs(rng, fam, 20_000, 0.5)
+    score = label_pairs(codes, off, pa, pb, gp)
+    model = Prefilter(seed=3)
+    losses = train(model, TrainingData(codes, off, pa, pb, score), epochs=3, batch_pairs=2048,
+                   max_len=20)
+    assert losses[-1] < losses[0]
+    qc, qo, _ = synthetic.families(rng, 60, 1, 10, 20)
+    q, r = DeviceSet.upload(qc, qo), DeviceSet.upload(codes, off)
+    cutoff = 60
+    res = search(model, q, r, cutoff, 150.0, gp, 20)
+    eq, er, es = exact_threshold(q, r, cutoff, gp)
+    exact = {(int(a), int(b)): int(s) for a, b, s in zip(eq, er, es)}
+    for a, b, s in zip(res.query, res.ref, res.score):
+        assert exact[(int(a), int(b))] == int(s)
+    assert 0.0 <= recall(res.query, res.ref, eq, er) <= 1.0
+    model.close()
