@@ -471,6 +471,126 @@ def run_batch(torch, dev, lib, stream, args) -> dict:
         kernels, roofline, cpu, glookups=lookups / (ms_step / 1e3) / 1e9)
 
 
+# Learned prefilter: train on 200,000 CDR3s (50,000 families of 4) with 2M pairs labelled by
+# the exact gap-block kernel (BLOSUM62, gap_open 28, central prior 21), then search 50,000
+# queries (a fresh member of 50,000 of the reference families) against 1,000,000 references
+# (250,000 families of 4) at a fixed score cutoff.
+PF_LEN = (10, 20)
+PF_TRAIN = (50_000, 4)
+PF_PAIRS, PF_NEAR = 2_000_000, 0.4
+PF_REFS = (250_000, 4)
+PF_QUERIES = 50_000
+PF_CUTOFF = 48
+PF_EPOCHS = 8
+PF_QUANTILE = 0.998
+
+
+def run_prefilter(torch, dev, lib, stream, args) -> dict:
+    from seqtree_amd import prefilter as pf
+    from seqtree_amd import synthetic as syn
+
+    seed = SEED + 4
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    tc, to, tfam = syn.families(rng, *PF_TRAIN, *PF_LEN)
+    gp = pf.GapblockParams.blosum62_central(PF_LEN[1])
+    pa, pb = pf.sample_pairs(rng, tfam, PF_PAIRS, PF_NEAR)
+    score = pf.label_pairs(tc, to, pa, pb, gp)
+    roots: list = []
+    rc, ro, rfam = syn.families(rng, *PF_REFS, *PF_LEN, roots_out=roots)
+    qfam = rng.choice(PF_REFS[0], size=PF_QUERIES, replace=False)
+    qc, qo = syn.join(syn._mutate_many(rng, [roots[f] for f in qfam], 1, *PF_LEN))
+    data_s = time.perf_counter() - t0
+
+    model = pf.Prefilter(seed=seed)
+    t0 = time.perf_counter()
+    hold = rng.random(len(pa)) < 0.02  # held out for the margin calibration
+    losses = pf.train(model, pf.TrainingData(tc, to, pa[~hold], pb[~hold], score[~hold]),
+                      epochs=args.pf_epochs, seed=seed, max_len=PF_LEN[1],
+                      log=lambda m: print(m, file=sys.stderr, flush=True))
+    train_s = time.perf_counter() - t0
+    dist = pf.embed_distances(model, tc, to, pa[hold], pb[hold], PF_LEN[1])
+    tau = pf.calibrate_tau(dist, score[hold], args.pf_cutoff, quantile=PF_QUANTILE)
+    close = score[hold] <= args.pf_cutoff
+    calib = {"held_out_pairs": int(hold.sum()), "held_out_close": int(close.sum()),
+             "quantile": PF_QUANTILE,
+             "far_pass_rate": float((dist[~close] <= tau).mean()),
+             "abs_err_median": float(np.median(np.abs(dist - score[hold])))}
+
+    Q, R = pf.DeviceSet.upload(qc, qo), pf.DeviceSet.upload(rc, ro)
+    pairs = float(Q.n) * float(R.n)
+    # exhaustive exact pass: the judge of recall, and the time to beat
+    ex_times = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        eq, er, es = pf.exact_threshold(Q, R, args.pf_cutoff, gp)
+        ex_times.append(time.perf_counter() - t0)
+    ex_s = min(ex_times)
+
+    searcher = pf.Searcher(model, gp, PF_LEN[1])
+
+    def step():
+        # the whole pipeline: embed the references (the index) and the queries, screen,
+        # rescore exactly, keep <= cutoff
+        t_index = searcher.index(R)
+        r = searcher.query(Q, args.pf_cutoff, tau)
+        r.seconds["encode_refs"] = t_index
+        return r
+
+    for _ in range(args.warmup):
+        res = step()
+    torch.cuda.synchronize()
+    lib.lib.st_timing_enable(1)
+    lib.timing()
+    times, query_times = [], []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        res = step()
+        times.append(time.perf_counter() - t0)
+        query_times.append(time.perf_counter() - t0 - res.seconds["encode_refs"])
+    lib.lib.st_timing_enable(0)
+    kernels = lib.timing()
+    ms_step = 1e3 * sum(times) / len(times)
+    ms_query = 1e3 * sum(query_times) / len(query_times)
+    rec = pf.recall(res.query, res.ref, eq, er)
+    label, launches, avg_s = dominant(kernels)
+    if label == "prefilter_screen":
+        flops = 2.0 * 128 * pairs
+    else:  # encoder: three convolution GEMMs over both sets' rows
+        slot = PF_LEN[1] + 4
+        rows = (Q.n + R.n) * slot
+        flops = 2.0 * rows * 128 * (5 * 32 + 2 * 5 * 128) / max(launches / args.steps, 1)
+    achieved = flops / avg_s / 1e12
+    roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": 2500.0,
+                "unit": "TFLOP/s", "frac": round(achieved / 2500.0, 4), "traffic": None,
+                "kernel": label, "launches": launches, "avg_launch_ms": round(avg_s * 1e3, 4),
+                "flops_per_launch": flops}
+    cpu = None
+    if args.cpu_rows > 0:
+        qs, rs = syn.decode(qc, qo), syn.decode(rc, ro)
+        cpu = cpu_reference("gapblock", qs, rs, min(args.cpu_rows, 20),
+                            seconds=args.cpu_seconds, pen=gp.pen, go=gp.gap_open,
+                            cube=gp.cube.tolist(), width=gp.width)
+    return base_line(
+        "learned-prefilter effective pairs/s (exact gap-block hits <= cutoff)",
+        pairs / (ms_step / 1e3), args.steps, args.warmup, ms_step,
+        {"workload": "prefilter_cdr3_families_50000x1000000_blosum62_go28_central21",
+         "queries": Q.n, "refs": R.n, "cutoff": args.pf_cutoff, "seed": seed,
+         "train_sequences": len(to) - 1, "train_pairs": int((~hold).sum()),
+         "epochs": args.pf_epochs, "pairs_per_step": pairs, "parallelism": "single GPU"},
+        kernels, roofline, cpu,
+        recall=rec, exact_hits=int(len(eq)), found_hits=int(len(res.query)),
+        candidates=res.candidates, tau_score=round(tau, 3),
+        exhaustive_pairs_per_s=pairs / ex_s, exhaustive_ms=1e3 * ex_s,
+        speedup_vs_exhaustive=ex_s / (ms_step / 1e3),
+        query_only_ms=ms_query, speedup_query_only=ex_s / (ms_query / 1e3),
+        stage_ms={k: round(1e3 * v, 3) for k, v in res.seconds.items()},
+        train_s=round(train_s, 2), data_s=round(data_s, 2), calibration=calib,
+        train_loss=[round(x, 5) for x in losses],
+        label_quantiles={"near": np.quantile(score[:int(PF_PAIRS * PF_NEAR)], [0.1, 0.5, 0.9]).tolist(),
+                         "random": np.quantile(score[int(PF_PAIRS * PF_NEAR):], [0.001, 0.01, 0.5]).tolist()})
+
+
 def run_distance(torch, dev, lib, stream, args, which: str) -> dict:
     rng = np.random.default_rng(SEED + 2)
     if which == "hamming":
@@ -553,7 +673,10 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="wall time the CPU baseline sample is repeated for")
     ap.add_argument("--workload", default="gotoh",
-                    choices=["gotoh", "gapblock", "hamming", "levenshtein", "batch"])
+                    choices=["gotoh", "gapblock", "hamming", "levenshtein", "batch",
+                             "prefilter"])
+    ap.add_argument("--pf-epochs", type=int, default=PF_EPOCHS)
+    ap.add_argument("--pf-cutoff", type=int, default=PF_CUTOFF)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scale for rocprofv3 --pmc "
                          "FETCH_SIZE / WRITE_SIZE readings)")
@@ -580,6 +703,8 @@ def main() -> int:
         line = run_gapblock(torch, dev, lib, stream, args)
     elif args.workload == "batch":
         line = run_batch(torch, dev, lib, stream, args)
+    elif args.workload == "prefilter":
+        line = run_prefilter(torch, dev, lib, stream, args)
     else:
         line = run_distance(torch, dev, lib, stream, args, args.workload)
     if line.get("roofline") and line["roofline"]["bound"] == "valu":

@@ -257,6 +257,12 @@ int32_t st_prefilter_screen(const void* q_emb, const float* q_norm, uint32_t n_q
  * exactly rescored prefilter candidates that are real hits. */
 int32_t st_select_le(const int32_t* values, uint64_t n, int32_t cutoff, uint64_t capacity,
                      uint32_t* index, uint64_t* count, void* stream, st_status* status);
+/* The same selection gathering (pair_a[t], pair_b[t], values[t]) into out_a / out_b /
+ * out_value (device) instead of writing indices. */
+int32_t st_select_pairs_le(const int32_t* values, const uint32_t* pair_a, const uint32_t* pair_b,
+                           uint64_t n, int32_t cutoff, uint64_t capacity, uint32_t* out_a,
+                           uint32_t* out_b, int32_t* out_value, uint64_t* count, void* stream,
+                           st_status* status);
 
 /* ---- device memory (no counterpart in the reference) ------------------------------------
  * Minimal allocation and synchronous copies so a caller without its own GPU allocator can

@@ -789,14 +789,35 @@ PfActs pf_acts(Arena& ar, uint64_t R, uint32_t C, hipStream_t s) {
     return a;
 }
 
-uint64_t pf_rows(uint32_t n, uint32_t slot) { return (uint64_t(n) * slot + 127) / 128 * 128; }
+// Rows of a packed set: every sequence plus its two zero rows, rounded up to the GEMM tile.
+struct PfRows {
+    uint64_t used;
+    uint64_t R;
+};
 
-// Encoder forward over one block of sequences; H3 is left in `ha`.
+PfRows pf_rows(uint64_t off_first, uint64_t off_last, uint32_t n) {
+    const uint64_t used = (off_last - off_first) + 2ull * n;
+    return {used, (used + 127) / 128 * 128};
+}
+
+uint64_t read_offset(const uint64_t* off, uint32_t i, hipStream_t s) {
+    uint64_t h = 0;
+    d2h(&h, off + i, 1, s);
+    sync(s);
+    return h;
+}
+
+// Encoder forward over one packed block of sequences: X0 (and `valid`) for every row, then
+// the three convolutions; H3 is left in h3.
 void pf_forward(const __bf16* B, const float* P, const uint8_t* codes, const uint64_t* off,
-                uint32_t n, uint32_t slot, uint64_t R, const PfActs& x0, const PfActs& h1,
+                uint32_t n, const PfRows& rows, const PfActs& x0, const PfActs& h1,
                 const PfActs& h2, const PfActs& h3, uint8_t* valid, hipStream_t s) {
     LaunchTimer t("prefilter_encode", s);
-    pf::launch_embed(codes, off, n, slot, R, B + pf::kOffEmbB, x0.rows, valid, s);
+    const uint64_t R = rows.R;
+    launch_fill_u32(reinterpret_cast<uint32_t*>(valid), R / 4, 0u, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(x0.rows + rows.used * pf::kEmbDim),
+                    (R - rows.used) * pf::kEmbDim / 2, 0u, s);
+    pf::launch_embed(codes, off, n, B + pf::kOffEmbB, x0.rows, valid, s);
     pf::launch_conv(1, 0, x0.rows, B + pf::kOffWt1, P + pf::kOffB1, nullptr, valid, h1.rows, R, s);
     pf::launch_conv(2, 0, h1.rows, B + pf::kOffWt2, P + pf::kOffB2, nullptr, valid, h2.rows, R, s);
     pf::launch_conv(3, 0, h2.rows, B + pf::kOffWt3, P + pf::kOffB3, nullptr, valid, h3.rows, R, s);
@@ -986,15 +1007,27 @@ int32_t st_prefilter_get_grads(const st_prefilter* model, float* grads, uint64_t
 
 int32_t st_select_le(const int32_t* values, uint64_t n, int32_t cutoff, uint64_t capacity,
                      uint32_t* index, uint64_t* count, void* stream, st_status* status) {
+    return st_select_pairs_le(values, nullptr, nullptr, n, cutoff, capacity, index, nullptr,
+                              nullptr, count, stream, status);
+}
+
+int32_t st_select_pairs_le(const int32_t* values, const uint32_t* pair_a, const uint32_t* pair_b,
+                           uint64_t n, int32_t cutoff, uint64_t capacity, uint32_t* out_a,
+                           uint32_t* out_b, int32_t* out_value, uint64_t* count, void* stream,
+                           st_status* status) {
     return guarded(status, [&] {
         require_device();
+        if ((pair_a == nullptr) != (pair_b == nullptr) ||
+            (pair_a && (out_b == nullptr || out_value == nullptr)))
+            throw HipError{ST_INVALID_ARGUMENT, "seqtree_hip: incomplete pair arrays"};
         const hipStream_t s = static_cast<hipStream_t>(stream);
         Arena ar(s);
         unsigned long long* c = ar.get<unsigned long long>(1);
         launch_fill_u32(reinterpret_cast<uint32_t*>(c), 2, 0u, s);
         {
             LaunchTimer t("select_le", s);
-            pf::launch_select_le(values, n, cutoff, index, capacity, c, s);
+            pf::launch_select_le(values, n, cutoff, pair_a, pair_b, out_a, out_b, out_value,
+                                 capacity, c, s);
         }
         launched("select_le");
         unsigned long long h = 0;
@@ -1016,8 +1049,9 @@ int32_t st_prefilter_train_step(st_prefilter* model, const uint8_t* codes,
         require_device();
         const hipStream_t s = static_cast<hipStream_t>(stream);
         PrefilterImpl& m = model->impl;
-        const uint32_t slot = max_len + 2 * kGuardRows;
-        const uint64_t R = pf_rows(n_seqs, slot);
+        const PfRows rows = pf_rows(read_offset(offsets, 0, s), read_offset(offsets, n_seqs, s),
+                                    n_seqs);
+        const uint64_t R = rows.R;
         Arena ar(s);
         const PfActs x0 = pf_acts(ar, R, pf::kEmbDim, s), h1 = pf_acts(ar, R, pf::kChannels, s),
                      h2 = pf_acts(ar, R, pf::kChannels, s), h3 = pf_acts(ar, R, pf::kChannels, s),
@@ -1031,13 +1065,15 @@ int32_t st_prefilter_train_step(st_prefilter* model, const uint8_t* codes,
         launch_fill_u32(reinterpret_cast<uint32_t*>(loss), 1, 0u, s);
         launch_fill_u32(reinterpret_cast<uint32_t*>(m.G), pf::kParams, 0u, s);
 
-        pf_forward(m.B, m.P, codes, offsets, n_seqs, slot, R, x0, h1, h2, h3, valid, s);
+        pf_forward(m.B, m.P, codes, offsets, n_seqs, rows, x0, h1, h2, h3, valid, s);
         {
             LaunchTimer t("prefilter_train", s);
-            pf::launch_pool(h3.rows, offsets, n_seqs, slot, ef, nullptr, nullptr, s);
+            pf::launch_pool(h3.rows, offsets, n_seqs, ef, nullptr, nullptr, s);
             pf::launch_pair_loss(ef, pair_a, pair_b, target, weight, n_pairs, dE, loss, s);
-            // backward: layer 3, 2, 1, embedding
-            pf::launch_pool_bwd(h3.rows, dE, offsets, n_seqs, slot, R, ga.rows, s);
+            // backward: layer 3, 2, 1, embedding (rows past rows_used stay zero)
+            launch_fill_u32(reinterpret_cast<uint32_t*>(ga.rows + rows.used * pf::kChannels),
+                            (R - rows.used) * pf::kChannels / 2, 0u, s);
+            pf::launch_pool_bwd(h3.rows, dE, offsets, n_seqs, ga.rows, s);
             pf::launch_dw(3, h2.rows, ga.rows, R, m.G + pf::kOffW3, m.G + pf::kOffB3, s);
             pf::launch_conv(3, 1, ga.rows, m.B + pf::kOffWf3, nullptr, h2.rows, valid, gb.rows, R,
                             s);
@@ -1047,7 +1083,7 @@ int32_t st_prefilter_train_step(st_prefilter* model, const uint8_t* codes,
             pf::launch_dw(1, x0.rows, ga.rows, R, m.G + pf::kOffW1, m.G + pf::kOffB1, s);
             pf::launch_conv(1, 2, ga.rows, m.B + pf::kOffWf1, nullptr, nullptr, valid, dx0.rows, R,
                             s);
-            pf::launch_embed_bwd(dx0.rows, codes, offsets, n_seqs, slot, m.G + pf::kOffEmb, s);
+            pf::launch_embed_bwd(dx0.rows, codes, offsets, n_seqs, m.G + pf::kOffEmb, s);
             m.step += 1;
             const float b1 = 0.9f, b2 = 0.999f;
             const float c1 = 1.f - std::pow(b1, float(m.step)), c2 = 1.f - std::pow(b2, float(m.step));
@@ -1072,10 +1108,12 @@ int32_t st_prefilter_encode(const st_prefilter* model, const uint8_t* codes,
         require_device();
         const hipStream_t s = static_cast<hipStream_t>(stream);
         const PrefilterImpl& m = model->impl;
-        const uint32_t slot = max_len + 2 * kGuardRows;
-        // blocks of sequences sized to ~256 MB per 128-channel activation buffer
-        const uint32_t blk = uint32_t(std::min<uint64_t>(n, std::max<uint64_t>(1024, (uint64_t(1) << 20) / slot)));
-        const uint64_t Rb = pf_rows(blk, slot);
+        // blocks of about 2^20 rows (256 MB per 128-channel activation buffer)
+        const uint32_t blk = uint32_t(std::min<uint64_t>(n, std::max<uint64_t>(1024, (uint64_t(1) << 20) / (max_len + 2))));
+        const uint64_t Rb = (uint64_t(blk) * (max_len + 2) + 127) / 128 * 128;
+        std::vector<uint64_t> hoff(n + 1);
+        d2h(hoff.data(), offsets, n + 1, s);
+        sync(s);
         Arena ar(s);
         const PfActs x0 = pf_acts(ar, Rb, pf::kEmbDim, s), ha = pf_acts(ar, Rb, pf::kChannels, s),
                      hb = pf_acts(ar, Rb, pf::kChannels, s);
@@ -1083,10 +1121,13 @@ int32_t st_prefilter_encode(const st_prefilter* model, const uint8_t* codes,
         __bf16* out = static_cast<__bf16*>(emb);
         for (uint32_t s0 = 0; s0 < n; s0 += blk) {
             const uint32_t cnt = std::min(blk, n - s0);
-            const uint64_t R = pf_rows(cnt, slot);
-            pf_forward(m.B, m.P, codes, offsets + s0, cnt, slot, R, x0, ha, hb, ha, valid, s);
+            const PfRows rows = pf_rows(hoff[s0], hoff[s0 + cnt], cnt);
+            if (rows.R > Rb)
+                throw HipError{ST_INVALID_ARGUMENT,
+                               "seqtree_hip: a sequence is longer than max_len"};
+            pf_forward(m.B, m.P, codes, offsets + s0, cnt, rows, x0, ha, hb, ha, valid, s);
             LaunchTimer t("prefilter_pool", s);
-            pf::launch_pool(ha.rows, offsets + s0, cnt, slot, nullptr,
+            pf::launch_pool(ha.rows, offsets + s0, cnt, nullptr,
                             out + uint64_t(s0) * pf::kChannels, norm + s0, s);
         }
         launched("prefilter encode");

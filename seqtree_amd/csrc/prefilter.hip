@@ -3,7 +3,7 @@
 // counterpart in the reference: every score the prefilter lets through is rescored by the
 // exact gap-block kernel, and the exhaustive exact pass stays the judge of recall.
 //
-// Encoder (all sequences of a set padded to one slot of `slot` = Lmax + 4 rows):
+// Encoder (the sequences of a set packed one after another, each followed by two zero rows):
 //   X0[r]  = emb[code(r)]                      32 channels, zero on rows outside a sequence
 //   Hl[r]  = relu(sum_{k<5} X_{l-1}[r-2+k] . W_l[k] + b_l)   128 channels, zeroed likewise
 //   e(s)   = mean_{p < len(s)} H3[row(s, p)]   128-d
@@ -11,10 +11,12 @@
 // product: one BF16 MFMA GEMM over all (query, reference) pairs with FP32 accumulation, whose
 // epilogue keeps only pairs under the screening threshold.
 //
-// Row layout of an activation buffer: rows [0, R) with R = rows rounded up to the 128-row
-// GEMM tile, plus two zero rows before row 0 and after row R-1, so the five-row
-// convolution window of every row is in bounds. A row of a sequence slot is "valid" when it
-// holds a residue (rows < n * slot, 2 <= p < len + 2 inside the slot).
+// Row layout of an activation buffer: sequence s of a set whose offsets start at off[0]
+// occupies rows start(s) = (off[s] - off[0]) + 2 s .. start(s) + len(s) - 1, followed by two
+// zero rows that pad both its own window and the next sequence's; rows_used = start(n). The
+// buffer holds R rows (rows_used rounded up to the 128-row GEMM tile, zero past rows_used)
+// plus two zero guard rows before row 0 and after row R - 1, so the five-row convolution
+// window of every row is in bounds. A row is "valid" when it holds a residue.
 //
 // Convolutions as GEMMs: the im2col row of output row r is the contiguous span
 // X[(r - 2) * CIN, (r + 3) * CIN) of the row-major activations, so A is X read with row
@@ -42,28 +44,28 @@ __device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) {
 }
 
 // ---- embedding -----------------------------------------------------------------------
+__device__ __forceinline__ uint64_t start_row(const uint64_t* off, uint32_t s) {
+    return (off[s] - off[0]) + 2ull * s;
+}
+
+// One thread per (sequence, channel): the sequence's rows and its two zero rows.
 __global__ __launch_bounds__(256) void embed_kernel(const uint8_t* __restrict__ codes,
                                                     const uint64_t* __restrict__ off, uint32_t n,
-                                                    uint32_t slot, uint64_t R,
                                                     const __bf16* __restrict__ emb,
                                                     __bf16* __restrict__ X0,
                                                     uint8_t* __restrict__ valid) {
     const uint64_t idx = uint64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (idx >= R * kEmbDim) return;
-    const uint64_t r = idx / kEmbDim;
-    const uint32_t c = uint32_t(idx % kEmbDim);
-    const uint64_t s = r / slot;
-    const int64_t p = int64_t(r % slot) - 2;
-    bool ok = false;
-    __bf16 v = f2bf(0.f);
-    if (s < n) {
-        const uint64_t o = off[s];
-        const int64_t len = int64_t(off[s + 1] - o);
-        ok = p >= 0 && p < len;
-        if (ok) v = emb[(uint32_t(codes[o + uint64_t(p)]) % kEmbRows) * kEmbDim + c];
+    if (idx >= uint64_t(n) * kEmbDim) return;
+    const uint32_t s = uint32_t(idx / kEmbDim), c = uint32_t(idx % kEmbDim);
+    const uint64_t o = off[s];
+    const uint32_t len = uint32_t(off[s + 1] - o);
+    const uint64_t r0 = start_row(off, s);
+    for (uint32_t p = 0; p < len + 2; ++p) {
+        const bool ok = p < len;
+        X0[(r0 + p) * kEmbDim + c] =
+            ok ? emb[(uint32_t(codes[o + p]) % kEmbRows) * kEmbDim + c] : f2bf(0.f);
+        if (c == 0) valid[r0 + p] = ok ? 1 : 0;
     }
-    X0[idx] = v;
-    if (c == 0) valid[r] = ok ? 1 : 0;
 }
 
 // ---- convolution GEMM (MFMA) -----------------------------------------------------------
@@ -75,6 +77,9 @@ __global__ __launch_bounds__(256) void embed_kernel(const uint8_t* __restrict__ 
 constexpr uint32_t kKChunk = 64;
 constexpr uint32_t kBPad = 8;  // bf16 elements of padding per LDS row
 
+// Software-pipelined: while the MFMAs consume K-chunk c (A fragments in registers, B staged in
+// one of two LDS buffers), the global loads of chunk c + 1 are in flight; one barrier per
+// chunk.
 template <uint32_t CIN, uint32_t N, int EPI>
 __global__ __launch_bounds__(256) void conv_kernel(const __bf16* __restrict__ X,
                                                    const __bf16* __restrict__ Bt,
@@ -85,8 +90,12 @@ __global__ __launch_bounds__(256) void conv_kernel(const __bf16* __restrict__ X,
     constexpr uint32_t K = 5 * CIN;
     constexpr uint32_t NB = N / 32;
     constexpr uint32_t KC = K % kKChunk == 0 ? kKChunk : 32;
+    constexpr uint32_t NCH = K / KC;
+    constexpr uint32_t KS = KC / 16;               // MFMA steps per chunk
+    constexpr uint32_t PIECES = N * (KC / 8);      // 16-byte pieces of a B chunk
+    constexpr uint32_t PPT = (PIECES + 255) / 256;  // pieces per thread
     static_assert(K % KC == 0 && KC % 16 == 0, "K chunking");
-    __shared__ __bf16 Bs[N][KC + kBPad];
+    __shared__ __bf16 Bs[2][N][KC + kBPad];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t li = lane & 31u, lk = lane >> 5;
     const uint64_t r0 = uint64_t(blockIdx.x) * 128 + wave * 32;
@@ -96,23 +105,56 @@ __global__ __launch_bounds__(256) void conv_kernel(const __bf16* __restrict__ X,
 #pragma unroll
     for (uint32_t j = 0; j < NB; ++j) acc[j] = f32x16{0.f};
 
-    for (uint32_t kc = 0; kc < K; kc += KC) {
-        // stage Bt[:, kc, kc + KC) (16-byte pieces)
-        for (uint32_t e = tid; e < N * (KC / 8); e += 256) {
-            const uint32_t n = e / (KC / 8), piece = e % (KC / 8);
-            *reinterpret_cast<bf16x8*>(&Bs[n][piece * 8]) =
-                *reinterpret_cast<const bf16x8*>(Bt + uint64_t(n) * K + kc + piece * 8);
-        }
-        __syncthreads();
+    bf16x8 bq[PPT], aq[KS], acur[KS];
+    auto load_b = [&](uint32_t kc) {
 #pragma unroll
-        for (uint32_t ks = 0; ks < KC; ks += 16) {
-            const bf16x8 a = *reinterpret_cast<const bf16x8*>(arow + kc + ks);
-#pragma unroll
-            for (uint32_t j = 0; j < NB; ++j) {
-                const bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[j * 32 + li][ks + 8 * lk]);
-                acc[j] = mfma(a, b, acc[j]);
+        for (uint32_t u = 0; u < PPT; ++u) {
+            const uint32_t e = tid + u * 256;
+            if (e < PIECES) {
+                const uint32_t n = e / (KC / 8), piece = e % (KC / 8);
+                bq[u] = *reinterpret_cast<const bf16x8*>(Bt + uint64_t(n) * K + kc + piece * 8);
             }
         }
+    };
+    auto store_b = [&](uint32_t buf) {
+#pragma unroll
+        for (uint32_t u = 0; u < PPT; ++u) {
+            const uint32_t e = tid + u * 256;
+            if (e < PIECES) {
+                const uint32_t n = e / (KC / 8), piece = e % (KC / 8);
+                *reinterpret_cast<bf16x8*>(&Bs[buf][n][piece * 8]) = bq[u];
+            }
+        }
+    };
+    auto load_a = [&](uint32_t kc) {
+#pragma unroll
+        for (uint32_t ks = 0; ks < KS; ++ks)
+            aq[ks] = *reinterpret_cast<const bf16x8*>(arow + kc + ks * 16);
+    };
+
+    load_b(0);
+    load_a(0);
+    store_b(0);
+    __syncthreads();
+#pragma unroll 1
+    for (uint32_t c = 0; c < NCH; ++c) {
+#pragma unroll
+        for (uint32_t ks = 0; ks < KS; ++ks) acur[ks] = aq[ks];
+        if (c + 1 < NCH) {
+            load_b((c + 1) * KC);
+            load_a((c + 1) * KC);
+        }
+        const uint32_t buf = c & 1u;
+#pragma unroll
+        for (uint32_t ks = 0; ks < KS; ++ks) {
+#pragma unroll
+            for (uint32_t j = 0; j < NB; ++j) {
+                const bf16x8 b =
+                    *reinterpret_cast<const bf16x8*>(&Bs[buf][j * 32 + li][ks * 16 + 8 * lk]);
+                acc[j] = mfma(acur[ks], b, acc[j]);
+            }
+        }
+        if (c + 1 < NCH) store_b(buf ^ 1u);
         __syncthreads();
     }
 #pragma unroll
@@ -141,13 +183,13 @@ __global__ __launch_bounds__(256) void conv_kernel(const __bf16* __restrict__ X,
 // rounding and the squared norm of the BF16 rounding (the operand the Gram GEMM sees).
 __global__ __launch_bounds__(kChannels) void pool_kernel(const __bf16* __restrict__ H,
                                                          const uint64_t* __restrict__ off,
-                                                         uint32_t slot, float* __restrict__ ef,
+                                                         float* __restrict__ ef,
                                                          __bf16* __restrict__ eb,
                                                          float* __restrict__ norm) {
     __shared__ float part[kChannels / 64];
     const uint32_t s = blockIdx.x, c = threadIdx.x;
     const uint32_t len = uint32_t(off[s + 1] - off[s]);
-    const uint64_t r0 = uint64_t(s) * slot + 2;
+    const uint64_t r0 = start_row(off, s);
     float sum = 0.f;
     for (uint32_t p = 0; p < len; ++p) sum += bf2f(H[(r0 + p) * kChannels + c]);
     const float v = len ? sum / float(len) : 0.f;
@@ -168,29 +210,53 @@ __global__ __launch_bounds__(kChannels) void pool_kernel(const __bf16* __restric
 
 // ---- Gram screen (MFMA) --------------------------------------------------------------------
 // Workgroup: 4 waves x 64 queries (two 32-row MFMA blocks per wave, operands kept in
-// registers for the whole K = 128) against the references [r_begin, r_end) in tiles of 32
-// staged through LDS. Pair (q, r) is kept when dot(q, r) >= hq[q] + hr[r], i.e.
-// |q|^2 + |r|^2 - 2 dot <= tau with hq = (|q|^2 - tau) / 2 and hr = |r|^2 / 2.
+// registers for the whole K = 128) against the references [r_begin, r_end) in tiles of 64.
+// Tiles stream through a three-buffer LDS ring that all four waves share: in iteration t the
+// workgroup stores tile t + 1 (loaded into registers during iteration t - 1), issues the
+// global loads of tile t + 2 and runs the MFMAs of tile t, with one barrier per iteration.
+// Pair (q, r) is kept when dot(q, r) >= hq[q] + hr[r], i.e. |q|^2 + |r|^2 - 2 dot <= tau
+// with hq = (|q|^2 - tau) / 2 and hr = |r|^2 / 2. Kept pairs go to the wave's own LDS queue
+// and are flushed to the global arrays with one global atomic per queue-full (a tile with
+// more hits than the queue holds is written straight to global memory).
 constexpr uint32_t kGramQ = 256;
-constexpr uint32_t kGramRowPad = kChannels + 8;
+constexpr uint32_t kTileR = 64;
+constexpr uint32_t kRowPad = kChannels + 8;     // bf16 per LDS row (16-byte skew)
+constexpr uint32_t kWaveQueue = 256;
+constexpr uint32_t kPiecesPerThread = kTileR * kChannels / 8 / 256;  // 16-byte pieces
 
-__device__ __forceinline__ void push_hit(bool hit, uint32_t q, uint32_t r, uint32_t* hq_out,
-                                         uint32_t* hr_out, uint64_t cap,
-                                         unsigned long long* count) {
-    const unsigned long long mask = __ballot(hit);
-    if (mask == 0) return;
+// Exclusive prefix sum of v over the wave and the wave total.
+__device__ __forceinline__ uint32_t wave_scan(uint32_t v, uint32_t* total) {
     const uint32_t lane = __lane_id();
-    const uint32_t leader = __ffsll(mask) - 1;
-    unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(mask));
-    base = __shfl(base, leader);
-    if (hit) {
-        const unsigned long long idx = base + __popcll(mask & ((1ull << lane) - 1));
-        if (idx < cap) {
-            hq_out[idx] = q;
-            hr_out[idx] = r;
-        }
+    uint32_t x = v;
+#pragma unroll
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
     }
+    *total = __shfl(x, 63);
+    return x - v;
+}
+
+// Orders this wave's LDS writes before its later LDS reads by other lanes.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ void flush_queue(const uint32_t* qq, const uint32_t* qr, uint32_t n,
+                                            uint32_t* hit_q, uint32_t* hit_r, uint64_t cap,
+                                            unsigned long long* count) {
+    const uint32_t lane = __lane_id();
+    unsigned long long g = 0;
+    if (lane == 0) g = atomicAdd(count, (unsigned long long)n);
+    g = __shfl(g, 0);
+    wave_sync();
+    for (uint32_t i = lane; i < n; i += 64)
+        if (g + i < cap) {
+            hit_q[g + i] = qq[i];
+            hit_r[g + i] = qr[i];
+        }
+    wave_sync();
 }
 
 __global__ __launch_bounds__(256) void gram_kernel(const __bf16* __restrict__ Q,
@@ -200,12 +266,18 @@ __global__ __launch_bounds__(256) void gram_kernel(const __bf16* __restrict__ Q,
                                                    float tau, uint32_t r_per_group,
                                                    uint32_t* hit_q, uint32_t* hit_r, uint64_t cap,
                                                    unsigned long long* count) {
-    __shared__ __bf16 Bs[32][kGramRowPad];
+    __shared__ __bf16 ring[3][kTileR][kRowPad];
+    __shared__ uint32_t queue_q[4][kWaveQueue], queue_r[4][kWaveQueue];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t li = lane & 31u, lk = lane >> 5;
     const uint32_t q0 = blockIdx.x * kGramQ + wave * 64;
     const uint32_t r_begin = blockIdx.y * r_per_group;
     const uint32_t r_end = r_begin + r_per_group < nr ? r_begin + r_per_group : nr;
+    const uint32_t ntiles = r_end > r_begin ? (r_end - r_begin + kTileR - 1) / kTileR : 0;
+    uint32_t* qq = queue_q[wave];
+    uint32_t* qr = queue_r[wave];
+    uint32_t qn = 0;  // wave-uniform queue fill
+    const bool active = q0 < nq;  // waves past the last query still help stage the tiles
 
     bf16x8 a[2][8];
     float h[2][16];
@@ -224,44 +296,90 @@ __global__ __launch_bounds__(256) void gram_kernel(const __bf16* __restrict__ Q,
         }
     }
 
-    for (uint32_t t0 = r_begin; t0 < r_end; t0 += 32) {
-        // stage references t0 .. t0 + 31 (rows past nr read as zero)
-        {
-            const uint32_t rr = tid >> 3, piece = (tid & 7u) * 2;
-            const uint32_t r = t0 + rr;
+    // thread tid stages pieces tid + 256 u of a tile: row (tid + 256 u) / 16, piece % 16
+    bf16x8 stage[kPiecesPerThread];
+    auto load_tile = [&](uint32_t t) {
 #pragma unroll
-            for (uint32_t u = 0; u < 2; ++u)
-                *reinterpret_cast<bf16x8*>(&Bs[rr][(piece + u) * 8]) =
-                    r < nr ? *reinterpret_cast<const bf16x8*>(Rm + uint64_t(r) * kChannels +
-                                                              (piece + u) * 8)
+        for (uint32_t u = 0; u < kPiecesPerThread; ++u) {
+            const uint32_t e = tid + 256 * u, row = e / 16, piece = e % 16;
+            const uint32_t r = r_begin + t * kTileR + row;
+            stage[u] = (t < ntiles && r < r_end)
+                           ? *reinterpret_cast<const bf16x8*>(Rm + uint64_t(r) * kChannels +
+                                                              piece * 8)
                            : bf16x8{};
         }
-        __syncthreads();
-        f32x16 acc0 = f32x16{0.f}, acc1 = f32x16{0.f};
+    };
+    auto store_tile = [&](uint32_t buf) {
 #pragma unroll
-        for (uint32_t ks = 0; ks < 8; ++ks) {
-            const bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[li][ks * 16 + 8 * lk]);
-            acc0 = mfma(a[0][ks], b, acc0);
-            acc1 = mfma(a[1][ks], b, acc1);
+        for (uint32_t u = 0; u < kPiecesPerThread; ++u) {
+            const uint32_t e = tid + 256 * u, row = e / 16, piece = e % 16;
+            *reinterpret_cast<bf16x8*>(&ring[buf][row][piece * 8]) = stage[u];
         }
-        const uint32_t r = t0 + li;
-        const float hrv = r < r_end ? 0.5f * r_norm[r] : __builtin_inff();
-        bool any = false;
+    };
+
+    load_tile(0);
+    store_tile(0);
+    load_tile(1);
+    __syncthreads();
+#pragma unroll 1
+    for (uint32_t t = 0; t < ntiles; ++t) {
+        const uint32_t buf = t % 3;
+        store_tile((t + 1) % 3);  // tile t + 1 (zeros past the end)
+        load_tile(t + 2);
+        if (active) {
 #pragma unroll
-        for (uint32_t e = 0; e < 16; ++e) {
-            any |= acc0[e] >= h[0][e] + hrv;
-            any |= acc1[e] >= h[1][e] + hrv;
-        }
-        if (__any(any)) {
+            for (uint32_t half = 0; half < 2; ++half) {
+                f32x16 acc0 = f32x16{0.f}, acc1 = f32x16{0.f};
 #pragma unroll
-            for (uint32_t e = 0; e < 16; ++e) {
-                const uint32_t qa = q0 + (e / 4) * 8 + lk * 4 + (e % 4);
-                push_hit(acc0[e] >= h[0][e] + hrv, qa, r, hit_q, hit_r, cap, count);
-                push_hit(acc1[e] >= h[1][e] + hrv, qa + 32, r, hit_q, hit_r, cap, count);
+                for (uint32_t ks = 0; ks < 8; ++ks) {
+                    const bf16x8 b = *reinterpret_cast<const bf16x8*>(
+                        &ring[buf][half * 32 + li][ks * 16 + 8 * lk]);
+                    acc0 = mfma(a[0][ks], b, acc0);
+                    acc1 = mfma(a[1][ks], b, acc1);
+                }
+                const uint32_t r = r_begin + t * kTileR + half * 32 + li;
+                const float hrv = r < r_end ? 0.5f * r_norm[r] : __builtin_inff();
+                uint32_t bits = 0;  // bit b * 16 + e: accumulator e of block b is a hit
+#pragma unroll
+                for (uint32_t e = 0; e < 16; ++e) {
+                    bits |= uint32_t(acc0[e] >= h[0][e] + hrv) << e;
+                    bits |= uint32_t(acc1[e] >= h[1][e] + hrv) << (16 + e);
+                }
+                if (__any(bits != 0)) {
+                    uint32_t total = 0;
+                    const uint32_t pre = wave_scan(__popc(bits), &total);
+                    if (qn + total > kWaveQueue && qn) {
+                        flush_queue(qq, qr, qn, hit_q, hit_r, cap, count);
+                        qn = 0;
+                    }
+                    unsigned long long gb = 0;
+                    const bool direct = total > kWaveQueue;
+                    if (direct) {
+                        if (lane == 0) gb = atomicAdd(count, (unsigned long long)total);
+                        gb = __shfl(gb, 0);
+                    }
+                    uint32_t k = pre;
+                    while (bits) {
+                        const uint32_t bit = __ffs(bits) - 1;
+                        bits &= bits - 1;
+                        const uint32_t e = bit & 15u, blk = bit >> 4;
+                        const uint32_t qa = q0 + blk * 32 + (e / 4) * 8 + lk * 4 + (e % 4);
+                        if (!direct) {
+                            qq[qn + k] = qa;
+                            qr[qn + k] = r;
+                        } else if (gb + k < cap) {
+                            hit_q[gb + k] = qa;
+                            hit_r[gb + k] = r;
+                        }
+                        ++k;
+                    }
+                    if (!direct) qn += total;
+                }
             }
         }
-        __syncthreads();  // the tile is rewritten next
+        __syncthreads();
     }
+    if (qn) flush_queue(qq, qr, qn, hit_q, hit_r, cap, count);
 }
 
 // ---- training -----------------------------------------------------------------------------
@@ -288,23 +406,23 @@ __global__ __launch_bounds__(kChannels) void pair_loss_kernel(
     if (c == 0) atomicAdd(loss, w[t] * err * err);
 }
 
-// dY3[r][c] = dE[s][c] / len where H3[r][c] > 0 (zero elsewhere, invalid rows included).
+// dY3[r][c] = dE[s][c] / len where H3[r][c] > 0 on the rows of sequence s, and zero on its
+// two padding rows (the rows past rows_used are zeroed separately). One thread per
+// (sequence, channel).
 __global__ __launch_bounds__(256) void pool_bwd_kernel(const __bf16* __restrict__ H,
                                                        const float* __restrict__ dE,
                                                        const uint64_t* __restrict__ off,
-                                                       uint32_t n, uint32_t slot, uint64_t R,
-                                                       __bf16* __restrict__ dY) {
+                                                       uint32_t n, __bf16* __restrict__ dY) {
     const uint64_t idx = uint64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (idx >= R * kChannels) return;
-    const uint64_t r = idx / kChannels;
-    const uint32_t c = uint32_t(idx % kChannels);
-    const uint64_t s = r / slot;
-    float v = 0.f;
-    if (s < n && bf2f(H[idx]) > 0.f) {
-        const uint32_t len = uint32_t(off[s + 1] - off[s]);
-        v = dE[s * kChannels + c] / float(len);
+    if (idx >= uint64_t(n) * kChannels) return;
+    const uint32_t s = uint32_t(idx / kChannels), c = uint32_t(idx % kChannels);
+    const uint32_t len = uint32_t(off[s + 1] - off[s]);
+    const uint64_t r0 = start_row(off, s);
+    const float g = len ? dE[uint64_t(s) * kChannels + c] / float(len) : 0.f;
+    for (uint32_t p = 0; p < len + 2; ++p) {
+        const uint64_t i = (r0 + p) * kChannels + c;
+        dY[i] = f2bf(p < len && bf2f(H[i]) > 0.f ? g : 0.f);
     }
-    dY[idx] = f2bf(v);
 }
 
 // dW[kk][co] += sum_r X[(r - 2) CIN + kk] dY[r][co], and (blockIdx.x == 0) db[co] += sum_r
@@ -371,8 +489,7 @@ __global__ __launch_bounds__(256) void dw_kernel(const __bf16* __restrict__ X,
 __global__ __launch_bounds__(256) void embed_bwd_kernel(const __bf16* __restrict__ dX0,
                                                         const uint8_t* __restrict__ codes,
                                                         const uint64_t* __restrict__ off,
-                                                        uint32_t n, uint32_t slot,
-                                                        uint32_t seqs_per_block,
+                                                        uint32_t n, uint32_t seqs_per_block,
                                                         float* __restrict__ dEmb) {
     __shared__ float acc[kMaxAlphabet * kEmbDim];
     const uint32_t tid = threadIdx.x;
@@ -383,9 +500,10 @@ __global__ __launch_bounds__(256) void embed_bwd_kernel(const __bf16* __restrict
     for (uint32_t s = s0; s < s1; ++s) {
         const uint64_t o = off[s];
         const uint32_t len = uint32_t(off[s + 1] - o);
+        const uint64_t r0 = start_row(off, s);
         for (uint32_t e = tid; e < len * kEmbDim; e += 256) {
             const uint32_t p = e / kEmbDim, c = e % kEmbDim;
-            const uint64_t r = uint64_t(s) * slot + 2 + p;
+            const uint64_t r = r0 + p;
             atomicAdd(&acc[(uint32_t(codes[o + p]) % kEmbRows) * kEmbDim + c],
                       bf2f(dX0[r * kEmbDim + c]));
         }
@@ -434,10 +552,15 @@ __global__ __launch_bounds__(256) void pack_weights_kernel(const float* __restri
     }
 }
 
-// Indices t < n with v[t] <= cutoff, in no particular order (one atomic per wave).
+// Entries t < n with v[t] <= cutoff, in no particular order (one atomic per wave): their
+// index, or with pa / pb given the gathered (pa[t], pb[t], v[t]).
 __global__ __launch_bounds__(256) void select_le_kernel(const int32_t* __restrict__ v, uint64_t n,
-                                                        int32_t cutoff, uint32_t* __restrict__ idx,
-                                                        uint64_t cap,
+                                                        int32_t cutoff,
+                                                        const uint32_t* __restrict__ pa,
+                                                        const uint32_t* __restrict__ pb,
+                                                        uint32_t* __restrict__ out_a,
+                                                        uint32_t* __restrict__ out_b,
+                                                        int32_t* __restrict__ out_v, uint64_t cap,
                                                         unsigned long long* __restrict__ count) {
     const uint64_t stride = uint64_t(gridDim.x) * 256;
     for (uint64_t t0 = uint64_t(blockIdx.x) * 256; t0 < n; t0 += stride) {
@@ -452,7 +575,15 @@ __global__ __launch_bounds__(256) void select_le_kernel(const int32_t* __restric
         base = __shfl(base, leader);
         if (hit) {
             const unsigned long long k = base + __popcll(mask & ((1ull << lane) - 1));
-            if (k < cap) idx[k] = uint32_t(t);
+            if (k < cap) {
+                if (pa) {
+                    out_a[k] = pa[t];
+                    out_b[k] = pb[t];
+                    out_v[k] = v[t];
+                } else {
+                    out_a[k] = uint32_t(t);
+                }
+            }
         }
     }
 }
@@ -460,18 +591,20 @@ __global__ __launch_bounds__(256) void select_le_kernel(const int32_t* __restric
 }  // namespace
 
 // ---- launchers ------------------------------------------------------------------------------
-void launch_select_le(const int32_t* v, uint64_t n, int32_t cutoff, uint32_t* idx, uint64_t cap,
-                      unsigned long long* count, hipStream_t s) {
+void launch_select_le(const int32_t* v, uint64_t n, int32_t cutoff, const uint32_t* pa,
+                      const uint32_t* pb, uint32_t* out_a, uint32_t* out_b, int32_t* out_v,
+                      uint64_t cap, unsigned long long* count, hipStream_t s) {
     if (n == 0) return;
     const uint64_t blocks = (n + 255) / 256;
     hipLaunchKernelGGL(select_le_kernel, dim3(uint32_t(blocks < 65536 ? blocks : 65536)), dim3(256),
-                       0, s, v, n, cutoff, idx, cap, count);
+                       0, s, v, n, cutoff, pa, pb, out_a, out_b, out_v, cap, count);
 }
-void launch_embed(const uint8_t* codes, const uint64_t* off, uint32_t n, uint32_t slot, uint64_t R,
-                  const __bf16* emb, __bf16* X0, uint8_t* valid, hipStream_t s) {
-    const uint64_t total = R * kEmbDim;
+void launch_embed(const uint8_t* codes, const uint64_t* off, uint32_t n, const __bf16* emb,
+                  __bf16* X0, uint8_t* valid, hipStream_t s) {
+    const uint64_t total = uint64_t(n) * kEmbDim;
+    if (total == 0) return;
     hipLaunchKernelGGL(embed_kernel, dim3(uint32_t((total + 255) / 256)), dim3(256), 0, s, codes,
-                       off, n, slot, R, emb, X0, valid);
+                       off, n, emb, X0, valid);
 }
 
 void launch_conv(int layer, int epi, const __bf16* X, const __bf16* Bt, const float* bias,
@@ -492,10 +625,10 @@ void launch_conv(int layer, int epi, const __bf16* X, const __bf16* Bt, const fl
                            mask, valid, Y);
 }
 
-void launch_pool(const __bf16* H, const uint64_t* off, uint32_t n, uint32_t slot, float* ef,
-                 __bf16* eb, float* norm, hipStream_t s) {
+void launch_pool(const __bf16* H, const uint64_t* off, uint32_t n, float* ef, __bf16* eb,
+                 float* norm, hipStream_t s) {
     if (n == 0) return;
-    hipLaunchKernelGGL(pool_kernel, dim3(n), dim3(kChannels), 0, s, H, off, slot, ef, eb, norm);
+    hipLaunchKernelGGL(pool_kernel, dim3(n), dim3(kChannels), 0, s, H, off, ef, eb, norm);
 }
 
 void launch_gram(const __bf16* Q, const float* q_norm, uint32_t nq, const __bf16* Rm,
@@ -521,10 +654,11 @@ void launch_pair_loss(const float* ef, const uint32_t* pa, const uint32_t* pb, c
 }
 
 void launch_pool_bwd(const __bf16* H, const float* dE, const uint64_t* off, uint32_t n,
-                     uint32_t slot, uint64_t R, __bf16* dY, hipStream_t s) {
-    const uint64_t total = R * kChannels;
+                     __bf16* dY, hipStream_t s) {
+    const uint64_t total = uint64_t(n) * kChannels;
+    if (total == 0) return;
     hipLaunchKernelGGL(pool_bwd_kernel, dim3(uint32_t((total + 255) / 256)), dim3(256), 0, s, H, dE,
-                       off, n, slot, R, dY);
+                       off, n, dY);
 }
 
 void launch_dw(int layer, const __bf16* X, const __bf16* dY, uint64_t R, float* dW, float* db,
@@ -542,11 +676,11 @@ void launch_dw(int layer, const __bf16* X, const __bf16* dY, uint64_t R, float* 
 }
 
 void launch_embed_bwd(const __bf16* dX0, const uint8_t* codes, const uint64_t* off, uint32_t n,
-                      uint32_t slot, float* dEmb, hipStream_t s) {
+                      float* dEmb, hipStream_t s) {
     if (n == 0) return;
     const uint32_t per = 64;
     hipLaunchKernelGGL(embed_bwd_kernel, dim3((n + per - 1) / per), dim3(256), 0, s, dX0, codes,
-                       off, n, slot, per, dEmb);
+                       off, n, per, dEmb);
 }
 
 void launch_adam(float* p, const float* g, float* m, float* v, uint32_t n, float lr, float b1,

@@ -84,30 +84,33 @@ constexpr uint32_t kOffWf2 = kOffWf1 + 5 * kEmbDim * kChannels;
 constexpr uint32_t kOffWf3 = kOffWf2 + 5 * kChannels * kChannels;
 constexpr uint32_t kBf16Params = kOffWf3 + 5 * kChannels * kChannels;
 
-void launch_embed(const uint8_t* codes, const uint64_t* off, uint32_t n, uint32_t slot, uint64_t R,
-                  const __bf16* emb, __bf16* X0, uint8_t* valid, hipStream_t s);
+// Sequence s of a set occupies rows (off[s] - off[0]) + 2 s onward (see prefilter.hip).
+void launch_embed(const uint8_t* codes, const uint64_t* off, uint32_t n, const __bf16* emb,
+                  __bf16* X0, uint8_t* valid, hipStream_t s);
 // layer 1..3; epi 0 forward (bias, relu, valid rows), 1 backward data masked by mask > 0,
 // 2 backward data of the embedding layer (valid rows). R: rows, a multiple of 128.
 void launch_conv(int layer, int epi, const __bf16* X, const __bf16* Bt, const float* bias,
                  const __bf16* mask, const uint8_t* valid, __bf16* Y, uint64_t R, hipStream_t s);
-void launch_pool(const __bf16* H, const uint64_t* off, uint32_t n, uint32_t slot, float* ef,
-                 __bf16* eb, float* norm, hipStream_t s);
+void launch_pool(const __bf16* H, const uint64_t* off, uint32_t n, float* ef, __bf16* eb,
+                 float* norm, hipStream_t s);
 void launch_gram(const __bf16* Q, const float* q_norm, uint32_t nq, const __bf16* Rm,
                  const float* r_norm, uint32_t nr, float tau, uint32_t* hit_q, uint32_t* hit_r,
                  uint64_t cap, unsigned long long* count, uint32_t cus, hipStream_t s);
 void launch_pair_loss(const float* ef, const uint32_t* pa, const uint32_t* pb, const float* y,
                       const float* w, uint32_t n_pairs, float* dE, float* loss, hipStream_t s);
 void launch_pool_bwd(const __bf16* H, const float* dE, const uint64_t* off, uint32_t n,
-                     uint32_t slot, uint64_t R, __bf16* dY, hipStream_t s);
+                     __bf16* dY, hipStream_t s);
 void launch_dw(int layer, const __bf16* X, const __bf16* dY, uint64_t R, float* dW, float* db,
                hipStream_t s);
 void launch_embed_bwd(const __bf16* dX0, const uint8_t* codes, const uint64_t* off, uint32_t n,
-                      uint32_t slot, float* dEmb, hipStream_t s);
+                      float* dEmb, hipStream_t s);
 void launch_adam(float* p, const float* g, float* m, float* v, uint32_t n, float lr, float b1,
                  float b2, float c1, float c2, float eps, hipStream_t s);
 void launch_pack_weights(const float* P, __bf16* B, hipStream_t s);
-void launch_select_le(const int32_t* v, uint64_t n, int32_t cutoff, uint32_t* idx, uint64_t cap,
-                      unsigned long long* count, hipStream_t s);
+// Entries with v[t] <= cutoff: indices into out_a, or (pa, pb given) (pa[t], pb[t], v[t]).
+void launch_select_le(const int32_t* v, uint64_t n, int32_t cutoff, const uint32_t* pa,
+                      const uint32_t* pb, uint32_t* out_a, uint32_t* out_b, int32_t* out_v,
+                      uint64_t cap, unsigned long long* count, hipStream_t s);
 }  // namespace pf
 
 // ---- probe.hip -------------------------------------------------------------------------

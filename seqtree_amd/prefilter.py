@@ -62,6 +62,9 @@ def _bind():
                                       ctypes.POINTER(u64), vp, S]
     L.st_select_le.restype = i32
     L.st_select_le.argtypes = [vp, u64, i32, u64, vp, ctypes.POINTER(u64), vp, S]
+    L.st_select_pairs_le.restype = i32
+    L.st_select_pairs_le.argtypes = [vp, vp, vp, u64, i32, u64, vp, vp, vp, ctypes.POINTER(u64),
+                                     vp, S]
     L.st_gapblock_pairs_device.restype = i32
     L.st_gapblock_pairs_device.argtypes = [vp, vp, u64, u32, vp, vp, u64, u32, vp, u32, i32, i32,
                                            vp, u32, vp, vp, u64, vp, vp, S]
@@ -213,21 +216,26 @@ class Prefilter:
         return emb, norm
 
 
-def screen(q_emb, q_norm, nq: int, r_emb, r_norm, nr: int, tau: float, capacity: int):
-    """Device (query, ref) index arrays of the pairs with estimated distance <= tau, and
-    their count (grows the buffers and repeats when the capacity is short)."""
+def screen(q_emb, q_norm, nq: int, r_emb, r_norm, nr: int, tau: float, capacity: int,
+           out: tuple | None = None):
+    """Device (query, ref) index arrays of the pairs with estimated distance <= tau (distance
+    units), and their count. `out` may pass preallocated (hit_q, hit_r) buffers of `capacity`
+    entries; the buffers grow (and the screen repeats) when the capacity is short."""
     L = _bind()
     while True:
-        hq, hr = DeviceArray(capacity, np.uint32), DeviceArray(capacity, np.uint32)
+        if out is None or out[0].shape[0] < capacity:
+            out = (DeviceArray(capacity, np.uint32), DeviceArray(capacity, np.uint32))
+        hq, hr = out
         count = ctypes.c_uint64()
         st = Status()
         rc = L.st_prefilter_screen(q_emb.ptr, q_norm.ptr, nq, r_emb.ptr, r_norm.ptr, nr, tau,
-                                   capacity, hq.ptr, hr.ptr, ctypes.byref(count), None,
+                                   hq.shape[0], hq.ptr, hr.ptr, ctypes.byref(count), None,
                                    ctypes.byref(st))
         check(rc, st, "st_prefilter_screen")
-        if count.value <= capacity:
+        if count.value <= hq.shape[0]:
             return hq, hr, int(count.value)
         capacity = int(count.value)
+        out = None
 
 
 def select_le(values: DeviceArray, n: int, cutoff: int) -> np.ndarray:
@@ -293,6 +301,9 @@ def train(model: Prefilter, data: TrainingData, epochs: int, batch_pairs: int = 
     max_len = max_len or int(lens.max())
     y_all = (data.score / SCORE_UNIT).astype(np.float32)
     w_all = pair_weights(data.score)
+    from .synthetic import padded
+
+    table = padded(data.seqs_codes, data.seqs_off, max_len)
     n = len(data.pair_a)
     steps = (n + batch_pairs - 1) // batch_pairs
     total = epochs * steps
@@ -312,10 +323,10 @@ def train(model: Prefilter, data: TrainingData, epochs: int, batch_pairs: int = 
             a, b = data.pair_a[sel], data.pair_b[sel]
             uniq, inv = np.unique(np.concatenate([a, b]), return_inverse=True)
             ia, ib = inv[:len(sel)], inv[len(sel):]
-            sl = [data.seqs_codes[int(data.seqs_off[u]):int(data.seqs_off[u + 1])] for u in uniq]
+            ul = lens[uniq]
             off = np.zeros(len(uniq) + 1, np.uint64)
-            off[1:] = np.cumsum([len(x) for x in sl])
-            codes = np.concatenate(sl).astype(np.uint8)
+            off[1:] = np.cumsum(ul)
+            codes = table[uniq][np.arange(max_len)[None, :] < ul[:, None]]
             _upload(d_codes, codes)
             _upload(d_off, off)
             for dst, src in ((d_pa, ia.astype(np.uint32)), (d_pb, ib.astype(np.uint32)),
@@ -374,26 +385,71 @@ class SearchResult:
     seconds: dict
 
 
+class Searcher:
+    """Thresholded gap-block search through the prefilter, with device buffers kept across
+    calls. ``index`` embeds the references once; ``query`` embeds a query set, screens it
+    against the index, rescores the candidates exactly and returns the hits <= cutoff."""
+
+    def __init__(self, model: Prefilter, gp: GapblockParams, max_len: int,
+                 capacity: int = 1 << 24, hit_capacity: int = 1 << 21):
+        self.model, self.gp, self.max_len = model, gp, max_len
+        self._cand = None
+        self._cap = capacity
+        self._scores = None
+        self._hits = None
+        self._hit_cap = hit_capacity
+        self.refs = None
+        self._count = ctypes.c_uint64()
+
+    def index(self, r: DeviceSet) -> float:
+        t0 = time.perf_counter()
+        self.refs = r
+        self.r_emb, self.r_norm = self.model.encode(r, self.max_len)
+        _device.synchronize()
+        return time.perf_counter() - t0
+
+    def query(self, q: DeviceSet, cutoff: int, tau_score: float) -> SearchResult:
+        L = _bind()
+        t = {}
+        t0 = time.perf_counter()
+        qe, qn = self.model.encode(q, self.max_len)
+        _device.synchronize()
+        t["encode_queries"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        cq, cr, nc = screen(qe, qn, q.n, self.r_emb, self.r_norm, self.refs.n,
+                            tau_score / SCORE_UNIT, self._cap, self._cand)
+        self._cand, self._cap = (cq, cr), cq.shape[0]
+        t["screen"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        if self._scores is None or self._scores.shape[0] < max(nc, 1):
+            self._scores = DeviceArray(max(nc, self._cap), np.int32)
+        scores = exact_pairs(q, self.refs, cq, cr, nc, self.gp, self._scores)
+        while True:
+            if self._hits is None or self._hits[0].shape[0] < self._hit_cap:
+                self._hits = (DeviceArray(self._hit_cap, np.uint32),
+                              DeviceArray(self._hit_cap, np.uint32),
+                              DeviceArray(self._hit_cap, np.int32))
+            hq, hr, hs = self._hits
+            st = Status()
+            check(L.st_select_pairs_le(scores.ptr, cq.ptr, cr.ptr, nc, cutoff, self._hit_cap,
+                                       hq.ptr, hr.ptr, hs.ptr, ctypes.byref(self._count), None,
+                                       ctypes.byref(st)), st, "st_select_pairs_le")
+            k = int(self._count.value)
+            if k <= self._hit_cap:
+                break
+            self._hit_cap = k
+        t["rescore"] = time.perf_counter() - t0
+        return SearchResult(hq.numpy()[:k], hr.numpy()[:k], hs.numpy()[:k], nc, t)
+
+
 def search(model: Prefilter, q: DeviceSet, r: DeviceSet, cutoff: int, tau_score: float,
-           gp: GapblockParams, max_len: int, capacity: int = 1 << 26) -> SearchResult:
-    """Embed, screen at tau (score units), rescore the candidates exactly, keep <= cutoff."""
-    t = {}
-    t0 = time.perf_counter()
-    qe, qn = model.encode(q, max_len)
-    re_, rn = model.encode(r, max_len)
-    _device.synchronize()
-    t["encode"] = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    cq, cr, nc = screen(qe, qn, q.n, re_, rn, r.n, tau_score / SCORE_UNIT, capacity)
-    t["screen"] = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    scores = exact_pairs(q, r, cq, cr, nc, gp)
-    keep = select_le(scores, nc, cutoff)
-    t["rescore"] = time.perf_counter() - t0
-    hq = cq.numpy()[:nc][keep] if nc else np.zeros(0, np.uint32)
-    hr = cr.numpy()[:nc][keep] if nc else np.zeros(0, np.uint32)
-    hs = scores.numpy()[:nc][keep] if nc else np.zeros(0, np.int32)
-    return SearchResult(hq, hr, hs, nc, t)
+           gp: GapblockParams, max_len: int) -> SearchResult:
+    """One-shot search: index the references, then query (see Searcher)."""
+    s = Searcher(model, gp, max_len)
+    t_index = s.index(r)
+    res = s.query(q, cutoff, tau_score)
+    res.seconds["encode_refs"] = t_index
+    return res
 
 
 def recall(found_q, found_r, exact_q, exact_r) -> float:

@@ -53,29 +53,60 @@ def join(seqs: list[np.ndarray]):
 def mutate(rng: np.random.Generator, s: np.ndarray, lo: int, hi: int) -> np.ndarray:
     """1-3 point edits inside the body (substitution 70%, insertion 15%, deletion 15%),
     keeping the length within [lo, hi] and the 'C' / 'F' ends."""
-    t = list(s)
-    for _ in range(int(rng.integers(1, 4))):
-        op = rng.random()
-        body = len(t) - 2
-        if op < 0.15 and len(t) < hi:
-            p = 1 + int(rng.integers(0, body + 1))
-            t.insert(p, int(_BG_CODES[rng.choice(20, p=BG_FREQ)]))
-        elif op < 0.30 and len(t) > lo and body > 0:
-            del t[1 + int(rng.integers(0, body))]
-        elif body > 0:
-            t[1 + int(rng.integers(0, body))] = int(_BG_CODES[rng.choice(20, p=BG_FREQ)])
-    return np.array(t, np.uint8)
+    return _mutate_many(rng, [s], 1, lo, hi)[0]
 
 
-def families(rng: np.random.Generator, n_families: int, per_family: int, lo: int, hi: int):
+def _mutate_many(rng: np.random.Generator, roots: list[np.ndarray], per: int, lo: int,
+                 hi: int) -> list[np.ndarray]:
+    """`per` mutants of every root (see mutate), with the random draws made in bulk."""
+    n = len(roots) * per
+    k = rng.integers(1, 4, size=n)
+    op = rng.random((n, 3))
+    pos = rng.random((n, 3))
+    res = _BG_CODES[rng.choice(20, size=(n, 3), p=BG_FREQ)].tolist()
+    k, op, pos = k.tolist(), op.tolist(), pos.tolist()
+    out = []
+    j = 0
+    for r in roots:
+        base = r.tolist()
+        for _ in range(per):
+            t = list(base)
+            for e in range(k[j]):
+                body = len(t) - 2
+                o = op[j][e]
+                if o < 0.15 and len(t) < hi:
+                    t.insert(1 + int(pos[j][e] * (body + 1)), res[j][e])
+                elif o < 0.30 and len(t) > lo and body > 0:
+                    del t[1 + int(pos[j][e] * body)]
+                elif body > 0:
+                    t[1 + int(pos[j][e] * body)] = res[j][e]
+            out.append(np.array(t, np.uint8))
+            j += 1
+    return out
+
+
+def families(rng: np.random.Generator, n_families: int, per_family: int, lo: int, hi: int,
+             roots_out: list | None = None):
     """n_families roots, each with per_family members derived by mutate(): (codes, offsets,
-    family id per sequence). Members of a family are consecutive."""
+    family id per sequence). Members of a family are consecutive; the roots themselves are
+    appended to roots_out when given."""
     rc, ro, _ = cdr3_csr(rng, n_families, lo, hi)
     roots = split(rc, ro)
-    seqs = [mutate(rng, r, lo, hi) for r in roots for _ in range(per_family)]
+    seqs = _mutate_many(rng, roots, per_family, lo, hi)
     fam = np.repeat(np.arange(n_families, dtype=np.int64), per_family)
     codes, off = join(seqs)
+    if roots_out is not None:
+        roots_out.extend(roots)
     return codes, off, fam
+
+
+def padded(codes: np.ndarray, off: np.ndarray, width: int) -> np.ndarray:
+    """Sequences as rows of a [n][width] array (zero past each length)."""
+    lens = np.diff(off.astype(np.int64))
+    out = np.zeros((len(lens), width), np.uint8)
+    mask = np.arange(width)[None, :] < lens[:, None]
+    out[mask] = codes[:int(off[-1])]
+    return out
 
 
 def decode(codes: np.ndarray, off: np.ndarray) -> list[str]:

@@ -327,3 +327,41 @@ def test_batch_encoded_set_validation():
         _encode(EncodedSet(np.array([0, 1], np.uint8), np.array([0, 3], np.uint64)), syms)
     codes, off = _encode(EncodedSet(np.zeros(0, np.uint8), np.zeros(1, np.uint64)), syms)
     assert list(off) == [0]
+
+
+def test_synthetic_generators_are_seeded_and_shaped():
+    from seqtree_amd import synthetic as syn
+
+    a = syn.families(np.random.default_rng(1), 50, 4, 10, 20)
+    b = syn.families(np.random.default_rng(1), 50, 4, 10, 20)
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    codes, off, fam = a
+    lens = np.diff(off.astype(np.int64))
+    assert lens.min() >= 10 and lens.max() <= 20 and len(fam) == 200
+    seqs = syn.decode(codes, off)
+    assert all(s[0] == "C" and s[-1] == "F" for s in seqs)
+    assert set("".join(seqs)) <= set(syn.BG_RESIDUES)
+    c2, o2, l2 = syn.cdr3_csr(np.random.default_rng(2), 100, 6, 30)
+    assert np.array_equal(np.diff(o2.astype(np.int64)), l2) and l2.min() >= 6 and l2.max() <= 30
+    P = syn.padded(c2, o2, 30)
+    assert np.array_equal(P[3, :l2[3]], c2[int(o2[3]):int(o2[4])]) and not P[3, l2[3]:].any()
+
+
+def test_prefilter_pair_sampling_and_calibration():
+    from seqtree_amd import prefilter as pf
+
+    fam = np.repeat(np.arange(100), 4)
+    a, b = pf.sample_pairs(np.random.default_rng(3), fam, 10_000, 0.5)
+    near = slice(0, 5000)
+    assert (fam[a[near]] == fam[b[near]]).all() and (a[near] != b[near]).all()
+    assert ((a >= 0) & (a < 400) & (b >= 0) & (b < 400)).all()
+    # the screening threshold keeps the requested share of close pairs, never below cutoff
+    exact = np.array([10, 20, 30, 40, 200, 300])
+    model = np.array([12.0, 35.0, 29.0, 90.0, 80.0, 400.0])
+    assert pf.calibrate_tau(model, exact, 45, quantile=1.0) == 90.0
+    assert pf.calibrate_tau(model, exact, 45, quantile=0.0) == 45.0
+    assert pf.calibrate_tau(model, exact, 5) == 5.0
+    w = pf.pair_weights(np.array([0.0, 50.0, 100.0]))
+    assert w[0] > w[1] > w[2] > 0
+    assert pf.recall(np.array([1, 2]), np.array([3, 4]), np.array([1, 2, 5]),
+                     np.array([3, 9, 9])) == 1 / 3
