@@ -2,12 +2,13 @@
 //
 // A batch is many independent gap-block jobs (each its own query set, reference set,
 // penalty table, gap costs and prior) of uneven sizes. Every job's query rows are cut into
-// work units of `queries_per_unit` queries; a unit is the full row block of those queries
-// against all of the job's references. Units are numbered job by job, heaviest jobs
-// (largest reference sets) first, and handed out through one device atomic counter to a
-// fixed grid of workgroups that loop until the counter runs past the last unit — so the
-// launch shape does not depend on the batch, and the whole batch (reference packing,
-// counter reset, scoring) can be captured once in a hipGraph and replayed.
+// blocks of `queries_per_unit` queries and its reference chunks into groups small enough to
+// stay in L2; a work unit is one query block against one chunk group. Units are numbered job
+// by job (heaviest jobs first) and, inside a job, group-major, so the units in flight at one
+// time stream the same reference chunks. They are handed out through one device atomic
+// counter to a fixed grid of workgroups that loop until the counter runs past the last unit
+// -- so the launch shape does not depend on the batch, and the whole batch (reference
+// packing, counter reset, scoring) can be captured once in a hipGraph and replayed.
 //
 // Inside a unit the workgroup builds each query's penalty profile in LDS and sweeps the
 // job's chunk-sorted references exactly as the single-job kernel does (same per-pair
@@ -25,8 +26,8 @@ using gb::pair_score;
 using gb::pair_score_fast;
 
 template <bool PROFILE, bool EXACT, bool SHORT, uint32_t THREADS>
-__device__ void run_unit(const BatchJob& J, uint64_t q_begin, uint32_t qt, int32_t* lds,
-                         int32_t* stage) {
+__device__ void run_unit(const BatchJob& J, uint64_t q_begin, uint32_t qt, uint32_t c_begin,
+                         uint32_t c_end, int32_t* lds, int32_t* stage) {
     constexpr uint32_t kSlots = kChunk / THREADS;
     const uint32_t tid = threadIdx.x;
     const uint32_t A = J.A;
@@ -50,7 +51,7 @@ __device__ void run_unit(const BatchJob& J, uint64_t q_begin, uint32_t qt, int32
             __syncthreads();
         }
         const Look<PROFILE> look{lds, q};
-        for (uint32_t c = 0; c < J.refs.nchunks; ++c) {
+        for (uint32_t c = c_begin; c < c_end; ++c) {
 #pragma unroll
             for (uint32_t it = 0; it < kSlots; ++it) {
                 const uint32_t slot = c * kChunk + it * THREADS + tid;
@@ -112,15 +113,19 @@ __global__ __launch_bounds__(THREADS) void gapblock_batch_kernel(
         __syncthreads();  // every thread has read the unit before thread 0 takes the next
         if (u >= total_units) break;
         const BatchJob& J = jobs[j];
-        const uint64_t q_begin = (u - J.unit_begin) * qt;
+        const uint64_t local = u - J.unit_begin;
+        const uint32_t g = uint32_t(local / J.qblocks);
+        const uint64_t q_begin = (local % J.qblocks) * qt;
+        const uint32_t c0 = g * J.group;
+        const uint32_t c1 = c0 + J.group < J.refs.nchunks ? c0 + J.group : J.refs.nchunks;
         if (J.short_path) {
-            run_unit<true, false, true, THREADS>(J, q_begin, qt, lds, stage);
+            run_unit<true, false, true, THREADS>(J, q_begin, qt, c0, c1, lds, stage);
         } else if (J.profile) {
-            if (J.exact) run_unit<true, true, false, THREADS>(J, q_begin, qt, lds, stage);
-            else run_unit<true, false, false, THREADS>(J, q_begin, qt, lds, stage);
+            if (J.exact) run_unit<true, true, false, THREADS>(J, q_begin, qt, c0, c1, lds, stage);
+            else run_unit<true, false, false, THREADS>(J, q_begin, qt, c0, c1, lds, stage);
         } else {
-            if (J.exact) run_unit<false, true, false, THREADS>(J, q_begin, qt, lds, stage);
-            else run_unit<false, false, false, THREADS>(J, q_begin, qt, lds, stage);
+            if (J.exact) run_unit<false, true, false, THREADS>(J, q_begin, qt, c0, c1, lds, stage);
+            else run_unit<false, false, false, THREADS>(J, q_begin, qt, c0, c1, lds, stage);
         }
         __syncthreads();
     }

@@ -238,6 +238,23 @@ struct GapblockPlan {
     bool short_path;
 };
 
+// Reference chunks per gap-block workgroup: at most what keeps a group's packed words around
+// 512 KB (the workgroups resident together stream one group, see gapblock_kernel), at most
+// what leaves about four workgroups per CU in all, and balanced so the groups are equal.
+uint32_t chunk_group(uint64_t n_queries, uint32_t nchunks, uint32_t nwords) {
+    if (nchunks <= 1) return 1;
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const uint64_t units = n_queries * uint64_t(nchunks);
+    const uint64_t by_fill = std::max<uint64_t>(1, units / (uint64_t(cus) * 4));
+    const uint64_t chunk_bytes = uint64_t(kChunk) * (nwords + 1) * 4;
+    const uint64_t by_l2 = std::max<uint64_t>(1, (uint64_t(512) << 10) / chunk_bytes);
+    const uint64_t g0 = std::min<uint64_t>({uint64_t(nchunks), by_fill, by_l2});
+    const uint64_t ngroups = (nchunks + g0 - 1) / g0;
+    return uint32_t((nchunks + ngroups - 1) / ngroups);
+}
+
 // Short-launch eligibility (see gapblock_core.hpp) and its past-the-end prior value.
 // T bounds |prefix sum| of one diagonal; with D = P0 - Pd in [-2T, 2T], big = INT32_MAX - 2T
 // keeps D + big from wrapping, and 6T + prior_abs < INT32_MAX puts D + big above every real
@@ -296,17 +313,7 @@ GapblockPlan gapblock_plan(Arena& ar, const uint8_t* q, const uint64_t* q_off, u
     p.exact = worst >= int64_t(INT32_MAX);
     p.profile = size_t(q_max) * gb::kRowBytes <= 48 * 1024;  // one table row per position
     p.short_path = pack && short_launch(q_max, r_max, A, p.exact, pen_abs, prior_abs, &a.big);
-    // chunks per workgroup: about four workgroups per CU in all (one resident round at the
-    // occupancy these kernels reach), so the query tables are built as rarely as possible
-    a.group = 1;
-    if (pack && a.refs.nchunks > 1) {
-        int dev = 0, cus = 256;
-        if (hipGetDevice(&dev) == hipSuccess)
-            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        const uint64_t units = N * uint64_t(a.refs.nchunks);
-        const uint64_t target = uint64_t(cus) * 4;
-        a.group = uint32_t(std::min<uint64_t>(a.refs.nchunks, std::max<uint64_t>(1, units / target)));
-    }
+    a.group = pack ? chunk_group(N, a.refs.nchunks, a.refs.nwords) : 1;
     if (p.short_path) {
         a.prof_rows = q_max + gb::kFastMaxL;
         a.slice_rows = r_max + 1;
@@ -418,7 +425,9 @@ struct BatchPlanImpl {
         uint64_t u = 0;
         for (auto& j : dev_jobs) {
             j.unit_begin = u;
-            u += (j.n_queries + qt - 1) / qt;
+            j.qblocks = uint32_t((j.n_queries + qt - 1) / qt);
+            const uint32_t ngroups = (j.refs.nchunks + j.group - 1) / j.group;
+            u += uint64_t(j.qblocks) * ngroups;
         }
         total_units = u;
         h2d(d_jobs, dev_jobs.data(), dev_jobs.size(), s);
@@ -1111,17 +1120,21 @@ int32_t st_prefilter_encode(const st_prefilter* model, const uint8_t* codes,
         // blocks of about 2^20 rows (256 MB per 128-channel activation buffer)
         const uint32_t blk = uint32_t(std::min<uint64_t>(n, std::max<uint64_t>(1024, (uint64_t(1) << 20) / (max_len + 2))));
         const uint64_t Rb = (uint64_t(blk) * (max_len + 2) + 127) / 128 * 128;
-        std::vector<uint64_t> hoff(n + 1);
-        d2h(hoff.data(), offsets, n + 1, s);
-        sync(s);
+        // the offsets at the block boundaries (blocks of blk sequences, then n)
+        const uint32_t nb = (n + blk - 1) / blk;
         Arena ar(s);
+        uint64_t* dbound = ar.get<uint64_t>(nb + 1);
+        pf::launch_gather_offsets(offsets, blk, n, nb + 1, dbound, s);
+        std::vector<uint64_t> bound(nb + 1);
+        d2h(bound.data(), dbound, nb + 1, s);
+        sync(s);
         const PfActs x0 = pf_acts(ar, Rb, pf::kEmbDim, s), ha = pf_acts(ar, Rb, pf::kChannels, s),
                      hb = pf_acts(ar, Rb, pf::kChannels, s);
         uint8_t* valid = ar.get<uint8_t>(Rb);
         __bf16* out = static_cast<__bf16*>(emb);
         for (uint32_t s0 = 0; s0 < n; s0 += blk) {
             const uint32_t cnt = std::min(blk, n - s0);
-            const PfRows rows = pf_rows(hoff[s0], hoff[s0 + cnt], cnt);
+            const PfRows rows = pf_rows(bound[s0 / blk], bound[s0 / blk + 1], cnt);
             if (rows.R > Rb)
                 throw HipError{ST_INVALID_ARGUMENT,
                                "seqtree_hip: a sequence is longer than max_len"};
@@ -1448,6 +1461,13 @@ int32_t st_gapblock_batch_plan_create(const st_gapblock_job* jobs, uint32_t n_jo
             const int64_t Lmax = std::max(j.q_max_len, j.r_max_len);
             d.exact = 2 * Lmax * pen_abs + prior_abs + int64_t(j.gap_open) +
                           Lmax * int64_t(j.gap_extend) >= int64_t(INT32_MAX);
+            {  // chunk groups that stay in L2 (see chunk_group), balanced
+                const uint64_t chunk_bytes = uint64_t(kChunk) * (pr.nwords + 1) * 4;
+                const uint64_t g0 = std::min<uint64_t>(
+                    pr.nchunks, std::max<uint64_t>(1, (uint64_t(512) << 10) / chunk_bytes));
+                const uint64_t ng = (pr.nchunks + g0 - 1) / g0;
+                d.group = uint32_t((pr.nchunks + ng - 1) / ng);
+            }
             d.profile = size_t(j.q_max_len) * gb::kRowBytes <= 48 * 1024;
             lds = std::max(lds, size_t(d.profile ? j.q_max_len : A) * gb::kRowBytes);
             int32_t big = 0;

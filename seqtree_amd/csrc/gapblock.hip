@@ -51,17 +51,19 @@ __device__ __forceinline__ void append_hit(bool hit, uint32_t qi, uint32_t k, in
 // SHORT: the short-launch layout (requires PROFILE && !EXACT): profile rows, then the
 // prior slice, both in the dynamic LDS.
 // A workgroup owns query i and chunks [g * group, min((g + 1) * group, nchunks)): the
-// query's tables are built once and reused across the group's chunks.
+// query's tables are built once and reused across the group's chunks. Workgroups are
+// numbered group-major (all queries of group 0 first), so the workgroups resident at one
+// time stream the same few reference chunks and find them in L2.
 template <bool PROFILE, bool EXACT, bool THRESH, bool SHORT>
-__global__ __launch_bounds__(kThreads) void gapblock_kernel(GapblockArgs a, uint64_t q_base) {
+__global__ __launch_bounds__(kThreads) void gapblock_kernel(GapblockArgs a, uint64_t q_base,
+                                                            uint32_t nq) {
     extern __shared__ int32_t lds[];
     __shared__ int32_t stage[THRESH ? 1 : kChunk];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t nch = a.refs.nchunks;
-    const uint32_t ngroups = (nch + a.group - 1) / a.group;
-    const uint32_t g = blockIdx.x % ngroups;
-    const uint64_t i = q_base + blockIdx.x / ngroups;
+    const uint32_t g = blockIdx.x / nq;
+    const uint64_t i = q_base + blockIdx.x % nq;
     const uint32_t c_end = (g + 1) * a.group < nch ? (g + 1) * a.group : nch;
     const uint64_t q0 = a.q_off[i];
     const uint32_t m = uint32_t(a.q_off[i + 1] - q0);
@@ -150,7 +152,8 @@ void launch_variant(const GapblockArgs& a, size_t lds_bytes, hipStream_t s) {
     for (uint64_t q0 = 0; q0 < a.n_queries; q0 += max_q) {
         const uint64_t nq = a.n_queries - q0 < max_q ? a.n_queries - q0 : max_q;
         hipLaunchKernelGGL((gapblock_kernel<PROFILE, EXACT, THRESH, SHORT>),
-                           dim3(uint32_t(nq * ngroups)), dim3(kThreads), lds_bytes, s, a, q0);
+                           dim3(uint32_t(nq * ngroups)), dim3(kThreads), lds_bytes, s, a, q0,
+                           uint32_t(nq));
     }
 }
 

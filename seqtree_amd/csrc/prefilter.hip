@@ -29,6 +29,8 @@
 
 #include "st_internal.hpp"
 
+#include <algorithm>
+
 namespace st {
 namespace pf {
 namespace {
@@ -74,7 +76,6 @@ __global__ __launch_bounds__(256) void embed_kernel(const uint8_t* __restrict__ 
 // EPI 1 (backward data, hidden layer): zero where mask[r][n] <= 0 (mask = this layer's input
 //        activation, itself zero on invalid rows).
 // EPI 2 (backward data, embedding layer): zero on invalid rows.
-constexpr uint32_t kKChunk = 64;
 constexpr uint32_t kBPad = 8;  // bf16 elements of padding per LDS row
 
 // Software-pipelined: while the MFMAs consume K-chunk c (A fragments in registers, B staged in
@@ -89,7 +90,7 @@ __global__ __launch_bounds__(256) void conv_kernel(const __bf16* __restrict__ X,
                                                    __bf16* __restrict__ Y) {
     constexpr uint32_t K = 5 * CIN;
     constexpr uint32_t NB = N / 32;
-    constexpr uint32_t KC = K % kKChunk == 0 ? kKChunk : 32;
+    constexpr uint32_t KC = K % 128 == 0 ? 128 : (K % 80 == 0 ? 80 : 32);
     constexpr uint32_t NCH = K / KC;
     constexpr uint32_t KS = KC / 16;               // MFMA steps per chunk
     constexpr uint32_t PIECES = N * (KC / 8);      // 16-byte pieces of a B chunk
@@ -136,7 +137,9 @@ __global__ __launch_bounds__(256) void conv_kernel(const __bf16* __restrict__ X,
     load_a(0);
     store_b(0);
     __syncthreads();
-#pragma unroll 1
+    // fully unrolled, so the A fragments of consecutive chunks live in distinct registers and
+    // no copy of a value still in flight forces an early wait
+#pragma unroll
     for (uint32_t c = 0; c < NCH; ++c) {
 #pragma unroll
         for (uint32_t ks = 0; ks < KS; ++ks) acur[ks] = aq[ks];
@@ -181,29 +184,31 @@ __global__ __launch_bounds__(256) void conv_kernel(const __bf16* __restrict__ X,
 // ---- mean pool ---------------------------------------------------------------------------
 // One workgroup of kChannels threads per sequence. Writes the FP32 embedding, its BF16
 // rounding and the squared norm of the BF16 rounding (the operand the Gram GEMM sees).
-__global__ __launch_bounds__(kChannels) void pool_kernel(const __bf16* __restrict__ H,
-                                                         const uint64_t* __restrict__ off,
-                                                         float* __restrict__ ef,
-                                                         __bf16* __restrict__ eb,
-                                                         float* __restrict__ norm) {
-    __shared__ float part[kChannels / 64];
-    const uint32_t s = blockIdx.x, c = threadIdx.x;
-    const uint32_t len = uint32_t(off[s + 1] - off[s]);
-    const uint64_t r0 = start_row(off, s);
+constexpr uint32_t kPoolSeqs = 4;  // sequences per pooling workgroup (kChannels threads each)
+
+__global__ __launch_bounds__(kPoolSeqs * kChannels) void pool_kernel(
+    const __bf16* __restrict__ H, const uint64_t* __restrict__ off, uint32_t n,
+    float* __restrict__ ef, __bf16* __restrict__ eb, float* __restrict__ norm) {
+    __shared__ float parts[kPoolSeqs][kChannels / 64];
+    const uint32_t g = threadIdx.x / kChannels, c = threadIdx.x % kChannels;
+    const uint32_t s = blockIdx.x * kPoolSeqs + g;
+    const bool ok = s < n;  // every thread reaches the barrier below
+    const uint32_t len = ok ? uint32_t(off[s + 1] - off[s]) : 0u;
+    const uint64_t r0 = ok ? start_row(off, s) : 0;
     float sum = 0.f;
     for (uint32_t p = 0; p < len; ++p) sum += bf2f(H[(r0 + p) * kChannels + c]);
     const float v = len ? sum / float(len) : 0.f;
-    if (ef) ef[uint64_t(s) * kChannels + c] = v;
+    if (ok && ef) ef[uint64_t(s) * kChannels + c] = v;
     const __bf16 vb = f2bf(v);
-    if (eb) eb[uint64_t(s) * kChannels + c] = vb;
+    if (ok && eb) eb[uint64_t(s) * kChannels + c] = vb;
     float sq = bf2f(vb) * bf2f(vb);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
-    if ((c & 63u) == 0) part[c >> 6] = sq;
+    if ((c & 63u) == 0) parts[g][c >> 6] = sq;
     __syncthreads();
-    if (c == 0 && norm) {
+    if (ok && c == 0 && norm) {
         float t = 0.f;
-        for (uint32_t w = 0; w < kChannels / 64; ++w) t += part[w];
+        for (uint32_t w = 0; w < kChannels / 64; ++w) t += parts[g][w];
         norm[s] = t;
     }
 }
@@ -267,6 +272,7 @@ __global__ __launch_bounds__(256) void gram_kernel(const __bf16* __restrict__ Q,
                                                    uint32_t* hit_q, uint32_t* hit_r, uint64_t cap,
                                                    unsigned long long* count) {
     __shared__ __bf16 ring[3][kTileR][kRowPad];
+    __shared__ float ring_hr[3][kTileR];  // -|r|^2 / 2 of the tile's references (-inf past nr)
     __shared__ uint32_t queue_q[4][kWaveQueue], queue_r[4][kWaveQueue];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t li = lane & 31u, lk = lane >> 5;
@@ -296,87 +302,113 @@ __global__ __launch_bounds__(256) void gram_kernel(const __bf16* __restrict__ Q,
         }
     }
 
-    // thread tid stages pieces tid + 256 u of a tile: row (tid + 256 u) / 16, piece % 16
-    bf16x8 stage[kPiecesPerThread];
-    auto load_tile = [&](uint32_t t) {
+    // A tile in flight: thread tid holds pieces tid + 256 u (row (tid + 256 u) / 16, piece % 16)
+    // and, below kTileR, the row's raw norm (no arithmetic on a value still in flight).
+    struct Stage {
+        bf16x8 v[kPiecesPerThread];
+        float norm;
+        bool ok;
+    };
+    auto load_tile = [&](uint32_t t, Stage& st) {
 #pragma unroll
         for (uint32_t u = 0; u < kPiecesPerThread; ++u) {
             const uint32_t e = tid + 256 * u, row = e / 16, piece = e % 16;
             const uint32_t r = r_begin + t * kTileR + row;
-            stage[u] = (t < ntiles && r < r_end)
-                           ? *reinterpret_cast<const bf16x8*>(Rm + uint64_t(r) * kChannels +
-                                                              piece * 8)
-                           : bf16x8{};
+            st.v[u] = (t < ntiles && r < r_end)
+                          ? *reinterpret_cast<const bf16x8*>(Rm + uint64_t(r) * kChannels +
+                                                             piece * 8)
+                          : bf16x8{};
+        }
+        if (tid < kTileR) {
+            const uint32_t r = r_begin + t * kTileR + tid;
+            st.ok = t < ntiles && r < r_end;
+            st.norm = r_norm[st.ok ? r : 0];
         }
     };
-    auto store_tile = [&](uint32_t buf) {
+    auto store_tile = [&](uint32_t buf, const Stage& st) {
 #pragma unroll
         for (uint32_t u = 0; u < kPiecesPerThread; ++u) {
             const uint32_t e = tid + 256 * u, row = e / 16, piece = e % 16;
-            *reinterpret_cast<bf16x8*>(&ring[buf][row][piece * 8]) = stage[u];
+            *reinterpret_cast<bf16x8*>(&ring[buf][row][piece * 8]) = st.v[u];
+        }
+        if (tid < kTileR) ring_hr[buf][tid] = st.ok ? -0.5f * st.norm : -__builtin_inff();
+    };
+    auto compute_tile = [&](uint32_t t, uint32_t buf) {
+        if (!active) return;
+#pragma unroll
+        for (uint32_t half = 0; half < 2; ++half) {
+            // accumulators start at -hr - hq (every accumulator of a lane is in column li,
+            // element e in row (e / 4) * 8 + lk * 4 + e % 4), so a hit is acc >= 0 and a
+            // tile without hits is found with a max tree and one compare
+            const uint32_t r = r_begin + t * kTileR + half * 32 + li;
+            const float hrv = ring_hr[buf][half * 32 + li];
+            f32x16 acc0, acc1;
+#pragma unroll
+            for (uint32_t e = 0; e < 16; ++e) {
+                acc0[e] = hrv - h[0][e];
+                acc1[e] = hrv - h[1][e];
+            }
+#pragma unroll
+            for (uint32_t ks = 0; ks < 8; ++ks) {
+                const bf16x8 b =
+                    *reinterpret_cast<const bf16x8*>(&ring[buf][half * 32 + li][ks * 16 + 8 * lk]);
+                acc0 = mfma(a[0][ks], b, acc0);
+                acc1 = mfma(a[1][ks], b, acc1);
+            }
+            float m = acc0[0];
+#pragma unroll
+            for (uint32_t e = 1; e < 16; ++e) m = fmaxf(m, acc0[e]);
+#pragma unroll
+            for (uint32_t e = 0; e < 16; ++e) m = fmaxf(m, acc1[e]);
+            if (!__any(m >= 0.f)) continue;
+            uint32_t bits = 0;  // bit b * 16 + e: accumulator e of block b is a hit
+#pragma unroll
+            for (uint32_t e = 0; e < 16; ++e) {
+                bits |= uint32_t(acc0[e] >= 0.f) << e;
+                bits |= uint32_t(acc1[e] >= 0.f) << (16 + e);
+            }
+            uint32_t total = 0;
+            const uint32_t pre = wave_scan(__popc(bits), &total);
+            if (qn + total > kWaveQueue && qn) {
+                flush_queue(qq, qr, qn, hit_q, hit_r, cap, count);
+                qn = 0;
+            }
+            unsigned long long gb = 0;
+            const bool direct = total > kWaveQueue;
+            if (direct) {
+                if (lane == 0) gb = atomicAdd(count, (unsigned long long)total);
+                gb = __shfl(gb, 0);
+            }
+            uint32_t k = pre;
+            while (bits) {
+                const uint32_t bit = __ffs(bits) - 1;
+                bits &= bits - 1;
+                const uint32_t e = bit & 15u, blk = bit >> 4;
+                const uint32_t qa = q0 + blk * 32 + (e / 4) * 8 + lk * 4 + (e % 4);
+                if (!direct) {
+                    qq[qn + k] = qa;
+                    qr[qn + k] = r;
+                } else if (gb + k < cap) {
+                    hit_q[gb + k] = qa;
+                    hit_r[gb + k] = r;
+                }
+                ++k;
+            }
+            if (!direct) qn += total;
         }
     };
-
-    load_tile(0);
-    store_tile(0);
-    load_tile(1);
+    // Three-buffer ring: iteration t stores tile t + 1 (loaded during iteration t - 1),
+    // issues the loads of tile t + 2 and computes tile t, with one barrier.
+    Stage st;
+    load_tile(0, st);
+    store_tile(0, st);
+    load_tile(1, st);
     __syncthreads();
 #pragma unroll 1
     for (uint32_t t = 0; t < ntiles; ++t) {
-        const uint32_t buf = t % 3;
-        store_tile((t + 1) % 3);  // tile t + 1 (zeros past the end)
-        load_tile(t + 2);
-        if (active) {
-#pragma unroll
-            for (uint32_t half = 0; half < 2; ++half) {
-                f32x16 acc0 = f32x16{0.f}, acc1 = f32x16{0.f};
-#pragma unroll
-                for (uint32_t ks = 0; ks < 8; ++ks) {
-                    const bf16x8 b = *reinterpret_cast<const bf16x8*>(
-                        &ring[buf][half * 32 + li][ks * 16 + 8 * lk]);
-                    acc0 = mfma(a[0][ks], b, acc0);
-                    acc1 = mfma(a[1][ks], b, acc1);
-                }
-                const uint32_t r = r_begin + t * kTileR + half * 32 + li;
-                const float hrv = r < r_end ? 0.5f * r_norm[r] : __builtin_inff();
-                uint32_t bits = 0;  // bit b * 16 + e: accumulator e of block b is a hit
-#pragma unroll
-                for (uint32_t e = 0; e < 16; ++e) {
-                    bits |= uint32_t(acc0[e] >= h[0][e] + hrv) << e;
-                    bits |= uint32_t(acc1[e] >= h[1][e] + hrv) << (16 + e);
-                }
-                if (__any(bits != 0)) {
-                    uint32_t total = 0;
-                    const uint32_t pre = wave_scan(__popc(bits), &total);
-                    if (qn + total > kWaveQueue && qn) {
-                        flush_queue(qq, qr, qn, hit_q, hit_r, cap, count);
-                        qn = 0;
-                    }
-                    unsigned long long gb = 0;
-                    const bool direct = total > kWaveQueue;
-                    if (direct) {
-                        if (lane == 0) gb = atomicAdd(count, (unsigned long long)total);
-                        gb = __shfl(gb, 0);
-                    }
-                    uint32_t k = pre;
-                    while (bits) {
-                        const uint32_t bit = __ffs(bits) - 1;
-                        bits &= bits - 1;
-                        const uint32_t e = bit & 15u, blk = bit >> 4;
-                        const uint32_t qa = q0 + blk * 32 + (e / 4) * 8 + lk * 4 + (e % 4);
-                        if (!direct) {
-                            qq[qn + k] = qa;
-                            qr[qn + k] = r;
-                        } else if (gb + k < cap) {
-                            hit_q[gb + k] = qa;
-                            hit_r[gb + k] = r;
-                        }
-                        ++k;
-                    }
-                    if (!direct) qn += total;
-                }
-            }
-        }
+        store_tile((t + 1) % 3, st);
+        load_tile(t + 2, st);
+        compute_tile(t, t % 3);
         __syncthreads();
     }
     if (qn) flush_queue(qq, qr, qn, hit_q, hit_r, cap, count);
@@ -552,29 +584,34 @@ __global__ __launch_bounds__(256) void pack_weights_kernel(const float* __restri
     }
 }
 
-// Entries t < n with v[t] <= cutoff, in no particular order (one atomic per wave): their
-// index, or with pa / pb given the gathered (pa[t], pb[t], v[t]).
+// Entries t < n with v[t] <= cutoff, in no particular order: their index, or with pa / pb
+// given the gathered (pa[t], pb[t], v[t]). Each workgroup scans a contiguous range, collects
+// its entries in LDS and claims space in the output with one global atomic per LDS-full.
+constexpr uint32_t kSelQueue = 1024;
+
 __global__ __launch_bounds__(256) void select_le_kernel(const int32_t* __restrict__ v, uint64_t n,
-                                                        int32_t cutoff,
+                                                        int32_t cutoff, uint64_t per_block,
                                                         const uint32_t* __restrict__ pa,
                                                         const uint32_t* __restrict__ pb,
                                                         uint32_t* __restrict__ out_a,
                                                         uint32_t* __restrict__ out_b,
                                                         int32_t* __restrict__ out_v, uint64_t cap,
                                                         unsigned long long* __restrict__ count) {
-    const uint64_t stride = uint64_t(gridDim.x) * 256;
-    for (uint64_t t0 = uint64_t(blockIdx.x) * 256; t0 < n; t0 += stride) {
-        const uint64_t t = t0 + threadIdx.x;
-        const bool hit = t < n && v[t] <= cutoff;
-        const unsigned long long mask = __ballot(hit);
-        if (mask == 0) continue;
-        const uint32_t lane = __lane_id();
-        const uint32_t leader = __ffsll(mask) - 1;
-        unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(mask));
-        base = __shfl(base, leader);
-        if (hit) {
-            const unsigned long long k = base + __popcll(mask & ((1ull << lane) - 1));
+    __shared__ uint64_t qidx[kSelQueue];
+    __shared__ uint32_t qn;
+    __shared__ unsigned long long base;
+    const uint32_t tid = threadIdx.x;
+    const uint64_t b0 = uint64_t(blockIdx.x) * per_block;
+    const uint64_t b1 = b0 + per_block < n ? b0 + per_block : n;
+    if (tid == 0) qn = 0;
+    __syncthreads();
+    auto flush = [&]() {  // all threads; qn <= kSelQueue
+        const uint32_t m = qn;
+        if (tid == 0) base = atomicAdd(count, (unsigned long long)m);
+        __syncthreads();
+        for (uint32_t i = tid; i < m; i += 256) {
+            const unsigned long long k = base + i;
+            const uint64_t t = qidx[i];
             if (k < cap) {
                 if (pa) {
                     out_a[k] = pa[t];
@@ -585,19 +622,48 @@ __global__ __launch_bounds__(256) void select_le_kernel(const int32_t* __restric
                 }
             }
         }
+        __syncthreads();
+        if (tid == 0) qn = 0;
+        __syncthreads();
+    };
+    for (uint64_t t0 = b0; t0 < b1; t0 += 256) {
+        const uint64_t t = t0 + tid;
+        if (t < b1 && v[t] <= cutoff) {
+            const uint32_t slot = atomicAdd(&qn, 1u);  // LDS atomic
+            if (slot < kSelQueue) qidx[slot] = t;
+        }
+        __syncthreads();
+        if (qn > kSelQueue - 256) flush();  // keep room for one more round of 256
     }
+    __syncthreads();
+    if (qn) flush();
+}
+
+// dst[i] = src[min(i * stride, last)] for i < count (block boundaries of an offsets array).
+__global__ void gather_offsets_kernel(const uint64_t* __restrict__ src, uint32_t stride,
+                                      uint32_t last, uint32_t count, uint64_t* __restrict__ dst) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < count) dst[i] = src[uint64_t(i) * stride < last ? uint64_t(i) * stride : last];
 }
 
 }  // namespace
 
 // ---- launchers ------------------------------------------------------------------------------
+void launch_gather_offsets(const uint64_t* src, uint32_t stride, uint32_t last, uint32_t count,
+                           uint64_t* dst, hipStream_t s) {
+    if (count == 0) return;
+    hipLaunchKernelGGL(gather_offsets_kernel, dim3((count + 255) / 256), dim3(256), 0, s, src,
+                       stride, last, count, dst);
+}
 void launch_select_le(const int32_t* v, uint64_t n, int32_t cutoff, const uint32_t* pa,
                       const uint32_t* pb, uint32_t* out_a, uint32_t* out_b, int32_t* out_v,
                       uint64_t cap, unsigned long long* count, hipStream_t s) {
     if (n == 0) return;
-    const uint64_t blocks = (n + 255) / 256;
-    hipLaunchKernelGGL(select_le_kernel, dim3(uint32_t(blocks < 65536 ? blocks : 65536)), dim3(256),
-                       0, s, v, n, cutoff, pa, pb, out_a, out_b, out_v, cap, count);
+    // about 2048 workgroups, each a contiguous multiple of 256 entries
+    const uint64_t per = std::max<uint64_t>(256, ((n + 2047) / 2048 + 255) / 256 * 256);
+    const uint64_t blocks = (n + per - 1) / per;
+    hipLaunchKernelGGL(select_le_kernel, dim3(uint32_t(blocks)), dim3(256), 0, s, v, n, cutoff, per,
+                       pa, pb, out_a, out_b, out_v, cap, count);
 }
 void launch_embed(const uint8_t* codes, const uint64_t* off, uint32_t n, const __bf16* emb,
                   __bf16* X0, uint8_t* valid, hipStream_t s) {
@@ -628,7 +694,8 @@ void launch_conv(int layer, int epi, const __bf16* X, const __bf16* Bt, const fl
 void launch_pool(const __bf16* H, const uint64_t* off, uint32_t n, float* ef, __bf16* eb,
                  float* norm, hipStream_t s) {
     if (n == 0) return;
-    hipLaunchKernelGGL(pool_kernel, dim3(n), dim3(kChannels), 0, s, H, off, ef, eb, norm);
+    hipLaunchKernelGGL(pool_kernel, dim3((n + kPoolSeqs - 1) / kPoolSeqs), dim3(kPoolSeqs * kChannels),
+                       0, s, H, off, n, ef, eb, norm);
 }
 
 void launch_gram(const __bf16* Q, const float* q_norm, uint32_t nq, const __bf16* Rm,

@@ -107,6 +107,8 @@ void launch_embed_bwd(const __bf16* dX0, const uint8_t* codes, const uint64_t* o
 void launch_adam(float* p, const float* g, float* m, float* v, uint32_t n, float lr, float b1,
                  float b2, float c1, float c2, float eps, hipStream_t s);
 void launch_pack_weights(const float* P, __bf16* B, hipStream_t s);
+void launch_gather_offsets(const uint64_t* src, uint32_t stride, uint32_t last, uint32_t count,
+                           uint64_t* dst, hipStream_t s);
 // Entries with v[t] <= cutoff: indices into out_a, or (pa, pb given) (pa[t], pb[t], v[t]).
 void launch_select_le(const int32_t* v, uint64_t n, int32_t cutoff, const uint32_t* pa,
                       const uint32_t* pb, uint32_t* out_a, uint32_t* out_b, int32_t* out_v,
@@ -182,6 +184,8 @@ struct BatchJob {
     uint32_t prof_rows;
     uint32_t slice_rows;
     int32_t big;
+    uint32_t group;        // reference chunks per work unit
+    uint32_t qblocks;      // query blocks (of queries_per_unit) per chunk group
     uint64_t unit_begin;   // index of the job's first work unit
 };
 // Tile shape of the persistent launch: queries per work unit, threads per workgroup

@@ -31,6 +31,9 @@ def _jobs(st, rng):
     jobs.append(GapblockJob([], cdr3_like(rng, 50, 6, 30), b62, 28, 1, None))
     jobs.append(GapblockJob([rand_seq(rng, 500, 650) for _ in range(3)],
                             [rand_seq(rng, 480, 700) for _ in range(40)], b62, 28, 1, None))
+    # enough references (20 chunks) for two chunk groups per query block
+    jobs.append(GapblockJob(cdr3_like(rng, 5, 6, 30), cdr3_like(rng, 20_000, 6, 30), b62, 28, 1,
+                            st.central_prior(21)))
     # a wrapping prior forces the exact evaluation order inside the batch
     jobs.append(GapblockJob(cdr3_like(rng, 20, 6, 16), cdr3_like(rng, 90, 6, 16), b62, 28, 1,
                             lambda i, d, n: (2**31 - 1) - 5 * i))

@@ -220,3 +220,24 @@ def test_short_launch_mixed_waves_and_chunk_groups(gpu, oracle_lib, prior):
     for i0 in range(0, len(q), 400):
         blk = np.asarray(st.score_matrix(q[i0:i0 + 7], r, m, 28, 1, gp))
         assert np.array_equal(full[i0:i0 + 7], blk)
+
+
+def test_many_reference_chunks_in_groups(gpu, oracle_lib):
+    """Enough references (40 chunks) that workgroups take chunk groups in group-major
+    order, checked against the oracle on sampled rows and against single-row calls."""
+    st = gpu
+    rng = random.Random(91)
+    m = st.SubstitutionMatrix.blosum62()
+    q = cdr3_like(rng, 300, 6, 20)
+    r = cdr3_like(rng, 40_000, 6, 20)
+    gp = st.central_prior(21)
+    full = np.asarray(st.score_matrix(q, r, m, 28, 1, gp))
+    rows = sorted(rng.sample(range(len(q)), 5))
+    sub = _check(st, oracle_lib, [q[i] for i in rows], r, m, 28, 1, gp)
+    assert np.array_equal(full[rows], sub)
+    for i in rng.sample(range(len(q)), 3):
+        assert np.array_equal(full[i:i + 1], np.asarray(st.score_matrix(q[i:i + 1], r, m, 28, 1, gp)))
+    hq, hr, hs = st.gapblock.score_threshold(q, r, 60, m, 28, 1, gp)
+    want = np.argwhere(full <= 60)
+    got = np.stack([hq, hr], 1)
+    assert np.array_equal(got, want) and np.array_equal(hs, full[full <= 60])
