@@ -147,6 +147,8 @@ class Lib:
             fn = getattr(L, name)
             fn.restype = i32
             fn.argtypes = [vp, vp, u64, u32, vp, vp, u64, u32, vp, vp, vp]
+        L.st_probe_mfma.restype = i32
+        L.st_probe_mfma.argtypes = [ctypes.POINTER(ctypes.c_double), vp]
         L.st_probe_valu.restype = i32
         L.st_probe_valu.argtypes = [i32, ctypes.c_char_p, u64, ctypes.POINTER(ctypes.c_double),
                                     vp]
@@ -709,6 +711,11 @@ def main() -> int:
         line = run_distance(torch, dev, lib, stream, args, args.workload)
     if line.get("roofline") and line["roofline"]["bound"] == "valu":
         line["roofline"]["probe_tops"] = valu_probe(lib)
+    if line.get("roofline") and line["roofline"]["bound"] == "mfma":
+        rate = ctypes.c_double()
+        st = Status()
+        if lib.lib.st_probe_mfma(ctypes.byref(rate), ctypes.byref(st)) == 0:
+            line["roofline"]["probe_tflops_bf16_32x32x16"] = round(rate.value / 1e12, 1)
     print(json.dumps(line))
     return 0
 

@@ -289,6 +289,9 @@ int32_t st_timing_collect(char* report, uint64_t capacity);
  * is ST_INVALID_ARGUMENT. */
 int32_t st_probe_valu(int32_t kind, char* name, uint64_t name_capacity, double* lane_ops_per_s,
                       st_status* status);
+/* Measured rate of v_mfma_f32_32x32x16_bf16 (four independent chains per wave, registers
+ * only) over the whole device, in FLOP/s: the attainable BF16 MFMA peak. */
+int32_t st_probe_mfma(double* flops_per_s, st_status* status);
 
 #ifdef __cplusplus
 }

@@ -1176,6 +1176,46 @@ int32_t st_prefilter_screen(const void* q_emb, const float* q_norm, uint32_t n_q
     });
 }
 
+int32_t st_probe_mfma(double* flops_per_s, st_status* status) {
+    return guarded(status, [&] {
+        require_device();
+        int dev = 0, cus = 0;
+        check(hipGetDevice(&dev), "hipGetDevice");
+        check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+              "hipDeviceGetAttribute");
+        const uint32_t blocks = uint32_t(cus) * 16, iters = 2000;
+        float* out = nullptr;
+        check(hipMalloc(&out, size_t(blocks) * 256 * sizeof(float)), "hipMalloc");
+        hipEvent_t a = nullptr, b = nullptr;
+        float best = 0.f;
+        double flops = 0.0;
+        try {
+            check(hipEventCreate(&a), "hipEventCreate");
+            check(hipEventCreate(&b), "hipEventCreate");
+            for (int rep = 0; rep < 4; ++rep) {  // the first run is a warm-up
+                check(hipEventRecord(a, nullptr), "hipEventRecord");
+                flops = launch_probe_mfma(out, blocks, iters, nullptr);
+                check(hipGetLastError(), "probe launch");
+                check(hipEventRecord(b, nullptr), "hipEventRecord");
+                check(hipEventSynchronize(b), "hipEventSynchronize");
+                float ms = 0.f;
+                check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+                if (rep == 1 || (rep > 1 && ms < best)) best = ms;
+            }
+        } catch (...) {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+            (void)hipFree(out);
+            throw;
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        check(hipFree(out), "hipFree");
+        *flops_per_s = flops / (double(best) * 1e-3);
+        return ok(status);
+    });
+}
+
 void st_timing_enable(int32_t on) {
     std::lock_guard<std::mutex> lk(g_timing_mu);
     g_timing = on != 0;

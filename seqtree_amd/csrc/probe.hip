@@ -1,5 +1,5 @@
-// VALU issue-rate probe: the measured integer-VALU peak that the scoring kernels' rooflines
-// are quoted against.
+// Issue-rate probes: the measured integer-VALU peak that the scoring kernels' rooflines are
+// quoted against, and the BF16 MFMA rate the prefilter's GEMMs are compared with.
 //
 // Each thread runs 16 independent chains of one VALU instruction, written as inline assembly
 // so the opcode and operand kinds are exactly the ones named (registers only, no memory
@@ -37,6 +37,33 @@ ST_PROBE_KERNEL(probe_bfi_b32, "v_bfi_b32 %0, %0, %1, %2")
 
 #undef ST_PROBE_KERNEL
 
+// MFMA rate: four independent accumulator chains of v_mfma_f32_32x32x16_bf16 per wave.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__global__ __launch_bounds__(256) void probe_mfma_32x32x16_bf16(float* out, uint32_t iters) {
+    bf16x8 a, b;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        a[i] = __bf16(float(threadIdx.x + i) * 0.001f);
+        b[i] = __bf16(float(i) * 0.002f);
+    }
+    f32x16 acc[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[c] = f32x16{0.f};
+    for (uint32_t it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[c], 0, 0, 0);
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) s += acc[c][e];
+    out[size_t(blockIdx.x) * 256 + threadIdx.x] = s;
+}
+
 using ProbeKernel = void (*)(uint32_t*, uint32_t, uint32_t);
 constexpr ProbeKernel kProbes[] = {probe_add_u32, probe_max_i16, probe_max_i32,
                                    probe_pk_max_i16, probe_max3_i32, probe_bfi_b32};
@@ -51,6 +78,11 @@ const char* probe_name(int kind) {
 
 void launch_probe(int kind, uint32_t* out, uint32_t blocks, uint32_t iters, hipStream_t s) {
     hipLaunchKernelGGL(kProbes[kind], dim3(blocks), dim3(256), 0, s, out, iters, 12345u);
+}
+
+double launch_probe_mfma(float* out, uint32_t blocks, uint32_t iters, hipStream_t s) {
+    hipLaunchKernelGGL(probe_mfma_32x32x16_bf16, dim3(blocks), dim3(256), 0, s, out, iters);
+    return double(blocks) * 4 /* waves */ * iters * 4 /* chains */ * 32.0 * 32 * 16 * 2;
 }
 
 }  // namespace st

@@ -120,6 +120,8 @@ void launch_select_le(const int32_t* v, uint64_t n, int32_t cutoff, const uint32
 // `iters` instructions of the named kind (probe_name returns null past the last kind).
 const char* probe_name(int kind);
 void launch_probe(int kind, uint32_t* out, uint32_t blocks, uint32_t iters, hipStream_t s);
+// v_mfma_f32_32x32x16_bf16 chains (256-thread workgroups); returns the FLOPs launched.
+double launch_probe_mfma(float* out, uint32_t blocks, uint32_t iters, hipStream_t s);
 
 // ---- gapblock.hip ----------------------------------------------------------------------
 // Thresholded output: every pair with score <= cutoff is appended as (query, ref, score);
