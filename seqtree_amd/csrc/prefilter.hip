@@ -30,6 +30,7 @@
 #include "st_internal.hpp"
 
 #include <algorithm>
+#include <cstddef>
 
 namespace st {
 namespace pf {
@@ -216,36 +217,57 @@ __global__ __launch_bounds__(kPoolSeqs * kChannels) void pool_kernel(
 // ---- Gram screen (MFMA) --------------------------------------------------------------------
 // Workgroup: 4 waves x 64 queries (two 32-row MFMA blocks per wave, operands kept in
 // registers for the whole K = 128) against the references [r_begin, r_end) in tiles of 64.
-// Tiles stream through a three-buffer LDS ring that all four waves share: in iteration t the
-// workgroup stores tile t + 1 (loaded into registers during iteration t - 1), issues the
-// global loads of tile t + 2 and runs the MFMAs of tile t, with one barrier per iteration.
-// Pair (q, r) is kept when dot(q, r) >= hq[q] + hr[r], i.e. |q|^2 + |r|^2 - 2 dot <= tau
-// with hq = (|q|^2 - tau) / 2 and hr = |r|^2 / 2. Kept pairs go to the wave's own LDS queue
-// and are flushed to the global arrays with one global atomic per queue-full (a tile with
-// more hits than the queue holds is written straight to global memory).
+// Reference tiles go from global memory straight into a four-buffer LDS ring by LDS-DMA loads
+// (global_load_lds_dwordx4, no staging registers), issued three tiles ahead of the MFMAs that
+// read them. A tile row of 128 BF16 (256 bytes) keeps its 16-byte pieces XOR-swizzled by the
+// row (piece p of row r in slot p ^ (r % 16)): each wave's DMA writes one contiguous kilobyte
+// and the operand reads of 16 consecutive rows hit 16 distinct bank groups.
+//
+// Pair (q, r) is kept when dot(q, r) - hq[q] >= hr[r], i.e. |q|^2 + |r|^2 - 2 dot <= tau, with
+// hq = (|q|^2 - tau) / 2 and hr = |r|^2 / 2. Each MFMA chain starts from -hq held in registers
+// that stay unchanged, and all accumulators of a lane belong to that lane's reference column,
+// so a tile without hits costs a max tree and one compare per column. Hits are extracted with
+// one ballot per accumulator of the row groups that hold any, appended to the wave's LDS queue
+// and flushed to the global arrays with one atomic per queue-full.
 constexpr uint32_t kGramQ = 256;
 constexpr uint32_t kTileR = 64;
-constexpr uint32_t kRowPad = kChannels + 8;     // bf16 per LDS row (16-byte skew)
+constexpr uint32_t kRing = 4;
+constexpr uint32_t kTilePieces = kTileR * kChannels / 8;  // 16-byte pieces per tile
+constexpr uint32_t kLoadsPerTile = 5;  // DMA loads per wave and tile: 4 x 1 KB + the norms
+constexpr uint32_t kNormSlot = 64;      // floats of a wave's norm DMA (16 rows, lanes 16+ repeat)
+// Buffer resource word 3 for raw (untyped) loads on gfx9-family parts.
+constexpr int kBufferFlags = 0x00020000;
 constexpr uint32_t kWaveQueue = 256;
-constexpr uint32_t kPiecesPerThread = kTileR * kChannels / 8 / 256;  // 16-byte pieces
 
-// Exclusive prefix sum of v over the wave and the wave total.
-__device__ __forceinline__ uint32_t wave_scan(uint32_t v, uint32_t* total) {
-    const uint32_t lane = __lane_id();
-    uint32_t x = v;
-#pragma unroll
-    for (uint32_t o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o);
-        if (lane >= o) x += y;
-    }
-    *total = __shfl(x, 63);
-    return x - v;
+typedef __attribute__((address_space(3))) void* lds_void;
+
+// s_waitcnt vmcnt(N), leaving the other counters alone (gfx9 field layout).
+template <uint32_t N>
+__device__ __forceinline__ void wait_vmcnt() {
+    static_assert(N < 64, "vmcnt holds six bits");
+    __builtin_amdgcn_s_waitcnt((N & 0xFu) | ((N >> 4) << 14) | (0x7u << 4) | (0xFu << 8));
 }
 
 // Orders this wave's LDS writes before its later LDS reads by other lanes.
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
+}
+
+// The hit queues are written and read only through these asm statements, which the compiler
+// does not see as memory operations: as visible LDS stores (which it cannot tell apart from the
+// ring-slot DMA targets) they would make its wait-count pass drain all DMA in flight. Volatile
+// asm statements keep their order, a wave's LDS operations execute in order, and the load waits
+// for its own result before the asm ends.
+__device__ __forceinline__ void queue_store(uint32_t* p, uint32_t v) {
+    const uint32_t addr = uint32_t(reinterpret_cast<uintptr_t>((lds_void)p));
+    asm volatile("ds_write_b32 %0, %1" ::"v"(addr), "v"(v));
+}
+__device__ __forceinline__ uint32_t queue_load(const uint32_t* p) {
+    const uint32_t addr = uint32_t(reinterpret_cast<uintptr_t>((lds_void)(const_cast<uint32_t*>(p))));
+    uint32_t v;
+    asm volatile("ds_read_b32 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=v"(v) : "v"(addr));
+    return v;
 }
 
 __device__ __forceinline__ void flush_queue(const uint32_t* qq, const uint32_t* qr, uint32_t n,
@@ -258,35 +280,227 @@ __device__ __forceinline__ void flush_queue(const uint32_t* qq, const uint32_t* 
     wave_sync();
     for (uint32_t i = lane; i < n; i += 64)
         if (g + i < cap) {
-            hit_q[g + i] = qq[i];
-            hit_r[g + i] = qr[i];
+            hit_q[g + i] = queue_load(qq + i);
+            hit_r[g + i] = queue_load(qr + i);
         }
     wave_sync();
 }
 
-__global__ __launch_bounds__(256) void gram_kernel(const __bf16* __restrict__ Q,
+// One ring slot: a tile of 64 reference rows of 128 BF16, each row's 16-byte pieces
+// XOR-swizzled (piece p of row r in slot p ^ (r % 16)), and their squared norms as four wave
+// segments of kNormSlot floats (rows 16w .. 16w + 15 in segment w, entries 0 .. 15).
+struct GramSlot {
+    bf16x8 rows[kTilePieces];
+    float norms[4 * kNormSlot];
+};
+
+// Per-lane byte offsets of the tile DMA (loop-invariant): wave w loads kilobytes 4w .. 4w + 3
+// of a tile (rows 16w .. 16w + 15, pieces swizzled) and the norms of rows 16w .. 16w + 15
+// (lanes 16 .. 63 repeat them, so that no lane is masked off).
+struct GramDmaLanes {
+    uint32_t rows[4];
+    uint32_t norm;
+};
+
+__device__ __forceinline__ GramDmaLanes gram_dma_lanes() {
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    GramDmaLanes o;
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+        const uint32_t row = 4 * (wave * 4 + u) + lane / 16;
+        o.rows[u] = row * (kChannels * 2) + ((lane % 16) ^ (row % 16)) * 16;
+    }
+    o.norm = (wave * 16 + lane % 16) * 4;
+    return o;
+}
+
+// DMA of tile t of the group (buffer resources over the group's rows and norms, so rows past
+// the group's end read as zeros) into a slot. kLoadsPerTile buffer loads per wave, none of them
+// under a branch, with loop-invariant lane offsets and the tile in the scalar offset.
+__device__ __forceinline__ void gram_dma(GramSlot* dst, __amdgpu_buffer_rsrc_t rows,
+                                         __amdgpu_buffer_rsrc_t norms, const GramDmaLanes& l,
+                                         uint32_t t) {
+    const uint32_t wave = threadIdx.x >> 6;
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rows, (lds_void)(dst->rows + (wave * 4 + u) * 64),
+                                                 16, l.rows[u], t * (kTileR * kChannels * 2), 0,
+                                                 0);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(norms, (lds_void)(dst->norms + wave * kNormSlot), 4,
+                                             l.norm, t * (kTileR * 4), 0, 0);
+}
+
+struct GramHits {
+    uint32_t* hit_q;
+    uint32_t* hit_r;
+    uint64_t cap;
+    unsigned long long* count;
+};
+
+// The eight operand pieces (K = 128) of this lane's row in one reference half of a slot and the
+// row's squared norm, read by one asm statement that also waits for them. The screen's pipeline
+// has no LDS read the compiler can see, so its wait-count pass never drains the ring-slot DMA
+// in flight (whose ordering is the barrier waits); the LDS byte addresses come in registers.
+__device__ __forceinline__ void read_half(const uint32_t (&addr)[8], uint32_t norm_addr,
+                                          bf16x8 (&b)[8], float& norm) {
+    asm volatile(
+        "ds_read_b128 %0, %9\n\t"
+        "ds_read_b128 %1, %10\n\t"
+        "ds_read_b128 %2, %11\n\t"
+        "ds_read_b128 %3, %12\n\t"
+        "ds_read_b128 %4, %13\n\t"
+        "ds_read_b128 %5, %14\n\t"
+        "ds_read_b128 %6, %15\n\t"
+        "ds_read_b128 %7, %16\n\t"
+        "ds_read_b32 %8, %17\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(b[0]), "=&v"(b[1]), "=&v"(b[2]), "=&v"(b[3]), "=&v"(b[4]), "=&v"(b[5]),
+          "=&v"(b[6]), "=&v"(b[7]), "=&v"(norm)
+        : "v"(addr[0]), "v"(addr[1]), "v"(addr[2]), "v"(addr[3]), "v"(addr[4]), "v"(addr[5]),
+          "v"(addr[6]), "v"(addr[7]), "v"(norm_addr));
+}
+
+// MFMAs and hit extraction of the tile in `cur`, whose first reference is `base`.
+__device__ __forceinline__ void gram_tile(const GramSlot* cur, uint32_t* qq, uint32_t* qr,
+                                          uint32_t base, uint32_t r_end,
+                                          const bf16x8 (&a)[2][8], const f32x16 (&hneg)[2],
+                                          uint32_t q0, uint32_t& qn, const GramHits& out) {
+    const uint32_t lane = __lane_id(), li = lane & 31u, lk = lane >> 5;
+    const uint32_t slot = uint32_t(reinterpret_cast<uintptr_t>((lds_void)(const_cast<GramSlot*>(cur))));
+#pragma unroll
+    for (uint32_t h = 0; h < 2; ++h) {
+        f32x16 acc[2];  // [query block], reference half h
+        // piece 2 ks + lk of row h * 32 + li sits at byte rb XOR 32 ks of the slot's rows
+        const uint32_t row = h * 32 + li;
+        const uint32_t rb = row * 256 + (((lk ^ (li % 16)) & 1u) << 4) + ((li % 16) >> 1 << 5);
+        uint32_t addr[8];
+#pragma unroll
+        for (uint32_t ks = 0; ks < 8; ++ks) addr[ks] = slot + (rb ^ (ks << 5));
+        bf16x8 b[8];
+        float norm;
+        read_half(addr, slot + uint32_t(offsetof(GramSlot, norms)) +
+                            ((row / 16) * kNormSlot + row % 16) * 4,
+                  b, norm);
+#pragma unroll
+        for (uint32_t ks = 0; ks < 8; ++ks) {
+            acc[0] = mfma(a[0][ks], b[ks], ks ? acc[0] : hneg[0]);
+            acc[1] = mfma(a[1][ks], b[ks], ks ? acc[1] : hneg[1]);
+        }
+        const uint32_t r = base + row;
+        const float hr = r < r_end ? 0.5f * norm : __builtin_inff();
+        float g[2][4];  // maxima over the accumulators of rows 8k .. 8k + 7 of each block
+#pragma unroll
+        for (uint32_t b = 0; b < 2; ++b)
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k)
+                g[b][k] = fmaxf(fmaxf(acc[b][4 * k], acc[b][4 * k + 1]),
+                                fmaxf(acc[b][4 * k + 2], acc[b][4 * k + 3]));
+        const float m = fmaxf(fmaxf(fmaxf(g[0][0], g[0][1]), fmaxf(g[0][2], g[0][3])),
+                              fmaxf(fmaxf(g[1][0], g[1][1]), fmaxf(g[1][2], g[1][3])));
+        if (!__any(m >= hr)) continue;
+#pragma unroll
+        for (uint32_t b = 0; b < 2; ++b) {
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) {
+                if (!__any(g[b][k] >= hr)) continue;
+#pragma unroll
+                for (uint32_t j = 0; j < 4; ++j) {
+                    const bool hit = acc[b][4 * k + j] >= hr;
+                    const uint64_t mask = __ballot(hit);
+                    if (!mask) continue;
+                    const uint32_t n = uint32_t(__popcll(mask));
+                    if (qn + n > kWaveQueue) {
+                        flush_queue(qq, qr, qn, out.hit_q, out.hit_r, out.cap, out.count);
+                        qn = 0;
+                    }
+                    if (hit) {
+                        const uint32_t pos =
+                            qn + __builtin_amdgcn_mbcnt_hi(
+                                     uint32_t(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+                        queue_store(qq + pos, q0 + b * 32 + k * 8 + lk * 4 + j);
+                        queue_store(qr + pos, r);
+                    }
+                    qn += n;
+                }
+            }
+        }
+    }
+}
+
+// Wait until at most 2 x kLoadsPerTile of this wave's DMA are in flight, then a workgroup
+// barrier. One asm statement, so that no DMA can be scheduled between the two.
+__device__ __forceinline__ void gram_sync() {
+    static_assert(kLoadsPerTile == 5 && kRing == 4, "the wait counts two tiles of 5 DMA");
+    asm volatile("s_waitcnt vmcnt(10) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// The pipeline over tiles [0, ntiles4) of the group (ntiles4 a multiple of 4; tiles past the
+// last real one hold no row before r_end and find no hits). Phase p of a round waits for tile
+// t + p (slot p), passes a barrier (every wave's share of that tile has landed, and every wave
+// is done with slot p - 1, the previous phase's), issues tile t + p + 3 into slot p - 1 and
+// computes slot p. The four slots are separate restrict pointers, which tells the compiler (and
+// its wait-count pass) that a DMA into one slot never aliases the reads of another, so the DMA
+// of later tiles stays in flight over the current tile's MFMAs; its ordering is the waits
+// above.
+__device__ __forceinline__ void gram_pipeline(GramSlot* __restrict__ s0,
+                                              GramSlot* __restrict__ s1,
+                                              GramSlot* __restrict__ s2,
+                                              GramSlot* __restrict__ s3,
+                                              uint32_t* __restrict__ qq,
+                                              uint32_t* __restrict__ qr,
+                                              __amdgpu_buffer_rsrc_t rows,
+                                              __amdgpu_buffer_rsrc_t norms, uint32_t r_begin,
+                                              uint32_t r_end, uint32_t ntiles4,
+                                              const bf16x8 (&a)[2][8], const f32x16 (&hneg)[2],
+                                              uint32_t q0, uint32_t& qn, const GramHits& out) {
+    const GramDmaLanes l = gram_dma_lanes();
+    gram_dma(s0, rows, norms, l, 0);
+    gram_dma(s1, rows, norms, l, 1);
+    gram_dma(s2, rows, norms, l, 2);
+#pragma unroll 1
+    for (uint32_t t = 0; t < ntiles4; t += 4) {
+        const uint32_t b = r_begin + t * kTileR;
+        gram_sync();
+        gram_dma(s3, rows, norms, l, t + 3);
+        gram_tile(s0, qq, qr, b, r_end, a, hneg, q0, qn, out);
+        gram_sync();
+        gram_dma(s0, rows, norms, l, t + 4);
+        gram_tile(s1, qq, qr, b + kTileR, r_end, a, hneg, q0, qn, out);
+        gram_sync();
+        gram_dma(s1, rows, norms, l, t + 5);
+        gram_tile(s2, qq, qr, b + 2 * kTileR, r_end, a, hneg, q0, qn, out);
+        gram_sync();
+        gram_dma(s2, rows, norms, l, t + 6);
+        gram_tile(s3, qq, qr, b + 3 * kTileR, r_end, a, hneg, q0, qn, out);
+    }
+}
+
+// Grid: q_blocks x groups workgroups, query block fastest (the workgroups of one reference
+// group run together and share its tiles in L2).
+__global__ __launch_bounds__(256, 2) void gram_kernel(const __bf16* __restrict__ Q,
                                                    const float* __restrict__ q_norm, uint32_t nq,
                                                    const __bf16* __restrict__ Rm,
                                                    const float* __restrict__ r_norm, uint32_t nr,
                                                    float tau, uint32_t r_per_group,
-                                                   uint32_t* hit_q, uint32_t* hit_r, uint64_t cap,
+                                                   uint32_t q_blocks, uint32_t* hit_q,
+                                                   uint32_t* hit_r, uint64_t cap,
                                                    unsigned long long* count) {
-    __shared__ __bf16 ring[3][kTileR][kRowPad];
-    __shared__ float ring_hr[3][kTileR];  // -|r|^2 / 2 of the tile's references (-inf past nr)
+    __shared__ GramSlot ring[kRing];
     __shared__ uint32_t queue_q[4][kWaveQueue], queue_r[4][kWaveQueue];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t li = lane & 31u, lk = lane >> 5;
-    const uint32_t q0 = blockIdx.x * kGramQ + wave * 64;
-    const uint32_t r_begin = blockIdx.y * r_per_group;
+    const uint32_t qb = blockIdx.x % q_blocks, grp = blockIdx.x / q_blocks;
+    const uint32_t q0 = qb * kGramQ + wave * 64;
+    const uint32_t r_begin = grp * r_per_group;
     const uint32_t r_end = r_begin + r_per_group < nr ? r_begin + r_per_group : nr;
-    const uint32_t ntiles = r_end > r_begin ? (r_end - r_begin + kTileR - 1) / kTileR : 0;
-    uint32_t* qq = queue_q[wave];
-    uint32_t* qr = queue_r[wave];
-    uint32_t qn = 0;  // wave-uniform queue fill
-    const bool active = q0 < nq;  // waves past the last query still help stage the tiles
+    if (r_begin >= r_end) return;  // the whole workgroup, before any barrier
+    const uint32_t ntiles = (r_end - r_begin + kTileR - 1) / kTileR;
+    uint32_t qn = 0;  // wave-uniform queue fill (waves past the last query find no hits)
+    const GramHits out{hit_q, hit_r, cap, count};
 
     bf16x8 a[2][8];
-    float h[2][16];
+    f32x16 hneg[2];
 #pragma unroll
     for (uint32_t b = 0; b < 2; ++b) {
         const uint32_t q = q0 + b * 32 + li;
@@ -298,120 +512,23 @@ __global__ __launch_bounds__(256) void gram_kernel(const __bf16* __restrict__ Q,
 #pragma unroll
         for (uint32_t e = 0; e < 16; ++e) {
             const uint32_t qe = q0 + b * 32 + (e / 4) * 8 + lk * 4 + (e % 4);
-            h[b][e] = qe < nq ? 0.5f * (q_norm[qe] - tau) : __builtin_inff();
+            hneg[b][e] = qe < nq ? -0.5f * (q_norm[qe] - tau) : -__builtin_inff();
         }
     }
-
-    // A tile in flight: thread tid holds pieces tid + 256 u (row (tid + 256 u) / 16, piece % 16)
-    // and, below kTileR, the row's raw norm (no arithmetic on a value still in flight).
-    struct Stage {
-        bf16x8 v[kPiecesPerThread];
-        float norm;
-        bool ok;
-    };
-    auto load_tile = [&](uint32_t t, Stage& st) {
-#pragma unroll
-        for (uint32_t u = 0; u < kPiecesPerThread; ++u) {
-            const uint32_t e = tid + 256 * u, row = e / 16, piece = e % 16;
-            const uint32_t r = r_begin + t * kTileR + row;
-            st.v[u] = (t < ntiles && r < r_end)
-                          ? *reinterpret_cast<const bf16x8*>(Rm + uint64_t(r) * kChannels +
-                                                             piece * 8)
-                          : bf16x8{};
-        }
-        if (tid < kTileR) {
-            const uint32_t r = r_begin + t * kTileR + tid;
-            st.ok = t < ntiles && r < r_end;
-            st.norm = r_norm[st.ok ? r : 0];
-        }
-    };
-    auto store_tile = [&](uint32_t buf, const Stage& st) {
-#pragma unroll
-        for (uint32_t u = 0; u < kPiecesPerThread; ++u) {
-            const uint32_t e = tid + 256 * u, row = e / 16, piece = e % 16;
-            *reinterpret_cast<bf16x8*>(&ring[buf][row][piece * 8]) = st.v[u];
-        }
-        if (tid < kTileR) ring_hr[buf][tid] = st.ok ? -0.5f * st.norm : -__builtin_inff();
-    };
-    auto compute_tile = [&](uint32_t t, uint32_t buf) {
-        if (!active) return;
-#pragma unroll
-        for (uint32_t half = 0; half < 2; ++half) {
-            // accumulators start at -hr - hq (every accumulator of a lane is in column li,
-            // element e in row (e / 4) * 8 + lk * 4 + e % 4), so a hit is acc >= 0 and a
-            // tile without hits is found with a max tree and one compare
-            const uint32_t r = r_begin + t * kTileR + half * 32 + li;
-            const float hrv = ring_hr[buf][half * 32 + li];
-            f32x16 acc0, acc1;
-#pragma unroll
-            for (uint32_t e = 0; e < 16; ++e) {
-                acc0[e] = hrv - h[0][e];
-                acc1[e] = hrv - h[1][e];
-            }
-#pragma unroll
-            for (uint32_t ks = 0; ks < 8; ++ks) {
-                const bf16x8 b =
-                    *reinterpret_cast<const bf16x8*>(&ring[buf][half * 32 + li][ks * 16 + 8 * lk]);
-                acc0 = mfma(a[0][ks], b, acc0);
-                acc1 = mfma(a[1][ks], b, acc1);
-            }
-            float m = acc0[0];
-#pragma unroll
-            for (uint32_t e = 1; e < 16; ++e) m = fmaxf(m, acc0[e]);
-#pragma unroll
-            for (uint32_t e = 0; e < 16; ++e) m = fmaxf(m, acc1[e]);
-            if (!__any(m >= 0.f)) continue;
-            uint32_t bits = 0;  // bit b * 16 + e: accumulator e of block b is a hit
-#pragma unroll
-            for (uint32_t e = 0; e < 16; ++e) {
-                bits |= uint32_t(acc0[e] >= 0.f) << e;
-                bits |= uint32_t(acc1[e] >= 0.f) << (16 + e);
-            }
-            uint32_t total = 0;
-            const uint32_t pre = wave_scan(__popc(bits), &total);
-            if (qn + total > kWaveQueue && qn) {
-                flush_queue(qq, qr, qn, hit_q, hit_r, cap, count);
-                qn = 0;
-            }
-            unsigned long long gb = 0;
-            const bool direct = total > kWaveQueue;
-            if (direct) {
-                if (lane == 0) gb = atomicAdd(count, (unsigned long long)total);
-                gb = __shfl(gb, 0);
-            }
-            uint32_t k = pre;
-            while (bits) {
-                const uint32_t bit = __ffs(bits) - 1;
-                bits &= bits - 1;
-                const uint32_t e = bit & 15u, blk = bit >> 4;
-                const uint32_t qa = q0 + blk * 32 + (e / 4) * 8 + lk * 4 + (e % 4);
-                if (!direct) {
-                    qq[qn + k] = qa;
-                    qr[qn + k] = r;
-                } else if (gb + k < cap) {
-                    hit_q[gb + k] = qa;
-                    hit_r[gb + k] = r;
-                }
-                ++k;
-            }
-            if (!direct) qn += total;
-        }
-    };
-    // Three-buffer ring: iteration t stores tile t + 1 (loaded during iteration t - 1),
-    // issues the loads of tile t + 2 and computes tile t, with one barrier.
-    Stage st;
-    load_tile(0, st);
-    store_tile(0, st);
-    load_tile(1, st);
-    __syncthreads();
-#pragma unroll 1
-    for (uint32_t t = 0; t < ntiles; ++t) {
-        store_tile((t + 1) % 3, st);
-        load_tile(t + 2, st);
-        compute_tile(t, t % 3);
-        __syncthreads();
-    }
-    if (qn) flush_queue(qq, qr, qn, hit_q, hit_r, cap, count);
+    // the query operands are complete before the pipeline, whose waits count tile DMA only
+    wait_vmcnt<0>();
+    // buffer resources over the group's rows and norms (raw, bounds-checked: reads past the
+    // group return zeros), the offsets in bytes from its first row
+    const __amdgpu_buffer_rsrc_t rows = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<__bf16*>(Rm) + uint64_t(r_begin) * kChannels, 0,
+        int((r_end - r_begin) * kChannels * 2), kBufferFlags);
+    const __amdgpu_buffer_rsrc_t norms = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float*>(r_norm) + r_begin, 0, int((r_end - r_begin) * 4), kBufferFlags);
+    gram_pipeline(&ring[0], &ring[1], &ring[2], &ring[3], queue_q[wave], queue_r[wave], rows,
+                  norms, r_begin, r_end, (ntiles + 3) / 4 * 4, a, hneg, q0, qn, out);
+    if (qn) flush_queue(queue_q[wave], queue_r[wave], qn, hit_q, hit_r, cap, count);
+    // no DMA may still be writing this workgroup's LDS when it ends
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
 // ---- training -----------------------------------------------------------------------------
@@ -702,15 +819,38 @@ void launch_gram(const __bf16* Q, const float* q_norm, uint32_t nq, const __bf16
                  const float* r_norm, uint32_t nr, float tau, uint32_t* hit_q, uint32_t* hit_r,
                  uint64_t cap, unsigned long long* count, uint32_t cus, hipStream_t s) {
     if (nq == 0 || nr == 0) return;
+    static int per_cu = 0;  // resident workgroups per CU (the kernel's LDS and registers)
+    if (per_cu == 0) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &n, reinterpret_cast<const void*>(gram_kernel), 256, 0) != hipSuccess || n < 1)
+            n = 1;
+        per_cu = n;
+    }
+    const uint64_t slots = uint64_t(per_cu) * cus;
     const uint32_t gq = (nq + kGramQ - 1) / kGramQ;
-    // reference groups: enough workgroups for a few per CU, each a whole number of tiles
-    uint32_t groups = (4 * cus + gq - 1) / gq;
-    const uint32_t tiles = (nr + 31) / 32;
-    groups = groups < 1 ? 1 : (groups > tiles ? tiles : groups);
-    const uint32_t per = ((tiles + groups - 1) / groups) * 32;
-    groups = (nr + per - 1) / per;
-    hipLaunchKernelGGL(gram_kernel, dim3(gq, groups), dim3(256), 0, s, Q, q_norm, nq, Rm, r_norm,
-                       nr, tau, per, hit_q, hit_r, cap, count);
+    const uint32_t tiles = (nr + kTileR - 1) / kTileR;
+    // The workgroups run in rounds of `slots`; split the references into the number of groups
+    // (whole tiles each, at least 16) that minimises rounds x (tiles per group + a few tiles of
+    // pipeline fill).
+    // (A group's rows are addressed by a buffer resource: fewer than 2^31 bytes each.)
+    uint32_t per = std::min<uint32_t>(tiles, (1u << 31) / (kTileR * kChannels * 2) - 1);
+    double best = -1.0;
+    const uint32_t g_max = std::max<uint32_t>(1, std::min<uint32_t>(tiles / 16, 4096));
+    for (uint32_t g = 1; g <= g_max; ++g) {
+        const uint32_t pt = (tiles + g - 1) / g;
+        const uint64_t wgs = uint64_t(gq) * ((tiles + pt - 1) / pt);
+        if (g > 1 && wgs > 0x7FFFFFFFull) break;  // grid size limit
+        if (uint64_t(pt) * kTileR * kChannels * 2 >= (1ull << 31)) continue;  // buffer range
+        const double cost = double((wgs + slots - 1) / slots) * (pt + 4);
+        if (best < 0 || cost < best) {
+            best = cost;
+            per = pt;
+        }
+    }
+    const uint32_t groups = (tiles + per - 1) / per;
+    hipLaunchKernelGGL(gram_kernel, dim3(gq * groups), dim3(256), 0, s, Q, q_norm, nq, Rm, r_norm,
+                       nr, tau, per * kTileR, gq, hit_q, hit_r, cap, count);
 }
 
 void launch_pair_loss(const float* ef, const uint32_t* pa, const uint32_t* pb, const float* y,
