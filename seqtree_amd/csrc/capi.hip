@@ -833,6 +833,21 @@ void pf_forward(const __bf16* B, const float* P, const uint8_t* codes, const uin
     launched("prefilter forward");
 }
 
+// Inference forward over one packed block: X0 and `valid`, then the fused three-layer kernel;
+// H3 is left in h3 (H1 and H2 never leave LDS).
+void pf_encode_forward(const __bf16* B, const float* P, const uint8_t* codes, const uint64_t* off,
+                       uint32_t n, const PfRows& rows, const PfActs& x0, const PfActs& h3,
+                       uint8_t* valid, hipStream_t s) {
+    LaunchTimer t("prefilter_encode", s);
+    const uint64_t R = rows.R;
+    launch_fill_u32(reinterpret_cast<uint32_t*>(valid), R / 4, 0u, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(x0.rows + rows.used * pf::kEmbDim),
+                    (R - rows.used) * pf::kEmbDim / 2, 0u, s);
+    pf::launch_embed(codes, off, n, B + pf::kOffEmbB, x0.rows, valid, s);
+    pf::launch_encode(x0.rows, valid, B, P, h3.rows, R, s);
+    launched("prefilter encode forward");
+}
+
 struct PrefilterImpl {
     float* P = nullptr;   // FP32 master parameters
     float* G = nullptr;   // gradients
@@ -1128,8 +1143,7 @@ int32_t st_prefilter_encode(const st_prefilter* model, const uint8_t* codes,
         std::vector<uint64_t> bound(nb + 1);
         d2h(bound.data(), dbound, nb + 1, s);
         sync(s);
-        const PfActs x0 = pf_acts(ar, Rb, pf::kEmbDim, s), ha = pf_acts(ar, Rb, pf::kChannels, s),
-                     hb = pf_acts(ar, Rb, pf::kChannels, s);
+        const PfActs x0 = pf_acts(ar, Rb, pf::kEmbDim, s), ha = pf_acts(ar, Rb, pf::kChannels, s);
         uint8_t* valid = ar.get<uint8_t>(Rb);
         __bf16* out = static_cast<__bf16*>(emb);
         for (uint32_t s0 = 0; s0 < n; s0 += blk) {
@@ -1138,7 +1152,7 @@ int32_t st_prefilter_encode(const st_prefilter* model, const uint8_t* codes,
             if (rows.R > Rb)
                 throw HipError{ST_INVALID_ARGUMENT,
                                "seqtree_hip: a sequence is longer than max_len"};
-            pf_forward(m.B, m.P, codes, offsets + s0, cnt, rows, x0, ha, hb, ha, valid, s);
+            pf_encode_forward(m.B, m.P, codes, offsets + s0, cnt, rows, x0, ha, valid, s);
             LaunchTimer t("prefilter_pool", s);
             pf::launch_pool(ha.rows, offsets + s0, cnt, nullptr,
                             out + uint64_t(s0) * pf::kChannels, norm + s0, s);

@@ -182,6 +182,142 @@ __global__ __launch_bounds__(256) void conv_kernel(const __bf16* __restrict__ X,
     }
 }
 
+// ---- fused encoder (inference) -------------------------------------------------------------
+// One workgroup computes H3 for the 128 rows [r0, r0 + 128) straight from X0, without writing
+// H1 or H2 to global memory: conv1 over the 136 rows [r0 - 4, r0 + 132), conv2 over the 132
+// rows [r0 - 2, r0 + 130), conv3 over the 128 rows, each layer's output kept in LDS as the BF16
+// rounding the unfused kernels store. Wave w computes output channels [32 w, 32 w + 32) of
+// every 32-row block (five for conv1 and conv2, whose rows past the needed ones are clamped and
+// discarded, four for conv3); its weight fragments come from global memory (L2) into a
+// register ring eight k-steps ahead, the activation fragments from LDS.
+//
+// Frames: output frame row j of every layer reads its input frame rows j .. j + 4 (the input
+// frame starts two rows earlier). Frames start at global rows r0 - 6 (X0), r0 - 4 (H1),
+// r0 - 2 (H2) and r0 (H3).
+constexpr uint32_t kEncRows = 128;
+constexpr uint32_t kXStride = 80;   // bytes per X0 row in LDS: 64 + 16 (conflict-free reads)
+constexpr uint32_t kHStride = 272;  // bytes per H row in LDS: 256 + 16
+constexpr uint32_t kXRows = kEncRows + 12, kH1Rows = kEncRows + 8, kH2Rows = kEncRows + 4;
+// H1 (136 rows), later the H3 staging; X0, later H2 (132 rows plus 4 rows that the last block's
+// discarded rows land in)
+constexpr uint32_t kEncRegionA = kH1Rows * kHStride;
+constexpr uint32_t kEncRegionB = (kH2Rows + 4) * kHStride;
+constexpr uint32_t kEncMaskRows = 160;  // per-layer valid masks, one byte per frame row
+static_assert(kXRows * kXStride <= kEncRegionB && kEncRows * kHStride <= kEncRegionA, "LDS");
+
+// acc[b] = sum_k A[32 b + li (clamped)][k] * W[k][32 w + li] over K = 5 CIN, A row j of the
+// input frame span j .. j + 4 in LDS (`in`, row stride `stride` bytes).
+template <uint32_t CIN, uint32_t NBLK>
+__device__ __forceinline__ void enc_layer(const char* in, uint32_t stride, uint32_t clamp,
+                                          const __bf16* __restrict__ Wt, f32x16 (&acc)[NBLK]) {
+    constexpr uint32_t K = 5 * CIN, KS = K / 16, P = 8;
+    const uint32_t lane = __lane_id(), li = lane & 31u, lk = lane >> 5, w = threadIdx.x >> 6;
+    const __bf16* wrow = Wt + uint64_t(32 * w + li) * K + 8 * lk;
+    const char* arow[NBLK];
+#pragma unroll
+    for (uint32_t b = 0; b < NBLK; ++b) {
+        const uint32_t j = 32 * b + li;
+        arow[b] = in + (j < clamp ? j : clamp) * stride + 16 * lk;
+        acc[b] = f32x16{0.f};
+    }
+    bf16x8 ring[P];
+#pragma unroll
+    for (uint32_t ks = 0; ks < P && ks < KS; ++ks)
+        ring[ks] = *reinterpret_cast<const bf16x8*>(wrow + 16 * ks);
+#pragma unroll
+    for (uint32_t ks = 0; ks < KS; ++ks) {
+        bf16x8 next{};
+        if (ks + P < KS) next = *reinterpret_cast<const bf16x8*>(wrow + 16 * (ks + P));
+        asm volatile("" ::: "memory");  // issue the weight load P k-steps ahead, here
+        const bf16x8 bw = ring[ks % P];
+        const uint32_t kk = (16 * ks) / CIN, ci = (16 * ks) % CIN;
+#pragma unroll
+        for (uint32_t b = 0; b < NBLK; ++b) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(arow[b] + kk * stride + ci * 2);
+            acc[b] = mfma(a, bw, acc[b]);
+        }
+        if (ks + P < KS) ring[ks % P] = next;
+    }
+}
+
+// acc + bias, ReLU and the frame's valid mask (bytes, one per frame row), rounded to BF16 into
+// `out` (LDS, row stride kHStride); the last block stores its first LAST row groups of 8 only
+// (the rows past those are not needed).
+template <uint32_t NBLK, uint32_t LAST>
+__device__ __forceinline__ void enc_store(const f32x16 (&acc)[NBLK], float bn,
+                                          const uint8_t* mask, char* out) {
+    const uint32_t lane = __lane_id(), li = lane & 31u, lk = lane >> 5, w = threadIdx.x >> 6;
+    const uint32_t n = 32 * w + li;
+#pragma unroll
+    for (uint32_t b = 0; b < NBLK; ++b)
+#pragma unroll
+        for (uint32_t g = 0; g < (b + 1 < NBLK ? 4u : LAST); ++g) {
+            const uint32_t j0 = 32 * b + 8 * g + 4 * lk;  // rows j0 .. j0 + 3
+            const uint32_t m4 = *reinterpret_cast<const uint32_t*>(mask + j0);
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) {
+                const float v = acc[b][4 * g + u] + bn;
+                const bool keep = v > 0.f && ((m4 >> (8 * u)) & 0xFFu);
+                *reinterpret_cast<__bf16*>(out + (j0 + u) * kHStride + n * 2) =
+                    f2bf(keep ? v : 0.f);
+            }
+        }
+}
+
+__global__ __launch_bounds__(256, 2) void encode_kernel(const __bf16* __restrict__ X0,
+                                                        const uint8_t* __restrict__ valid,
+                                                        const __bf16* __restrict__ B,
+                                                        const float* __restrict__ P,
+                                                        __bf16* __restrict__ H3, uint64_t R) {
+    __shared__ __attribute__((aligned(16))) char regA[kEncRegionA];
+    __shared__ __attribute__((aligned(16))) char regB[kEncRegionB];
+    __shared__ __attribute__((aligned(4))) uint8_t mask[3][kEncMaskRows];  // conv1..3 frames
+    const uint32_t tid = threadIdx.x, n = 32 * (tid >> 6) + (tid & 31u);
+    const int64_t r0 = int64_t(blockIdx.x) * kEncRows;
+    const float bn1 = P[kOffB1 + n], bn2 = P[kOffB2 + n], bn3 = P[kOffB3 + n];
+
+    // X0 window (rows outside [0, R) are zero) and the three frames' valid masks
+    for (uint32_t e = tid; e < kXRows * 4; e += 256) {
+        const uint32_t m = e / 4, piece = e % 4;
+        const int64_t g = r0 - 6 + m;
+        bf16x8 v{};
+        if (g >= 0 && g < int64_t(R))
+            v = *reinterpret_cast<const bf16x8*>(X0 + uint64_t(g) * kEmbDim + piece * 8);
+        *reinterpret_cast<bf16x8*>(regB + m * kXStride + piece * 16) = v;
+    }
+    for (uint32_t e = tid; e < 3 * kEncMaskRows; e += 256) {
+        const uint32_t layer = e / kEncMaskRows, j = e % kEncMaskRows;
+        const int64_t g = r0 - 4 + 2 * int64_t(layer) + j;
+        mask[layer][j] = (g >= 0 && g < int64_t(R)) ? valid[g] : 0;
+    }
+    __syncthreads();
+    {
+        f32x16 acc[5];
+        enc_layer<kEmbDim, 5>(regB, kXStride, kH1Rows - 1, B + kOffWt1, acc);
+        enc_store<5, 1>(acc, bn1, mask[0], regA);
+    }
+    __syncthreads();
+    {
+        f32x16 acc[5];
+        enc_layer<kChannels, 5>(regA, kHStride, kH2Rows - 1, B + kOffWt2, acc);
+        __syncthreads();  // every wave is done reading X0 before H2 overwrites it
+        enc_store<5, 1>(acc, bn2, mask[1], regB);
+    }
+    __syncthreads();
+    {
+        f32x16 acc[4];
+        enc_layer<kChannels, 4>(regB, kHStride, kEncRows - 1, B + kOffWt3, acc);
+        enc_store<4, 4>(acc, bn3, mask[2], regA);
+    }
+    __syncthreads();
+    // H3 rows [r0, r0 + 128): 16-byte pieces, consecutive threads on consecutive pieces
+    for (uint32_t e = tid; e < kEncRows * 16; e += 256) {
+        const uint32_t o = e / 16, piece = e % 16;
+        *reinterpret_cast<bf16x8*>(H3 + (uint64_t(r0) + o) * kChannels + piece * 8) =
+            *reinterpret_cast<const bf16x8*>(regA + o * kHStride + piece * 16);
+    }
+}
+
 // ---- mean pool ---------------------------------------------------------------------------
 // One workgroup of kChannels threads per sequence. Writes the FP32 embedding, its BF16
 // rounding and the squared norm of the BF16 rounding (the operand the Gram GEMM sees).
@@ -806,6 +942,13 @@ void launch_conv(int layer, int epi, const __bf16* X, const __bf16* Bt, const fl
     else
         hipLaunchKernelGGL((conv_kernel<kChannels, kEmbDim, 2>), grid, block, 0, s, X, Bt, bias,
                            mask, valid, Y);
+}
+
+void launch_encode(const __bf16* X0, const uint8_t* valid, const __bf16* B, const float* P,
+                   __bf16* H3, uint64_t R, hipStream_t s) {
+    if (R == 0) return;
+    hipLaunchKernelGGL(encode_kernel, dim3(uint32_t(R / kEncRows)), dim3(256), 0, s, X0, valid, B,
+                       P, H3, R);
 }
 
 void launch_pool(const __bf16* H, const uint64_t* off, uint32_t n, float* ef, __bf16* eb,

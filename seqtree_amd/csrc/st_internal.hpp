@@ -91,6 +91,10 @@ void launch_embed(const uint8_t* codes, const uint64_t* off, uint32_t n, const _
 // 2 backward data of the embedding layer (valid rows). R: rows, a multiple of 128.
 void launch_conv(int layer, int epi, const __bf16* X, const __bf16* Bt, const float* bias,
                  const __bf16* mask, const uint8_t* valid, __bf16* Y, uint64_t R, hipStream_t s);
+// Fused inference forward: H3 (R rows, R a multiple of 128) from X0 and `valid`, with the
+// BF16 operand copies B and the FP32 parameters P (biases).
+void launch_encode(const __bf16* X0, const uint8_t* valid, const __bf16* B, const float* P,
+                   __bf16* H3, uint64_t R, hipStream_t s);
 void launch_pool(const __bf16* H, const uint64_t* off, uint32_t n, float* ef, __bf16* eb,
                  float* norm, hipStream_t s);
 void launch_gram(const __bf16* Q, const float* q_norm, uint32_t nq, const __bf16* Rm,
