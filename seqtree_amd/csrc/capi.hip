@@ -826,25 +826,41 @@ void pf_forward(const __bf16* B, const float* P, const uint8_t* codes, const uin
     launch_fill_u32(reinterpret_cast<uint32_t*>(valid), R / 4, 0u, s);
     launch_fill_u32(reinterpret_cast<uint32_t*>(x0.rows + rows.used * pf::kEmbDim),
                     (R - rows.used) * pf::kEmbDim / 2, 0u, s);
-    pf::launch_embed(codes, off, n, B + pf::kOffEmbB, x0.rows, valid, s);
+    pf::launch_embed(codes, off, n, B + pf::kOffEmbB, x0.rows, valid, nullptr, s);
     pf::launch_conv(1, 0, x0.rows, B + pf::kOffWt1, P + pf::kOffB1, nullptr, valid, h1.rows, R, s);
     pf::launch_conv(2, 0, h1.rows, B + pf::kOffWt2, P + pf::kOffB2, nullptr, valid, h2.rows, R, s);
     pf::launch_conv(3, 0, h2.rows, B + pf::kOffWt3, P + pf::kOffB3, nullptr, valid, h3.rows, R, s);
     launched("prefilter forward");
 }
 
-// Inference forward over one packed block: X0 and `valid`, then the fused three-layer kernel;
-// H3 is left in h3 (H1 and H2 never leave LDS).
+// Scratch of the fused encoder for blocks of up to R rows.
+struct PfEncodeScratch {
+    uint32_t* tile_first;
+    float* part;
+    uint32_t* head_seq;
+};
+
+PfEncodeScratch pf_encode_scratch(Arena& ar, uint64_t R) {
+    const uint64_t tiles = R / 128;
+    return {ar.get<uint32_t>(tiles), ar.get<float>(tiles * 2 * pf::kChannels),
+            ar.get<uint32_t>(tiles)};
+}
+
+// Inference forward and mean pool over one packed block: X0, `valid` and the tiles' first
+// sequences, then the fused kernel (H1 .. H3 never leave LDS) writing the BF16 embeddings eb
+// and their squared norms.
 void pf_encode_forward(const __bf16* B, const float* P, const uint8_t* codes, const uint64_t* off,
-                       uint32_t n, const PfRows& rows, const PfActs& x0, const PfActs& h3,
-                       uint8_t* valid, hipStream_t s) {
+                       uint32_t n, const PfRows& rows, const PfActs& x0, uint8_t* valid,
+                       const PfEncodeScratch& sc, __bf16* eb, float* norm, hipStream_t s) {
     LaunchTimer t("prefilter_encode", s);
     const uint64_t R = rows.R;
     launch_fill_u32(reinterpret_cast<uint32_t*>(valid), R / 4, 0u, s);
     launch_fill_u32(reinterpret_cast<uint32_t*>(x0.rows + rows.used * pf::kEmbDim),
                     (R - rows.used) * pf::kEmbDim / 2, 0u, s);
-    pf::launch_embed(codes, off, n, B + pf::kOffEmbB, x0.rows, valid, s);
-    pf::launch_encode(x0.rows, valid, B, P, h3.rows, R, s);
+    launch_fill_u32(sc.tile_first, R / 128, n, s);
+    pf::launch_embed(codes, off, n, B + pf::kOffEmbB, x0.rows, valid, sc.tile_first, s);
+    pf::launch_encode(x0.rows, valid, B, P, R, off, n, sc.tile_first, eb, norm, sc.part,
+                      sc.head_seq, s);
     launched("prefilter encode forward");
 }
 
@@ -1143,7 +1159,8 @@ int32_t st_prefilter_encode(const st_prefilter* model, const uint8_t* codes,
         std::vector<uint64_t> bound(nb + 1);
         d2h(bound.data(), dbound, nb + 1, s);
         sync(s);
-        const PfActs x0 = pf_acts(ar, Rb, pf::kEmbDim, s), ha = pf_acts(ar, Rb, pf::kChannels, s);
+        const PfActs x0 = pf_acts(ar, Rb, pf::kEmbDim, s);
+        const PfEncodeScratch sc = pf_encode_scratch(ar, Rb);
         uint8_t* valid = ar.get<uint8_t>(Rb);
         __bf16* out = static_cast<__bf16*>(emb);
         for (uint32_t s0 = 0; s0 < n; s0 += blk) {
@@ -1152,10 +1169,8 @@ int32_t st_prefilter_encode(const st_prefilter* model, const uint8_t* codes,
             if (rows.R > Rb)
                 throw HipError{ST_INVALID_ARGUMENT,
                                "seqtree_hip: a sequence is longer than max_len"};
-            pf_encode_forward(m.B, m.P, codes, offsets + s0, cnt, rows, x0, ha, valid, s);
-            LaunchTimer t("prefilter_pool", s);
-            pf::launch_pool(ha.rows, offsets + s0, cnt, nullptr,
-                            out + uint64_t(s0) * pf::kChannels, norm + s0, s);
+            pf_encode_forward(m.B, m.P, codes, offsets + s0, cnt, rows, x0, valid, sc,
+                              out + uint64_t(s0) * pf::kChannels, norm + s0, s);
         }
         launched("prefilter encode");
         return ok(status);

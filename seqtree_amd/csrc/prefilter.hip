@@ -47,22 +47,30 @@ __device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) {
 }
 
 // ---- embedding -----------------------------------------------------------------------
+constexpr uint32_t kEncTile = 128;  // rows per workgroup of the fused encoder
 __device__ __forceinline__ uint64_t start_row(const uint64_t* off, uint32_t s) {
     return (off[s] - off[0]) + 2ull * s;
 }
 
-// One thread per (sequence, channel): the sequence's rows and its two zero rows.
+// One thread per (sequence, channel): the sequence's rows and its two zero rows. With
+// tile_first, also records for every 128-row tile whose first row falls in this sequence's
+// span (its rows and two zero rows; spans partition the used rows) the sequence index.
 __global__ __launch_bounds__(256) void embed_kernel(const uint8_t* __restrict__ codes,
                                                     const uint64_t* __restrict__ off, uint32_t n,
                                                     const __bf16* __restrict__ emb,
                                                     __bf16* __restrict__ X0,
-                                                    uint8_t* __restrict__ valid) {
+                                                    uint8_t* __restrict__ valid,
+                                                    uint32_t* __restrict__ tile_first) {
     const uint64_t idx = uint64_t(blockIdx.x) * 256 + threadIdx.x;
     if (idx >= uint64_t(n) * kEmbDim) return;
     const uint32_t s = uint32_t(idx / kEmbDim), c = uint32_t(idx % kEmbDim);
     const uint64_t o = off[s];
     const uint32_t len = uint32_t(off[s + 1] - o);
     const uint64_t r0 = start_row(off, s);
+    if (tile_first && c == 0) {
+        const uint64_t t = (r0 + kEncTile - 1) / kEncTile;  // spans are shorter than a tile
+        if (t * kEncTile < r0 + len + 2) tile_first[t] = s;
+    }
     for (uint32_t p = 0; p < len + 2; ++p) {
         const bool ok = p < len;
         X0[(r0 + p) * kEmbDim + c] =
@@ -194,7 +202,7 @@ __global__ __launch_bounds__(256) void conv_kernel(const __bf16* __restrict__ X,
 // Frames: output frame row j of every layer reads its input frame rows j .. j + 4 (the input
 // frame starts two rows earlier). Frames start at global rows r0 - 6 (X0), r0 - 4 (H1),
 // r0 - 2 (H2) and r0 (H3).
-constexpr uint32_t kEncRows = 128;
+constexpr uint32_t kEncRows = kEncTile;
 constexpr uint32_t kXStride = 80;   // bytes per X0 row in LDS: 64 + 16 (conflict-free reads)
 constexpr uint32_t kHStride = 272;  // bytes per H row in LDS: 256 + 16
 constexpr uint32_t kXRows = kEncRows + 12, kH1Rows = kEncRows + 8, kH2Rows = kEncRows + 4;
@@ -264,19 +272,60 @@ __device__ __forceinline__ void enc_store(const f32x16 (&acc)[NBLK], float bn,
         }
 }
 
+// Pooling records of the fused encoder, per tile t: the FP32 channel sums of the sequence that
+// starts in tile t and runs past its end (`head`, with its index in head_seq, or kNoSeq) and
+// of the rows in tile t of the sequence that started before it (`tail`).
+constexpr uint32_t kNoSeq = 0xFFFFFFFFu;
+constexpr uint32_t kEncMaxSeqs = kEncRows / 2 + 1;  // sequences meeting one tile, at most
+
+struct EncPool {
+    const uint64_t* off;       // the block's offsets, off[0 .. n]
+    uint32_t n;
+    const uint32_t* tile_first;  // per tile: the sequence whose span holds its first row, or n
+    __bf16* eb;                // per sequence: the BF16 embedding
+    float* norm;               // per sequence: squared norm of eb
+    float* part;               // per tile: head sums [128], tail sums [128]
+    uint32_t* head_seq;        // per tile
+};
+
+// Mean of `sum` over `len` rows, rounded to BF16, stored with its squared norm (a reduction over
+// the 128 threads c of one half of the workgroup; every thread of the workgroup calls this the
+// same number of times).
+__device__ __forceinline__ void enc_finish(float sum, uint32_t len, bool active, uint32_t s,
+                                           uint32_t c, uint32_t half, float (*red)[2],
+                                           const EncPool& p) {
+    const float v = len ? sum / float(len) : 0.f;
+    const __bf16 vb = f2bf(v);
+    if (active) p.eb[uint64_t(s) * kChannels + c] = vb;
+    float sq = active ? bf2f(vb) * bf2f(vb) : 0.f;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
+    if ((c & 63u) == 0) red[half][c >> 6] = sq;
+    __syncthreads();
+    if (active && c == 0) p.norm[s] = red[half][0] + red[half][1];
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(256, 2) void encode_kernel(const __bf16* __restrict__ X0,
                                                         const uint8_t* __restrict__ valid,
                                                         const __bf16* __restrict__ B,
-                                                        const float* __restrict__ P,
-                                                        __bf16* __restrict__ H3, uint64_t R) {
+                                                        const float* __restrict__ P, uint64_t R,
+                                                        EncPool pool) {
     __shared__ __attribute__((aligned(16))) char regA[kEncRegionA];
     __shared__ __attribute__((aligned(16))) char regB[kEncRegionB];
     __shared__ __attribute__((aligned(4))) uint8_t mask[3][kEncMaskRows];  // conv1..3 frames
+    __shared__ uint64_t soff[kEncMaxSeqs + 1];  // offsets of the sequences meeting this tile
+    __shared__ float red[2][2];
+    __shared__ uint32_t head_s;
     const uint32_t tid = threadIdx.x, n = 32 * (tid >> 6) + (tid & 31u);
-    const int64_t r0 = int64_t(blockIdx.x) * kEncRows;
+    const uint32_t t = blockIdx.x;
+    const int64_t r0 = int64_t(t) * kEncRows;
     const float bn1 = P[kOffB1 + n], bn2 = P[kOffB2 + n], bn3 = P[kOffB3 + n];
+    const uint32_t s0 = pool.tile_first[t];
+    const uint64_t off0 = pool.off[0];
 
-    // X0 window (rows outside [0, R) are zero) and the three frames' valid masks
+    // X0 window (rows outside [0, R) are zero), the three frames' valid masks and the
+    // offsets of the tile's sequences
     for (uint32_t e = tid; e < kXRows * 4; e += 256) {
         const uint32_t m = e / 4, piece = e % 4;
         const int64_t g = r0 - 6 + m;
@@ -290,6 +339,11 @@ __global__ __launch_bounds__(256, 2) void encode_kernel(const __bf16* __restrict
         const int64_t g = r0 - 4 + 2 * int64_t(layer) + j;
         mask[layer][j] = (g >= 0 && g < int64_t(R)) ? valid[g] : 0;
     }
+    if (tid <= kEncMaxSeqs) {
+        const uint64_t si = uint64_t(s0) + tid;
+        soff[tid] = pool.off[si < pool.n ? si : pool.n];
+    }
+    if (tid == 0) head_s = kNoSeq;
     __syncthreads();
     {
         f32x16 acc[5];
@@ -310,12 +364,73 @@ __global__ __launch_bounds__(256, 2) void encode_kernel(const __bf16* __restrict
         enc_store<4, 4>(acc, bn3, mask[2], regA);
     }
     __syncthreads();
-    // H3 rows [r0, r0 + 128): 16-byte pieces, consecutive threads on consecutive pieces
-    for (uint32_t e = tid; e < kEncRows * 16; e += 256) {
-        const uint32_t o = e / 16, piece = e % 16;
-        *reinterpret_cast<bf16x8*>(H3 + (uint64_t(r0) + o) * kChannels + piece * 8) =
-            *reinterpret_cast<const bf16x8*>(regA + o * kHStride + piece * 16);
+
+    // Mean pool of H3 (rows [r0, r0 + 128) in regA): the sequences s0 + j, j < cnt, meet this
+    // tile; each half of the workgroup takes every other one, thread c its channel c. A
+    // sequence wholly inside the tile is finished here; the one running past the tile's end
+    // leaves its head sums, the one that started before it its tail sums, and
+    // encode_finish_kernel joins the two.
+    const uint64_t tile_end = uint64_t(r0) + kEncRows;
+    uint32_t cnt = 0;
+    while (cnt < kEncMaxSeqs && s0 + cnt < pool.n &&
+           (soff[cnt] - off0) + 2ull * (s0 + cnt) < tile_end)
+        ++cnt;
+    const uint32_t half = tid >> 7, c = tid & 127u;
+    for (uint32_t it = 0; it < (cnt + 1) / 2; ++it) {
+        const uint32_t j = 2 * it + half, s = s0 + j;
+        const bool in = j < cnt;
+        const uint64_t st = in ? (soff[j] - off0) + 2ull * s : 0;
+        const uint32_t len = in ? uint32_t(soff[j + 1] - soff[j]) : 0u;
+        const uint64_t lo = st > uint64_t(r0) ? st : uint64_t(r0);
+        const uint64_t hi = st + len < tile_end ? st + len : tile_end;
+        // four interleaved partial sums (independent LDS reads in flight), combined in a
+        // fixed order
+        const char* col = regA + c * 2;
+        const uint32_t a0 = uint32_t(lo - uint64_t(r0));
+        const uint32_t a1 = hi > lo ? uint32_t(hi - uint64_t(r0)) : a0;  // empty: rows ended
+        float q0 = 0.f, q1 = 0.f, q2 = 0.f, q3 = 0.f;
+        uint32_t o = a0;
+        for (; o + 4 <= a1; o += 4) {
+            q0 += bf2f(*reinterpret_cast<const __bf16*>(col + o * kHStride));
+            q1 += bf2f(*reinterpret_cast<const __bf16*>(col + (o + 1) * kHStride));
+            q2 += bf2f(*reinterpret_cast<const __bf16*>(col + (o + 2) * kHStride));
+            q3 += bf2f(*reinterpret_cast<const __bf16*>(col + (o + 3) * kHStride));
+        }
+        for (; o < a1; ++o) q0 += bf2f(*reinterpret_cast<const __bf16*>(col + o * kHStride));
+        const float sum = (q0 + q1) + (q2 + q3);
+        const bool before = st < uint64_t(r0);
+        const bool tail = in && before && st + len > uint64_t(r0);
+        const bool head = in && !before && st + len > tile_end;
+        const bool whole = in && !before && !head;
+        if (tail) pool.part[(uint64_t(t) * 2 + 1) * kChannels + c] = sum;
+        if (head) {
+            pool.part[uint64_t(t) * 2 * kChannels + c] = sum;
+            if (c == 0) head_s = s;
+        }
+        enc_finish(sum, len, whole, s, c, half, red, pool);
     }
+    if (tid == 0) pool.head_seq[t] = head_s;
+}
+
+// Joins the head sums of tile t and the tail sums of tile t + 1 of the sequence cut by their
+// boundary. One workgroup of kChannels threads per tile.
+__global__ __launch_bounds__(kChannels) void encode_finish_kernel(EncPool pool) {
+    __shared__ float red[2][2];
+    const uint32_t t = blockIdx.x, c = threadIdx.x;
+    const uint32_t s = pool.head_seq[t];
+    if (s == kNoSeq) return;  // the whole workgroup
+    const float sum = pool.part[uint64_t(t) * 2 * kChannels + c] +
+                      pool.part[(uint64_t(t + 1) * 2 + 1) * kChannels + c];
+    const uint32_t len = uint32_t(pool.off[s + 1] - pool.off[s]);
+    const float v = sum / float(len);
+    const __bf16 vb = f2bf(v);
+    pool.eb[uint64_t(s) * kChannels + c] = vb;
+    float sq = bf2f(vb) * bf2f(vb);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
+    if ((c & 63u) == 0) red[0][c >> 6] = sq;
+    __syncthreads();
+    if (c == 0) pool.norm[s] = red[0][0] + red[0][1];
 }
 
 // ---- mean pool ---------------------------------------------------------------------------
@@ -919,11 +1034,11 @@ void launch_select_le(const int32_t* v, uint64_t n, int32_t cutoff, const uint32
                        pa, pb, out_a, out_b, out_v, cap, count);
 }
 void launch_embed(const uint8_t* codes, const uint64_t* off, uint32_t n, const __bf16* emb,
-                  __bf16* X0, uint8_t* valid, hipStream_t s) {
+                  __bf16* X0, uint8_t* valid, uint32_t* tile_first, hipStream_t s) {
     const uint64_t total = uint64_t(n) * kEmbDim;
     if (total == 0) return;
     hipLaunchKernelGGL(embed_kernel, dim3(uint32_t((total + 255) / 256)), dim3(256), 0, s, codes,
-                       off, n, emb, X0, valid);
+                       off, n, emb, X0, valid, tile_first);
 }
 
 void launch_conv(int layer, int epi, const __bf16* X, const __bf16* Bt, const float* bias,
@@ -945,10 +1060,13 @@ void launch_conv(int layer, int epi, const __bf16* X, const __bf16* Bt, const fl
 }
 
 void launch_encode(const __bf16* X0, const uint8_t* valid, const __bf16* B, const float* P,
-                   __bf16* H3, uint64_t R, hipStream_t s) {
+                   uint64_t R, const uint64_t* off, uint32_t n, const uint32_t* tile_first,
+                   __bf16* eb, float* norm, float* part, uint32_t* head_seq, hipStream_t s) {
     if (R == 0) return;
-    hipLaunchKernelGGL(encode_kernel, dim3(uint32_t(R / kEncRows)), dim3(256), 0, s, X0, valid, B,
-                       P, H3, R);
+    const EncPool pool{off, n, tile_first, eb, norm, part, head_seq};
+    const uint32_t tiles = uint32_t(R / kEncRows);
+    hipLaunchKernelGGL(encode_kernel, dim3(tiles), dim3(256), 0, s, X0, valid, B, P, R, pool);
+    hipLaunchKernelGGL(encode_finish_kernel, dim3(tiles), dim3(kChannels), 0, s, pool);
 }
 
 void launch_pool(const __bf16* H, const uint64_t* off, uint32_t n, float* ef, __bf16* eb,

@@ -85,16 +85,22 @@ constexpr uint32_t kOffWf3 = kOffWf2 + 5 * kChannels * kChannels;
 constexpr uint32_t kBf16Params = kOffWf3 + 5 * kChannels * kChannels;
 
 // Sequence s of a set occupies rows (off[s] - off[0]) + 2 s onward (see prefilter.hip).
+// tile_first (or nullptr): per 128-row tile of the fused encoder, the sequence whose span holds
+// the tile's first row (tiles past the used rows keep their prior contents).
 void launch_embed(const uint8_t* codes, const uint64_t* off, uint32_t n, const __bf16* emb,
-                  __bf16* X0, uint8_t* valid, hipStream_t s);
+                  __bf16* X0, uint8_t* valid, uint32_t* tile_first, hipStream_t s);
 // layer 1..3; epi 0 forward (bias, relu, valid rows), 1 backward data masked by mask > 0,
 // 2 backward data of the embedding layer (valid rows). R: rows, a multiple of 128.
 void launch_conv(int layer, int epi, const __bf16* X, const __bf16* Bt, const float* bias,
                  const __bf16* mask, const uint8_t* valid, __bf16* Y, uint64_t R, hipStream_t s);
-// Fused inference forward: H3 (R rows, R a multiple of 128) from X0 and `valid`, with the
-// BF16 operand copies B and the FP32 parameters P (biases).
+// Fused inference forward and mean pool over R rows (a multiple of 128): the BF16 embeddings
+// eb and their squared norms for the n sequences of offsets off, from X0 and `valid`, with the
+// BF16 operand copies B and the FP32 parameters P (biases). tile_first comes from
+// launch_embed (tiles past the used rows hold n); part (R / 64 x 128 floats) and head_seq
+// (R / 128) are scratch.
 void launch_encode(const __bf16* X0, const uint8_t* valid, const __bf16* B, const float* P,
-                   __bf16* H3, uint64_t R, hipStream_t s);
+                   uint64_t R, const uint64_t* off, uint32_t n, const uint32_t* tile_first,
+                   __bf16* eb, float* norm, float* part, uint32_t* head_seq, hipStream_t s);
 void launch_pool(const __bf16* H, const uint64_t* off, uint32_t n, float* ef, __bf16* eb,
                  float* norm, hipStream_t s);
 void launch_gram(const __bf16* Q, const float* q_norm, uint32_t nq, const __bf16* Rm,

@@ -164,6 +164,26 @@ def test_encode_matches_numpy_model(gpu):
     model.close()
 
 
+def test_encode_empty_and_longest_sequences(gpu):
+    """Lengths 0 .. 64: sequences cut by the encoder's 128-row tiles (pooled in two parts) and
+    empty ones (zero embedding) next to whole ones."""
+    from seqtree_amd.prefilter import DeviceSet, Prefilter, _bf16_to_f32
+
+    rng = np.random.default_rng(13)
+    model = Prefilter(seed=4)
+    params = rand_params(rng, model.n_params)
+    model.load(params)
+    seqs, codes, off = rand_set(rng, 700, 0, 64)
+    emb, norm = model.encode(DeviceSet.upload(codes, off), 64)
+    got = _bf16_to_f32(emb.numpy())
+    want, _ = forward(params, seqs)
+    assert np.allclose(got, want, rtol=3e-2, atol=3e-2 * np.abs(want).max())
+    empty = np.array([len(q) == 0 for q in seqs])
+    assert empty.any() and not got[empty].any()
+    assert np.allclose(norm.numpy(), (got.astype(np.float64) ** 2).sum(1), rtol=1e-4)
+    model.close()
+
+
 def test_training_gradients_match_numpy_model(gpu):
     from seqtree_amd._device import DeviceArray
     from seqtree_amd.prefilter import DeviceSet, Prefilter
