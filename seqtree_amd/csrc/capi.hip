@@ -525,14 +525,20 @@ void gotoh_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, uint32_t 
     };
 
     if (dist) {
-        const uint64_t stride = self_len + 1;
-        const uint32_t slots = slots_for(stride);
-        int64_t* scratch = ar.get<int64_t>(size_t(slots) * 6 * stride);
         int64_t* qs = ar.get<int64_t>(N);
         int64_t* rs = ar.get<int64_t>(K);
         LaunchTimer t("gotoh_self", s);
-        launch_gotoh_self(a, q, q_off, N, qs, scratch, slots, stride, s);
-        launch_gotoh_self(a, r, r_off, K, rs, scratch, slots, stride, s);
+        if (gotoh_self_fits_lds(uint32_t(self_len),
+                                std::max({mx, iabs64(go), iabs64(ge)}))) {
+            launch_gotoh_self_lds(a, q, q_off, N, qs, s);
+            launch_gotoh_self_lds(a, r, r_off, K, rs, s);
+        } else {
+            const uint64_t stride = self_len + 1;
+            const uint32_t slots = slots_for(stride);
+            int64_t* scratch = ar.get<int64_t>(size_t(slots) * 6 * stride);
+            launch_gotoh_self(a, q, q_off, N, qs, scratch, slots, stride, s);
+            launch_gotoh_self(a, r, r_off, K, rs, scratch, slots, stride, s);
+        }
         launched("gotoh_self");
         a.q_self = qs;
         a.r_self = rs;

@@ -308,6 +308,77 @@ __global__ __launch_bounds__(256) void gotoh_self_kernel(GotohArgs a,
     }
 }
 
+// Self-scores s(x, x) of strings of at most kSelfMaxLen residues: one thread per string, its
+// three DP rows (M, X, Y over the string, int32) in LDS, column-interleaved across the wave so
+// that lanes touching the same column hit distinct banks. Rows are updated in place: the cell
+// (i, j) reads row i - 1 at j from LDS and keeps row i - 1 at j - 1 (the diagonal) and row i at
+// j - 1 (the left neighbour) in registers. Same recurrence, boundary values and order of max as
+// gotoh_i64; the host runs it only when every value stays far inside int32 (kNeg32 and a few
+// gap steps below it included), so the results are equal.
+constexpr uint32_t kSelfMaxLen = 64;
+constexpr uint32_t kSelfThreads = 64;
+
+template <bool LOCAL>
+__global__ __launch_bounds__(kSelfThreads) void gotoh_self_lds_kernel(
+    GotohArgs a, const uint8_t* __restrict__ codes, const uint64_t* __restrict__ off, uint64_t n,
+    int64_t* __restrict__ res) {
+    __shared__ int32_t rows[3][kSelfMaxLen + 1][kSelfThreads];
+    constexpr int32_t kNeg32 = -(1 << 29);
+    const uint32_t lane = threadIdx.x;
+    const uint64_t x = uint64_t(blockIdx.x) * kSelfThreads + lane;
+    if (x >= n) return;  // no barriers below
+    const uint64_t o = off[x];
+    const uint32_t L = uint32_t(off[x + 1] - o);
+    const uint8_t* q = codes + o;
+    const int32_t go = a.gap_open, ge = a.gap_extend;
+    if (L == 0) {
+        res[x] = 0;
+        return;
+    }
+    int32_t (*M)[kSelfThreads] = rows[0];
+    int32_t (*X)[kSelfThreads] = rows[1];
+    int32_t (*Y)[kSelfThreads] = rows[2];
+    M[0][lane] = 0;
+    X[0][lane] = kNeg32;
+    Y[0][lane] = kNeg32;
+    for (uint32_t j = 1; j <= L; ++j) {
+        M[j][lane] = LOCAL ? 0 : kNeg32;
+        X[j][lane] = kNeg32;
+        Y[j][lane] = LOCAL ? kNeg32 : -(go + int32_t(j - 1) * ge);
+    }
+    int32_t best = LOCAL ? 0 : kNeg32;
+    for (uint32_t i = 1; i <= L; ++i) {
+        // column 0 of row i, and row i - 1 at column 0 as the first diagonal
+        int32_t dm = M[0][lane], dx = X[0][lane], dy = Y[0][lane];
+        int32_t lm = LOCAL ? 0 : kNeg32;
+        int32_t lx = LOCAL ? kNeg32 : -(go + int32_t(i - 1) * ge);
+        int32_t ly = kNeg32;
+        M[0][lane] = lm;
+        X[0][lane] = lx;
+        Y[0][lane] = ly;
+        const int32_t* srow = a.sim + uint32_t(q[i - 1]) * a.A;
+        for (uint32_t j = 1; j <= L; ++j) {
+            const int32_t um = M[j][lane], ux = X[j][lane], uy = Y[j][lane];
+            const int32_t sv = srow[q[j - 1]];
+            int32_t mv = max(dm, max(dx, dy)) + sv;
+            if (LOCAL && mv < 0) mv = 0;
+            const int32_t xv = max(um - go, max(ux - ge, uy - go));
+            const int32_t yv = max(lm - go, max(ly - ge, lx - go));
+            if (LOCAL && mv > best) best = mv;
+            M[j][lane] = mv;
+            X[j][lane] = xv;
+            Y[j][lane] = yv;
+            dm = um;
+            dx = ux;
+            dy = uy;
+            lm = mv;
+            lx = xv;
+            ly = yv;
+        }
+    }
+    res[x] = LOCAL ? best : max(M[L][lane], max(X[L][lane], Y[L][lane]));
+}
+
 template <class C, int R, bool HFORM>
 void launch_form(const GotohArgs& a, const uint32_t* list, uint32_t count, hipStream_t s) {
     const uint64_t nch = a.refs.nchunks;
@@ -382,6 +453,24 @@ void launch_gotoh_self(const GotohArgs& a, const uint8_t* codes, const uint64_t*
     if (n == 0) return;
     hipLaunchKernelGGL(gotoh_self_kernel, dim3(slots / 256), dim3(256), 0, s, a, codes, off, n,
                        res, scratch, stride);
+}
+
+bool gotoh_self_fits_lds(uint32_t max_len, int64_t mx) {
+    // every finite value is within (2 max_len + 2) * mx of 0; sentinel-derived values stay
+    // within kNeg32 - (2 max_len + 2) * mx; both far inside int32 for mx < 2^20
+    return max_len <= kSelfMaxLen && mx < (int64_t(1) << 20);
+}
+
+void launch_gotoh_self_lds(const GotohArgs& a, const uint8_t* codes, const uint64_t* off,
+                           uint64_t n, int64_t* res, hipStream_t s) {
+    if (n == 0) return;
+    const dim3 grid(uint32_t((n + kSelfThreads - 1) / kSelfThreads));
+    if (a.local)
+        hipLaunchKernelGGL(gotoh_self_lds_kernel<true>, grid, dim3(kSelfThreads), 0, s, a, codes,
+                           off, n, res);
+    else
+        hipLaunchKernelGGL(gotoh_self_lds_kernel<false>, grid, dim3(kSelfThreads), 0, s, a, codes,
+                           off, n, res);
 }
 
 }  // namespace st

@@ -244,6 +244,11 @@ void launch_gotoh_generic(const GotohArgs& a, const uint32_t* list, uint32_t cou
 void launch_gotoh_self(const GotohArgs& a, const uint8_t* codes, const uint64_t* off, uint64_t n,
                        int64_t* res, int64_t* scratch, uint32_t slots, uint64_t stride,
                        hipStream_t s);
+// Self-scores with the DP rows in LDS (strings of at most 64 residues, |scores|, gap costs
+// < 2^20: gotoh_self_fits_lds); the same results as launch_gotoh_self.
+bool gotoh_self_fits_lds(uint32_t max_len, int64_t mx);
+void launch_gotoh_self_lds(const GotohArgs& a, const uint8_t* codes, const uint64_t* off,
+                           uint64_t n, int64_t* res, hipStream_t s);
 
 // ---- distance.hip ----------------------------------------------------------------------
 struct DistanceArgs {

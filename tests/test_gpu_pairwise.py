@@ -69,6 +69,21 @@ def test_distance_matrices(gpu, oracle_lib, name, mode):
     _check(st, oracle_lib, q, r, m, mode, 11, 1, dist=True)
 
 
+def test_distance_self_scores_up_to_64_residues(gpu, oracle_lib):
+    """Sets of at most 64 residues take the LDS self-score kernel: lengths 0 and 50..64, an
+    asymmetric matrix and gap_extend > gap_open included."""
+    st = gpu
+    rng = random.Random(23)
+    grid = np.random.default_rng(23).integers(-7, 9, size=(24, 24)).astype(np.int32)
+    for m, sim in ((st.SubstitutionMatrix.blosum62(), None),
+                   (st.SubstitutionMatrix.from_similarity(grid.tolist()), grid)):
+        q = [rand_seq(rng, 50, 64) for _ in range(40)] + [""]
+        r = [rand_seq(rng, 0, 64) for _ in range(300)] + ["A" * 64]
+        for mode in ("global", "local"):
+            for go, ge in ((11, 1), (2, 5)):
+                _check(st, oracle_lib, q, r, m, mode, go, ge, dist=True, sim=sim)
+
+
 def test_known_answers(gpu):
     st = gpu
     m = st.SubstitutionMatrix.blosum62()
