@@ -38,8 +38,8 @@ __device__ void run_unit(const BatchJob& J, uint64_t q_begin, uint32_t qt, uint3
         __syncthreads();
     }
     for (uint64_t i = q_begin; i < q_end; ++i) {
-        const uint64_t q0 = J.q_off[i];
-        const uint32_t m = uint32_t(J.q_off[i + 1] - q0);
+        const uint64_t q0 = gb::global_ptr(J.q_off)[i];
+        const uint32_t m = uint32_t(gb::global_ptr(J.q_off)[i + 1] - q0);
         const uint8_t* q = J.q_codes + q0;
         if constexpr (PROFILE) {
             __syncthreads();  // the previous query's profile is no longer read
@@ -55,9 +55,9 @@ __device__ void run_unit(const BatchJob& J, uint64_t q_begin, uint32_t qt, uint3
 #pragma unroll
             for (uint32_t it = 0; it < kSlots; ++it) {
                 const uint32_t slot = c * kChunk + it * THREADS + tid;
-                const uint32_t k = J.refs.orig[slot];
-                const uint32_t n = J.refs.len[slot];
-                const PackedWords words{J.refs.words + slot, J.refs.npad};
+                const uint32_t k = gb::global_ptr(J.refs.orig)[slot];
+                const uint32_t n = gb::global_ptr(J.refs.len)[slot];
+                const PackedWords words{gb::global_ptr(J.refs.words) + slot, J.refs.npad};
                 int32_t score = 0;
                 if constexpr (SHORT) {
                     score = gb::pair_score_short(lds, slice, words, k, m, n, J.prior, J.w1,

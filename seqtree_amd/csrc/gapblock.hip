@@ -89,7 +89,7 @@ __global__ __launch_bounds__(kThreads) void gapblock_kernel(GapblockArgs a, uint
             const uint32_t slot = c * kChunk + it * kThreads + tid;
             const uint32_t k = a.refs.orig[slot];
             const uint32_t n = a.refs.len[slot];
-            const PackedWords words{a.refs.words + slot, a.refs.npad};
+            const PackedWords words{gb::global_ptr(a.refs.words) + slot, a.refs.npad};
             int32_t score = 0;
             if constexpr (SHORT) {
                 score = gb::pair_score_short(lds, slice, words, k, m, n, a.prior, a.w1,

@@ -84,9 +84,20 @@ struct Look {
     }
 };
 
+// Global-memory pointer type. Pointers that reach a kernel through a descriptor in memory
+// (the ragged batch's jobs) are otherwise generic, and generic (flat) loads count against the
+// LDS wait counter as well, serialising them with the penalty lookups.
+template <class T>
+using GlobalPtr = const __attribute__((address_space(1))) T*;
+
+template <class T>
+__device__ __forceinline__ GlobalPtr<T> global_ptr(const T* p) {
+    return (GlobalPtr<T>)p;
+}
+
 // Reference residues as little-endian words of byte offsets: residues 4w..4w+3.
 struct PackedWords {  // the chunk-packed layout, one slot
-    const uint32_t* wp;
+    GlobalPtr<uint32_t> wp;
     uint32_t npad;
     __device__ __forceinline__ uint32_t operator()(uint32_t w) const {
         return wp[uint64_t(w) * npad];
@@ -192,7 +203,7 @@ __device__ __forceinline__ int32_t pair_score(const Look<PROFILE>& look, const W
 template <int MODE, uint32_t NW, class Words>
 __device__ __forceinline__ int32_t pair_score_uniform(const int32_t* prof, const Words& words,
                                                       uint32_t L, uint32_t d,
-                                                      const int32_t* prow, int32_t go,
+                                                      GlobalPtr<int32_t> prow, int32_t go,
                                                       int32_t ge) {
     constexpr uint32_t kT = 4 * NW;
     uint32_t x[NW];
@@ -253,7 +264,8 @@ __device__ __forceinline__ int32_t pair_score_uniform(const int32_t* prof, const
 
 template <int MODE, class Words>
 __device__ __forceinline__ int32_t pair_score_words(const int32_t* prof, const Words& words,
-                                                    uint32_t L, uint32_t d, const int32_t* prow,
+                                                    uint32_t L, uint32_t d,
+                                                    GlobalPtr<int32_t> prow,
                                                     int32_t go, int32_t ge) {
     switch ((L + 3) / 4) {
         case 0:
@@ -277,7 +289,8 @@ __device__ __forceinline__ int32_t pair_score_fast(const int32_t* prof, const Wo
     const uint32_t L = m < n ? m : n;
     const uint32_t d = m > n ? m - n : n - m;
     if (d == 0) return pair_score_words<0>(prof, words, L, 0, nullptr, go, ge);
-    const int32_t* prow = prior ? prior + (uint64_t(m > n ? m : n) * w1 + d) * w1 : nullptr;
+    const GlobalPtr<int32_t> prow =
+        prior ? global_ptr(prior) + (uint64_t(m > n ? m : n) * w1 + d) * w1 : nullptr;
     if (m > n) return pair_score_words<1>(prof, words, L, d, prow, go, ge);
     return pair_score_words<2>(prof, words, L, d, prow, go, ge);
 }
