@@ -226,6 +226,66 @@ std::vector<uint32_t> classify(Arena& ar, const uint64_t* off, uint64_t n, const
 
 int64_t iabs64(int64_t v) { return v < 0 ? -v : v; }
 
+// ---- device copies of small host tables --------------------------------------------------
+// Penalty tables and prior cubes are a few KB to about a MB and usually the same from call to
+// call; uploading them from pageable host memory on every call costs more GPU time than a
+// small score matrix takes to compute. They are kept by content (device, exact values, zero
+// padding) for the life of the process, read-only, so any number of calls on any streams can
+// share one. At most kTableCacheMax entries / kTableCacheBytes; past that a call uploads its
+// own stream-ordered copy as before.
+struct DeviceTable {
+    const int32_t* dev;
+    int64_t abs_max;  // max |entry|
+};
+struct TableEntry {
+    int device;
+    std::vector<int32_t> host;
+    size_t pad;
+    int32_t* dev;
+    int64_t abs_max;
+};
+constexpr size_t kTableCacheMax = 64;
+constexpr size_t kTableCacheBytes = size_t(64) << 20;
+std::mutex g_table_mu;
+std::vector<TableEntry> g_tables;
+size_t g_table_bytes = 0;
+
+DeviceTable table_on_device(Arena& ar, const int32_t* host, size_t count, size_t pad,
+                            hipStream_t s) {
+    int dev = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    {
+        std::lock_guard<std::mutex> lk(g_table_mu);
+        for (const TableEntry& e : g_tables)
+            if (e.device == dev && e.pad == pad && e.host.size() == count &&
+                std::equal(host, host + count, e.host.begin()))
+                return {e.dev, e.abs_max};
+    }
+    int64_t abs_max = 0;
+    for (size_t i = 0; i < count; ++i) abs_max = std::max(abs_max, iabs64(host[i]));
+    const size_t bytes = (count + pad) * sizeof(int32_t);
+    {
+        std::lock_guard<std::mutex> lk(g_table_mu);
+        if (g_tables.size() < kTableCacheMax && g_table_bytes + bytes <= kTableCacheBytes) {
+            std::vector<int32_t> staged(count + pad, 0);
+            std::copy(host, host + count, staged.begin());
+            int32_t* d = nullptr;
+            check(hipMalloc(&d, bytes), "hipMalloc");
+            if (hipMemcpy(d, staged.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+                (void)hipFree(d);
+                check(hipGetLastError(), "table upload");
+            }
+            g_tables.push_back({dev, std::vector<int32_t>(host, host + count), pad, d, abs_max});
+            g_table_bytes += bytes;
+            return {d, abs_max};
+        }
+    }
+    int32_t* d = ar.get<int32_t>(count + pad);
+    h2d(d, host, count, s);
+    if (pad) launch_fill_u32(reinterpret_cast<uint32_t*>(d + count), pad, 0u, s);
+    return {d, abs_max};
+}
+
 // ------------------------------------------------------------------------------------------
 // gap-block
 // ------------------------------------------------------------------------------------------
@@ -278,20 +338,18 @@ GapblockPlan gapblock_plan(Arena& ar, const uint8_t* q, const uint64_t* q_off, u
                        "seqtree_hip: prior_width " + std::to_string(W) +
                            " is below the longest sequence (" +
                            std::to_string(std::max(q_max, r_max)) + ")"};
-    int32_t* pen = ar.get<int32_t>(size_t(A) * A);
-    h2d(pen, pen_h, size_t(A) * A, s);
+    const DeviceTable pen_t = table_on_device(ar, pen_h, size_t(A) * A, 0, s);
+    const int32_t* pen = pen_t.dev;
+    const int64_t pen_abs = pen_t.abs_max;
     const size_t w1 = size_t(W) + 1;
-    int32_t* prior = nullptr;
+    const int32_t* prior = nullptr;
     int64_t prior_abs = 0;
     if (prior_h) {
-        const size_t cube = w1 * w1 * w1;
-        prior = ar.get<int32_t>(cube + gb::kPriorPad);
-        h2d(prior, prior_h, cube, s);
-        launch_fill_u32(reinterpret_cast<uint32_t*>(prior + cube), gb::kPriorPad, 0u, s);
-        for (size_t e = 0; e < cube; ++e) prior_abs = std::max(prior_abs, iabs64(prior_h[e]));
+        // device prior cubes carry kPriorPad ints of zero padding (see gapblock_core.hpp)
+        const DeviceTable pr = table_on_device(ar, prior_h, w1 * w1 * w1, gb::kPriorPad, s);
+        prior = pr.dev;
+        prior_abs = pr.abs_max;
     }
-    int64_t pen_abs = 0;
-    for (size_t e = 0; e < size_t(A) * A; ++e) pen_abs = std::max(pen_abs, iabs64(pen_h[e]));
 
     GapblockPlan p{};
     GapblockArgs& a = p.a;
