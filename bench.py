@@ -154,6 +154,8 @@ class Lib:
             fn.argtypes = [vp, vp, u64, u32, vp, vp, u64, u32, vp, vp, vp]
         L.st_probe_mfma.restype = i32
         L.st_probe_mfma.argtypes = [ctypes.POINTER(ctypes.c_double), vp]
+        L.st_probe_hbm_write.restype = i32
+        L.st_probe_hbm_write.argtypes = [u64, ctypes.POINTER(ctypes.c_double), vp]
         L.st_probe_valu.restype = i32
         L.st_probe_valu.argtypes = [i32, ctypes.c_char_p, u64, ctypes.POINTER(ctypes.c_double),
                                     vp]
@@ -721,6 +723,12 @@ def main() -> int:
         st = Status()
         if lib.lib.st_probe_mfma(ctypes.byref(rate), ctypes.byref(st)) == 0:
             line["roofline"]["probe_tflops_bf16_32x32x16"] = round(rate.value / 1e12, 1)
+    if line.get("roofline") and line["roofline"]["bound"] == "hbm":
+        # the attainable write rate: a plain 16-byte-store fill of 4 GiB
+        rate = ctypes.c_double()
+        st = Status()
+        if lib.lib.st_probe_hbm_write(4 << 30, ctypes.byref(rate), ctypes.byref(st)) == 0:
+            line["roofline"]["probe_write_gbs"] = round(rate.value / 1e9, 1)
     print(json.dumps(line))
     return 0
 

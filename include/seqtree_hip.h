@@ -292,6 +292,10 @@ int32_t st_probe_valu(int32_t kind, char* name, uint64_t name_capacity, double* 
 /* Measured rate of v_mfma_f32_32x32x16_bf16 (four independent chains per wave, registers
  * only) over the whole device, in FLOP/s: the attainable BF16 MFMA peak. */
 int32_t st_probe_mfma(double* flops_per_s, st_status* status);
+/* Measured HBM write rate of a plain 16-byte-store fill of `bytes` device bytes (at least
+ * 1 MiB, a multiple of 16), in bytes per second: the attainable write bandwidth the
+ * write-bound kernels' rooflines are checked against. */
+int32_t st_probe_hbm_write(uint64_t bytes, double* bytes_per_s, st_status* status);
 
 #ifdef __cplusplus
 }

@@ -1309,6 +1309,49 @@ int32_t st_probe_mfma(double* flops_per_s, st_status* status) {
     });
 }
 
+int32_t st_probe_hbm_write(uint64_t bytes, double* bytes_per_s, st_status* status) {
+    return guarded(status, [&] {
+        require_device();
+        if (bytes < (uint64_t(1) << 20) || bytes % 16)
+            throw HipError{ST_INVALID_ARGUMENT,
+                           "seqtree_hip: the write probe takes at least 1 MiB, a multiple of 16"};
+        int dev = 0, cus = 0;
+        check(hipGetDevice(&dev), "hipGetDevice");
+        check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+              "hipDeviceGetAttribute");
+        void* buf = nullptr;
+        check(hipMalloc(&buf, bytes), "hipMalloc");
+        hipEvent_t a = nullptr, b = nullptr;
+        double best = 0.0;
+        try {
+            check(hipEventCreate(&a), "hipEventCreate");
+            check(hipEventCreate(&b), "hipEventCreate");
+            // grids of 4 and 64 workgroups per CU; the first run of each is a warm-up
+            for (uint32_t per_cu : {4u, 64u})
+                for (int rep = 0; rep < 3; ++rep) {
+                    check(hipEventRecord(a, nullptr), "hipEventRecord");
+                    launch_probe_fill(buf, bytes, uint32_t(cus) * per_cu, uint32_t(rep), nullptr);
+                    check(hipGetLastError(), "probe launch");
+                    check(hipEventRecord(b, nullptr), "hipEventRecord");
+                    check(hipEventSynchronize(b), "hipEventSynchronize");
+                    float ms = 0.f;
+                    check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+                    if (rep > 0) best = std::max(best, double(bytes) / (double(ms) * 1e-3));
+                }
+        } catch (...) {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+            (void)hipFree(buf);
+            throw;
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        check(hipFree(buf), "hipFree");
+        *bytes_per_s = best;
+        return ok(status);
+    });
+}
+
 void st_timing_enable(int32_t on) {
     std::lock_guard<std::mutex> lk(g_timing_mu);
     g_timing = on != 0;

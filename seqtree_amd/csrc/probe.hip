@@ -80,6 +80,22 @@ void launch_probe(int kind, uint32_t* out, uint32_t blocks, uint32_t iters, hipS
     hipLaunchKernelGGL(kProbes[kind], dim3(blocks), dim3(256), 0, s, out, iters, 12345u);
 }
 
+// HBM write rate: a grid-stride fill of `n` 16-byte vectors (the store pattern of the
+// write-bound distance kernels).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void probe_fill_kernel(u32x4* __restrict__ p, uint64_t n,
+                                                         uint32_t v) {
+    const uint64_t stride = uint64_t(gridDim.x) * 256;
+    for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride)
+        __builtin_nontemporal_store(u32x4{v, v + 1, v + 2, v + 3}, p + i);
+}
+
+void launch_probe_fill(void* p, uint64_t bytes, uint32_t blocks, uint32_t v, hipStream_t s) {
+    hipLaunchKernelGGL(probe_fill_kernel, dim3(blocks), dim3(256), 0, s,
+                       static_cast<u32x4*>(p), bytes / 16, v);
+}
+
 double launch_probe_mfma(float* out, uint32_t blocks, uint32_t iters, hipStream_t s) {
     hipLaunchKernelGGL(probe_mfma_32x32x16_bf16, dim3(blocks), dim3(256), 0, s, out, iters);
     return double(blocks) * 4 /* waves */ * iters * 4 /* chains */ * 32.0 * 32 * 16 * 2;

@@ -132,6 +132,7 @@ const char* probe_name(int kind);
 void launch_probe(int kind, uint32_t* out, uint32_t blocks, uint32_t iters, hipStream_t s);
 // v_mfma_f32_32x32x16_bf16 chains (256-thread workgroups); returns the FLOPs launched.
 double launch_probe_mfma(float* out, uint32_t blocks, uint32_t iters, hipStream_t s);
+void launch_probe_fill(void* p, uint64_t bytes, uint32_t blocks, uint32_t v, hipStream_t s);
 
 // ---- gapblock.hip ----------------------------------------------------------------------
 // Thresholded output: every pair with score <= cutoff is appended as (query, ref, score);
