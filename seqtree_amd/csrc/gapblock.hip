@@ -84,11 +84,25 @@ __global__ __launch_bounds__(kThreads) void gapblock_kernel(GapblockArgs a, uint
     __syncthreads();
 
     const Look<PROFILE> look{lds, q};
+    // the next slot's column and length are loaded one pair ahead (across chunks too)
+    uint32_t next_k = 0, next_n = 0;
+    if (g * a.group < c_end) {
+        next_k = a.refs.orig[g * a.group * kChunk + tid];
+        next_n = a.refs.len[g * a.group * kChunk + tid];
+    }
     for (uint32_t c = g * a.group; c < c_end; ++c) {
         for (uint32_t it = 0; it < kSlotsPerThread; ++it) {
             const uint32_t slot = c * kChunk + it * kThreads + tid;
-            const uint32_t k = a.refs.orig[slot];
-            const uint32_t n = a.refs.len[slot];
+            const uint32_t k = next_k;
+            const uint32_t n = next_n;
+            {
+                const uint32_t it2 = it + 1 < kSlotsPerThread ? it + 1 : 0;
+                const uint32_t c2 = it + 1 < kSlotsPerThread ? c : c + 1;
+                if (c2 < c_end) {
+                    next_k = a.refs.orig[c2 * kChunk + it2 * kThreads + tid];
+                    next_n = a.refs.len[c2 * kChunk + it2 * kThreads + tid];
+                }
+            }
             const PackedWords words{gb::global_ptr(a.refs.words) + slot, a.refs.npad};
             int32_t score = 0;
             if constexpr (SHORT) {

@@ -343,6 +343,7 @@ __device__ __forceinline__ int32_t pair_score_mixed(const int32_t* prof, const i
 
 // One reference slot of a short launch: the uniform fast path when the wave's references
 // share one length, else the mixed path. m is the workgroup's query length (<= kFastMaxL).
+// The wave must hold 64 consecutive slots of the chunk-packed layout.
 template <class Words>
 __device__ __forceinline__ int32_t pair_score_short(const int32_t* prof, const int32_t* slice,
                                                     const Words& words, uint32_t k, uint32_t m,
@@ -350,13 +351,13 @@ __device__ __forceinline__ int32_t pair_score_short(const int32_t* prof, const i
                                                     uint32_t w1, int32_t go, int32_t ge) {
     const uint32_t n0 = __builtin_amdgcn_readfirstlane(n);
     if (__all(k == kNoSlot || n == n0)) return pair_score_fast(prof, words, m, n0, prior, w1, go, ge);
-    uint32_t nmax = n;
-#pragma unroll
-    for (uint32_t o = 32; o > 0; o >>= 1) {
-        const uint32_t v = __shfl_xor(nmax, int(o));
-        nmax = v > nmax ? v : nmax;
-    }
-    const uint32_t Lmax = __builtin_amdgcn_readfirstlane(m < nmax ? m : nmax);
+    // The wave's longest reference: a wave holds 64 consecutive slots of a length-sorted
+    // chunk whose padding slots (length 0) come last, so it is the length in its last real
+    // slot (and some lane is real here: an all-padding wave takes the uniform path).
+    const uint64_t real = __ballot(k != kNoSlot);
+    const uint32_t last = 63u - uint32_t(__builtin_clzll(real));
+    const uint32_t nmax = uint32_t(__builtin_amdgcn_readlane(int(n), int(last)));
+    const uint32_t Lmax = m < nmax ? m : nmax;
     switch ((Lmax + 3) / 4) {
         case 0:
         case 1: return pair_score_mixed<1>(prof, slice, words, m, n, go, ge);
