@@ -60,8 +60,10 @@ __device__ void run_unit(const BatchJob& J, uint64_t q_begin, uint32_t qt, uint3
                 const PackedWords words{gb::global_ptr(J.refs.words) + slot, J.refs.npad};
                 int32_t score = 0;
                 if constexpr (SHORT) {
-                    score = gb::pair_score_short(lds, slice, words, k, m, n, J.prior, J.w1,
-                                                 J.gap_open, J.gap_extend);
+                    score = gb::pair_score_short(
+                        lds, slice,
+                        gb::buffer_words(J.refs.words, J.refs.npad, J.refs.nwords, slot), k, m,
+                        n, J.prior, J.w1, J.gap_open, J.gap_extend);
                 } else {
                     const uint32_t n0 = __builtin_amdgcn_readfirstlane(n);
                     if (PROFILE && !EXACT && __all(k == kNoSlot || n == n0) &&

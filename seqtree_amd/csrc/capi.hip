@@ -320,9 +320,11 @@ uint32_t chunk_group(uint64_t n_queries, uint32_t nchunks, uint32_t nwords) {
 // keeps D + big from wrapping, and 6T + prior_abs < INT32_MAX puts D + big above every real
 // candidate D + prior <= 2T + prior_abs.
 bool short_launch(uint32_t q_max, uint32_t r_max, uint32_t A, bool exact, int64_t pen_abs,
-                  int64_t prior_abs, int32_t* big) {
+                  int64_t prior_abs, const PackedRefs& refs, int32_t* big) {
     const int64_t T = int64_t(std::max(q_max, r_max)) * pen_abs;
     if (exact || A > 31 || q_max > gb::kFastMaxL || r_max > gb::kShortMaxRef) return false;
+    // the short kernels read the packed words through one buffer resource
+    if (uint64_t(refs.nwords + 1) * refs.npad * 4 > gb::kBufferMaxBytes) return false;
     if (6 * T + prior_abs >= int64_t(INT32_MAX)) return false;
     *big = int32_t(int64_t(INT32_MAX) - 2 * T);
     return true;
@@ -370,7 +372,8 @@ GapblockPlan gapblock_plan(Arena& ar, const uint8_t* q, const uint64_t* q_off, u
     const int64_t worst = 2 * Lmax * pen_abs + prior_abs + int64_t(go) + Lmax * int64_t(ge);
     p.exact = worst >= int64_t(INT32_MAX);
     p.profile = size_t(q_max) * gb::kRowBytes <= 48 * 1024;  // one table row per position
-    p.short_path = pack && short_launch(q_max, r_max, A, p.exact, pen_abs, prior_abs, &a.big);
+    p.short_path =
+        pack && short_launch(q_max, r_max, A, p.exact, pen_abs, prior_abs, a.refs, &a.big);
     a.group = pack ? chunk_group(N, a.refs.nchunks, a.refs.nwords) : 1;
     if (p.short_path) {
         a.prof_rows = q_max + gb::kFastMaxL;
@@ -1648,7 +1651,7 @@ int32_t st_gapblock_batch_plan_create(const st_gapblock_job* jobs, uint32_t n_jo
             lds = std::max(lds, size_t(d.profile ? j.q_max_len : A) * gb::kRowBytes);
             int32_t big = 0;
             d.short_path = short_launch(j.q_max_len, j.r_max_len, A, d.exact, pen_abs, prior_abs,
-                                        &big);
+                                        pr, &big);
             if (d.short_path) {
                 d.big = big;
                 d.prof_rows = j.q_max_len + gb::kFastMaxL;

@@ -106,8 +106,9 @@ __global__ __launch_bounds__(kThreads) void gapblock_kernel(GapblockArgs a, uint
             const PackedWords words{gb::global_ptr(a.refs.words) + slot, a.refs.npad};
             int32_t score = 0;
             if constexpr (SHORT) {
-                score = gb::pair_score_short(lds, slice, words, k, m, n, a.prior, a.w1,
-                                             a.gap_open, a.gap_extend);
+                score = gb::pair_score_short(
+                    lds, slice, gb::buffer_words(a.refs.words, a.refs.npad, a.refs.nwords, slot),
+                    k, m, n, a.prior, a.w1, a.gap_open, a.gap_extend);
             } else {
                 // one reference length across the wave (padding lanes excepted): uniform path
                 const uint32_t n0 = __builtin_amdgcn_readfirstlane(n);

@@ -102,7 +102,38 @@ struct PackedWords {  // the chunk-packed layout, one slot
     __device__ __forceinline__ uint32_t operator()(uint32_t w) const {
         return wp[uint64_t(w) * npad];
     }
+    // the same slot's words from word `w` on
+    __device__ __forceinline__ PackedWords shift(uint32_t w) const {
+        return {wp + uint64_t(w) * npad, npad};
+    }
 };
+
+// The chunk-packed layout through a buffer resource over the whole words array (short
+// launches; the host keeps the array under 2^31 bytes): the lane's word w is at byte
+// voff + w * stride with stride = npad * 4, so the word index goes into the load's scalar
+// offset and a load costs no vector address arithmetic.
+struct BufferWords {
+    __amdgpu_buffer_rsrc_t rsrc;
+    uint32_t voff;
+    uint32_t stride;
+    __device__ __forceinline__ uint32_t operator()(uint32_t w) const {
+        return uint32_t(__builtin_amdgcn_raw_buffer_load_b32(rsrc, voff, w * stride, 0));
+    }
+    __device__ __forceinline__ BufferWords shift(uint32_t w) const {
+        return {rsrc, voff + w * stride, stride};
+    }
+};
+
+// Buffer resource word 3 for raw (untyped) loads on gfx9-family parts.
+constexpr int kBufferFlags = 0x00020000;
+constexpr uint64_t kBufferMaxBytes = (uint64_t(1) << 31) - 1;
+
+__device__ __forceinline__ BufferWords buffer_words(const uint32_t* words, uint32_t npad,
+                                                    uint32_t nwords, uint32_t slot) {
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint32_t*>(words), 0, int(uint64_t(nwords + 1) * npad * 4), kBufferFlags);
+    return {r, slot * 4u, npad * 4u};
+}
 
 struct CsrWords {  // straight from the CSR codes (pair-list mode)
     const uint8_t* r;
@@ -222,9 +253,10 @@ __device__ __forceinline__ int32_t pair_score_uniform(const int32_t* prof, const
         if constexpr (MODE == 2) {
             // reference longer: the diagonal-d residue of position t is r[t + d]
             const uint32_t dq = d >> 2, sh = d & 3u;
+            const auto wz = words.shift(dq);
             uint32_t z[NW + 1];
 #pragma unroll
-            for (uint32_t j = 0; j <= NW; ++j) z[j] = words(dq + j);
+            for (uint32_t j = 0; j <= NW; ++j) z[j] = wz(j);
 #pragma unroll
             for (uint32_t w = 0; w < NW; ++w) y[w] = __builtin_amdgcn_alignbyte(z[w + 1], z[w], sh);
         }
@@ -313,8 +345,9 @@ __device__ __forceinline__ int32_t pair_score_mixed(const int32_t* prof, const i
     uint32_t x[NW], z[NW + 1];
 #pragma unroll
     for (uint32_t w = 0; w < NW; ++w) x[w] = words(w);
+    const auto wz = words.shift(dq);
 #pragma unroll
-    for (uint32_t j = 0; j <= NW; ++j) z[j] = words(dq + j);
+    for (uint32_t j = 0; j <= NW; ++j) z[j] = wz(j);
     uint32_t D = 0, nS = 0;  // D = P0 - Pd, nS = -Pd
     int32_t best = prow[0];
 #pragma unroll
