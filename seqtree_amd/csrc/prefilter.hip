@@ -639,21 +639,27 @@ __device__ __forceinline__ void gram_tile(const GramSlot* cur, uint32_t* qq, uin
         }
         const uint32_t r = base + row;
         const float hr = r < r_end ? 0.5f * norm : __builtin_inff();
-        float g[2][4];  // maxima over the accumulators of rows 8k .. 8k + 7 of each block
+        // hr >= 0, and on non-negative floats the float order is the order of the bit patterns
+        // as signed integers, below which every negative float's pattern lies: acc >= hr
+        // exactly when bits(acc) >= bits(hr) as int32. The maxima are taken on the bit
+        // patterns (integer max, no NaN canonicalisation; the accumulators hold no NaN).
+        const int32_t hr_i = __float_as_int(hr);
+        int32_t g[2][4];  // maxima over the accumulators of rows 8k .. 8k + 7 of each block
 #pragma unroll
         for (uint32_t b = 0; b < 2; ++b)
 #pragma unroll
             for (uint32_t k = 0; k < 4; ++k)
-                g[b][k] = fmaxf(fmaxf(acc[b][4 * k], acc[b][4 * k + 1]),
-                                fmaxf(acc[b][4 * k + 2], acc[b][4 * k + 3]));
-        const float m = fmaxf(fmaxf(fmaxf(g[0][0], g[0][1]), fmaxf(g[0][2], g[0][3])),
-                              fmaxf(fmaxf(g[1][0], g[1][1]), fmaxf(g[1][2], g[1][3])));
-        if (!__any(m >= hr)) continue;
+                g[b][k] = max(max(__float_as_int(acc[b][4 * k]), __float_as_int(acc[b][4 * k + 1])),
+                              max(__float_as_int(acc[b][4 * k + 2]),
+                                  __float_as_int(acc[b][4 * k + 3])));
+        const int32_t m = max(max(max(g[0][0], g[0][1]), max(g[0][2], g[0][3])),
+                              max(max(g[1][0], g[1][1]), max(g[1][2], g[1][3])));
+        if (!__any(m >= hr_i)) continue;
 #pragma unroll
         for (uint32_t b = 0; b < 2; ++b) {
 #pragma unroll
             for (uint32_t k = 0; k < 4; ++k) {
-                if (!__any(g[b][k] >= hr)) continue;
+                if (!__any(g[b][k] >= hr_i)) continue;
 #pragma unroll
                 for (uint32_t j = 0; j < 4; ++j) {
                     const bool hit = acc[b][4 * k + j] >= hr;
