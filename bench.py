@@ -571,7 +571,7 @@ def run_prefilter(torch, dev, lib, stream, args) -> dict:
         flops = 2.0 * rows * 128 * (5 * 32 + 2 * 5 * 128) / max(launches / args.steps, 1)
     achieved = flops / avg_s / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": 2500.0,
-                "unit": "TFLOP/s", "frac": round(achieved / 2500.0, 4), "traffic": None,
+                "unit": "TFLOP/s", "frac": round(achieved / 2500.0, 4), "traffic": traffic_for(label),
                 "kernel": label, "launches": launches, "avg_launch_ms": round(avg_s * 1e3, 4),
                 "flops_per_launch": flops}
     cpu = None
