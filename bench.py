@@ -197,13 +197,12 @@ class DeviceSet:
 
 
 def time_steps(torch, lib: Lib, stream, step, steps: int, warmup: int):
-    """Untimed warmup, then `steps` steps bracketed by HIP events on `stream`; per-kernel
-    launch times come from the library's own events on the same stream."""
+    """Untimed warmup, then `steps` steps bracketed by HIP events on `stream` with the
+    library's launch timing off (the step time: no per-launch events on the host path), then
+    `steps` more with it on, whose per-kernel launch times come from the library's own events
+    on the same stream."""
     for _ in range(warmup):
         step()
-    torch.cuda.synchronize()
-    lib.lib.st_timing_enable(1)
-    lib.timing()  # drop anything recorded before the timed region
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     ev0.record(stream)
@@ -211,8 +210,14 @@ def time_steps(torch, lib: Lib, stream, step, steps: int, warmup: int):
         step()
     ev1.record(stream)
     torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / steps
+    lib.lib.st_timing_enable(1)
+    lib.timing()  # drop anything recorded before
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
     lib.lib.st_timing_enable(0)
-    return ev0.elapsed_time(ev1) / steps, lib.timing()
+    return ms, lib.timing()
 
 
 def traffic_for(label: str):

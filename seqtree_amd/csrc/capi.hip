@@ -99,6 +99,9 @@ bool pools_supported() {
 
 // Stream-ordered scratch for one call. Everything allocated here is released on the same
 // stream when the arena goes out of scope (after a stream sync when pools are unavailable).
+// Requests of at most kSmall bytes are carved (256-byte aligned) out of shared blocks of
+// kBlock bytes, larger ones get their own allocation: a stream-ordered free costs the host
+// ~10-20 us on this runtime, so a call with a handful of small buffers frees once.
 class Arena {
 public:
     explicit Arena(hipStream_t s) : s_(s) {}
@@ -113,19 +116,35 @@ public:
     }
     template <class T>
     T* get(size_t count) {
+        const size_t bytes = (std::max<size_t>(count * sizeof(T), 1) + 255) / 256 * 256;
+        if (bytes > kSmall) return static_cast<T*>(raw(bytes));
+        if (block_ == nullptr || used_ + bytes > kBlock) {
+            block_ = static_cast<char*>(raw(kBlock));
+            used_ = 0;
+        }
+        T* p = reinterpret_cast<T*>(block_ + used_);
+        used_ += bytes;
+        return p;
+    }
+
+private:
+    static constexpr size_t kSmall = size_t(256) << 10;
+    static constexpr size_t kBlock = size_t(1) << 20;
+
+    void* raw(size_t bytes) {
         void* p = nullptr;
-        const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
         if (pools_supported())
             check(hipMallocAsync(&p, bytes, s_), "hipMallocAsync");
         else
             check(hipMalloc(&p, bytes), "hipMalloc");
         bufs_.push_back(p);
-        return static_cast<T*>(p);
+        return p;
     }
 
-private:
     hipStream_t s_;
     std::vector<void*> bufs_;
+    char* block_ = nullptr;
+    size_t used_ = 0;
 };
 
 class OwnedStream {
@@ -296,7 +315,18 @@ struct GapblockPlan {
     bool exact;
     bool profile;
     bool short_path;
+    bool twin;  // short launch with two queries of one length per workgroup
 };
+
+// Twin rows (two queries in the 16-bit halves, gapblock_core.hpp) need every sum of a short
+// launch inside int16: 6 T + R < 2^15 with T = q_max * max|pen| and R = max|prior|. Sets the
+// 16-bit sentinel.
+bool twin_launch(uint64_t N, uint32_t q_max, int64_t pen_abs, int64_t prior_abs, int32_t* big) {
+    const int64_t T = int64_t(q_max) * pen_abs;
+    if (N < 2 || q_max > kTwinMaxLen || 6 * T + prior_abs >= 32767) return false;
+    *big = int32_t(32767 - 2 * T);
+    return true;
+}
 
 // Reference chunks per gap-block workgroup: at most what keeps a group's packed words around
 // 512 KB (the workgroups resident together stream one group, see gapblock_kernel), at most
@@ -378,11 +408,26 @@ GapblockPlan gapblock_plan(Arena& ar, const uint8_t* q, const uint64_t* q_off, u
     if (p.short_path) {
         a.prof_rows = q_max + gb::kFastMaxL;
         a.slice_rows = r_max + 1;
+        p.twin = twin_launch(N, q_max, pen_abs, prior_abs, &a.big);
     }
     return p;
 }
 
-void gapblock_run(const GapblockPlan& p, uint32_t q_max, hipStream_t s) {
+void gapblock_run(Arena& ar, const GapblockPlan& p, uint32_t q_max, hipStream_t s) {
+    if (p.twin) {
+        // one scratch block: the twins (8-byte entries) first, then the length-sorted indices
+        const uint64_t cap = twin_capacity(p.a.n_queries);
+        uint2* twins = ar.get<uint2>(cap + (p.a.n_queries + 1) / 2);
+        uint32_t* sorted = reinterpret_cast<uint32_t*>(twins + cap);
+        {
+            LaunchTimer t("gapblock_twin_list", s);
+            launch_twin_queries(p.a.q_off, p.a.n_queries, sorted, twins, s);
+        }
+        LaunchTimer t(std::string(p.a.hits.count ? "gapblock_threshold" : "gapblock") + "_twin",
+                      s);
+        launch_gapblock_twin(p.a, twins, cap, s);
+        return;
+    }
     LaunchTimer t(std::string(p.a.hits.count ? "gapblock_threshold" : "gapblock") +
                       (p.short_path ? "_short" : p.profile ? "_profile" : "_table") +
                       (p.exact ? "_exact" : ""),
@@ -399,7 +444,7 @@ void gapblock_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, uint32
     GapblockPlan p = gapblock_plan(ar, q, q_off, N, q_max, r, r_off, K, r_max, pen_h, A, go, ge,
                                    prior_h, W, true, s);
     p.a.out = out;
-    gapblock_run(p, q_max, s);
+    gapblock_run(ar, p, q_max, s);
     launched("gapblock");
 }
 
@@ -414,7 +459,7 @@ void gapblock_threshold_device(const uint8_t* q, const uint64_t* q_off, uint64_t
     GapblockPlan p = gapblock_plan(ar, q, q_off, N, q_max, r, r_off, K, r_max, pen_h, A, go, ge,
                                    prior_h, W, true, s);
     p.a.hits = hits;
-    gapblock_run(p, q_max, s);
+    gapblock_run(ar, p, q_max, s);
     launched("gapblock_threshold");
 }
 

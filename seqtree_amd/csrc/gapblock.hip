@@ -139,6 +139,81 @@ __global__ __launch_bounds__(kThreads) void gapblock_kernel(GapblockArgs a, uint
     }
 }
 
+// Twin short launch: the short layout with two queries of one length per workgroup, packed
+// into the 16-bit halves of the profile and the prior slice (twin rows, gapblock_core.hpp).
+// Workgroup = (twin p, chunk group g), group-major like gapblock_kernel; twins[p] = (a, b)
+// with b == a for the last query of a length with an odd count (one row written), and
+// (kNoSlot, kNoSlot) past the last twin (the workgroup exits before any barrier).
+template <bool THRESH>
+__global__ __launch_bounds__(kThreads) void gapblock_twin_kernel(GapblockArgs a,
+                                                                 const uint2* __restrict__ twins,
+                                                                 uint32_t nt) {
+    extern __shared__ int32_t lds[];
+    __shared__ int32_t stage[THRESH ? 1 : 2 * kChunk];
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nch = a.refs.nchunks;
+    const uint32_t g = blockIdx.x / nt;
+    const uint2 tw = twins[blockIdx.x % nt];
+    if (tw.x == kNoSlot) return;
+    const uint32_t c_end = (g + 1) * a.group < nch ? (g + 1) * a.group : nch;
+    const uint64_t ia = tw.x, ib = tw.y;
+    const uint64_t qa0 = a.q_off[ia], qb0 = a.q_off[ib];
+    const uint32_t m = uint32_t(a.q_off[ia + 1] - qa0);
+    uint32_t* prof = reinterpret_cast<uint32_t*>(lds);
+    uint32_t* slice = prof + a.prof_rows * gb::kRowInts;
+    gb::build_rows2(prof, a.prof_rows, m, a.pen, a.A, a.q_codes + qa0, a.q_codes + qb0, tid,
+                    kThreads);
+    gb::build_slice2(slice, a.slice_rows - 1, m, a.prior, a.w1, a.big, tid, kThreads);
+    __syncthreads();
+
+    uint32_t next_k = 0, next_n = 0;
+    if (g * a.group < c_end) {
+        next_k = a.refs.orig[g * a.group * kChunk + tid];
+        next_n = a.refs.len[g * a.group * kChunk + tid];
+    }
+    for (uint32_t c = g * a.group; c < c_end; ++c) {
+        for (uint32_t it = 0; it < kSlotsPerThread; ++it) {
+            const uint32_t slot = c * kChunk + it * kThreads + tid;
+            const uint32_t k = next_k;
+            const uint32_t n = next_n;
+            {
+                const uint32_t it2 = it + 1 < kSlotsPerThread ? it + 1 : 0;
+                const uint32_t c2 = it + 1 < kSlotsPerThread ? c : c + 1;
+                if (c2 < c_end) {
+                    next_k = a.refs.orig[c2 * kChunk + it2 * kThreads + tid];
+                    next_n = a.refs.len[c2 * kChunk + it2 * kThreads + tid];
+                }
+            }
+            int32_t sa = 0, sb = 0;
+            gb::twin_score_short(prof, slice,
+                                 gb::buffer_words(a.refs.words, a.refs.npad, a.refs.nwords, slot),
+                                 k, m, n, a.gap_open, a.gap_extend, sa, sb);
+            if constexpr (THRESH) {
+                append_hit(k != kNoSlot && sa <= a.hits.cutoff, uint32_t(ia), k, sa, a.hits);
+                if (ib != ia)
+                    append_hit(k != kNoSlot && sb <= a.hits.cutoff, uint32_t(ib), k, sb, a.hits);
+            } else if (k != kNoSlot) {
+                stage[k - c * kChunk] = sa;
+                stage[kChunk + k - c * kChunk] = sb;
+            }
+        }
+        if constexpr (!THRESH) {
+            __syncthreads();
+            const uint64_t col0 = uint64_t(c) * kChunk;
+            const uint64_t lim64 = a.n_refs - col0;
+            const uint32_t lim = lim64 < kChunk ? uint32_t(lim64) : kChunk;
+            int32_t* row_a = a.out + ia * a.n_refs + col0;
+            for (uint32_t t = tid; t < lim; t += kThreads) row_a[t] = stage[t];
+            if (ib != ia) {
+                int32_t* row_b = a.out + ib * a.n_refs + col0;
+                for (uint32_t t = tid; t < lim; t += kThreads) row_b[t] = stage[kChunk + t];
+            }
+            __syncthreads();  // the stage is rewritten by the next chunk
+        }
+    }
+}
+
 // Pair-list mode: thread t scores (pair_query[t], pair_ref[t]) straight from the CSR sets with
 // the A x A penalty table in LDS.
 template <bool EXACT>
@@ -200,6 +275,25 @@ void launch_gapblock(const GapblockArgs& a, bool use_profile, uint32_t q_max_len
     const size_t lds_bytes = lds ? lds : sizeof(int32_t);
     if (a.hits.count) launch_mode<true>(a, use_profile, lds_bytes, exact_wrap, short_path, s);
     else launch_mode<false>(a, use_profile, lds_bytes, exact_wrap, short_path, s);
+}
+
+void launch_gapblock_twin(const GapblockArgs& a, const uint2* twins, uint64_t n_twins,
+                          hipStream_t s) {
+    if (n_twins == 0 || a.refs.nchunks == 0) return;
+    const size_t lds = size_t(a.prof_rows) * gb::kRowBytes +
+                       size_t(a.slice_rows) * gb::kSliceStride * sizeof(uint32_t);
+    const uint64_t ngroups = (a.refs.nchunks + a.group - 1) / a.group;
+    const uint64_t max_t = (uint64_t(1) << 30) / ngroups;  // launches under 2^30 workgroups
+    for (uint64_t t0 = 0; t0 < n_twins; t0 += max_t) {
+        const uint32_t nt = uint32_t(n_twins - t0 < max_t ? n_twins - t0 : max_t);
+        const dim3 grid(uint32_t(nt * ngroups));
+        if (a.hits.count)
+            hipLaunchKernelGGL(gapblock_twin_kernel<true>, grid, dim3(kThreads), lds, s, a,
+                               twins + t0, nt);
+        else
+            hipLaunchKernelGGL(gapblock_twin_kernel<false>, grid, dim3(kThreads), lds, s, a,
+                               twins + t0, nt);
+    }
 }
 
 void launch_gapblock_pairs(const GapblockArgs& a, const uint8_t* r_codes, const uint64_t* r_off,

@@ -404,5 +404,166 @@ __device__ __forceinline__ int32_t pair_score_short(const int32_t* prof, const i
     }
 }
 
+// ---- twin rows: two queries of one length in the 16-bit halves of every word ------------
+// Short launches can score two queries a and b of the same length m at once: every table
+// word holds query a's value in its low half and query b's in its high half (profile rows
+// pen[q_a[t]][c] | pen[q_b[t]][c] << 16 and the negations; prior-slice entries duplicated).
+// With equal lengths the two pairs share d, the diagonal and the prior row, so one lookup,
+// one address and one packed add or min serve both. The host admits a twin launch only when
+// 6 T + R < 2^15 (T = q_max * max|pen|, R = max|prior|, the short-launch bound at 16 bits):
+// then |D| <= 2T, |nS| <= T, real candidates lie in [-2T - R, 2T + R], the sentinel
+// big = 2^15 - 1 - 2T keeps D + big in (2T + R, 2^15 - 1], and no half ever wraps, so the
+// signed 16-bit minimum and the final int32 combination equal the int32 path bit for bit.
+using u16x2 = uint16_t __attribute__((ext_vector_type(2)));
+using i16x2 = int16_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t pack16(int32_t lo, int32_t hi) {
+    return (uint32_t(lo) & 0xFFFFu) | (uint32_t(hi) << 16);
+}
+
+__device__ __forceinline__ u16x2 lds_at2(const uint32_t* base, uint32_t byte_off) {
+    return __builtin_bit_cast(
+        u16x2, *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(base) + byte_off));
+}
+
+// build_rows for two queries of length `valid` (codes qa, qb), packed as above.
+__device__ __forceinline__ void build_rows2(uint32_t* lds, uint32_t rows, uint32_t valid,
+                                            const int32_t* pen, uint32_t A, const uint8_t* qa,
+                                            const uint8_t* qb, uint32_t tid, uint32_t threads) {
+    for (uint32_t e = tid; e < rows * kRowInts; e += threads) {
+        const uint32_t r = e / kRowInts, c = e % kRowInts, b = c % 32;
+        int32_t va = 0, vb = 0;
+        if (r < valid && b < A) {
+            va = pen[uint32_t(qa[r]) * A + b];
+            vb = pen[uint32_t(qb[r]) * A + b];
+        }
+        lds[e] = c < 32 ? pack16(va, vb) : pack16(-va, -vb);
+    }
+}
+
+// build_slice with every entry duplicated into both halves (big: the 16-bit sentinel).
+__device__ __forceinline__ void build_slice2(uint32_t* slice, uint32_t r_max, uint32_t m,
+                                             const int32_t* prior, uint32_t w1, int32_t big,
+                                             uint32_t tid, uint32_t threads) {
+    for (uint32_t e = tid; e < (r_max + 1) * kSliceStride; e += threads) {
+        const uint32_t n = e / kSliceStride, i = e % kSliceStride;
+        const uint32_t L = m < n ? m : n, d = m > n ? m - n : n - m;
+        int32_t v = big;
+        if (i <= L) v = (prior && d) ? prior[(uint64_t(m > n ? m : n) * w1 + d) * w1 + i] : 0;
+        slice[e] = pack16(v, v);
+    }
+}
+
+// The int32 result of one half: Sd = -nS, then Sd + best + go + (d - 1) ge (d > 0), in the
+// wrapping order of pair_score_mixed.
+__device__ __forceinline__ int32_t twin_finish(int16_t ns, int16_t best, uint32_t d, int32_t go,
+                                               int32_t ge) {
+    const uint32_t Sd = 0u - uint32_t(int32_t(ns));
+    return d == 0 ? int32_t(Sd)
+                  : int32_t(Sd + uint32_t(int32_t(best)) + uint32_t(go) + (d - 1) * uint32_t(ge));
+}
+
+// pair_score_mixed on twin rows: per-lane n, one instruction stream of 4 * NW positions.
+template <uint32_t NW, class Words>
+__device__ __forceinline__ void twin_score_mixed(const uint32_t* prof, const uint32_t* slice,
+                                                 const Words& words, uint32_t m, uint32_t n,
+                                                 int32_t go, int32_t ge, int32_t& sa,
+                                                 int32_t& sb) {
+    const uint32_t d = m > n ? m - n : n - m;
+    const bool ql = m >= n;
+    const uint32_t dq = ql ? 0u : d >> 2, sh = ql ? 0u : d & 3u;
+    const uint32_t off1 = (ql ? d * kRowBytes : 0u) + kNegBytes;
+    const uint32_t* prow = slice + n * kSliceStride;
+    uint32_t x[NW], z[NW + 1];
+#pragma unroll
+    for (uint32_t w = 0; w < NW; ++w) x[w] = words(w);
+    const auto wz = words.shift(dq);
+#pragma unroll
+    for (uint32_t j = 0; j <= NW; ++j) z[j] = wz(j);
+    u16x2 D = {0, 0}, nS = {0, 0};  // per half: D = P0 - Pd, nS = -Pd
+    i16x2 best = __builtin_bit_cast(i16x2, prow[0]);
+#pragma unroll
+    for (uint32_t w = 0; w < NW; ++w) {
+        const uint32_t y = __builtin_amdgcn_alignbyte(z[w + 1], z[w], sh);
+#pragma unroll
+        for (uint32_t b = 0; b < 4; ++b) {
+            const uint32_t t = 4 * w + b;
+            const uint32_t rc4 = (x[w] >> (8 * b)) & 0xFFu;
+            const uint32_t rd4 = (y >> (8 * b)) & 0xFFu;
+            const u16x2 p0 = lds_at2(prof, t * kRowBytes + rc4);
+            const u16x2 npd = lds_at2(prof, t * kRowBytes + (rd4 + off1));
+            D += p0;
+            D += npd;
+            nS += npd;
+            const i16x2 cand = __builtin_bit_cast(i16x2, D + __builtin_bit_cast(u16x2, prow[t + 1]));
+            best = __builtin_elementwise_min(best, cand);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    const i16x2 ns = __builtin_bit_cast(i16x2, nS);
+    sa = twin_finish(ns.x, best.x, d, go, ge);
+    sb = twin_finish(ns.y, best.y, d, go, ge);
+}
+
+// Every real lane has n == m (d == 0): the two sums only, over the m (wave-uniform) positions.
+template <uint32_t NW, class Words>
+__device__ __forceinline__ void twin_score_sum(const uint32_t* prof, const Words& words,
+                                               uint32_t m, int32_t& sa, int32_t& sb) {
+    uint32_t x[NW];
+#pragma unroll
+    for (uint32_t w = 0; w < NW; ++w) x[w] = words(w);
+    u16x2 S = {0, 0};
+#pragma unroll
+    for (uint32_t t = 0; t < 4 * NW; ++t) {
+        if (t + 4 >= 4 * NW && t >= m) break;  // only the last word can be partial
+        S += lds_at2(prof, t * kRowBytes + ((x[t / 4] >> (8 * (t % 4))) & 0xFFu));
+    }
+    const i16x2 s = __builtin_bit_cast(i16x2, S);
+    sa = s.x;
+    sb = s.y;
+}
+
+// pair_score_short on twin rows: both scores of reference slot k against the twin queries
+// (length m <= kFastMaxL). The wave must hold 64 consecutive slots of the chunk-packed layout.
+template <class Words>
+__device__ __forceinline__ void twin_score_short(const uint32_t* prof, const uint32_t* slice,
+                                                 const Words& words, uint32_t k, uint32_t m,
+                                                 uint32_t n, int32_t go, int32_t ge, int32_t& sa,
+                                                 int32_t& sb) {
+    const uint64_t real = __ballot(k != kNoSlot);
+    if (real == 0) {  // an all-padding wave
+        sa = sb = 0;
+        return;
+    }
+    if (__all(k == kNoSlot || n == m)) {
+        switch ((m + 3) / 4) {
+            case 0:
+            case 1: return twin_score_sum<1>(prof, words, m, sa, sb);
+            case 2: return twin_score_sum<2>(prof, words, m, sa, sb);
+            case 3: return twin_score_sum<3>(prof, words, m, sa, sb);
+            case 4: return twin_score_sum<4>(prof, words, m, sa, sb);
+            case 5: return twin_score_sum<5>(prof, words, m, sa, sb);
+            case 6: return twin_score_sum<6>(prof, words, m, sa, sb);
+            case 7: return twin_score_sum<7>(prof, words, m, sa, sb);
+            default: return twin_score_sum<8>(prof, words, m, sa, sb);
+        }
+    }
+    // the wave's longest reference is the length in its last real slot (as pair_score_short)
+    const uint32_t last = 63u - uint32_t(__builtin_clzll(real));
+    const uint32_t nmax = uint32_t(__builtin_amdgcn_readlane(int(n), int(last)));
+    const uint32_t Lmax = m < nmax ? m : nmax;
+    switch ((Lmax + 3) / 4) {
+        case 0:
+        case 1: return twin_score_mixed<1>(prof, slice, words, m, n, go, ge, sa, sb);
+        case 2: return twin_score_mixed<2>(prof, slice, words, m, n, go, ge, sa, sb);
+        case 3: return twin_score_mixed<3>(prof, slice, words, m, n, go, ge, sa, sb);
+        case 4: return twin_score_mixed<4>(prof, slice, words, m, n, go, ge, sa, sb);
+        case 5: return twin_score_mixed<5>(prof, slice, words, m, n, go, ge, sa, sb);
+        case 6: return twin_score_mixed<6>(prof, slice, words, m, n, go, ge, sa, sb);
+        case 7: return twin_score_mixed<7>(prof, slice, words, m, n, go, ge, sa, sb);
+        default: return twin_score_mixed<8>(prof, slice, words, m, n, go, ge, sa, sb);
+    }
+}
+
 }  // namespace gb
 }  // namespace st

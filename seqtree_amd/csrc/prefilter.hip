@@ -733,8 +733,19 @@ __device__ __forceinline__ void gram_pipeline(GramSlot* __restrict__ s0,
     }
 }
 
-// Grid: q_blocks x groups workgroups, query block fastest (the workgroups of one reference
-// group run together and share its tiles in L2).
+// Grid: q_blocks x groups workgroups in group-major order, query block fastest, renumbered
+// per XCD (xcd_order): the workgroups of one reference group run together on the same XCD and
+// share its tiles in that XCD's L2.
+// The dispatcher hands workgroup b of a launch to XCD b % 8 (MI355X: 8 XCDs of 32 CUs, each
+// with its own L2). Renumbering so that every XCD runs one contiguous range of the group-major
+// order makes the workgroups resident on an XCD stream the same reference group through that
+// XCD's L2 at the same time. A bijection on [0, n).
+__device__ inline uint32_t xcd_order(uint32_t b, uint32_t n) {
+    constexpr uint32_t kXcds = 8;
+    const uint32_t x = b % kXcds, per = n / kXcds, rem = n % kXcds;
+    return x * per + (x < rem ? x : rem) + b / kXcds;
+}
+
 __global__ __launch_bounds__(256, 2) void gram_kernel(const __bf16* __restrict__ Q,
                                                    const float* __restrict__ q_norm, uint32_t nq,
                                                    const __bf16* __restrict__ Rm,
@@ -747,7 +758,8 @@ __global__ __launch_bounds__(256, 2) void gram_kernel(const __bf16* __restrict__
     __shared__ uint32_t queue_q[4][kWaveQueue], queue_r[4][kWaveQueue];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t li = lane & 31u, lk = lane >> 5;
-    const uint32_t qb = blockIdx.x % q_blocks, grp = blockIdx.x / q_blocks;
+    const uint32_t id = xcd_order(blockIdx.x, gridDim.x);
+    const uint32_t qb = id % q_blocks, grp = id / q_blocks;
     const uint32_t q0 = qb * kGramQ + wave * 64;
     const uint32_t r_begin = grp * r_per_group;
     const uint32_t r_end = r_begin + r_per_group < nr ? r_begin + r_per_group : nr;

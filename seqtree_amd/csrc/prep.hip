@@ -76,6 +76,62 @@ __global__ void classify_kernel(const uint64_t* __restrict__ off, uint64_t n, Cl
     lists[uint64_t(c) * n + pos] = uint32_t(i);
 }
 
+// One workgroup: a histogram of the lengths in LDS, prefix sums of the counts and of the
+// twins per length, a scatter of the query indices by length into `sorted`, then the twins.
+// The order inside one length follows the LDS atomics, which changes which queries share a
+// workgroup but no score. Lengths past kTwinMaxLen (which the host's bound excludes) are
+// clamped so that no access leaves the tables.
+constexpr uint32_t kTwinThreads = 1024;
+constexpr uint32_t kTwinBins = kTwinMaxLen + 1;
+
+__global__ __launch_bounds__(kTwinThreads) void twin_queries_kernel(
+    const uint64_t* __restrict__ off, uint64_t n, uint32_t* __restrict__ sorted,
+    uint2* __restrict__ twins, uint64_t cap) {
+    __shared__ uint32_t count[kTwinBins], cursor[kTwinBins], start[kTwinBins];
+    __shared__ uint64_t tstart[kTwinBins + 1];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t b = tid; b < kTwinBins; b += kTwinThreads) {
+        count[b] = 0;
+        cursor[b] = 0;
+    }
+    __syncthreads();
+    for (uint64_t i = tid; i < n; i += kTwinThreads) {
+        const uint64_t len = off[i + 1] - off[i];
+        atomicAdd(&count[len < kTwinBins ? len : kTwinBins - 1], 1u);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t s = 0;
+        uint64_t t = 0;
+        for (uint32_t b = 0; b < kTwinBins; ++b) {
+            start[b] = s;
+            tstart[b] = t;
+            s += count[b];
+            t += (count[b] + 1) / 2;
+        }
+        tstart[kTwinBins] = t;
+    }
+    __syncthreads();
+    for (uint64_t i = tid; i < n; i += kTwinThreads) {
+        const uint64_t len64 = off[i + 1] - off[i];
+        const uint32_t len = uint32_t(len64 < kTwinBins ? len64 : kTwinBins - 1);
+        sorted[start[len] + atomicAdd(&cursor[len], 1u)] = uint32_t(i);
+    }
+    __syncthreads();  // `sorted` is complete and visible to the whole workgroup
+    const uint64_t total = tstart[kTwinBins];
+    for (uint64_t p = tid; p < cap; p += kTwinThreads) {
+        uint2 v = make_uint2(kNoSlot, kNoSlot);
+        if (p < total) {
+            uint32_t b = 0;
+            while (tstart[b + 1] <= p) ++b;  // the length whose twins include p
+            const uint32_t j = uint32_t(p - tstart[b]);
+            const uint32_t qa = sorted[start[b] + 2 * j];
+            v = make_uint2(qa, 2 * j + 1 < count[b] ? sorted[start[b] + 2 * j + 1] : qa);
+        }
+        twins[p] = v;
+    }
+}
+
 __global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -97,6 +153,13 @@ void launch_classify(const uint64_t* off, uint64_t n, const ClassBounds& b, uint
     if (n == 0) return;
     const uint32_t blocks = uint32_t((n + 255) / 256);
     hipLaunchKernelGGL(classify_kernel, dim3(blocks), dim3(256), 0, s, off, n, b, counts, lists);
+}
+
+void launch_twin_queries(const uint64_t* off, uint64_t n, uint32_t* sorted, uint2* twins,
+                         hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(twin_queries_kernel, dim3(1), dim3(kTwinThreads), 0, s, off, n, sorted,
+                       twins, twin_capacity(n));
 }
 
 void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {

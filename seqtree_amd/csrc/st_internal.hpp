@@ -57,6 +57,15 @@ void launch_classify(const uint64_t* off, uint64_t n, const ClassBounds& b, uint
 
 void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 
+// Twin list of a query set whose lengths are at most kTwinMaxLen: the queries grouped by
+// length and paired inside each length, a length with an odd count ending in a query paired
+// with itself; entries past the twin count are (kNoSlot, kNoSlot). `sorted` is n entries of
+// scratch; `twins` holds twin_capacity(n) entries.
+constexpr uint32_t kTwinMaxLen = 32;
+inline uint64_t twin_capacity(uint64_t n) { return (n + kTwinMaxLen + 1) / 2; }
+void launch_twin_queries(const uint64_t* off, uint64_t n, uint32_t* sorted, uint2* twins,
+                         hipStream_t s);
+
 // ---- prefilter.hip ---------------------------------------------------------------------
 namespace pf {
 constexpr uint32_t kEmbDim = 32;         // embedding width
@@ -172,6 +181,11 @@ struct GapblockArgs {
 // short_path: the short-launch kernels (a.prof_rows / a.slice_rows / a.big set).
 void launch_gapblock(const GapblockArgs& a, bool use_profile, uint32_t q_max_len, bool exact_wrap,
                      bool short_path, hipStream_t s);
+// Short launch with two queries of one length per workgroup (the twin rows of
+// gapblock_core.hpp; a.big holds the 16-bit sentinel): twins[0 .. n_twins) from
+// launch_twin_queries.
+void launch_gapblock_twin(const GapblockArgs& a, const uint2* twins, uint64_t n_twins,
+                          hipStream_t s);
 // Scores an explicit list of (query, ref) pairs into a.out[t]; references come as CSR.
 void launch_gapblock_pairs(const GapblockArgs& a, const uint8_t* r_codes, const uint64_t* r_off,
                            const uint32_t* pair_query, const uint32_t* pair_ref, uint64_t n_pairs,
