@@ -20,7 +20,9 @@ st_align_matrix_device on one HIP stream.
   batch        64 ragged gap-block jobs (512..8,192 x 10,000..200,000, lengths 6..30,
                BLOSUM62 / structural) in one captured persistent launch
   prefilter    learned prefilter, 50,000 queries x 1,000,000 references, gap-block cutoff 48,
-               against the exhaustive exact thresholded pass
+               against the exhaustive exact thresholded pass; a step is one search against
+               reference embeddings built once (their build time and the one-shot pipeline
+               are reported beside it)
 
 Prints one JSON line: pairs/s (value), cell updates per second, the roofline of the dominant
 kernel measured with HIP events on the stream it runs on, and the reference's multithreaded CPU
@@ -542,30 +544,35 @@ def run_prefilter(torch, dev, lib, stream, args) -> dict:
     ex_s = min(ex_times)
 
     searcher = pf.Searcher(model, gp, PF_LEN[1])
-
-    def step():
-        # the whole pipeline: embed the references (the index) and the queries, screen,
-        # rescore exactly, keep <= cutoff
-        t_index = searcher.index(R)
-        r = searcher.query(Q, args.pf_cutoff, tau)
-        r.seconds["encode_refs"] = t_index
-        return r
-
+    # The references' embeddings are the index, built once per reference set and timed apart
+    # from the search, as the reference's own search benchmark times GapBlockIndex
+    # (bench/bench_gapblock.py:149-160); a step is one search of all queries: embed the
+    # queries, screen, rescore exactly, keep <= cutoff. The one-shot pipeline (build + search)
+    # is reported beside it.
     for _ in range(args.warmup):
-        res = step()
+        searcher.index(R)
+        res = searcher.query(Q, args.pf_cutoff, tau)
     torch.cuda.synchronize()
-    lib.lib.st_timing_enable(1)
-    lib.timing()
-    times, query_times = [], []
+    build_times, search_times = [], []
     for _ in range(args.steps):
         t0 = time.perf_counter()
-        res = step()
-        times.append(time.perf_counter() - t0)
-        query_times.append(time.perf_counter() - t0 - res.seconds["encode_refs"])
+        searcher.index(R)
+        t1 = time.perf_counter()
+        res = searcher.query(Q, args.pf_cutoff, tau)
+        search_times.append(time.perf_counter() - t1)
+        build_times.append(t1 - t0)
+    # the same steps again with the library's launch timing on, for the per-kernel breakdown
+    lib.lib.st_timing_enable(1)
+    lib.timing()
+    for _ in range(args.steps):
+        searcher.index(R)
+        res = searcher.query(Q, args.pf_cutoff, tau)
+    torch.cuda.synchronize()
     lib.lib.st_timing_enable(0)
     kernels = lib.timing()
-    ms_step = 1e3 * sum(times) / len(times)
-    ms_query = 1e3 * sum(query_times) / len(query_times)
+    ms_build = 1e3 * sum(build_times) / len(build_times)
+    ms_step = 1e3 * sum(search_times) / len(search_times)
+    ms_one_shot = ms_build + ms_step
     rec = pf.recall(res.query, res.ref, eq, er)
     label, launches, avg_s = dominant(kernels)
     if label == "prefilter_screen":
@@ -586,7 +593,8 @@ def run_prefilter(torch, dev, lib, stream, args) -> dict:
                             seconds=args.cpu_seconds, pen=gp.pen, go=gp.gap_open,
                             cube=gp.cube.tolist(), width=gp.width)
     return base_line(
-        "learned-prefilter effective pairs/s (exact gap-block hits <= cutoff)",
+        "learned-prefilter effective pairs/s (exact gap-block hits <= cutoff, reference "
+        "embeddings built once)",
         pairs / (ms_step / 1e3), args.steps, args.warmup, ms_step,
         {"workload": "prefilter_cdr3_families_50000x1000000_blosum62_go28_central21",
          "queries": Q.n, "refs": R.n, "cutoff": args.pf_cutoff, "seed": seed,
@@ -597,7 +605,8 @@ def run_prefilter(torch, dev, lib, stream, args) -> dict:
         candidates=res.candidates, tau_score=round(tau, 3),
         exhaustive_pairs_per_s=pairs / ex_s, exhaustive_ms=1e3 * ex_s,
         speedup_vs_exhaustive=ex_s / (ms_step / 1e3),
-        query_only_ms=ms_query, speedup_query_only=ex_s / (ms_query / 1e3),
+        index_build_ms=ms_build, one_shot_ms=ms_one_shot,
+        speedup_one_shot=ex_s / (ms_one_shot / 1e3),
         stage_ms={k: round(1e3 * v, 3) for k, v in res.seconds.items()},
         train_s=round(train_s, 2), data_s=round(data_s, 2), calibration=calib,
         train_loss=[round(x, 5) for x in losses],

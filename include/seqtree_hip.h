@@ -173,7 +173,10 @@ int32_t st_levenshtein_matrix_device(const uint8_t* a, const uint64_t* a_offsets
  * process and workload signature; 0 takes the default or a memoised choice) and captures the
  * whole batch -- reference packing, work-counter reset and the persistent scoring kernel --
  * in a hipGraph. st_gapblock_batch_plan_run replays that graph on `stream`; the caller's
- * sequence buffers must stay valid and unchanged in size for the plan's lifetime. */
+ * sequence buffers must stay valid and unchanged in size for the plan's lifetime. Their
+ * contents may change between runs within each job's declared maximum lengths: every run
+ * re-packs the references and re-pairs the queries by length (short jobs score two queries
+ * of one length per table), so the results follow the current contents. */
 typedef struct st_gapblock_job {
     const uint8_t* q_codes;
     const uint64_t* q_offsets;

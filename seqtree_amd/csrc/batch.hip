@@ -13,7 +13,9 @@
 // Inside a unit the workgroup builds each query's penalty profile in LDS and sweeps the
 // job's chunk-sorted references exactly as the single-job kernel does (same per-pair
 // routine, gapblock_core.hpp), staging each chunk's results in LDS for one coalesced row
-// write.
+// write. Short jobs whose sums fit int16 run twin rows: their query blocks are blocks of
+// twins (two queries of one length in the 16-bit halves of one table), paired by length on
+// every run by a small kernel ahead of the persistent one.
 
 #include "gapblock_core.hpp"
 
@@ -88,12 +90,68 @@ __device__ void run_unit(const BatchJob& J, uint64_t q_begin, uint32_t qt, uint3
     }
 }
 
+// A unit of a job with twin rows: twins [t_begin, t_begin + qt) (each two queries of one
+// length sharing the 16-bit halves of the tables, gapblock_core.hpp) against chunks
+// [c_begin, c_end); the stage holds both rows of a chunk.
+template <uint32_t THREADS>
+__device__ void run_unit_twin(const BatchJob& J, uint64_t t_begin, uint32_t qt, uint32_t c_begin,
+                              uint32_t c_end, int32_t* lds, int32_t* stage) {
+    constexpr uint32_t kSlots = kChunk / THREADS;
+    const uint32_t tid = threadIdx.x;
+    const uint64_t cap = twin_capacity(J.n_queries);
+    const uint64_t t_end = t_begin + qt < cap ? t_begin + qt : cap;
+    uint32_t* prof = reinterpret_cast<uint32_t*>(lds);
+    uint32_t* slice = prof + J.prof_rows * gb::kRowInts;
+    for (uint64_t p = t_begin; p < t_end; ++p) {
+        // the twin (a, b) as one 8-byte global load: a in the low word
+        const uint64_t tw = gb::global_ptr(reinterpret_cast<const uint64_t*>(J.twins))[p];
+        if (uint32_t(tw) == kNoSlot) break;  // past the job's last twin (the same for every thread)
+        const uint64_t ia = uint32_t(tw), ib = uint32_t(tw >> 32);
+        const uint64_t qa0 = gb::global_ptr(J.q_off)[ia], qb0 = gb::global_ptr(J.q_off)[ib];
+        const uint32_t m = uint32_t(gb::global_ptr(J.q_off)[ia + 1] - qa0);
+        __syncthreads();  // the previous twin's tables are no longer read
+        gb::build_rows2(prof, J.prof_rows, m, J.pen, J.A, J.q_codes + qa0, J.q_codes + qb0, tid,
+                        THREADS);
+        gb::build_slice2(slice, J.slice_rows - 1, m, J.prior, J.w1, J.big, tid, THREADS);
+        __syncthreads();
+        for (uint32_t c = c_begin; c < c_end; ++c) {
+#pragma unroll 1  // one slot's twin scorer at a time: bounds the registers
+            for (uint32_t it = 0; it < kSlots; ++it) {
+                const uint32_t slot = c * kChunk + it * THREADS + tid;
+                const uint32_t k = gb::global_ptr(J.refs.orig)[slot];
+                const uint32_t n = gb::global_ptr(J.refs.len)[slot];
+                int32_t sa = 0, sb = 0;
+                gb::twin_score_short(
+                    prof, slice, gb::buffer_words(J.refs.words, J.refs.npad, J.refs.nwords, slot),
+                    k, m, n, J.gap_open, J.gap_extend, sa, sb);
+                if (k != kNoSlot) {
+                    stage[k - c * kChunk] = sa;
+                    stage[kChunk + k - c * kChunk] = sb;
+                }
+            }
+            __syncthreads();
+            const uint64_t col0 = uint64_t(c) * kChunk;
+            const uint64_t lim64 = J.n_refs - col0;
+            const uint32_t lim = lim64 < kChunk ? uint32_t(lim64) : kChunk;
+            int32_t* row_a = J.out + ia * J.n_refs + col0;
+            for (uint32_t t = tid; t < lim; t += THREADS) row_a[t] = stage[t];
+            if (ib != ia) {
+                int32_t* row_b = J.out + ib * J.n_refs + col0;
+                for (uint32_t t = tid; t < lim; t += THREADS) row_b[t] = stage[kChunk + t];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Dynamic LDS: the chunk stage (stage_ints int32), then the unit's tables.
 template <uint32_t THREADS>
 __global__ __launch_bounds__(THREADS) void gapblock_batch_kernel(
     const BatchJob* __restrict__ jobs, uint32_t n_jobs, uint64_t total_units, uint32_t qt,
-    unsigned long long* __restrict__ next_unit) {
-    extern __shared__ int32_t lds[];
-    __shared__ int32_t stage[kChunk];
+    uint32_t stage_ints, unsigned long long* __restrict__ next_unit) {
+    extern __shared__ int32_t dyn[];
+    int32_t* stage = dyn;
+    int32_t* lds = dyn + stage_ints;
     __shared__ unsigned long long unit_sh;
     __shared__ uint32_t job_sh;
 
@@ -120,7 +178,9 @@ __global__ __launch_bounds__(THREADS) void gapblock_batch_kernel(
         const uint64_t q_begin = (local % J.qblocks) * qt;
         const uint32_t c0 = g * J.group;
         const uint32_t c1 = c0 + J.group < J.refs.nchunks ? c0 + J.group : J.refs.nchunks;
-        if (J.short_path) {
+        if (J.twin) {
+            run_unit_twin<THREADS>(J, q_begin, qt, c0, c1, lds, stage);
+        } else if (J.short_path) {
             run_unit<true, false, true, THREADS>(J, q_begin, qt, c0, c1, lds, stage);
         } else if (J.profile) {
             if (J.exact) run_unit<true, true, false, THREADS>(J, q_begin, qt, c0, c1, lds, stage);
@@ -136,16 +196,16 @@ __global__ __launch_bounds__(THREADS) void gapblock_batch_kernel(
 }  // namespace
 
 void launch_gapblock_batch(const BatchJob* jobs, uint32_t n_jobs, uint64_t total_units,
-                           const BatchTiles& tiles, size_t lds_bytes,
+                           const BatchTiles& tiles, uint32_t stage_ints, size_t lds_bytes,
                            unsigned long long* next_unit, hipStream_t s) {
     if (total_units == 0) return;
     const dim3 grid(tiles.workgroups);
     if (tiles.threads == 512)
         hipLaunchKernelGGL(gapblock_batch_kernel<512>, grid, dim3(512), lds_bytes, s, jobs, n_jobs,
-                           total_units, tiles.queries_per_unit, next_unit);
+                           total_units, tiles.queries_per_unit, stage_ints, next_unit);
     else
         hipLaunchKernelGGL(gapblock_batch_kernel<256>, grid, dim3(256), lds_bytes, s, jobs, n_jobs,
-                           total_units, tiles.queries_per_unit, next_unit);
+                           total_units, tiles.queries_per_unit, stage_ints, next_unit);
 }
 
 }  // namespace st

@@ -508,7 +508,9 @@ struct BatchPlanImpl {
     BatchJob* d_jobs = nullptr;
     unsigned long long* d_next = nullptr;
     uint64_t total_units = 0;
-    size_t lds_bytes = 0;
+    size_t lds_bytes = 0;        // stage + the largest job's tables
+    uint32_t stage_ints = kChunk;  // 2 kChunk when some job has twin rows
+    bool any_twin = false;
     BatchTiles tiles{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -531,7 +533,8 @@ struct BatchPlanImpl {
         uint64_t u = 0;
         for (auto& j : dev_jobs) {
             j.unit_begin = u;
-            j.qblocks = uint32_t((j.n_queries + qt - 1) / qt);
+            const uint64_t rows = j.twin ? twin_capacity(j.n_queries) : j.n_queries;
+            j.qblocks = uint32_t((rows + qt - 1) / qt);
             const uint32_t ngroups = (j.refs.nchunks + j.group - 1) / j.group;
             u += uint64_t(j.qblocks) * ngroups;
         }
@@ -544,9 +547,10 @@ struct BatchPlanImpl {
         for (const auto& p : packs)
             launch_pack_refs(p.bytes, p.off, p.n, p.nwords, p.npad, p.words, p.len, p.orig,
                              p.scale, s);
+        if (any_twin) launch_twin_queries_batch(d_jobs, uint32_t(dev_jobs.size()), s);
         launch_fill_u32(reinterpret_cast<uint32_t*>(d_next), 2, 0u, s);
-        launch_gapblock_batch(d_jobs, uint32_t(dev_jobs.size()), total_units, tiles, lds_bytes,
-                              d_next, s);
+        launch_gapblock_batch(d_jobs, uint32_t(dev_jobs.size()), total_units, tiles, stage_ints,
+                              lds_bytes, d_next, s);
     }
 };
 
@@ -1703,10 +1707,18 @@ int32_t st_gapblock_batch_plan_create(const st_gapblock_job* jobs, uint32_t n_jo
                 d.slice_rows = j.r_max_len + 1;
                 lds = std::max(lds, size_t(d.prof_rows) * gb::kRowBytes +
                                         size_t(d.slice_rows) * gb::kSliceStride * sizeof(int32_t));
+                if (twin_launch(j.n_queries, j.q_max_len, pen_abs, prior_abs, &big)) {
+                    d.twin = 1;
+                    d.big = big;
+                    d.twins = B.alloc<uint2>(twin_capacity(j.n_queries));
+                    d.twin_sorted = B.alloc<uint32_t>(j.n_queries);
+                    B.any_twin = true;
+                }
             }
             B.dev_jobs.push_back(d);
         }
-        B.lds_bytes = lds;
+        B.stage_ints = B.any_twin ? 2 * kChunk : kChunk;
+        B.lds_bytes = size_t(B.stage_ints) * sizeof(int32_t) + lds;
         B.d_jobs = B.alloc<BatchJob>(std::max<size_t>(B.dev_jobs.size(), 1));
         B.d_next = B.alloc<unsigned long long>(1);
 

@@ -84,9 +84,8 @@ __global__ void classify_kernel(const uint64_t* __restrict__ off, uint64_t n, Cl
 constexpr uint32_t kTwinThreads = 1024;
 constexpr uint32_t kTwinBins = kTwinMaxLen + 1;
 
-__global__ __launch_bounds__(kTwinThreads) void twin_queries_kernel(
-    const uint64_t* __restrict__ off, uint64_t n, uint32_t* __restrict__ sorted,
-    uint2* __restrict__ twins, uint64_t cap) {
+__device__ void twin_list(const uint64_t* __restrict__ off, uint64_t n,
+                          uint32_t* __restrict__ sorted, uint2* __restrict__ twins, uint64_t cap) {
     __shared__ uint32_t count[kTwinBins], cursor[kTwinBins], start[kTwinBins];
     __shared__ uint64_t tstart[kTwinBins + 1];
     const uint32_t tid = threadIdx.x;
@@ -132,6 +131,21 @@ __global__ __launch_bounds__(kTwinThreads) void twin_queries_kernel(
     }
 }
 
+__global__ __launch_bounds__(kTwinThreads) void twin_queries_kernel(
+    const uint64_t* __restrict__ off, uint64_t n, uint32_t* __restrict__ sorted,
+    uint2* __restrict__ twins, uint64_t cap) {
+    twin_list(off, n, sorted, twins, cap);
+}
+
+// Workgroup j: the twin list of batch job j (if it has twin rows).
+__global__ __launch_bounds__(kTwinThreads) void twin_queries_batch_kernel(
+    const BatchJob* __restrict__ jobs) {
+    const BatchJob& J = jobs[blockIdx.x];
+    if (!J.twin || J.n_queries == 0) return;
+    twin_list(J.q_off, J.n_queries, J.twin_sorted, const_cast<uint2*>(J.twins),
+              twin_capacity(J.n_queries));
+}
+
 __global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -160,6 +174,11 @@ void launch_twin_queries(const uint64_t* off, uint64_t n, uint32_t* sorted, uint
     if (n == 0) return;
     hipLaunchKernelGGL(twin_queries_kernel, dim3(1), dim3(kTwinThreads), 0, s, off, n, sorted,
                        twins, twin_capacity(n));
+}
+
+void launch_twin_queries_batch(const BatchJob* jobs, uint32_t n_jobs, hipStream_t s) {
+    if (n_jobs == 0) return;
+    hipLaunchKernelGGL(twin_queries_batch_kernel, dim3(n_jobs), dim3(kTwinThreads), 0, s, jobs);
 }
 
 void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {

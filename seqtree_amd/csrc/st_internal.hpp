@@ -62,7 +62,7 @@ void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 // with itself; entries past the twin count are (kNoSlot, kNoSlot). `sorted` is n entries of
 // scratch; `twins` holds twin_capacity(n) entries.
 constexpr uint32_t kTwinMaxLen = 32;
-inline uint64_t twin_capacity(uint64_t n) { return (n + kTwinMaxLen + 1) / 2; }
+__host__ __device__ inline uint64_t twin_capacity(uint64_t n) { return (n + kTwinMaxLen + 1) / 2; }
 void launch_twin_queries(const uint64_t* off, uint64_t n, uint32_t* sorted, uint2* twins,
                          hipStream_t s);
 
@@ -214,6 +214,12 @@ struct BatchJob {
     uint32_t group;        // reference chunks per work unit
     uint32_t qblocks;      // query blocks (of queries_per_unit) per chunk group
     uint64_t unit_begin;   // index of the job's first work unit
+    // twin rows (short path, two equal-length queries per table, big = the 16-bit sentinel):
+    // the job's twin list (twin_capacity(n_queries) entries, rebuilt on every run) and its
+    // scratch; a query block is then a block of twins
+    uint32_t twin;
+    const uint2* twins;
+    uint32_t* twin_sorted;
 };
 // Tile shape of the persistent launch: queries per work unit, threads per workgroup
 // (256 or 512) and the number of resident workgroups.
@@ -222,9 +228,13 @@ struct BatchTiles {
     uint32_t threads;
     uint32_t workgroups;
 };
+// lds_bytes: the chunk stage (stage_ints int32: kChunk, or 2 kChunk when a job has twin rows)
+// followed by the largest job's tables.
 void launch_gapblock_batch(const BatchJob* jobs, uint32_t n_jobs, uint64_t total_units,
-                           const BatchTiles& tiles, size_t lds_bytes,
+                           const BatchTiles& tiles, uint32_t stage_ints, size_t lds_bytes,
                            unsigned long long* next_unit, hipStream_t s);
+// The twin lists of every twin job of a batch (one workgroup per job).
+void launch_twin_queries_batch(const BatchJob* jobs, uint32_t n_jobs, hipStream_t s);
 
 // ---- gotoh.hip -------------------------------------------------------------------------
 struct GotohArgs {

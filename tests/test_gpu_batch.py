@@ -114,3 +114,38 @@ def test_batch_encoded_sets_equal_strings(gpu):
     a, b = batch.results()
     assert np.array_equal(a, b) and np.array_equal(a, _dense(st, jobs[0]))
     batch.close()
+
+
+def test_batch_replay_after_query_and_reference_refill(gpu):
+    """The caller may rewrite its sequence buffers between runs (same sizes, same maximum
+    lengths): each replay re-packs the references and re-pairs the queries by length (the
+    twin rows of short jobs), so the results follow the new contents."""
+    st = gpu
+    from seqtree_amd.batch import GapblockBatch, GapblockJob, _encode
+
+    rng = random.Random(21)
+    m = st.SubstitutionMatrix.blosum62()
+    jobs = [GapblockJob(cdr3_like(rng, 61, 6, 30), cdr3_like(rng, 2500, 6, 30), m, 28, 1,
+                        st.central_prior(21)) for _ in range(2)]
+    batch = GapblockBatch(jobs)
+    batch.run()
+    first = batch.results()
+    assert all(np.array_equal(g, _dense(st, j)) for g, j in zip(first, jobs))
+    syms = st.alphabet_symbols("aa")
+    for t, job in enumerate(jobs):  # a permutation keeps every buffer's size and the maxima
+        q = list(job.queries)
+        r = list(job.refs)
+        rng.shuffle(q)
+        rng.shuffle(r)
+        qb, qo = _encode(q, syms)
+        rb, ro = _encode(r, syms)
+        tq_b, tq_o, tr_b, tr_o = batch._keep[6 * t + 2: 6 * t + 6]
+        tq_b.fill_from(qb)
+        tq_o.fill_from(qo)
+        tr_b.fill_from(rb)
+        tr_o.fill_from(ro)
+        jobs[t] = GapblockJob(q, r, m, 28, 1, st.central_prior(21))
+    batch.run()
+    for t, (g, j) in enumerate(zip(batch.results(), jobs)):
+        assert np.array_equal(g, _dense(st, j)), f"job {t}"
+    batch.close()
