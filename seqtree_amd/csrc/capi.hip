@@ -210,18 +210,26 @@ void launched(const char* what) { check(hipGetLastError(), what); }
 
 uint32_t words_for(uint32_t max_len) { return (max_len + 3) / 4; }
 
-// Packs the reference side (see st_internal.hpp for the layout); scale as launch_pack_refs.
-PackedRefs pack_refs(Arena& ar, const uint8_t* bytes, const uint64_t* off, uint64_t n,
-                     uint32_t max_len, hipStream_t s, uint32_t scale = 1) {
-    LaunchTimer t("pack_refs", s);
+// The packed layout's geometry for n references of at most max_len residues.
+PackedRefs packed_geometry(uint64_t n, uint32_t max_len) {
     PackedRefs p{};
     p.nchunks = uint32_t((n + kChunk - 1) / kChunk);
     p.npad = p.nchunks * kChunk;
     p.nwords = words_for(max_len);
+    return p;
+}
+
+// Packs the reference side (see st_internal.hpp for the layout); scale as launch_pack_refs.
+// A twin job, when given, has its twin list built by the same launch.
+PackedRefs pack_refs(Arena& ar, const uint8_t* bytes, const uint64_t* off, uint64_t n,
+                     uint32_t max_len, hipStream_t s, uint32_t scale = 1,
+                     const TwinJob& tw = TwinJob{}) {
+    LaunchTimer t("pack_refs", s);
+    PackedRefs p = packed_geometry(n, max_len);
     uint32_t* words = ar.get<uint32_t>(size_t(p.nwords + 1) * p.npad);
     uint32_t* len = ar.get<uint32_t>(p.npad);
     uint32_t* orig = ar.get<uint32_t>(p.npad);
-    launch_pack_refs(bytes, off, n, p.nwords, p.npad, words, len, orig, scale, s);
+    launch_pack_refs(bytes, off, n, p.nwords, p.npad, words, len, orig, scale, max_len, tw, s);
     launched("pack_refs");
     p.words = words;
     p.len = len;
@@ -315,15 +323,17 @@ struct GapblockPlan {
     bool exact;
     bool profile;
     bool short_path;
-    bool twin;  // short launch with two queries of one length per workgroup
+    bool twin;            // short launch with two queries of one length per workgroup
+    uint2* twins = nullptr;  // its twin list (built by the packing launch)
 };
 
 // Twin rows (two queries in the 16-bit halves, gapblock_core.hpp) need every sum of a short
 // launch inside int16: 6 T + R < 2^15 with T = q_max * max|pen| and R = max|prior|. Sets the
-// 16-bit sentinel.
+// 16-bit sentinel. (The pairing kernel indexes queries in 32 bits: N < 2^31.)
 bool twin_launch(uint64_t N, uint32_t q_max, int64_t pen_abs, int64_t prior_abs, int32_t* big) {
     const int64_t T = int64_t(q_max) * pen_abs;
-    if (N < 2 || q_max > kTwinMaxLen || 6 * T + prior_abs >= 32767) return false;
+    if (N < 2 || N >= (uint64_t(1) << 31) || q_max > kTwinMaxLen || 6 * T + prior_abs >= 32767)
+        return false;
     *big = int32_t(32767 - 2 * T);
     return true;
 }
@@ -389,7 +399,6 @@ GapblockPlan gapblock_plan(Arena& ar, const uint8_t* q, const uint64_t* q_off, u
     a.q_off = q_off;
     a.n_queries = N;
     a.n_refs = K;
-    if (pack) a.refs = pack_refs(ar, r, r_off, K, r_max, s, 4);
     a.pen = pen;
     a.A = A;
     a.gap_open = go;
@@ -402,30 +411,36 @@ GapblockPlan gapblock_plan(Arena& ar, const uint8_t* q, const uint64_t* q_off, u
     const int64_t worst = 2 * Lmax * pen_abs + prior_abs + int64_t(go) + Lmax * int64_t(ge);
     p.exact = worst >= int64_t(INT32_MAX);
     p.profile = size_t(q_max) * gb::kRowBytes <= 48 * 1024;  // one table row per position
-    p.short_path =
-        pack && short_launch(q_max, r_max, A, p.exact, pen_abs, prior_abs, a.refs, &a.big);
-    a.group = pack ? chunk_group(N, a.refs.nchunks, a.refs.nwords) : 1;
+    if (!pack) {
+        a.group = 1;
+        return p;
+    }
+    const PackedRefs geo = packed_geometry(K, r_max);
+    p.short_path = short_launch(q_max, r_max, A, p.exact, pen_abs, prior_abs, geo, &a.big);
+    TwinJob tw{};
     if (p.short_path) {
         a.prof_rows = q_max + gb::kFastMaxL;
         a.slice_rows = r_max + 1;
         p.twin = twin_launch(N, q_max, pen_abs, prior_abs, &a.big);
+        if (p.twin) {
+            // one scratch block: the twins (8-byte entries), then the length-sorted indices;
+            // the packing launch builds the list
+            const uint64_t cap = twin_capacity(N);
+            p.twins = ar.get<uint2>(cap + (N + 1) / 2);
+            tw = {q_off, N, reinterpret_cast<uint32_t*>(p.twins + cap), p.twins};
+        }
     }
+    a.refs = pack_refs(ar, r, r_off, K, r_max, s, 4, tw);
+    // workgroups per chunk group: one per query (one per twin with twin rows)
+    a.group = chunk_group(p.twin ? twin_capacity(N) : N, a.refs.nchunks, a.refs.nwords);
     return p;
 }
 
-void gapblock_run(Arena& ar, const GapblockPlan& p, uint32_t q_max, hipStream_t s) {
+void gapblock_run(const GapblockPlan& p, uint32_t q_max, hipStream_t s) {
     if (p.twin) {
-        // one scratch block: the twins (8-byte entries) first, then the length-sorted indices
-        const uint64_t cap = twin_capacity(p.a.n_queries);
-        uint2* twins = ar.get<uint2>(cap + (p.a.n_queries + 1) / 2);
-        uint32_t* sorted = reinterpret_cast<uint32_t*>(twins + cap);
-        {
-            LaunchTimer t("gapblock_twin_list", s);
-            launch_twin_queries(p.a.q_off, p.a.n_queries, sorted, twins, s);
-        }
         LaunchTimer t(std::string(p.a.hits.count ? "gapblock_threshold" : "gapblock") + "_twin",
                       s);
-        launch_gapblock_twin(p.a, twins, cap, s);
+        launch_gapblock_twin(p.a, p.twins, twin_capacity(p.a.n_queries), s);
         return;
     }
     LaunchTimer t(std::string(p.a.hits.count ? "gapblock_threshold" : "gapblock") +
@@ -444,7 +459,7 @@ void gapblock_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, uint32
     GapblockPlan p = gapblock_plan(ar, q, q_off, N, q_max, r, r_off, K, r_max, pen_h, A, go, ge,
                                    prior_h, W, true, s);
     p.a.out = out;
-    gapblock_run(ar, p, q_max, s);
+    gapblock_run(p, q_max, s);
     launched("gapblock");
 }
 
@@ -459,7 +474,7 @@ void gapblock_threshold_device(const uint8_t* q, const uint64_t* q_off, uint64_t
     GapblockPlan p = gapblock_plan(ar, q, q_off, N, q_max, r, r_off, K, r_max, pen_h, A, go, ge,
                                    prior_h, W, true, s);
     p.a.hits = hits;
-    gapblock_run(ar, p, q_max, s);
+    gapblock_run(p, q_max, s);
     launched("gapblock_threshold");
 }
 
@@ -502,7 +517,8 @@ struct BatchPlanImpl {
         uint64_t n;
         uint32_t nwords, npad;
         uint32_t *words, *len, *orig;
-        uint32_t scale;
+        uint32_t scale, max_len;
+        TwinJob tw;  // the job's twin list, built by the same launch (twin jobs)
     };
     std::vector<PackArgs> packs;
     BatchJob* d_jobs = nullptr;
@@ -546,8 +562,7 @@ struct BatchPlanImpl {
     void enqueue(hipStream_t s) {
         for (const auto& p : packs)
             launch_pack_refs(p.bytes, p.off, p.n, p.nwords, p.npad, p.words, p.len, p.orig,
-                             p.scale, s);
-        if (any_twin) launch_twin_queries_batch(d_jobs, uint32_t(dev_jobs.size()), s);
+                             p.scale, p.max_len, p.tw, s);
         launch_fill_u32(reinterpret_cast<uint32_t*>(d_next), 2, 0u, s);
         launch_gapblock_batch(d_jobs, uint32_t(dev_jobs.size()), total_units, tiles, stage_ints,
                               lds_bytes, d_next, s);
@@ -1672,7 +1687,7 @@ int32_t st_gapblock_batch_plan_create(const st_gapblock_job* jobs, uint32_t n_jo
             pr.len = len;
             pr.orig = orig;
             B.packs.push_back({j.r_codes, j.r_offsets, j.n_refs, pr.nwords, pr.npad, words, len,
-                               orig, 4u});
+                               orig, 4u, j.r_max_len, TwinJob{}});
             BatchJob d{};
             d.q_codes = j.q_codes;
             d.q_off = j.q_offsets;
@@ -1710,8 +1725,10 @@ int32_t st_gapblock_batch_plan_create(const st_gapblock_job* jobs, uint32_t n_jo
                 if (twin_launch(j.n_queries, j.q_max_len, pen_abs, prior_abs, &big)) {
                     d.twin = 1;
                     d.big = big;
-                    d.twins = B.alloc<uint2>(twin_capacity(j.n_queries));
+                    uint2* twins = B.alloc<uint2>(twin_capacity(j.n_queries));
+                    d.twins = twins;
                     d.twin_sorted = B.alloc<uint32_t>(j.n_queries);
+                    B.packs.back().tw = {j.q_offsets, j.n_queries, d.twin_sorted, twins};
                     B.any_twin = true;
                 }
             }

@@ -41,9 +41,17 @@ struct PackedRefs {
 // scale multiplies every byte (1: raw bytes / codes, zero past the end; 4: codes as LDS byte
 // offsets, the gap-block kernels' form -- codes are < 32, so code * 4 still fits a byte --
 // with code 31 (byte 124) past the end).
+// max_len bounds the reference lengths (below 256: a counting sort by length). When
+// tw.twins is set the same launch also builds that query set's twin list (below).
+struct TwinJob {
+    const uint64_t* off;
+    uint64_t n;
+    uint32_t* sorted;
+    uint2* twins;
+};
 void launch_pack_refs(const uint8_t* bytes, const uint64_t* off, uint64_t n, uint32_t nwords,
                       uint32_t npad, uint32_t* words, uint32_t* len, uint32_t* orig,
-                      uint32_t scale, hipStream_t s);
+                      uint32_t scale, uint32_t max_len, const TwinJob& tw, hipStream_t s);
 
 // Buckets the n queries by length: class c receives every query with
 // length <= bounds[c] (and > bounds[c-1]); the last bound must be 0xFFFFFFFF.
@@ -57,14 +65,13 @@ void launch_classify(const uint64_t* off, uint64_t n, const ClassBounds& b, uint
 
 void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 
-// Twin list of a query set whose lengths are at most kTwinMaxLen: the queries grouped by
-// length and paired inside each length, a length with an odd count ending in a query paired
-// with itself; entries past the twin count are (kNoSlot, kNoSlot). `sorted` is n entries of
-// scratch; `twins` holds twin_capacity(n) entries.
+// Twin list of a query set whose lengths are at most kTwinMaxLen (built by a packing launch
+// given a TwinJob): the queries grouped by length and paired inside each length, a length
+// with an odd count ending in a query paired with itself; entries past the twin count are
+// (kNoSlot, kNoSlot). `sorted` is n entries of scratch; `twins` holds twin_capacity(n)
+// entries.
 constexpr uint32_t kTwinMaxLen = 32;
 __host__ __device__ inline uint64_t twin_capacity(uint64_t n) { return (n + kTwinMaxLen + 1) / 2; }
-void launch_twin_queries(const uint64_t* off, uint64_t n, uint32_t* sorted, uint2* twins,
-                         hipStream_t s);
 
 // ---- prefilter.hip ---------------------------------------------------------------------
 namespace pf {
@@ -182,8 +189,8 @@ struct GapblockArgs {
 void launch_gapblock(const GapblockArgs& a, bool use_profile, uint32_t q_max_len, bool exact_wrap,
                      bool short_path, hipStream_t s);
 // Short launch with two queries of one length per workgroup (the twin rows of
-// gapblock_core.hpp; a.big holds the 16-bit sentinel): twins[0 .. n_twins) from
-// launch_twin_queries.
+// gapblock_core.hpp; a.big holds the 16-bit sentinel): twins[0 .. n_twins), a twin list
+// (see TwinJob).
 void launch_gapblock_twin(const GapblockArgs& a, const uint2* twins, uint64_t n_twins,
                           hipStream_t s);
 // Scores an explicit list of (query, ref) pairs into a.out[t]; references come as CSR.
@@ -215,8 +222,8 @@ struct BatchJob {
     uint32_t qblocks;      // query blocks (of queries_per_unit) per chunk group
     uint64_t unit_begin;   // index of the job's first work unit
     // twin rows (short path, two equal-length queries per table, big = the 16-bit sentinel):
-    // the job's twin list (twin_capacity(n_queries) entries, rebuilt on every run) and its
-    // scratch; a query block is then a block of twins
+    // the job's twin list (twin_capacity(n_queries) entries, rebuilt on every run by the
+    // job's packing launch) and its scratch; a query block is then a block of twins
     uint32_t twin;
     const uint2* twins;
     uint32_t* twin_sorted;
@@ -233,8 +240,7 @@ struct BatchTiles {
 void launch_gapblock_batch(const BatchJob* jobs, uint32_t n_jobs, uint64_t total_units,
                            const BatchTiles& tiles, uint32_t stage_ints, size_t lds_bytes,
                            unsigned long long* next_unit, hipStream_t s);
-// The twin lists of every twin job of a batch (one workgroup per job).
-void launch_twin_queries_batch(const BatchJob* jobs, uint32_t n_jobs, hipStream_t s);
+
 
 // ---- gotoh.hip -------------------------------------------------------------------------
 struct GotohArgs {
