@@ -27,6 +27,7 @@ REFERENCE_TREE = Path("/root/reference")
 _c_u8p = ctypes.POINTER(ctypes.c_uint8)
 _c_u64p = ctypes.POINTER(ctypes.c_uint64)
 _c_i32p = ctypes.POINTER(ctypes.c_int32)
+_c_u32p = ctypes.POINTER(ctypes.c_uint32)
 _c_i64p = ctypes.POINTER(ctypes.c_int64)
 
 ALPHABETS = {"aa": 0, "nt": 1, "iupac": 2}
@@ -77,6 +78,11 @@ class Oracle:
         lib.or_gotoh.restype = ctypes.c_int64
         lib.or_gotoh.argtypes = [_c_u8p, ctypes.c_uint64, _c_u8p, ctypes.c_uint64, _c_i32p,
                                  ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+        lib.or_align_pair.restype = None
+        lib.or_align_pair.argtypes = [
+            _c_u8p, ctypes.c_uint64, _c_u8p, ctypes.c_uint64, _c_i32p, ctypes.c_int32,
+            ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64),
+            _c_u8p, _c_u64p, _c_u64p, _c_u64p]
         lib.or_hamming.restype = ctypes.c_int32
         lib.or_hamming.argtypes = [_c_u8p, ctypes.c_uint64, _c_u8p, ctypes.c_uint64]
         lib.or_levenshtein.restype = ctypes.c_int32
@@ -129,6 +135,23 @@ class Oracle:
         rc, ro = pack([r], enc)
         return int(self.lib.or_gotoh(_p(qc, _c_u8p), int(qo[1]), _p(rc, _c_u8p), int(ro[1]),
                                      _p(sim, _c_i32p), sim.shape[0], int(local), go, ge))
+
+    def align_pair(self, q, r, sim, local=False, go=11, ge=1, alphabet="aa"):
+        """(score as int64, ops, first aligned query residue, first aligned reference
+        residue) of or_align_pair; q and r are strings (or bytes) in `alphabet`."""
+        sim = np.ascontiguousarray(sim, dtype=np.int32)
+        enc = self.encoder(alphabet)
+        qc, qo = pack([q], enc)
+        rc, ro = pack([r], enc)
+        m, n = int(qo[1]), int(ro[1])
+        ops = np.zeros(m + n + 1, np.uint8)
+        score = ctypes.c_int64()
+        meta = np.zeros(3, np.uint64)
+        self.lib.or_align_pair(_p(qc, _c_u8p), m, _p(rc, _c_u8p), n, _p(sim, _c_i32p),
+                               sim.shape[0], int(local), go, ge, ctypes.byref(score),
+                               _p(ops, _c_u8p), _p(meta[0:], _c_u64p), _p(meta[1:], _c_u64p),
+                               _p(meta[2:], _c_u64p))
+        return score.value, bytes(ops[: int(meta[0])]).decode(), int(meta[1]), int(meta[2])
 
     def hamming(self, a, b):
         x, xo = pack([a])
@@ -214,6 +237,15 @@ class Reference:
             ctypes.c_char_p, ctypes.c_uint64, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int,
             ctypes.c_int, _c_i32p, ctypes.c_int32, ctypes.c_int, ctypes.c_int32, ctypes.c_int32,
             _c_i32p, ctypes.c_char_p, ctypes.c_int]
+        lib.ref_align_pair.argtypes = [
+            ctypes.c_char_p, ctypes.c_uint64, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int,
+            ctypes.c_int, _c_i32p, ctypes.c_int32, ctypes.c_int, ctypes.c_int32, ctypes.c_int32,
+            _c_i32p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, _c_u64p,
+            ctypes.c_char_p, ctypes.c_int]
+        lib.ref_align_pairs.argtypes = [
+            ctypes.c_char_p, _c_u64p, ctypes.c_uint64, ctypes.c_char_p, _c_u64p, ctypes.c_uint64,
+            _c_u32p, _c_u32p, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+            ctypes.c_int32, ctypes.c_int32, ctypes.c_int, _c_u64p, ctypes.c_char_p, ctypes.c_int]
         lib.ref_distance_matrix.argtypes = [
             ctypes.c_int, ctypes.c_char_p, _c_u64p, ctypes.c_uint64, ctypes.c_char_p, _c_u64p,
             ctypes.c_uint64, ctypes.c_int, _c_i32p, ctypes.c_char_p, ctypes.c_int]
@@ -300,6 +332,43 @@ class Reference:
         if rc:
             raise ReferenceError_(rc, err.value.decode())
         return out.value
+
+    def align_pair(self, q, r, matrix, local=False, go=11, ge=1, alphabet="aa"):
+        """The reference's align_pair: (score, aligned_query, aligned_ref, ops)."""
+        k, grid, size = self._matrix(matrix)
+        qb, rb = _bytes(q), _bytes(r)
+        cap = len(qb) + len(rb) + 1
+        aq, ar, ops = (ctypes.create_string_buffer(cap) for _ in range(3))
+        score = ctypes.c_int32()
+        n = np.zeros(1, np.uint64)
+        err = self._err()
+        rc = self.lib.ref_align_pair(qb, len(qb), rb, len(rb), ALPHABETS[alphabet], k,
+                                     None if grid is None else _p(grid, _c_i32p), size,
+                                     int(local), go, ge, ctypes.byref(score), aq, ar, ops,
+                                     _p(n, _c_u64p), err, 512)
+        if rc:
+            raise ReferenceError_(rc, err.value.decode())
+        L = int(n[0])
+        return score.value, aq.raw[:L].decode("latin-1"), ar.raw[:L].decode("latin-1"), \
+            ops.raw[:L].decode()
+
+    def align_pairs(self, q, r, pq, pr, matrix, local=False, go=11, ge=1, alphabet="aa",
+                    threads=1):
+        """align_pair over index pairs on `threads` workers; returns the total op count."""
+        k, _, _ = self._matrix(matrix)
+        qb, qo = self._csr(q)
+        rb, ro = self._csr(r)
+        pq = np.ascontiguousarray(pq, np.uint32)
+        pr = np.ascontiguousarray(pr, np.uint32)
+        total = np.zeros(1, np.uint64)
+        err = self._err()
+        rc = self.lib.ref_align_pairs(qb, _p(qo, _c_u64p), len(q), rb, _p(ro, _c_u64p), len(r),
+                                      _p(pq, _c_u32p), _p(pr, _c_u32p), len(pq),
+                                      ALPHABETS[alphabet], k, int(local), go, ge, threads,
+                                      _p(total, _c_u64p), err, 512)
+        if rc:
+            raise ReferenceError_(rc, err.value.decode())
+        return int(total[0])
 
     def distance_matrix(self, which, a, b, threads=1):
         ab, ao = self._csr(a)

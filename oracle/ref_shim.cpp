@@ -10,11 +10,13 @@
 
 #include "seqtree/seqtree.hpp"
 
+#include <atomic>
 #include <cstdint>
 #include <cstring>
 #include <exception>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 using namespace seqtree;
@@ -128,6 +130,60 @@ int ref_align_score(const char* q, uint64_t m, const char* r, uint64_t n, int al
             throw std::runtime_error("pairwise needs a matrix");
         *out = align_score(std::string_view(q, m), std::string_view(r, n), mat, Alphabet(alphabet),
                            local ? AlignMode::Local : AlignMode::Global, go, ge);
+    });
+}
+
+// align_pair (src/pairwise.cpp:141-267): the score and the three strings, each written to
+// a buffer of at least m + n bytes, their common length in *len.
+int ref_align_pair(const char* q, uint64_t m, const char* r, uint64_t n, int alphabet, int kind,
+                   const int32_t* custom_sim, int32_t custom_size, int local, int32_t go,
+                   int32_t ge, int32_t* score, char* aq, char* ar, char* ops, uint64_t* len,
+                   char* err, int errlen) {
+    return guarded(err, errlen, [&] {
+        SubstitutionMatrix mat;
+        if (!make_matrix(kind, custom_sim, custom_size, mat))
+            throw std::runtime_error("pairwise needs a matrix");
+        const Alignment a =
+            align_pair(std::string_view(q, m), std::string_view(r, n), mat, Alphabet(alphabet),
+                       local ? AlignMode::Local : AlignMode::Global, go, ge);
+        *score = a.score;
+        *len = a.ops.size();
+        std::memcpy(aq, a.aligned_query.data(), a.aligned_query.size());
+        std::memcpy(ar, a.aligned_ref.data(), a.aligned_ref.size());
+        std::memcpy(ops, a.ops.data(), a.ops.size());
+    });
+}
+
+// align_pair over a list of (query, reference) index pairs on `threads` workers (pairs taken
+// from an atomic counter): the CPU baseline of batched traceback. Returns the total op count.
+int ref_align_pairs(const char* qb, const uint64_t* qo, uint64_t N, const char* rb,
+                    const uint64_t* ro, uint64_t K, const uint32_t* pq, const uint32_t* pr,
+                    uint64_t n_pairs, int alphabet, int kind, int local, int32_t go, int32_t ge,
+                    int threads, uint64_t* total_ops, char* err, int errlen) {
+    return guarded(err, errlen, [&] {
+        SubstitutionMatrix mat;
+        if (!make_matrix(kind, nullptr, 0, mat))
+            throw std::runtime_error("pairwise needs a matrix");
+        (void)N;
+        (void)K;
+        std::atomic<uint64_t> next{0}, ops{0};
+        auto work = [&] {
+            uint64_t mine = 0;
+            for (uint64_t t; (t = next.fetch_add(1)) < n_pairs;) {
+                const uint32_t a = pq[t], b = pr[t];
+                const Alignment al = align_pair(
+                    std::string_view(qb + qo[a], qo[a + 1] - qo[a]),
+                    std::string_view(rb + ro[b], ro[b + 1] - ro[b]), mat, Alphabet(alphabet),
+                    local ? AlignMode::Local : AlignMode::Global, go, ge);
+                mine += al.ops.size();
+            }
+            ops += mine;
+        };
+        std::vector<std::thread> pool;
+        for (int w = 1; w < threads; ++w) pool.emplace_back(work);
+        work();
+        for (auto& th : pool) th.join();
+        *total_ops = ops.load();
     });
 }
 

@@ -158,6 +158,125 @@ int64_t or_gotoh(const uint8_t* q, uint64_t m, const uint8_t* r, uint64_t n, con
 }
 
 /* ---------------------------------------------------------------------------------------
+ * Gotoh alignment with traceback (reference src/pairwise.cpp:141-267): full int64 M/X/Y
+ * matrices and their back-pointers, the reference's argmax order (M, then X, then Y, each
+ * replacing only on a strictly larger value), the first strict maximum in row-major order as
+ * the local end, and its traceback rules. Writes the score, the ops in forward order ('M'
+ * equal codes, 'S' substitution, 'D' query residue against a gap, 'I' reference residue
+ * against a gap), their count and the first aligned residue of each side (the aligned
+ * strings are those residues, decoded, in op order).
+ * ------------------------------------------------------------------------------------- */
+enum { OR_M = 0, OR_X = 1, OR_Y = 2, OR_STOP = 3 };
+
+static int64_t or_pick(int64_t vm, int64_t vx, int64_t vy, uint8_t* who) {
+    int64_t v = vm;
+    *who = OR_M;
+    if (vx > v) { v = vx; *who = OR_X; }
+    if (vy > v) { v = vy; *who = OR_Y; }
+    return v;
+}
+
+void or_align_pair(const uint8_t* q, uint64_t m, const uint8_t* r, uint64_t n,
+                   const int32_t* sim, int32_t A, int32_t local, int32_t go, int32_t ge,
+                   int64_t* score, uint8_t* ops, uint64_t* n_ops, uint64_t* q_begin,
+                   uint64_t* r_begin) {
+    const int64_t NEG = INT64_MIN / 4;
+    *n_ops = 0;
+    *q_begin = 0;
+    *r_begin = 0;
+    if (m == 0 || n == 0) {
+        *score = local ? 0 : -gap_run((int64_t)(m ? m : n), go, ge);
+        if (!local) {
+            for (uint64_t i = 0; i < m; ++i) ops[(*n_ops)++] = 'D';
+            for (uint64_t j = 0; j < n; ++j) ops[(*n_ops)++] = 'I';
+        }
+        return;
+    }
+    const uint64_t W = n + 1, sz = (m + 1) * W;
+    int64_t* M = (int64_t*)malloc(sizeof(int64_t) * 3 * sz);
+    int64_t *X = M + sz, *Y = X + sz;
+    uint8_t* bM = (uint8_t*)malloc(3 * sz);
+    uint8_t *bX = bM + sz, *bY = bX + sz;
+    for (uint64_t k = 0; k < sz; ++k) {
+        M[k] = X[k] = Y[k] = NEG;
+        bM[k] = OR_STOP;
+        bX[k] = bY[k] = OR_M;
+    }
+    M[0] = 0;
+    for (uint64_t j = 1; j <= n; ++j) {
+        M[j] = local ? 0 : NEG;
+        Y[j] = local ? NEG : -gap_run((int64_t)j, go, ge);
+        bY[j] = j > 1 ? OR_Y : OR_M;
+    }
+    for (uint64_t i = 1; i <= m; ++i) {
+        M[i * W] = local ? 0 : NEG;
+        X[i * W] = local ? NEG : -gap_run((int64_t)i, go, ge);
+        bX[i * W] = i > 1 ? OR_X : OR_M;
+    }
+    int64_t best = local ? 0 : NEG;
+    uint64_t bi = 0, bj = 0;
+    for (uint64_t i = 1; i <= m; ++i)
+        for (uint64_t j = 1; j <= n; ++j) {
+            const uint64_t k = i * W + j, kd = (i - 1) * W + (j - 1), ku = (i - 1) * W + j,
+                           kl = i * W + (j - 1);
+            uint8_t who;
+            int64_t mv = or_pick(M[kd], X[kd], Y[kd], &who) +
+                         (int64_t)sim[(size_t)q[i - 1] * (size_t)A + r[j - 1]];
+            if (local && mv < 0) { mv = 0; who = OR_STOP; }
+            M[k] = mv;
+            bM[k] = who;
+            X[k] = or_pick(M[ku] - go, X[ku] - ge, Y[ku] - go, &bX[k]);
+            Y[k] = or_pick(M[kl] - go, X[kl] - go, Y[kl] - ge, &bY[k]);
+            if (local && mv > best) { best = mv; bi = i; bj = j; }
+        }
+    uint64_t i, j;
+    uint8_t st;
+    if (local) {
+        *score = best;
+        i = bi;
+        j = bj;
+        st = OR_M;
+        if (best == 0) { free(M); free(bM); return; }
+    } else {
+        const uint64_t end = m * W + n;
+        *score = or_pick(M[end], X[end], Y[end], &st);
+        i = m;
+        j = n;
+    }
+    uint64_t cnt = 0;  /* ops written backwards from the end of a buffer of m + n */
+    uint8_t* rev = (uint8_t*)malloc(m + n);
+    while (i > 0 || j > 0) {
+        const uint64_t k = i * W + j;
+        if (st == OR_M) {
+            if (i == 0 || j == 0) break;
+            if (local && M[k] == 0) break;
+            rev[cnt++] = q[i - 1] == r[j - 1] ? 'M' : 'S';
+            st = bM[k];
+            --i;
+            --j;
+            if (st == OR_STOP) break;
+        } else if (st == OR_X) {
+            if (i == 0) break;
+            rev[cnt++] = 'D';
+            st = bX[k];
+            --i;
+        } else {
+            if (j == 0) break;
+            rev[cnt++] = 'I';
+            st = bY[k];
+            --j;
+        }
+    }
+    for (uint64_t t = 0; t < cnt; ++t) ops[t] = rev[cnt - 1 - t];
+    *n_ops = cnt;
+    *q_begin = i;
+    *r_begin = j;
+    free(rev);
+    free(M);
+    free(bM);
+}
+
+/* ---------------------------------------------------------------------------------------
  * Unit-cost distances (reference src/distance.cpp:68-95).
  * ------------------------------------------------------------------------------------- */
 int32_t or_hamming(const uint8_t* a, uint64_t m, const uint8_t* b, uint64_t n) {

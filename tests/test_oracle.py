@@ -327,3 +327,75 @@ def test_brute_helpers_are_not_vacuous():
     sd = {a: {b: (4 if a == b else -1) for b in "AK"} for a in "AK"}
     assert brute_global("AAA", "KKAAA", sd, 11, 1) == 12 - 12
     assert brute_local("KAAK", "AA", sd, 11, 1) == 8
+
+
+# -------------------------------------------------------------------------- traceback (8f-2)
+
+def aligned_strings(q, r, ops, qb, rb):
+    """The aligned strings an op string spells out from residues q[qb:], r[rb:]."""
+    aq, ar, i, j = [], [], qb, rb
+    for o in ops:
+        if o in "MS":
+            aq.append(q[i]); ar.append(r[j]); i += 1; j += 1
+        elif o == "D":
+            aq.append(q[i]); ar.append("-"); i += 1
+        else:
+            aq.append("-"); ar.append(r[j]); j += 1
+    return "".join(aq), "".join(ar)
+
+
+def rescore(aq, ar, sd, go, ge):
+    """Gotoh score of an explicit alignment (the reference's own test re-scorer, restated)."""
+    total, prev = 0, None
+    for x, y in zip(aq, ar):
+        if x == "-":
+            total -= ge if prev == "I" else go
+            prev = "I"
+        elif y == "-":
+            total -= ge if prev == "D" else go
+            prev = "D"
+        else:
+            total += sd[x][y]
+            prev = None
+    return total
+
+
+@pytest.mark.parametrize("local", [False, True])
+@pytest.mark.parametrize("name,go,ge", [("blosum62", 11, 1), ("blosum62", 5, 5),
+                                        ("pam250", 1, 3), ("blosum45", 0, 0),
+                                        ("blosum62", 1, 10), ("structural", 2, 100)])
+def test_align_pair_matches_reference(oracle_lib, reference, name, go, ge, local):
+    """Score, ops and aligned strings equal the compiled reference's align_pair, including
+    tie-heavy inputs (few residue kinds) where the back-pointer order decides the path."""
+    sim, _, _ = _tables(reference, name)
+    rng = random.Random(hash((name, go, ge, local)) & 0xFFFF)
+    cases = [(rand_seq(rng, 0, 22), rand_seq(rng, 0, 22)) for _ in range(70)]
+    cases += [(rand_seq(rng, 1, 14, "AAC"), rand_seq(rng, 1, 14, "AAC")) for _ in range(70)]
+    cases += [("", ""), ("", "ACD"), ("WYK", ""), ("W", "W"), ("WWW", "KKK")]
+    for q, r in cases:
+        s, ops, qb, rb = oracle_lib.align_pair(q, r, sim, local, go, ge)
+        want = reference.align_pair(q, r, name, local, go, ge)
+        assert (s, ops) == (want[0], want[3]), (q, r)
+        assert aligned_strings(q, r, ops, qb, rb) == (want[1], want[2]), (q, r)
+
+
+def test_align_pair_known_answer_and_rescore(oracle_lib, reference):
+    """The reference's docstring example (python/seqtree/pairwise.py:109-112) and its test that
+    a traceback re-scores to the reported score (tests/python/test_pairwise.py:194-212)."""
+    sim, _, _ = _tables(reference, "blosum62")
+    q, r = "CASSLGQAYEQYF", "CASSPGQAYEQF"
+    s, ops, qb, rb = oracle_lib.align_pair(q, r, sim)
+    assert aligned_strings(q, r, ops, qb, rb) == ("CASSLGQAYEQYF", "CASSPGQAYEQ-F")
+    assert s == 45
+    sd = _sim_dict(sim)
+    rng = random.Random(17)
+    for local in (False, True):
+        for go, ge in [(11, 1), (5, 5), (1, 3), (0, 0), (1, 10), (2, 100)]:
+            for _ in range(30):
+                a, b = rand_seq(rng, 4, 20), rand_seq(rng, 4, 20)
+                s, ops, qb, rb = oracle_lib.align_pair(a, b, sim, local, go, ge)
+                aq, ar = aligned_strings(a, b, ops, qb, rb)
+                assert rescore(aq, ar, sd, go, ge) == s
+                assert s == oracle_lib.gotoh(a, b, sim, local, go, ge)
+                if not local:
+                    assert aq.replace("-", "") == a and ar.replace("-", "") == b
