@@ -109,6 +109,34 @@ int32_t st_gapblock_pairs(const uint8_t* q_codes, const uint64_t* q_offsets, uin
                           const uint32_t* pair_query, const uint32_t* pair_ref, uint64_t n_pairs,
                           int32_t* out, st_status* status);
 
+/* ---- Gotoh alignment with traceback (src/pairwise.cpp:141-267, `align_pair`) ----------
+ * Scores and traces back every (pair_query[p], pair_ref[p]) with the reference's recurrences,
+ * boundary values, argmax order and traceback rules. Outputs per pair: score[p] (int64; the
+ * reference narrows it to int32 and raises OverflowError past that), the ops in forward order
+ * at ops + p * ops_stride ('M' equal codes, 'S' substitution, 'D' query residue against a
+ * gap, 'I' reference residue against a gap; ops_stride >= q_max_len + r_max_len), their count
+ * n_ops[p], and the first aligned residue of each side (q_begin[p], r_begin[p]: 0 in global
+ * mode; in local mode where the matched sub-sequences start). The aligned strings are those
+ * residues, decoded, in op order. align_pair is the one-pair case; the batch has no
+ * counterpart in the reference (it is the traceback step for candidate pairs). */
+int32_t st_align_pairs(const uint8_t* q_codes, const uint64_t* q_offsets, uint64_t n_queries,
+                       const uint8_t* r_codes, const uint64_t* r_offsets, uint64_t n_refs,
+                       const uint32_t* pair_query, const uint32_t* pair_ref, uint64_t n_pairs,
+                       const int32_t* sim, uint32_t alphabet_size, int32_t local,
+                       int32_t gap_open, int32_t gap_extend, uint8_t* ops, uint64_t ops_stride,
+                       int64_t* score, uint32_t* n_ops, uint32_t* q_begin, uint32_t* r_begin,
+                       st_status* status);
+/* Device-resident sets, pairs and outputs on `stream` (pair indices must be in range). */
+int32_t st_align_pairs_device(const uint8_t* q_codes, const uint64_t* q_offsets,
+                              uint64_t n_queries, uint32_t q_max_len, const uint8_t* r_codes,
+                              const uint64_t* r_offsets, uint64_t n_refs, uint32_t r_max_len,
+                              const uint32_t* pair_query, const uint32_t* pair_ref,
+                              uint64_t n_pairs, const int32_t* sim, uint32_t alphabet_size,
+                              int32_t local, int32_t gap_open, int32_t gap_extend, uint8_t* ops,
+                              uint64_t ops_stride, int64_t* score, uint32_t* n_ops,
+                              uint32_t* q_begin, uint32_t* r_begin, void* stream,
+                              st_status* status);
+
 /* ---- device-memory entry points ------------------------------------------------------
  * Same computations with the sequences (CSR bytes and offsets) and the output already in
  * device memory; the small tables (pen, sim, prior) stay host pointers. The work is enqueued

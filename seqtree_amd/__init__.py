@@ -3,11 +3,12 @@
 A drop-in for the reference's dense-matrix API (seqtree 0.6.0, ``python/seqtree/__init__.py``
 hot-path re-exports): ``score_matrix`` and the gap priors (:mod:`seqtree_amd.gapblock`), Gotoh
 ``score_matrix`` / ``dist_matrix`` (:mod:`seqtree_amd.pairwise`) and ``hamming_matrix`` /
-``levenshtein_matrix`` (:mod:`seqtree_amd.distance`). Results come back as the same zero-copy
-``ScoreMatrix``. The scoring runs in hand-written HIP kernels; importing this package loads
+``levenshtein_matrix`` (:mod:`seqtree_amd.distance`), plus Gotoh alignment with traceback
+(``pairwise.align``, ``pairwise.align_pairs``; ``Alignment``). Results come back as the same
+zero-copy ``ScoreMatrix``. The scoring runs in hand-written HIP kernels; importing this package loads
 the native library and fails if it has not been built.
 """
-from ._core import ScoreMatrix, SubstitutionMatrix, alphabet_symbols, amino_acids
+from ._core import Alignment, ScoreMatrix, SubstitutionMatrix, alphabet_symbols, amino_acids
 from ._core import device_count, version
 from . import distance, gapblock, pairwise
 from .gapblock import (
@@ -19,6 +20,7 @@ __all__ = [
     "distance",
     "gapblock",
     "pairwise",
+    "Alignment",
     "ScoreMatrix",
     "SubstitutionMatrix",
     "alphabet_symbols",

@@ -27,7 +27,7 @@ CSRC = HERE / "csrc"
 HOST = CSRC / "host"
 BUILD = HERE / "_build"
 
-HIP_SOURCES = ["prep.hip", "gapblock.hip", "batch.hip", "gotoh.hip", "distance.hip",
+HIP_SOURCES = ["prep.hip", "gapblock.hip", "batch.hip", "gotoh.hip", "traceback.hip", "distance.hip",
                "probe.hip", "prefilter.hip", "capi.hip"]
 HOST_SOURCES = ["codec.cpp", "substitution_matrix.cpp", "matrix_data.cpp", "api.cpp"]
 

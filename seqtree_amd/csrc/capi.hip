@@ -995,6 +995,66 @@ void pf_encode_forward(const __bf16* B, const float* P, const uint8_t* codes, co
     launched("prefilter encode forward");
 }
 
+// ------------------------------------------------------------------------------------------
+// Gotoh alignment with traceback of a pair list
+// ------------------------------------------------------------------------------------------
+// Runs the traceback kernels on device-resident sets: int32 cells when every value stays far
+// inside int32 ((q_max + r_max + 2) * max(|sim|, go, ge) < 2^28, so the -2^29 sentinel and a
+// few gap steps below it keep every comparison of the int64 DP), int64 cells otherwise. The
+// direction bytes (and, past the LDS limit or for int64, the DP rows) take per-workgroup
+// scratch; the grid is sized so that scratch stays under 512 MB.
+void traceback_device(const uint8_t* q, const uint64_t* q_off, uint32_t q_max, const uint8_t* r,
+                      const uint64_t* r_off, uint32_t r_max, const uint32_t* pq,
+                      const uint32_t* pr, uint64_t n_pairs, const int32_t* sim_h, uint32_t A,
+                      int32_t local, int32_t go, int32_t ge, uint8_t* ops, uint64_t ops_stride,
+                      int64_t* score, uint32_t* n_ops, uint32_t* q_begin, uint32_t* r_begin,
+                      hipStream_t s) {
+    if (go < 0 || ge < 0)
+        throw HipError{ST_INVALID_ARGUMENT, "seqtree: gap_open and gap_extend must be >= 0"};
+    if (ops_stride < uint64_t(q_max) + r_max)
+        throw HipError{ST_INVALID_ARGUMENT,
+                       "seqtree_hip: ops_stride is below q_max_len + r_max_len"};
+    if (n_pairs == 0) return;
+    Arena ar(s);
+    const DeviceTable sim = table_on_device(ar, sim_h, size_t(A) * A, 0, s);
+    const int64_t mx = std::max<int64_t>({sim.abs_max, int64_t(go), int64_t(ge)});
+    const bool wide = (int64_t(q_max) + r_max + 2) * mx >= (int64_t(1) << 28);
+    const bool lds_rows = !wide && r_max <= kTracebackLdsMaxLen;
+    const uint64_t W = uint64_t(r_max) + 1;
+    const uint64_t dir_bytes = (uint64_t(q_max) + 1) * W * 64;  // per workgroup
+    const uint64_t row_bytes = lds_rows ? 0 : 3 * W * 64 * (wide ? 8 : 4);
+    const uint64_t need = (n_pairs + 63) / 64;
+    const uint64_t fit = std::max<uint64_t>(1, (uint64_t(512) << 20) / (dir_bytes + row_bytes));
+    const uint32_t wgs = uint32_t(std::min<uint64_t>({need, fit, 65535}));
+    uint8_t* dirs = ar.get<uint8_t>(dir_bytes * wgs);
+    void* rows = row_bytes ? static_cast<void*>(ar.get<uint8_t>(row_bytes * wgs)) : nullptr;
+    TracebackArgs a{};
+    a.q_codes = q;
+    a.q_off = q_off;
+    a.r_codes = r;
+    a.r_off = r_off;
+    a.pair_q = pq;
+    a.pair_r = pr;
+    a.n_pairs = n_pairs;
+    a.sim = sim.dev;
+    a.A = A;
+    a.local = local;
+    a.gap_open = go;
+    a.gap_extend = ge;
+    a.q_max = q_max;
+    a.r_max = r_max;
+    a.ops = ops;
+    a.ops_stride = ops_stride;
+    a.score = score;
+    a.n_ops = n_ops;
+    a.q_begin = q_begin;
+    a.r_begin = r_begin;
+    LaunchTimer t(std::string("traceback_") + (wide ? "i64" : "i32") + (local ? "_local" : "_global"),
+                  s);
+    launch_traceback(a, wide, lds_rows, wgs, rows, dirs, s);
+    launched("traceback");
+}
+
 struct PrefilterImpl {
     float* P = nullptr;   // FP32 master parameters
     float* G = nullptr;   // gradients
@@ -1618,6 +1678,73 @@ int32_t st_gapblock_pairs(const uint8_t* q_codes, const uint64_t* q_offsets, uin
                               rh.max_len, pen, alphabet_size, gap_open, gap_extend, prior,
                               prior_width, pq, pr, n_pairs, dout, s);
         d2h(out, dout, n_pairs, s);
+        sync(s);
+        return ok(status);
+    });
+}
+
+int32_t st_align_pairs_device(const uint8_t* q_codes, const uint64_t* q_offsets,
+                              uint64_t n_queries, uint32_t q_max_len, const uint8_t* r_codes,
+                              const uint64_t* r_offsets, uint64_t n_refs, uint32_t r_max_len,
+                              const uint32_t* pair_query, const uint32_t* pair_ref,
+                              uint64_t n_pairs, const int32_t* sim, uint32_t alphabet_size,
+                              int32_t local, int32_t gap_open, int32_t gap_extend, uint8_t* ops,
+                              uint64_t ops_stride, int64_t* score, uint32_t* n_ops,
+                              uint32_t* q_begin, uint32_t* r_begin, void* stream,
+                              st_status* status) {
+    if (alphabet_size == 0 || alphabet_size > kMaxAlphabet) return bad_alphabet(status, alphabet_size);
+    (void)n_queries;
+    (void)n_refs;
+    return guarded(status, [&] {
+        require_device();
+        traceback_device(q_codes, q_offsets, q_max_len, r_codes, r_offsets, r_max_len,
+                         pair_query, pair_ref, n_pairs, sim, alphabet_size, local, gap_open,
+                         gap_extend, ops, ops_stride, score, n_ops, q_begin, r_begin,
+                         static_cast<hipStream_t>(stream));
+        return ok(status);
+    });
+}
+
+int32_t st_align_pairs(const uint8_t* q_codes, const uint64_t* q_offsets, uint64_t n_queries,
+                       const uint8_t* r_codes, const uint64_t* r_offsets, uint64_t n_refs,
+                       const uint32_t* pair_query, const uint32_t* pair_ref, uint64_t n_pairs,
+                       const int32_t* sim, uint32_t alphabet_size, int32_t local,
+                       int32_t gap_open, int32_t gap_extend, uint8_t* ops, uint64_t ops_stride,
+                       int64_t* score, uint32_t* n_ops, uint32_t* q_begin, uint32_t* r_begin,
+                       st_status* status) {
+    if (alphabet_size == 0 || alphabet_size > kMaxAlphabet) return bad_alphabet(status, alphabet_size);
+    if (n_pairs == 0) return ok(status);
+    for (uint64_t t = 0; t < n_pairs; ++t)
+        if (pair_query[t] >= n_queries || pair_ref[t] >= n_refs) {
+            set_status(status, ST_INVALID_ARGUMENT,
+                       "seqtree_hip: pair %llu indexes past the query or reference set",
+                       (unsigned long long)t);
+            return int(ST_INVALID_ARGUMENT);
+        }
+    return guarded(status, [&] {
+        require_device();
+        const HostSet qh = host_set(q_codes, q_offsets, n_queries);
+        const HostSet rh = host_set(r_codes, r_offsets, n_refs);
+        OwnedStream os;
+        const hipStream_t s = os.get();
+        Arena ar(s);
+        const DeviceSet qd = upload(ar, qh, s);
+        const DeviceSet rd = upload(ar, rh, s);
+        uint32_t* pq = ar.get<uint32_t>(n_pairs);
+        uint32_t* pr = ar.get<uint32_t>(n_pairs);
+        h2d(pq, pair_query, n_pairs, s);
+        h2d(pr, pair_ref, n_pairs, s);
+        uint8_t* d_ops = ar.get<uint8_t>(n_pairs * ops_stride);
+        int64_t* d_score = ar.get<int64_t>(n_pairs);
+        uint32_t* d_meta = ar.get<uint32_t>(3 * n_pairs);
+        traceback_device(qd.bytes, qd.off, qh.max_len, rd.bytes, rd.off, rh.max_len, pq, pr,
+                         n_pairs, sim, alphabet_size, local, gap_open, gap_extend, d_ops,
+                         ops_stride, d_score, d_meta, d_meta + n_pairs, d_meta + 2 * n_pairs, s);
+        d2h(ops, d_ops, n_pairs * ops_stride, s);
+        d2h(score, d_score, n_pairs, s);
+        d2h(n_ops, d_meta, n_pairs, s);
+        d2h(q_begin, d_meta + n_pairs, n_pairs, s);
+        d2h(r_begin, d_meta + 2 * n_pairs, n_pairs, s);
         sync(s);
         return ok(status);
     });

@@ -6,11 +6,14 @@
 //   gapblock_matrix     src/gapblock.cpp:12-22 (encode), 66-103 (checks, order, early return)
 //   align_*_matrix      src/pairwise.cpp:51-68 (encode, check), 273-286, 338-352
 //   align_score         src/pairwise.cpp:131-139
+//   align_pair          src/pairwise.cpp:141-267 (traceback on the device; the empty-input
+//                       case and the narrowing to int32 on the host, as the reference)
 //   hamming/levenshtein src/distance.cpp:68-109
 #include "seqtree.hpp"
 #include "seqtree_hip.h"
 
 #include <algorithm>
+#include <climits>
 #include <stdexcept>
 #include <string>
 
@@ -121,6 +124,91 @@ int32_t align_score(std::string_view query, std::string_view ref, const Substitu
                                    mode == AlignMode::Local, 0, gap_open, gap_extend, &out, &st);
     raise(rc, st);
     return out;
+}
+
+namespace {
+
+// The reference's narrow() for Alignment::score (pairwise.cpp:40-50).
+int32_t narrow_score(int64_t v) {
+    if (v < INT32_MIN || v > INT32_MAX)
+        throw std::overflow_error("seqtree: the alignment score does not fit in 32 bits -- the "
+                                  "gap costs are absurdly large for these sequences");
+    return int32_t(v);
+}
+
+// The Alignment the reference builds from a traceback: decoded residues in op order, or --
+// when a side is empty -- the caller's own characters (pairwise.cpp:150-165).
+Alignment build_alignment(std::string_view query, std::string_view ref, const Codec& codec,
+                          const uint8_t* q, const uint8_t* r, int64_t score, const uint8_t* ops,
+                          uint32_t n_ops, uint32_t qb, uint32_t rb) {
+    Alignment al;
+    al.score = narrow_score(score);
+    const bool empty = query.empty() || ref.empty();
+    al.ops.assign(reinterpret_cast<const char*>(ops), n_ops);
+    al.aligned_query.reserve(n_ops);
+    al.aligned_ref.reserve(n_ops);
+    uint32_t i = qb, j = rb;
+    for (uint32_t t = 0; t < n_ops; ++t) {
+        const char o = char(ops[t]);
+        if (o == 'M' || o == 'S') {
+            al.aligned_query += empty ? query[i] : codec.decode(q[i]);
+            al.aligned_ref += empty ? ref[j] : codec.decode(r[j]);
+            ++i;
+            ++j;
+        } else if (o == 'D') {
+            al.aligned_query += empty ? query[i] : codec.decode(q[i]);
+            al.aligned_ref += '-';
+            ++i;
+        } else {
+            al.aligned_query += '-';
+            al.aligned_ref += empty ? ref[j] : codec.decode(r[j]);
+            ++j;
+        }
+    }
+    return al;
+}
+
+}  // namespace
+
+std::vector<Alignment> align_pairs(const std::vector<std::string>& queries,
+                                   const std::vector<std::string>& refs,
+                                   const std::vector<uint32_t>& pair_query,
+                                   const std::vector<uint32_t>& pair_ref,
+                                   const SubstitutionMatrix& mat, Alphabet alphabet,
+                                   AlignMode mode, int32_t gap_open, int32_t gap_extend) {
+    const Codec codec(alphabet);
+    check_pairwise(mat, codec, gap_open, gap_extend);
+    if (pair_query.size() != pair_ref.size())
+        throw std::invalid_argument("seqtree: pair_query and pair_ref differ in length");
+    const size_t P = pair_query.size();
+    std::vector<Alignment> out(P);
+    if (P == 0) return out;
+    const Csr q = pack_pairwise(queries, codec);
+    const Csr r = pack_pairwise(refs, codec);
+    const uint64_t stride = uint64_t(q.max_len) + r.max_len + 1;
+    std::vector<uint8_t> ops(P * stride);
+    std::vector<int64_t> score(P);
+    std::vector<uint32_t> n_ops(P), qb(P), rb(P);
+    st_status st{};
+    raise(st_align_pairs(q.bytes.data(), q.off.data(), queries.size(), r.bytes.data(),
+                         r.off.data(), refs.size(), pair_query.data(), pair_ref.data(), P,
+                         mat.similarities().data(), mat.size(), mode == AlignMode::Local,
+                         gap_open, gap_extend, ops.data(), stride, score.data(), n_ops.data(),
+                         qb.data(), rb.data(), &st),
+          st);
+    for (size_t p = 0; p < P; ++p) {
+        const uint32_t a = pair_query[p], b = pair_ref[p];
+        out[p] = build_alignment(queries[a], refs[b], codec, q.bytes.data() + q.off[a],
+                                 r.bytes.data() + r.off[b], score[p], ops.data() + p * stride,
+                                 n_ops[p], qb[p], rb[p]);
+    }
+    return out;
+}
+
+Alignment align_pair(std::string_view query, std::string_view ref, const SubstitutionMatrix& mat,
+                     Alphabet alphabet, AlignMode mode, int32_t gap_open, int32_t gap_extend) {
+    return align_pairs({std::string(query)}, {std::string(ref)}, {0u}, {0u}, mat, alphabet, mode,
+                       gap_open, gap_extend)[0];
 }
 
 std::vector<int32_t> align_score_matrix(const std::vector<std::string>& queries,

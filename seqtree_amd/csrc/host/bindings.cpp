@@ -175,6 +175,50 @@ PYBIND11_MODULE(_core, m) {
         py::arg("gap_open") = 11, py::arg("gap_extend") = 1, py::arg("alphabet") = "aa",
         "Optimal Gotoh similarity score ('global' Needleman-Wunsch, 'local' Smith-Waterman).");
 
+    py::class_<Alignment>(m, "Alignment",
+                          "An alignment: aligned strings, ops and the similarity score.")
+        .def_readonly("aligned_query", &Alignment::aligned_query)
+        .def_readonly("aligned_ref", &Alignment::aligned_ref)
+        .def_readonly("ops", &Alignment::ops)
+        .def_readonly("score", &Alignment::score)
+        .def("__repr__", [](const Alignment& a) {
+            return "Alignment(score=" + std::to_string(a.score) + ", ops='" + a.ops + "')";
+        });
+
+    m.def(
+        "align_pair",
+        [](const std::string& q, const std::string& r, const SubstitutionMatrix& mat,
+           const std::string& mode, int32_t gap_open, int32_t gap_extend,
+           const std::string& alphabet) {
+            const Alphabet a = parse_alphabet(alphabet);
+            const AlignMode md = parse_align_mode(mode);
+            py::gil_scoped_release release;
+            return align_pair(q, r, mat, a, md, gap_open, gap_extend);
+        },
+        py::arg("query"), py::arg("ref"), py::arg("matrix"), py::arg("mode") = "global",
+        py::arg("gap_open") = 11, py::arg("gap_extend") = 1, py::arg("alphabet") = "aa",
+        "As align_score, but also returns the aligned strings and ops (Alignment.score is the "
+        "similarity, not a penalty).");
+
+    m.def(
+        "align_pairs",
+        [](const std::vector<std::string>& q, const std::vector<std::string>& r,
+           const std::vector<uint32_t>& pq, const std::vector<uint32_t>& pr,
+           const SubstitutionMatrix& mat, const std::string& mode, int32_t gap_open,
+           int32_t gap_extend, const std::string& alphabet) {
+            const Alphabet a = parse_alphabet(alphabet);
+            const AlignMode md = parse_align_mode(mode);
+            for (size_t t = 0; t < pq.size() && t < pr.size(); ++t)
+                if (pq[t] >= q.size() || pr[t] >= r.size())
+                    throw py::index_error("pair index out of range");
+            py::gil_scoped_release release;
+            return align_pairs(q, r, pq, pr, mat, a, md, gap_open, gap_extend);
+        },
+        py::arg("queries"), py::arg("refs"), py::arg("pair_query"), py::arg("pair_ref"),
+        py::arg("matrix"), py::arg("mode") = "global", py::arg("gap_open") = 11,
+        py::arg("gap_extend") = 1, py::arg("alphabet") = "aa",
+        "align_pair for every (queries[pair_query[p]], refs[pair_ref[p]]) in one device batch.");
+
     m.def(
         "align_score_matrix",
         [](const std::vector<std::string>& q, const std::vector<std::string>& r,

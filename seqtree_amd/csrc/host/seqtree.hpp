@@ -71,6 +71,27 @@ enum class AlignMode { Global, Local };
 int32_t align_score(std::string_view query, std::string_view ref, const SubstitutionMatrix&,
                     Alphabet, AlignMode, int32_t gap_open, int32_t gap_extend);
 
+// Optimal score plus the aligned strings and ops (reference types.hpp:67-72).
+struct Alignment {
+    std::string aligned_query;
+    std::string aligned_ref;
+    std::string ops;  // per column: 'M' match, 'S' substitution, 'I' insertion, 'D' deletion
+    int32_t score = 0;
+};
+
+// Gotoh alignment with traceback of one pair (reference pairwise.cpp:141-267).
+Alignment align_pair(std::string_view query, std::string_view ref, const SubstitutionMatrix&,
+                     Alphabet, AlignMode, int32_t gap_open, int32_t gap_extend);
+
+// align_pair for every (queries[pair_query[p]], refs[pair_ref[p]]), in one device batch (no
+// reference counterpart: the traceback step for candidate pairs).
+std::vector<Alignment> align_pairs(const std::vector<std::string>& queries,
+                                   const std::vector<std::string>& refs,
+                                   const std::vector<uint32_t>& pair_query,
+                                   const std::vector<uint32_t>& pair_ref,
+                                   const SubstitutionMatrix&, Alphabet, AlignMode,
+                                   int32_t gap_open, int32_t gap_extend);
+
 // Dense N*K similarity matrix, row-major.
 std::vector<int32_t> align_score_matrix(const std::vector<std::string>& queries,
                                         const std::vector<std::string>& refs,

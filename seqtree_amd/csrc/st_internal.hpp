@@ -281,6 +281,38 @@ bool gotoh_self_fits_lds(uint32_t max_len, int64_t mx);
 void launch_gotoh_self_lds(const GotohArgs& a, const uint8_t* codes, const uint64_t* off,
                            uint64_t n, int64_t* res, hipStream_t s);
 
+// ---- traceback.hip ---------------------------------------------------------------------
+// Gotoh alignment with traceback of (pair_q[p], pair_r[p]) for p < n_pairs: score[p] (int64),
+// the ops in forward order at ops + p * ops_stride (ops_stride >= q_max + r_max), their count
+// and the first aligned residue of each side.
+struct TracebackArgs {
+    const uint8_t* q_codes;
+    const uint64_t* q_off;
+    const uint8_t* r_codes;
+    const uint64_t* r_off;
+    const uint32_t* pair_q;
+    const uint32_t* pair_r;
+    uint64_t n_pairs;
+    const int32_t* sim;  // device, A*A [query code][ref code]
+    uint32_t A;
+    int32_t local;
+    int32_t gap_open, gap_extend;
+    uint32_t q_max, r_max;  // bounds of the sequence lengths
+    uint8_t* ops;
+    uint64_t ops_stride;
+    int64_t* score;
+    uint32_t* n_ops;
+    uint32_t* q_begin;
+    uint32_t* r_begin;
+};
+// Longest reference whose DP rows fit the LDS (int32 cells).
+constexpr uint32_t kTracebackLdsMaxLen = 64;
+// wide: int64 cells (else int32, valid under the host's range bound); lds_rows: int32 rows in
+// LDS (r_max <= kTracebackLdsMaxLen), else in `rows` (3 (r_max + 1) 64 cells per workgroup);
+// dirs: (q_max + 1) (r_max + 1) 64 bytes per workgroup.
+void launch_traceback(const TracebackArgs& a, bool wide, bool lds_rows, uint32_t workgroups,
+                      void* rows, uint8_t* dirs, hipStream_t s);
+
 // ---- distance.hip ----------------------------------------------------------------------
 struct DistanceArgs {
     const uint8_t* a_bytes;
