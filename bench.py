@@ -19,6 +19,8 @@ st_align_matrix_device on one HIP stream.
                by 1-3 random substitutions / insertions / deletions
   batch        64 ragged gap-block jobs (512..8,192 x 10,000..200,000, lengths 6..30,
                BLOSUM62 / structural) in one captured persistent launch
+  traceback    Gotoh alignment with traceback (align_pair) of 1,000,000 candidate pairs
+               drawn from two 20,000-string CDR3-like sets, BLOSUM62 11/1, global and local
   prefilter    learned prefilter, 50,000 queries x 1,000,000 references, gap-block cutoff 48,
                against the exhaustive exact thresholded pass; a step is one search against
                reference embeddings built once (their build time and the one-shot pipeline
@@ -154,6 +156,9 @@ class Lib:
             fn = getattr(L, name)
             fn.restype = i32
             fn.argtypes = [vp, vp, u64, u32, vp, vp, u64, u32, vp, vp, vp]
+        L.st_align_pairs_device.restype = i32
+        L.st_align_pairs_device.argtypes = [vp, vp, u64, u32, vp, vp, u64, u32, vp, vp, u64, vp,
+                                            u32, i32, i32, i32, vp, u64, vp, vp, vp, vp, vp, vp]
         L.st_probe_mfma.restype = i32
         L.st_probe_mfma.argtypes = [ctypes.POINTER(ctypes.c_double), vp]
         L.st_probe_hbm_write.restype = i32
@@ -686,6 +691,94 @@ def copy_calibration() -> int:
     return 0
 
 
+TB_PAIRS = 1_000_000
+
+
+def run_traceback(torch, dev, lib, stream, args) -> dict:
+    """Gotoh alignment with traceback of candidate pairs (the step after a thresholded search):
+    1,000,000 random (query, reference) pairs of two 20,000-string CDR3-like sets, both modes,
+    through st_align_pairs_device (scores, ops and the aligned sub-sequence starts)."""
+    rng = np.random.default_rng(SEED + 7)
+    qs = cdr3_set(rng, N_QUERIES, LEN_LO, LEN_HI)
+    rs = cdr3_set(rng, N_REFS, LEN_LO, LEN_HI)
+    Q, R = DeviceSet(torch, dev, qs), DeviceSet(torch, dev, rs)
+    pq = rng.integers(0, Q.n, TB_PAIRS).astype(np.uint32)
+    pr = rng.integers(0, R.n, TB_PAIRS).astype(np.uint32)
+    d_pq = torch.from_numpy(pq.view(np.int32)).to(dev)
+    d_pr = torch.from_numpy(pr.view(np.int32)).to(dev)
+    sim, _, _ = matrix_tables("blosum62")
+    stride = Q.max_len + R.max_len
+    ops = torch.empty(TB_PAIRS * stride, dtype=torch.uint8, device=dev)
+    score = torch.empty(TB_PAIRS, dtype=torch.int64, device=dev)
+    meta = torch.empty(3 * TB_PAIRS, dtype=torch.int32, device=dev)
+    st = Status()
+
+    def step():
+        for local in (0, 1):
+            rc = lib.lib.st_align_pairs_device(
+                Q.bytes.data_ptr(), Q.off.data_ptr(), Q.n, Q.max_len, R.bytes.data_ptr(),
+                R.off.data_ptr(), R.n, R.max_len, d_pq.data_ptr(), d_pr.data_ptr(), TB_PAIRS,
+                sim.ctypes.data, len(AA_ORDER), local, GAP_OPEN, GAP_EXTEND, ops.data_ptr(),
+                stride, score.data_ptr(), meta.data_ptr(), meta.data_ptr() + 4 * TB_PAIRS,
+                meta.data_ptr() + 8 * TB_PAIRS, ctypes.c_void_p(stream.cuda_stream),
+                ctypes.byref(st))
+            check(rc, st, "st_align_pairs_device")
+
+    ms_step, kernels = time_steps(torch, lib, stream, step, args.steps, args.warmup)
+    cells = float((Q.lens[pq].astype(np.float64) * R.lens[pr]).sum())
+    label, launches, avg_s = dominant(kernels)
+    local = label.endswith("_local")
+    # the score recurrences of the cell (K_MIN, as the score kernels); the back-pointers and
+    # their byte come on top, so the fraction below understates the work done
+    roofline = valu_roofline(label, launches, avg_s, cells * K_MIN["local" if local else "global"],
+                             cells_per_launch=cells,
+                             ops_per_cell=K_MIN["local" if local else "global"])
+    cpu = None
+    if args.cpu_rows > 0:
+        to_str = lambda v: "".join(AA_ORDER[c] for c in v)  # noqa: E731
+        cpu = cpu_traceback([to_str(s) for s in qs], [to_str(s) for s in rs], pq, pr,
+                            seconds=args.cpu_seconds, sim=sim)
+    return base_line(
+        "candidate-pair alignments with traceback/s (Gotoh NW+SW align_pair)",
+        2 * TB_PAIRS / (ms_step / 1e3), args.steps, args.warmup, ms_step,
+        {"workload": "traceback_cdr3_1000000pairs_of_20000x20000_blosum62_go11_ge1_global+local",
+         "queries": Q.n, "refs": R.n, "pairs": TB_PAIRS, "lengths": [LEN_LO, LEN_HI],
+         "seed": SEED + 7, "alignments_per_step": 2 * TB_PAIRS, "cells_per_step": 2 * cells,
+         "parallelism": "single GPU"},
+        kernels, roofline, cpu, gcups=2 * cells / (ms_step / 1e3) / 1e9)
+
+
+def cpu_traceback(qs, rs, pq, pr, seconds: float = 10.0, **kw) -> dict:
+    """The reference's own align_pair (oracle/_ref) over the first pairs of the same list on
+    every host core, both modes, repeated until `seconds` of wall time have passed; the plain-C
+    port (one thread) where the compiled reference is absent."""
+    import oracle
+
+    threads = oracle.host_threads()
+    have_ref = oracle.reference_available()
+    n = 20_000
+    pairs, calls = 0, 0
+    t0 = time.perf_counter()
+    while True:
+        for local in (False, True):
+            if have_ref:
+                oracle.Reference().align_pairs(qs, rs, pq[:n], pr[:n], "blosum62", local,
+                                                GAP_OPEN, GAP_EXTEND, threads=threads)
+            else:
+                orc = oracle.Oracle()
+                for t in range(n // 20):
+                    orc.align_pair(qs[pq[t]], rs[pr[t]], kw["sim"], local, GAP_OPEN, GAP_EXTEND)
+            pairs += n if have_ref else n // 20
+        calls += 1
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            break
+    return {"value": pairs / dt, "unit": "alignments/s",
+            "cores": threads if have_ref else 1, "kind": "reference" if have_ref else "port",
+            "sample": f"the first {n if have_ref else n // 20} pairs of the same list, global + "
+                      f"local, {calls} rounds: {pairs} alignments in {dt:.2f} s wall"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -697,7 +790,7 @@ def main() -> int:
                     help="wall time the CPU baseline sample is repeated for")
     ap.add_argument("--workload", default="gotoh",
                     choices=["gotoh", "gapblock", "hamming", "levenshtein", "batch",
-                             "prefilter"])
+                             "traceback", "prefilter"])
     ap.add_argument("--pf-epochs", type=int, default=PF_EPOCHS)
     ap.add_argument("--pf-cutoff", type=int, default=PF_CUTOFF)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -728,6 +821,8 @@ def main() -> int:
         line = run_batch(torch, dev, lib, stream, args)
     elif args.workload == "prefilter":
         line = run_prefilter(torch, dev, lib, stream, args)
+    elif args.workload == "traceback":
+        line = run_traceback(torch, dev, lib, stream, args)
     else:
         line = run_distance(torch, dev, lib, stream, args, args.workload)
     if line.get("roofline") and line["roofline"]["bound"] == "valu":

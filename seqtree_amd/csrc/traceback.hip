@@ -56,15 +56,21 @@ __device__ __forceinline__ T gap_run(uint32_t L, int32_t go, int32_t ge) {
     return L == 0 ? T(0) : T(go) + T(L - 1) * T(ge);
 }
 
-// rows: 3 * W * 64 T per workgroup (LDS or global scratch), dirs: (q_max + 1) * W * 64 bytes
-// per workgroup; W = r_max + 1.
+// Dynamic LDS: the A x A similarity table, then (LDS_ROWS) the lane's reference codes
+// [W][64] and the three DP rows [3][W][64]; otherwise rows in `rows_g` (3 * W * 64 T per
+// workgroup) and codes read from the CSR set. dirs: (q_max + 1) * W * 64 bytes per workgroup;
+// W = r_max + 1.
 template <class T, bool LOCAL, bool LDS_ROWS>
 __global__ __launch_bounds__(kTbLanes) void traceback_kernel(
     TracebackArgs a, T* __restrict__ rows_g, uint8_t* __restrict__ dirs_g) {
-    constexpr uint32_t kLdsW = kTracebackLdsMaxLen + 1;
-    __shared__ T rows_l[LDS_ROWS ? 3 * kLdsW * kTbLanes : 1];
+    extern __shared__ int32_t tb_lds[];
     const uint32_t lane = threadIdx.x;
     const uint32_t W = a.r_max + 1;
+    int32_t* sim_l = tb_lds;
+    for (uint32_t e = lane; e < a.A * a.A; e += kTbLanes) sim_l[e] = a.sim[e];
+    __syncthreads();  // the only barrier: the pair loop below has none
+    uint8_t* rc_l = reinterpret_cast<uint8_t*>(tb_lds + kMaxAlphabet * kMaxAlphabet);
+    T* rows_l = reinterpret_cast<T*>(rc_l + ((W * kTbLanes + 15) & ~15u));
     T* rows = LDS_ROWS ? rows_l : rows_g + uint64_t(blockIdx.x) * 3 * W * kTbLanes;
     uint8_t* dirs = dirs_g + uint64_t(blockIdx.x) * (a.q_max + 1) * W * kTbLanes;
     // lane-interleaved views: row s at column j, and the direction byte of cell (i, j)
@@ -92,6 +98,8 @@ __global__ __launch_bounds__(kTbLanes) void traceback_kernel(
                 for (uint32_t t = 0; t < n; ++t) ops[cnt++] = 'I';
             }
         } else {
+            if constexpr (LDS_ROWS)  // the lane's reference codes, lane-interleaved
+                for (uint32_t c = 0; c < n; ++c) rc_l[c * kTbLanes + lane] = r[c];
             // row 0
             R(0, 0) = 0;
             R(1, 0) = NEG;
@@ -115,11 +123,12 @@ __global__ __launch_bounds__(kTbLanes) void traceback_kernel(
                 R(1, 0) = lx;
                 R(2, 0) = ly;
                 D(row, 0) = kStStop | ((row > 1 ? kStX : kStM) << 2) | (kStM << 4);
-                const int32_t* srow = a.sim + uint32_t(q[row - 1]) * a.A;
+                const int32_t* srow = sim_l + uint32_t(q[row - 1]) * a.A;
                 for (uint32_t c = 1; c <= n; ++c) {
                     const T um = R(0, c), ux = R(1, c), uy = R(2, c);
                     uint8_t wm, wx, wy;
-                    T mv = pick(dm, dx, dy, wm) + T(srow[r[c - 1]]);
+                    const uint32_t rc = LDS_ROWS ? rc_l[(c - 1) * kTbLanes + lane] : r[c - 1];
+                    T mv = pick(dm, dx, dy, wm) + T(srow[rc]);
                     if (LOCAL && mv < 0) {
                         mv = 0;
                         wm = kStStop;
@@ -195,15 +204,19 @@ __global__ __launch_bounds__(kTbLanes) void traceback_kernel(
 template <class T, bool LOCAL>
 void launch_t(const TracebackArgs& a, bool lds_rows, uint32_t workgroups, void* rows,
               uint8_t* dirs, hipStream_t s) {
-    if constexpr (sizeof(T) == 4) {  // LDS rows for int32 only (3 x 65 x 64 x 4 bytes)
+    const size_t W = size_t(a.r_max) + 1;
+    const size_t sim_bytes = kMaxAlphabet * kMaxAlphabet * sizeof(int32_t);
+    if constexpr (sizeof(T) == 4) {  // LDS rows for int32 only
         if (lds_rows) {
+            const size_t lds = sim_bytes + ((W * kTbLanes + 15) & ~size_t(15)) +
+                               3 * W * kTbLanes * sizeof(T);
             hipLaunchKernelGGL((traceback_kernel<T, LOCAL, true>), dim3(workgroups),
-                               dim3(kTbLanes), 0, s, a, static_cast<T*>(rows), dirs);
+                               dim3(kTbLanes), lds, s, a, static_cast<T*>(rows), dirs);
             return;
         }
     }
-    hipLaunchKernelGGL((traceback_kernel<T, LOCAL, false>), dim3(workgroups), dim3(kTbLanes), 0,
-                       s, a, static_cast<T*>(rows), dirs);
+    hipLaunchKernelGGL((traceback_kernel<T, LOCAL, false>), dim3(workgroups), dim3(kTbLanes),
+                       sim_bytes, s, a, static_cast<T*>(rows), dirs);
 }
 
 }  // namespace
