@@ -21,6 +21,8 @@ st_align_matrix_device on one HIP stream.
                BLOSUM62 / structural) in one captured persistent launch
   traceback    Gotoh alignment with traceback (align_pair) of 1,000,000 candidate pairs
                drawn from two 20,000-string CDR3-like sets, BLOSUM62 11/1, global and local
+  counts       per-query histograms of exact gap-block scores, 1,000 queries x 250,000
+               control (the control pass of threshold_for_evalue), ceiling 64
   prefilter    learned prefilter, 50,000 queries x 1,000,000 references, gap-block cutoff 48,
                against the exhaustive exact thresholded pass; a step is one search against
                reference embeddings built once (their build time and the one-shot pipeline
@@ -159,6 +161,9 @@ class Lib:
         L.st_align_pairs_device.restype = i32
         L.st_align_pairs_device.argtypes = [vp, vp, u64, u32, vp, vp, u64, u32, vp, vp, u64, vp,
                                             u32, i32, i32, i32, vp, u64, vp, vp, vp, vp, vp, vp]
+        L.st_gapblock_histogram_device.restype = i32
+        L.st_gapblock_histogram_device.argtypes = [vp, vp, u64, u32, vp, vp, u64, u32, vp, u32,
+                                                   i32, i32, vp, u32, u32, vp, vp, vp]
         L.st_probe_mfma.restype = i32
         L.st_probe_mfma.argtypes = [ctypes.POINTER(ctypes.c_double), vp]
         L.st_probe_hbm_write.restype = i32
@@ -748,6 +753,78 @@ def run_traceback(torch, dev, lib, stream, args) -> dict:
         kernels, roofline, cpu, gcups=2 * cells / (ms_step / 1e3) / 1e9)
 
 
+COUNTS_QUERIES, COUNTS_CONTROL, COUNTS_CEILING = 1000, 250_000, 64
+
+
+def run_counts(torch, dev, lib, stream, args) -> dict:
+    """Control-calibrated counts: per-query histograms of the exact gap-block scores of 1,000
+    CDR3-like queries against a 250,000-string control (lengths 10..20, BLOSUM62, gap_open 28,
+    central prior 21) up to the ceiling 64 -- the one control pass behind threshold_for_evalue
+    and the control half of evalues. A step is st_gapblock_histogram_device: the full score
+    rows scored by the gap-block kernels, each score added to its query's LDS histogram in the
+    kernel (no score matrix is written)."""
+    from seqtree_amd.gapblock import _prior_cube, central_prior
+
+    rng = np.random.default_rng(SEED + 9)
+    qs = cdr3_set(rng, COUNTS_QUERIES, 10, 20)
+    rs = cdr3_set(rng, COUNTS_CONTROL, 10, 20)
+    Q, R = DeviceSet(torch, dev, qs), DeviceSet(torch, dev, rs)
+    _, pen, scale = matrix_tables("blosum62")
+    go, lam = 2 * scale, int(1.5 * scale)
+    width = max(Q.max_len, R.max_len)
+    cube = np.array(_prior_cube(central_prior(lam), width), np.int32)
+    bins = COUNTS_CEILING + 1
+    hist = torch.empty((Q.n, bins + 2), dtype=torch.int64, device=dev)
+    st = Status()
+
+    def step():
+        rc = lib.lib.st_gapblock_histogram_device(
+            Q.bytes.data_ptr(), Q.off.data_ptr(), Q.n, Q.max_len, R.bytes.data_ptr(),
+            R.off.data_ptr(), R.n, R.max_len, pen.ctypes.data, len(AA_ORDER), go, 1,
+            cube.ctypes.data, width, bins, hist.data_ptr(), ctypes.c_void_p(stream.cuda_stream),
+            ctypes.byref(st))
+        check(rc, st, "st_gapblock_histogram_device")
+
+    ms_step, kernels = time_steps(torch, lib, stream, step, args.steps, args.warmup)
+    pairs = Q.n * R.n
+    h = hist.cpu().numpy()
+    if not (h.sum(axis=1) == R.n).all():
+        raise RuntimeError("histogram rows do not sum to the control size")
+    L = np.minimum(Q.lens[:, None], R.lens[None, :])
+    d = np.abs(Q.lens[:, None] - R.lens[None, :])
+    ops = float(np.where(d > 0, 5 * L, L).sum())
+    label, launches, avg_s = dominant(kernels)
+    # one launch per row block: the step's ops averaged over the launches
+    roofline = valu_roofline(label, launches, avg_s, ops * args.steps / launches,
+                             ops_per_position=5, row_blocks=launches // args.steps) \
+        if label.startswith("gapblock") else None
+    hist_label = "score_histogram"
+    if hist_label in kernels:
+        n_h, ms_h = kernels[hist_label]
+        bytes_h = 4.0 * pairs / n_h  # every score of the block is read once
+        ach = bytes_h / (ms_h / n_h / 1e3) / 1e9
+        info = {"kernel": hist_label, "bound": "hbm", "achieved_gbs": round(ach, 1),
+                "peak_gbs": 8000.0, "frac": round(ach / 8000.0, 4),
+                "avg_launch_ms": round(ms_h / n_h, 4), "bytes_per_launch": bytes_h}
+    else:
+        info = None
+    cpu = None
+    if args.cpu_rows > 0:
+        to_str = lambda v: "".join(AA_ORDER[c] for c in v)  # noqa: E731
+        cpu = cpu_reference("gapblock", [to_str(s) for s in qs], [to_str(s) for s in rs],
+                            max(1, min(args.cpu_rows, Q.n) // 25), seconds=args.cpu_seconds,
+                            pen=pen, go=go, cube=cube.tolist(), width=width)
+        cpu["sample"] += " (score rows only; the counting on top is not timed)"
+    extra = {"histogram_roofline": info} if info else {}
+    return base_line(
+        "query-control pairs/s (control-calibrated gap-block score counts)",
+        pairs / (ms_step / 1e3), args.steps, args.warmup, ms_step,
+        {"workload": "counts_cdr3_1000x250000_blosum62_go28_ge1_central21_ceiling64",
+         "queries": Q.n, "refs": R.n, "lengths": [10, 20], "seed": SEED + 9, "bins": bins,
+         "pairs_per_step": pairs, "parallelism": "single GPU"},
+        kernels, roofline, cpu, **extra)
+
+
 def cpu_traceback(qs, rs, pq, pr, seconds: float = 10.0, **kw) -> dict:
     """The reference's own align_pair (oracle/_ref) over the first pairs of the same list on
     every host core, both modes, repeated until `seconds` of wall time have passed; the plain-C
@@ -790,7 +867,7 @@ def main() -> int:
                     help="wall time the CPU baseline sample is repeated for")
     ap.add_argument("--workload", default="gotoh",
                     choices=["gotoh", "gapblock", "hamming", "levenshtein", "batch",
-                             "traceback", "prefilter"])
+                             "traceback", "counts", "prefilter"])
     ap.add_argument("--pf-epochs", type=int, default=PF_EPOCHS)
     ap.add_argument("--pf-cutoff", type=int, default=PF_CUTOFF)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -823,6 +900,8 @@ def main() -> int:
         line = run_prefilter(torch, dev, lib, stream, args)
     elif args.workload == "traceback":
         line = run_traceback(torch, dev, lib, stream, args)
+    elif args.workload == "counts":
+        line = run_counts(torch, dev, lib, stream, args)
     else:
         line = run_distance(torch, dev, lib, stream, args, args.workload)
     if line.get("roofline") and line["roofline"]["bound"] == "valu":

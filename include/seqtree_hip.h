@@ -109,6 +109,29 @@ int32_t st_gapblock_pairs(const uint8_t* q_codes, const uint64_t* q_offsets, uin
                           const uint32_t* pair_query, const uint32_t* pair_ref, uint64_t n_pairs,
                           int32_t* out, st_status* status);
 
+/* ---- per-query score histograms (control-calibrated counts; no counterpart in the
+ * reference, whose E-values count Index search hits, python/seqtree/evalue.py:45-178) ------
+ * histogram[i][bins + 2] (uint64) of the exact gap-block scores of query i against every
+ * reference, as st_gapblock_matrix computes them: bin 0 counts scores below zero, bin 1 + s
+ * the score s in [0, bins), bin bins + 1 scores >= bins. 1 <= bins <= 16382. The counts of
+ * references within any cutoff below `bins`, and every order statistic of the scores up to
+ * it, follow from one histogram. Up to 1022 bins the scoring kernels add each score to an LDS
+ * histogram of their query rows and no score matrix is formed; wider histograms go through
+ * dense row blocks (at most 512 MB of scratch) and a reduction pass. */
+int32_t st_gapblock_histogram(const uint8_t* q_codes, const uint64_t* q_offsets,
+                              uint64_t n_queries, const uint8_t* r_codes,
+                              const uint64_t* r_offsets, uint64_t n_refs, const int32_t* pen,
+                              uint32_t alphabet_size, int32_t gap_open, int32_t gap_extend,
+                              const int32_t* prior, uint32_t prior_width, uint32_t bins,
+                              uint64_t* histogram, st_status* status);
+int32_t st_gapblock_histogram_device(const uint8_t* q_codes, const uint64_t* q_offsets,
+                                     uint64_t n_queries, uint32_t q_max_len,
+                                     const uint8_t* r_codes, const uint64_t* r_offsets,
+                                     uint64_t n_refs, uint32_t r_max_len, const int32_t* pen,
+                                     uint32_t alphabet_size, int32_t gap_open, int32_t gap_extend,
+                                     const int32_t* prior, uint32_t prior_width, uint32_t bins,
+                                     uint64_t* histogram, void* stream, st_status* status);
+
 /* ---- Gotoh alignment with traceback (src/pairwise.cpp:141-267, `align_pair`) ----------
  * Scores and traces back every (pair_query[p], pair_ref[p]) with the reference's recurrences,
  * boundary values, argmax order and traceback rules. Outputs per pair: score[p] (int64; the

@@ -4,13 +4,14 @@ A drop-in for the reference's dense-matrix API (seqtree 0.6.0, ``python/seqtree/
 hot-path re-exports): ``score_matrix`` and the gap priors (:mod:`seqtree_amd.gapblock`), Gotoh
 ``score_matrix`` / ``dist_matrix`` (:mod:`seqtree_amd.pairwise`) and ``hamming_matrix`` /
 ``levenshtein_matrix`` (:mod:`seqtree_amd.distance`), plus Gotoh alignment with traceback
-(``pairwise.align``, ``pairwise.align_pairs``; ``Alignment``). Results come back as the same
+(``pairwise.align``, ``pairwise.align_pairs``; ``Alignment``) and control-calibrated E-values
+over exhaustive gap-block balls (:mod:`seqtree_amd.evalue`). Results come back as the same
 zero-copy ``ScoreMatrix``. The scoring runs in hand-written HIP kernels; importing this package loads
 the native library and fails if it has not been built.
 """
 from ._core import Alignment, ScoreMatrix, SubstitutionMatrix, alphabet_symbols, amino_acids
 from ._core import device_count, version
-from . import distance, gapblock, pairwise
+from . import distance, evalue, gapblock, pairwise
 from .gapblock import (
     UNREACHABLE, central_prior, frame_prior, gap_cost, gapblock_score, positions_prior,
     profile_prior, score_matrix,
@@ -18,6 +19,7 @@ from .gapblock import (
 
 __all__ = [
     "distance",
+    "evalue",
     "gapblock",
     "pairwise",
     "Alignment",

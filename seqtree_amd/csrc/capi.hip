@@ -437,14 +437,15 @@ GapblockPlan gapblock_plan(Arena& ar, const uint8_t* q, const uint64_t* q_off, u
 }
 
 void gapblock_run(const GapblockPlan& p, uint32_t q_max, hipStream_t s) {
+    const std::string mode = p.a.hist         ? "gapblock_histogram"
+                             : p.a.hits.count ? "gapblock_threshold"
+                                              : "gapblock";
     if (p.twin) {
-        LaunchTimer t(std::string(p.a.hits.count ? "gapblock_threshold" : "gapblock") + "_twin",
-                      s);
+        LaunchTimer t(mode + "_twin", s);
         launch_gapblock_twin(p.a, p.twins, twin_capacity(p.a.n_queries), s);
         return;
     }
-    LaunchTimer t(std::string(p.a.hits.count ? "gapblock_threshold" : "gapblock") +
-                      (p.short_path ? "_short" : p.profile ? "_profile" : "_table") +
+    LaunchTimer t(mode + (p.short_path ? "_short" : p.profile ? "_profile" : "_table") +
                       (p.exact ? "_exact" : ""),
                   s);
     launch_gapblock(p.a, p.profile, q_max, p.exact, p.short_path, s);
@@ -1055,6 +1056,52 @@ void traceback_device(const uint8_t* q, const uint64_t* q_off, uint32_t q_max, c
     launched("traceback");
 }
 
+// ------------------------------------------------------------------------------------------
+// Per-query histograms of exact gap-block scores (control-calibrated counts)
+// ------------------------------------------------------------------------------------------
+// The dense kernels score blocks of query rows into a scratch block (at most 512 MB), and a
+// histogram pass folds each row into out[row][H + 2] (uint64; bin 0 below zero, 1 + s for
+// 0 <= s < H, H + 1 at H or above): the count of references within any cutoff below H and
+// every order statistic of the scores follow from it.
+void gapblock_histogram_device(const uint8_t* q, const uint64_t* q_off, uint64_t N,
+                               uint32_t q_max, const uint8_t* r, const uint64_t* r_off,
+                               uint64_t K, uint32_t r_max, const int32_t* pen_h, uint32_t A,
+                               int32_t go, int32_t ge, const int32_t* prior_h, uint32_t W,
+                               uint32_t H, uint64_t* out, hipStream_t s) {
+    if (H == 0 || uint64_t(H) + 2 > kMaxHistBins)
+        throw HipError{ST_INVALID_ARGUMENT, "seqtree_hip: histogram bins must be 1.." +
+                                                std::to_string(kMaxHistBins - 2)};
+    if (N == 0) return;
+    launch_fill_u32(reinterpret_cast<uint32_t*>(out), 2 * N * (uint64_t(H) + 2), 0u, s);
+    if (K == 0) return;
+    Arena ar(s);
+    if (H + 2 <= kHistFusedBins) {
+        // fused: the scoring kernels add each score to an LDS histogram of their query rows,
+        // no score matrix is written
+        GapblockPlan p = gapblock_plan(ar, q, q_off, N, q_max, r, r_off, K, r_max, pen_h, A, go,
+                                       ge, prior_h, W, true, s);
+        p.a.hist = reinterpret_cast<unsigned long long*>(out);
+        p.a.hist_bins = H;
+        gapblock_run(p, q_max, s);
+        launched("gapblock_histogram");
+        return;
+    }
+    // wider histograms: dense row blocks, then one reduction pass per block
+    const uint64_t blk = std::min<uint64_t>({N, std::max<uint64_t>(1, (uint64_t(512) << 20) / (4 * K)),
+                                             65535});
+    int32_t* dense = ar.get<int32_t>(blk * K);
+    for (uint64_t r0 = 0; r0 < N; r0 += blk) {
+        const uint64_t rows = std::min(blk, N - r0);
+        gapblock_device(q, q_off + r0, rows, q_max, r, r_off, K, r_max, pen_h, A, go, ge, prior_h,
+                        W, dense, s);
+        LaunchTimer t("score_histogram", s);
+        launch_score_histogram(dense, rows, K, H,
+                               reinterpret_cast<unsigned long long*>(out + r0 * (uint64_t(H) + 2)),
+                               s);
+        launched("score_histogram");
+    }
+}
+
 struct PrefilterImpl {
     float* P = nullptr;   // FP32 master parameters
     float* G = nullptr;   // gradients
@@ -1563,6 +1610,51 @@ int32_t st_gapblock_matrix(const uint8_t* q_codes, const uint64_t* q_offsets, ui
                                      rh.max_len, pen, alphabet_size, gap_open, gap_extend, prior,
                                      prior_width, dout, s);
                  });
+        return ok(status);
+    });
+}
+
+int32_t st_gapblock_histogram_device(const uint8_t* q_codes, const uint64_t* q_offsets,
+                                     uint64_t n_queries, uint32_t q_max_len,
+                                     const uint8_t* r_codes, const uint64_t* r_offsets,
+                                     uint64_t n_refs, uint32_t r_max_len, const int32_t* pen,
+                                     uint32_t alphabet_size, int32_t gap_open, int32_t gap_extend,
+                                     const int32_t* prior, uint32_t prior_width, uint32_t bins,
+                                     uint64_t* histogram, void* stream, st_status* status) {
+    if (alphabet_size == 0 || alphabet_size > kMaxAlphabet) return bad_alphabet(status, alphabet_size);
+    return guarded(status, [&] {
+        require_device();
+        gapblock_histogram_device(q_codes, q_offsets, n_queries, q_max_len, r_codes, r_offsets,
+                                  n_refs, r_max_len, pen, alphabet_size, gap_open, gap_extend,
+                                  prior, prior_width, bins, histogram,
+                                  static_cast<hipStream_t>(stream));
+        return ok(status);
+    });
+}
+
+int32_t st_gapblock_histogram(const uint8_t* q_codes, const uint64_t* q_offsets,
+                              uint64_t n_queries, const uint8_t* r_codes,
+                              const uint64_t* r_offsets, uint64_t n_refs, const int32_t* pen,
+                              uint32_t alphabet_size, int32_t gap_open, int32_t gap_extend,
+                              const int32_t* prior, uint32_t prior_width, uint32_t bins,
+                              uint64_t* histogram, st_status* status) {
+    if (alphabet_size == 0 || alphabet_size > kMaxAlphabet) return bad_alphabet(status, alphabet_size);
+    return guarded(status, [&] {
+        require_device();
+        if (n_queries == 0) return ok(status);
+        const HostSet qh = host_set(q_codes, q_offsets, n_queries);
+        const HostSet rh = host_set(r_codes, r_offsets, n_refs);
+        OwnedStream os;
+        const hipStream_t s = os.get();
+        Arena ar(s);
+        const DeviceSet qd = upload(ar, qh, s);
+        const DeviceSet rd = upload(ar, rh, s);
+        uint64_t* d = ar.get<uint64_t>(n_queries * (uint64_t(bins) + 2));
+        gapblock_histogram_device(qd.bytes, qd.off, n_queries, qh.max_len, rd.bytes, rd.off,
+                                  n_refs, rh.max_len, pen, alphabet_size, gap_open, gap_extend,
+                                  prior, prior_width, bins, d, s);
+        d2h(histogram, d, n_queries * (uint64_t(bins) + 2), s);
+        sync(s);
         return ok(status);
     });
 }

@@ -48,17 +48,51 @@ __device__ __forceinline__ void append_hit(bool hit, uint32_t qi, uint32_t k, in
     }
 }
 
+// Histogram output: the workgroup's LDS bins (kHistFusedBins per query row) count the scores
+// in [0, H); scores below zero and at or above H are counted in each lane's registers (at a
+// low ceiling nearly every score is above it, and LDS atomics on one address serialise across
+// the wave) and added once at the end, when the non-zero bins go to the global row.
+__device__ __forceinline__ void hist_add(uint32_t* bins, bool valid, int32_t score, uint32_t H,
+                                         uint32_t& below, uint32_t& above) {
+    if (!valid) return;
+    if (score < 0) ++below;
+    else if (uint32_t(score) >= H) ++above;
+    else atomicAdd(&bins[uint32_t(score) + 1], 1u);
+}
+
+// After a barrier that follows every hist_add and the register counts' atomics.
+__device__ __forceinline__ void hist_flush(const uint32_t* bins, uint32_t H,
+                                           unsigned long long* row, uint32_t tid) {
+    for (uint32_t b = tid; b < H + 2; b += kThreads)
+        if (bins[b]) atomicAdd(&row[b], (unsigned long long)bins[b]);
+}
+
+__device__ __forceinline__ void hist_tail(uint32_t* bins, uint32_t H, uint32_t below,
+                                          uint32_t above) {
+    if (below) atomicAdd(&bins[0], below);
+    if (above) atomicAdd(&bins[H + 1], above);
+}
+
+// OUT: kOutDense writes the N x K matrix, kOutHits appends hits under the cutoff, kOutHist
+// adds each query row's scores to its histogram.
+constexpr int kOutDense = 0, kOutHits = 1, kOutHist = 2;
+
+template <int OUT>
+constexpr uint32_t stage_ints(uint32_t rows) {
+    return OUT == kOutDense ? rows * kChunk : OUT == kOutHist ? rows * kHistFusedBins : 1u;
+}
+
 // SHORT: the short-launch layout (requires PROFILE && !EXACT): profile rows, then the
 // prior slice, both in the dynamic LDS.
 // A workgroup owns query i and chunks [g * group, min((g + 1) * group, nchunks)): the
 // query's tables are built once and reused across the group's chunks. Workgroups are
 // numbered group-major (all queries of group 0 first), so the workgroups resident at one
 // time stream the same few reference chunks and find them in L2.
-template <bool PROFILE, bool EXACT, bool THRESH, bool SHORT>
+template <bool PROFILE, bool EXACT, int OUT, bool SHORT>
 __global__ __launch_bounds__(kThreads) void gapblock_kernel(GapblockArgs a, uint64_t q_base,
                                                             uint32_t nq) {
     extern __shared__ int32_t lds[];
-    __shared__ int32_t stage[THRESH ? 1 : kChunk];
+    __shared__ int32_t stage[stage_ints<OUT>(1)];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t nch = a.refs.nchunks;
@@ -81,6 +115,10 @@ __global__ __launch_bounds__(kThreads) void gapblock_kernel(GapblockArgs a, uint
     } else {
         gb::build_rows(lds, A, A, a.pen, A, [](uint32_t r) { return r; }, tid, kThreads);
     }
+    uint32_t* bins = reinterpret_cast<uint32_t*>(stage);
+    uint32_t below = 0, above = 0;
+    if constexpr (OUT == kOutHist)
+        for (uint32_t b = tid; b < a.hist_bins + 2; b += kThreads) bins[b] = 0;
     __syncthreads();
 
     const Look<PROFILE> look{lds, q};
@@ -121,13 +159,15 @@ __global__ __launch_bounds__(kThreads) void gapblock_kernel(GapblockArgs a, uint
                                                        a.gap_open, a.gap_extend);
                 }
             }
-            if constexpr (THRESH) {
+            if constexpr (OUT == kOutHits) {
                 append_hit(k != kNoSlot && score <= a.hits.cutoff, uint32_t(i), k, score, a.hits);
+            } else if constexpr (OUT == kOutHist) {
+                hist_add(bins, k != kNoSlot, score, a.hist_bins, below, above);
             } else {
                 if (k != kNoSlot) stage[k - c * kChunk] = score;
             }
         }
-        if constexpr (!THRESH) {
+        if constexpr (OUT == kOutDense) {
             __syncthreads();
             const uint64_t col0 = uint64_t(c) * kChunk;
             const uint64_t lim64 = a.n_refs - col0;
@@ -137,6 +177,11 @@ __global__ __launch_bounds__(kThreads) void gapblock_kernel(GapblockArgs a, uint
             __syncthreads();  // the stage is rewritten by the next chunk
         }
     }
+    if constexpr (OUT == kOutHist) {
+        hist_tail(bins, a.hist_bins, below, above);
+        __syncthreads();
+        hist_flush(bins, a.hist_bins, a.hist + i * (a.hist_bins + 2), tid);
+    }
 }
 
 // Twin short launch: the short layout with two queries of one length per workgroup, packed
@@ -144,12 +189,12 @@ __global__ __launch_bounds__(kThreads) void gapblock_kernel(GapblockArgs a, uint
 // Workgroup = (twin p, chunk group g), group-major like gapblock_kernel; twins[p] = (a, b)
 // with b == a for the last query of a length with an odd count (one row written), and
 // (kNoSlot, kNoSlot) past the last twin (the workgroup exits before any barrier).
-template <bool THRESH>
+template <int OUT>
 __global__ __launch_bounds__(kThreads) void gapblock_twin_kernel(GapblockArgs a,
                                                                  const uint2* __restrict__ twins,
                                                                  uint32_t nt) {
     extern __shared__ int32_t lds[];
-    __shared__ int32_t stage[THRESH ? 1 : 2 * kChunk];
+    __shared__ int32_t stage[stage_ints<OUT>(2)];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t nch = a.refs.nchunks;
@@ -165,6 +210,12 @@ __global__ __launch_bounds__(kThreads) void gapblock_twin_kernel(GapblockArgs a,
     gb::build_rows2(prof, a.prof_rows, m, a.pen, a.A, a.q_codes + qa0, a.q_codes + qb0, tid,
                     kThreads);
     gb::build_slice2(slice, a.slice_rows - 1, m, a.prior, a.w1, a.big, tid, kThreads);
+    // histogram rows of a and b (b's unused when the twin is one query)
+    uint32_t* bins_a = reinterpret_cast<uint32_t*>(stage);
+    uint32_t* bins_b = bins_a + kHistFusedBins;
+    uint32_t below_a = 0, above_a = 0, below_b = 0, above_b = 0;
+    if constexpr (OUT == kOutHist)
+        for (uint32_t b = tid; b < 2 * kHistFusedBins; b += kThreads) bins_a[b] = 0;
     __syncthreads();
 
     uint32_t next_k = 0, next_n = 0;
@@ -189,16 +240,19 @@ __global__ __launch_bounds__(kThreads) void gapblock_twin_kernel(GapblockArgs a,
             gb::twin_score_short(prof, slice,
                                  gb::buffer_words(a.refs.words, a.refs.npad, a.refs.nwords, slot),
                                  k, m, n, a.gap_open, a.gap_extend, sa, sb);
-            if constexpr (THRESH) {
+            if constexpr (OUT == kOutHits) {
                 append_hit(k != kNoSlot && sa <= a.hits.cutoff, uint32_t(ia), k, sa, a.hits);
                 if (ib != ia)
                     append_hit(k != kNoSlot && sb <= a.hits.cutoff, uint32_t(ib), k, sb, a.hits);
+            } else if constexpr (OUT == kOutHist) {
+                hist_add(bins_a, k != kNoSlot, sa, a.hist_bins, below_a, above_a);
+                hist_add(bins_b, k != kNoSlot && ib != ia, sb, a.hist_bins, below_b, above_b);
             } else if (k != kNoSlot) {
                 stage[k - c * kChunk] = sa;
                 stage[kChunk + k - c * kChunk] = sb;
             }
         }
-        if constexpr (!THRESH) {
+        if constexpr (OUT == kOutDense) {
             __syncthreads();
             const uint64_t col0 = uint64_t(c) * kChunk;
             const uint64_t lim64 = a.n_refs - col0;
@@ -211,6 +265,13 @@ __global__ __launch_bounds__(kThreads) void gapblock_twin_kernel(GapblockArgs a,
             }
             __syncthreads();  // the stage is rewritten by the next chunk
         }
+    }
+    if constexpr (OUT == kOutHist) {
+        hist_tail(bins_a, a.hist_bins, below_a, above_a);
+        hist_tail(bins_b, a.hist_bins, below_b, above_b);
+        __syncthreads();
+        hist_flush(bins_a, a.hist_bins, a.hist + ia * (a.hist_bins + 2), tid);
+        if (ib != ia) hist_flush(bins_b, a.hist_bins, a.hist + ib * (a.hist_bins + 2), tid);
     }
 }
 
@@ -235,29 +296,29 @@ __global__ __launch_bounds__(kThreads) void gapblock_pairs_kernel(
     }
 }
 
-template <bool PROFILE, bool EXACT, bool THRESH, bool SHORT>
+template <bool PROFILE, bool EXACT, int OUT, bool SHORT>
 void launch_variant(const GapblockArgs& a, size_t lds_bytes, hipStream_t s) {
     const uint64_t ngroups = (a.refs.nchunks + a.group - 1) / a.group;
     const uint64_t max_q = (uint64_t(1) << 30) / ngroups;  // launches under 2^30 workgroups
     for (uint64_t q0 = 0; q0 < a.n_queries; q0 += max_q) {
         const uint64_t nq = a.n_queries - q0 < max_q ? a.n_queries - q0 : max_q;
-        hipLaunchKernelGGL((gapblock_kernel<PROFILE, EXACT, THRESH, SHORT>),
+        hipLaunchKernelGGL((gapblock_kernel<PROFILE, EXACT, OUT, SHORT>),
                            dim3(uint32_t(nq * ngroups)), dim3(kThreads), lds_bytes, s, a, q0,
                            uint32_t(nq));
     }
 }
 
-template <bool THRESH>
+template <int OUT>
 void launch_mode(const GapblockArgs& a, bool use_profile, size_t lds_bytes, bool exact_wrap,
                  bool short_path, hipStream_t s) {
     if (short_path) {
-        launch_variant<true, false, THRESH, true>(a, lds_bytes, s);
+        launch_variant<true, false, OUT, true>(a, lds_bytes, s);
     } else if (use_profile) {
-        if (exact_wrap) launch_variant<true, true, THRESH, false>(a, lds_bytes, s);
-        else launch_variant<true, false, THRESH, false>(a, lds_bytes, s);
+        if (exact_wrap) launch_variant<true, true, OUT, false>(a, lds_bytes, s);
+        else launch_variant<true, false, OUT, false>(a, lds_bytes, s);
     } else {
-        if (exact_wrap) launch_variant<false, true, THRESH, false>(a, lds_bytes, s);
-        else launch_variant<false, false, THRESH, false>(a, lds_bytes, s);
+        if (exact_wrap) launch_variant<false, true, OUT, false>(a, lds_bytes, s);
+        else launch_variant<false, false, OUT, false>(a, lds_bytes, s);
     }
 }
 
@@ -273,8 +334,9 @@ void launch_gapblock(const GapblockArgs& a, bool use_profile, uint32_t q_max_len
                          size_t(a.slice_rows) * gb::kSliceStride * sizeof(int32_t)
                    : size_t(use_profile ? q_max_len : a.A) * gb::kRowBytes;
     const size_t lds_bytes = lds ? lds : sizeof(int32_t);
-    if (a.hits.count) launch_mode<true>(a, use_profile, lds_bytes, exact_wrap, short_path, s);
-    else launch_mode<false>(a, use_profile, lds_bytes, exact_wrap, short_path, s);
+    if (a.hist) launch_mode<kOutHist>(a, use_profile, lds_bytes, exact_wrap, short_path, s);
+    else if (a.hits.count) launch_mode<kOutHits>(a, use_profile, lds_bytes, exact_wrap, short_path, s);
+    else launch_mode<kOutDense>(a, use_profile, lds_bytes, exact_wrap, short_path, s);
 }
 
 void launch_gapblock_twin(const GapblockArgs& a, const uint2* twins, uint64_t n_twins,
@@ -287,11 +349,14 @@ void launch_gapblock_twin(const GapblockArgs& a, const uint2* twins, uint64_t n_
     for (uint64_t t0 = 0; t0 < n_twins; t0 += max_t) {
         const uint32_t nt = uint32_t(n_twins - t0 < max_t ? n_twins - t0 : max_t);
         const dim3 grid(uint32_t(nt * ngroups));
-        if (a.hits.count)
-            hipLaunchKernelGGL(gapblock_twin_kernel<true>, grid, dim3(kThreads), lds, s, a,
+        if (a.hist)
+            hipLaunchKernelGGL(gapblock_twin_kernel<kOutHist>, grid, dim3(kThreads), lds, s, a,
+                               twins + t0, nt);
+        else if (a.hits.count)
+            hipLaunchKernelGGL(gapblock_twin_kernel<kOutHits>, grid, dim3(kThreads), lds, s, a,
                                twins + t0, nt);
         else
-            hipLaunchKernelGGL(gapblock_twin_kernel<false>, grid, dim3(kThreads), lds, s, a,
+            hipLaunchKernelGGL(gapblock_twin_kernel<kOutDense>, grid, dim3(kThreads), lds, s, a,
                                twins + t0, nt);
     }
 }

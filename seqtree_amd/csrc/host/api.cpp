@@ -347,6 +347,26 @@ std::vector<int32_t> gapblock_pairs(const std::vector<std::string>& queries,
     return out;
 }
 
+std::vector<uint64_t> gapblock_histogram(const std::vector<std::string>& queries,
+                                         const std::vector<std::string>& refs, Alphabet alphabet,
+                                         const SubstitutionMatrix* matrix, int32_t gap_open,
+                                         int32_t gap_extend, const std::vector<int32_t>& prior,
+                                         uint32_t prior_width, uint32_t bins) {
+    if (bins < 1 || bins > 16382) throw std::invalid_argument("bins must be in [1, 16382]");
+    const GapblockInput in = gapblock_input(queries, refs, alphabet, matrix, gap_open, gap_extend,
+                                            prior, prior_width);
+    std::vector<uint64_t> out(queries.size() * (size_t(bins) + 2), 0);
+    if (in.empty) return out;  // no queries or no references
+    st_status st{};
+    raise(st_gapblock_histogram(in.q.bytes.data(), in.q.off.data(), queries.size(),
+                                in.r.bytes.data(), in.r.off.data(), refs.size(), in.pen.data(),
+                                in.A, gap_open, gap_extend,
+                                prior.empty() ? nullptr : prior.data(), prior_width, bins,
+                                out.data(), &st),
+          st);
+    return out;
+}
+
 std::vector<int32_t> gapblock_matrix(const std::vector<std::string>& queries,
                                      const std::vector<std::string>& refs, Alphabet alphabet,
                                      const SubstitutionMatrix* matrix, int32_t gap_open,

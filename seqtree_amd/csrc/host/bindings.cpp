@@ -296,6 +296,29 @@ PYBIND11_MODULE(_core, m) {
         "ref index, score), sorted by (query, ref). GIL released.");
 
     m.def(
+        "gapblock_histogram",
+        [](const std::vector<std::string>& queries, const std::vector<std::string>& refs,
+           uint32_t bins, const std::string& alphabet,
+           const std::optional<SubstitutionMatrix>& matrix, int32_t gap_open, int32_t gap_extend,
+           const std::vector<int32_t>& prior, uint32_t prior_width) {
+            const Alphabet a = parse_alphabet(alphabet);
+            std::vector<uint64_t> v;
+            {
+                py::gil_scoped_release release;
+                v = gapblock_histogram(queries, refs, a, matrix ? &*matrix : nullptr, gap_open,
+                                       gap_extend, prior, prior_width, bins);
+            }
+            py::array_t<uint64_t> out({py::ssize_t(queries.size()), py::ssize_t(bins) + 2});
+            std::copy(v.begin(), v.end(), out.mutable_data());
+            return out;
+        },
+        py::arg("queries"), py::arg("refs"), py::arg("bins"), py::arg("alphabet") = "aa",
+        py::arg("matrix") = std::nullopt, py::arg("gap_open") = 1, py::arg("gap_extend") = 1,
+        py::arg("prior") = std::vector<int32_t>{}, py::arg("prior_width") = 0,
+        "Per-query histograms of the exact gap-block scores, uint64 [queries][bins + 2]: bin 0 "
+        "scores < 0, bin 1 + s the score s < bins, the last bin scores >= bins. GIL released.");
+
+    m.def(
         "gapblock_pairs",
         [](const std::vector<std::string>& queries, const std::vector<std::string>& refs,
            const std::vector<uint32_t>& pair_query, const std::vector<uint32_t>& pair_ref,

@@ -125,6 +125,15 @@ GapblockHits gapblock_threshold(const std::vector<std::string>& queries,
                                 int32_t gap_extend, const std::vector<int32_t>& prior,
                                 uint32_t prior_width, int32_t cutoff);
 
+// Per-query histograms of the exact gap-block scores against every reference, row-major
+// [queries.size()][bins + 2]: bin 0 counts scores below zero, bin 1 + s the score s in
+// [0, bins), bin bins + 1 scores >= bins (the counts behind control-calibrated E-values).
+std::vector<uint64_t> gapblock_histogram(const std::vector<std::string>& queries,
+                                         const std::vector<std::string>& refs, Alphabet,
+                                         const SubstitutionMatrix* matrix, int32_t gap_open,
+                                         int32_t gap_extend, const std::vector<int32_t>& prior,
+                                         uint32_t prior_width, uint32_t bins);
+
 // Gap-block scores of explicit (query index, ref index) pairs.
 std::vector<int32_t> gapblock_pairs(const std::vector<std::string>& queries,
                                     const std::vector<std::string>& refs, Alphabet,

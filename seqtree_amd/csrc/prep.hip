@@ -231,6 +231,39 @@ __global__ void classify_kernel(const uint64_t* __restrict__ off, uint64_t n, Cl
     lists[uint64_t(c) * n + pos] = uint32_t(i);
 }
 
+// Per-row histograms of an int32 matrix block: workgroup (row, segment) counts its segment's
+// values in LDS -- bin 0 below zero, bin 1 + v for 0 <= v < H, bin H + 1 at H or above -- and
+// adds the non-zero bins to the row's global histogram.
+constexpr uint32_t kHistThreads = 256;
+constexpr uint32_t kHistSegment = 1u << 16;
+
+__global__ __launch_bounds__(kHistThreads) void score_histogram_kernel(
+    const int32_t* __restrict__ m, uint64_t cols, uint32_t H, unsigned long long* __restrict__ out) {
+    extern __shared__ uint32_t bins[];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t row = blockIdx.y;
+    const uint64_t c0 = uint64_t(blockIdx.x) * kHistSegment;
+    const uint64_t c1 = c0 + kHistSegment < cols ? c0 + kHistSegment : cols;
+    for (uint32_t b = tid; b < H + 2; b += kHistThreads) bins[b] = 0;
+    __syncthreads();
+    const int32_t* v = m + row * cols;
+    // scores outside [0, H) are counted in registers: at a low ceiling nearly every score is
+    // above it, and LDS atomics on one address serialise across the wave
+    uint32_t below = 0, above = 0;
+    for (uint64_t c = c0 + tid; c < c1; c += kHistThreads) {
+        const int32_t x = v[c];
+        if (x < 0) ++below;
+        else if (uint32_t(x) >= H) ++above;
+        else atomicAdd(&bins[uint32_t(x) + 1], 1u);
+    }
+    if (below) atomicAdd(&bins[0], below);
+    if (above) atomicAdd(&bins[H + 1], above);
+    __syncthreads();
+    unsigned long long* o = out + row * (H + 2);
+    for (uint32_t b = tid; b < H + 2; b += kHistThreads)
+        if (bins[b]) atomicAdd(&o[b], (unsigned long long)bins[b]);
+}
+
 __global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -252,6 +285,14 @@ void launch_classify(const uint64_t* off, uint64_t n, const ClassBounds& b, uint
     if (n == 0) return;
     const uint32_t blocks = uint32_t((n + 255) / 256);
     hipLaunchKernelGGL(classify_kernel, dim3(blocks), dim3(256), 0, s, off, n, b, counts, lists);
+}
+
+void launch_score_histogram(const int32_t* m, uint64_t rows, uint64_t cols, uint32_t H,
+                            unsigned long long* out, hipStream_t s) {
+    if (rows == 0 || cols == 0) return;
+    const dim3 grid(uint32_t((cols + kHistSegment - 1) / kHistSegment), uint32_t(rows));
+    hipLaunchKernelGGL(score_histogram_kernel, grid, dim3(kHistThreads),
+                       size_t(H + 2) * sizeof(uint32_t), s, m, cols, H, out);
 }
 
 void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {

@@ -25,6 +25,7 @@ namespace st {
 constexpr uint32_t kChunk = 1024;        // reference slots per scoring workgroup
 constexpr uint32_t kThreads = 256;       // threads per scoring workgroup (4 wave64s)
 constexpr uint32_t kSlotsPerThread = kChunk / kThreads;
+constexpr uint32_t kHistFusedBins = 1024;  // LDS histogram bins per query row of a workgroup
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 constexpr uint32_t kMaxAlphabet = 32;    // codec sizes are <= 32 (seqtree.hpp:15-16)
 
@@ -64,6 +65,13 @@ void launch_classify(const uint64_t* off, uint64_t n, const ClassBounds& b, uint
                      uint32_t* lists, hipStream_t s);
 
 void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+
+// Adds each row of the rows x cols int32 matrix m to its histogram out[row][H + 2]: bin 0
+// counts values below zero, bin 1 + v the value v in [0, H), bin H + 1 values >= H.
+// H + 2 <= kMaxHistBins. rows < 65536 per launch.
+constexpr uint32_t kMaxHistBins = 16384;
+void launch_score_histogram(const int32_t* m, uint64_t rows, uint64_t cols, uint32_t H,
+                            unsigned long long* out, hipStream_t s);
 
 // Twin list of a query set whose lengths are at most kTwinMaxLen (built by a packing launch
 // given a TwinJob): the queries grouped by length and paired inside each length, a length
@@ -174,6 +182,11 @@ struct GapblockArgs {
     uint32_t w1;           // prior_width + 1
     int32_t* out;          // dense N x K output (or one score per pair in pair-list mode)
     GapblockHits hits;     // hits.count != nullptr selects the thresholded output
+    // hist != nullptr selects the histogram output (before hits and the dense matrix):
+    // hist[i][hist_bins + 2] += bin 0 scores < 0, bin 1 + s scores s < hist_bins, bin
+    // hist_bins + 1 the rest; hist_bins + 2 <= kHistFusedBins
+    unsigned long long* hist;
+    uint32_t hist_bins;
     // short launches (queries <= 32, references <= 64, A <= 31): profile rows per workgroup
     // (query length + 32 zero rows), prior-slice rows (reference lengths 0..r_max) and the
     // slice's past-the-end value

@@ -3,6 +3,7 @@ from __future__ import annotations
 
 import random
 import sys
+import zlib
 from pathlib import Path
 
 import pytest
@@ -47,6 +48,11 @@ def gpu(st):
     if st.device_count() < 1:
         pytest.fail("no HIP device visible for a gpu-marked test")
     return st
+
+
+def seed_of(*key) -> int:
+    """A seed fixed by the test's parameters (``hash`` of a string changes per process)."""
+    return zlib.crc32(repr(key).encode()) & 0xFFFF
 
 
 def rand_seq(rng: random.Random, lo: int, hi: int, alphabet: str = AA) -> str:

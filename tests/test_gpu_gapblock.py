@@ -7,7 +7,7 @@ import random
 import numpy as np
 import pytest
 
-from conftest import AA, cdr3_like, rand_seq
+from conftest import AA, cdr3_like, rand_seq, seed_of
 
 pytestmark = pytest.mark.gpu
 
@@ -68,7 +68,7 @@ def test_matrix_equals_scalar_scorer_and_oracle(gpu, oracle_lib, name):
 @pytest.mark.parametrize("go,ge", [(1, 1), (28, 1), (0, 3), (2, 5), (0, 0)])
 def test_random_ragged_sets(gpu, oracle_lib, matrix, go, ge):
     st = gpu
-    rng = random.Random(hash((matrix, go, ge)) & 0xFFFF)
+    rng = random.Random(seed_of(matrix, go, ge))
     q = [rand_seq(rng, 0, 33) for _ in range(37)]
     r = [rand_seq(rng, 0, 33) for _ in range(2100)]  # three chunks, the last one partial
     mat = getattr(st.SubstitutionMatrix, matrix)() if matrix else None

@@ -8,7 +8,7 @@ import random
 import numpy as np
 import pytest
 
-from conftest import cdr3_like, rand_seq
+from conftest import cdr3_like, rand_seq, seed_of
 
 pytestmark = pytest.mark.gpu
 
@@ -49,7 +49,7 @@ def _check(st, oracle_lib, q, r, m, mode, go, ge, dist=False, alphabet="aa", sim
 def test_settings_grid(gpu, oracle_lib, name, mode, go, ge):
     st = gpu
     m = getattr(st.SubstitutionMatrix, name)()
-    rng = random.Random(hash((name, mode, go, ge)) & 0xFFFF)
+    rng = random.Random(seed_of(name, mode, go, ge))
     q = [rand_seq(rng, 0, 26) for _ in range(23)] + ["A", "W" * 3]
     r = [rand_seq(rng, 0, 26) for _ in range(1100)] + ["", "KKAAA"]
     _check(st, oracle_lib, q, r, m, mode, go, ge)
@@ -60,7 +60,7 @@ def test_settings_grid(gpu, oracle_lib, name, mode, go, ge):
 def test_distance_matrices(gpu, oracle_lib, name, mode):
     st = gpu
     m = getattr(st.SubstitutionMatrix, name)()
-    rng = random.Random(hash((name, mode)) & 0xFFFF)
+    rng = random.Random(seed_of(name, mode))
     seqs = cdr3_like(rng, 40, 8, 25)
     d = _check(st, oracle_lib, seqs, seqs, m, mode, 11, 1, dist=True)
     assert (np.diag(d) == 0).all() and (d >= 0).all() and (d == d.T).all()
