@@ -23,6 +23,8 @@ st_align_matrix_device on one HIP stream.
                drawn from two 20,000-string CDR3-like sets, BLOSUM62 11/1, global and local
   counts       per-query histograms of exact gap-block scores, 1,000 queries x 250,000
                control (the control pass of threshold_for_evalue), ceiling 64
+  germline     4,000 x 4,000 Gotoh dist_matrix of germline-V-length strings (82..98),
+               BLOSUM62 11/1, global and local (the multi-stripe register kernel)
   prefilter    learned prefilter, 50,000 queries x 1,000,000 references, gap-block cutoff 48,
                against the exhaustive exact thresholded pass; a step is one search against
                reference embeddings built once (their build time and the one-shot pipeline
@@ -76,6 +78,59 @@ K_MIN = {"global": 8.0, "local": 10.0}
 # instructions (v_add_u32, v_max_i16); three-operand, packed and SGPR forms issue at half that
 # rate, so a kernel built from them tops out near 50% of this peak.
 VALU_PEAK_TOPS = 157.3 / 2
+
+
+GERMLINE_N, GERMLINE_LEN = 4000, (82, 98)
+
+
+def run_germline(torch, dev, lib, stream, args) -> dict:
+    """Germline-V-length all-vs-all Gotoh distances (the reference benchmark's 'germline' shape,
+    bench/bench_pairwise.py: lengths 90 +- 8, residues uniform over the 20 amino acids,
+    BLOSUM62 11/1): 4,000 x 4,000, global and local dist_matrix. Every query is longer than
+    one register stripe, so the multi-stripe register kernel carries the work."""
+    rng = np.random.default_rng(SEED + 11)
+    aa = np.array([AA_ORDER.index(c) for c in BG_RESIDUES], np.uint8)
+    lens = rng.integers(GERMLINE_LEN[0], GERMLINE_LEN[1] + 1, size=2 * GERMLINE_N)
+    seqs = [aa[rng.integers(0, 20, size=int(L))] for L in lens]
+    qs, rs = seqs[:GERMLINE_N], seqs[GERMLINE_N:]
+    Q, R = DeviceSet(torch, dev, qs), DeviceSet(torch, dev, rs)
+    sim, _, _ = matrix_tables("blosum62")
+    outs = {mode: torch.empty((Q.n, R.n), dtype=torch.int32, device=dev)
+            for mode in ("global", "local")}
+    st = Status()
+
+    def step():
+        for mode in ("global", "local"):
+            rc = lib.lib.st_align_matrix_device(
+                Q.bytes.data_ptr(), Q.off.data_ptr(), Q.n, Q.max_len, R.bytes.data_ptr(),
+                R.off.data_ptr(), R.n, R.max_len, sim.ctypes.data, len(AA_ORDER),
+                int(mode == "local"), 1, GAP_OPEN, GAP_EXTEND, outs[mode].data_ptr(),
+                ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st))
+            check(rc, st, "st_align_matrix_device")
+
+    ms_step, kernels = time_steps(torch, lib, stream, step, args.steps, args.warmup)
+    pairs_per_step = 2 * Q.n * R.n
+    cells_per_mode = float(Q.lens.sum()) * float(R.lens.sum())
+    label, launches, avg_s = dominant(kernels)
+    roofline = None
+    if "_stripes_" in label:
+        mode = label.rsplit("_", 1)[1]
+        roofline = valu_roofline(label, launches, avg_s, cells_per_mode * K_MIN[mode],
+                                 cells_per_launch=cells_per_mode, ops_per_cell=K_MIN[mode])
+    cpu = None
+    if args.cpu_rows > 0:
+        to_str = lambda v: "".join(AA_ORDER[c] for c in v)  # noqa: E731
+        cpu = cpu_reference("gotoh", [to_str(s) for s in qs], [to_str(s) for s in rs],
+                            max(1, args.cpu_rows // 10), seconds=args.cpu_seconds, sim=sim)
+    return base_line(
+        "all-vs-all pairs/s (Gotoh NW+SW dist_matrix, germline-V lengths)",
+        pairs_per_step / (ms_step / 1e3), args.steps, args.warmup, ms_step,
+        {"workload": "gotoh_dist_matrix_germline_4000x4000_len82-98_blosum62_go11_ge1_"
+                     "global+local",
+         "queries": Q.n, "refs": R.n, "lengths": list(GERMLINE_LEN), "seed": SEED + 11,
+         "pairs_per_step": pairs_per_step, "cells_per_step": 2 * cells_per_mode,
+         "parallelism": "single GPU"},
+        kernels, roofline, cpu, gcups=2 * cells_per_mode / (ms_step / 1e3) / 1e9)
 
 
 def cdr3_set(rng: np.random.Generator, n: int, lo: int, hi: int) -> list[np.ndarray]:
@@ -867,7 +922,7 @@ def main() -> int:
                     help="wall time the CPU baseline sample is repeated for")
     ap.add_argument("--workload", default="gotoh",
                     choices=["gotoh", "gapblock", "hamming", "levenshtein", "batch",
-                             "traceback", "counts", "prefilter"])
+                             "traceback", "counts", "germline", "prefilter"])
     ap.add_argument("--pf-epochs", type=int, default=PF_EPOCHS)
     ap.add_argument("--pf-cutoff", type=int, default=PF_CUTOFF)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -902,6 +957,8 @@ def main() -> int:
         line = run_traceback(torch, dev, lib, stream, args)
     elif args.workload == "counts":
         line = run_counts(torch, dev, lib, stream, args)
+    elif args.workload == "germline":
+        line = run_germline(torch, dev, lib, stream, args)
     else:
         line = run_distance(torch, dev, lib, stream, args, args.workload)
     if line.get("roofline") and line["roofline"]["bound"] == "valu":

@@ -656,8 +656,8 @@ void gotoh_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, uint32_t 
         LaunchTimer t("gotoh_self", s);
         if (gotoh_self_fits_lds(uint32_t(self_len),
                                 std::max({mx, iabs64(go), iabs64(ge)}))) {
-            launch_gotoh_self_lds(a, q, q_off, N, qs, s);
-            launch_gotoh_self_lds(a, r, r_off, K, rs, s);
+            launch_gotoh_self_lds(a, q, q_off, N, q_max, qs, s);
+            launch_gotoh_self_lds(a, r, r_off, K, r_max, rs, s);
         } else {
             const uint64_t stride = self_len + 1;
             const uint32_t slots = slots_for(stride);
@@ -670,12 +670,15 @@ void gotoh_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, uint32_t 
         a.r_self = rs;
     }
 
-    // Range proof for the register kernels: with R <= 64 stripe rows (dummies included)
-    // every reachable cell is a partial-alignment score bounded by B = (64 + r_max) * mx;
-    // B, the sentinel (-2^30 in int32, -2^14 in int16) and one gap step below either must
-    // stay inside the cell type. A distance adds at most 4 * (q_max + r_max) * mx, formed
-    // in int64 and stored as int32.
-    const int64_t B = (64 + int64_t(r_max)) * mx;
+    // Range proof for the register kernels: with R <= 64 stripe rows (dummies included), or
+    // the query swept in stripes of 64 rows up to its padded length q_pad, every reachable
+    // cell is a partial-alignment score bounded by B = (max(64, q_pad) + r_max) * mx; B, the
+    // sentinel (-2^30 in int32, -2^14 in int16) and one gap step below either must stay
+    // inside the cell type. A distance adds at most 4 * (q_max + r_max) * mx, formed in
+    // int64 and stored as int32.
+    const int64_t q_pad = std::max<int64_t>(
+        64, (int64_t(q_max) + kStripeRows - 1) / kStripeRows * kStripeRows);
+    const int64_t B = (q_pad + int64_t(r_max)) * mx;
     const int64_t G = std::max<int64_t>(go, ge);
     bool fast = B + 2 * G < (int64_t(1) << 30);
     // the packed cell uses the H form only, which needs gap_open >= gap_extend
@@ -710,7 +713,29 @@ void gotoh_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, uint32_t 
         }
     }
     const int gen_class = cb.count - 1;
-    if (counts[gen_class] > 0) {
+    if (fast && counts[gen_class] > 0) {
+        // queries past one register stripe: the multi-stripe register kernel
+        const uint32_t n_long = counts[gen_class];
+        const bool hform = packed16 || go >= ge;
+        const uint32_t W = std::max<uint32_t>(r_max, 1);
+        int dev = 0, cus = 256;
+        if (hipGetDevice(&dev) == hipSuccess)
+            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        const uint64_t units = gotoh_stripes_units(n_long, packed16, a.refs.nchunks);
+        const uint64_t per_wg = gotoh_stripes_words(1, W, hform) * sizeof(uint32_t);
+        const uint64_t by_mem = std::max<uint64_t>(1, (uint64_t(1) << 30) / per_wg);
+        const uint32_t grid =
+            uint32_t(std::min<uint64_t>({units, uint64_t(cus) * 8, by_mem}));
+        uint32_t* bnd = ar.get<uint32_t>(gotoh_stripes_words(grid, W, hform));
+        {
+            LaunchTimer t(std::string(packed16 ? "gotoh_i16x2_stripes" : "gotoh_i32_stripes") +
+                              (local ? "_local" : "_global"),
+                          s);
+            launch_gotoh_stripes(a, packed16, lists + size_t(gen_class) * N, n_long, grid, bnd,
+                                 W, s);
+        }
+        launched("gotoh_stripes");
+    } else if (counts[gen_class] > 0) {
         const uint64_t stride = shorter + 1;
         const uint32_t slots = slots_for(stride);
         int64_t* scratch = ar.get<int64_t>(size_t(slots) * 6 * stride);

@@ -210,6 +210,203 @@ __global__ __launch_bounds__(kThreads) void gotoh_reg_kernel(GotohArgs a,
     }
 }
 
+// Multi-stripe register Gotoh for queries longer than one register stripe (germline V genes
+// of ~90 residues and beyond): the same lane-per-reference sweep and the same cell update as
+// gotoh_reg_kernel, repeated over the query in stripes of R rows. Between stripes each lane
+// keeps the bottom row of its stripe -- per reference column the H (H - go in the H form) and
+// X values, and U in the U/V form -- in its own lane-interleaved slice of `bnd`, which the
+// next stripe's top row reads as "the row above" and overwrites in place, column by column.
+// The grid is persistent (units = query group x reference chunk, strided by gridDim.x) so
+// the boundary scratch is sized by the grid, not by the matrix; every workgroup owns
+// NB * W * kThreads words of it. A query's score is taken from the stripe holding its last
+// row; two packed queries of different lengths run to the longer one's last stripe.
+template <class C, int R, bool LOCAL, bool HFORM>
+__global__ __launch_bounds__(kThreads) void gotoh_stripes_kernel(
+    GotohArgs a, const uint32_t* __restrict__ list, uint32_t count, uint64_t n_units,
+    uint32_t* __restrict__ bnd_all, uint32_t W) {
+    using T = typename C::T;
+    constexpr int Q = C::Q;
+    constexpr uint32_t NB = HFORM ? 2 : 3;  // boundary words per column: H, X (, U)
+    __shared__ uint32_t prof[R * 32];
+    __shared__ int32_t stage[Q][kChunk];
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nch = a.refs.nchunks;
+    uint32_t* bnd = bnd_all + uint64_t(blockIdx.x) * NB * W * kThreads + tid;
+    const uint32_t A = a.A;
+    const int32_t shift = HFORM ? a.gap_open : 0;  // profile holds s + go in the H form
+    const int32_t dummy = LOCAL ? C::kNeg : shift;  // rows past a query's end never win
+    const T go = C::splat(a.gap_open), ge = C::splat(a.gap_extend);
+    const T neg = C::splat(C::kNeg), zero = C::splat(0);
+    const uint32_t npad = a.refs.npad;
+
+    for (uint64_t u = blockIdx.x; u < n_units; u += gridDim.x) {
+        const uint32_t c = uint32_t(u % nch);
+        const uint64_t g = u / nch;
+        uint32_t qi[Q], m[Q];
+        bool valid[Q];
+        const uint8_t* q[Q];
+        uint32_t rows = 0;
+#pragma unroll
+        for (int h = 0; h < Q; ++h) {
+            const uint64_t idx = g * Q + h;
+            valid[h] = idx < count;
+            qi[h] = list[valid[h] ? idx : g * Q];
+            const uint64_t q0 = a.q_off[qi[h]];
+            m[h] = uint32_t(a.q_off[qi[h] + 1] - q0);
+            q[h] = a.q_codes + q0;
+            rows = m[h] > rows ? m[h] : rows;
+        }
+        const uint32_t S = rows == 0 ? 1 : (rows + R - 1) / R;
+
+        for (uint32_t it = 0; it < kSlotsPerThread; ++it) {
+            const uint32_t slot = c * kChunk + it * kThreads + tid;
+            const uint32_t k = a.refs.orig[slot];
+            const bool live = k != kNoSlot;
+            const uint32_t n = live ? a.refs.len[slot] : 0u;  // padding lanes sweep nothing
+            const uint32_t* wp = a.refs.words + slot;
+            int32_t res[Q];
+#pragma unroll
+            for (int h = 0; h < Q; ++h) res[h] = 0;
+            T best = zero;
+
+            for (uint32_t sidx = 0; sidx < S; ++sidx) {
+                const uint32_t i0 = sidx * R;  // this stripe holds DP rows i0 + 1 .. i0 + R
+                __syncthreads();  // every lane is past the previous stripe's profile
+                for (uint32_t e = tid; e < uint32_t(R) * 32; e += kThreads) {
+                    const uint32_t r = e >> 5, b = e & 31u, i = i0 + r;
+                    int32_t v[Q];
+#pragma unroll
+                    for (int h = 0; h < Q; ++h)
+                        v[h] = (i < m[h] && b < A) ? a.sim[uint32_t(q[h][i]) * A + b] + shift
+                                                   : dummy;
+                    prof[e] = C::pack(v);
+                }
+                __syncthreads();
+
+                // column 0: H[i][0] = -gap_run(i) (global) or 0 (local), Y = -inf, X = H
+                T V[HFORM ? 1 : R], Y[R], Hs[R];
+                {
+                    T gr = LOCAL ? zero
+                                 : C::splat(-(a.gap_open + int32_t(i0) * a.gap_extend));
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        Hs[r] = HFORM ? gr - go : gr;
+                        if constexpr (!HFORM) V[r] = gr;
+                        Y[r] = neg;
+                        if (!LOCAL) gr = gr - ge;
+                    }
+                }
+                // H[i0][j - 1] (minus go in the H form): the top row's diagonal, from j = 0
+                T hprev;
+                if (LOCAL || i0 == 0) {
+                    hprev = HFORM ? zero - go : zero;
+                } else {
+                    const T h0 = C::splat(-(a.gap_open + int32_t(i0 - 1) * a.gap_extend));
+                    hprev = HFORM ? h0 - go : h0;
+                }
+                T g0 = C::splat(-a.gap_open);  // -gap_run(j + 1) on row 0
+                // rows of this stripe any query of the workgroup reaches (the last stripe
+                // stops after them, in steps of 4 rows)
+                const uint32_t rl = __builtin_amdgcn_readfirstlane(
+                    rows - i0 < uint32_t(R) ? rows - i0 : uint32_t(R));
+                uint32_t word = 0;
+                for (uint32_t j = 0; j < n; ++j) {
+                    if ((j & 3u) == 0) word = wp[uint64_t(j >> 2) * npad];
+                    const uint32_t rc = (word >> (8 * (j & 3u))) & 0xFFu;
+                    const uint32_t* pcol = prof + rc;
+                    uint32_t* bj = bnd + uint64_t(j) * NB * kThreads;
+                    T up, hd, xup;  // H form: Hg_up ; U/V form: U_up
+                    if (i0 == 0) {
+                        xup = neg;
+                        if (LOCAL) {
+                            up = HFORM ? zero - go : zero;
+                            hd = up;
+                        } else {
+                            up = HFORM ? g0 - go : g0;
+                            hd = hprev;
+                            hprev = up;
+                            g0 = g0 - ge;
+                        }
+                    } else {
+                        const T hup = C::load(bj[0]);
+                        xup = C::load(bj[kThreads]);
+                        if constexpr (HFORM) up = hup;
+                        else up = C::load(bj[2 * kThreads]);
+                        hd = hprev;
+                        hprev = hup;
+                    }
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        if ((r & 3) == 0 && uint32_t(r) >= rl) break;  // uniform
+                        const T s = C::load(pcol[r * 32]);
+                        T mv = hd + s;
+                        if (LOCAL) mv = C::vmax(mv, zero);
+                        if constexpr (HFORM) {
+                            const T xv = C::vmax(up, xup - ge);
+                            const T yv = C::vmax(Hs[r], Y[r] - ge);
+                            hd = Hs[r];
+                            Hs[r] = C::vmax(C::vmax(mv, xv), yv) - go;
+                            up = Hs[r];
+                            xup = xv;
+                            Y[r] = yv;
+                        } else {
+                            const T xv = C::vmax(up - go, xup - ge);
+                            const T yv = C::vmax(V[r] - go, Y[r] - ge);
+                            hd = Hs[r];
+                            up = C::vmax(mv, yv);
+                            xup = xv;
+                            V[r] = C::vmax(mv, xv);
+                            Y[r] = yv;
+                            Hs[r] = C::vmax(up, xv);
+                        }
+                        if (LOCAL) best = C::vmax(best, mv);
+                    }
+                    if (sidx + 1 < S) {  // the bottom row (rl == R) for the next stripe's top row
+                        bj[0] = __builtin_bit_cast(uint32_t, Hs[R - 1]);
+                        bj[kThreads] = __builtin_bit_cast(uint32_t, xup);
+                        if constexpr (!HFORM) bj[2 * kThreads] = __builtin_bit_cast(uint32_t, up);
+                    }
+                }
+                if (!LOCAL) {
+#pragma unroll
+                    for (int h = 0; h < Q; ++h) {
+                        if (m[h] == 0 || (m[h] - 1) / R != sidx) continue;
+                        const uint32_t rl = (m[h] - 1) % R;
+                        int32_t v = C::half(Hs[0], h);
+#pragma unroll
+                        for (int r = 1; r < R; ++r)
+                            if (uint32_t(r) == rl) v = C::half(Hs[r], h);
+                        res[h] = HFORM ? v + a.gap_open : v;
+                    }
+                }
+            }
+
+#pragma unroll
+            for (int h = 0; h < Q; ++h) {
+                int32_t v = res[h];
+                if (LOCAL) v = C::half(best, h);
+                else if (m[h] == 0) v = n == 0 ? 0 : -(a.gap_open + int32_t(n - 1) * a.gap_extend);
+                if (live) {
+                    if (a.dist) v = int32_t(a.q_self[qi[h]] + a.r_self[k] - 2 * int64_t(v));
+                    stage[h][k - c * kChunk] = v;
+                }
+            }
+        }
+        __syncthreads();
+
+        const uint64_t col0 = uint64_t(c) * kChunk;
+        const uint64_t lim64 = a.n_refs - col0;
+        const uint32_t lim = lim64 < kChunk ? uint32_t(lim64) : kChunk;
+#pragma unroll
+        for (int h = 0; h < Q; ++h) {
+            if (!valid[h]) continue;
+            int32_t* row = a.out + uint64_t(qi[h]) * a.n_refs + col0;
+            for (uint32_t t = tid; t < lim; t += kThreads) row[t] = stage[h][t];
+        }
+    }
+}
+
 // Exact int64 restatement of the reference DP for one pair; buf holds 6 * (min(m,n)+1).
 __device__ int64_t gotoh_i64(const uint8_t* q, uint32_t m, const uint8_t* r, uint32_t n,
                              const int32_t* __restrict__ sim, const int32_t* __restrict__ simT,
@@ -310,64 +507,77 @@ __global__ __launch_bounds__(256) void gotoh_self_kernel(GotohArgs a,
 
 // Self-scores s(x, x) of strings of at most kSelfMaxLen residues: one thread per string, its
 // three DP rows (M, X, Y over the string, int32) in LDS, column-interleaved across the wave so
-// that lanes touching the same column hit distinct banks. Rows are updated in place: the cell
+// that lanes touching the same column hit distinct banks, with the similarity table and the
+// lane's own codes (also interleaved) in LDS beside them. Rows are updated in place: the cell
 // (i, j) reads row i - 1 at j from LDS and keeps row i - 1 at j - 1 (the diagonal) and row i at
 // j - 1 (the left neighbour) in registers. Same recurrence, boundary values and order of max as
 // gotoh_i64; the host runs it only when every value stays far inside int32 (kNeg32 and a few
-// gap steps below it included), so the results are equal.
-constexpr uint32_t kSelfMaxLen = 64;
+// gap steps below it included), so the results are equal. Dynamic LDS: A x A table, codes
+// [W][64] bytes, rows [3][W][64] int32 with W = longest string + 1.
+constexpr uint32_t kSelfMaxLen = 160;
 constexpr uint32_t kSelfThreads = 64;
+
+size_t self_lds_bytes(uint32_t W) {
+    return kMaxAlphabet * kMaxAlphabet * sizeof(int32_t) + ((size_t(W) * kSelfThreads + 15) & ~size_t(15)) +
+           3 * size_t(W) * kSelfThreads * sizeof(int32_t);
+}
 
 template <bool LOCAL>
 __global__ __launch_bounds__(kSelfThreads) void gotoh_self_lds_kernel(
     GotohArgs a, const uint8_t* __restrict__ codes, const uint64_t* __restrict__ off, uint64_t n,
-    int64_t* __restrict__ res) {
-    __shared__ int32_t rows[3][kSelfMaxLen + 1][kSelfThreads];
+    int64_t* __restrict__ res, uint32_t W) {
+    extern __shared__ int32_t self_lds[];
     constexpr int32_t kNeg32 = -(1 << 29);
     const uint32_t lane = threadIdx.x;
+    const uint32_t A = a.A;
+    int32_t* sim_l = self_lds;
+    for (uint32_t e = lane; e < A * A; e += kSelfThreads) sim_l[e] = a.sim[e];
+    __syncthreads();  // the only barrier
+    uint8_t* qc = reinterpret_cast<uint8_t*>(self_lds + kMaxAlphabet * kMaxAlphabet);
+    int32_t* rows = reinterpret_cast<int32_t*>(qc + ((W * kSelfThreads + 15) & ~15u));
     const uint64_t x = uint64_t(blockIdx.x) * kSelfThreads + lane;
-    if (x >= n) return;  // no barriers below
+    if (x >= n) return;
     const uint64_t o = off[x];
     const uint32_t L = uint32_t(off[x + 1] - o);
-    const uint8_t* q = codes + o;
     const int32_t go = a.gap_open, ge = a.gap_extend;
     if (L == 0) {
         res[x] = 0;
         return;
     }
-    int32_t (*M)[kSelfThreads] = rows[0];
-    int32_t (*X)[kSelfThreads] = rows[1];
-    int32_t (*Y)[kSelfThreads] = rows[2];
-    M[0][lane] = 0;
-    X[0][lane] = kNeg32;
-    Y[0][lane] = kNeg32;
+    for (uint32_t c = 0; c < L; ++c) qc[c * kSelfThreads + lane] = codes[o + c];
+    auto M = [&](uint32_t j) -> int32_t& { return rows[(0 * W + j) * kSelfThreads + lane]; };
+    auto X = [&](uint32_t j) -> int32_t& { return rows[(1 * W + j) * kSelfThreads + lane]; };
+    auto Y = [&](uint32_t j) -> int32_t& { return rows[(2 * W + j) * kSelfThreads + lane]; };
+    M(0) = 0;
+    X(0) = kNeg32;
+    Y(0) = kNeg32;
     for (uint32_t j = 1; j <= L; ++j) {
-        M[j][lane] = LOCAL ? 0 : kNeg32;
-        X[j][lane] = kNeg32;
-        Y[j][lane] = LOCAL ? kNeg32 : -(go + int32_t(j - 1) * ge);
+        M(j) = LOCAL ? 0 : kNeg32;
+        X(j) = kNeg32;
+        Y(j) = LOCAL ? kNeg32 : -(go + int32_t(j - 1) * ge);
     }
     int32_t best = LOCAL ? 0 : kNeg32;
     for (uint32_t i = 1; i <= L; ++i) {
         // column 0 of row i, and row i - 1 at column 0 as the first diagonal
-        int32_t dm = M[0][lane], dx = X[0][lane], dy = Y[0][lane];
+        int32_t dm = M(0), dx = X(0), dy = Y(0);
         int32_t lm = LOCAL ? 0 : kNeg32;
         int32_t lx = LOCAL ? kNeg32 : -(go + int32_t(i - 1) * ge);
         int32_t ly = kNeg32;
-        M[0][lane] = lm;
-        X[0][lane] = lx;
-        Y[0][lane] = ly;
-        const int32_t* srow = a.sim + uint32_t(q[i - 1]) * a.A;
+        M(0) = lm;
+        X(0) = lx;
+        Y(0) = ly;
+        const int32_t* srow = sim_l + uint32_t(qc[(i - 1) * kSelfThreads + lane]) * A;
         for (uint32_t j = 1; j <= L; ++j) {
-            const int32_t um = M[j][lane], ux = X[j][lane], uy = Y[j][lane];
-            const int32_t sv = srow[q[j - 1]];
+            const int32_t um = M(j), ux = X(j), uy = Y(j);
+            const int32_t sv = srow[qc[(j - 1) * kSelfThreads + lane]];
             int32_t mv = max(dm, max(dx, dy)) + sv;
             if (LOCAL && mv < 0) mv = 0;
             const int32_t xv = max(um - go, max(ux - ge, uy - go));
             const int32_t yv = max(lm - go, max(ly - ge, lx - go));
             if (LOCAL && mv > best) best = mv;
-            M[j][lane] = mv;
-            X[j][lane] = xv;
-            Y[j][lane] = yv;
+            M(j) = mv;
+            X(j) = xv;
+            Y(j) = yv;
             dm = um;
             dx = ux;
             dy = uy;
@@ -376,7 +586,7 @@ __global__ __launch_bounds__(kSelfThreads) void gotoh_self_lds_kernel(
             ly = yv;
         }
     }
-    res[x] = LOCAL ? best : max(M[L][lane], max(X[L][lane], Y[L][lane]));
+    res[x] = LOCAL ? best : max(M(L), max(X(L), Y(L)));
 }
 
 template <class C, int R, bool HFORM>
@@ -431,7 +641,39 @@ void launch_cell(const GotohArgs& a, int rows, const uint32_t* list, uint32_t co
     }
 }
 
+template <class C, bool HFORM>
+void launch_stripes_form(const GotohArgs& a, const uint32_t* list, uint32_t count,
+                         uint64_t n_units, uint32_t grid, uint32_t* bnd, uint32_t W,
+                         hipStream_t s) {
+    if (a.local)
+        hipLaunchKernelGGL((gotoh_stripes_kernel<C, kStripeRows, true, HFORM>), dim3(grid),
+                           dim3(kThreads), 0, s, a, list, count, n_units, bnd, W);
+    else
+        hipLaunchKernelGGL((gotoh_stripes_kernel<C, kStripeRows, false, HFORM>), dim3(grid),
+                           dim3(kThreads), 0, s, a, list, count, n_units, bnd, W);
+}
+
 }  // namespace
+
+uint64_t gotoh_stripes_units(uint32_t count, bool packed16, uint32_t nchunks) {
+    const uint64_t q = packed16 ? 2 : 1;
+    return (uint64_t(count) + q - 1) / q * nchunks;
+}
+
+uint64_t gotoh_stripes_words(uint32_t grid, uint32_t W, bool hform) {
+    return uint64_t(grid) * (hform ? 2 : 3) * W * kThreads;
+}
+
+void launch_gotoh_stripes(const GotohArgs& a, bool packed16, const uint32_t* list,
+                          uint32_t count, uint32_t grid, uint32_t* bnd, uint32_t W,
+                          hipStream_t s) {
+    if (count == 0 || a.refs.nchunks == 0 || grid == 0) return;
+    const uint64_t units = gotoh_stripes_units(count, packed16, a.refs.nchunks);
+    if (packed16) launch_stripes_form<I16x2, true>(a, list, count, units, grid, bnd, W, s);
+    else if (a.gap_open < a.gap_extend)
+        launch_stripes_form<I32, false>(a, list, count, units, grid, bnd, W, s);
+    else launch_stripes_form<I32, true>(a, list, count, units, grid, bnd, W, s);
+}
 
 void launch_gotoh_fast(const GotohArgs& a, int rows, bool packed16, const uint32_t* list,
                        uint32_t count, hipStream_t s) {
@@ -457,20 +699,22 @@ void launch_gotoh_self(const GotohArgs& a, const uint8_t* codes, const uint64_t*
 
 bool gotoh_self_fits_lds(uint32_t max_len, int64_t mx) {
     // every finite value is within (2 max_len + 2) * mx of 0; sentinel-derived values stay
-    // within kNeg32 - (2 max_len + 2) * mx; both far inside int32 for mx < 2^20
-    return max_len <= kSelfMaxLen && mx < (int64_t(1) << 20);
+    // within kNeg32 - (2 max_len + 2) * mx; both far inside int32 while that span is < 2^28
+    return max_len <= kSelfMaxLen && (2 * int64_t(max_len) + 2) * mx < (int64_t(1) << 28);
 }
 
 void launch_gotoh_self_lds(const GotohArgs& a, const uint8_t* codes, const uint64_t* off,
-                           uint64_t n, int64_t* res, hipStream_t s) {
+                           uint64_t n, uint32_t max_len, int64_t* res, hipStream_t s) {
     if (n == 0) return;
     const dim3 grid(uint32_t((n + kSelfThreads - 1) / kSelfThreads));
+    const uint32_t W = max_len + 1;
+    const size_t lds = self_lds_bytes(W);
     if (a.local)
-        hipLaunchKernelGGL(gotoh_self_lds_kernel<true>, grid, dim3(kSelfThreads), 0, s, a, codes,
-                           off, n, res);
+        hipLaunchKernelGGL(gotoh_self_lds_kernel<true>, grid, dim3(kSelfThreads), lds, s, a,
+                           codes, off, n, res, W);
     else
-        hipLaunchKernelGGL(gotoh_self_lds_kernel<false>, grid, dim3(kSelfThreads), 0, s, a, codes,
-                           off, n, res);
+        hipLaunchKernelGGL(gotoh_self_lds_kernel<false>, grid, dim3(kSelfThreads), lds, s, a,
+                           codes, off, n, res, W);
 }
 
 }  // namespace st

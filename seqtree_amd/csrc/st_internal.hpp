@@ -280,6 +280,16 @@ struct GotohArgs {
 // int16 halves (needs the int16 range proof), else one query in int32 (int32 proof).
 void launch_gotoh_fast(const GotohArgs& a, int rows, bool packed16, const uint32_t* list,
                        uint32_t count, hipStream_t s);
+// Register kernel over stripes of kStripeRows query rows, for queries longer than one stripe
+// (same range proofs with the query's padded length in the bound). Persistent grid of `grid`
+// workgroups over gotoh_stripes_units(...) units; bnd holds gotoh_stripes_words(grid, W,
+// gap_open >= gap_extend || packed16) words, W >= the longest reference.
+constexpr int kStripeRows = 32;
+uint64_t gotoh_stripes_units(uint32_t count, bool packed16, uint32_t nchunks);
+uint64_t gotoh_stripes_words(uint32_t grid, uint32_t W, bool hform);
+void launch_gotoh_stripes(const GotohArgs& a, bool packed16, const uint32_t* list,
+                          uint32_t count, uint32_t grid, uint32_t* bnd, uint32_t W,
+                          hipStream_t s);
 // int64 per-thread DP for arbitrary lengths and gap costs: every (list[t], k) pair, k < n_refs.
 // scratch holds `slots` regions of 6 * (stride) int64.
 void launch_gotoh_generic(const GotohArgs& a, const uint32_t* list, uint32_t count,
@@ -288,11 +298,12 @@ void launch_gotoh_generic(const GotohArgs& a, const uint32_t* list, uint32_t cou
 void launch_gotoh_self(const GotohArgs& a, const uint8_t* codes, const uint64_t* off, uint64_t n,
                        int64_t* res, int64_t* scratch, uint32_t slots, uint64_t stride,
                        hipStream_t s);
-// Self-scores with the DP rows in LDS (strings of at most 64 residues, |scores|, gap costs
-// < 2^20: gotoh_self_fits_lds); the same results as launch_gotoh_self.
+// Self-scores with the DP rows in LDS (strings of at most 160 residues and (2 max_len + 2) *
+// max(|s|, gap costs) < 2^28: gotoh_self_fits_lds); the same results as launch_gotoh_self.
+// max_len: the longest string of the set (sizes the LDS rows).
 bool gotoh_self_fits_lds(uint32_t max_len, int64_t mx);
 void launch_gotoh_self_lds(const GotohArgs& a, const uint8_t* codes, const uint64_t* off,
-                           uint64_t n, int64_t* res, hipStream_t s);
+                           uint64_t n, uint32_t max_len, int64_t* res, hipStream_t s);
 
 // ---- traceback.hip ---------------------------------------------------------------------
 // Gotoh alignment with traceback of (pair_q[p], pair_r[p]) for p < n_pairs: score[p] (int64),

@@ -61,3 +61,25 @@ def rand_seq(rng: random.Random, lo: int, hi: int, alphabet: str = AA) -> str:
 
 def cdr3_like(rng: random.Random, n: int, lo: int, hi: int) -> list[str]:
     return ["C" + rand_seq(rng, lo - 2, hi - 2) + "F" for _ in range(n)]
+
+
+def launch_labels(fn):
+    """Runs fn with the library's launch timing on; returns (fn's result, the set of launch
+    labels recorded meanwhile), which names the kernels that ran."""
+    import ctypes
+
+    from seqtree_amd import _device
+
+    L = _device.lib()
+    L.st_timing_enable.argtypes = [ctypes.c_int32]
+    L.st_timing_collect.argtypes = [ctypes.c_char_p, ctypes.c_uint64]
+    L.st_timing_collect.restype = ctypes.c_int32
+    L.st_timing_enable(1)
+    try:
+        out = fn()
+    finally:
+        buf = ctypes.create_string_buffer(1 << 16)
+        n = L.st_timing_collect(buf, len(buf))
+        L.st_timing_enable(0)
+    assert n >= 0
+    return out, {line.split()[0] for line in buf.value.decode().splitlines()}

@@ -9,7 +9,7 @@ import random
 import numpy as np
 import pytest
 
-from conftest import cdr3_like, rand_seq, seed_of
+from conftest import cdr3_like, launch_labels, rand_seq, seed_of
 
 pytestmark = pytest.mark.gpu
 
@@ -62,27 +62,6 @@ def test_histogram_equals_oracle_bincount(gpu, oracle_lib, case):
     assert (got.sum(axis=1) == len(r)).all()
 
 
-def _launch_labels(fn):
-    """Runs fn with the library's launch timing on; returns (fn's result, launch labels)."""
-    import ctypes
-
-    from seqtree_amd import _device
-
-    L = _device.lib()
-    L.st_timing_enable.argtypes = [ctypes.c_int32]
-    L.st_timing_collect.argtypes = [ctypes.c_char_p, ctypes.c_uint64]
-    L.st_timing_collect.restype = ctypes.c_int32
-    L.st_timing_enable(1)
-    try:
-        out = fn()
-    finally:
-        buf = ctypes.create_string_buffer(1 << 16)
-        n = L.st_timing_collect(buf, len(buf))
-        L.st_timing_enable(0)
-    assert n >= 0
-    return out, {line.split()[0] for line in buf.value.decode().splitlines()}
-
-
 def _variant_case(st, variant):
     rng = random.Random(seed_of(variant))
     blosum = st.SubstitutionMatrix.blosum62()
@@ -116,7 +95,7 @@ def test_fused_histogram_in_every_kernel_variant(gpu, oracle_lib, variant):
     q, r, m, go, ge, prior, H = _variant_case(st, variant)
     w = max(len(x) for x in q + r)
     cube = _prior_cube(prior, w) if prior is not None else []
-    got, labels = _launch_labels(
+    got, labels = launch_labels(
         lambda: _core.gapblock_histogram(q, r, H, "aa", m, go, ge, cube, w))
     label = {"twin": "gapblock_histogram_twin", "short": "gapblock_histogram_short",
              "profile": "gapblock_histogram_profile", "table": "gapblock_histogram_table",

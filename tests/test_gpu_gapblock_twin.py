@@ -4,13 +4,12 @@ result must equal the int32 short launch (single-query calls) and the CPU oracle
 at the edge of the int16 range proof, for odd counts per length and for empty queries."""
 from __future__ import annotations
 
-import ctypes
 import random
 
 import numpy as np
 import pytest
 
-from conftest import cdr3_like, rand_seq
+from conftest import cdr3_like, launch_labels, rand_seq
 
 pytestmark = pytest.mark.gpu
 
@@ -18,25 +17,6 @@ pytestmark = pytest.mark.gpu
 def _pen(st, matrix):
     syms = st.alphabet_symbols("aa")
     return np.array([[matrix.penalty(a, b) for b in syms] for a in syms], np.int32)
-
-
-def _launch_labels(fn):
-    """Runs fn with the library's launch timing on; returns (fn's result, launch labels)."""
-    from seqtree_amd import _device
-
-    L = _device.lib()
-    L.st_timing_enable.argtypes = [ctypes.c_int32]
-    L.st_timing_collect.argtypes = [ctypes.c_char_p, ctypes.c_uint64]
-    L.st_timing_collect.restype = ctypes.c_int32
-    L.st_timing_enable(1)
-    try:
-        out = fn()
-    finally:
-        buf = ctypes.create_string_buffer(1 << 16)
-        n = L.st_timing_collect(buf, len(buf))
-        L.st_timing_enable(0)
-    assert n >= 0
-    return out, {line.split()[0] for line in buf.value.decode().splitlines()}
 
 
 def _oracle_rows(st, oracle_lib, q, r, m, go, ge, prior):
@@ -60,7 +40,7 @@ def test_twin_rows_equal_single_rows_and_oracle(gpu, oracle_lib, prior):
     q = special + [rand_seq(rng, 0, 32) for _ in range(220)]
     rng.shuffle(q)
     r = [rand_seq(rng, 0, 64) for _ in range(3000)] + q[:5]
-    full, labels = _launch_labels(lambda: np.asarray(st.score_matrix(q, r, m, 28, 1, gp)))
+    full, labels = launch_labels(lambda: np.asarray(st.score_matrix(q, r, m, 28, 1, gp)))
     assert "gapblock_twin" in labels, labels
     special_rows = [i for i, s in enumerate(q) if s in special]
     rows = sorted(set(special_rows) | set(rng.sample(range(len(q)), 30)))
@@ -80,7 +60,7 @@ def test_all_lengths_distinct(gpu, oracle_lib):
     q = [rand_seq(rng, n, n) for n in range(33)]
     rng.shuffle(q)
     r = [rand_seq(rng, 0, 40) for _ in range(1500)]
-    full, labels = _launch_labels(lambda: np.asarray(st.score_matrix(q, r, m, 17, 2)))
+    full, labels = launch_labels(lambda: np.asarray(st.score_matrix(q, r, m, 17, 2)))
     assert "gapblock_twin" in labels
     assert np.array_equal(full, _oracle_rows(st, oracle_lib, q, r, m, 17, 2, None))
 
@@ -98,7 +78,7 @@ def test_int16_bound_edge(gpu, oracle_lib, over):
     R = 32766 - 6 * T + over
     assert R > 0
     gp = lambda i, d, n: R  # noqa: E731
-    full, labels = _launch_labels(lambda: np.asarray(st.score_matrix(q, r, m, 28, 1, gp)))
+    full, labels = launch_labels(lambda: np.asarray(st.score_matrix(q, r, m, 28, 1, gp)))
     assert ("gapblock_twin" in labels) == (over == 0), labels
     assert np.array_equal(full, _oracle_rows(st, oracle_lib, q, r, m, 28, 1, gp))
 
@@ -111,7 +91,7 @@ def test_twin_threshold_equals_dense(gpu):
     q = cdr3_like(rng, 301, 8, 20)
     r = cdr3_like(rng, 4000, 8, 20)
     gp = st.central_prior(21)
-    (hq, hr, hs), labels = _launch_labels(
+    (hq, hr, hs), labels = launch_labels(
         lambda: st.gapblock.score_threshold(q, r, 70, m, 28, 1, gp))
     assert "gapblock_threshold_twin" in labels, labels
     dense = np.concatenate([np.asarray(st.score_matrix(q[i:i + 1], r, m, 28, 1, gp))

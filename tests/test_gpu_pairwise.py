@@ -8,7 +8,7 @@ import random
 import numpy as np
 import pytest
 
-from conftest import cdr3_like, rand_seq, seed_of
+from conftest import cdr3_like, launch_labels, rand_seq, seed_of
 
 pytestmark = pytest.mark.gpu
 
@@ -69,16 +69,18 @@ def test_distance_matrices(gpu, oracle_lib, name, mode):
     _check(st, oracle_lib, q, r, m, mode, 11, 1, dist=True)
 
 
-def test_distance_self_scores_up_to_64_residues(gpu, oracle_lib):
-    """Sets of at most 64 residues take the LDS self-score kernel: lengths 0 and 50..64, an
-    asymmetric matrix and gap_extend > gap_open included."""
+@pytest.mark.parametrize("longest", [64, 160])
+def test_distance_self_scores_in_lds(gpu, oracle_lib, longest):
+    """Sets of at most 160 residues take the LDS self-score kernel (rows sized to the longest
+    string): lengths 0 and up to the limit, an asymmetric matrix and gap_extend > gap_open
+    included."""
     st = gpu
-    rng = random.Random(23)
+    rng = random.Random(23 + longest)
     grid = np.random.default_rng(23).integers(-7, 9, size=(24, 24)).astype(np.int32)
     for m, sim in ((st.SubstitutionMatrix.blosum62(), None),
                    (st.SubstitutionMatrix.from_similarity(grid.tolist()), grid)):
-        q = [rand_seq(rng, 50, 64) for _ in range(40)] + [""]
-        r = [rand_seq(rng, 0, 64) for _ in range(300)] + ["A" * 64]
+        q = [rand_seq(rng, longest - 14, longest) for _ in range(40)] + [""]
+        r = [rand_seq(rng, 0, longest) for _ in range(300)] + ["A" * longest]
         for mode in ("global", "local"):
             for go, ge in ((11, 1), (2, 5)):
                 _check(st, oracle_lib, q, r, m, mode, go, ge, dist=True, sim=sim)
@@ -129,8 +131,8 @@ def test_overflow_raises_with_the_right_quantity(gpu):
 
 
 def test_long_queries_and_references(gpu, oracle_lib):
-    """Queries past 64 residues run the int64 per-thread DP; long references stay on the
-    register kernel."""
+    """Queries past 64 residues run the multi-stripe register kernel; long references stay on
+    the register kernels."""
     st = gpu
     m = st.SubstitutionMatrix.blosum62()
     rng = random.Random(17)
@@ -139,6 +141,46 @@ def test_long_queries_and_references(gpu, oracle_lib):
     for mode in ("global", "local"):
         _check(st, oracle_lib, q, r, m, mode, 12, 1)
         _check(st, oracle_lib, q, r, m, mode, 12, 1, dist=True)
+
+
+STRIPE_CASES = [
+    # (gap_open, gap_extend, label): int16 pairs in the H form, int32 in the H form, int32 in
+    # the U/V form (gap_open < gap_extend), and int64 past the int32 proof
+    (11, 1, "gotoh_i16x2_stripes"), (1000, 3, "gotoh_i32_stripes"),
+    (1, 3, "gotoh_i32_stripes"), (10**7, 1, "gotoh_generic"),
+]
+
+
+@pytest.mark.parametrize("mode", ["global", "local"])
+@pytest.mark.parametrize("go,ge,label", STRIPE_CASES)
+def test_long_queries_run_in_stripes(gpu, oracle_lib, mode, go, ge, label):
+    """Queries past one register stripe (32 rows): lengths on and around the stripe edges,
+    germline-V-like lengths (~90) and a few long ones, mixed with short queries in one call,
+    against references up to 260 residues; scores and distances equal the oracle's."""
+    st = gpu
+    m = st.SubstitutionMatrix.blosum62()
+    rng = random.Random(seed_of(mode, go, ge))
+    edges = [31, 32, 33, 63, 64, 65, 95, 96, 97, 128, 129]
+    q = [rand_seq(rng, L, L) for L in edges] + [rand_seq(rng, 82, 98) for _ in range(24)]
+    q += [rand_seq(rng, 150, 250) for _ in range(3)] + [rand_seq(rng, 0, 30) for _ in range(8)]
+    r = [rand_seq(rng, 0, 120) for _ in range(500)] + [rand_seq(rng, 200, 260) for _ in range(8)]
+    r += q[:6]
+    _, labels = launch_labels(lambda: _check(st, oracle_lib, q, r, m, mode, go, ge))
+    assert any(x.startswith(label) for x in labels), labels
+    _check(st, oracle_lib, q, r, m, mode, go, ge, dist=True)
+
+
+def test_germline_v_all_vs_all(gpu, oracle_lib):
+    """The reference benchmark's germline shape (bench/bench_pairwise.py: lengths 90 +- 8,
+    BLOSUM62 11/1), all against all, global and local, scores and distances."""
+    st = gpu
+    m = st.SubstitutionMatrix.blosum62()
+    rng = random.Random(90)
+    xs = [rand_seq(rng, 82, 98) for _ in range(200)]
+    for mode in ("global", "local"):
+        got = _check(st, oracle_lib, xs, xs, m, mode, 11, 1)
+        d = _check(st, oracle_lib, xs, xs, m, mode, 11, 1, dist=True)
+        assert (np.diag(d) == 0).all() and (got == got.T).all()
 
 
 def test_asymmetric_custom_matrix(gpu, oracle_lib):
