@@ -1028,7 +1028,9 @@ void pf_encode_forward(const __bf16* B, const float* P, const uint8_t* codes, co
 // inside int32 ((q_max + r_max + 2) * max(|sim|, go, ge) < 2^28, so the -2^29 sentinel and a
 // few gap steps below it keep every comparison of the int64 DP), int64 cells otherwise. The
 // direction bytes (and, past the LDS limit or for int64, the DP rows) take per-workgroup
-// scratch; the grid is sized so that scratch stays under 512 MB.
+// scratch; the grid is sized so that scratch stays under 1 GiB. Lists of 4,096 pairs or more
+// are processed in shape order (a counting sort by lengths clamped to 31), so a wave's lanes
+// run the same number of cells; every output still goes to the pair's own index.
 void traceback_device(const uint8_t* q, const uint64_t* q_off, uint32_t q_max, const uint8_t* r,
                       const uint64_t* r_off, uint32_t r_max, const uint32_t* pq,
                       const uint32_t* pr, uint64_t n_pairs, const int32_t* sim_h, uint32_t A,
@@ -1045,12 +1047,15 @@ void traceback_device(const uint8_t* q, const uint64_t* q_off, uint32_t q_max, c
     const DeviceTable sim = table_on_device(ar, sim_h, size_t(A) * A, 0, s);
     const int64_t mx = std::max<int64_t>({sim.abs_max, int64_t(go), int64_t(ge)});
     const bool wide = (int64_t(q_max) + r_max + 2) * mx >= (int64_t(1) << 28);
+    // registers for the rows of short pairs, LDS rows up to 64 reference residues, else scratch
+    const bool reg = !wide && r_max <= kTbRegCols && q_max <= kTbRegMaxQ;
     const bool lds_rows = !wide && r_max <= kTracebackLdsMaxLen;
     const uint64_t W = uint64_t(r_max) + 1;
-    const uint64_t dir_bytes = (uint64_t(q_max) + 1) * W * 64;  // per workgroup
+    const uint64_t dir_bytes = reg ? (uint64_t(q_max) + 1) * kTbRegGroups * 64 * 4
+                                   : (uint64_t(q_max) + 1) * W * 64;  // per workgroup
     const uint64_t row_bytes = lds_rows ? 0 : 3 * W * 64 * (wide ? 8 : 4);
     const uint64_t need = (n_pairs + 63) / 64;
-    const uint64_t fit = std::max<uint64_t>(1, (uint64_t(512) << 20) / (dir_bytes + row_bytes));
+    const uint64_t fit = std::max<uint64_t>(1, (uint64_t(1) << 30) / (dir_bytes + row_bytes));
     const uint32_t wgs = uint32_t(std::min<uint64_t>({need, fit, 65535}));
     uint8_t* dirs = ar.get<uint8_t>(dir_bytes * wgs);
     void* rows = row_bytes ? static_cast<void*>(ar.get<uint8_t>(row_bytes * wgs)) : nullptr;
@@ -1075,9 +1080,23 @@ void traceback_device(const uint8_t* q, const uint64_t* q_off, uint32_t q_max, c
     a.n_ops = n_ops;
     a.q_begin = q_begin;
     a.r_begin = r_begin;
-    LaunchTimer t(std::string("traceback_") + (wide ? "i64" : "i32") + (local ? "_local" : "_global"),
+    if (n_pairs >= 4096 && n_pairs < (uint64_t(1) << 32)) {
+        // lanes of one wave take pairs of one shape (the DP and the walk run per lane)
+        const uint64_t table_n = uint64_t(traceback_order_slices(n_pairs)) * kTracebackShapeBins;
+        uint32_t* table = ar.get<uint32_t>(table_n + 2 * n_pairs);
+        uint32_t* order = table + table_n + n_pairs;
+        {
+            LaunchTimer t("traceback_order", s);
+            launch_traceback_order(a, table, table + table_n, order, s);
+        }
+        launched("traceback_order");
+        a.order = order;
+    }
+    LaunchTimer t(std::string("traceback_") + (reg ? "reg" : wide ? "i64" : "i32") +
+                      (local ? "_local" : "_global"),
                   s);
-    launch_traceback(a, wide, lds_rows, wgs, rows, dirs, s);
+    if (reg) launch_traceback_reg(a, wgs, reinterpret_cast<uint32_t*>(dirs), s);
+    else launch_traceback(a, wide, lds_rows, wgs, rows, dirs, s);
     launched("traceback");
 }
 

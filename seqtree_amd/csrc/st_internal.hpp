@@ -328,6 +328,7 @@ struct TracebackArgs {
     uint32_t* n_ops;
     uint32_t* q_begin;
     uint32_t* r_begin;
+    const uint32_t* order;  // processing order of the pairs (a permutation), or nullptr
 };
 // Longest reference whose DP rows fit the LDS (int32 cells).
 constexpr uint32_t kTracebackLdsMaxLen = 64;
@@ -336,6 +337,21 @@ constexpr uint32_t kTracebackLdsMaxLen = 64;
 // dirs: (q_max + 1) (r_max + 1) 64 bytes per workgroup.
 void launch_traceback(const TracebackArgs& a, bool wide, bool lds_rows, uint32_t workgroups,
                       void* rows, uint8_t* dirs, hipStream_t s);
+// Register variant (int32 cells): references of at most kTbRegCols residues and queries of at
+// most kTbRegMaxQ; dirs holds (q_max + 1) * kTbRegGroups * 64 dwords per workgroup.
+constexpr uint32_t kTbRegCols = 32;
+constexpr uint32_t kTbRegGroups = (kTbRegCols + 1 + 3) / 4;
+constexpr uint32_t kTbRegMaxQ = 64;
+void launch_traceback_reg(const TracebackArgs& a, uint32_t workgroups, uint32_t* dirs,
+                          hipStream_t s);
+// Shape order for the traceback (n_pairs < 2^32): counting sort of the pairs by their
+// clamped (query, reference) lengths into order[n_pairs], over traceback_order_slices(n)
+// contiguous slices; table holds slices * kTracebackShapeBins counters, keys n_pairs.
+constexpr uint32_t kTracebackShapeBins = 1024;
+constexpr uint32_t kTracebackShapeSlices = 256;
+uint32_t traceback_order_slices(uint64_t n_pairs);
+void launch_traceback_order(const TracebackArgs& a, uint32_t* table, uint32_t* keys,
+                            uint32_t* order, hipStream_t s);
 
 // ---- distance.hip ----------------------------------------------------------------------
 struct DistanceArgs {

@@ -82,7 +82,8 @@ __global__ __launch_bounds__(kTbLanes) void traceback_kernel(
     const int32_t go = a.gap_open, ge = a.gap_extend;
     const uint64_t stride = uint64_t(gridDim.x) * kTbLanes;
 
-    for (uint64_t p = uint64_t(blockIdx.x) * kTbLanes + lane; p < a.n_pairs; p += stride) {
+    for (uint64_t t = uint64_t(blockIdx.x) * kTbLanes + lane; t < a.n_pairs; t += stride) {
+        const uint64_t p = a.order ? a.order[t] : t;  // shape order: lanes finish together
         const uint32_t qi = a.pair_q[p], ri = a.pair_r[p];
         const uint64_t q0 = a.q_off[qi], r0 = a.r_off[ri];
         const uint32_t m = uint32_t(a.q_off[qi + 1] - q0), n = uint32_t(a.r_off[ri + 1] - r0);
@@ -201,6 +202,266 @@ __global__ __launch_bounds__(kTbLanes) void traceback_kernel(
     }
 }
 
+// Register variant for int32 cells, references of at most kTbRegCols residues and queries of
+// at most kTbRegMaxQ: the three DP rows live in registers (columns 0..kTbRegCols, unrolled,
+// so every index is a constant) and the lane's reference codes in eight packed registers. A
+// row stops at the first 4-column group no lane of the wave reaches. The direction bytes of
+// four consecutive columns go out as one dword (dwords [row][(r_max + 4) / 4][64 lanes] per
+// workgroup), and the ops are staged in LDS ([q_max + r_max][64 lanes]) and written forward
+// once the walk has ended. Same recurrences, argmax order, boundary bytes and traceback rules
+// as traceback_kernel.
+template <bool LOCAL>
+__global__ __launch_bounds__(kTbLanes) void traceback_reg_kernel(TracebackArgs a,
+                                                                 uint32_t* __restrict__ dirs_g) {
+    constexpr uint32_t NC = kTbRegCols;
+    constexpr uint32_t W4 = kTbRegGroups;
+    extern __shared__ int32_t tb_lds[];
+    const uint32_t lane = threadIdx.x;
+    int32_t* sim_l = tb_lds;
+    for (uint32_t e = lane; e < a.A * a.A; e += kTbLanes) sim_l[e] = a.sim[e];
+    __syncthreads();  // the only barrier: the pair loop below has none
+    uint8_t* ops_l = reinterpret_cast<uint8_t*>(tb_lds + kMaxAlphabet * kMaxAlphabet);
+    uint32_t* dirs = dirs_g + uint64_t(blockIdx.x) * (a.q_max + 1) * W4 * kTbLanes + lane;
+    const uint8_t* dbytes =
+        reinterpret_cast<const uint8_t*>(dirs_g + uint64_t(blockIdx.x) * (a.q_max + 1) * W4 * kTbLanes);
+    auto D = [&](uint32_t i, uint32_t j) -> uint8_t {
+        return dbytes[((uint64_t(i) * W4 + (j >> 2)) * kTbLanes + lane) * 4 + (j & 3u)];
+    };
+    constexpr int32_t NEG = NegInf<int32_t>::v;
+    const int32_t go = a.gap_open, ge = a.gap_extend;
+    const uint64_t stride = uint64_t(gridDim.x) * kTbLanes;
+
+    for (uint64_t t = uint64_t(blockIdx.x) * kTbLanes + lane; t < a.n_pairs; t += stride) {
+        const uint64_t p = a.order ? a.order[t] : t;
+        const uint32_t qi = a.pair_q[p], ri = a.pair_r[p];
+        const uint64_t q0 = a.q_off[qi], r0 = a.r_off[ri];
+        const uint32_t m = uint32_t(a.q_off[qi + 1] - q0), n = uint32_t(a.r_off[ri + 1] - r0);
+        const uint8_t* q = a.q_codes + q0;
+        const uint8_t* r = a.r_codes + r0;
+        uint8_t* ops = a.ops + p * a.ops_stride;
+        uint32_t cnt = 0, i = 0, j = 0;
+        int32_t score;
+        if (m == 0 || n == 0) {
+            score = LOCAL ? 0 : -gap_run<int32_t>(m ? m : n, go, ge);
+            if (!LOCAL) {
+                for (uint32_t t2 = 0; t2 < m; ++t2) ops[cnt++] = 'D';
+                for (uint32_t t2 = 0; t2 < n; ++t2) ops[cnt++] = 'I';
+            }
+        } else {
+            uint32_t rcw[NC / 4];
+#pragma unroll
+            for (uint32_t w = 0; w < NC / 4; ++w) rcw[w] = 0;
+#pragma unroll
+            for (uint32_t c = 0; c < NC; ++c)
+                if (c < n) rcw[c / 4] |= uint32_t(r[c]) << (8 * (c % 4));
+            int32_t rm[NC + 1], rx[NC + 1], ry[NC + 1];  // the row above, updated in place
+            rm[0] = 0;
+            rx[0] = NEG;
+            ry[0] = NEG;
+#pragma unroll
+            for (uint32_t c = 1; c <= NC; ++c) {
+                rm[c] = LOCAL ? 0 : NEG;
+                rx[c] = NEG;
+                ry[c] = LOCAL ? NEG : -gap_run<int32_t>(c, go, ge);
+            }
+            // row 0's direction bytes are constants
+#pragma unroll
+            for (uint32_t g = 0; g < W4; ++g) {
+                uint32_t v = 0;
+#pragma unroll
+                for (uint32_t b = 0; b < 4; ++b) {
+                    const uint32_t c = 4 * g + b;
+                    const uint32_t byte =
+                        c == 0 ? (kStStop | (kStM << 2) | (kStM << 4))
+                               : (kStStop | (kStM << 2) | ((c > 1 ? kStY : kStM) << 4));
+                    v |= byte << (8 * b);
+                }
+                dirs[uint64_t(g) * kTbLanes] = v;
+            }
+            int32_t best = LOCAL ? 0 : NEG;
+            uint32_t bi = 0, bj = 0;
+            uint32_t qc_next = q[0];
+            for (uint32_t row = 1; row <= m; ++row) {
+                const int32_t* srow = sim_l + qc_next * a.A;
+                if (row < m) qc_next = q[row];
+                uint32_t* drow = dirs + uint64_t(row) * W4 * kTbLanes;
+                int32_t dm = rm[0], dx = rx[0], dy = ry[0];
+                int32_t lm = LOCAL ? 0 : NEG;
+                int32_t lx = LOCAL ? NEG : -gap_run<int32_t>(row, go, ge);
+                int32_t ly = NEG;
+                rm[0] = lm;
+                rx[0] = lx;
+                ry[0] = ly;
+                uint32_t acc = kStStop | ((row > 1 ? kStX : kStM) << 2) | (kStM << 4);
+                uint32_t g_open = 0;  // the group acc belongs to
+#pragma unroll
+                for (uint32_t c = 1; c <= NC; ++c) {
+                    if (c % 4 == 0) {  // columns c - 4 .. c - 1 are done
+                        drow[uint64_t(c / 4 - 1) * kTbLanes] = acc;
+                        acc = 0;
+                        g_open = c / 4;
+                        if (!__any(c <= n)) break;  // uniform: no lane has column c
+                    }
+                    const uint32_t rc = (rcw[(c - 1) / 4] >> (8 * ((c - 1) % 4))) & 0xFFu;
+                    const int32_t um = rm[c], ux = rx[c], uy = ry[c];
+                    uint8_t wm, wx, wy;
+                    int32_t mv = pick(dm, dx, dy, wm) + srow[rc];
+                    if (LOCAL && mv < 0) {
+                        mv = 0;
+                        wm = kStStop;
+                    }
+                    const int32_t xv = pick(um - go, ux - ge, uy - go, wx);
+                    const int32_t yv = pick(lm - go, lx - go, ly - ge, wy);
+                    if (LOCAL && c <= n && mv > best) {
+                        best = mv;
+                        bi = row;
+                        bj = c;
+                    }
+                    rm[c] = mv;
+                    rx[c] = xv;
+                    ry[c] = yv;
+                    acc |= uint32_t(wm | (wx << 2) | (wy << 4) | (mv == 0 ? kZeroBit : 0))
+                           << (8 * (c % 4));
+                    dm = um;
+                    dx = ux;
+                    dy = uy;
+                    lm = mv;
+                    lx = xv;
+                    ly = yv;
+                }
+                drow[uint64_t(g_open) * kTbLanes] = acc;  // the last, partial group
+            }
+            uint8_t st;
+            bool run = true;
+            if (LOCAL) {
+                score = best;
+                i = bi;
+                j = bj;
+                st = kStM;
+                run = best != 0;
+            } else {
+                int32_t fm = rm[0], fx = rx[0], fy = ry[0];
+#pragma unroll
+                for (uint32_t c = 1; c <= NC; ++c)
+                    if (c == n) {
+                        fm = rm[c];
+                        fx = rx[c];
+                        fy = ry[c];
+                    }
+                score = pick(fm, fx, fy, st);
+                i = m;
+                j = n;
+            }
+            while (run && (i > 0 || j > 0)) {
+                const uint8_t dv = D(i, j);
+                uint8_t op;
+                if (st == kStM) {
+                    if (i == 0 || j == 0) break;
+                    if (LOCAL && (dv & kZeroBit)) break;
+                    op = q[i - 1] == r[j - 1] ? 'M' : 'S';
+                    st = dv & 3u;
+                    --i;
+                    --j;
+                } else if (st == kStX) {
+                    if (i == 0) break;
+                    op = 'D';
+                    st = (dv >> 2) & 3u;
+                    --i;
+                } else {
+                    if (j == 0) break;
+                    op = 'I';
+                    st = (dv >> 4) & 3u;
+                    --j;
+                }
+                ops_l[cnt * kTbLanes + lane] = op;
+                ++cnt;
+                if (st == kStStop) break;
+            }
+            for (uint32_t t2 = 0; t2 < cnt; ++t2) ops[t2] = ops_l[(cnt - 1 - t2) * kTbLanes + lane];
+        }
+        a.score[p] = int64_t(score);
+        a.n_ops[p] = cnt;
+        a.q_begin[p] = i;
+        a.r_begin[p] = j;
+    }
+}
+
+// Shape order of the pairs: a counting sort by (min(m, 31), min(n, 31)), so that the lanes of
+// a wave get pairs of one shape and run the same number of DP cells and traceback steps.
+// Workgroup b owns a contiguous slice of the list. Count: an LDS histogram of the slice's
+// keys, written as row b of a [workgroups][bins] table. Scan (one workgroup): per bin, the
+// bin's start in the order plus the counts of the slices before b, in place. Scatter: each
+// slice places its pairs from its own row of offsets with LDS atomics only. The order inside
+// a bin does not matter: every output is written at the pair's own index.
+constexpr uint32_t kShapeThreads = 256;
+
+__device__ __forceinline__ uint32_t shape_key(const TracebackArgs& a, uint64_t p) {
+    const uint32_t qi = a.pair_q[p], ri = a.pair_r[p];
+    const uint64_t m = a.q_off[qi + 1] - a.q_off[qi], n = a.r_off[ri + 1] - a.r_off[ri];
+    return uint32_t(m < 31 ? m : 31) * 32 + uint32_t(n < 31 ? n : 31);
+}
+
+__device__ __forceinline__ void shape_slice(uint64_t n, uint64_t& p0, uint64_t& p1) {
+    const uint64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    p0 = uint64_t(blockIdx.x) * chunk;
+    p1 = p0 + chunk < n ? p0 + chunk : n;
+}
+
+__global__ __launch_bounds__(kShapeThreads) void shape_count_kernel(TracebackArgs a,
+                                                                    uint32_t* __restrict__ table,
+                                                                    uint32_t* __restrict__ keys) {
+    __shared__ uint32_t h[kTracebackShapeBins];
+    for (uint32_t b = threadIdx.x; b < kTracebackShapeBins; b += kShapeThreads) h[b] = 0;
+    __syncthreads();
+    uint64_t p0, p1;
+    shape_slice(a.n_pairs, p0, p1);
+    for (uint64_t p = p0 + threadIdx.x; p < p1; p += kShapeThreads) {
+        const uint32_t k = shape_key(a, p);
+        keys[p] = k;
+        atomicAdd(&h[k], 1u);
+    }
+    __syncthreads();
+    uint32_t* row = table + uint64_t(blockIdx.x) * kTracebackShapeBins;
+    for (uint32_t b = threadIdx.x; b < kTracebackShapeBins; b += kShapeThreads) row[b] = h[b];
+}
+
+// One workgroup of kTracebackShapeBins threads, thread k owning bin k.
+__global__ __launch_bounds__(kTracebackShapeBins) void shape_scan_kernel(
+    uint32_t* __restrict__ table, uint32_t slices) {
+    __shared__ uint32_t part[kTracebackShapeBins];
+    const uint32_t k = threadIdx.x;
+    uint32_t total = 0;
+    for (uint32_t b = 0; b < slices; ++b) total += table[uint64_t(b) * kTracebackShapeBins + k];
+    part[k] = total;
+    __syncthreads();
+    for (uint32_t off = 1; off < kTracebackShapeBins; off <<= 1) {  // inclusive scan
+        const uint32_t x = k >= off ? part[k - off] : 0u;
+        __syncthreads();
+        part[k] += x;
+        __syncthreads();
+    }
+    uint32_t run = part[k] - total;  // where bin k starts
+    for (uint32_t b = 0; b < slices; ++b) {
+        uint32_t& v = table[uint64_t(b) * kTracebackShapeBins + k];
+        const uint32_t c = v;
+        v = run;
+        run += c;
+    }
+}
+
+__global__ __launch_bounds__(kShapeThreads) void shape_scatter_kernel(
+    uint64_t n, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ table,
+    uint32_t* __restrict__ order) {
+    __shared__ uint32_t cur[kTracebackShapeBins];
+    const uint32_t* row = table + uint64_t(blockIdx.x) * kTracebackShapeBins;
+    for (uint32_t b = threadIdx.x; b < kTracebackShapeBins; b += kShapeThreads) cur[b] = row[b];
+    __syncthreads();
+    uint64_t p0, p1;
+    shape_slice(n, p0, p1);
+    for (uint64_t p = p0 + threadIdx.x; p < p1; p += kShapeThreads)
+        order[atomicAdd(&cur[keys[p]], 1u)] = uint32_t(p);
+}
+
 template <class T, bool LOCAL>
 void launch_t(const TracebackArgs& a, bool lds_rows, uint32_t workgroups, void* rows,
               uint8_t* dirs, hipStream_t s) {
@@ -220,6 +481,36 @@ void launch_t(const TracebackArgs& a, bool lds_rows, uint32_t workgroups, void* 
 }
 
 }  // namespace
+
+uint32_t traceback_order_slices(uint64_t n_pairs) {
+    const uint64_t slices = (n_pairs + 4095) / 4096;
+    return uint32_t(slices < kTracebackShapeSlices ? slices : kTracebackShapeSlices);
+}
+
+void launch_traceback_order(const TracebackArgs& a, uint32_t* table, uint32_t* keys,
+                            uint32_t* order, hipStream_t s) {
+    if (a.n_pairs == 0) return;
+    const uint32_t slices = traceback_order_slices(a.n_pairs);
+    hipLaunchKernelGGL(shape_count_kernel, dim3(slices), dim3(kShapeThreads), 0, s, a, table,
+                       keys);
+    hipLaunchKernelGGL(shape_scan_kernel, dim3(1), dim3(kTracebackShapeBins), 0, s, table,
+                       slices);
+    hipLaunchKernelGGL(shape_scatter_kernel, dim3(slices), dim3(kShapeThreads), 0, s,
+                       a.n_pairs, keys, table, order);
+}
+
+void launch_traceback_reg(const TracebackArgs& a, uint32_t workgroups, uint32_t* dirs,
+                          hipStream_t s) {
+    if (a.n_pairs == 0 || workgroups == 0) return;
+    const size_t lds = kMaxAlphabet * kMaxAlphabet * sizeof(int32_t) +
+                       size_t(a.q_max + a.r_max) * kTbLanes;
+    if (a.local)
+        hipLaunchKernelGGL(traceback_reg_kernel<true>, dim3(workgroups), dim3(kTbLanes), lds, s,
+                           a, dirs);
+    else
+        hipLaunchKernelGGL(traceback_reg_kernel<false>, dim3(workgroups), dim3(kTbLanes), lds, s,
+                           a, dirs);
+}
 
 void launch_traceback(const TracebackArgs& a, bool wide, bool lds_rows, uint32_t workgroups,
                       void* rows, uint8_t* dirs, hipStream_t s) {

@@ -11,7 +11,7 @@ import random
 import numpy as np
 import pytest
 
-from conftest import cdr3_like, rand_seq, seed_of
+from conftest import cdr3_like, launch_labels, rand_seq, seed_of
 
 pytestmark = pytest.mark.gpu
 
@@ -73,6 +73,60 @@ def test_align_pairs_batch_equals_the_oracle(gpu, oracle_lib):
             aq, ar = _strings(a, b, ops, qb, rb)
             g = got[t]
             assert (g.score, g.ops, g.aligned_query, g.aligned_ref) == (s, ops, aq, ar), t
+
+
+def test_large_lists_run_in_shape_order(gpu, oracle_lib):
+    """From 4,096 pairs on, the pairs are processed in shape order (sorted by clamped
+    lengths); every output must still land at its own pair's index. Lengths past the clamp
+    (63) and repeated pairs included."""
+    st = gpu
+    m = st.SubstitutionMatrix.blosum62()
+    sim = _sim(st, m)
+    rng = random.Random(47)
+    q = cdr3_like(rng, 300, 6, 30) + [rand_seq(rng, 60, 70) for _ in range(10)]
+    r = cdr3_like(rng, 400, 6, 30) + [rand_seq(rng, 60, 66) for _ in range(10)] + [""]
+    npr = np.random.default_rng(47)
+    n = 6000
+    pq = npr.integers(0, len(q), n).astype(np.uint32)
+    pr = npr.integers(0, len(r), n).astype(np.uint32)
+    pq[:50], pr[:50] = 3, 5
+    for mode in ("global", "local"):
+        got = st.pairwise.align_pairs(q, r, pq, pr, m, mode=mode)
+        assert len(got) == n
+        for t in range(n):
+            a, b = q[pq[t]], r[pr[t]]
+            s_, ops, qb, rb = oracle_lib.align_pair(a, b, sim, mode == "local", 11, 1)
+            g = got[t]
+            assert (g.score, g.ops) == (s_, ops), t
+            assert (g.aligned_query, g.aligned_ref) == _strings(a, b, ops, qb, rb), t
+
+
+@pytest.mark.parametrize("go,ge", [(11, 1), (1, 3), (0, 0)])
+def test_register_rows_kernel(gpu, oracle_lib, go, ge):
+    """References of at most 32 residues and queries of at most 64 take the register-row
+    kernel: every reference length 1..32 (the 4-column groups and the last partial one),
+    queries up to 64, lists below and above the shape-order threshold."""
+    st = gpu
+    m = st.SubstitutionMatrix.blosum62()
+    sim = _sim(st, m)
+    rng = random.Random(seed_of("reg", go, ge))
+    q = [rand_seq(rng, 1, 64) for _ in range(60)] + ["W" * 64, "A"]
+    r = [rand_seq(rng, L, L) for L in range(1, 33)] + [rand_seq(rng, 1, 12, "AAC")
+                                                     for _ in range(20)] + [""]
+    npr = np.random.default_rng(seed_of("reg", go, ge))
+    for n in (700, 5000):
+        pq = npr.integers(0, len(q), n).astype(np.uint32)
+        pr = npr.integers(0, len(r), n).astype(np.uint32)
+        for mode in ("global", "local"):
+            got, labels = launch_labels(lambda: st.pairwise.align_pairs(
+                q, r, pq, pr, m, mode=mode, gap_open=go, gap_extend=ge))
+            assert f"traceback_reg_{mode}" in labels, labels
+            for t in range(0, n, 1 if n < 1000 else 3):
+                a, b = q[pq[t]], r[pr[t]]
+                s_, ops, qb, rb = oracle_lib.align_pair(a, b, sim, mode == "local", go, ge)
+                g = got[t]
+                assert (g.score, g.ops) == (s_, ops), (t, a, b)
+                assert (g.aligned_query, g.aligned_ref) == _strings(a, b, ops, qb, rb), t
 
 
 def test_long_sequences_take_the_scratch_rows(gpu, oracle_lib):
