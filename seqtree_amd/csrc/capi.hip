@@ -1082,7 +1082,8 @@ void traceback_device(const uint8_t* q, const uint64_t* q_off, uint32_t q_max, c
     a.r_begin = r_begin;
     if (n_pairs >= 4096 && n_pairs < (uint64_t(1) << 32)) {
         // lanes of one wave take pairs of one shape (the DP and the walk run per lane)
-        const uint64_t table_n = uint64_t(traceback_order_slices(n_pairs)) * kTracebackShapeBins;
+        const uint64_t table_n =
+            (uint64_t(traceback_order_slices(n_pairs)) + 1) * kTracebackShapeBins;
         uint32_t* table = ar.get<uint32_t>(table_n + 2 * n_pairs);
         uint32_t* order = table + table_n + n_pairs;
         {

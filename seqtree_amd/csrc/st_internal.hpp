@@ -346,7 +346,7 @@ void launch_traceback_reg(const TracebackArgs& a, uint32_t workgroups, uint32_t*
                           hipStream_t s);
 // Shape order for the traceback (n_pairs < 2^32): counting sort of the pairs by their
 // clamped (query, reference) lengths into order[n_pairs], over traceback_order_slices(n)
-// contiguous slices; table holds slices * kTracebackShapeBins counters, keys n_pairs.
+// contiguous slices; table holds (slices + 1) * kTracebackShapeBins counters, keys n_pairs.
 constexpr uint32_t kTracebackShapeBins = 1024;
 constexpr uint32_t kTracebackShapeSlices = 256;
 uint32_t traceback_order_slices(uint64_t n_pairs);

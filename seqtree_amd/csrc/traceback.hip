@@ -389,9 +389,10 @@ __global__ __launch_bounds__(kTbLanes) void traceback_reg_kernel(TracebackArgs a
 // Shape order of the pairs: a counting sort by (min(m, 31), min(n, 31)), so that the lanes of
 // a wave get pairs of one shape and run the same number of DP cells and traceback steps.
 // Workgroup b owns a contiguous slice of the list. Count: an LDS histogram of the slice's
-// keys, written as row b of a [workgroups][bins] table. Scan (one workgroup): per bin, the
-// bin's start in the order plus the counts of the slices before b, in place. Scatter: each
-// slice places its pairs from its own row of offsets with LDS atomics only. The order inside
+// keys, written as row b of a [workgroups][bins] table. Column pass (one wave per bin): the
+// counts of the slices before b, in place, and the bin's total; scan (one workgroup): where
+// each bin starts. Scatter: each slice places its pairs from its own row of offsets plus the
+// bin starts, with LDS atomics only. The order inside
 // a bin does not matter: every output is written at the pair's own index.
 constexpr uint32_t kShapeThreads = 256;
 
@@ -425,14 +426,36 @@ __global__ __launch_bounds__(kShapeThreads) void shape_count_kernel(TracebackArg
     for (uint32_t b = threadIdx.x; b < kTracebackShapeBins; b += kShapeThreads) row[b] = h[b];
 }
 
-// One workgroup of kTracebackShapeBins threads, thread k owning bin k.
-__global__ __launch_bounds__(kTracebackShapeBins) void shape_scan_kernel(
-    uint32_t* __restrict__ table, uint32_t slices) {
+// Column pass: one wave per bin k turns its column of slice counts into an exclusive prefix
+// over the slices (in place) and writes the bin's total.
+__global__ __launch_bounds__(kShapeThreads) void shape_column_kernel(uint32_t* __restrict__ table,
+                                                                     uint32_t slices,
+                                                                     uint32_t* __restrict__ total) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t k = blockIdx.x * (kShapeThreads / 64) + (threadIdx.x >> 6);
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < slices; b0 += 64) {
+        const uint32_t b = b0 + lane;
+        uint32_t* cell = table + uint64_t(b) * kTracebackShapeBins + k;
+        const uint32_t v = b < slices ? *cell : 0u;
+        uint32_t x = v;  // inclusive wave scan
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (b < slices) *cell = carry + x - v;
+        carry += __shfl(x, 63);
+    }
+    if (lane == 0) total[k] = carry;
+}
+
+// One workgroup of kTracebackShapeBins threads: exclusive scan of the bin totals (where each
+// bin starts in the order), in place.
+__global__ __launch_bounds__(kTracebackShapeBins) void shape_scan_kernel(uint32_t* __restrict__ total) {
     __shared__ uint32_t part[kTracebackShapeBins];
     const uint32_t k = threadIdx.x;
-    uint32_t total = 0;
-    for (uint32_t b = 0; b < slices; ++b) total += table[uint64_t(b) * kTracebackShapeBins + k];
-    part[k] = total;
+    const uint32_t own = total[k];
+    part[k] = own;
     __syncthreads();
     for (uint32_t off = 1; off < kTracebackShapeBins; off <<= 1) {  // inclusive scan
         const uint32_t x = k >= off ? part[k - off] : 0u;
@@ -440,21 +463,16 @@ __global__ __launch_bounds__(kTracebackShapeBins) void shape_scan_kernel(
         part[k] += x;
         __syncthreads();
     }
-    uint32_t run = part[k] - total;  // where bin k starts
-    for (uint32_t b = 0; b < slices; ++b) {
-        uint32_t& v = table[uint64_t(b) * kTracebackShapeBins + k];
-        const uint32_t c = v;
-        v = run;
-        run += c;
-    }
+    total[k] = part[k] - own;
 }
 
 __global__ __launch_bounds__(kShapeThreads) void shape_scatter_kernel(
     uint64_t n, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ table,
-    uint32_t* __restrict__ order) {
+    const uint32_t* __restrict__ start, uint32_t* __restrict__ order) {
     __shared__ uint32_t cur[kTracebackShapeBins];
     const uint32_t* row = table + uint64_t(blockIdx.x) * kTracebackShapeBins;
-    for (uint32_t b = threadIdx.x; b < kTracebackShapeBins; b += kShapeThreads) cur[b] = row[b];
+    for (uint32_t b = threadIdx.x; b < kTracebackShapeBins; b += kShapeThreads)
+        cur[b] = start[b] + row[b];
     __syncthreads();
     uint64_t p0, p1;
     shape_slice(n, p0, p1);
@@ -493,10 +511,12 @@ void launch_traceback_order(const TracebackArgs& a, uint32_t* table, uint32_t* k
     const uint32_t slices = traceback_order_slices(a.n_pairs);
     hipLaunchKernelGGL(shape_count_kernel, dim3(slices), dim3(kShapeThreads), 0, s, a, table,
                        keys);
-    hipLaunchKernelGGL(shape_scan_kernel, dim3(1), dim3(kTracebackShapeBins), 0, s, table,
-                       slices);
+    uint32_t* total = table + uint64_t(slices) * kTracebackShapeBins;
+    hipLaunchKernelGGL(shape_column_kernel, dim3(kTracebackShapeBins / (kShapeThreads / 64)),
+                       dim3(kShapeThreads), 0, s, table, slices, total);
+    hipLaunchKernelGGL(shape_scan_kernel, dim3(1), dim3(kTracebackShapeBins), 0, s, total);
     hipLaunchKernelGGL(shape_scatter_kernel, dim3(slices), dim3(kShapeThreads), 0, s,
-                       a.n_pairs, keys, table, order);
+                       a.n_pairs, keys, table, total, order);
 }
 
 void launch_traceback_reg(const TracebackArgs& a, uint32_t workgroups, uint32_t* dirs,
