@@ -732,7 +732,7 @@ void gotoh_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, uint32_t 
                               (local ? "_local" : "_global"),
                           s);
             launch_gotoh_stripes(a, packed16, lists + size_t(gen_class) * N, n_long, grid, bnd,
-                                 W, s);
+                                 W, q_max, s);
         }
         launched("gotoh_stripes");
     } else if (counts[gen_class] > 0) {

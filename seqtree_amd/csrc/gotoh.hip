@@ -220,14 +220,19 @@ __global__ __launch_bounds__(kThreads) void gotoh_reg_kernel(GotohArgs a,
 // the boundary scratch is sized by the grid, not by the matrix; every workgroup owns
 // NB * W * kThreads words of it. A query's score is taken from the stripe holding its last
 // row; two packed queries of different lengths run to the longer one's last stripe.
+//
+// prof_rows: the LDS profile's rows (dynamic LDS). When they hold every stripe of the longest
+// query (full profile), the profile is built once per unit and the slot and stripe loops run
+// without barriers; otherwise (prof_rows == R) each stripe rebuilds its R rows between two
+// barriers.
 template <class C, int R, bool LOCAL, bool HFORM>
 __global__ __launch_bounds__(kThreads) void gotoh_stripes_kernel(
     GotohArgs a, const uint32_t* __restrict__ list, uint32_t count, uint64_t n_units,
-    uint32_t* __restrict__ bnd_all, uint32_t W) {
+    uint32_t* __restrict__ bnd_all, uint32_t W, uint32_t prof_rows) {
     using T = typename C::T;
     constexpr int Q = C::Q;
     constexpr uint32_t NB = HFORM ? 2 : 3;  // boundary words per column: H, X (, U)
-    __shared__ uint32_t prof[R * 32];
+    extern __shared__ uint32_t prof[];  // [prof_rows][32]
     __shared__ int32_t stage[Q][kChunk];
 
     const uint32_t tid = threadIdx.x;
@@ -258,6 +263,23 @@ __global__ __launch_bounds__(kThreads) void gotoh_stripes_kernel(
             rows = m[h] > rows ? m[h] : rows;
         }
         const uint32_t S = rows == 0 ? 1 : (rows + R - 1) / R;
+        const bool full = S * uint32_t(R) <= prof_rows;  // uniform
+        auto build = [&](uint32_t r0, uint32_t nr) {  // profile rows r0 .. r0 + nr - 1
+            for (uint32_t e = tid; e < nr * 32; e += kThreads) {
+                const uint32_t r = e >> 5, b = e & 31u, i = r0 + r;
+                int32_t v[Q];
+#pragma unroll
+                for (int h = 0; h < Q; ++h)
+                    v[h] = (i < m[h] && b < A) ? a.sim[uint32_t(q[h][i]) * A + b] + shift
+                                               : dummy;
+                prof[(full ? i : r) * 32 + b] = C::pack(v);
+            }
+        };
+        if (full) {
+            __syncthreads();  // every lane is past the previous unit's profile and stage
+            build(0, S * uint32_t(R));
+            __syncthreads();
+        }
 
         for (uint32_t it = 0; it < kSlotsPerThread; ++it) {
             const uint32_t slot = c * kChunk + it * kThreads + tid;
@@ -272,17 +294,12 @@ __global__ __launch_bounds__(kThreads) void gotoh_stripes_kernel(
 
             for (uint32_t sidx = 0; sidx < S; ++sidx) {
                 const uint32_t i0 = sidx * R;  // this stripe holds DP rows i0 + 1 .. i0 + R
-                __syncthreads();  // every lane is past the previous stripe's profile
-                for (uint32_t e = tid; e < uint32_t(R) * 32; e += kThreads) {
-                    const uint32_t r = e >> 5, b = e & 31u, i = i0 + r;
-                    int32_t v[Q];
-#pragma unroll
-                    for (int h = 0; h < Q; ++h)
-                        v[h] = (i < m[h] && b < A) ? a.sim[uint32_t(q[h][i]) * A + b] + shift
-                                                   : dummy;
-                    prof[e] = C::pack(v);
+                if (!full) {
+                    __syncthreads();  // every lane is past the previous stripe's profile
+                    build(i0, R);
+                    __syncthreads();
                 }
-                __syncthreads();
+                const uint32_t* sprof = prof + (full ? i0 * 32 : 0u);
 
                 // column 0: H[i][0] = -gap_run(i) (global) or 0 (local), Y = -inf, X = H
                 T V[HFORM ? 1 : R], Y[R], Hs[R];
@@ -314,7 +331,7 @@ __global__ __launch_bounds__(kThreads) void gotoh_stripes_kernel(
                 for (uint32_t j = 0; j < n; ++j) {
                     if ((j & 3u) == 0) word = wp[uint64_t(j >> 2) * npad];
                     const uint32_t rc = (word >> (8 * (j & 3u))) & 0xFFu;
-                    const uint32_t* pcol = prof + rc;
+                    const uint32_t* pcol = sprof + rc;
                     uint32_t* bj = bnd + uint64_t(j) * NB * kThreads;
                     T up, hd, xup;  // H form: Hg_up ; U/V form: U_up
                     if (i0 == 0) {
@@ -644,13 +661,17 @@ void launch_cell(const GotohArgs& a, int rows, const uint32_t* list, uint32_t co
 template <class C, bool HFORM>
 void launch_stripes_form(const GotohArgs& a, const uint32_t* list, uint32_t count,
                          uint64_t n_units, uint32_t grid, uint32_t* bnd, uint32_t W,
-                         hipStream_t s) {
+                         uint32_t q_max, hipStream_t s) {
+    // the whole profile when it fits kStripeFullRows rows, else one stripe at a time
+    const uint32_t q_pad = (q_max + kStripeRows - 1) / kStripeRows * kStripeRows;
+    const uint32_t rows = q_pad <= kStripeFullRows ? q_pad : uint32_t(kStripeRows);
+    const size_t lds = size_t(rows) * 32 * sizeof(uint32_t);
     if (a.local)
         hipLaunchKernelGGL((gotoh_stripes_kernel<C, kStripeRows, true, HFORM>), dim3(grid),
-                           dim3(kThreads), 0, s, a, list, count, n_units, bnd, W);
+                           dim3(kThreads), lds, s, a, list, count, n_units, bnd, W, rows);
     else
         hipLaunchKernelGGL((gotoh_stripes_kernel<C, kStripeRows, false, HFORM>), dim3(grid),
-                           dim3(kThreads), 0, s, a, list, count, n_units, bnd, W);
+                           dim3(kThreads), lds, s, a, list, count, n_units, bnd, W, rows);
 }
 
 }  // namespace
@@ -666,13 +687,14 @@ uint64_t gotoh_stripes_words(uint32_t grid, uint32_t W, bool hform) {
 
 void launch_gotoh_stripes(const GotohArgs& a, bool packed16, const uint32_t* list,
                           uint32_t count, uint32_t grid, uint32_t* bnd, uint32_t W,
-                          hipStream_t s) {
+                          uint32_t q_max, hipStream_t s) {
     if (count == 0 || a.refs.nchunks == 0 || grid == 0) return;
     const uint64_t units = gotoh_stripes_units(count, packed16, a.refs.nchunks);
-    if (packed16) launch_stripes_form<I16x2, true>(a, list, count, units, grid, bnd, W, s);
+    if (packed16)
+        launch_stripes_form<I16x2, true>(a, list, count, units, grid, bnd, W, q_max, s);
     else if (a.gap_open < a.gap_extend)
-        launch_stripes_form<I32, false>(a, list, count, units, grid, bnd, W, s);
-    else launch_stripes_form<I32, true>(a, list, count, units, grid, bnd, W, s);
+        launch_stripes_form<I32, false>(a, list, count, units, grid, bnd, W, q_max, s);
+    else launch_stripes_form<I32, true>(a, list, count, units, grid, bnd, W, q_max, s);
 }
 
 void launch_gotoh_fast(const GotohArgs& a, int rows, bool packed16, const uint32_t* list,

@@ -284,12 +284,15 @@ void launch_gotoh_fast(const GotohArgs& a, int rows, bool packed16, const uint32
 // (same range proofs with the query's padded length in the bound). Persistent grid of `grid`
 // workgroups over gotoh_stripes_units(...) units; bnd holds gotoh_stripes_words(grid, W,
 // gap_open >= gap_extend || packed16) words, W >= the longest reference.
+// q_max: the longest query of the list (queries padded to at most kStripeFullRows rows keep
+// their whole profile in LDS).
 constexpr int kStripeRows = 32;
+constexpr uint32_t kStripeFullRows = 256;
 uint64_t gotoh_stripes_units(uint32_t count, bool packed16, uint32_t nchunks);
 uint64_t gotoh_stripes_words(uint32_t grid, uint32_t W, bool hform);
 void launch_gotoh_stripes(const GotohArgs& a, bool packed16, const uint32_t* list,
                           uint32_t count, uint32_t grid, uint32_t* bnd, uint32_t W,
-                          hipStream_t s);
+                          uint32_t q_max, hipStream_t s);
 // int64 per-thread DP for arbitrary lengths and gap costs: every (list[t], k) pair, k < n_refs.
 // scratch holds `slots` regions of 6 * (stride) int64.
 void launch_gotoh_generic(const GotohArgs& a, const uint32_t* list, uint32_t count,
