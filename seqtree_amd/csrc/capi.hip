@@ -656,8 +656,8 @@ void gotoh_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, uint32_t 
         LaunchTimer t("gotoh_self", s);
         if (gotoh_self_fits_lds(uint32_t(self_len),
                                 std::max({mx, iabs64(go), iabs64(ge)}))) {
-            launch_gotoh_self_lds(a, q, q_off, N, q_max, qs, s);
-            launch_gotoh_self_lds(a, r, r_off, K, r_max, rs, s);
+            const SelfSets sets{{q, r}, {q_off, r_off}, {N, K}, {qs, rs}, 0};
+            launch_gotoh_self_lds(a, sets, uint32_t(self_len), s);
         } else {
             const uint64_t stride = self_len + 1;
             const uint32_t slots = slots_for(stride);

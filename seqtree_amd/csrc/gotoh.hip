@@ -539,10 +539,11 @@ size_t self_lds_bytes(uint32_t W) {
            3 * size_t(W) * kSelfThreads * sizeof(int32_t);
 }
 
+// Both sets in one launch: workgroups [0, nb0) take set 0, the rest set 1, so the two sets'
+// strings run side by side (a set of a few thousand strings fills only a few dozen CUs).
 template <bool LOCAL>
-__global__ __launch_bounds__(kSelfThreads) void gotoh_self_lds_kernel(
-    GotohArgs a, const uint8_t* __restrict__ codes, const uint64_t* __restrict__ off, uint64_t n,
-    int64_t* __restrict__ res, uint32_t W) {
+__global__ __launch_bounds__(kSelfThreads) void gotoh_self_lds_kernel(GotohArgs a, SelfSets sets,
+                                                                     uint32_t W) {
     extern __shared__ int32_t self_lds[];
     constexpr int32_t kNeg32 = -(1 << 29);
     const uint32_t lane = threadIdx.x;
@@ -552,7 +553,12 @@ __global__ __launch_bounds__(kSelfThreads) void gotoh_self_lds_kernel(
     __syncthreads();  // the only barrier
     uint8_t* qc = reinterpret_cast<uint8_t*>(self_lds + kMaxAlphabet * kMaxAlphabet);
     int32_t* rows = reinterpret_cast<int32_t*>(qc + ((W * kSelfThreads + 15) & ~15u));
-    const uint64_t x = uint64_t(blockIdx.x) * kSelfThreads + lane;
+    const bool second = blockIdx.x >= sets.nb0;
+    const uint8_t* codes = second ? sets.codes[1] : sets.codes[0];
+    const uint64_t* off = second ? sets.off[1] : sets.off[0];
+    const uint64_t n = second ? sets.n[1] : sets.n[0];
+    int64_t* res = second ? sets.res[1] : sets.res[0];
+    const uint64_t x = uint64_t(second ? blockIdx.x - sets.nb0 : blockIdx.x) * kSelfThreads + lane;
     if (x >= n) return;
     const uint64_t o = off[x];
     const uint32_t L = uint32_t(off[x + 1] - o);
@@ -725,18 +731,20 @@ bool gotoh_self_fits_lds(uint32_t max_len, int64_t mx) {
     return max_len <= kSelfMaxLen && (2 * int64_t(max_len) + 2) * mx < (int64_t(1) << 28);
 }
 
-void launch_gotoh_self_lds(const GotohArgs& a, const uint8_t* codes, const uint64_t* off,
-                           uint64_t n, uint32_t max_len, int64_t* res, hipStream_t s) {
-    if (n == 0) return;
-    const dim3 grid(uint32_t((n + kSelfThreads - 1) / kSelfThreads));
+void launch_gotoh_self_lds(const GotohArgs& a, const SelfSets& sets, uint32_t max_len,
+                           hipStream_t s) {
+    SelfSets t = sets;
+    t.nb0 = uint32_t((sets.n[0] + kSelfThreads - 1) / kSelfThreads);
+    const uint32_t nb1 = uint32_t((sets.n[1] + kSelfThreads - 1) / kSelfThreads);
+    if (t.nb0 + nb1 == 0) return;
     const uint32_t W = max_len + 1;
     const size_t lds = self_lds_bytes(W);
     if (a.local)
-        hipLaunchKernelGGL(gotoh_self_lds_kernel<true>, grid, dim3(kSelfThreads), lds, s, a,
-                           codes, off, n, res, W);
+        hipLaunchKernelGGL(gotoh_self_lds_kernel<true>, dim3(t.nb0 + nb1), dim3(kSelfThreads),
+                           lds, s, a, t, W);
     else
-        hipLaunchKernelGGL(gotoh_self_lds_kernel<false>, grid, dim3(kSelfThreads), lds, s, a,
-                           codes, off, n, res, W);
+        hipLaunchKernelGGL(gotoh_self_lds_kernel<false>, dim3(t.nb0 + nb1), dim3(kSelfThreads),
+                           lds, s, a, t, W);
 }
 
 }  // namespace st

@@ -303,10 +303,18 @@ void launch_gotoh_self(const GotohArgs& a, const uint8_t* codes, const uint64_t*
                        hipStream_t s);
 // Self-scores with the DP rows in LDS (strings of at most 160 residues and (2 max_len + 2) *
 // max(|s|, gap costs) < 2^28: gotoh_self_fits_lds); the same results as launch_gotoh_self.
-// max_len: the longest string of the set (sizes the LDS rows).
+// Two CSR sets in one launch (either may be empty); max_len: the longest string of both
+// (sizes the LDS rows); nb0 is set by the launcher.
+struct SelfSets {
+    const uint8_t* codes[2];
+    const uint64_t* off[2];
+    uint64_t n[2];
+    int64_t* res[2];
+    uint32_t nb0;
+};
 bool gotoh_self_fits_lds(uint32_t max_len, int64_t mx);
-void launch_gotoh_self_lds(const GotohArgs& a, const uint8_t* codes, const uint64_t* off,
-                           uint64_t n, uint32_t max_len, int64_t* res, hipStream_t s);
+void launch_gotoh_self_lds(const GotohArgs& a, const SelfSets& sets, uint32_t max_len,
+                           hipStream_t s);
 
 // ---- traceback.hip ---------------------------------------------------------------------
 // Gotoh alignment with traceback of (pair_q[p], pair_r[p]) for p < n_pairs: score[p] (int64),
