@@ -127,13 +127,18 @@ __global__ __launch_bounds__(kPackThreads) void pack_refs_kernel(
             bins[b] = acc;
         }
         __syncthreads();
-        if (tid == 0) {  // bin starts
-            uint32_t acc = 0;
-            for (uint32_t b = 0; b < kSortBins; ++b) {
-                const uint32_t c = bins[b];
-                bins[b] = acc;
-                acc += c;
+        {  // bin starts: exclusive prefix of the bins (Hillis-Steele, kSortBins <= threads)
+            static_assert(kSortBins <= kPackThreads, "one thread per bin");
+            const uint32_t own = tid < kSortBins ? bins[tid] : 0u;
+            uint32_t x = own;
+            for (uint32_t d = 1; d < kSortBins; d <<= 1) {
+                const uint32_t y = (tid < kSortBins && tid >= d) ? bins[tid - d] : 0u;
+                __syncthreads();
+                x += y;
+                if (tid < kSortBins) bins[tid] = x;
+                __syncthreads();
             }
+            if (tid < kSortBins) bins[tid] = x - own;
         }
         __syncthreads();
 #pragma unroll
