@@ -599,7 +599,7 @@ def run_prefilter(torch, dev, lib, stream, args) -> dict:
 
     Q, R = pf.DeviceSet.upload(qc, qo), pf.DeviceSet.upload(rc, ro)
     pairs = float(Q.n) * float(R.n)
-    # exhaustive exact pass: the judge of recall, and the time to beat
+    # exhaustive exact pass: the measure of recall, and the time to beat
     ex_times = []
     for _ in range(2):
         t0 = time.perf_counter()

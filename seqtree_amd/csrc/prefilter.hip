@@ -1,7 +1,7 @@
 // Learned prefilter for thresholded gap-block search: a small 1D-conv sequence encoder and a
 // Gram-distance screen, trained and run with hand-written gfx950 kernels. There is no
 // counterpart in the reference: every score the prefilter lets through is rescored by the
-// exact gap-block kernel, and the exhaustive exact pass stays the judge of recall.
+// exact gap-block kernel, and the exhaustive exact pass stays the measure of recall.
 //
 // Encoder (the sequences of a set packed one after another, each followed by two zero rows):
 //   X0[r]  = emb[code(r)]                      32 channels, zero on rows outside a sequence
