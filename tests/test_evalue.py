@@ -108,6 +108,34 @@ def test_histogram_inversion_equals_the_score_list_inversion():
             assert want == [_brute_theta(s, N, M, e, theta_max, ex) for s in scores], trial
 
 
+def test_score_list_inversion_with_negative_and_out_of_range_scores():
+    """Scores below zero count in every cumulative count; scores above theta_max never do;
+    exclude_exact drops exact zeros only. Checked against the literal definition."""
+    rng = random.Random(11)
+    for trial in range(300):
+        theta_max = rng.randint(0, 12)
+        N, M = rng.randint(1, 200), rng.randint(1, 2000)
+        e = rng.choice([0.1, 0.5, 1.0, 3.0, 8.0, 30.0])
+        scores = [[rng.randint(-3, theta_max + 4) for _ in range(rng.randint(0, 20))]
+                  for _ in range(5)]
+        for ex in (False, True):
+            want = [_brute_theta(s, N, M, e, theta_max, ex) for s in scores]
+            assert thetas_from_scores(scores, N, M, e, theta_max, exclude_exact=ex) == want, trial
+
+
+def test_poisson_sf_point_masses_against_closed_form():
+    """Small k: the lower tail is exp(-lam) * sum_{i<k} lam^i / i!, evaluated here in exact
+    rational arithmetic scaled by exp(-lam)."""
+    from fractions import Fraction
+
+    for lam in (0.1, 0.9, 3.0, 12.5):
+        for k in range(1, 15):
+            lam_q = Fraction(lam)
+            lower = sum(lam_q ** i / math.factorial(i) for i in range(k))
+            want = 1.0 - math.exp(-lam) * float(lower)
+            assert abs(_poisson_sf(k, lam) - max(0.0, want)) < 1e-13, (k, lam)
+
+
 def test_evalue_rows():
     rows = _evalue_rows([5, 0], [0, 4], N=100, M=1000)
     assert rows[0]["rule_of_three"] and rows[0]["E"] == pytest.approx(0.3)
