@@ -149,7 +149,11 @@ int32_t st_align_pairs(const uint8_t* q_codes, const uint64_t* q_offsets, uint64
                        int32_t gap_open, int32_t gap_extend, uint8_t* ops, uint64_t ops_stride,
                        int64_t* score, uint32_t* n_ops, uint32_t* q_begin, uint32_t* r_begin,
                        st_status* status);
-/* Device-resident sets, pairs and outputs on `stream` (pair indices must be in range). */
+/* Device-resident sets, pairs and outputs on `stream`. Pair indices must be in range, and
+ * q_max_len / r_max_len must bound the lengths of the queries and references the pairs name:
+ * the direction scratch is sized by them. A pair longer than either bound is skipped and the
+ * call returns ST_INVALID_ARGUMENT; the check reads back one device flag, so the call
+ * synchronises `stream` once. */
 int32_t st_align_pairs_device(const uint8_t* q_codes, const uint64_t* q_offsets,
                               uint64_t n_queries, uint32_t q_max_len, const uint8_t* r_codes,
                               const uint64_t* r_offsets, uint64_t n_refs, uint32_t r_max_len,
@@ -159,6 +163,41 @@ int32_t st_align_pairs_device(const uint8_t* q_codes, const uint64_t* q_offsets,
                               uint64_t ops_stride, int64_t* score, uint32_t* n_ops,
                               uint32_t* q_begin, uint32_t* r_begin, void* stream,
                               st_status* status);
+
+/* ---- penalty-space global alignment with traceback (src/index.cpp:105-179) -----------
+ * Replaces the compute of seqtree::Index::align (src/index.cpp:105-179, declared at
+ * include/seqtree/seqtree.hpp:118-120) for a list of pairs: the global alignment of query
+ * pair_query[p] against reference pair_ref[p] that minimises substitution penalty plus affine
+ * gap cost (a run of L gap columns costs gap_open + (L - 1) gap_extend; an X<->Y switch opens a
+ * new run), with the reference's tie order (the minimum over the M, X, Y states keeps the first
+ * of equal values) and its int32 score. pen is the alphabet_size x alphabet_size penalty table
+ * indexed [query code][reference code] (SubstitutionMatrix::penalty), or NULL for unit cost (0
+ * for equal codes, 1 otherwise). Outputs per pair: score[p], the ops in forward order at
+ * ops + p * ops_stride ('M' equal codes, 'S' substitution, 'D' query residue against a gap,
+ * 'I' reference residue against a gap; ops_stride >= q_max_len + r_max_len) and their count
+ * n_ops[p]; the alignment is global, so it starts at residue 0 of both sides. Negative gap
+ * costs are ST_INVALID_ARGUMENT with the reference's message. The results equal the
+ * reference's wherever its int32 arithmetic stays finite (every cost below INT32_MAX / 4),
+ * beyond which the reference's own walk reads outside its strings. */
+int32_t st_index_align_pairs(const uint8_t* q_codes, const uint64_t* q_offsets,
+                             uint64_t n_queries, const uint8_t* r_codes,
+                             const uint64_t* r_offsets, uint64_t n_refs,
+                             const uint32_t* pair_query, const uint32_t* pair_ref,
+                             uint64_t n_pairs, const int32_t* pen, uint32_t alphabet_size,
+                             int32_t gap_open, int32_t gap_extend, uint8_t* ops,
+                             uint64_t ops_stride, int32_t* score, uint32_t* n_ops,
+                             st_status* status);
+/* Device-resident sets, pairs and outputs on `stream`, with the length-bound contract and the
+ * one synchronisation of st_align_pairs_device; pen stays a host pointer (or NULL). */
+int32_t st_index_align_pairs_device(const uint8_t* q_codes, const uint64_t* q_offsets,
+                                    uint64_t n_queries, uint32_t q_max_len,
+                                    const uint8_t* r_codes, const uint64_t* r_offsets,
+                                    uint64_t n_refs, uint32_t r_max_len,
+                                    const uint32_t* pair_query, const uint32_t* pair_ref,
+                                    uint64_t n_pairs, const int32_t* pen, uint32_t alphabet_size,
+                                    int32_t gap_open, int32_t gap_extend, uint8_t* ops,
+                                    uint64_t ops_stride, int32_t* score, uint32_t* n_ops,
+                                    void* stream, st_status* status);
 
 /* ---- device-memory entry points ------------------------------------------------------
  * Same computations with the sequences (CSR bytes and offsets) and the output already in

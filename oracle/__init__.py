@@ -3,7 +3,8 @@
 * :class:`Oracle` wraps ``oracle/liboracle.so``, the plain-C restatement of the reference's
   all-vs-all scoring algorithms (``oracle/seqtree_oracle.c``).
 * :class:`Reference` wraps ``oracle/_ref/libseqtree_ref.so``, the reference's own C++ sources
-  (``src/{gapblock,pairwise,distance,substitution_matrix,codec}.cpp``) compiled by
+  (``src/{gapblock,pairwise,distance,substitution_matrix,codec}.cpp``, plus ``index.cpp`` and
+  the trie and engine sources it links against, for ``Index::align``) compiled by
   ``oracle/Makefile`` together with ``oracle/ref_shim.cpp``. It exists only where the
   reference tree was available at build time.
 
@@ -83,6 +84,10 @@ class Oracle:
             _c_u8p, ctypes.c_uint64, _c_u8p, ctypes.c_uint64, _c_i32p, ctypes.c_int32,
             ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64),
             _c_u8p, _c_u64p, _c_u64p, _c_u64p]
+        lib.or_index_align.restype = None
+        lib.or_index_align.argtypes = [
+            _c_u8p, ctypes.c_uint64, _c_u8p, ctypes.c_uint64, _c_i32p, ctypes.c_int32,
+            ctypes.c_int32, ctypes.c_int32, _c_i32p, _c_u8p, _c_u64p]
         lib.or_hamming.restype = ctypes.c_int32
         lib.or_hamming.argtypes = [_c_u8p, ctypes.c_uint64, _c_u8p, ctypes.c_uint64]
         lib.or_levenshtein.restype = ctypes.c_int32
@@ -152,6 +157,23 @@ class Oracle:
                                _p(ops, _c_u8p), _p(meta[0:], _c_u64p), _p(meta[1:], _c_u64p),
                                _p(meta[2:], _c_u64p))
         return score.value, bytes(ops[: int(meta[0])]).decode(), int(meta[1]), int(meta[2])
+
+    def index_align(self, q, r, pen=None, go=1, ge=1, alphabet="aa"):
+        """(penalty score, ops) of or_index_align, the reference's Index::align; pen is the
+        penalty table (None for unit cost); q and r are strings (or bytes) in `alphabet`."""
+        enc = self.encoder(alphabet)
+        qc, qo = pack([q], enc)
+        rc, ro = pack([r], enc)
+        m, n = int(qo[1]), int(ro[1])
+        tab = None if pen is None else np.ascontiguousarray(pen, dtype=np.int32)
+        ops = np.zeros(m + n + 1, np.uint8)
+        score = np.zeros(1, np.int32)
+        cnt = np.zeros(1, np.uint64)
+        self.lib.or_index_align(_p(qc, _c_u8p), m, _p(rc, _c_u8p), n,
+                                None if tab is None else _p(tab, _c_i32p),
+                                0 if tab is None else tab.shape[0], go, ge, _p(score, _c_i32p),
+                                _p(ops, _c_u8p), _p(cnt, _c_u64p))
+        return int(score[0]), bytes(ops[: int(cnt[0])]).decode()
 
     def hamming(self, a, b):
         x, xo = pack([a])
@@ -246,6 +268,12 @@ class Reference:
             ctypes.c_char_p, _c_u64p, ctypes.c_uint64, ctypes.c_char_p, _c_u64p, ctypes.c_uint64,
             _c_u32p, _c_u32p, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
             ctypes.c_int32, ctypes.c_int32, ctypes.c_int, _c_u64p, ctypes.c_char_p, ctypes.c_int]
+        lib.ref_index_align_pairs.argtypes = [
+            ctypes.c_char_p, _c_u64p, ctypes.c_uint64, ctypes.c_char_p, _c_u64p, ctypes.c_uint64,
+            _c_u32p, _c_u32p, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, _c_i32p,
+            ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int, _c_i32p,
+            ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, _c_u64p,
+            ctypes.c_char_p, ctypes.c_int]
         lib.ref_distance_matrix.argtypes = [
             ctypes.c_int, ctypes.c_char_p, _c_u64p, ctypes.c_uint64, ctypes.c_char_p, _c_u64p,
             ctypes.c_uint64, ctypes.c_int, _c_i32p, ctypes.c_char_p, ctypes.c_int]
@@ -369,6 +397,39 @@ class Reference:
         if rc:
             raise ReferenceError_(rc, err.value.decode())
         return int(total[0])
+
+    def index_align_pairs(self, q, r, pq, pr, matrix=None, go=1, ge=1, alphabet="aa",
+                          threads=1, strings=True):
+        """Index::align of (q[pq[t]], ref_id pr[t]) against Index.build(r): per pair
+        (score, aligned_query, aligned_ref, ops), or with strings=False only the scores and
+        the op counts (the CPU baseline)."""
+        k, grid, size = self._matrix(matrix)
+        qb, qo = self._csr(q)
+        rb, ro = self._csr(r)
+        pq = np.ascontiguousarray(pq, np.uint32)
+        pr = np.ascontiguousarray(pr, np.uint32)
+        P = len(pq)
+        stride = (max(map(len, q), default=0) + max(map(len, r), default=0) + 1) if strings else 0
+        bufs = [ctypes.create_string_buffer(max(1, P * stride)) for _ in range(3)] \
+            if strings else [None, None, None]
+        score = np.zeros(max(P, 1), np.int32)
+        lens = np.zeros(max(P, 1), np.uint64)
+        err = self._err()
+        rc = self.lib.ref_index_align_pairs(
+            qb, _p(qo, _c_u64p), len(q), rb, _p(ro, _c_u64p), len(r), _p(pq, _c_u32p),
+            _p(pr, _c_u32p), P, ALPHABETS[alphabet], k,
+            None if grid is None else _p(grid, _c_i32p), size, go, ge, threads,
+            _p(score, _c_i32p), bufs[0], bufs[1], bufs[2], stride, _p(lens, _c_u64p), err, 512)
+        if rc:
+            raise ReferenceError_(rc, err.value.decode())
+        if not strings:
+            return score[:P], lens[:P]
+        out = []
+        for t in range(P):
+            a, L = t * stride, int(lens[t])
+            out.append((int(score[t]), bufs[1].raw[a:a + L].decode("latin-1"),
+                        bufs[2].raw[a:a + L].decode("latin-1"), bufs[0].raw[a:a + L].decode()))
+        return out
 
     def distance_matrix(self, which, a, b, threads=1):
         ab, ao = self._csr(a)

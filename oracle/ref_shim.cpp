@@ -187,6 +187,54 @@ int ref_align_pairs(const char* qb, const uint64_t* qo, uint64_t N, const char* 
     });
 }
 
+// Index::align (src/index.cpp:105-179) for every (queries[pq[t]], refs[pr[t]]) against one
+// Index built over refs, on `threads` workers taking pairs from an atomic counter. kind -1 is
+// unit cost (a null matrix). Per pair: the penalty score, and the ops and aligned strings
+// (at most m + n bytes each, at t * stride) with their common length.
+int ref_index_align_pairs(const char* qb, const uint64_t* qo, uint64_t N, const char* rb,
+                          const uint64_t* ro, uint64_t K, const uint32_t* pq, const uint32_t* pr,
+                          uint64_t n_pairs, int alphabet, int kind, const int32_t* custom_sim,
+                          int32_t custom_size, int32_t go, int32_t ge, int threads,
+                          int32_t* score, char* ops, char* aq, char* ar, uint64_t stride,
+                          uint64_t* len, char* err, int errlen) {
+    return guarded(err, errlen, [&] {
+        (void)N;
+        auto idx = Index::build(unpack(rb, ro, K), Alphabet(alphabet));
+        SubstitutionMatrix mat;
+        const bool has = make_matrix(kind, custom_sim, custom_size, mat);
+        SearchParams p;
+        p.matrix = has ? &mat : nullptr;
+        p.gap_open = go;
+        p.gap_extend = ge;
+        std::atomic<uint64_t> next{0};
+        std::exception_ptr first;
+        std::atomic<bool> failed{false};
+        auto work = [&] {
+            try {
+                for (uint64_t t; !failed && (t = next.fetch_add(1)) < n_pairs;) {
+                    const uint32_t a = pq[t];
+                    const Alignment al =
+                        idx->align(std::string_view(qb + qo[a], qo[a + 1] - qo[a]), pr[t], p);
+                    score[t] = al.score;
+                    len[t] = al.ops.size();
+                    if (ops) {
+                        std::memcpy(ops + t * stride, al.ops.data(), al.ops.size());
+                        std::memcpy(aq + t * stride, al.aligned_query.data(), al.ops.size());
+                        std::memcpy(ar + t * stride, al.aligned_ref.data(), al.ops.size());
+                    }
+                }
+            } catch (...) {
+                if (!failed.exchange(true)) first = std::current_exception();
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int w = 1; w < threads; ++w) pool.emplace_back(work);
+        work();
+        for (auto& th : pool) th.join();
+        if (first) std::rethrow_exception(first);
+    });
+}
+
 int ref_distance_matrix(int which, const char* ab, const uint64_t* ao, uint64_t N, const char* bb,
                         const uint64_t* bo, uint64_t K, int threads, int32_t* out, char* err,
                         int errlen) {

@@ -277,6 +277,95 @@ void or_align_pair(const uint8_t* q, uint64_t m, const uint8_t* r, uint64_t n,
 }
 
 /* ---------------------------------------------------------------------------------------
+ * Penalty-space global alignment with traceback (reference src/index.cpp:105-179,
+ * Index::align): minimises substitution penalty plus affine gap cost, a gap run of length L
+ * costing go + (L - 1) ge and an X<->Y switch opening a new run. Three full int32 matrices
+ * (M both residues, X query residue against a gap, Y reference residue against a gap),
+ * every cell not reachable from (0, 0) held at kInf = INT32_MAX / 4, and per state the
+ * predecessor state; the minimum over (M, X, Y) keeps the first of equal values (M, then X,
+ * then Y, replacing only on a strictly smaller one). pen is the A x A penalty table indexed
+ * [query code][reference code], or NULL for unit cost (0 equal codes, 1 otherwise). The
+ * arithmetic wraps like the reference's int32 code; the walk never leaves the matrix (the
+ * reference's walk does only for gap costs near kInf, where it reads outside its strings).
+ * Writes the penalty score, the ops in forward order ('M' equal codes, 'S', 'D' query
+ * residue against a gap, 'I' reference residue against a gap) and their count.
+ * ------------------------------------------------------------------------------------- */
+static int32_t wrap_add(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+
+static int32_t or_least(int32_t vm, int32_t vx, int32_t vy, uint8_t* state) {
+    int32_t v = vm;
+    *state = OR_M;
+    if (vx < v) { v = vx; *state = OR_X; }
+    if (vy < v) { v = vy; *state = OR_Y; }
+    return v;
+}
+
+void or_index_align(const uint8_t* q, uint64_t m, const uint8_t* r, uint64_t n,
+                    const int32_t* pen, int32_t A, int32_t go, int32_t ge, int32_t* score,
+                    uint8_t* ops, uint64_t* n_ops) {
+    const int32_t INF = INT32_MAX / 4;
+    const uint64_t W = n + 1, cells = (m + 1) * W;
+    /* cell k of state s: val[3 k + s], from[3 k + s] */
+    int32_t* val = (int32_t*)malloc(sizeof(int32_t) * 3 * cells);
+    uint8_t* from = (uint8_t*)calloc(3 * cells, 1);
+    for (uint64_t k = 0; k < 3 * cells; ++k) val[k] = INF;
+    val[0] = 0;  /* M at (0, 0) */
+    for (uint64_t i = 1; i <= m; ++i) {  /* column 0: a leading run of X */
+        val[3 * (i * W) + OR_X] = wrap_add(go, (int32_t)((uint32_t)(i - 1) * (uint32_t)ge));
+        from[3 * (i * W) + OR_X] = i > 1 ? OR_X : OR_M;
+    }
+    for (uint64_t j = 1; j <= n; ++j) {  /* row 0: a leading run of Y */
+        val[3 * j + OR_Y] = wrap_add(go, (int32_t)((uint32_t)(j - 1) * (uint32_t)ge));
+        from[3 * j + OR_Y] = j > 1 ? OR_Y : OR_M;
+    }
+    for (uint64_t i = 1; i <= m; ++i)
+        for (uint64_t j = 1; j <= n; ++j) {
+            const int32_t* d = val + 3 * ((i - 1) * W + (j - 1));
+            const int32_t* u = val + 3 * ((i - 1) * W + j);
+            const int32_t* l = val + 3 * (i * W + (j - 1));
+            int32_t* c = val + 3 * (i * W + j);
+            uint8_t* f = from + 3 * (i * W + j);
+            const uint8_t a = q[i - 1], b = r[j - 1];
+            const int32_t sub = pen ? pen[(size_t)a * (size_t)A + b] : (a != b);
+            c[OR_M] = wrap_add(sub, or_least(d[OR_M], d[OR_X], d[OR_Y], &f[OR_M]));
+            c[OR_X] = or_least(wrap_add(u[OR_M], go), wrap_add(u[OR_X], ge),
+                               wrap_add(u[OR_Y], go), &f[OR_X]);
+            c[OR_Y] = or_least(wrap_add(l[OR_M], go), wrap_add(l[OR_X], go),
+                               wrap_add(l[OR_Y], ge), &f[OR_Y]);
+        }
+    uint8_t st;
+    const int32_t* end = val + 3 * (m * W + n);
+    *score = or_least(end[OR_M], end[OR_X], end[OR_Y], &st);
+    uint64_t i = m, j = n, cnt = 0;
+    uint8_t* rev = (uint8_t*)malloc(m + n + 1);
+    while (i > 0 || j > 0) {
+        const uint8_t* f = from + 3 * (i * W + j);
+        if (st == OR_M) {
+            if (i == 0 || j == 0) break;
+            rev[cnt++] = q[i - 1] == r[j - 1] ? 'M' : 'S';
+            st = f[OR_M];
+            --i;
+            --j;
+        } else if (st == OR_X) {
+            if (i == 0) break;
+            rev[cnt++] = 'D';
+            st = f[OR_X];
+            --i;
+        } else {
+            if (j == 0) break;
+            rev[cnt++] = 'I';
+            st = f[OR_Y];
+            --j;
+        }
+    }
+    for (uint64_t t = 0; t < cnt; ++t) ops[t] = rev[cnt - 1 - t];
+    *n_ops = cnt;
+    free(rev);
+    free(from);
+    free(val);
+}
+
+/* ---------------------------------------------------------------------------------------
  * Unit-cost distances (reference src/distance.cpp:68-95).
  * ------------------------------------------------------------------------------------- */
 int32_t or_hamming(const uint8_t* a, uint64_t m, const uint8_t* b, uint64_t n) {

@@ -4,12 +4,17 @@ A drop-in for the reference's dense-matrix API (seqtree 0.6.0, ``python/seqtree/
 hot-path re-exports): ``score_matrix`` and the gap priors (:mod:`seqtree_amd.gapblock`), Gotoh
 ``score_matrix`` / ``dist_matrix`` (:mod:`seqtree_amd.pairwise`) and ``hamming_matrix`` /
 ``levenshtein_matrix`` (:mod:`seqtree_amd.distance`), plus Gotoh alignment with traceback
-(``pairwise.align``, ``pairwise.align_pairs``; ``Alignment``) and control-calibrated E-values
+(``pairwise.align``, ``pairwise.align_pairs``; ``Alignment``), the penalty-space
+``Index.align`` of a query to a stored reference (``Index``, ``SearchParams``; batched as
+``Index.align_batch``) and control-calibrated E-values
 over exhaustive gap-block balls (:mod:`seqtree_amd.evalue`). Results come back as the same
 zero-copy ``ScoreMatrix``. The scoring runs in hand-written HIP kernels; importing this package loads
 the native library and fails if it has not been built.
 """
-from ._core import Alignment, ScoreMatrix, SubstitutionMatrix, alphabet_symbols, amino_acids
+from ._core import (
+    Alignment, Index, ScoreMatrix, SearchParams, SubstitutionMatrix, alphabet_symbols,
+    amino_acids,
+)
 from ._core import device_count, version
 from . import distance, evalue, gapblock, pairwise
 from .gapblock import (
@@ -23,7 +28,9 @@ __all__ = [
     "gapblock",
     "pairwise",
     "Alignment",
+    "Index",
     "ScoreMatrix",
+    "SearchParams",
     "SubstitutionMatrix",
     "alphabet_symbols",
     "amino_acids",

@@ -1031,19 +1031,30 @@ void pf_encode_forward(const __bf16* B, const float* P, const uint8_t* codes, co
 // scratch; the grid is sized so that scratch stays under 1 GiB. Lists of 4,096 pairs or more
 // are processed in shape order (a counting sort by lengths clamped to 31), so a wave's lanes
 // run the same number of cells; every output still goes to the pair's own index.
+//
+// The penalty form (Index::align, src/index.cpp:105-179) runs the same kernels in global mode
+// on the negated penalty table (sim_h holds -penalty) with minus the reference's int32 kInf as
+// the int64 cells' -inf, and writes the penalty as int32 to score32 instead of score.
+//
+// Each pair's lengths are checked against q_max / r_max on the device: a longer pair is
+// skipped and the call fails with ST_INVALID_ARGUMENT after one stream synchronisation (the
+// direction scratch is sized by the bounds).
 void traceback_device(const uint8_t* q, const uint64_t* q_off, uint32_t q_max, const uint8_t* r,
                       const uint64_t* r_off, uint32_t r_max, const uint32_t* pq,
                       const uint32_t* pr, uint64_t n_pairs, const int32_t* sim_h, uint32_t A,
                       int32_t local, int32_t go, int32_t ge, uint8_t* ops, uint64_t ops_stride,
                       int64_t* score, uint32_t* n_ops, uint32_t* q_begin, uint32_t* r_begin,
-                      hipStream_t s) {
+                      hipStream_t s, int32_t* score32 = nullptr) {
     if (go < 0 || ge < 0)
         throw HipError{ST_INVALID_ARGUMENT, "seqtree: gap_open and gap_extend must be >= 0"};
     if (ops_stride < uint64_t(q_max) + r_max)
         throw HipError{ST_INVALID_ARGUMENT,
                        "seqtree_hip: ops_stride is below q_max_len + r_max_len"};
     if (n_pairs == 0) return;
+    const bool penalty = score32 != nullptr;
     Arena ar(s);
+    uint32_t* too_long = ar.get<uint32_t>(1);
+    launch_fill_u32(too_long, 1, 0u, s);
     const DeviceTable sim = table_on_device(ar, sim_h, size_t(A) * A, 0, s);
     const int64_t mx = std::max<int64_t>({sim.abs_max, int64_t(go), int64_t(ge)});
     const bool wide = (int64_t(q_max) + r_max + 2) * mx >= (int64_t(1) << 28);
@@ -1080,6 +1091,9 @@ void traceback_device(const uint8_t* q, const uint64_t* q_off, uint32_t q_max, c
     a.n_ops = n_ops;
     a.q_begin = q_begin;
     a.r_begin = r_begin;
+    a.neg_inf = penalty ? -int64_t(INT32_MAX / 4) : INT64_MIN / 4;
+    a.score32 = score32;
+    a.too_long = too_long;
     if (n_pairs >= 4096 && n_pairs < (uint64_t(1) << 32)) {
         // lanes of one wave take pairs of one shape (the DP and the walk run per lane)
         const uint64_t table_n =
@@ -1093,12 +1107,30 @@ void traceback_device(const uint8_t* q, const uint64_t* q_off, uint32_t q_max, c
         launched("traceback_order");
         a.order = order;
     }
-    LaunchTimer t(std::string("traceback_") + (reg ? "reg" : wide ? "i64" : "i32") +
-                      (local ? "_local" : "_global"),
-                  s);
-    if (reg) launch_traceback_reg(a, wgs, reinterpret_cast<uint32_t*>(dirs), s);
-    else launch_traceback(a, wide, lds_rows, wgs, rows, dirs, s);
+    {
+        LaunchTimer t(std::string("traceback_") + (reg ? "reg" : wide ? "i64" : "i32") +
+                          (penalty ? "_penalty" : local ? "_local" : "_global"),
+                      s);
+        if (reg) launch_traceback_reg(a, wgs, reinterpret_cast<uint32_t*>(dirs), s);
+        else launch_traceback(a, wide, lds_rows, wgs, rows, dirs, s);
+    }
     launched("traceback");
+    uint32_t flag = 0;
+    d2h(&flag, too_long, 1, s);
+    sync(s);
+    if (flag)
+        throw HipError{ST_INVALID_ARGUMENT,
+                       "seqtree_hip: a pair is longer than q_max_len / r_max_len"};
+}
+
+// The negated penalty table of Index::align: -penalty[a][b], or -(a != b) for unit cost.
+std::vector<int32_t> negated_penalties(const int32_t* pen, uint32_t A) {
+    std::vector<int32_t> t(size_t(A) * A);
+    for (uint32_t x = 0; x < A; ++x)
+        for (uint32_t y = 0; y < A; ++y)
+            t[size_t(x) * A + y] = pen ? int32_t(0u - uint32_t(pen[size_t(x) * A + y]))
+                                       : -int32_t(x != y);
+    return t;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1169,6 +1201,58 @@ void pf_check_set(uint32_t n, uint32_t max_len) {
                                                 " residues (got " + std::to_string(max_len) + ")"};
     if (n == 0) throw HipError{ST_INVALID_ARGUMENT, "seqtree_hip: empty sequence set"};
 }
+
+// Pair indices past either set: ST_INVALID_ARGUMENT naming the first such pair.
+bool bad_pair_index(st_status* status, const uint32_t* pq, const uint32_t* pr, uint64_t n_pairs,
+                    uint64_t n_queries, uint64_t n_refs) {
+    for (uint64_t t = 0; t < n_pairs; ++t)
+        if (pq[t] >= n_queries || pr[t] >= n_refs) {
+            set_status(status, ST_INVALID_ARGUMENT,
+                       "seqtree_hip: pair %llu indexes past the query or reference set",
+                       (unsigned long long)t);
+            return true;
+        }
+    return false;
+}
+
+// Device copies and outputs of a host-memory pair traceback on a private stream: both sets,
+// the pair list, the ops, the int64 scores (or, for the penalty form, the int32 scores) and the
+// per-pair op count and first aligned residues.
+struct PairTraceback {
+    OwnedStream os;
+    Arena ar;
+    HostSet qh, rh;
+    DeviceSet qd, rd;
+    uint32_t *pq, *pr, *meta;
+    uint8_t* ops;
+    int64_t* score = nullptr;
+    int32_t* score32 = nullptr;
+    uint64_t n, stride;
+
+    PairTraceback(const uint8_t* q, const uint64_t* q_off, uint64_t nq, const uint8_t* r,
+                  const uint64_t* r_off, uint64_t nr, const uint32_t* pair_q,
+                  const uint32_t* pair_r, uint64_t n_pairs, uint64_t ops_stride, bool penalty)
+        : ar(os.get()), qh(host_set(q, q_off, nq)), rh(host_set(r, r_off, nr)),
+          qd(upload(ar, qh, os.get())), rd(upload(ar, rh, os.get())), n(n_pairs),
+          stride(ops_stride) {
+        const hipStream_t st = os.get();
+        pq = ar.get<uint32_t>(n);
+        pr = ar.get<uint32_t>(n);
+        h2d(pq, pair_q, n, st);
+        h2d(pr, pair_r, n, st);
+        ops = ar.get<uint8_t>(n * stride);
+        if (penalty) score32 = ar.get<int32_t>(n);
+        else score = ar.get<int64_t>(n);
+        meta = ar.get<uint32_t>(3 * n);
+    }
+    hipStream_t s() const { return os.get(); }
+    void fetch(uint8_t* ops_h, uint32_t* n_ops, uint32_t* q_begin, uint32_t* r_begin) {
+        d2h(ops_h, ops, n * stride, s());
+        d2h(n_ops, meta, n, s());
+        if (q_begin) d2h(q_begin, meta + n, n, s());
+        if (r_begin) d2h(r_begin, meta + 2 * n, n, s());
+    }
+};
 
 }  // namespace
 
@@ -1851,38 +1935,74 @@ int32_t st_align_pairs(const uint8_t* q_codes, const uint64_t* q_offsets, uint64
                        st_status* status) {
     if (alphabet_size == 0 || alphabet_size > kMaxAlphabet) return bad_alphabet(status, alphabet_size);
     if (n_pairs == 0) return ok(status);
-    for (uint64_t t = 0; t < n_pairs; ++t)
-        if (pair_query[t] >= n_queries || pair_ref[t] >= n_refs) {
-            set_status(status, ST_INVALID_ARGUMENT,
-                       "seqtree_hip: pair %llu indexes past the query or reference set",
-                       (unsigned long long)t);
-            return int(ST_INVALID_ARGUMENT);
-        }
+    if (bad_pair_index(status, pair_query, pair_ref, n_pairs, n_queries, n_refs))
+        return int(ST_INVALID_ARGUMENT);
     return guarded(status, [&] {
         require_device();
-        const HostSet qh = host_set(q_codes, q_offsets, n_queries);
-        const HostSet rh = host_set(r_codes, r_offsets, n_refs);
-        OwnedStream os;
-        const hipStream_t s = os.get();
+        PairTraceback t(q_codes, q_offsets, n_queries, r_codes, r_offsets, n_refs, pair_query,
+                        pair_ref, n_pairs, ops_stride, false);
+        traceback_device(t.qd.bytes, t.qd.off, t.qh.max_len, t.rd.bytes, t.rd.off, t.rh.max_len,
+                         t.pq, t.pr, n_pairs, sim, alphabet_size, local, gap_open, gap_extend,
+                         t.ops, ops_stride, t.score, t.meta, t.meta + n_pairs,
+                         t.meta + 2 * n_pairs, t.s());
+        t.fetch(ops, n_ops, q_begin, r_begin);
+        d2h(score, t.score, n_pairs, t.s());
+        sync(t.s());
+        return ok(status);
+    });
+}
+
+int32_t st_index_align_pairs_device(const uint8_t* q_codes, const uint64_t* q_offsets,
+                                    uint64_t n_queries, uint32_t q_max_len,
+                                    const uint8_t* r_codes, const uint64_t* r_offsets,
+                                    uint64_t n_refs, uint32_t r_max_len,
+                                    const uint32_t* pair_query, const uint32_t* pair_ref,
+                                    uint64_t n_pairs, const int32_t* pen, uint32_t alphabet_size,
+                                    int32_t gap_open, int32_t gap_extend, uint8_t* ops,
+                                    uint64_t ops_stride, int32_t* score, uint32_t* n_ops,
+                                    void* stream, st_status* status) {
+    if (alphabet_size == 0 || alphabet_size > kMaxAlphabet) return bad_alphabet(status, alphabet_size);
+    (void)n_queries;
+    (void)n_refs;
+    return guarded(status, [&] {
+        require_device();
+        const hipStream_t s = static_cast<hipStream_t>(stream);
         Arena ar(s);
-        const DeviceSet qd = upload(ar, qh, s);
-        const DeviceSet rd = upload(ar, rh, s);
-        uint32_t* pq = ar.get<uint32_t>(n_pairs);
-        uint32_t* pr = ar.get<uint32_t>(n_pairs);
-        h2d(pq, pair_query, n_pairs, s);
-        h2d(pr, pair_ref, n_pairs, s);
-        uint8_t* d_ops = ar.get<uint8_t>(n_pairs * ops_stride);
-        int64_t* d_score = ar.get<int64_t>(n_pairs);
-        uint32_t* d_meta = ar.get<uint32_t>(3 * n_pairs);
-        traceback_device(qd.bytes, qd.off, qh.max_len, rd.bytes, rd.off, rh.max_len, pq, pr,
-                         n_pairs, sim, alphabet_size, local, gap_open, gap_extend, d_ops,
-                         ops_stride, d_score, d_meta, d_meta + n_pairs, d_meta + 2 * n_pairs, s);
-        d2h(ops, d_ops, n_pairs * ops_stride, s);
-        d2h(score, d_score, n_pairs, s);
-        d2h(n_ops, d_meta, n_pairs, s);
-        d2h(q_begin, d_meta + n_pairs, n_pairs, s);
-        d2h(r_begin, d_meta + 2 * n_pairs, n_pairs, s);
-        sync(s);
+        // the first aligned residues are (0, 0) in a global alignment: device scratch only
+        uint32_t* begins = ar.get<uint32_t>(2 * std::max<uint64_t>(n_pairs, 1));
+        const std::vector<int32_t> neg = negated_penalties(pen, alphabet_size);
+        traceback_device(q_codes, q_offsets, q_max_len, r_codes, r_offsets, r_max_len,
+                         pair_query, pair_ref, n_pairs, neg.data(), alphabet_size, 0, gap_open,
+                         gap_extend, ops, ops_stride, nullptr, n_ops, begins, begins + n_pairs,
+                         s, score);
+        return ok(status);
+    });
+}
+
+int32_t st_index_align_pairs(const uint8_t* q_codes, const uint64_t* q_offsets,
+                             uint64_t n_queries, const uint8_t* r_codes,
+                             const uint64_t* r_offsets, uint64_t n_refs,
+                             const uint32_t* pair_query, const uint32_t* pair_ref,
+                             uint64_t n_pairs, const int32_t* pen, uint32_t alphabet_size,
+                             int32_t gap_open, int32_t gap_extend, uint8_t* ops,
+                             uint64_t ops_stride, int32_t* score, uint32_t* n_ops,
+                             st_status* status) {
+    if (alphabet_size == 0 || alphabet_size > kMaxAlphabet) return bad_alphabet(status, alphabet_size);
+    if (n_pairs == 0) return ok(status);
+    if (bad_pair_index(status, pair_query, pair_ref, n_pairs, n_queries, n_refs))
+        return int(ST_INVALID_ARGUMENT);
+    return guarded(status, [&] {
+        require_device();
+        PairTraceback t(q_codes, q_offsets, n_queries, r_codes, r_offsets, n_refs, pair_query,
+                        pair_ref, n_pairs, ops_stride, true);
+        const std::vector<int32_t> neg = negated_penalties(pen, alphabet_size);
+        traceback_device(t.qd.bytes, t.qd.off, t.qh.max_len, t.rd.bytes, t.rd.off, t.rh.max_len,
+                         t.pq, t.pr, n_pairs, neg.data(), alphabet_size, 0, gap_open, gap_extend,
+                         t.ops, ops_stride, nullptr, t.meta, t.meta + n_pairs,
+                         t.meta + 2 * n_pairs, t.s(), t.score32);
+        t.fetch(ops, n_ops, nullptr, nullptr);
+        d2h(score, t.score32, n_pairs, t.s());
+        sync(t.s());
         return ok(status);
     });
 }

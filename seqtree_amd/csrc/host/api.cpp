@@ -9,6 +9,10 @@
 //   align_pair          src/pairwise.cpp:141-267 (traceback on the device; the empty-input
 //                       case and the narrowing to int32 on the host, as the reference)
 //   hamming/levenshtein src/distance.cpp:68-109
+//   Index::build        src/index.cpp:93-97, src/trie.cpp:43-70 (stored strings, symbol check)
+//   Index::align        src/index.cpp:105-179 (checks and their order; the traceback on the
+//                       device in penalty form, the aligned strings built here from the raw
+//                       characters, as the reference does)
 #include "seqtree.hpp"
 #include "seqtree_hip.h"
 
@@ -16,6 +20,7 @@
 #include <climits>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 
 namespace seqtree {
 
@@ -442,6 +447,109 @@ std::vector<int32_t> levenshtein_matrix(const std::vector<std::string>& a,
     raise(st_levenshtein_matrix(x.bytes.data(), x.off.data(), N, y.bytes.data(), y.off.data(), K,
                                 out.data(), &st),
           st);
+    return out;
+}
+
+std::unique_ptr<Index> Index::build(std::vector<std::string> refs, Alphabet a) {
+    std::unique_ptr<Index> idx(new Index(a));
+    const Codec& codec = idx->codec_;
+    size_t total = 0;
+    for (const auto& s : refs) total += s.size();
+    idx->data_.reserve(total);
+    idx->codes_.resize(total);
+    idx->off_.assign(1, 0);
+    idx->off_.reserve(refs.size() + 1);
+    for (uint32_t id = 0; id < refs.size(); ++id) {
+        const std::string& s = refs[id];
+        for (size_t t = 0; t < s.size(); ++t) {
+            const uint8_t code = codec.encode(s[t]);
+            if (code == Codec::kInvalid)
+                throw std::invalid_argument(std::string("invalid symbol '") + s[t] +
+                                            "' in reference #" + std::to_string(id));
+            idx->codes_[idx->data_.size() + t] = code;
+        }
+        idx->data_ += s;
+        idx->off_.push_back(idx->data_.size());
+    }
+    return idx;
+}
+
+std::string_view Index::ref_seq(uint32_t ref_id) const {
+    if (ref_id >= size()) throw std::out_of_range("seqtree: ref_id out of range");
+    return std::string_view(data_.data() + off_[ref_id], off_[ref_id + 1] - off_[ref_id]);
+}
+
+Alignment Index::align(std::string_view query, uint32_t ref_id, const SearchParams& p) const {
+    return align_batch({std::string(query)}, {ref_id}, p)[0];
+}
+
+std::vector<Alignment> Index::align_batch(const std::vector<std::string>& queries,
+                                          const std::vector<uint32_t>& ref_ids,
+                                          const SearchParams& p) const {
+    if (queries.size() != ref_ids.size())
+        throw std::invalid_argument("seqtree: queries and ref_ids differ in length");
+    // the reference's order: query symbols, then gap costs (src/index.cpp:109-113)
+    const Csr q = pack(queries, [&](char ch) {
+        const uint8_t code = codec_.encode(ch);
+        if (code == Codec::kInvalid)
+            throw std::invalid_argument(std::string("invalid symbol '") + ch + "' in query");
+        return code;
+    });
+    if (p.gap_open < 0 || p.gap_extend < 0)
+        throw std::invalid_argument("seqtree: gap_open and gap_extend must be >= 0");
+    if (p.matrix && p.matrix->size() != codec_.size())
+        throw std::invalid_argument("matrix size does not match the alphabet");
+    for (uint32_t id : ref_ids)
+        if (id >= size()) throw std::out_of_range("seqtree: ref_id out of range");
+    const size_t P = queries.size();
+    std::vector<Alignment> out(P);
+    if (P == 0) return out;
+    // only the references the pairs name travel to the device, each once
+    std::unordered_map<uint32_t, uint32_t> slot;
+    std::vector<uint32_t> pair_ref(P), pair_query(P);
+    Csr r;
+    r.off.push_back(0);
+    for (size_t t = 0; t < P; ++t) {
+        pair_query[t] = uint32_t(t);
+        const uint32_t id = ref_ids[t];
+        auto it = slot.find(id);
+        if (it == slot.end()) {
+            it = slot.emplace(id, uint32_t(r.off.size() - 1)).first;
+            r.bytes.insert(r.bytes.end(), codes_.begin() + off_[id], codes_.begin() + off_[id + 1]);
+            r.off.push_back(r.bytes.size());
+            r.max_len = std::max<uint32_t>(r.max_len, uint32_t(off_[id + 1] - off_[id]));
+        }
+        pair_ref[t] = it->second;
+    }
+    if (r.bytes.empty()) r.bytes.push_back(0);  // a valid pointer for all-empty references
+    std::vector<uint8_t> qbytes = q.bytes;
+    if (qbytes.empty()) qbytes.push_back(0);
+    const uint64_t stride = uint64_t(q.max_len) + r.max_len + 1;
+    std::vector<uint8_t> ops(P * stride);
+    std::vector<int32_t> score(P);
+    std::vector<uint32_t> n_ops(P);
+    st_status st{};
+    raise(st_index_align_pairs(qbytes.data(), q.off.data(), P, r.bytes.data(), r.off.data(),
+                               r.off.size() - 1, pair_query.data(), pair_ref.data(), P,
+                               p.matrix ? p.matrix->penalties().data() : nullptr, codec_.size(),
+                               p.gap_open, p.gap_extend, ops.data(), stride, score.data(),
+                               n_ops.data(), &st),
+          st);
+    for (size_t t = 0; t < P; ++t) {
+        const std::string& query = queries[t];
+        const std::string_view ref = ref_seq(ref_ids[t]);
+        Alignment& al = out[t];
+        al.score = score[t];
+        al.ops.assign(reinterpret_cast<const char*>(ops.data() + t * stride), n_ops[t]);
+        al.aligned_query.reserve(n_ops[t]);
+        al.aligned_ref.reserve(n_ops[t]);
+        size_t i = 0, j = 0;
+        for (const char o : al.ops) {
+            const bool takes_q = o != 'I', takes_r = o != 'D';
+            al.aligned_query += takes_q ? query[i++] : '-';
+            al.aligned_ref += takes_r ? ref[j++] : '-';
+        }
+    }
     return out;
 }
 

@@ -1,6 +1,8 @@
 // pybind11 module `_core`: the reference's hot-path bindings (src/_bindings.cpp:34-40
 // parse_alphabet, 203-233 ScoreMatrix and gapblock_matrix, 240-315 SubstitutionMatrix,
-// 346-350 alphabet helpers, 470-615 ScoreMatrix buffer protocol and the matrix functions)
+// 346-350 alphabet helpers, 470-615 ScoreMatrix buffer protocol and the matrix functions;
+// 42-55, 64-121, 197-201, 352-397, 420-455 SearchParams and Index.align, without the trie
+// search)
 // with the same names, argument names, defaults and error types, over the GPU C++ API.
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
@@ -61,6 +63,92 @@ uint8_t residue(const SubstitutionMatrix& m, const std::string& a, const char* w
     if (i == std::string::npos) throw py::value_error("unknown amino acid; expected one of " + aa);
     if (i >= m.size()) throw py::value_error("residue out of range for this matrix's alphabet");
     return uint8_t(i);
+}
+
+Engine parse_engine(const std::string& e) {
+    const std::string l = lower(e);
+    if (l == "auto") return Engine::Auto;
+    if (l == "seqtrie") return Engine::SeqTrie;
+    if (l == "seqtm") return Engine::SeqTm;
+    throw py::value_error("unknown engine '" + e + "' (use 'auto', 'seqtrie', or 'seqtm')");
+}
+
+Mode parse_mode(const std::string& m) {
+    const std::string l = lower(m);
+    if (l == "all") return Mode::AllHits;
+    if (l == "top") return Mode::TopHit;
+    throw py::value_error("unknown mode '" + m + "' (use 'all' or 'top')");
+}
+
+constexpr const char* kMatrixNames =
+    "'identity', 'BLOSUM62', 'BLOSUM45', 'BLOSUM80', 'PAM250', 'PAM100', 'structural'";
+
+bool is_matrix_name(const std::string& l) {
+    static const char* names[] = {"",       "identity", "blosum62", "blosum45",
+                                  "blosum80", "pam250",  "pam100",   "structural"};
+    return std::any_of(std::begin(names), std::end(names), [&](const char* n) { return l == n; });
+}
+
+// The Python-side search parameters: a matrix given by name or as an object.
+struct PyParams {
+    int max_subs = 0, max_ins = 0, max_dels = 0, max_total_edits = 0;
+    long max_penalty = 0;
+    std::string matrix;                             // a built-in name ("" = unit cost)
+    std::optional<SubstitutionMatrix> matrix_obj;   // or an explicit matrix, which wins
+    int gap_open = 1, gap_extend = 1;
+    std::string engine = "auto", mode = "all";
+};
+
+void set_matrix(PyParams& p, const py::object& m) {
+    p.matrix.clear();
+    p.matrix_obj.reset();
+    if (m.is_none()) return;
+    if (py::isinstance<py::str>(m)) {
+        std::string name = m.cast<std::string>();
+        if (!is_matrix_name(lower(name)))
+            throw py::value_error("unknown matrix '" + name + "' (use '', " + kMatrixNames +
+                                  ", or a SubstitutionMatrix)");
+        p.matrix = std::move(name);
+    } else if (py::isinstance<SubstitutionMatrix>(m)) {
+        p.matrix_obj = m.cast<SubstitutionMatrix>();
+    } else {
+        throw py::type_error("matrix must be a name string or a SubstitutionMatrix");
+    }
+}
+
+// The matrix a parameter set names for an alphabet; nullopt is unit cost.
+std::optional<SubstitutionMatrix> params_matrix(const PyParams& p, Alphabet a) {
+    const uint8_t size = Codec(a).size();
+    if (p.matrix_obj) {
+        if (p.matrix_obj->size() != size)
+            throw py::value_error("matrix size does not match the alphabet");
+        return p.matrix_obj;
+    }
+    if (p.matrix.empty()) return std::nullopt;
+    const std::string l = lower(p.matrix);
+    if (l == "identity") return SubstitutionMatrix::unit(size);
+    if (a != Alphabet::AminoAcid) throw py::value_error(l + " requires the amino-acid alphabet");
+    if (l == "blosum62") return SubstitutionMatrix::blosum62();
+    if (l == "blosum45") return SubstitutionMatrix::blosum45();
+    if (l == "blosum80") return SubstitutionMatrix::blosum80();
+    if (l == "pam250") return SubstitutionMatrix::pam250();
+    if (l == "pam100") return SubstitutionMatrix::pam100();
+    return SubstitutionMatrix::structural();
+}
+
+SearchParams to_cpp(const PyParams& pp, const SubstitutionMatrix* mat) {
+    SearchParams p;
+    p.engine = parse_engine(pp.engine);
+    p.mode = parse_mode(pp.mode);
+    p.max_substitutions = uint16_t(pp.max_subs);
+    p.max_insertions = uint16_t(pp.max_ins);
+    p.max_deletions = uint16_t(pp.max_dels);
+    p.max_total_edits = uint16_t(pp.max_total_edits);
+    p.max_score_penalty = int32_t(pp.max_penalty);
+    p.matrix = mat;
+    p.gap_open = pp.gap_open;
+    p.gap_extend = pp.gap_extend;
+    return p;
 }
 
 template <class F>
@@ -176,7 +264,8 @@ PYBIND11_MODULE(_core, m) {
         "Optimal Gotoh similarity score ('global' Needleman-Wunsch, 'local' Smith-Waterman).");
 
     py::class_<Alignment>(m, "Alignment",
-                          "An alignment: aligned strings, ops and the similarity score.")
+                          "An alignment: aligned strings, ops and the score (the similarity for "
+                          "align_pair / align_pairs, the penalty for Index.align).")
         .def_readonly("aligned_query", &Alignment::aligned_query)
         .def_readonly("aligned_ref", &Alignment::aligned_ref)
         .def_readonly("ops", &Alignment::ops)
@@ -373,6 +462,98 @@ PYBIND11_MODULE(_core, m) {
         },
         py::arg("a"), py::arg("b"), py::arg("threads") = 0,
         "Dense (len(a), len(b)) int32 Levenshtein matrix, GIL released.");
+
+    py::class_<PyParams>(m, "SearchParams",
+                         "Scope and budget of the reference's search, and the scoring of "
+                         "Index.align: matrix (a built-in name, '' for unit cost, or a "
+                         "SubstitutionMatrix), gap_open and gap_extend. engine is "
+                         "'auto'|'seqtrie'|'seqtm', mode is 'all'|'top'.")
+        .def(py::init([](int max_subs, int max_ins, int max_dels, int max_total_edits,
+                         long max_penalty, py::object matrix, int gap_open, int gap_extend,
+                         std::string engine, std::string mode) {
+                 PyParams p;
+                 p.max_subs = max_subs;
+                 p.max_ins = max_ins;
+                 p.max_dels = max_dels;
+                 p.max_total_edits = max_total_edits;
+                 p.max_penalty = max_penalty;
+                 set_matrix(p, matrix);
+                 p.gap_open = gap_open;
+                 p.gap_extend = gap_extend;
+                 parse_engine(engine);  // rejected here, as the reference does
+                 parse_mode(mode);
+                 p.engine = std::move(engine);
+                 p.mode = std::move(mode);
+                 return p;
+             }),
+             py::arg("max_subs") = 0, py::arg("max_ins") = 0, py::arg("max_dels") = 0,
+             py::arg("max_total_edits") = 0, py::arg("max_penalty") = 0,
+             py::arg("matrix") = "", py::arg("gap_open") = 1, py::arg("gap_extend") = 1,
+             py::arg("engine") = "auto", py::arg("mode") = "all")
+        .def_readwrite("max_subs", &PyParams::max_subs)
+        .def_readwrite("max_ins", &PyParams::max_ins)
+        .def_readwrite("max_dels", &PyParams::max_dels)
+        .def_readwrite("max_total_edits", &PyParams::max_total_edits)
+        .def_readwrite("max_penalty", &PyParams::max_penalty)
+        .def_property(
+            "matrix",
+            [](const PyParams& p) -> py::object {
+                if (p.matrix_obj) return py::cast(*p.matrix_obj);
+                return py::cast(p.matrix);
+            },
+            [](PyParams& p, const py::object& v) { set_matrix(p, v); })
+        .def_readwrite("gap_open", &PyParams::gap_open)
+        .def_readwrite("gap_extend", &PyParams::gap_extend)
+        .def_property(
+            "engine", [](const PyParams& p) { return p.engine; },
+            [](PyParams& p, std::string v) {
+                parse_engine(v);
+                p.engine = std::move(v);
+            })
+        .def_property(
+            "mode", [](const PyParams& p) { return p.mode; },
+            [](PyParams& p, std::string v) {
+                parse_mode(v);
+                p.mode = std::move(v);
+            });
+
+    py::class_<Index>(m, "Index",
+                      "A reference set addressed by position (ref_id = index in ``refs``), "
+                      "with on-demand global alignment of a query to any reference on the GPU.")
+        .def_static(
+            "build",
+            [](std::vector<std::string> refs, const std::string& alphabet) {
+                return Index::build(std::move(refs), parse_alphabet(alphabet));
+            },
+            py::arg("refs"), py::arg("alphabet") = "aa",
+            "Store the references. ``alphabet`` is 'aa', 'nt', or 'iupac'. Raises ValueError "
+            "on a symbol outside the alphabet.")
+        .def("__len__", &Index::size)
+        .def(
+            "ref_seq", [](const Index& i, uint32_t id) { return std::string(i.ref_seq(id)); },
+            py::arg("ref_id"), "The reference sequence string of a reference id.")
+        .def(
+            "align",
+            [](const Index& idx, uint32_t ref_id, const std::string& q, const PyParams& pp) {
+                const auto mat = params_matrix(pp, idx.alphabet());
+                const SearchParams cp = to_cpp(pp, mat ? &*mat : nullptr);
+                py::gil_scoped_release release;
+                return idx.align(q, ref_id, cp);
+            },
+            py::arg("ref_id"), py::arg("query"), py::arg("params"),
+            "Minimum-penalty global alignment of ``query`` to a reference (score is the "
+            "penalty: substitution penalties plus gap_open + (L-1)*gap_extend per gap run).")
+        .def(
+            "align_batch",
+            [](const Index& idx, const std::vector<uint32_t>& ref_ids,
+               const std::vector<std::string>& queries, const PyParams& pp) {
+                const auto mat = params_matrix(pp, idx.alphabet());
+                const SearchParams cp = to_cpp(pp, mat ? &*mat : nullptr);
+                py::gil_scoped_release release;
+                return idx.align_batch(queries, ref_ids, cp);
+            },
+            py::arg("ref_ids"), py::arg("queries"), py::arg("params"),
+            "align() of every (ref_ids[t], queries[t]) in one device batch.");
 
     m.def("version", [] { return std::string(st_version()); }, "Native library version.");
     m.def("device_count", [] { return st_device_count(); }, "Visible HIP devices.");

@@ -1,7 +1,8 @@
 // C++ API of the MI355X scoring path: the same declarations, argument meaning, validation
 // order and exceptions as the reference's include/seqtree/seqtree.hpp (Codec :15-31,
 // SubstitutionMatrix :43-75, AlignMode :186, the scalar and matrix functions :188-242) and
-// include/seqtree/types.hpp:9 (Alphabet), restricted to the all-vs-all scoring path.
+// include/seqtree/types.hpp:9 (Alphabet), restricted to the all-vs-all scoring path, plus
+// Index::align (seqtree.hpp:108-120, types.hpp:31-54 SearchParams) without the trie search.
 //
 // The dense-matrix functions run on the GPU through the C-ABI in seqtree_hip.h. Their
 // `threads` argument is accepted for signature compatibility and ignored: the host does only
@@ -9,6 +10,7 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <string_view>
 #include <vector>
@@ -150,5 +152,58 @@ std::vector<int32_t> hamming_matrix(const std::vector<std::string>& a,
                                     const std::vector<std::string>& b, int threads = 0);
 std::vector<int32_t> levenshtein_matrix(const std::vector<std::string>& a,
                                         const std::vector<std::string>& b, int threads = 0);
+
+// Search scope and budget (reference include/seqtree/types.hpp:23-54), kept field for field
+// so code that fills one compiles unchanged. Index::align reads the scoring fields (matrix,
+// gap_open, gap_extend); the search scope and budget fields belong to the reference's trie
+// search, which this library does not provide.
+enum class Engine : uint8_t { Auto, SeqTrie, SeqTm };
+enum class Mode : uint8_t { AllHits, TopHit };
+struct SearchParams {
+    Engine engine = Engine::Auto;
+    Mode mode = Mode::AllHits;
+    uint16_t max_substitutions = 0;
+    uint16_t max_insertions = 0;
+    uint16_t max_deletions = 0;
+    uint16_t max_total_edits = 0;
+    int32_t max_score_penalty = 0;
+    const SubstitutionMatrix* matrix = nullptr;  // null => unit cost
+    int32_t gap_open = 1;
+    int32_t gap_extend = 1;
+    uint32_t max_hits = 0;
+};
+
+// A reference set addressed by position (reference seqtree.hpp:108-142): build() validates and
+// stores the references, align() is the reference's on-demand global alignment of a query to
+// one of them (src/index.cpp:105-179), on the GPU. align_batch() traces back many
+// (query, reference id) pairs in one device batch (no reference counterpart). The trie, its
+// search and its serialisation are not part of this library.
+class Index {
+public:
+    // Throws std::invalid_argument("invalid symbol 'c' in reference #id") (src/trie.cpp:60-66).
+    static std::unique_ptr<Index> build(std::vector<std::string> refs, Alphabet);
+
+    uint32_t size() const { return uint32_t(off_.size() - 1); }
+    Alphabet alphabet() const { return codec_.alphabet(); }
+    const Codec& codec() const { return codec_; }
+    std::string_view ref_seq(uint32_t ref_id) const;
+
+    // Minimum-penalty global alignment with affine gaps: substitution penalties from
+    // p.matrix (unit cost when null), gap runs costing gap_open + (L - 1) * gap_extend.
+    // Alignment::score is that penalty. Throws std::invalid_argument for a query symbol
+    // outside the alphabet ("invalid symbol 'c' in query"), then for negative gap costs, and
+    // std::out_of_range for a ref_id past the set.
+    Alignment align(std::string_view query, uint32_t ref_id, const SearchParams& p) const;
+    std::vector<Alignment> align_batch(const std::vector<std::string>& queries,
+                                       const std::vector<uint32_t>& ref_ids,
+                                       const SearchParams& p) const;
+
+private:
+    explicit Index(Alphabet a) : codec_(a) {}
+    Codec codec_;
+    std::string data_;             // the references as given, back to back
+    std::vector<uint64_t> off_;    // size() + 1 offsets into data_ (and codes_)
+    std::vector<uint8_t> codes_;   // their codec codes
+};
 
 }  // namespace seqtree

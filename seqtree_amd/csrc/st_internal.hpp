@@ -340,6 +340,15 @@ struct TracebackArgs {
     uint32_t* q_begin;
     uint32_t* r_begin;
     const uint32_t* order;  // processing order of the pairs (a permutation), or nullptr
+    // minus infinity of the int64 cells: INT64_MIN / 4 for align_pair's similarity DP,
+    // -(INT32_MAX / 4) for the penalty form (the negation of Index::align's int32 kInf)
+    int64_t neg_inf;
+    // penalty form (Index::align, global only; sim holds the negated penalties): the pair's
+    // penalty -score goes to score32[p] as int32 and score is not written
+    int32_t* score32;
+    // device flag, set to 1 by a pair longer than q_max or r_max (the pair is skipped: no ops,
+    // score 0)
+    uint32_t* too_long;
 };
 // Longest reference whose DP rows fit the LDS (int32 cells).
 constexpr uint32_t kTracebackLdsMaxLen = 64;

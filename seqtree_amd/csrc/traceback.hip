@@ -15,6 +15,18 @@
 // T is int32 when the host's range bound holds (every value far inside int32, the sentinel
 // -2^29 and a few gap steps below it included, so every comparison and tie is the int64
 // one's) and int64 otherwise, with the reference's own -inf = INT64_MIN / 4.
+//
+// The same kernels run the penalty form of src/index.cpp:105-179 (`Index::align`): a global
+// alignment minimising substitution penalty plus affine gap cost, with its minimum over
+// (M, X, Y) keeping the first of equal values. Negating the penalties turns that minimum into
+// this maximum with the same state order and the same strict comparisons, and the boundary
+// back-pointers are the same, so the host passes sim = -penalty, and the kernels write -score.
+// The int64 cells then use minus the reference's int32 kInf (INT32_MAX / 4) as -inf, which
+// makes every int64 sum the reference's int32 one wherever the latter does not overflow.
+//
+// A pair longer than the host's bounds (q_max, r_max) would overrun the per-workgroup
+// direction scratch; it is skipped and raises the too_long flag, which the host turns into an
+// error.
 
 #include "st_internal.hpp"
 
@@ -56,6 +68,23 @@ __device__ __forceinline__ T gap_run(uint32_t L, int32_t go, int32_t ge) {
     return L == 0 ? T(0) : T(go) + T(L - 1) * T(ge);
 }
 
+// One pair's outputs: the similarity score (int64), or in the penalty form the penalty
+// -score (int32, as the reference's int32 result).
+__device__ __forceinline__ void put_result(const TracebackArgs& a, uint64_t p, int64_t score,
+                                           uint32_t cnt, uint32_t i, uint32_t j) {
+    if (a.score32) a.score32[p] = int32_t(-score);
+    else a.score[p] = score;
+    a.n_ops[p] = cnt;
+    a.q_begin[p] = i;
+    a.r_begin[p] = j;
+}
+
+// A pair outside the host's length bounds: flagged for the host, outputs left defined.
+__device__ __forceinline__ void skip_pair(const TracebackArgs& a, uint64_t p) {
+    atomicOr(a.too_long, 1u);
+    put_result(a, p, 0, 0, 0, 0);
+}
+
 // Dynamic LDS: the A x A similarity table, then (LDS_ROWS) the lane's reference codes
 // [W][64] and the three DP rows [3][W][64]; otherwise rows in `rows_g` (3 * W * 64 T per
 // workgroup) and codes read from the CSR set. dirs: (q_max + 1) * W * 64 bytes per workgroup;
@@ -78,7 +107,9 @@ __global__ __launch_bounds__(kTbLanes) void traceback_kernel(
     auto D = [&](uint32_t i, uint32_t j) -> uint8_t& {
         return dirs[(uint64_t(i) * W + j) * kTbLanes + lane];
     };
-    const T NEG = NegInf<T>::v;
+    T NEG;
+    if constexpr (sizeof(T) == 8) NEG = T(a.neg_inf);
+    else NEG = NegInf<T>::v;
     const int32_t go = a.gap_open, ge = a.gap_extend;
     const uint64_t stride = uint64_t(gridDim.x) * kTbLanes;
 
@@ -86,7 +117,12 @@ __global__ __launch_bounds__(kTbLanes) void traceback_kernel(
         const uint64_t p = a.order ? a.order[t] : t;  // shape order: lanes finish together
         const uint32_t qi = a.pair_q[p], ri = a.pair_r[p];
         const uint64_t q0 = a.q_off[qi], r0 = a.r_off[ri];
-        const uint32_t m = uint32_t(a.q_off[qi + 1] - q0), n = uint32_t(a.r_off[ri + 1] - r0);
+        const uint64_t m64 = a.q_off[qi + 1] - q0, n64 = a.r_off[ri + 1] - r0;
+        if (m64 > a.q_max || n64 > a.r_max) {
+            skip_pair(a, p);
+            continue;
+        }
+        const uint32_t m = uint32_t(m64), n = uint32_t(n64);
         const uint8_t* q = a.q_codes + q0;
         const uint8_t* r = a.r_codes + r0;
         uint8_t* ops = a.ops + p * a.ops_stride;
@@ -195,10 +231,7 @@ __global__ __launch_bounds__(kTbLanes) void traceback_kernel(
                 ops[cnt - 1 - t] = x;
             }
         }
-        a.score[p] = int64_t(score);
-        a.n_ops[p] = cnt;
-        a.q_begin[p] = i;
-        a.r_begin[p] = j;
+        put_result(a, p, int64_t(score), cnt, i, j);
     }
 }
 
@@ -206,7 +239,7 @@ __global__ __launch_bounds__(kTbLanes) void traceback_kernel(
 // at most kTbRegMaxQ: the three DP rows live in registers (columns 0..kTbRegCols, unrolled,
 // so every index is a constant) and the lane's reference codes in eight packed registers. A
 // row stops at the first 4-column group no lane of the wave reaches. The direction bytes of
-// four consecutive columns go out as one dword (dwords [row][(r_max + 4) / 4][64 lanes] per
+// four consecutive columns go out as one dword (dwords [row][kTbRegGroups][64 lanes] per
 // workgroup), and the ops are staged in LDS ([q_max + r_max][64 lanes]) and written forward
 // once the walk has ended. Same recurrences, argmax order, boundary bytes and traceback rules
 // as traceback_kernel. Held to three waves per SIMD (168 VGPRs, a few spilled outside the
@@ -236,7 +269,12 @@ __global__ __launch_bounds__(kTbLanes) __attribute__((amdgpu_waves_per_eu(3, 8))
         const uint64_t p = a.order ? a.order[t] : t;
         const uint32_t qi = a.pair_q[p], ri = a.pair_r[p];
         const uint64_t q0 = a.q_off[qi], r0 = a.r_off[ri];
-        const uint32_t m = uint32_t(a.q_off[qi + 1] - q0), n = uint32_t(a.r_off[ri + 1] - r0);
+        const uint64_t m64 = a.q_off[qi + 1] - q0, n64 = a.r_off[ri + 1] - r0;
+        if (m64 > a.q_max || n64 > a.r_max) {  // r_max <= kTbRegCols, q_max <= kTbRegMaxQ
+            skip_pair(a, p);
+            continue;
+        }
+        const uint32_t m = uint32_t(m64), n = uint32_t(n64);
         const uint8_t* q = a.q_codes + q0;
         const uint8_t* r = a.r_codes + r0;
         uint8_t* ops = a.ops + p * a.ops_stride;
@@ -380,10 +418,7 @@ __global__ __launch_bounds__(kTbLanes) __attribute__((amdgpu_waves_per_eu(3, 8))
             }
             for (uint32_t t2 = 0; t2 < cnt; ++t2) ops[t2] = ops_l[(cnt - 1 - t2) * kTbLanes + lane];
         }
-        a.score[p] = int64_t(score);
-        a.n_ops[p] = cnt;
-        a.q_begin[p] = i;
-        a.r_begin[p] = j;
+        put_result(a, p, int64_t(score), cnt, i, j);
     }
 }
 

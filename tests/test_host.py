@@ -301,6 +301,44 @@ def test_compute_without_a_device_fails_loudly(st):
         st.distance.levenshtein_matrix(["A"], ["B"])
 
 
+def test_index_and_search_params_validation(st):
+    """Index.build / Index.align / SearchParams raise as the reference's binding does
+    (src/trie.cpp:60-66, src/index.cpp:109-113, src/_bindings.cpp:42-121, 352-397), before
+    any device work."""
+    with pytest.raises(ValueError, match=r"invalid symbol '1' in reference #1"):
+        st.Index.build(["CASS", "CA1S"])
+    idx = st.Index.build(["CASSLGQAYEQYF", "CASSPGTGVYGYTF"], "aa")
+    assert len(idx) == 2 and idx.ref_seq(1) == "CASSPGTGVYGYTF"
+    with pytest.raises(IndexError):
+        idx.ref_seq(2)
+    with pytest.raises(ValueError, match="unknown mode 'local'"):
+        st.SearchParams(mode="local")
+    with pytest.raises(ValueError, match="unknown engine"):
+        st.SearchParams(engine="fast")
+    with pytest.raises(ValueError, match="unknown matrix 'foo'"):
+        st.SearchParams(matrix="foo")
+    with pytest.raises(TypeError):
+        st.SearchParams(matrix=3)
+    p = st.SearchParams(matrix="BLOSUM62", gap_open=11)
+    assert (p.matrix, p.gap_open, p.gap_extend, p.engine, p.mode) == ("BLOSUM62", 11, 1, "auto",
+                                                                       "all")
+    p.matrix = st.SubstitutionMatrix.blosum62()
+    assert isinstance(p.matrix, st.SubstitutionMatrix)
+    with pytest.raises(ValueError, match="invalid symbol '!' in query"):
+        idx.align(0, "CASS!", st.SearchParams(gap_open=-1))
+    with pytest.raises(ValueError, match=">= 0"):
+        idx.align(0, "CASS", st.SearchParams(gap_open=-1))
+    with pytest.raises(ValueError, match="matrix size does not match"):
+        idx.align(0, "CASS", st.SearchParams(matrix=st.SubstitutionMatrix.unit(4)))
+    with pytest.raises(IndexError):
+        idx.align(2, "CASS", st.SearchParams())
+    with pytest.raises(ValueError, match="requires the amino-acid alphabet"):
+        st.Index.build(["ACGT"], "nt").align(0, "ACG", st.SearchParams(matrix="BLOSUM62"))
+    with pytest.raises(ValueError, match="differ in length"):
+        idx.align_batch([0, 1], ["CASS"], st.SearchParams())
+    assert idx.align_batch([], [], st.SearchParams()) == []
+
+
 def test_pair_list_validation(st):
     m = st.SubstitutionMatrix.blosum62()
     with pytest.raises(ValueError, match="indexes past"):

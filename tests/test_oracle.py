@@ -399,3 +399,93 @@ def test_align_pair_known_answer_and_rescore(oracle_lib, reference):
                 assert s == oracle_lib.gotoh(a, b, sim, local, go, ge)
                 if not local:
                     assert aq.replace("-", "") == a and ar.replace("-", "") == b
+
+
+# ------------------------------------------------------------ Index::align (penalty space)
+
+def _penalty_of(cols, q, r, pen, go, ge):
+    """Penalty of an explicit global alignment: substitution penalties plus go for the first
+    column of each gap run and ge for every further one (X then Y opens a new run)."""
+    return -score_alignment(cols, q, r, {a: {b: -v for b, v in row.items()}
+                                         for a, row in pen.items()}, go, ge)
+
+
+def _unit_dict(alphabet=AA24):
+    return {a: {b: int(a != b) for b in alphabet} for a in alphabet}
+
+
+def _ops_to_cols(ops):
+    return ops.replace("S", "M").replace("D", "X").replace("I", "Y")
+
+
+INDEX_REFS = ["CASSLGQAYEQYF", "CASSPGTGVYGYTF", "CASRQGAWDTQYF", "CASSFRSSYNEQFF",
+              "CAWSVSGGGTDTQYF", "CASSLAPGATNEKLFF", "CSARDRTGNGYTF", "CASSIRSSYEQYF"]
+
+
+def test_index_align_against_alignment_enumeration(oracle_lib, reference):
+    """The score is the minimum penalty over every global alignment, and the returned ops
+    spell an alignment that costs exactly that (unit cost and BLOSUM62 penalties)."""
+    _, pen62, _ = _tables(reference, "blosum62")
+    rng = random.Random(23)
+    for pen, pd in ((None, _unit_dict()), (pen62, _pen_dict(pen62))):
+        for go, ge in [(1, 1), (3, 1), (11, 1), (1, 4), (0, 0), (2, 0)]:
+            for _ in range(25):
+                q, r = rand_seq(rng, 0, 5, "ACDK"), rand_seq(rng, 0, 5, "ACDK")
+                s, ops = oracle_lib.index_align(q, r, pen, go, ge)
+                best = min(_penalty_of(c, q, r, pd, go, ge) for c in _aligns(len(q), len(r)))
+                assert s == best, (q, r, go, ge)
+                assert _penalty_of(_ops_to_cols(ops), q, r, pd, go, ge) == s, (q, r, ops)
+                i = j = 0
+                for o in ops:  # 'M' exactly where the aligned residues are equal
+                    if o in "MS":
+                        assert (o == "M") == (q[i] == r[j]), (q, r, ops)
+                    i += o in "MSD"
+                    j += o in "MSI"
+                assert (i, j) == (len(q), len(r))
+
+
+@pytest.mark.parametrize("matrix,go,ge", [(None, 1, 1), ("blosum62", 11, 1), ("blosum62", 14, 1),
+                                          ("blosum62", 1, 1), ("structural", 7, 3),
+                                          ("pam250", 0, 0), (None, 2, 5), ("blosum45", 40, 1)])
+def test_index_align_matches_reference(oracle_lib, reference, matrix, go, ge):
+    """Score and ops equal the compiled reference's Index::align, tie-heavy inputs included
+    (few residue kinds, where the strict-< state order decides among equal paths)."""
+    pen = None if matrix is None else _tables(reference, matrix)[1]
+    rng = random.Random(seed_of("index", matrix, go, ge))
+    qs = [rand_seq(rng, 0, 24) for _ in range(60)] + \
+        [rand_seq(rng, 1, 14, "AAC") for _ in range(60)] + ["", "W", "WWW", "KAK"]
+    rs = [rand_seq(rng, 0, 24) for _ in range(30)] + \
+        [rand_seq(rng, 1, 14, "AAC") for _ in range(30)] + ["", "W", "KKK"]
+    pq = list(range(len(qs))) * 2
+    pr = [rng.randrange(len(rs)) for _ in pq]
+    want = reference.index_align_pairs(qs, rs, pq, pr, matrix, go, ge)
+    for a, b, w in zip(pq, pr, want):
+        s, ops = oracle_lib.index_align(qs[a], rs[b], pen, go, ge)
+        assert (s, ops) == (w[0], w[3]), (qs[a], rs[b])
+        # the reference's aligned strings are the raw residues in op order
+        assert aligned_strings(qs[a], rs[b], ops, 0, 0) == (w[1], w[2])
+
+
+def test_index_align_known_answers(oracle_lib, reference):
+    """The reference's own Index.align tests: tests/python/test_engine_guards.py:50-80,
+    test_matrix.py:28-40 and test_sanity.py:86-98."""
+    _, pen62, scale62 = _tables(reference, "blosum62")
+    ref0 = INDEX_REFS[0]
+    cheap = oracle_lib.index_align("CASSLGYEQYF", ref0, pen62, 11, 1)
+    dear = oracle_lib.index_align("CASSLGYEQYF", ref0, pen62, 11, 9)
+    assert cheap[0] < dear[0] and dear[0] - cheap[0] == 8
+    lin = oracle_lib.index_align("CASSLGYEQYF", ref0, None, 1, 1)
+    assert lin[0] == 2 and lin[1].count("I") + lin[1].count("D") == 2
+    degenerate = oracle_lib.index_align("CASSIRSSYEQYF", ref0, pen62, 1, 1)
+    sane = oracle_lib.index_align("CASSIRSSYEQYF", ref0, pen62, scale62, 1)
+    assert degenerate[1].count("I") > 0 and sane[1] == "M" * 4 + "S" * 4 + "M" * 5
+    want = {None: 2, "structural": 18, "pam250": 24, "blosum62": 28, "pam100": 30}
+    for name, expected in want.items():
+        pen = None if name is None else _tables(reference, name)[1]
+        assert oracle_lib.index_align("CASSLAPGATNEKLFF", "CASSLELGATNEKLFF", pen, 100,
+                                      1)[0] == expected
+    assert oracle_lib.index_align("CASSPGQAYEQF", "CASSPGQAYEQF", pen62, 1, 1) == (0, "M" * 12)
+    assert oracle_lib.index_align("CASSPGQAYEQY", "CASSPGQAYEQF", pen62, 2 * scale62,
+                                  1)[1] == "M" * 11 + "S"
+    assert oracle_lib.index_align("", "", None, 1, 1) == (0, "")
+    assert oracle_lib.index_align("", "ACD", None, 3, 2) == (7, "III")
