@@ -213,6 +213,26 @@ int32_t st_gapblock_matrix_device(const uint8_t* q_codes, const uint64_t* q_offs
                                   int32_t gap_extend, const int32_t* prior, uint32_t prior_width,
                                   int32_t* out, void* stream, st_status* status);
 
+/* The allocation-free form of st_gapblock_matrix_device: same arguments, same results. It has
+ * no counterpart in the reference; it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). Its scratch
+ * (the device penalty table and prior cube, the packed references and the twin list) lives
+ * in a static 32 MiB device array that each GPU the library runs on holds until the process
+ * ends; no call allocates or frees device memory, and the tables go in with stream-ordered
+ * copies. Sets whose scratch would exceed 32 MiB (roughly 700,000 references of up to 30
+ * residues) return ST_INVALID_ARGUMENT before any work is enqueued. Every call on one GPU
+ * shares that GPU's array and leaves its contents behind: calls that overlap in time on one
+ * GPU (on different streams, or from different threads) race and must be serialised by the
+ * caller; consecutive calls on one stream are ordered and safe. */
+int32_t st_gapblock_matrix_device_noalloc(const uint8_t* q_codes, const uint64_t* q_offsets,
+                                          uint64_t n_queries, uint32_t q_max_len,
+                                          const uint8_t* r_codes, const uint64_t* r_offsets,
+                                          uint64_t n_refs, uint32_t r_max_len, const int32_t* pen,
+                                          uint32_t alphabet_size, int32_t gap_open,
+                                          int32_t gap_extend, const int32_t* prior,
+                                          uint32_t prior_width, int32_t* out, void* stream,
+                                          st_status* status);
+
 /* The device form of st_gapblock_threshold: hit_query / hit_ref / hit_score AND hit_count are
  * device memory. *hit_count is a device counter (zeroed by the call, then counted into by the
  * kernels); read it back once the stream's work is done. */

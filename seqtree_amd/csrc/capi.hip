@@ -22,6 +22,13 @@
 #include <string>
 #include <vector>
 
+// Device scratch of st_gapblock_matrix_device_noalloc: one array per GPU (each device holds
+// its own copy at its own address, kept until the process ends). Host code never uses this
+// name as an address -- it is a host-side stand-in -- but the device address that
+// hipGetSymbolAddress returns for the current device (noalloc_scratch below).
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
 namespace {
 
 using namespace st;
@@ -102,9 +109,15 @@ bool pools_supported() {
 // Requests of at most kSmall bytes are carved (256-byte aligned) out of shared blocks of
 // kBlock bytes, larger ones get their own allocation: a stream-ordered free costs the host
 // ~10-20 us on this runtime, so a call with a handful of small buffers frees once.
+//
+// A fixed arena carves every request from one caller-given device region instead, allocating
+// nothing, and throws ST_INVALID_ARGUMENT when the region is exhausted.
 class Arena {
 public:
     explicit Arena(hipStream_t s) : s_(s) {}
+    Arena(hipStream_t s, void* region, size_t bytes)
+        : s_(s), block_(static_cast<char*>(region)), fixed_bytes_(region ? bytes : 0) {}
+    bool fixed() const { return fixed_bytes_ != 0; }
     ~Arena() {
         if (bufs_.empty()) return;
         if (pools_supported()) {
@@ -117,6 +130,16 @@ public:
     template <class T>
     T* get(size_t count) {
         const size_t bytes = (std::max<size_t>(count * sizeof(T), 1) + 255) / 256 * 256;
+        if (fixed()) {
+            if (used_ + bytes > fixed_bytes_)
+                throw HipError{ST_INVALID_ARGUMENT,
+                               "seqtree_hip: the call needs more than the " +
+                                   std::to_string(fixed_bytes_ >> 20) +
+                                   " MiB of static scratch"};
+            T* p = reinterpret_cast<T*>(block_ + used_);
+            used_ += bytes;
+            return p;
+        }
         if (bytes > kSmall) return static_cast<T*>(raw(bytes));
         if (block_ == nullptr || used_ + bytes > kBlock) {
             block_ = static_cast<char*>(raw(kBlock));
@@ -145,6 +168,7 @@ private:
     std::vector<void*> bufs_;
     char* block_ = nullptr;
     size_t used_ = 0;
+    size_t fixed_bytes_ = 0;
 };
 
 class OwnedStream {
@@ -182,10 +206,13 @@ struct Pending {
 std::mutex g_timing_mu;
 bool g_timing = false;
 std::vector<Pending> g_pending;
+// set by the allocation-free entry point on its thread: no events are created meanwhile
+thread_local bool t_no_alloc = false;
 
 class LaunchTimer {
 public:
     LaunchTimer(std::string label, hipStream_t s) : s_(s) {
+        if (t_no_alloc) return;
         std::lock_guard<std::mutex> lk(g_timing_mu);
         if (!g_timing) return;
         on_ = hipEventCreate(&a_) == hipSuccess && hipEventCreate(&b_) == hipSuccess &&
@@ -279,6 +306,14 @@ size_t g_table_bytes = 0;
 
 DeviceTable table_on_device(Arena& ar, const int32_t* host, size_t count, size_t pad,
                             hipStream_t s) {
+    if (ar.fixed()) {  // no cache (its misses allocate): a stream-ordered copy into the arena
+        int64_t abs_max = 0;
+        for (size_t i = 0; i < count; ++i) abs_max = std::max(abs_max, iabs64(host[i]));
+        int32_t* d = ar.get<int32_t>(count + pad);
+        h2d(d, host, count, s);
+        if (pad) launch_fill_u32(reinterpret_cast<uint32_t*>(d + count), pad, 0u, s);
+        return {d, abs_max};
+    }
     int dev = 0;
     check(hipGetDevice(&dev), "hipGetDevice");
     {
@@ -454,9 +489,10 @@ void gapblock_run(const GapblockPlan& p, uint32_t q_max, hipStream_t s) {
 void gapblock_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, uint32_t q_max,
                      const uint8_t* r, const uint64_t* r_off, uint64_t K, uint32_t r_max,
                      const int32_t* pen_h, uint32_t A, int32_t go, int32_t ge,
-                     const int32_t* prior_h, uint32_t W, int32_t* out, hipStream_t s) {
+                     const int32_t* prior_h, uint32_t W, int32_t* out, hipStream_t s,
+                     void* region = nullptr, size_t region_bytes = 0) {
     if (N == 0 || K == 0) return;
-    Arena ar(s);
+    Arena ar(s, region, region_bytes);
     GapblockPlan p = gapblock_plan(ar, q, q_off, N, q_max, r, r_off, K, r_max, pen_h, A, go, ge,
                                    prior_h, W, true, s);
     p.a.out = out;
@@ -1254,6 +1290,45 @@ struct PairTraceback {
     }
 };
 
+// ---- allocation-free gap-block matrix (st_gapblock_matrix_device_noalloc) ------------------
+// The device address of the static scratch array on the current device, looked up once per
+// device with that device current and kept for the life of the process.
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_addr;
+
+void* noalloc_scratch() {
+    int dev = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    const auto it = g_scratch_addr.find(dev);
+    if (it != g_scratch_addr.end()) return it->second;
+    void* p = nullptr;
+    check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    g_scratch_addr.emplace(dev, p);
+    return p;
+}
+
+// Marks the calling thread as inside an allocation-free call (no timing events).
+struct NoAllocScope {
+    NoAllocScope() { t_no_alloc = true; }
+    ~NoAllocScope() { t_no_alloc = false; }
+};
+
+// An upper bound of the scratch a dense gap-block call carves (256-byte granules, as Arena):
+// the penalty table, the prior cube with its padding, the packed references (words, lengths,
+// columns) and the twin list with its sorted indices.
+uint64_t gapblock_scratch_bound(uint64_t N, uint64_t K, uint32_t r_max, uint32_t A,
+                                bool prior, uint32_t W) {
+    auto granule = [](uint64_t bytes) { return (std::max<uint64_t>(bytes, 1) + 255) / 256 * 256; };
+    const PackedRefs geo = packed_geometry(K, r_max);
+    const uint64_t w1 = uint64_t(W) + 1;
+    uint64_t b = granule(uint64_t(A) * A * 4);
+    if (prior) b += granule((w1 * w1 * w1 + gb::kPriorPad) * 4);
+    b += granule(uint64_t(geo.nwords + 1) * geo.npad * 4) + 2 * granule(uint64_t(geo.npad) * 4);
+    b += granule((twin_capacity(N) + (N + 1) / 2) * sizeof(uint2));
+    return b;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1713,6 +1788,34 @@ int32_t st_gapblock_matrix_device(const uint8_t* q_codes, const uint64_t* q_offs
         gapblock_device(q_codes, q_offsets, n_queries, q_max_len, r_codes, r_offsets, n_refs,
                         r_max_len, pen, alphabet_size, gap_open, gap_extend, prior, prior_width,
                         out, static_cast<hipStream_t>(stream));
+        return ok(status);
+    });
+}
+
+int32_t st_gapblock_matrix_device_noalloc(const uint8_t* q_codes, const uint64_t* q_offsets,
+                                          uint64_t n_queries, uint32_t q_max_len,
+                                          const uint8_t* r_codes, const uint64_t* r_offsets,
+                                          uint64_t n_refs, uint32_t r_max_len, const int32_t* pen,
+                                          uint32_t alphabet_size, int32_t gap_open,
+                                          int32_t gap_extend, const int32_t* prior,
+                                          uint32_t prior_width, int32_t* out, void* stream,
+                                          st_status* status) {
+    if (alphabet_size == 0 || alphabet_size > kMaxAlphabet) return bad_alphabet(status, alphabet_size);
+    return guarded(status, [&] {
+        require_device();
+        NoAllocScope no_alloc;
+        if (n_queries == 0 || n_refs == 0) return ok(status);
+        const uint64_t need = gapblock_scratch_bound(n_queries, n_refs, r_max_len, alphabet_size,
+                                                     prior != nullptr, prior_width);
+        if (need > kNoAllocScratchBytes)
+            throw HipError{ST_INVALID_ARGUMENT,
+                           "seqtree_hip: st_gapblock_matrix_device_noalloc needs " +
+                               std::to_string(need) + " bytes of scratch for these sets; its "
+                               "static scratch holds " + std::to_string(kNoAllocScratchBytes)};
+        gapblock_device(q_codes, q_offsets, n_queries, q_max_len, r_codes, r_offsets, n_refs,
+                        r_max_len, pen, alphabet_size, gap_open, gap_extend, prior, prior_width,
+                        out, static_cast<hipStream_t>(stream), noalloc_scratch(),
+                        kNoAllocScratchBytes);
         return ok(status);
     });
 }
