@@ -34,7 +34,11 @@ Prints one JSON line: pairs/s (value), cell updates per second, the roofline of 
 kernel measured with HIP events on the stream it runs on, and the reference's multithreaded CPU
 path timed on a bounded sample on this host.
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--workload gotoh]
+By default every workload runs and one line carries them all: the Gotoh dist_matrix (BASELINE
+C2) as the headline value, every other configuration under "configs" (value, step time,
+roofline, CPU baseline), the end-to-end drop-in timings and the scalar-call latencies.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--workload all|gotoh|...]
 """
 from __future__ import annotations
 
@@ -121,7 +125,7 @@ def run_germline(torch, dev, lib, stream, args) -> dict:
         to_str = lambda v: "".join(AA_ORDER[c] for c in v)  # noqa: E731
         cpu = cpu_reference("gotoh", [to_str(s) for s in qs], [to_str(s) for s in rs],
                             max(1, args.cpu_rows // 10), seconds=args.cpu_seconds, sim=sim)
-    return base_line(
+    line = base_line(
         "all-vs-all pairs/s (Gotoh NW+SW dist_matrix, germline-V lengths)",
         pairs_per_step / (ms_step / 1e3), args.steps, args.warmup, ms_step,
         {"workload": "gotoh_dist_matrix_germline_4000x4000_len82-98_blosum62_go11_ge1_"
@@ -130,6 +134,73 @@ def run_germline(torch, dev, lib, stream, args) -> dict:
          "pairs_per_step": pairs_per_step, "cells_per_step": 2 * cells_per_mode,
          "parallelism": "single GPU"},
         kernels, roofline, cpu, gcups=2 * cells_per_mode / (ms_step / 1e3) / 1e9)
+    line["dtype"] = "int16"
+    line["arithmetic"] = GOTOH_ARITHMETIC
+    return line
+
+
+GOTOH_ARITHMETIC = ("packed int16x2 Gotoh cells (two queries per register; exact under the "
+                    "host's range proof, int32 / int64 kernels past it)")
+
+
+def end_to_end(call, pairs: float, what: str, cpu) -> dict:
+    """The drop-in call as a user makes it (Python strings in, host ScoreMatrix out): one
+    warm-up, then the best of three, timed with perf_counter as the reference's own
+    benchmarks do (bench/bench_score_matrix.py:35-37, 70-77)."""
+    call()
+    best = min(_timed(call) for _ in range(3))
+    out = {"pairs_per_s": pairs / best, "ms_per_call": round(1e3 * best, 3), "what": what}
+    if cpu:
+        out["vs_cpu_baseline"] = round(pairs / best / cpu["value"], 2)
+    return out
+
+
+def _timed(call) -> float:
+    t0 = time.perf_counter()
+    call()
+    return time.perf_counter() - t0
+
+
+def scalar_latency(args) -> dict:
+    """Latency of the scalar entry points (one pair per call), median of 200 calls, against the
+    reference's own C++ scalar functions (oracle/_ref, through ctypes) on the host."""
+    import oracle
+    import seqtree_amd as st
+
+    m = st.SubstitutionMatrix.blosum62()
+    q, r = "CASSLGQAYEQYF", "CASSPGQAYEQF"
+    idx = st.Index.build([r], "aa")
+    params = st.SearchParams(matrix="BLOSUM62", gap_open=11, gap_extend=1)
+    umi_a, umi_b = "ACGTACGTACGT", "ACGTTCGTACGA"
+    read_a, read_b = "ACGT" * 12, "ACGA" * 12
+
+    def med(fn, n=200):
+        fn()
+        ts = []
+        for _ in range(n):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return round(1e6 * float(np.median(ts)), 1)
+
+    out = {"unit": "us per call (median of 200)",
+           "gpu": {"pairwise.score": med(lambda: st.pairwise.score(q, r, m)),
+                   "pairwise.align": med(lambda: st.pairwise.align(q, r, m)),
+                   "Index.align": med(lambda: idx.align(0, q, params)),
+                   "distance.hamming": med(lambda: st.distance.hamming(umi_a, umi_b)),
+                   "distance.levenshtein": med(lambda: st.distance.levenshtein(read_a, read_b))}}
+    if oracle.reference_available():
+        R = oracle.Reference()
+        out["reference_cpu"] = {
+            "align_score": med(lambda: R.align_score(q, r, "blosum62")),
+            "align_pair": med(lambda: R.align_pair(q, r, "blosum62")),
+            "Index.align": med(lambda: R.index_align_pairs([q], [r], [0], [0], "blosum62", 11,
+                                                           1)),
+            "hamming": med(lambda: R.distance_matrix("hamming", [umi_a], [umi_b])),
+            "levenshtein": med(lambda: R.distance_matrix("levenshtein", [read_a], [read_b]))}
+        out["reference_note"] = ("through ctypes (a few us of call overhead); Index.align "
+                                 "includes building a one-reference Index")
+    return out
 
 
 def cdr3_set(rng: np.random.Generator, n: int, lo: int, hi: int) -> list[np.ndarray]:
@@ -299,18 +370,57 @@ def dominant(kernels):
     return label, launches, ms / launches / 1e3
 
 
-def cpu_reference(kind: str, q, r, rows: int, seconds: float = 10.0, **kw) -> dict:
+def host_cpu() -> dict:
+    """The host the CPU baseline runs on: CPU model, logical CPUs and those this process may
+    use."""
+    import os
+
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        usable = os.cpu_count() or 1
+    return {"model": model, "nproc": os.cpu_count(), "usable_cpus": usable}
+
+
+# The north-star comparison is the reference's CPU path on 16 cores; it is also timed on every
+# core this process may use (the reference's threads=0).
+CPU_THREADS = 16
+
+
+def timed_rate(once, seconds: float):
+    """One untimed warm-up call, then calls of `once` (returning its work units) repeated until
+    `seconds` of wall time: (units per second, calls, units, seconds)."""
+    once()
+    units, calls = 0, 0
+    t0 = time.perf_counter()
+    while True:
+        units += once()
+        calls += 1
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            return units / dt, calls, units, dt
+
+
+def cpu_reference(kind: str, q, r, rows: int, seconds: float = 3.0, **kw) -> dict:
     """The reference's own C++ (oracle/_ref, compiled -O3 from its sources) on the first `rows`
-    queries against every reference, on every host core, repeated until `seconds` of wall time
-    have passed; the plain-C port where the compiled reference is absent."""
+    queries against every reference, with 16 worker threads and with every usable core; the
+    plain-C port where the compiled reference is absent."""
     import oracle
 
-    threads = oracle.host_threads()
+    host = host_cpu()
     qs = q[:rows]
     have_ref = oracle.reference_available()
     impl = oracle.Reference() if have_ref else oracle.Oracle()
 
-    def once() -> int:
+    def once(threads) -> int:
         if kind == "gotoh":
             for local in (False, True):
                 impl.align_matrix(qs, r, "blosum62" if have_ref else kw["sim"], local, True,
@@ -323,19 +433,17 @@ def cpu_reference(kind: str, q, r, rows: int, seconds: float = 10.0, **kw) -> di
         impl.distance_matrix(kind, qs, r, threads=threads)
         return len(qs) * len(r)
 
-    pairs, calls = 0, 0
-    t0 = time.perf_counter()
-    while True:
-        pairs += once()
-        calls += 1
-        dt = time.perf_counter() - t0
-        if dt >= seconds:
-            break
-    return {"value": pairs / dt, "unit": "pairs/s", "cores": threads,
-            "kind": "reference" if have_ref else "port",
-            "sample": f"first {len(qs)} queries x all {len(r)} references of the same workload"
-                      f"{' (global + local)' if kind == 'gotoh' else ''}, {calls} calls: "
-                      f"{pairs} pairs in {dt:.2f} s wall"}
+    rate, calls, pairs, dt = timed_rate(lambda: once(CPU_THREADS), seconds)
+    out = {"value": rate, "unit": "pairs/s", "cores": CPU_THREADS,
+           "kind": "reference" if have_ref else "port",
+           "sample": f"first {len(qs)} queries x all {len(r)} references of the same workload"
+                     f"{' (global + local)' if kind == 'gotoh' else ''}, {calls} calls after "
+                     f"one warm-up: {pairs} pairs in {dt:.2f} s wall",
+           "host": host}
+    if host["usable_cpus"] != CPU_THREADS:
+        all_rate, _, _, _ = timed_rate(lambda: once(host["usable_cpus"]), seconds)
+        out["all_cores"] = {"value": all_rate, "cores": host["usable_cpus"]}
+    return out
 
 
 def base_line(metric, value, steps, warmup, ms_step, config, kernels, roofline, cpu, **extra):
@@ -410,14 +518,29 @@ def run_gotoh(torch, dev, lib, stream, args) -> dict:
         to_str = lambda v: "".join(AA_ORDER[c] for c in v)  # noqa: E731
         cpu = cpu_reference("gotoh", [to_str(s) for s in qs], [to_str(s) for s in rs],
                             args.cpu_rows, seconds=args.cpu_seconds, sim=sim)
-    return base_line(
+    e2e = None
+    if args.end_to_end:
+        import seqtree_amd as st
+
+        to_str = lambda v: "".join(AA_ORDER[c] for c in v)  # noqa: E731
+        q_strs, r_strs = [to_str(x) for x in qs], [to_str(x) for x in rs]
+        m = st.SubstitutionMatrix.blosum62()
+        e2e = end_to_end(lambda: [st.pairwise.dist_matrix(q_strs, r_strs, m, mode=md)
+                                  for md in ("global", "local")], pairs_per_step,
+                         "seqtree_amd.pairwise.dist_matrix, global then local: str lists in, "
+                         "host ScoreMatrix out", cpu)
+    line = base_line(
         "all-vs-all pairs/s (Gotoh NW+SW dist_matrix)", pairs_per_step / (ms_step / 1e3),
         args.steps, args.warmup, ms_step,
         {"workload": "gotoh_dist_matrix_cdr3_20000x20000_blosum62_go11_ge1_global+local",
          "queries": Q.n, "refs": R.n, "lengths": [LEN_LO, LEN_HI], "seed": SEED,
          "pairs_per_step": pairs_per_step, "cells_per_step": 2 * cells_per_mode,
          "parallelism": "single GPU"},
-        kernels, roofline, cpu, gcups=2 * cells_per_mode / (ms_step / 1e3) / 1e9)
+        kernels, roofline, cpu, gcups=2 * cells_per_mode / (ms_step / 1e3) / 1e9,
+        end_to_end=e2e)
+    line["dtype"] = "int16"
+    line["arithmetic"] = GOTOH_ARITHMETIC
+    return line
 
 
 def run_gapblock(torch, dev, lib, stream, args) -> dict:
@@ -460,13 +583,24 @@ def run_gapblock(torch, dev, lib, stream, args) -> dict:
         cpu = cpu_reference("gapblock", [to_str(s) for s in qs], [to_str(s) for s in rs],
                             min(args.cpu_rows, Q.n), seconds=args.cpu_seconds, pen=pen, go=go,
                             cube=cube.tolist(), width=width)
+    e2e = None
+    if args.end_to_end:
+        import seqtree_amd as st
+
+        to_str = lambda v: "".join(AA_ORDER[c] for c in v)  # noqa: E731
+        q_strs, r_strs = [to_str(x) for x in qs], [to_str(x) for x in rs]
+        m = st.SubstitutionMatrix.blosum62()
+        prior = central_prior(lam)
+        e2e = end_to_end(lambda: st.score_matrix(q_strs, r_strs, m, go, 1, prior), pairs,
+                         "seqtree_amd.score_matrix: str lists in, prior cube built, host "
+                         "ScoreMatrix out", cpu)
     return base_line(
         "all-vs-all pairs/s (gap-block score_matrix)", pairs / (ms_step / 1e3), args.steps,
         args.warmup, ms_step,
         {"workload": "gapblock_score_matrix_cdr3_1000x10000_blosum62_go28_ge1_central21",
          "queries": Q.n, "refs": R.n, "lengths": [10, 20], "seed": SEED + 1,
          "pairs_per_step": pairs, "lookups_per_step": lookups, "parallelism": "single GPU"},
-        kernels, roofline, cpu, glookups=lookups / (ms_step / 1e3) / 1e9)
+        kernels, roofline, cpu, glookups=lookups / (ms_step / 1e3) / 1e9, end_to_end=e2e)
 
 
 # Ragged batch: 64 jobs, log-normal set sizes (median 2,000 queries and 50,000 references,
@@ -551,18 +685,63 @@ def run_batch(torch, dev, lib, stream, args) -> dict:
         kernels, roofline, cpu, glookups=lookups / (ms_step / 1e3) / 1e9)
 
 
-# Learned prefilter: train on 200,000 CDR3s (50,000 families of 4) with 2M pairs labelled by
-# the exact gap-block kernel (BLOSUM62, gap_open 28, central prior 21), then search 50,000
-# queries (a fresh member of 50,000 of the reference families) against 1,000,000 references
-# (250,000 families of 4) at a fixed score cutoff.
+# Learned prefilter (BASELINE C5): 200,000 synthetic CDR3s (independent draws and clonal
+# families of 4); 180,000 train on 2M pairs mined and labelled by the exact gap-block kernel
+# (BLOSUM62, gap_open 28, central prior 21), 20,000 calibrate the screen's margin; then search 50,000 queries x 1,000,000
+# references at a fixed score cutoff, on two reference/query sets: independent synthetic
+# CDR3s (the generator alone; the headline value) and planted families (250,000 families of 4
+# references, each query a fresh mutant of one family's root).
 PF_LEN = (10, 20)
-PF_TRAIN = (50_000, 4)
+PF_TRAIN_IID, PF_TRAIN_FAM = 90_000, (22_500, 4)   # 180,000 training sequences
+PF_CALIB_IID, PF_CALIB_FAM = 10_000, (2_500, 4)    # and 20,000 held out for the margin
 PF_PAIRS, PF_NEAR = 2_000_000, 0.4
-PF_REFS = (250_000, 4)
+PF_REFS, PF_FAMILIES = 1_000_000, (250_000, 4)
 PF_QUERIES = 50_000
 PF_CUTOFF = 48
 PF_EPOCHS = 8
-PF_QUANTILE = 0.998
+PF_QUANTILE = 0.999
+PF_AUDIT = 0.01  # queries searched exhaustively per search to calibrate the threshold
+
+
+def pf_sets(rng, syn):
+    ic, io, _ = syn.cdr3_csr(rng, PF_REFS, *PF_LEN)
+    jc, jo, _ = syn.cdr3_csr(rng, PF_QUERIES, *PF_LEN)
+    roots: list = []
+    rc, ro, _ = syn.families(rng, *PF_FAMILIES, *PF_LEN, roots_out=roots)
+    pick = rng.choice(PF_FAMILIES[0], size=PF_QUERIES, replace=False)
+    qc, qo = syn.join(syn._mutate_many(rng, [roots[f] for f in pick], 1, *PF_LEN))
+    return {"iid": (jc, jo, ic, io), "planted": (qc, qo, rc, ro)}
+
+
+def pf_search_set(torch, lib, pf, model, gp, tau, sets, args):
+    """One query / reference set: the exhaustive exact pass (best of two), then `steps`
+    searches through the prefilter, each step the reference index build and one search, timed
+    apart; recall against the exhaustive pass."""
+    qc, qo, rc, ro = sets
+    Q, R = pf.DeviceSet.upload(qc, qo), pf.DeviceSet.upload(rc, ro)
+    ex_times = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        eq, er, es = pf.exact_threshold(Q, R, args.pf_cutoff, gp)
+        ex_times.append(time.perf_counter() - t0)
+    ex_s = min(ex_times)
+    searcher = pf.Searcher(model, gp, PF_LEN[1])
+    for _ in range(args.warmup):
+        searcher.index(R)
+        res = searcher.query(Q, args.pf_cutoff, tau, audit=PF_AUDIT, quantile=PF_QUANTILE)
+    torch.cuda.synchronize()
+    build_times, search_times = [], []
+    for step in range(args.steps):
+        t0 = time.perf_counter()
+        searcher.index(R)
+        t1 = time.perf_counter()
+        res = searcher.query(Q, args.pf_cutoff, tau, audit=PF_AUDIT, quantile=PF_QUANTILE,
+                             seed=step)
+        search_times.append(time.perf_counter() - t1)
+        build_times.append(t1 - t0)
+    return {"Q": Q, "R": R, "exact": (eq, er, es), "ex_s": ex_s, "res": res,
+            "ms_build": 1e3 * float(np.mean(build_times)),
+            "ms_search": 1e3 * float(np.mean(search_times)), "searcher": searcher}
 
 
 def run_prefilter(torch, dev, lib, stream, args) -> dict:
@@ -572,110 +751,96 @@ def run_prefilter(torch, dev, lib, stream, args) -> dict:
     seed = SEED + 4
     rng = np.random.default_rng(seed)
     t0 = time.perf_counter()
-    tc, to, tfam = syn.families(rng, *PF_TRAIN, *PF_LEN)
+    tc, to = syn.training_set(rng, PF_TRAIN_IID, *PF_TRAIN_FAM, *PF_LEN)
+    hc, ho = syn.training_set(rng, PF_CALIB_IID, *PF_CALIB_FAM, *PF_LEN)
     gp = pf.GapblockParams.blosum62_central(PF_LEN[1])
-    pa, pb = pf.sample_pairs(rng, tfam, PF_PAIRS, PF_NEAR)
+    pa, pb, n_close = pf.mine_pairs(rng, tc, to, gp, PF_PAIRS, PF_NEAR, args.pf_cutoff,
+                                    2 * args.pf_cutoff)
     score = pf.label_pairs(tc, to, pa, pb, gp)
-    roots: list = []
-    rc, ro, rfam = syn.families(rng, *PF_REFS, *PF_LEN, roots_out=roots)
-    qfam = rng.choice(PF_REFS[0], size=PF_QUERIES, replace=False)
-    qc, qo = syn.join(syn._mutate_many(rng, [roots[f] for f in qfam], 1, *PF_LEN))
+    sets = pf_sets(rng, syn)
     data_s = time.perf_counter() - t0
 
     model = pf.Prefilter(seed=seed)
     t0 = time.perf_counter()
-    hold = rng.random(len(pa)) < 0.02  # held out for the margin calibration
-    losses = pf.train(model, pf.TrainingData(tc, to, pa[~hold], pb[~hold], score[~hold]),
-                      epochs=args.pf_epochs, seed=seed, max_len=PF_LEN[1],
+    losses = pf.train(model, pf.TrainingData(tc, to, pa, pb, score), epochs=args.pf_epochs,
+                      seed=seed, max_len=PF_LEN[1],
                       log=lambda m: print(m, file=sys.stderr, flush=True))
     train_s = time.perf_counter() - t0
-    dist = pf.embed_distances(model, tc, to, pa[hold], pb[hold], PF_LEN[1])
-    tau = pf.calibrate_tau(dist, score[hold], args.pf_cutoff, quantile=PF_QUANTILE)
-    close = score[hold] <= args.pf_cutoff
-    calib = {"held_out_pairs": int(hold.sum()), "held_out_close": int(close.sum()),
-             "quantile": PF_QUANTILE,
-             "far_pass_rate": float((dist[~close] <= tau).mean()),
-             "abs_err_median": float(np.median(np.abs(dist - score[hold])))}
+    tau, n_cal = pf.calibrate_on_set(model, hc, ho, gp, args.pf_cutoff, PF_QUANTILE, PF_LEN[1])
+    calib = {"calibration_sequences": len(ho) - 1, "calibration_close_pairs": int(n_cal),
+             "quantile": PF_QUANTILE, "tau_calibrated": round(tau, 3)}
 
-    Q, R = pf.DeviceSet.upload(qc, qo), pf.DeviceSet.upload(rc, ro)
-    pairs = float(Q.n) * float(R.n)
-    # exhaustive exact pass: the measure of recall, and the time to beat
-    ex_times = []
-    for _ in range(2):
-        t0 = time.perf_counter()
-        eq, er, es = pf.exact_threshold(Q, R, args.pf_cutoff, gp)
-        ex_times.append(time.perf_counter() - t0)
-    ex_s = min(ex_times)
-
-    searcher = pf.Searcher(model, gp, PF_LEN[1])
-    # The references' embeddings are the index, built once per reference set and timed apart
-    # from the search, as the reference's own search benchmark times GapBlockIndex
-    # (bench/bench_gapblock.py:149-160); a step is one search of all queries: embed the
-    # queries, screen, rescore exactly, keep <= cutoff. The one-shot pipeline (build + search)
-    # is reported beside it.
-    for _ in range(args.warmup):
-        searcher.index(R)
-        res = searcher.query(Q, args.pf_cutoff, tau)
-    torch.cuda.synchronize()
-    build_times, search_times = [], []
-    for _ in range(args.steps):
-        t0 = time.perf_counter()
-        searcher.index(R)
-        t1 = time.perf_counter()
-        res = searcher.query(Q, args.pf_cutoff, tau)
-        search_times.append(time.perf_counter() - t1)
-        build_times.append(t1 - t0)
-    # the same steps again with the library's launch timing on, for the per-kernel breakdown
+    per_set = {}
+    for name in ("iid", "planted"):
+        r = pf_search_set(torch, lib, pf, model, gp, tau, sets[name], args)
+        eq, er, _ = r["exact"]
+        res = r["res"]
+        pairs = float(r["Q"].n) * float(r["R"].n)
+        per_set[name] = {
+            "pairs_per_step": pairs, "queries": r["Q"].n, "refs": r["R"].n,
+            "effective_pairs_per_s": pairs / (r["ms_search"] / 1e3),
+            "ms_per_search": round(r["ms_search"], 3), "index_build_ms": round(r["ms_build"], 3),
+            "one_shot_ms": round(r["ms_build"] + r["ms_search"], 3),
+            "exhaustive_ms": round(1e3 * r["ex_s"], 3),
+            "exhaustive_pairs_per_s": pairs / r["ex_s"],
+            "speedup_search_only": r["ex_s"] / (r["ms_search"] / 1e3),
+            "speedup_one_shot": r["ex_s"] / ((r["ms_build"] + r["ms_search"]) / 1e3),
+            "recall": pf.recall(res.query, res.ref, eq, er), "exact_hits": int(len(eq)),
+            "found_hits": int(len(res.query)), "candidates": res.candidates,
+            "tau_score": round(res.tau, 3), "audit_queries": res.audit_queries,
+            "audit_hits": res.audit_hits,
+            "stage_ms": {k: round(1e3 * v, 3) for k, v in res.seconds.items()}}
+        if name == "iid":
+            main_set = r
+    # per-kernel breakdown of the headline set's steps, with the library's launch timing on
+    searcher, Q, R = main_set["searcher"], main_set["Q"], main_set["R"]
     lib.lib.st_timing_enable(1)
     lib.timing()
-    for _ in range(args.steps):
+    for step in range(args.steps):
         searcher.index(R)
-        res = searcher.query(Q, args.pf_cutoff, tau)
+        searcher.query(Q, args.pf_cutoff, tau, audit=PF_AUDIT, quantile=PF_QUANTILE, seed=step)
     torch.cuda.synchronize()
     lib.lib.st_timing_enable(0)
     kernels = lib.timing()
-    ms_build = 1e3 * sum(build_times) / len(build_times)
-    ms_step = 1e3 * sum(search_times) / len(search_times)
-    ms_one_shot = ms_build + ms_step
-    rec = pf.recall(res.query, res.ref, eq, er)
+    head = per_set["iid"]
+    pairs = head["pairs_per_step"]
     label, launches, avg_s = dominant(kernels)
     if label == "prefilter_screen":
         flops = 2.0 * 128 * pairs
-    else:  # encoder: three convolution GEMMs over both sets' rows
+    else:  # encoder: three convolution GEMMs over one set's rows per launch
         slot = PF_LEN[1] + 4
         rows = (Q.n + R.n) * slot
         flops = 2.0 * rows * 128 * (5 * 32 + 2 * 5 * 128) / max(launches / args.steps, 1)
     achieved = flops / avg_s / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": 2500.0,
-                "unit": "TFLOP/s", "frac": round(achieved / 2500.0, 4), "traffic": traffic_for(label),
-                "kernel": label, "launches": launches, "avg_launch_ms": round(avg_s * 1e3, 4),
-                "flops_per_launch": flops}
+                "unit": "TFLOP/s", "frac": round(achieved / 2500.0, 4),
+                "traffic": traffic_for(label), "kernel": label, "launches": launches,
+                "avg_launch_ms": round(avg_s * 1e3, 4), "flops_per_launch": flops}
     cpu = None
     if args.cpu_rows > 0:
+        qc, qo, rc, ro = sets["iid"]
         qs, rs = syn.decode(qc, qo), syn.decode(rc, ro)
         cpu = cpu_reference("gapblock", qs, rs, min(args.cpu_rows, 20),
                             seconds=args.cpu_seconds, pen=gp.pen, go=gp.gap_open,
                             cube=gp.cube.tolist(), width=gp.width)
-    return base_line(
+    line = base_line(
         "learned-prefilter effective pairs/s (exact gap-block hits <= cutoff, reference "
-        "embeddings built once)",
-        pairs / (ms_step / 1e3), args.steps, args.warmup, ms_step,
-        {"workload": "prefilter_cdr3_families_50000x1000000_blosum62_go28_central21",
+        "embeddings built once; independent synthetic CDR3s)",
+        head["effective_pairs_per_s"], args.steps, args.warmup, head["ms_per_search"],
+        {"workload": "prefilter_cdr3_50000x1000000_blosum62_go28_central21_cutoff48",
          "queries": Q.n, "refs": R.n, "cutoff": args.pf_cutoff, "seed": seed,
-         "train_sequences": len(to) - 1, "train_pairs": int((~hold).sum()),
-         "epochs": args.pf_epochs, "pairs_per_step": pairs, "parallelism": "single GPU"},
+         "train_sequences": len(to) - 1, "train_pairs": len(pa),
+         "mined_close_pairs": int(n_close), "epochs": args.pf_epochs,
+         "pairs_per_step": pairs, "parallelism": "single GPU"},
         kernels, roofline, cpu,
-        recall=rec, exact_hits=int(len(eq)), found_hits=int(len(res.query)),
-        candidates=res.candidates, tau_score=round(tau, 3),
-        exhaustive_pairs_per_s=pairs / ex_s, exhaustive_ms=1e3 * ex_s,
-        speedup_vs_exhaustive=ex_s / (ms_step / 1e3),
-        index_build_ms=ms_build, one_shot_ms=ms_one_shot,
-        speedup_one_shot=ex_s / (ms_one_shot / 1e3),
-        stage_ms={k: round(1e3 * v, 3) for k, v in res.seconds.items()},
+        **{k: v for k, v in head.items() if k not in ("pairs_per_step", "queries", "refs")},
+        planted=per_set["planted"], target="recall >= 0.995 and >= 5x the exhaustive exact pass",
         train_s=round(train_s, 2), data_s=round(data_s, 2), calibration=calib,
-        train_loss=[round(x, 5) for x in losses],
-        label_quantiles={"near": np.quantile(score[:int(PF_PAIRS * PF_NEAR)], [0.1, 0.5, 0.9]).tolist(),
-                         "random": np.quantile(score[int(PF_PAIRS * PF_NEAR):], [0.001, 0.01, 0.5]).tolist()})
+        train_loss=[round(x, 5) for x in losses])
+    line["dtype"] = "bf16"
+    line["arithmetic"] = ("BF16 MFMA with FP32 accumulation (encoder, screen); exact int32 / "
+                          "packed int16 gap-block rescoring")
+    return line
 
 
 def run_distance(torch, dev, lib, stream, args, which: str) -> dict:
@@ -879,35 +1044,112 @@ def run_counts(torch, dev, lib, stream, args) -> dict:
         kernels, roofline, cpu, **extra)
 
 
-def cpu_traceback(qs, rs, pq, pr, seconds: float = 10.0, **kw) -> dict:
-    """The reference's own align_pair (oracle/_ref) over the first pairs of the same list on
-    every host core, both modes, repeated until `seconds` of wall time have passed; the plain-C
-    port (one thread) where the compiled reference is absent."""
+def cpu_traceback(qs, rs, pq, pr, seconds: float = 3.0, **kw) -> dict:
+    """The reference's own align_pair (oracle/_ref) over the first pairs of the same list, both
+    modes, with 16 worker threads and with every usable core; the plain-C port (one thread)
+    where the compiled reference is absent."""
     import oracle
 
-    threads = oracle.host_threads()
+    host = host_cpu()
     have_ref = oracle.reference_available()
-    n = 20_000
-    pairs, calls = 0, 0
-    t0 = time.perf_counter()
-    while True:
+    n = 20_000 if have_ref else 1_000
+
+    def once(threads) -> int:
         for local in (False, True):
             if have_ref:
                 oracle.Reference().align_pairs(qs, rs, pq[:n], pr[:n], "blosum62", local,
                                                 GAP_OPEN, GAP_EXTEND, threads=threads)
             else:
                 orc = oracle.Oracle()
-                for t in range(n // 20):
+                for t in range(n):
                     orc.align_pair(qs[pq[t]], rs[pr[t]], kw["sim"], local, GAP_OPEN, GAP_EXTEND)
-            pairs += n if have_ref else n // 20
-        calls += 1
-        dt = time.perf_counter() - t0
-        if dt >= seconds:
-            break
-    return {"value": pairs / dt, "unit": "alignments/s",
-            "cores": threads if have_ref else 1, "kind": "reference" if have_ref else "port",
-            "sample": f"the first {n if have_ref else n // 20} pairs of the same list, global + "
-                      f"local, {calls} rounds: {pairs} alignments in {dt:.2f} s wall"}
+        return 2 * n
+
+    threads = CPU_THREADS if have_ref else 1
+    rate, calls, pairs, dt = timed_rate(lambda: once(threads), seconds)
+    out = {"value": rate, "unit": "alignments/s", "cores": threads,
+           "kind": "reference" if have_ref else "port",
+           "sample": f"the first {n} pairs of the same list, global + local, {calls} rounds "
+                     f"after one warm-up: {pairs} alignments in {dt:.2f} s wall", "host": host}
+    if have_ref and host["usable_cpus"] != CPU_THREADS:
+        all_rate, _, _, _ = timed_rate(lambda: once(host["usable_cpus"]), seconds)
+        out["all_cores"] = {"value": all_rate, "cores": host["usable_cpus"]}
+    return out
+
+
+RUNNERS = {
+    "gotoh": run_gotoh,            # C2 (the headline)
+    "gapblock": run_gapblock,      # C1
+    "hamming": lambda *a: run_distance(*a, "hamming"),          # C3
+    "levenshtein": lambda *a: run_distance(*a, "levenshtein"),  # C3
+    "batch": run_batch,            # C4
+    "prefilter": run_prefilter,    # C5
+    "traceback": run_traceback,
+    "counts": run_counts,
+    "germline": run_germline,
+}
+
+# The arithmetic each configuration's dominant kernel computes in (base_line's default int32 is
+# the output type).
+ARITHMETIC = {
+    "gapblock": ("int16", "twin 16-bit rows (two queries per register, packed int16 adds and "
+                          "mins; exact under the host's range proof, int32 past it)"),
+    "batch": ("int16", "twin 16-bit rows for the short jobs, int32 otherwise (range-proven)"),
+    "counts": ("int16", "twin 16-bit rows (range-proven), LDS uint32 histograms"),
+    "hamming": ("u8", "byte compares as XOR / zero-byte tests on packed 32-bit words"),
+    "levenshtein": ("u64", "Myers bit-vectors in 64-bit words (block carries up to 4 words)"),
+    "traceback": ("int32", "int32 Gotoh cells with one direction byte per cell (range-proven; "
+                           "int64 past it)"),
+}
+
+COMPACT_EXTRAS = ("gcups", "glookups", "end_to_end", "recall", "exact_hits", "found_hits",
+                  "candidates", "tau_score", "speedup_search_only", "speedup_one_shot",
+                  "index_build_ms", "one_shot_ms", "exhaustive_ms", "target",
+                  "histogram_roofline")
+
+
+def compact(line: dict) -> dict:
+    """One configuration's entry of the all-configs line: its value, step time, the roofline
+    of its dominant kernel and its CPU baseline, without the per-kernel breakdown."""
+    out = {k: line[k] for k in ("metric", "value", "unit", "ms_per_step", "dtype",
+                                "arithmetic") if k in line}
+    out["workload"] = line["config"]["workload"]
+    rf = line.get("roofline")
+    if rf:
+        out["roofline"] = {k: rf[k] for k in ("bound", "kernel", "achieved", "peak", "unit",
+                                              "frac", "traffic", "avg_launch_ms") if k in rf}
+    cb = line.get("cpu_baseline")
+    if cb:
+        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample",
+                                                  "all_cores") if k in cb}
+    for k in COMPACT_EXTRAS:
+        if line.get(k) is not None:
+            out[k] = line[k]
+    if "planted" in line:
+        out["planted"] = {k: line["planted"][k] for k in (
+            "effective_pairs_per_s", "ms_per_search", "recall", "exact_hits", "candidates",
+            "speedup_search_only", "speedup_one_shot", "exhaustive_ms", "one_shot_ms")}
+    return out
+
+
+def finish_roofline(line: dict, lib) -> None:
+    """The measured peak beside the published one for the dominant kernel's bound."""
+    rf = line.get("roofline")
+    if not rf:
+        return
+    if rf["bound"] == "valu":
+        rf["probe_tops"] = valu_probe(lib)
+    elif rf["bound"] == "mfma":
+        rate = ctypes.c_double()
+        st = Status()
+        if lib.lib.st_probe_mfma(ctypes.byref(rate), ctypes.byref(st)) == 0:
+            rf["probe_tflops_bf16_32x32x16"] = round(rate.value / 1e12, 1)
+    elif rf["bound"] == "hbm":
+        # the attainable write rate: a plain 16-byte-store fill of 4 GiB
+        rate = ctypes.c_double()
+        st = Status()
+        if lib.lib.st_probe_hbm_write(4 << 30, ctypes.byref(rate), ctypes.byref(st)) == 0:
+            rf["probe_write_gbs"] = round(rate.value / 1e9, 1)
 
 
 def main() -> int:
@@ -917,11 +1159,13 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-rows", type=int, default=1000,
                     help="query rows of the CPU baseline sample (0 skips it)")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0,
-                    help="wall time the CPU baseline sample is repeated for")
-    ap.add_argument("--workload", default="gotoh",
-                    choices=["gotoh", "gapblock", "hamming", "levenshtein", "batch",
-                             "traceback", "counts", "germline", "prefilter"])
+    ap.add_argument("--cpu-seconds", type=float, default=3.0,
+                    help="wall time each CPU baseline sample is repeated for")
+    ap.add_argument("--workload", default="all", choices=["all", *RUNNERS],
+                    help="'all' (default): every configuration, the Gotoh dist_matrix (C2) as "
+                         "the headline value and the rest under 'configs'")
+    ap.add_argument("--no-end-to-end", dest="end_to_end", action="store_false",
+                    help="skip the end-to-end (Python call, host output) timings")
     ap.add_argument("--pf-epochs", type=int, default=PF_EPOCHS)
     ap.add_argument("--pf-cutoff", type=int, default=PF_CUTOFF)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -944,35 +1188,36 @@ def main() -> int:
     dev = torch.device("cuda:0")
     lib = Lib()
     stream = torch.cuda.Stream(device=dev)
-    if args.workload == "gotoh":
-        line = run_gotoh(torch, dev, lib, stream, args)
-    elif args.workload == "gapblock":
-        line = run_gapblock(torch, dev, lib, stream, args)
-    elif args.workload == "batch":
-        line = run_batch(torch, dev, lib, stream, args)
-    elif args.workload == "prefilter":
-        line = run_prefilter(torch, dev, lib, stream, args)
-    elif args.workload == "traceback":
-        line = run_traceback(torch, dev, lib, stream, args)
-    elif args.workload == "counts":
-        line = run_counts(torch, dev, lib, stream, args)
-    elif args.workload == "germline":
-        line = run_germline(torch, dev, lib, stream, args)
-    else:
-        line = run_distance(torch, dev, lib, stream, args, args.workload)
-    if line.get("roofline") and line["roofline"]["bound"] == "valu":
-        line["roofline"]["probe_tops"] = valu_probe(lib)
-    if line.get("roofline") and line["roofline"]["bound"] == "mfma":
-        rate = ctypes.c_double()
-        st = Status()
-        if lib.lib.st_probe_mfma(ctypes.byref(rate), ctypes.byref(st)) == 0:
-            line["roofline"]["probe_tflops_bf16_32x32x16"] = round(rate.value / 1e12, 1)
-    if line.get("roofline") and line["roofline"]["bound"] == "hbm":
-        # the attainable write rate: a plain 16-byte-store fill of 4 GiB
-        rate = ctypes.c_double()
-        st = Status()
-        if lib.lib.st_probe_hbm_write(4 << 30, ctypes.byref(rate), ctypes.byref(st)) == 0:
-            line["roofline"]["probe_write_gbs"] = round(rate.value / 1e9, 1)
+    if args.workload != "all":
+        line = RUNNERS[args.workload](torch, dev, lib, stream, args)
+        finish_roofline(line, lib)
+        print(json.dumps(line))
+        return 0
+    t_all = time.perf_counter()
+    line = run_gotoh(torch, dev, lib, stream, args)
+    finish_roofline(line, lib)
+    line.pop("kernels_ms", None)
+    configs = {}
+    for name, fn in RUNNERS.items():
+        if name == "gotoh":
+            continue
+        t0 = time.perf_counter()
+        print(f"bench.py: {name} ...", file=sys.stderr, flush=True)
+        try:
+            sub = fn(torch, dev, lib, stream, args)
+            if name in ARITHMETIC:
+                sub["dtype"], sub["arithmetic"] = ARITHMETIC[name]
+            finish_roofline(sub, lib)
+            configs[name] = compact(sub)
+        except Exception as e:  # noqa: BLE001 -- one configuration's failure is reported
+            configs[name] = {"error": f"{type(e).__name__}: {e}"}
+        configs.setdefault(name, {})["wall_s"] = round(time.perf_counter() - t0, 1)
+    try:
+        configs["scalar_latency"] = scalar_latency(args)
+    except Exception as e:  # noqa: BLE001
+        configs["scalar_latency"] = {"error": f"{type(e).__name__}: {e}"}
+    line["configs"] = configs
+    line["bench_wall_s"] = round(time.perf_counter() - t_all, 1)
     print(json.dumps(line))
     return 0
 

@@ -365,6 +365,14 @@ int32_t st_prefilter_screen(const void* q_emb, const float* q_norm, uint32_t n_q
                             uint64_t capacity, uint32_t* hit_query, uint32_t* hit_ref,
                             uint64_t* hit_count, void* stream, st_status* status);
 
+/* The model distance |q|^2 + |r|^2 - 2 q.r (FP32 sum of the BF16 products) of each listed
+ * (query, reference) pair of two encoded sets, into distance[t] (device): what the screen
+ * estimates for that pair, used to calibrate its threshold on the exact hits of a sample. */
+int32_t st_prefilter_pair_distance(const void* q_emb, const float* q_norm, const void* r_emb,
+                                   const float* r_norm, const uint32_t* pair_query,
+                                   const uint32_t* pair_ref, uint64_t n_pairs, float* distance,
+                                   void* stream, st_status* status);
+
 /* Indices t < n with values[t] <= cutoff (device int32 values), in no particular order; the
  * first `capacity` go to index (device) and *count (host) receives the total. Used to keep the
  * exactly rescored prefilter candidates that are real hits. */

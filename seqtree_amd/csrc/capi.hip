@@ -1487,6 +1487,26 @@ int32_t st_prefilter_get_grads(const st_prefilter* model, float* grads, uint64_t
     });
 }
 
+int32_t st_prefilter_pair_distance(const void* q_emb, const float* q_norm, const void* r_emb,
+                                   const float* r_norm, const uint32_t* pair_query,
+                                   const uint32_t* pair_ref, uint64_t n_pairs, float* distance,
+                                   void* stream, st_status* status) {
+    return guarded(status, [&] {
+        require_device();
+        if (n_pairs >= (uint64_t(1) << 33))
+            throw HipError{ST_INVALID_ARGUMENT, "seqtree_hip: too many pairs for one call"};
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        {
+            LaunchTimer t("prefilter_pair_distance", s);
+            pf::launch_pair_distance(static_cast<const __bf16*>(q_emb), q_norm,
+                                     static_cast<const __bf16*>(r_emb), r_norm, pair_query,
+                                     pair_ref, n_pairs, distance, s);
+        }
+        launched("prefilter_pair_distance");
+        return ok(status);
+    });
+}
+
 int32_t st_select_le(const int32_t* values, uint64_t n, int32_t cutoff, uint64_t capacity,
                      uint32_t* index, uint64_t* count, void* stream, st_status* status) {
     return st_select_pairs_le(values, nullptr, nullptr, n, cutoff, capacity, index, nullptr,

@@ -1026,6 +1026,23 @@ __global__ __launch_bounds__(256) void select_le_kernel(const int32_t* __restric
 }
 
 // dst[i] = src[min(i * stride, last)] for i < count (block boundaries of an offsets array).
+// Model distance |q|^2 + |r|^2 - 2 q.r of listed (query, reference) pairs: one wave per pair,
+// two channels per lane, the dot product summed in FP32 across the wave.
+__global__ __launch_bounds__(256) void pair_distance_kernel(
+    const __bf16* __restrict__ Q, const float* __restrict__ qn, const __bf16* __restrict__ Rm,
+    const float* __restrict__ rn, const uint32_t* __restrict__ pq, const uint32_t* __restrict__ pr,
+    uint64_t n, float* __restrict__ out) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t t = (uint64_t(blockIdx.x) * 256 + threadIdx.x) >> 6;
+    if (t >= n) return;  // whole waves leave together (t is uniform across a wave)
+    const uint32_t a = pq[t], b = pr[t];
+    const __bf16* x = Q + uint64_t(a) * kChannels + 2 * lane;
+    const __bf16* y = Rm + uint64_t(b) * kChannels + 2 * lane;
+    float dot = float(x[0]) * float(y[0]) + float(x[1]) * float(y[1]);
+    for (int off = 32; off > 0; off >>= 1) dot += __shfl_xor(dot, off);
+    if (lane == 0) out[t] = qn[a] + rn[b] - 2.0f * dot;
+}
+
 __global__ void gather_offsets_kernel(const uint64_t* __restrict__ src, uint32_t stride,
                                       uint32_t last, uint32_t count, uint64_t* __restrict__ dst) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
@@ -1051,6 +1068,15 @@ void launch_select_le(const int32_t* v, uint64_t n, int32_t cutoff, const uint32
     hipLaunchKernelGGL(select_le_kernel, dim3(uint32_t(blocks)), dim3(256), 0, s, v, n, cutoff, per,
                        pa, pb, out_a, out_b, out_v, cap, count);
 }
+void launch_pair_distance(const __bf16* Q, const float* q_norm, const __bf16* Rm,
+                          const float* r_norm, const uint32_t* pq, const uint32_t* pr, uint64_t n,
+                          float* out, hipStream_t s) {
+    if (n == 0) return;
+    const uint64_t blocks = (n + 3) / 4;  // four waves (pairs) per workgroup
+    hipLaunchKernelGGL(pair_distance_kernel, dim3(uint32_t(blocks)), dim3(256), 0, s, Q, q_norm,
+                       Rm, r_norm, pq, pr, n, out);
+}
+
 void launch_embed(const uint8_t* codes, const uint64_t* off, uint32_t n, const __bf16* emb,
                   __bf16* X0, uint8_t* valid, uint32_t* tile_first, hipStream_t s) {
     const uint64_t total = uint64_t(n) * kEmbDim;

@@ -130,6 +130,10 @@ void launch_pool(const __bf16* H, const uint64_t* off, uint32_t n, float* ef, __
 void launch_gram(const __bf16* Q, const float* q_norm, uint32_t nq, const __bf16* Rm,
                  const float* r_norm, uint32_t nr, float tau, uint32_t* hit_q, uint32_t* hit_r,
                  uint64_t cap, unsigned long long* count, uint32_t cus, hipStream_t s);
+// Model distances |q|^2 + |r|^2 - 2 q.r of the pairs (Q[pq[t]], Rm[pr[t]]), FP32; n < 2^32 * 4.
+void launch_pair_distance(const __bf16* Q, const float* q_norm, const __bf16* Rm,
+                          const float* r_norm, const uint32_t* pq, const uint32_t* pr, uint64_t n,
+                          float* out, hipStream_t s);
 void launch_pair_loss(const float* ef, const uint32_t* pa, const uint32_t* pb, const float* y,
                       const float* w, uint32_t n_pairs, float* dE, float* loss, hipStream_t s);
 void launch_pool_bwd(const __bf16* H, const float* dE, const uint64_t* off, uint32_t n,

@@ -11,7 +11,10 @@ on pairs whose labels come from the exact gap-block kernel itself. A search then
 3. rescores the survivors with the exact gap-block pair kernel and keeps those <= cutoff.
 
 Every reported score is exact; what the screen can cost is recall, which the exhaustive exact
-pass (``st_gapblock_threshold_device``) measures. The reference has no prefilter: its
+pass (``st_gapblock_threshold_device``) measures. Training pairs are mined by that same
+exhaustive pass over the training set (every close pair, a band of near misses, random
+pairs), and the margin is calibrated twice: on held-out training pairs, and at search time on
+the exact hits of a small exhaustively searched sample of the queries (``audit``). The reference has no prefilter: its
 ``GapBlockIndex.search`` (gapblock.py:567-673) answers the same question with a
 deletion-variant trie.
 
@@ -60,6 +63,8 @@ def _bind():
     L.st_prefilter_screen.restype = i32
     L.st_prefilter_screen.argtypes = [vp, vp, u32, vp, vp, u32, f32, u64, vp, vp,
                                       ctypes.POINTER(u64), vp, S]
+    L.st_prefilter_pair_distance.restype = i32
+    L.st_prefilter_pair_distance.argtypes = [vp, vp, vp, vp, vp, vp, u64, vp, vp, S]
     L.st_select_le.restype = i32
     L.st_select_le.argtypes = [vp, u64, i32, u64, vp, ctypes.POINTER(u64), vp, S]
     L.st_select_pairs_le.restype = i32
@@ -83,6 +88,7 @@ class DeviceSet:
     offsets: DeviceArray
     n: int
     max_len: int
+    host: tuple | None = None  # (codes, offsets) on the host, when uploaded from there
 
     @classmethod
     def upload(cls, codes: np.ndarray, offsets: np.ndarray) -> "DeviceSet":
@@ -90,7 +96,18 @@ class DeviceSet:
         c = codes if len(codes) else np.zeros(1, np.uint8)
         return cls(DeviceArray.from_numpy(np.ascontiguousarray(c, np.uint8)),
                    DeviceArray.from_numpy(np.ascontiguousarray(offsets, np.uint64)),
-                   len(offsets) - 1, int(lens.max(initial=0)))
+                   len(offsets) - 1, int(lens.max(initial=0)), (codes, offsets))
+
+    def subset(self, idx: np.ndarray) -> "DeviceSet":
+        """The sequences idx (host indices) as a new device set; needs the host copies."""
+        codes, off = self.host
+        off = off.astype(np.int64)
+        lens = off[idx + 1] - off[idx]
+        sub_off = np.zeros(len(idx) + 1, np.uint64)
+        sub_off[1:] = np.cumsum(lens)
+        total = int(sub_off[-1])
+        take = np.repeat(off[idx] - sub_off[:-1].astype(np.int64), lens) + np.arange(total)
+        return DeviceSet.upload(codes[take], sub_off)
 
 
 @dataclass
@@ -238,6 +255,23 @@ def screen(q_emb, q_norm, nq: int, r_emb, r_norm, nr: int, tau: float, capacity:
         out = None
 
 
+def pair_distances(q_emb, q_norm, r_emb, r_norm, pq: np.ndarray, pr: np.ndarray) -> np.ndarray:
+    """Model distances (score units) of the pairs (queries pq[t], references pr[t]) of two
+    encoded sets, as the screen estimates them."""
+    L = _bind()
+    n = len(pq)
+    if n == 0:
+        return np.zeros(0, np.float32)
+    dq = DeviceArray.from_numpy(np.ascontiguousarray(pq, np.uint32))
+    dr = DeviceArray.from_numpy(np.ascontiguousarray(pr, np.uint32))
+    out = DeviceArray(n, np.float32)
+    st = Status()
+    check(L.st_prefilter_pair_distance(q_emb.ptr, q_norm.ptr, r_emb.ptr, r_norm.ptr, dq.ptr,
+                                       dr.ptr, n, out.ptr, None, ctypes.byref(st)), st,
+          "st_prefilter_pair_distance")
+    return out.numpy() * np.float32(SCORE_UNIT)
+
+
 def select_le(values: DeviceArray, n: int, cutoff: int) -> np.ndarray:
     """Host indices t < n with values[t] <= cutoff (sorted)."""
     L = _bind()
@@ -284,6 +318,31 @@ def sample_pairs(rng: np.random.Generator, fam: np.ndarray, n_pairs: int, near_f
     ra = rng.integers(0, n, size=n_pairs - n_near)
     rb = rng.integers(0, n, size=n_pairs - n_near)
     return np.concatenate([a, ra]).astype(np.int64), np.concatenate([b, rb]).astype(np.int64)
+
+
+def mine_pairs(rng: np.random.Generator, codes: np.ndarray, off: np.ndarray,
+               gp: GapblockParams, n_pairs: int, near_frac: float, cutoff: int,
+               mine_cutoff: int):
+    """Training pairs of one set: every pair with exact score <= mine_cutoff comes from one
+    exhaustive thresholded pass over the set (the exact kernel), and near_frac of the pairs
+    are drawn from those -- every pair <= cutoff up to half of them, the rest from the band
+    (cutoff, mine_cutoff] -- the remainder uniformly from all pairs. Returns (a, b, the
+    number of mined pairs <= cutoff)."""
+    s_all = DeviceSet.upload(codes, off)
+    a, b, sc = exact_threshold(s_all, s_all, mine_cutoff, gp)
+    keep = a < b  # each unordered pair once, no self pairs
+    a, b, sc = a[keep], b[keep], sc[keep]
+    close = np.flatnonzero(sc <= cutoff)
+    band = np.flatnonzero(sc > cutoff)
+    n_near = int(n_pairs * near_frac)
+    k_close = min(len(close), n_near // 2)
+    pick = np.concatenate([rng.choice(close, k_close, replace=False),
+                           rng.choice(band, min(len(band), n_near - k_close), replace=False)])
+    n = len(off) - 1
+    n_rand = n_pairs - len(pick)
+    pa = np.concatenate([a[pick], rng.integers(0, n, size=n_rand)]).astype(np.int64)
+    pb = np.concatenate([b[pick], rng.integers(0, n, size=n_rand)]).astype(np.int64)
+    return pa, pb, len(close)
 
 
 def pair_weights(score: np.ndarray) -> np.ndarray:
@@ -376,6 +435,21 @@ def calibrate_tau(model_dist: np.ndarray, exact: np.ndarray, cutoff: int,
     return float(max(cutoff, np.quantile(close, quantile)))
 
 
+def calibrate_on_set(model: Prefilter, codes: np.ndarray, off: np.ndarray, gp: GapblockParams,
+                     cutoff: int, quantile: float, max_len: int):
+    """Screening threshold from a calibration set of sequences the model was not trained on:
+    every pair of the set with exact score <= cutoff (one exhaustive thresholded pass), and the
+    `quantile` of their model distances (calibrate_tau). Returns (tau, the number of close
+    pairs). Sequences unseen in training make the margin hold for new queries and references,
+    which held-out pairs of training sequences understate."""
+    s_all = DeviceSet.upload(codes, off)
+    a, b, sc = exact_threshold(s_all, s_all, cutoff, gp)
+    keep = a < b
+    a, b, sc = a[keep].astype(np.int64), b[keep].astype(np.int64), sc[keep]
+    dist = embed_distances(model, codes, off, a, b, max_len)
+    return calibrate_tau(dist, sc, cutoff, quantile), len(sc)
+
+
 @dataclass
 class SearchResult:
     query: np.ndarray
@@ -383,6 +457,10 @@ class SearchResult:
     score: np.ndarray
     candidates: int
     seconds: dict
+    tau: float = 0.0           # the screening threshold used (score units)
+    audit_queries: int = 0     # queries searched exhaustively to calibrate it
+    audit_hits: int = 0        # their exact hits
+    audit_within: float = 1.0  # fraction of those hits the threshold keeps
 
 
 class Searcher:
@@ -408,16 +486,36 @@ class Searcher:
         _device.synchronize()
         return time.perf_counter() - t0
 
-    def query(self, q: DeviceSet, cutoff: int, tau_score: float) -> SearchResult:
+    def query(self, q: DeviceSet, cutoff: int, tau_score: float, audit: float = 0.0,
+              quantile: float = 0.999, seed: int = 0) -> SearchResult:
+        """With audit > 0, a random sample of that fraction of the queries (q needs its host
+        copies) is first searched exhaustively with the exact kernel; the screening threshold
+        becomes the larger of tau_score and the `quantile` of the model distances of the
+        sample's exact hits, so it is calibrated on this query and reference distribution
+        rather than only on the training pairs'."""
         L = _bind()
         t = {}
         t0 = time.perf_counter()
         qe, qn = self.model.encode(q, self.max_len)
         _device.synchronize()
         t["encode_queries"] = time.perf_counter() - t0
+        tau = float(tau_score)
+        n_audit, audit_hits, within = 0, 0, 1.0
+        if audit > 0:
+            t0 = time.perf_counter()
+            n_audit = max(1, min(q.n, int(round(audit * q.n))))
+            idx = np.sort(np.random.default_rng(seed).choice(q.n, n_audit, replace=False))
+            aq, ar, _ = exact_threshold(q.subset(idx), self.refs, cutoff, self.gp)
+            d = pair_distances(qe, qn, self.r_emb, self.r_norm, idx[aq], ar)
+            audit_hits = len(d)
+            if audit_hits:
+                # a little headroom for the screen's different FP32 summation order
+                tau = max(tau, float(np.quantile(d, quantile)) * 1.001)
+                within = float((d <= tau).mean())
+            t["audit"] = time.perf_counter() - t0
         t0 = time.perf_counter()
         cq, cr, nc = screen(qe, qn, q.n, self.r_emb, self.r_norm, self.refs.n,
-                            tau_score / SCORE_UNIT, self._cap, self._cand)
+                            tau / SCORE_UNIT, self._cap, self._cand)
         self._cand, self._cap = (cq, cr), cq.shape[0]
         t["screen"] = time.perf_counter() - t0
         t0 = time.perf_counter()
@@ -439,15 +537,17 @@ class Searcher:
                 break
             self._hit_cap = k
         t["rescore"] = time.perf_counter() - t0
-        return SearchResult(hq.numpy()[:k], hr.numpy()[:k], hs.numpy()[:k], nc, t)
+        return SearchResult(hq.numpy()[:k], hr.numpy()[:k], hs.numpy()[:k], nc, t, tau,
+                            n_audit, audit_hits, within)
 
 
 def search(model: Prefilter, q: DeviceSet, r: DeviceSet, cutoff: int, tau_score: float,
-           gp: GapblockParams, max_len: int) -> SearchResult:
+           gp: GapblockParams, max_len: int, audit: float = 0.0,
+           quantile: float = 0.999) -> SearchResult:
     """One-shot search: index the references, then query (see Searcher)."""
     s = Searcher(model, gp, max_len)
     t_index = s.index(r)
-    res = s.query(q, cutoff, tau_score)
+    res = s.query(q, cutoff, tau_score, audit=audit, quantile=quantile)
     res.seconds["encode_refs"] = t_index
     return res
 

@@ -113,3 +113,16 @@ def decode(codes: np.ndarray, off: np.ndarray) -> list[str]:
     lut = np.frombuffer(AA_ORDER.encode(), np.uint8)
     return [lut[codes[int(off[i]):int(off[i + 1])]].tobytes().decode()
             for i in range(len(off) - 1)]
+
+
+def training_set(rng: np.random.Generator, n_iid: int, n_families: int, per_family: int,
+                 lo: int, hi: int):
+    """A prefilter training set of CDR3-like sequences: n_iid independent draws of the
+    generator followed by n_families clonal families of per_family members, as (codes,
+    offsets). Independent draws supply the chance similarities of unrelated sequences, the
+    families the close pairs that differ by point edits, insertions and deletions included."""
+    c1, o1, _ = cdr3_csr(rng, n_iid, lo, hi)
+    if n_families == 0:
+        return c1, o1
+    c2, o2, _ = families(rng, n_families, per_family, lo, hi)
+    return np.concatenate([c1, c2]), np.concatenate([o1, o1[-1] + o2[1:]])

@@ -252,5 +252,89 @@ def test_small_search_reports_only_exact_hits(gpu):
     exact = {(int(a), int(b)): int(s) for a, b, s in zip(eq, er, es)}
     for a, b, s in zip(res.query, res.ref, res.score):
         assert exact[(int(a), int(b))] == int(s)
-    assert 0.0 <= recall(res.query, res.ref, eq, er) <= 1.0
+    assert len(set(zip(res.query.tolist(), res.ref.tolist()))) == len(res.query)  # no repeats
     model.close()
+
+
+# ------------------------------------------------------------------- C5 at a scaled size
+# The C5 recipe (200,000 synthetic CDR3s -- independent draws and families of 4 -- of which
+# 180,000 train on 2M pairs mined and labelled by the exact kernel for 8 epochs and 20,000
+# calibrate the margin) and its cutoff, searched with
+# 5,000 queries x 100,000 references of two kinds: independent CDR3s (the generator alone) and
+# planted families (each query a fresh mutant of one reference family's root).
+C5_LEN, C5_CUTOFF = (10, 20), 48
+
+
+@pytest.fixture(scope="module")
+def c5_model(gpu):
+    from seqtree_amd import synthetic as syn
+    from seqtree_amd.prefilter import (GapblockParams, Prefilter, TrainingData, calibrate_on_set,
+                                       label_pairs, mine_pairs, train)
+
+    rng = np.random.default_rng(2025)
+    # 180,000 training sequences and 20,000 held out for the calibration
+    tc, to = syn.training_set(rng, 90_000, 22_500, 4, *C5_LEN)
+    hc, ho = syn.training_set(rng, 10_000, 2_500, 4, *C5_LEN)
+    gp = GapblockParams.blosum62_central(C5_LEN[1])
+    pa, pb, n_close = mine_pairs(rng, tc, to, gp, 2_000_000, 0.4, C5_CUTOFF, 2 * C5_CUTOFF)
+    assert n_close > 10_000
+    score = label_pairs(tc, to, pa, pb, gp)
+    model = Prefilter(seed=2025)
+    losses = train(model, TrainingData(tc, to, pa, pb, score), epochs=8, max_len=C5_LEN[1],
+                   seed=2025)
+    assert losses[-1] < 0.5 * losses[0]
+    tau, n_cal = calibrate_on_set(model, hc, ho, gp, C5_CUTOFF, 0.999, C5_LEN[1])
+    assert n_cal > 1000
+    yield model, gp, tau
+    model.close()
+
+
+def _c5_sets(kind, rng):
+    from seqtree_amd import synthetic as syn
+
+    if kind == "iid":
+        rc, ro, _ = syn.cdr3_csr(rng, 100_000, *C5_LEN)
+        qc, qo, _ = syn.cdr3_csr(rng, 5_000, *C5_LEN)
+        return qc, qo, rc, ro
+    roots = []
+    rc, ro, _ = syn.families(rng, 25_000, 4, *C5_LEN, roots_out=roots)
+    pick = rng.choice(25_000, size=5_000, replace=False)
+    qc, qo = syn.join(syn._mutate_many(rng, [roots[f] for f in pick], 1, *C5_LEN))
+    return qc, qo, rc, ro
+
+
+@pytest.mark.parametrize("kind", ["iid", "planted"])
+def test_scaled_c5_recall(c5_model, oracle_lib, kind):
+    """Every reported hit is an exact hit with its exact score, the exhaustive pass equals the
+    oracle on sampled rows, and the prefilter finds at least 99.5 % of the exhaustive hits."""
+    from seqtree_amd import synthetic as syn
+    from seqtree_amd.prefilter import DeviceSet, exact_threshold, recall, search
+
+    model, gp, tau = c5_model
+    rng = np.random.default_rng(7 if kind == "iid" else 8)
+    qc, qo, rc, ro = _c5_sets(kind, rng)
+    Q, R = DeviceSet.upload(qc, qo), DeviceSet.upload(rc, ro)
+    print(f"scaled C5 {kind}: {Q.n} x {R.n}, held-out tau {tau:.1f}", flush=True)
+    eq, er, es = exact_threshold(Q, R, C5_CUTOFF, gp)
+    assert len(eq) > 1000, len(eq)
+    # the exhaustive pass against the oracle on sampled query rows
+    qs, rs = syn.decode(qc, qo), syn.decode(rc, ro)
+    rows = rng.choice(Q.n, size=4, replace=False)
+    dense = np.asarray(oracle_lib.gapblock_matrix([qs[i] for i in rows], rs, gp.pen,
+                                                  gp.gap_open, gp.gap_extend, gp.cube,
+                                                  gp.width)).reshape(len(rows), R.n)
+    for k, i in enumerate(rows):
+        want = {(int(j), int(dense[k, j])) for j in np.flatnonzero(dense[k] <= C5_CUTOFF)}
+        got = {(int(j), int(s)) for j, s in zip(er[eq == i], es[eq == i])}
+        assert got == want, (kind, i)
+    res = search(model, Q, R, C5_CUTOFF, tau, gp, C5_LEN[1], audit=0.02, quantile=0.999)
+    exact = dict(zip(((eq.astype(np.int64) << 32) | er).tolist(), es.tolist()))
+    keys = ((res.query.astype(np.int64) << 32) | res.ref).tolist()
+    assert len(set(keys)) == len(keys)
+    assert all(exact.get(k_) == int(s) for k_, s in zip(keys, res.score.tolist()))
+    rec = recall(res.query, res.ref, eq, er)
+    print(f"  exact hits {len(eq)}, found {len(res.query)}, recall {rec:.4f}, candidates "
+          f"{res.candidates}, tau {res.tau:.1f}, audit {res.audit_queries} queries / "
+          f"{res.audit_hits} hits", flush=True)
+    assert res.candidates < 0.01 * Q.n * R.n  # the screen does screen
+    assert rec >= 0.995, (kind, rec)
