@@ -16,10 +16,12 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <condition_variable>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 // Device scratch of st_gapblock_matrix_device_noalloc: one array per GPU (each device holds
@@ -118,6 +120,12 @@ public:
     Arena(hipStream_t s, void* region, size_t bytes)
         : s_(s), block_(static_cast<char*>(region)), fixed_bytes_(region ? bytes : 0) {}
     bool fixed() const { return fixed_bytes_ != 0; }
+    // A device region (owned by the caller, reused stream-ordered on s) that requests use
+    // first, falling back to allocations once it is full.
+    void prefer(void* region, size_t bytes) {
+        pref_ = static_cast<char*>(region);
+        pref_bytes_ = bytes;
+    }
     ~Arena() {
         if (bufs_.empty()) return;
         if (pools_supported()) {
@@ -138,6 +146,11 @@ public:
                                    " MiB of static scratch"};
             T* p = reinterpret_cast<T*>(block_ + used_);
             used_ += bytes;
+            return p;
+        }
+        if (pref_ && pref_used_ + bytes <= pref_bytes_) {
+            T* p = reinterpret_cast<T*>(pref_ + pref_used_);
+            pref_used_ += bytes;
             return p;
         }
         if (bytes > kSmall) return static_cast<T*>(raw(bytes));
@@ -169,19 +182,8 @@ private:
     char* block_ = nullptr;
     size_t used_ = 0;
     size_t fixed_bytes_ = 0;
-};
-
-class OwnedStream {
-public:
-    OwnedStream() { check(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking), "stream"); }
-    ~OwnedStream() {
-        (void)hipStreamSynchronize(s_);
-        (void)hipStreamDestroy(s_);
-    }
-    hipStream_t get() const { return s_; }
-
-private:
-    hipStream_t s_ = nullptr;
+    char* pref_ = nullptr;
+    size_t pref_bytes_ = 0, pref_used_ = 0;
 };
 
 template <class T>
@@ -195,6 +197,221 @@ void d2h(T* dst, const T* src, size_t count, hipStream_t s) {
 }
 
 void sync(hipStream_t s) { check(hipStreamSynchronize(s), "hipStreamSynchronize"); }
+
+// ---- host <-> device transfers of the host-memory entry points ----------------------------
+// Every host buffer a host-memory entry point reads or writes goes through pinned staging owned
+// by the library: the runtime's own path for pageable memory is slower, and where it pins the
+// caller's pages in place (large page-aligned buffers) the platform may refuse it. Copies
+// between the pinned slots and the caller's memory are split over a small pool of threads.
+
+// Persistent worker threads for host memcpy: copy() splits one copy into equal parts, the
+// calling thread takes the first, and returns when every part is done. One copy at a time.
+class CopyPool {
+public:
+    static CopyPool& get() {
+        static CopyPool* pool = new CopyPool();  // never destroyed: workers live with the process
+        return *pool;
+    }
+    void copy(void* dst, const void* src, size_t bytes) {
+        const size_t parts = bytes < (size_t(1) << 20) ? 1 : std::min<size_t>(workers_ + 1,
+                                                                               bytes >> 19);
+        if (parts <= 1) {
+            std::memcpy(dst, src, bytes);
+            return;
+        }
+        std::lock_guard<std::mutex> one(copy_mu_);
+        const size_t step = (bytes / parts + 63) & ~size_t(63);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            dst_ = static_cast<char*>(dst);
+            src_ = static_cast<const char*>(src);
+            bytes_ = bytes;
+            step_ = step;
+            next_ = 1;
+            parts_ = parts;
+            pending_ = parts - 1;
+            ++generation_;
+        }
+        cv_.notify_all();
+        std::memcpy(dst, src, std::min(step, bytes));
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [&] { return pending_ == 0; });
+    }
+
+private:
+    CopyPool() {
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        workers_ = std::min(7u, hw > 1 ? hw - 1 : 0u);
+        for (unsigned w = 0; w < workers_; ++w) std::thread([this] { run(); }).detach();
+    }
+    void run() {
+        uint64_t seen = 0;
+        for (;;) {
+            size_t part;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return generation_ != seen && next_ < parts_; });
+                part = next_++;
+                if (next_ >= parts_) seen = generation_;
+            }
+            const size_t at = part * step_;
+            if (at < bytes_) std::memcpy(dst_ + at, src_ + at, std::min(step_, bytes_ - at));
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--pending_ == 0) done_.notify_all();
+        }
+    }
+    unsigned workers_ = 0;
+    std::mutex copy_mu_, mu_;
+    std::condition_variable cv_, done_;
+    char* dst_ = nullptr;
+    const char* src_ = nullptr;
+    size_t bytes_ = 0, step_ = 0, next_ = 0, parts_ = 0, pending_ = 0;
+    uint64_t generation_ = 0;
+};
+
+// One transfer context: a stream, two pinned staging slots with an event each, and a small
+// device block for the scratch of small calls. A host-memory call holds one for its duration
+// (HostCall); the pool keeps them for reuse, so it grows to the number of concurrent calls.
+constexpr size_t kStageBytes = size_t(16) << 20;
+constexpr size_t kContextScratch = size_t(4) << 20;
+
+struct HostContext {
+    int device = 0;
+    hipStream_t stream = nullptr;  // uploads and scoring
+    hipStream_t copy = nullptr;    // downloads that overlap later scoring (download_after)
+    uint8_t* slot[2] = {nullptr, nullptr};
+    hipEvent_t ready[2] = {nullptr, nullptr};
+    void* scratch = nullptr;
+};
+
+std::mutex g_ctx_mu;
+std::vector<HostContext*> g_ctx_free;
+
+class HostCall {
+public:
+    HostCall() {
+        int dev = 0;
+        check(hipGetDevice(&dev), "hipGetDevice");
+        {
+            std::lock_guard<std::mutex> lk(g_ctx_mu);
+            for (size_t i = 0; i < g_ctx_free.size(); ++i)
+                if (g_ctx_free[i]->device == dev) {
+                    c_ = g_ctx_free[i];
+                    g_ctx_free.erase(g_ctx_free.begin() + i);
+                    break;
+                }
+        }
+        if (!c_) c_ = create(dev);
+    }
+    ~HostCall() {
+        // the streams are drained before the context (and its slots) can serve another call
+        if (hipStreamSynchronize(c_->stream) != hipSuccess ||
+            hipStreamSynchronize(c_->copy) != hipSuccess)
+            return;  // leak a broken context
+        release_marks();
+        std::lock_guard<std::mutex> lk(g_ctx_mu);
+        g_ctx_free.push_back(c_);
+    }
+    HostCall(const HostCall&) = delete;
+    HostCall& operator=(const HostCall&) = delete;
+
+    hipStream_t stream() const { return c_->stream; }
+    void* scratch() const { return c_->scratch; }
+
+    // host -> device, stream-ordered: each chunk is copied into a slot once that slot's
+    // previous transfer has finished, then sent.
+    void upload(void* dev, const void* host, size_t bytes) {
+        const char* src = static_cast<const char*>(host);
+        char* dst = static_cast<char*>(dev);
+        for (size_t at = 0; at < bytes; at += kStageBytes) {
+            const size_t n = std::min(kStageBytes, bytes - at);
+            const int k = turn_++ & 1;
+            check(hipEventSynchronize(c_->ready[k]), "hipEventSynchronize");
+            CopyPool::get().copy(c_->slot[k], src + at, n);
+            check(hipMemcpyAsync(dst + at, c_->slot[k], n, hipMemcpyHostToDevice, c_->stream),
+                  "H2D");
+            check(hipEventRecord(c_->ready[k], c_->stream), "hipEventRecord");
+        }
+    }
+    template <class T>
+    void put(T* dev, const T* host, size_t count) { upload(dev, host, count * sizeof(T)); }
+
+    // device -> host after everything enqueued so far on the call's stream: chunk c + 1
+    // travels while chunk c is copied out of its slot. Returns when the data is in `host`.
+    void download(void* host, const void* dev, size_t bytes) {
+        download_on(c_->stream, host, dev, bytes);
+    }
+    // The same on the copy stream once `after` (recorded on the call's stream after the
+    // uploads) has completed, so the scoring enqueued after `after` overlaps the transfer.
+    void download_after(hipEvent_t after, void* host, const void* dev, size_t bytes) {
+        check(hipStreamWaitEvent(c_->copy, after, 0), "hipStreamWaitEvent");
+        download_on(c_->copy, host, dev, bytes);
+    }
+    // An event recorded on the call's stream now (one of a reusable set).
+    hipEvent_t mark() {
+        if (marks_used_ == c_marks_.size()) {
+            hipEvent_t e;
+            check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+            c_marks_.push_back(e);
+        }
+        hipEvent_t e = c_marks_[marks_used_++];
+        check(hipEventRecord(e, c_->stream), "hipEventRecord");
+        return e;
+    }
+    template <class T>
+    void get(T* host, const T* dev, size_t count) { download(host, dev, count * sizeof(T)); }
+
+private:
+    void download_on(hipStream_t st, void* host, const void* dev, size_t bytes) {
+        if (bytes == 0) return;
+        const char* src = static_cast<const char*>(dev);
+        char* dst = static_cast<char*>(host);
+        const size_t chunks = (bytes + kStageBytes - 1) / kStageBytes;
+        auto send = [&](size_t c) {
+            const int k = int((turn_ + c) & 1);
+            const size_t at = c * kStageBytes, n = std::min(kStageBytes, bytes - at);
+            check(hipMemcpyAsync(c_->slot[k], src + at, n, hipMemcpyDeviceToHost, st), "D2H");
+            check(hipEventRecord(c_->ready[k], st), "hipEventRecord");
+        };
+        // both slots' earlier uploads have been read once the stream reaches this point
+        send(0);
+        for (size_t c = 0; c < chunks; ++c) {
+            if (c + 1 < chunks) send(c + 1);
+            const int k = int((turn_ + c) & 1);
+            check(hipEventSynchronize(c_->ready[k]), "hipEventSynchronize");
+            const size_t at = c * kStageBytes, n = std::min(kStageBytes, bytes - at);
+            CopyPool::get().copy(dst + at, c_->slot[k], n);
+        }
+        turn_ += chunks;
+    }
+    static HostContext* create(int dev) {
+        auto* c = new HostContext();
+        c->device = dev;
+        check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream");
+        check(hipStreamCreateWithFlags(&c->copy, hipStreamNonBlocking), "stream");
+        for (int k = 0; k < 2; ++k) {
+            check(hipHostMalloc(reinterpret_cast<void**>(&c->slot[k]), kStageBytes,
+                                hipHostMallocDefault),
+                  "hipHostMalloc");
+            check(hipEventCreateWithFlags(&c->ready[k], hipEventDisableTiming), "event");
+            check(hipEventRecord(c->ready[k], c->stream), "hipEventRecord");
+        }
+        check(hipMalloc(&c->scratch, kContextScratch), "hipMalloc");
+        return c;
+    }
+    HostContext* c_ = nullptr;
+    uint64_t turn_ = 0;
+    std::vector<hipEvent_t> c_marks_;
+    size_t marks_used_ = 0;
+
+public:
+    // events of mark() are released with the call (after the streams have drained)
+    void release_marks() {
+        for (hipEvent_t e : c_marks_) (void)hipEventDestroy(e);
+        c_marks_.clear();
+        marks_used_ = 0;
+    }
+};
 
 // ---- optional launch timing (st_timing_enable / st_timing_collect) ----------------------
 // When enabled, every kernel launch below is bracketed by two hipEvents recorded on the
@@ -886,28 +1103,48 @@ struct DeviceSet {
     const uint64_t* off;
 };
 
-DeviceSet upload(Arena& ar, const HostSet& h, hipStream_t s) {
+DeviceSet upload(Arena& ar, const HostSet& h, HostCall& call) {
     uint8_t* b = ar.get<uint8_t>(std::max<uint64_t>(h.total, 1));
     uint64_t* o = ar.get<uint64_t>(h.n + 1);
-    h2d(b, h.bytes, h.total, s);
-    h2d(o, h.off, h.n + 1, s);
+    call.put(b, h.bytes, h.total);
+    call.put(o, h.off, h.n + 1);
     return {b, o};
 }
 
-// Runs `body(q_dev_subset, row0, rows, out_dev)` over blocks of query rows whose output fits a
-// device budget, copying each block of the matrix back to `out`.
+// Runs `body(q_dev_subset, row0, rows, out_dev)` over blocks of query rows and copies each
+// block of the matrix to `out` while the next block is scored: the host enqueues block b, then
+// copies block b - 1 out (pinned staging on the copy stream, CopyPool into `out`). Blocks hold
+// about kRowBlockBytes of output, so a small matrix is one block; the device keeps at most
+// 8 GiB of output at a time.
+constexpr uint64_t kRowBlockBytes = uint64_t(128) << 20;
+
 template <class Body>
-void run_rows(const HostSet& q, uint64_t K, int32_t* out, hipStream_t s, Arena& ar,
+void run_rows(const HostSet& q, uint64_t K, int32_t* out, HostCall& call, Arena& ar,
               const DeviceSet& qd, Body&& body) {
-    const uint64_t budget = uint64_t(8) << 30;  // bytes of int32 output per block
-    const uint64_t rows_per = std::max<uint64_t>(1, budget / std::max<uint64_t>(K * 4, 1));
-    const uint64_t cap = std::min<uint64_t>(rows_per, q.n);
+    const uint64_t row_bytes = std::max<uint64_t>(K * 4, 1);
+    const uint64_t cap = std::min<uint64_t>(std::max<uint64_t>(1, (uint64_t(8) << 30) / row_bytes),
+                                            q.n);
+    const uint64_t n_blocks = std::max<uint64_t>(1, (cap * row_bytes + kRowBlockBytes - 1) /
+                                                        kRowBlockBytes);
+    const uint64_t block = (cap + n_blocks - 1) / n_blocks;
     int32_t* dout = ar.get<int32_t>(size_t(cap) * K);
     for (uint64_t r0 = 0; r0 < q.n; r0 += cap) {
         const uint64_t rows = std::min(cap, q.n - r0);
-        body(qd.off + r0, r0, rows, dout);
-        d2h(out + r0 * K, dout, size_t(rows) * K, s);
-        sync(s);
+        hipEvent_t prev = nullptr;
+        uint64_t prev_b0 = 0, prev_nb = 0;
+        for (uint64_t b0 = 0; b0 < rows; b0 += block) {
+            const uint64_t nb = std::min(block, rows - b0);
+            body(qd.off + r0 + b0, r0 + b0, nb, dout + b0 * K);
+            hipEvent_t done = call.mark();
+            if (prev)
+                call.download_after(prev, out + (r0 + prev_b0) * K, dout + prev_b0 * K,
+                                    size_t(prev_nb) * K * 4);
+            prev = done;
+            prev_b0 = b0;
+            prev_nb = nb;
+        }
+        call.download_after(prev, out + (r0 + prev_b0) * K, dout + prev_b0 * K,
+                            size_t(prev_nb) * K * 4);
     }
 }
 
@@ -1255,7 +1492,7 @@ bool bad_pair_index(st_status* status, const uint32_t* pq, const uint32_t* pr, u
 // the pair list, the ops, the int64 scores (or, for the penalty form, the int32 scores) and the
 // per-pair op count and first aligned residues.
 struct PairTraceback {
-    OwnedStream os;
+    HostCall call;
     Arena ar;
     HostSet qh, rh;
     DeviceSet qd, rd;
@@ -1268,25 +1505,24 @@ struct PairTraceback {
     PairTraceback(const uint8_t* q, const uint64_t* q_off, uint64_t nq, const uint8_t* r,
                   const uint64_t* r_off, uint64_t nr, const uint32_t* pair_q,
                   const uint32_t* pair_r, uint64_t n_pairs, uint64_t ops_stride, bool penalty)
-        : ar(os.get()), qh(host_set(q, q_off, nq)), rh(host_set(r, r_off, nr)),
-          qd(upload(ar, qh, os.get())), rd(upload(ar, rh, os.get())), n(n_pairs),
-          stride(ops_stride) {
-        const hipStream_t st = os.get();
+        : ar(call.stream()), qh(host_set(q, q_off, nq)), rh(host_set(r, r_off, nr)),
+          qd((ar.prefer(call.scratch(), kContextScratch), upload(ar, qh, call))),
+          rd(upload(ar, rh, call)), n(n_pairs), stride(ops_stride) {
         pq = ar.get<uint32_t>(n);
         pr = ar.get<uint32_t>(n);
-        h2d(pq, pair_q, n, st);
-        h2d(pr, pair_r, n, st);
+        call.put(pq, pair_q, n);
+        call.put(pr, pair_r, n);
         ops = ar.get<uint8_t>(n * stride);
         if (penalty) score32 = ar.get<int32_t>(n);
         else score = ar.get<int64_t>(n);
         meta = ar.get<uint32_t>(3 * n);
     }
-    hipStream_t s() const { return os.get(); }
+    hipStream_t s() const { return call.stream(); }
     void fetch(uint8_t* ops_h, uint32_t* n_ops, uint32_t* q_begin, uint32_t* r_begin) {
-        d2h(ops_h, ops, n * stride, s());
-        d2h(n_ops, meta, n, s());
-        if (q_begin) d2h(q_begin, meta + n, n, s());
-        if (r_begin) d2h(r_begin, meta + 2 * n, n, s());
+        call.get(ops_h, ops, n * stride);
+        call.get(n_ops, meta, n);
+        if (q_begin) call.get(q_begin, meta + n, n);
+        if (r_begin) call.get(r_begin, meta + 2 * n, n);
     }
 };
 
@@ -1851,12 +2087,13 @@ int32_t st_gapblock_matrix(const uint8_t* q_codes, const uint64_t* q_offsets, ui
         require_device();
         const HostSet qh = host_set(q_codes, q_offsets, n_queries);
         const HostSet rh = host_set(r_codes, r_offsets, n_refs);
-        OwnedStream os;
-        const hipStream_t s = os.get();
+        HostCall call;
+        const hipStream_t s = call.stream();
         Arena ar(s);
-        const DeviceSet qd = upload(ar, qh, s);
-        const DeviceSet rd = upload(ar, rh, s);
-        run_rows(qh, n_refs, out, s, ar, qd,
+        ar.prefer(call.scratch(), kContextScratch);
+        const DeviceSet qd = upload(ar, qh, call);
+        const DeviceSet rd = upload(ar, rh, call);
+        run_rows(qh, n_refs, out, call, ar, qd,
                  [&](const uint64_t* qoff, uint64_t, uint64_t rows, int32_t* dout) {
                      gapblock_device(qd.bytes, qoff, rows, qh.max_len, rd.bytes, rd.off, n_refs,
                                      rh.max_len, pen, alphabet_size, gap_open, gap_extend, prior,
@@ -1896,17 +2133,17 @@ int32_t st_gapblock_histogram(const uint8_t* q_codes, const uint64_t* q_offsets,
         if (n_queries == 0) return ok(status);
         const HostSet qh = host_set(q_codes, q_offsets, n_queries);
         const HostSet rh = host_set(r_codes, r_offsets, n_refs);
-        OwnedStream os;
-        const hipStream_t s = os.get();
+        HostCall call;
+        const hipStream_t s = call.stream();
         Arena ar(s);
-        const DeviceSet qd = upload(ar, qh, s);
-        const DeviceSet rd = upload(ar, rh, s);
+        ar.prefer(call.scratch(), kContextScratch);
+        const DeviceSet qd = upload(ar, qh, call);
+        const DeviceSet rd = upload(ar, rh, call);
         uint64_t* d = ar.get<uint64_t>(n_queries * (uint64_t(bins) + 2));
         gapblock_histogram_device(qd.bytes, qd.off, n_queries, qh.max_len, rd.bytes, rd.off,
                                   n_refs, rh.max_len, pen, alphabet_size, gap_open, gap_extend,
                                   prior, prior_width, bins, d, s);
-        d2h(histogram, d, n_queries * (uint64_t(bins) + 2), s);
-        sync(s);
+        call.get(histogram, d, n_queries * (uint64_t(bins) + 2));
         return ok(status);
     });
 }
@@ -1946,11 +2183,12 @@ int32_t st_gapblock_threshold(const uint8_t* q_codes, const uint64_t* q_offsets,
         require_device();
         const HostSet qh = host_set(q_codes, q_offsets, n_queries);
         const HostSet rh = host_set(r_codes, r_offsets, n_refs);
-        OwnedStream os;
-        const hipStream_t s = os.get();
+        HostCall call;
+        const hipStream_t s = call.stream();
         Arena ar(s);
-        const DeviceSet qd = upload(ar, qh, s);
-        const DeviceSet rd = upload(ar, rh, s);
+        ar.prefer(call.scratch(), kContextScratch);
+        const DeviceSet qd = upload(ar, qh, call);
+        const DeviceSet rd = upload(ar, rh, call);
         const uint64_t cap = std::max<uint64_t>(capacity, 1);
         GapblockHits h{cutoff, capacity, ar.get<uint32_t>(cap), ar.get<uint32_t>(cap),
                        ar.get<int32_t>(cap), ar.get<unsigned long long>(1)};
@@ -1958,13 +2196,11 @@ int32_t st_gapblock_threshold(const uint8_t* q_codes, const uint64_t* q_offsets,
                                   n_refs, rh.max_len, pen, alphabet_size, gap_open, gap_extend,
                                   prior, prior_width, h, s);
         unsigned long long count = 0;
-        d2h(&count, h.count, 1, s);
-        sync(s);
+        call.get(&count, h.count, 1);
         const uint64_t kept = std::min<uint64_t>(count, capacity);
-        d2h(hit_query, h.query, kept, s);
-        d2h(hit_ref, h.ref, kept, s);
-        d2h(hit_score, h.score, kept, s);
-        sync(s);
+        call.get(hit_query, h.query, kept);
+        call.get(hit_ref, h.ref, kept);
+        call.get(hit_score, h.score, kept);
         *hit_count = count;
         return ok(status);
     });
@@ -2008,21 +2244,21 @@ int32_t st_gapblock_pairs(const uint8_t* q_codes, const uint64_t* q_offsets, uin
         require_device();
         const HostSet qh = host_set(q_codes, q_offsets, n_queries);
         const HostSet rh = host_set(r_codes, r_offsets, n_refs);
-        OwnedStream os;
-        const hipStream_t s = os.get();
+        HostCall call;
+        const hipStream_t s = call.stream();
         Arena ar(s);
-        const DeviceSet qd = upload(ar, qh, s);
-        const DeviceSet rd = upload(ar, rh, s);
+        ar.prefer(call.scratch(), kContextScratch);
+        const DeviceSet qd = upload(ar, qh, call);
+        const DeviceSet rd = upload(ar, rh, call);
         uint32_t* pq = ar.get<uint32_t>(n_pairs);
         uint32_t* pr = ar.get<uint32_t>(n_pairs);
         int32_t* dout = ar.get<int32_t>(n_pairs);
-        h2d(pq, pair_query, n_pairs, s);
-        h2d(pr, pair_ref, n_pairs, s);
+        call.put(pq, pair_query, n_pairs);
+        call.put(pr, pair_ref, n_pairs);
         gapblock_pairs_device(qd.bytes, qd.off, n_queries, qh.max_len, rd.bytes, rd.off, n_refs,
                               rh.max_len, pen, alphabet_size, gap_open, gap_extend, prior,
                               prior_width, pq, pr, n_pairs, dout, s);
-        d2h(out, dout, n_pairs, s);
-        sync(s);
+        call.get(out, dout, n_pairs);
         return ok(status);
     });
 }
@@ -2069,8 +2305,7 @@ int32_t st_align_pairs(const uint8_t* q_codes, const uint64_t* q_offsets, uint64
                          t.ops, ops_stride, t.score, t.meta, t.meta + n_pairs,
                          t.meta + 2 * n_pairs, t.s());
         t.fetch(ops, n_ops, q_begin, r_begin);
-        d2h(score, t.score, n_pairs, t.s());
-        sync(t.s());
+        t.call.get(score, t.score, n_pairs);
         return ok(status);
     });
 }
@@ -2124,8 +2359,7 @@ int32_t st_index_align_pairs(const uint8_t* q_codes, const uint64_t* q_offsets,
                          t.ops, ops_stride, nullptr, t.meta, t.meta + n_pairs,
                          t.meta + 2 * n_pairs, t.s(), t.score32);
         t.fetch(ops, n_ops, nullptr, nullptr);
-        d2h(score, t.score32, n_pairs, t.s());
-        sync(t.s());
+        t.call.get(score, t.score32, n_pairs);
         return ok(status);
     });
 }
@@ -2359,13 +2593,14 @@ int32_t st_align_matrix(const uint8_t* q_codes, const uint64_t* q_offsets, uint6
         require_device();
         const HostSet qh = host_set(q_codes, q_offsets, n_queries);
         const HostSet rh = host_set(r_codes, r_offsets, n_refs);
-        OwnedStream os;
-        const hipStream_t s = os.get();
+        HostCall call;
+        const hipStream_t s = call.stream();
         Arena ar(s);
-        const DeviceSet qd = upload(ar, qh, s);
-        const DeviceSet rd = upload(ar, rh, s);
+        ar.prefer(call.scratch(), kContextScratch);
+        const DeviceSet qd = upload(ar, qh, call);
+        const DeviceSet rd = upload(ar, rh, call);
         bool any_overflow = false;
-        run_rows(qh, n_refs, out, s, ar, qd,
+        run_rows(qh, n_refs, out, call, ar, qd,
                  [&](const uint64_t* qoff, uint64_t, uint64_t rows, int32_t* dout) {
                      bool overflow = false;
                      gotoh_device(qd.bytes, qoff, rows, qh.max_len, rd.bytes, rd.off, n_refs,
@@ -2423,12 +2658,13 @@ int32_t st_hamming_matrix(const uint8_t* a, const uint64_t* a_offsets, uint64_t 
     if (n_a == 0 || n_b == 0) return ok(status);
     return guarded(status, [&] {
         require_device();
-        OwnedStream os;
-        const hipStream_t s = os.get();
+        HostCall call;
+        const hipStream_t s = call.stream();
         Arena ar(s);
-        const DeviceSet ad = upload(ar, ah, s);
-        const DeviceSet bd = upload(ar, bh, s);
-        run_rows(ah, n_b, out, s, ar, ad,
+        ar.prefer(call.scratch(), kContextScratch);
+        const DeviceSet ad = upload(ar, ah, call);
+        const DeviceSet bd = upload(ar, bh, call);
+        run_rows(ah, n_b, out, call, ar, ad,
                  [&](const uint64_t* aoff, uint64_t, uint64_t rows, int32_t* dout) {
                      hamming_device(ad.bytes, aoff, rows, ah.max_len, bd.bytes, bd.off, n_b,
                                     bh.max_len, dout, s);
@@ -2457,12 +2693,13 @@ int32_t st_levenshtein_matrix(const uint8_t* a, const uint64_t* a_offsets, uint6
         require_device();
         const HostSet ah = host_set(a, a_offsets, n_a);
         const HostSet bh = host_set(b, b_offsets, n_b);
-        OwnedStream os;
-        const hipStream_t s = os.get();
+        HostCall call;
+        const hipStream_t s = call.stream();
         Arena ar(s);
-        const DeviceSet ad = upload(ar, ah, s);
-        const DeviceSet bd = upload(ar, bh, s);
-        run_rows(ah, n_b, out, s, ar, ad,
+        ar.prefer(call.scratch(), kContextScratch);
+        const DeviceSet ad = upload(ar, ah, call);
+        const DeviceSet bd = upload(ar, bh, call);
+        run_rows(ah, n_b, out, call, ar, ad,
                  [&](const uint64_t* aoff, uint64_t, uint64_t rows, int32_t* dout) {
                      levenshtein_device(ad.bytes, aoff, rows, ah.max_len, bd.bytes, bd.off, n_b,
                                         bh.max_len, dout, s);

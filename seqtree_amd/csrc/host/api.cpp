@@ -95,23 +95,21 @@ void check_pairwise(const SubstitutionMatrix& mat, const Codec& codec, int32_t o
         throw std::invalid_argument("matrix size does not match the alphabet");
 }
 
-std::vector<int32_t> pairwise_matrix(const std::vector<std::string>& queries,
-                                     const std::vector<std::string>& refs,
-                                     const SubstitutionMatrix& mat, Alphabet alphabet,
-                                     AlignMode mode, int32_t go, int32_t ge, bool dist) {
+void pairwise_matrix(const std::vector<std::string>& queries,
+                     const std::vector<std::string>& refs, const SubstitutionMatrix& mat,
+                     Alphabet alphabet, AlignMode mode, int32_t go, int32_t ge, bool dist,
+                     int32_t* out) {
     const Codec codec(alphabet);
     check_pairwise(mat, codec, go, ge);
     const size_t N = queries.size(), K = refs.size();
-    std::vector<int32_t> out(N * K);
-    if (N == 0 || K == 0) return out;
+    if (N == 0 || K == 0) return;
     const Csr q = pack_pairwise(queries, codec);
     const Csr r = pack_pairwise(refs, codec);
     st_status st{};
     const int rc = st_align_matrix(q.bytes.data(), q.off.data(), N, r.bytes.data(), r.off.data(),
                                    K, mat.similarities().data(), mat.size(),
-                                   mode == AlignMode::Local, dist, go, ge, out.data(), &st);
+                                   mode == AlignMode::Local, dist, go, ge, out, &st);
     raise(rc, st);
-    return out;
 }
 
 }  // namespace
@@ -216,12 +214,28 @@ Alignment align_pair(std::string_view query, std::string_view ref, const Substit
                        gap_open, gap_extend)[0];
 }
 
+void align_score_matrix_into(const std::vector<std::string>& queries,
+                             const std::vector<std::string>& refs, const SubstitutionMatrix& mat,
+                             Alphabet alphabet, AlignMode mode, int32_t gap_open,
+                             int32_t gap_extend, int32_t* out) {
+    pairwise_matrix(queries, refs, mat, alphabet, mode, gap_open, gap_extend, false, out);
+}
+
+void align_dist_matrix_into(const std::vector<std::string>& queries,
+                            const std::vector<std::string>& refs, const SubstitutionMatrix& mat,
+                            Alphabet alphabet, AlignMode mode, int32_t gap_open,
+                            int32_t gap_extend, int32_t* out) {
+    pairwise_matrix(queries, refs, mat, alphabet, mode, gap_open, gap_extend, true, out);
+}
+
 std::vector<int32_t> align_score_matrix(const std::vector<std::string>& queries,
                                         const std::vector<std::string>& refs,
                                         const SubstitutionMatrix& mat, Alphabet alphabet,
                                         AlignMode mode, int32_t gap_open, int32_t gap_extend,
                                         int /*threads*/) {
-    return pairwise_matrix(queries, refs, mat, alphabet, mode, gap_open, gap_extend, false);
+    std::vector<int32_t> out(queries.size() * refs.size());
+    align_score_matrix_into(queries, refs, mat, alphabet, mode, gap_open, gap_extend, out.data());
+    return out;
 }
 
 std::vector<int32_t> align_dist_matrix(const std::vector<std::string>& queries,
@@ -229,7 +243,9 @@ std::vector<int32_t> align_dist_matrix(const std::vector<std::string>& queries,
                                        const SubstitutionMatrix& mat, Alphabet alphabet,
                                        AlignMode mode, int32_t gap_open, int32_t gap_extend,
                                        int /*threads*/) {
-    return pairwise_matrix(queries, refs, mat, alphabet, mode, gap_open, gap_extend, true);
+    std::vector<int32_t> out(queries.size() * refs.size());
+    align_dist_matrix_into(queries, refs, mat, alphabet, mode, gap_open, gap_extend, out.data());
+    return out;
 }
 
 namespace {
@@ -377,11 +393,21 @@ std::vector<int32_t> gapblock_matrix(const std::vector<std::string>& queries,
                                      const SubstitutionMatrix* matrix, int32_t gap_open,
                                      int32_t gap_extend, const std::vector<int32_t>& prior,
                                      uint32_t prior_width, int /*threads*/) {
+    std::vector<int32_t> out(queries.size() * refs.size());
+    gapblock_matrix_into(queries, refs, alphabet, matrix, gap_open, gap_extend, prior,
+                         prior_width, out.data());
+    return out;
+}
+
+void gapblock_matrix_into(const std::vector<std::string>& queries,
+                          const std::vector<std::string>& refs, Alphabet alphabet,
+                          const SubstitutionMatrix* matrix, int32_t gap_open, int32_t gap_extend,
+                          const std::vector<int32_t>& prior, uint32_t prior_width,
+                          int32_t* out) {
     const GapblockInput in = gapblock_input(queries, refs, alphabet, matrix, gap_open, gap_extend,
                                             prior, prior_width);
     const size_t N = queries.size(), K = refs.size();
-    std::vector<int32_t> out(N * K);
-    if (in.empty) return out;
+    if (in.empty) return;
     const Csr& q = in.q;
     const Csr& r = in.r;
     const std::vector<int32_t>& pen = in.pen;
@@ -391,9 +417,8 @@ std::vector<int32_t> gapblock_matrix(const std::vector<std::string>& queries,
     const int rc = st_gapblock_matrix(q.bytes.data(), q.off.data(), N, r.bytes.data(),
                                       r.off.data(), K, pen.data(), A, gap_open, gap_extend,
                                       prior.empty() ? nullptr : prior.data(), prior_width,
-                                      out.data(), &st);
+                                      out, &st);
     raise(rc, st);
-    return out;
 }
 
 int hamming(std::string_view a, std::string_view b) {
@@ -422,31 +447,39 @@ int levenshtein(std::string_view a, std::string_view b) {
     return out;
 }
 
-std::vector<int32_t> hamming_matrix(const std::vector<std::string>& a,
-                                    const std::vector<std::string>& b, int /*threads*/) {
+void hamming_matrix_into(const std::vector<std::string>& a, const std::vector<std::string>& b,
+                        int32_t* out) {
     const size_t N = a.size(), K = b.size();
-    std::vector<int32_t> out(N * K);
-    if (N == 0 || K == 0) return out;
+    if (N == 0 || K == 0) return;
     const Csr x = pack_raw(a);
     const Csr y = pack_raw(b);
     st_status st{};
-    raise(st_hamming_matrix(x.bytes.data(), x.off.data(), N, y.bytes.data(), y.off.data(), K,
-                            out.data(), &st),
+    raise(st_hamming_matrix(x.bytes.data(), x.off.data(), N, y.bytes.data(), y.off.data(), K, out, &st),
           st);
+}
+
+std::vector<int32_t> hamming_matrix(const std::vector<std::string>& a,
+                                   const std::vector<std::string>& b, int /*threads*/) {
+    std::vector<int32_t> out(a.size() * b.size());
+    hamming_matrix_into(a, b, out.data());
     return out;
 }
 
-std::vector<int32_t> levenshtein_matrix(const std::vector<std::string>& a,
-                                        const std::vector<std::string>& b, int /*threads*/) {
+void levenshtein_matrix_into(const std::vector<std::string>& a, const std::vector<std::string>& b,
+                        int32_t* out) {
     const size_t N = a.size(), K = b.size();
-    std::vector<int32_t> out(N * K);
-    if (N == 0 || K == 0) return out;
+    if (N == 0 || K == 0) return;
     const Csr x = pack_raw(a);
     const Csr y = pack_raw(b);
     st_status st{};
-    raise(st_levenshtein_matrix(x.bytes.data(), x.off.data(), N, y.bytes.data(), y.off.data(), K,
-                                out.data(), &st),
+    raise(st_levenshtein_matrix(x.bytes.data(), x.off.data(), N, y.bytes.data(), y.off.data(), K, out, &st),
           st);
+}
+
+std::vector<int32_t> levenshtein_matrix(const std::vector<std::string>& a,
+                                   const std::vector<std::string>& b, int /*threads*/) {
+    std::vector<int32_t> out(a.size() * b.size());
+    levenshtein_matrix_into(a, b, out.data());
     return out;
 }
 

@@ -8,7 +8,11 @@
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include <sys/mman.h>
+
 #include <algorithm>
+#include <cstdlib>
+#include <mutex>
 #include <optional>
 #include <string>
 #include <vector>
@@ -48,9 +52,87 @@ std::string alphabet_symbols(Alphabet a) {
     return s;
 }
 
+// Uninitialised int32 storage for a ScoreMatrix: 2 MiB aligned with transparent huge pages
+// requested, so the library's parallel copy-out faults the memory in as large pages, and
+// never zero-filled, since the matrix functions write every element. Buffers of released
+// matrices are kept for reuse (at most kCacheBuffers, kCacheBytes in all): a fresh page costs
+// the operating system a zero-fill on first touch, which for a large matrix is as much as the
+// copy-out itself. A request takes a cached buffer of at least its size and at most twice it.
+class HostBuffer {
+public:
+    HostBuffer() = default;
+    explicit HostBuffer(size_t count) {
+        const size_t align = size_t(2) << 20;
+        bytes_ = (std::max<size_t>(count, 16) * sizeof(int32_t) + align - 1) / align * align;
+        p_ = take(bytes_);
+        if (p_) return;
+        void* v = nullptr;
+        if (posix_memalign(&v, align, bytes_) != 0) throw std::bad_alloc();
+        (void)madvise(v, bytes_, MADV_HUGEPAGE);  // advice only; fine where unsupported
+        p_ = static_cast<int32_t*>(v);
+    }
+    HostBuffer(HostBuffer&& o) noexcept : p_(o.p_), bytes_(o.bytes_) { o.p_ = nullptr; }
+    HostBuffer& operator=(HostBuffer&& o) noexcept {
+        std::swap(p_, o.p_);
+        std::swap(bytes_, o.bytes_);
+        return *this;
+    }
+    HostBuffer(const HostBuffer&) = delete;
+    HostBuffer& operator=(const HostBuffer&) = delete;
+    ~HostBuffer() {
+        if (p_) give(p_, bytes_);
+    }
+    int32_t* data() const { return p_; }
+
+private:
+    static constexpr size_t kCacheBuffers = 4;
+    static constexpr size_t kCacheBytes = size_t(8) << 30;
+    struct Cached {
+        int32_t* p;
+        size_t bytes;
+    };
+    static std::mutex& mu() {
+        static std::mutex m;
+        return m;
+    }
+    static std::vector<Cached>& cache() {
+        static auto* c = new std::vector<Cached>();  // outlives every ScoreMatrix
+        return *c;
+    }
+    static int32_t* take(size_t bytes) {
+        std::lock_guard<std::mutex> lk(mu());
+        auto& c = cache();
+        size_t best = c.size();
+        for (size_t i = 0; i < c.size(); ++i)
+            if (c[i].bytes >= bytes && c[i].bytes <= 2 * bytes &&
+                (best == c.size() || c[i].bytes < c[best].bytes))
+                best = i;
+        if (best == c.size()) return nullptr;
+        int32_t* p = c[best].p;
+        c.erase(c.begin() + best);
+        return p;
+    }
+    static void give(int32_t* p, size_t bytes) {
+        std::lock_guard<std::mutex> lk(mu());
+        auto& c = cache();
+        size_t total = bytes;
+        for (const Cached& e : c) total += e.bytes;
+        while (!c.empty() && (c.size() >= kCacheBuffers || total > kCacheBytes)) {
+            total -= c.front().bytes;  // the oldest goes first
+            std::free(c.front().p);
+            c.erase(c.begin());
+        }
+        if (total <= kCacheBytes) c.push_back({p, bytes});
+        else std::free(p);
+    }
+
+    int32_t* p_ = nullptr;
+    size_t bytes_ = 0;
+};
+
 // Dense (rows, cols) int32 block lent to Python through the buffer protocol.
 struct ScoreMatrix {
-    std::vector<int32_t> data;
+    HostBuffer data;
     size_t rows = 0, cols = 0;
 };
 
@@ -151,6 +233,7 @@ SearchParams to_cpp(const PyParams& pp, const SubstitutionMatrix* mat) {
     return p;
 }
 
+// A rows x cols ScoreMatrix filled by compute(out) with the GIL released.
 template <class F>
 ScoreMatrix make_matrix(size_t rows, size_t cols, F&& compute) {
     ScoreMatrix out;
@@ -158,7 +241,8 @@ ScoreMatrix make_matrix(size_t rows, size_t cols, F&& compute) {
     out.cols = cols;
     {
         py::gil_scoped_release release;
-        out.data = compute();
+        out.data = HostBuffer(rows * cols);
+        compute(out.data.data());
     }
     return out;
 }
@@ -235,15 +319,15 @@ PYBIND11_MODULE(_core, m) {
             "row",
             [](const ScoreMatrix& s, size_t i) {
                 if (i >= s.rows) throw py::index_error("row out of range");
-                return std::vector<int32_t>(s.data.begin() + i * s.cols,
-                                            s.data.begin() + (i + 1) * s.cols);
+                return std::vector<int32_t>(s.data.data() + i * s.cols,
+                                            s.data.data() + (i + 1) * s.cols);
             },
             py::arg("i"), "Row i as a list, one value per reference.")
         .def("__getitem__",
              [](const ScoreMatrix& s, std::pair<size_t, size_t> ik) {
                  if (ik.first >= s.rows || ik.second >= s.cols)
                      throw py::index_error("index out of range");
-                 return s.data[ik.first * s.cols + ik.second];
+                 return s.data.data()[ik.first * s.cols + ik.second];
              })
         .def("__repr__", [](const ScoreMatrix& s) {
             return "ScoreMatrix(" + std::to_string(s.rows) + ", " + std::to_string(s.cols) + ")";
@@ -315,8 +399,9 @@ PYBIND11_MODULE(_core, m) {
            int32_t gap_extend, const std::string& alphabet, int threads) {
             const Alphabet a = parse_alphabet(alphabet);
             const AlignMode md = parse_align_mode(mode);
-            return make_matrix(q.size(), r.size(), [&] {
-                return align_score_matrix(q, r, mat, a, md, gap_open, gap_extend, threads);
+            (void)threads;
+            return make_matrix(q.size(), r.size(), [&](int32_t* out) {
+                align_score_matrix_into(q, r, mat, a, md, gap_open, gap_extend, out);
             });
         },
         py::arg("queries"), py::arg("refs"), py::arg("matrix"), py::arg("mode") = "global",
@@ -330,8 +415,9 @@ PYBIND11_MODULE(_core, m) {
            int32_t gap_extend, const std::string& alphabet, int threads) {
             const Alphabet a = parse_alphabet(alphabet);
             const AlignMode md = parse_align_mode(mode);
-            return make_matrix(q.size(), r.size(), [&] {
-                return align_dist_matrix(q, r, mat, a, md, gap_open, gap_extend, threads);
+            (void)threads;
+            return make_matrix(q.size(), r.size(), [&](int32_t* out) {
+                align_dist_matrix_into(q, r, mat, a, md, gap_open, gap_extend, out);
             });
         },
         py::arg("queries"), py::arg("refs"), py::arg("matrix"), py::arg("mode") = "global",
@@ -346,9 +432,10 @@ PYBIND11_MODULE(_core, m) {
            int32_t gap_open, int32_t gap_extend, const std::vector<int32_t>& prior,
            uint32_t prior_width, int threads) {
             const Alphabet a = parse_alphabet(alphabet);
-            return make_matrix(queries.size(), refs.size(), [&] {
-                return gapblock_matrix(queries, refs, a, matrix ? &*matrix : nullptr, gap_open,
-                                       gap_extend, prior, prior_width, threads);
+            (void)threads;
+            return make_matrix(queries.size(), refs.size(), [&](int32_t* out) {
+                gapblock_matrix_into(queries, refs, a, matrix ? &*matrix : nullptr, gap_open,
+                                     gap_extend, prior, prior_width, out);
             });
         },
         py::arg("queries"), py::arg("refs"), py::arg("alphabet") = "aa",
@@ -450,15 +537,18 @@ PYBIND11_MODULE(_core, m) {
     m.def(
         "hamming_matrix",
         [](const std::vector<std::string>& a, const std::vector<std::string>& b, int threads) {
-            return make_matrix(a.size(), b.size(), [&] { return hamming_matrix(a, b, threads); });
+            (void)threads;
+            return make_matrix(a.size(), b.size(),
+                               [&](int32_t* out) { hamming_matrix_into(a, b, out); });
         },
         py::arg("a"), py::arg("b"), py::arg("threads") = 0,
         "Dense (len(a), len(b)) int32 Hamming matrix, GIL released.");
     m.def(
         "levenshtein_matrix",
         [](const std::vector<std::string>& a, const std::vector<std::string>& b, int threads) {
+            (void)threads;
             return make_matrix(a.size(), b.size(),
-                               [&] { return levenshtein_matrix(a, b, threads); });
+                               [&](int32_t* out) { levenshtein_matrix_into(a, b, out); });
         },
         py::arg("a"), py::arg("b"), py::arg("threads") = 0,
         "Dense (len(a), len(b)) int32 Levenshtein matrix, GIL released.");

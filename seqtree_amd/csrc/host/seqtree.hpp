@@ -153,6 +153,27 @@ std::vector<int32_t> hamming_matrix(const std::vector<std::string>& a,
 std::vector<int32_t> levenshtein_matrix(const std::vector<std::string>& a,
                                         const std::vector<std::string>& b, int threads = 0);
 
+// The dense-matrix functions above, writing the row-major matrix into caller memory of
+// queries.size() * refs.size() int32 instead of returning a vector (no counterpart in the
+// reference): the same checks, order and exceptions. The pybind11 ScoreMatrix allocates its
+// buffer and calls these, which spares zero-filling a vector that the call overwrites.
+void gapblock_matrix_into(const std::vector<std::string>& queries,
+                          const std::vector<std::string>& refs, Alphabet,
+                          const SubstitutionMatrix* matrix, int32_t gap_open, int32_t gap_extend,
+                          const std::vector<int32_t>& prior, uint32_t prior_width, int32_t* out);
+void align_score_matrix_into(const std::vector<std::string>& queries,
+                             const std::vector<std::string>& refs, const SubstitutionMatrix&,
+                             Alphabet, AlignMode, int32_t gap_open, int32_t gap_extend,
+                             int32_t* out);
+void align_dist_matrix_into(const std::vector<std::string>& queries,
+                            const std::vector<std::string>& refs, const SubstitutionMatrix&,
+                            Alphabet, AlignMode, int32_t gap_open, int32_t gap_extend,
+                            int32_t* out);
+void hamming_matrix_into(const std::vector<std::string>& a, const std::vector<std::string>& b,
+                         int32_t* out);
+void levenshtein_matrix_into(const std::vector<std::string>& a,
+                             const std::vector<std::string>& b, int32_t* out);
+
 // Search scope and budget (reference include/seqtree/types.hpp:23-54), kept field for field
 // so code that fills one compiles unchanged. Index::align reads the scoring fields (matrix,
 // gap_open, gap_extend); the search scope and budget fields belong to the reference's trie

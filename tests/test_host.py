@@ -403,3 +403,25 @@ def test_prefilter_pair_sampling_and_calibration():
     assert w[0] > w[1] > w[2] > 0
     assert pf.recall(np.array([1, 2]), np.array([3, 4]), np.array([1, 2, 5]),
                      np.array([3, 9, 9])) == 1 / 3
+
+
+def test_array_prior_cubes_equal_the_per_cell_evaluation(st):
+    """The built-in priors' array form (used to build the cube fast) against calling the prior
+    cell by cell, for several parameters and widths."""
+    from seqtree_amd.gapblock import _prior_cube
+
+    priors = [st.central_prior(0), st.central_prior(7), st.central_prior(21),
+              st.frame_prior(3, 4), st.frame_prior(10, 0), st.frame_prior(1, 40),
+              st.positions_prior((3, 4, -4, -3)), st.positions_prior((9, -9)),
+              st.positions_prior((0,)), st.positions_prior((-1, 100))]
+    for prior in priors:
+        assert hasattr(prior, "cells")
+        for width in (0, 1, 5, 20, 33):
+            fast = _prior_cube(prior, width)
+            slow = _prior_cube(lambda i, d, m, p=prior: p(i, d, m), width)
+            assert fast == slow, (prior, width)
+            assert all(type(v) is int for v in fast[:50])
+    # parameters the array form does not take fall back to the per-cell path
+    assert not hasattr(st.central_prior(1.5), "cells")
+    assert _prior_cube(st.central_prior(1.5), 6) == _prior_cube(
+        lambda i, d, m: st.central_prior(1.5)(i, d, m), 6)
