@@ -25,6 +25,8 @@ st_align_matrix_device on one HIP stream.
                control (the control pass of threshold_for_evalue), ceiling 64
   germline     4,000 x 4,000 Gotoh dist_matrix of germline-V-length strings (82..98),
                BLOSUM62 11/1, global and local (the multi-stripe register kernel)
+  longpairs    64 x 64 Gotoh dist_matrix of 2,000-residue strings, global and local (few long
+               pairs: the one-wave-per-pair wavefront kernel)
   prefilter    learned prefilter, 50,000 queries x 1,000,000 references, gap-block cutoff 48,
                against the exhaustive exact thresholded pass; a step is one search against
                reference embeddings built once (their build time and the one-shot pipeline
@@ -136,6 +138,60 @@ def run_germline(torch, dev, lib, stream, args) -> dict:
         kernels, roofline, cpu, gcups=2 * cells_per_mode / (ms_step / 1e3) / 1e9)
     line["dtype"] = "int16"
     line["arithmetic"] = GOTOH_ARITHMETIC
+    return line
+
+
+LONG_N, LONG_LEN = 64, 2000
+
+
+def run_longpairs(torch, dev, lib, stream, args) -> dict:
+    """Few long pairs: 64 x 64 all-vs-all Gotoh distances of 2,000-residue strings (residues
+    uniform over the 20 amino acids, BLOSUM62 11/1), global and local. 4,096 pairs would give
+    a lane-per-pair kernel 64 waves; the intra-pair wavefront kernel runs a wave per pair."""
+    rng = np.random.default_rng(SEED + 13)
+    aa = np.array([AA_ORDER.index(c) for c in BG_RESIDUES], np.uint8)
+    seqs = [aa[rng.integers(0, 20, size=LONG_LEN)] for _ in range(2 * LONG_N)]
+    qs, rs = seqs[:LONG_N], seqs[LONG_N:]
+    Q, R = DeviceSet(torch, dev, qs), DeviceSet(torch, dev, rs)
+    sim, _, _ = matrix_tables("blosum62")
+    outs = {mode: torch.empty((Q.n, R.n), dtype=torch.int32, device=dev)
+            for mode in ("global", "local")}
+    st = Status()
+
+    def step():
+        for mode in ("global", "local"):
+            rc = lib.lib.st_align_matrix_device(
+                Q.bytes.data_ptr(), Q.off.data_ptr(), Q.n, Q.max_len, R.bytes.data_ptr(),
+                R.off.data_ptr(), R.n, R.max_len, sim.ctypes.data, len(AA_ORDER),
+                int(mode == "local"), 1, GAP_OPEN, GAP_EXTEND, outs[mode].data_ptr(),
+                ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st))
+            check(rc, st, "st_align_matrix_device")
+
+    ms_step, kernels = time_steps(torch, lib, stream, step, args.steps, args.warmup)
+    pairs_per_step = 2 * Q.n * R.n
+    cells_per_mode = float(Q.lens.sum()) * float(R.lens.sum())
+    label, launches, avg_s = dominant(kernels)
+    roofline = None
+    if label.startswith("gotoh_wave_") and "self" not in label:
+        mode = label.rsplit("_", 1)[1]
+        roofline = valu_roofline(label, launches, avg_s, cells_per_mode * K_MIN[mode],
+                                 cells_per_launch=cells_per_mode, ops_per_cell=K_MIN[mode])
+    cpu = None
+    if args.cpu_rows > 0:
+        to_str = lambda v: "".join(AA_ORDER[c] for c in v)  # noqa: E731
+        cpu = cpu_reference("gotoh", [to_str(x) for x in qs], [to_str(x) for x in rs], 2,
+                            seconds=args.cpu_seconds, sim=sim)
+    line = base_line(
+        "all-vs-all pairs/s (Gotoh NW+SW dist_matrix, few long pairs)",
+        pairs_per_step / (ms_step / 1e3), args.steps, args.warmup, ms_step,
+        {"workload": "gotoh_dist_matrix_long_64x64_len2000_blosum62_go11_ge1_global+local",
+         "queries": Q.n, "refs": R.n, "length": LONG_LEN, "seed": SEED + 13,
+         "pairs_per_step": pairs_per_step, "cells_per_step": 2 * cells_per_mode,
+         "parallelism": "single GPU"},
+        kernels, roofline, cpu, gcups=2 * cells_per_mode / (ms_step / 1e3) / 1e9)
+    line["dtype"] = "int32"
+    line["arithmetic"] = ("int32 Gotoh cells, one wave per pair with the query's rows spread "
+                          "over the lanes (range-proven)")
     return line
 
 
@@ -1087,6 +1143,7 @@ RUNNERS = {
     "traceback": run_traceback,
     "counts": run_counts,
     "germline": run_germline,
+    "longpairs": run_longpairs,
 }
 
 # The arithmetic each configuration's dominant kernel computes in (base_line's default int32 is

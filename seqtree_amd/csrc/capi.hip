@@ -852,6 +852,8 @@ std::string batch_signature(const std::vector<st_gapblock_job>& jobs) {
 // ------------------------------------------------------------------------------------------
 // Register-kernel row classes: a query of length m runs in the class R = roundup(m, 4).
 constexpr int kGotohClasses = 16;
+// Long queries run one wave per pair (gotoh_wave_kernel) up to this many pairs.
+constexpr uint64_t kWavePairsMax = uint64_t(1) << 18;
 constexpr int gotoh_rows(int c) { return 4 * (c + 1); }
 
 void gotoh_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, uint32_t q_max,
@@ -903,15 +905,39 @@ void gotoh_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, uint32_t 
         return uint32_t(slots);
     };
 
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    // the intra-pair wavefront kernel: int32 cells in the H form (see launch_gotoh_wave)
+    auto wave_fits = [&](uint32_t len_q, uint32_t len_r) {
+        return go >= ge && (int64_t(len_q) + len_r + 2) * mx < (int64_t(1) << 28);
+    };
+    auto run_wave = [&](const GotohWaveJob& job, uint32_t len_q, uint32_t len_r,
+                        const char* label) {
+        const uint32_t grid = uint32_t(std::min<uint64_t>((job.n_pairs + 3) / 4,
+                                                          uint64_t(cus) * 4));
+        int32_t* bnd = nullptr;
+        if (const uint64_t words = gotoh_wave_bnd_words(grid, len_q, len_r))
+            bnd = ar.get<int32_t>(words);
+        LaunchTimer t(std::string(label) + (local ? "_local" : "_global"), s);
+        launch_gotoh_wave(a, job, len_q, len_r, grid, bnd, s);
+    };
+
     if (dist) {
         int64_t* qs = ar.get<int64_t>(N);
         int64_t* rs = ar.get<int64_t>(K);
-        LaunchTimer t("gotoh_self", s);
         if (gotoh_self_fits_lds(uint32_t(self_len),
                                 std::max({mx, iabs64(go), iabs64(ge)}))) {
+            LaunchTimer t("gotoh_self", s);
             const SelfSets sets{{q, r}, {q_off, r_off}, {N, K}, {qs, rs}, 0};
             launch_gotoh_self_lds(a, sets, uint32_t(self_len), s);
+        } else if (wave_fits(uint32_t(self_len), uint32_t(self_len))) {
+            // long strings: one wave per string
+            const uint32_t L = uint32_t(self_len);
+            run_wave(GotohWaveJob{nullptr, N + K, L, L, {q, r}, {q_off, r_off}, {qs, rs}, N}, L,
+                     L, "gotoh_wave_self");
         } else {
+            LaunchTimer t("gotoh_self", s);
             const uint64_t stride = self_len + 1;
             const uint32_t slots = slots_for(stride);
             int64_t* scratch = ar.get<int64_t>(size_t(slots) * 6 * stride);
@@ -966,14 +992,20 @@ void gotoh_device(const uint8_t* q, const uint64_t* q_off, uint64_t N, uint32_t 
         }
     }
     const int gen_class = cb.count - 1;
-    if (fast && counts[gen_class] > 0) {
+    // Few long queries: a wave per pair rather than a lane per pair (the multi-stripe kernel
+    // below gives a lane to each pair and needs about 2^18 pairs to fill the chip).
+    const bool wave = fast && counts[gen_class] > 0 && wave_fits(q_max, r_max) &&
+                      uint64_t(counts[gen_class]) * K <= kWavePairsMax;
+    if (wave) {
+        run_wave(GotohWaveJob{lists + size_t(gen_class) * N, uint64_t(counts[gen_class]) * K,
+                              q_max, r_max, {}, {}, {}, 0},
+                 q_max, r_max, "gotoh_wave");
+        launched("gotoh_wave");
+    } else if (fast && counts[gen_class] > 0) {
         // queries past one register stripe: the multi-stripe register kernel
         const uint32_t n_long = counts[gen_class];
         const bool hform = packed16 || go >= ge;
         const uint32_t W = std::max<uint32_t>(r_max, 1);
-        int dev = 0, cus = 256;
-        if (hipGetDevice(&dev) == hipSuccess)
-            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         const uint64_t units = gotoh_stripes_units(n_long, packed16, a.refs.nchunks);
         const uint64_t per_wg = gotoh_stripes_words(1, W, hform) * sizeof(uint32_t);
         const uint64_t by_mem = std::max<uint64_t>(1, (uint64_t(1) << 30) / per_wg);

@@ -612,6 +612,194 @@ __global__ __launch_bounds__(kSelfThreads) void gotoh_self_lds_kernel(GotohArgs 
     res[x] = LOCAL ? best : max(M(L), max(X(L), Y(L)));
 }
 
+// Intra-pair wavefront kernel for long pairs that are few: one wave per (query, reference)
+// pair, so 64 lanes share one DP instead of one lane taking a whole pair. Lane l owns R
+// consecutive query rows of the current stripe of 64 R rows, with two registers per row (the
+// H form of gotoh_reg_kernel: Hg = H - go and Y; X flows down the rows in one register) and
+// the lane's query codes as byte offsets into an LDS table tab[ref code][query code] = s + go.
+// The lanes form a pipeline over the reference columns: at step t lane l scores column
+// j = t - l + 1 for its rows, taking the row above (Hg and X at column j) from lane l - 1's
+// bottom row of step t - 1 and the diagonal Hg from the value it received one step earlier,
+// both by a one-lane shuffle, and the residue r_j the same way; lane 0 takes them from the
+// stripe's top row (row 0's boundary, or the previous stripe's bottom row kept by lane 63 in
+// the wave's boundary buffer) and from the reference, read in 64-column blocks every 64 steps.
+// A stripe takes n + 63 steps and has no barrier; between stripes one workgroup-scope
+// release / acquire fence orders lane 63's boundary stores before the next stripe's loads.
+// Rows past the query's end compute values nobody reads: they are masked out of the local
+// maximum, and the global score is read at row m, column n. int32 cells (host range bound:
+// (q_max + r_max + 2) * max(|s|, go, ge) < 2^28) with kWaveNeg as -inf, so every comparison is
+// the int64 DP's. Same recurrences, boundary values and results as gotoh_i64 when
+// gap_open >= gap_extend.
+constexpr int32_t kWaveNeg = -(1 << 29);
+constexpr uint32_t kWaveThreads = 256;  // four independent waves per workgroup
+
+template <int R, bool LOCAL>
+__global__ __launch_bounds__(kWaveThreads) void gotoh_wave_kernel(GotohArgs a, GotohWaveJob job,
+                                                                  int32_t* __restrict__ bnd_all,
+                                                                  uint32_t W) {
+    __shared__ int32_t tab[kMaxAlphabet * kMaxAlphabet];  // [ref code][query code] = s + go
+    const int32_t go = a.gap_open, ge = a.gap_extend;
+    const uint32_t A = a.A;
+    for (uint32_t e = threadIdx.x; e < kMaxAlphabet * kMaxAlphabet; e += kWaveThreads) {
+        const uint32_t rc = e / kMaxAlphabet, qc = e % kMaxAlphabet;
+        tab[e] = (rc < A && qc < A) ? a.sim[qc * A + rc] + go : 0;
+    }
+    __syncthreads();  // the only barrier: the waves below are independent
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t gw = uint64_t(blockIdx.x) * (kWaveThreads / 64) + (threadIdx.x >> 6);
+    const uint64_t nw = uint64_t(gridDim.x) * (kWaveThreads / 64);
+    int32_t* bH = bnd_all ? bnd_all + gw * 2 * (uint64_t(W) + 1) : nullptr;
+    int32_t* bX = bH ? bH + (uint64_t(W) + 1) : nullptr;
+    constexpr uint32_t S = 64 * R;  // rows per stripe
+
+    for (uint64_t p = gw; p < job.n_pairs; p += nw) {
+        uint32_t qi, ri;
+        const uint8_t *q, *r;
+        uint32_t m, n;
+        if (job.list) {  // matrix: every listed query against every reference
+            qi = job.list[p / a.n_refs];
+            ri = uint32_t(p % a.n_refs);
+            const uint64_t q0 = a.q_off[qi], r0 = a.r_off[ri];
+            q = a.q_codes + q0;
+            r = a.r_codes + r0;
+            m = uint32_t(a.q_off[qi + 1] - q0);
+            n = uint32_t(a.r_off[ri + 1] - r0);
+        } else {  // self scores of two sets
+            const int set = p < job.n_first ? 0 : 1;
+            qi = ri = uint32_t(set ? p - job.n_first : p);
+            const uint64_t o = job.off[set][qi];
+            q = r = job.codes[set] + o;
+            m = n = uint32_t(job.off[set][qi + 1] - o);
+        }
+        if (m > job.q_max || n > job.r_max) continue;  // outside the bounds the host sized for
+        int64_t score;
+        if (m == 0 || n == 0) {
+            const uint32_t L = m ? m : n;
+            score = LOCAL || L == 0 ? 0 : -(int64_t(go) + int64_t(L - 1) * ge);
+        } else {
+            const uint32_t ns = (m + S - 1) / S;
+            int32_t best = 0, res = 0;
+            for (uint32_t st = 0; st < ns; ++st) {
+                const uint32_t row0 = st * S + lane * R;  // the lane's rows: row0 + 1 ..
+                uint32_t qo[R];
+                int32_t Hs[R], Y[R];
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    const uint32_t i = row0 + k + 1;  // 1-based query row
+                    qo[k] = i <= m ? uint32_t(q[i - 1]) : 0u;
+                    // column 0: H = X = -gap_run(i) (global) or 0 (local); Y unreachable
+                    const int32_t h0 = LOCAL ? 0 : -(go + int32_t(i - 1) * ge);
+                    Hs[k] = h0 - go;
+                    Y[k] = kWaveNeg;
+                }
+                // the row above the stripe at column 0 (lane 0's first diagonal)
+                int32_t top0;
+                if (st == 0) top0 = -go;  // H[0][0] = 0
+                else top0 = bH[0];
+                // this lane's bottom row at column 0, until the lane starts its columns
+                int32_t outH = Hs[R - 1], outX = kWaveNeg;
+                if (st + 1 < ns) {
+                    // top0's load completes before lane 63 overwrites bH[0] for the next stripe
+                    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+                    if (lane == 63) bH[0] = outH;
+                }
+                int32_t prevIn = lane == 0 ? top0 : 0;
+                uint32_t rcur = 0;
+                int32_t rblk = 0, hblk = 0, xblk = 0;
+                const uint32_t steps = n + 63;
+                for (uint32_t t = 0; t < steps; ++t) {
+                    if ((t & 63u) == 0) {  // lane 0's sources for the next 64 columns
+                        const uint32_t c = t + lane;  // 0-based column index j - 1
+                        rblk = c < n ? int32_t(r[c]) : 0;
+                        if (st > 0 && c < n) {
+                            hblk = bH[c + 1];
+                            xblk = bX[c + 1];
+                        }
+                    }
+                    const int32_t jt = int32_t(t) + 1;  // lane 0's column this step
+                    int32_t topH, topX;
+                    if (st == 0) {
+                        topH = (LOCAL ? 0 : -(go + (jt - 1) * ge)) - go;
+                        topX = kWaveNeg;
+                    } else {
+                        topH = __builtin_amdgcn_readlane(hblk, int(t & 63u));
+                        topX = __builtin_amdgcn_readlane(xblk, int(t & 63u));
+                    }
+                    const uint32_t r0c = uint32_t(__builtin_amdgcn_readlane(rblk, int(t & 63u)));
+                    int32_t inH = __shfl_up(outH, 1, 64), inX = __shfl_up(outX, 1, 64);
+                    uint32_t rc = uint32_t(__shfl_up(int32_t(rcur), 1, 64));
+                    if (lane == 0) {
+                        inH = topH;
+                        inX = topX;
+                        rc = r0c;
+                    }
+                    rcur = rc;
+                    const int32_t j = int32_t(t) - int32_t(lane) + 1;
+                    if (j >= 1 && j <= int32_t(n)) {
+                        const int32_t* trow = tab + rc * kMaxAlphabet;
+                        int32_t up = inH, xup = inX, hd = prevIn;
+#pragma unroll
+                        for (int k = 0; k < R; ++k) {
+                            int32_t mv = hd + trow[qo[k]];
+                            if (LOCAL) mv = max(mv, 0);
+                            const int32_t xv = max(up, xup - ge);
+                            const int32_t yv = max(Hs[k], Y[k] - ge);
+                            hd = Hs[k];
+                            Hs[k] = max(max(mv, xv), yv) - go;
+                            up = Hs[k];
+                            xup = xv;
+                            Y[k] = yv;
+                            if (LOCAL) {
+                                if (row0 + k < m) best = max(best, mv);
+                            } else if (row0 + k + 1 == m && j == int32_t(n)) {
+                                res = Hs[k] + go;
+                            }
+                        }
+                        outH = up;
+                        outX = xup;
+                        if (lane == 63 && st + 1 < ns) {
+                            bH[j] = outH;
+                            bX[j] = outX;
+                        }
+                    }
+                    prevIn = inH;
+                }
+                if (st + 1 < ns) {
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                }
+            }
+            if (LOCAL) {
+                for (int off = 32; off > 0; off >>= 1) best = max(best, __shfl_xor(best, off, 64));
+                score = best;
+            } else {
+                const uint32_t lane_m = ((m - 1) % S) / R;
+                score = __shfl(res, int(lane_m), 64);
+            }
+        }
+        if (lane == 0) {
+            if (!job.list) {
+                job.res[p < job.n_first ? 0 : 1][qi] = score;
+            } else {
+                const int64_t v = a.dist ? a.q_self[qi] + a.r_self[ri] - 2 * score : score;
+                if (v < INT32_MIN || v > INT32_MAX) *a.overflow = 1;
+                a.out[uint64_t(qi) * a.n_refs + ri] = int32_t(v);
+            }
+        }
+    }
+}
+
+template <int R>
+void launch_wave_rows(const GotohArgs& a, const GotohWaveJob& job, uint32_t grid, int32_t* bnd,
+                      uint32_t W, hipStream_t s) {
+    if (a.local)
+        hipLaunchKernelGGL((gotoh_wave_kernel<R, true>), dim3(grid), dim3(kWaveThreads), 0, s, a,
+                           job, bnd, W);
+    else
+        hipLaunchKernelGGL((gotoh_wave_kernel<R, false>), dim3(grid), dim3(kWaveThreads), 0, s,
+                           a, job, bnd, W);
+}
+
 template <class C, int R, bool HFORM>
 void launch_form(const GotohArgs& a, const uint32_t* list, uint32_t count, hipStream_t s) {
     const uint64_t nch = a.refs.nchunks;
@@ -723,6 +911,26 @@ void launch_gotoh_self(const GotohArgs& a, const uint8_t* codes, const uint64_t*
     if (n == 0) return;
     hipLaunchKernelGGL(gotoh_self_kernel, dim3(slots / 256), dim3(256), 0, s, a, codes, off, n,
                        res, scratch, stride);
+}
+
+uint32_t gotoh_wave_rows(uint32_t q_max) {
+    return q_max > 1024 ? 32u : q_max > 512 ? 16u : 8u;
+}
+
+uint64_t gotoh_wave_bnd_words(uint32_t grid, uint32_t q_max, uint32_t r_max) {
+    // one stripe needs no boundary buffer
+    if (q_max <= 64 * gotoh_wave_rows(q_max)) return 0;
+    return uint64_t(grid) * (kWaveThreads / 64) * 2 * (uint64_t(r_max) + 1);
+}
+
+void launch_gotoh_wave(const GotohArgs& a, const GotohWaveJob& job, uint32_t q_max,
+                       uint32_t r_max, uint32_t grid, int32_t* bnd, hipStream_t s) {
+    if (job.n_pairs == 0 || grid == 0) return;
+    switch (gotoh_wave_rows(q_max)) {
+        case 8: launch_wave_rows<8>(a, job, grid, bnd, r_max, s); break;
+        case 16: launch_wave_rows<16>(a, job, grid, bnd, r_max, s); break;
+        default: launch_wave_rows<32>(a, job, grid, bnd, r_max, s); break;
+    }
 }
 
 bool gotoh_self_fits_lds(uint32_t max_len, int64_t mx) {

@@ -317,6 +317,28 @@ struct SelfSets {
     uint32_t nb0;
 };
 bool gotoh_self_fits_lds(uint32_t max_len, int64_t mx);
+// Intra-pair wavefront kernel (one wave per pair, the query's rows spread over the lanes): for
+// long queries when the pairs are too few to give every lane a pair of its own. Matrix mode
+// (list != nullptr): every list[x] (x < count) against every reference, pairs count * n_refs,
+// written to a.out like the register kernels (dist and overflow included). Self mode
+// (list == nullptr): s(x, x) for the strings of two sets in one launch, the first n_first of
+// the n_pairs from (codes[0], off[0]) into res[0], the rest from set 1 into res[1]. Needs
+// gap_open >= gap_extend and (q_max + r_max + 2) * max(|s|, go, ge) < 2^28 (int32 cells);
+// the matrix mode also needs the reference side as CSR (a.r_codes, a.r_off). bnd holds
+// gotoh_wave_bnd_words(grid, q_max, r_max) int32 (none when one stripe covers q_max).
+struct GotohWaveJob {
+    const uint32_t* list;
+    uint64_t n_pairs;
+    uint32_t q_max, r_max;  // length bounds (pairs beyond them are skipped, not scored)
+    const uint8_t* codes[2];
+    const uint64_t* off[2];
+    int64_t* res[2];
+    uint64_t n_first;
+};
+uint32_t gotoh_wave_rows(uint32_t q_max);
+uint64_t gotoh_wave_bnd_words(uint32_t grid, uint32_t q_max, uint32_t r_max);
+void launch_gotoh_wave(const GotohArgs& a, const GotohWaveJob& job, uint32_t q_max,
+                       uint32_t r_max, uint32_t grid, int32_t* bnd, hipStream_t s);
 void launch_gotoh_self_lds(const GotohArgs& a, const SelfSets& sets, uint32_t max_len,
                            hipStream_t s);
 

@@ -165,9 +165,21 @@ def test_long_queries_run_in_stripes(gpu, oracle_lib, mode, go, ge, label):
     q += [rand_seq(rng, 150, 250) for _ in range(3)] + [rand_seq(rng, 0, 30) for _ in range(8)]
     r = [rand_seq(rng, 0, 120) for _ in range(500)] + [rand_seq(rng, 200, 260) for _ in range(8)]
     r += q[:6]
-    _, labels = launch_labels(lambda: _check(st, oracle_lib, q, r, m, mode, go, ge))
-    assert any(x.startswith(label) for x in labels), labels
-    _check(st, oracle_lib, q, r, m, mode, go, ge, dist=True)
+    # the 33 long queries meet the reference set 18 times over, so their pairs pass 2^18 and the
+    # lane-per-pair kernel runs (fewer go to the one-wave-per-pair kernel): the oracle checks
+    # the distinct columns, and every repeat must equal its original
+    reps = 18
+    for dist in (False, True):
+        want = _check(st, oracle_lib, q, r, m, mode, go, ge, dist=dist)
+        if want is None:  # the overflow error, checked by _check
+            continue
+        fn = st.pairwise.dist_matrix if dist else st.pairwise.score_matrix
+        got, labels = launch_labels(lambda: np.asarray(fn(q, r * reps, m, mode=mode,
+                                                          gap_open=go, gap_extend=ge)))
+        if not dist:
+            assert any(x.startswith(label) for x in labels), labels
+        got = got.reshape(len(q), reps, len(r))
+        assert all(np.array_equal(got[:, k], want) for k in range(reps))
 
 
 def test_germline_v_all_vs_all(gpu, oracle_lib):
@@ -230,3 +242,57 @@ def test_bad_arguments(gpu):
     with pytest.raises(ValueError, match="alphabet|symbol"):
         st.pairwise.score("AA#", "AA", m)
     assert st.pairwise.score_matrix([], ["A"], m).shape == (0, 1)
+
+
+# ------------------------------------------------ few long pairs: one wave per pair (8f-4)
+
+def _sim_table(st, m):
+    syms = st.amino_acids()
+    return np.array([[m.similarity(a, b) for b in syms] for a in syms], np.int32)
+
+
+@pytest.mark.parametrize("mode", ["global", "local"])
+def test_long_pairs_run_one_wave_per_pair(gpu, oracle_lib, mode):
+    """Queries of 500-3,000 residues against references of every length class, few enough
+    pairs for the intra-pair wavefront kernel; scores and distances (self-scores of long
+    strings included) against the oracle's int64 DP."""
+    st = gpu
+    rng = random.Random(seed_of("wave", mode))
+    aa = "ACDEFGHIKLMNPQRSTVWY"
+    qs = [rand_seq(rng, 500, 3000, aa) for _ in range(4)] + \
+        [rand_seq(rng, L, L, aa) for L in (65, 512, 513, 1025, 2048, 2049, 4097)]
+    rs = [rand_seq(rng, L, L, aa) for L in (1, 5, 63, 64, 65, 300)] + \
+        [rand_seq(rng, 1000, 3000, aa) for _ in range(3)]
+    m = st.SubstitutionMatrix.blosum62()
+    sim = _sim_table(st, m)
+    for go, ge in ((11, 1), (5, 5), (3, 2)):
+        got, labels = launch_labels(lambda: np.asarray(
+            st.pairwise.score_matrix(qs, rs, m, mode=mode, gap_open=go, gap_extend=ge)))
+        assert f"gotoh_wave_{mode}" in labels, labels
+        code, want = oracle_lib.align_matrix(qs, rs, sim, mode == "local", False, go, ge,
+                                             threads=8)
+        assert code == 0 and np.array_equal(got, np.asarray(want).reshape(got.shape)), (go, ge)
+    got, labels = launch_labels(lambda: np.asarray(
+        st.pairwise.dist_matrix(qs, rs, m, mode=mode)))
+    assert f"gotoh_wave_self_{mode}" in labels and f"gotoh_wave_{mode}" in labels, labels
+    code, want = oracle_lib.align_matrix(qs, rs, sim, mode == "local", True, 11, 1, threads=8)
+    assert code == 0 and np.array_equal(got, np.asarray(want).reshape(got.shape))
+
+
+def test_long_pairs_wave_equals_the_lane_per_pair_kernel(gpu):
+    """The same long queries scored by both long-query kernels: few references send them to
+    the wavefront kernel, many to the lane-per-pair multi-stripe kernel; the rows agree."""
+    st = gpu
+    rng = random.Random(12)
+    aa = "ACDEFGHIKLMNPQRSTVWY"
+    qs = [rand_seq(rng, 90, 700, aa) for _ in range(6)]
+    rs = [rand_seq(rng, 8, 400, aa) for _ in range(50_000)]  # 300,000 pairs: multi-stripe
+    m = st.SubstitutionMatrix.pam250()
+    for mode in ("global", "local"):
+        big, labels = launch_labels(lambda: np.asarray(
+            st.pairwise.score_matrix(qs, rs, m, mode=mode)))
+        assert f"gotoh_i16x2_stripes_{mode}" in labels or f"gotoh_i32_stripes_{mode}" in labels
+        few, labels = launch_labels(lambda: np.asarray(
+            st.pairwise.score_matrix(qs, rs[:3000], m, mode=mode)))
+        assert f"gotoh_wave_{mode}" in labels, labels
+        assert np.array_equal(big[:, :3000], few)
