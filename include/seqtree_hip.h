@@ -417,6 +417,11 @@ int32_t st_probe_mfma(double* flops_per_s, st_status* status);
  * 1 MiB, a multiple of 16), in bytes per second: the attainable write bandwidth the
  * write-bound kernels' rooflines are checked against. */
 int32_t st_probe_hbm_write(uint64_t bytes, double* bytes_per_s, st_status* status);
+/* Measured LDS read rate over the whole device, in bytes per second: 16 workgroups of 256
+ * threads per CU, each thread reading `dwords` (1, 2 or 4) consecutive dwords per load in a
+ * bank-conflict-free pattern. The attainable LDS bandwidth the table-lookup kernels'
+ * rooflines are checked against; other dword counts are ST_INVALID_ARGUMENT. */
+int32_t st_probe_lds(int32_t dwords, double* bytes_per_s, st_status* status);
 
 #ifdef __cplusplus
 }

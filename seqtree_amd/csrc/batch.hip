@@ -28,7 +28,7 @@ using gb::pair_score;
 using gb::pair_score_fast;
 
 template <bool PROFILE, bool EXACT, bool SHORT, uint32_t THREADS>
-__device__ void run_unit(const BatchJob& J, uint64_t q_begin, uint32_t qt, uint32_t c_begin,
+__device__ __forceinline__ void run_unit(const BatchJob& J, uint64_t q_begin, uint32_t qt, uint32_t c_begin,
                          uint32_t c_end, int32_t* lds, int32_t* stage) {
     constexpr uint32_t kSlots = kChunk / THREADS;
     const uint32_t tid = threadIdx.x;
@@ -94,14 +94,14 @@ __device__ void run_unit(const BatchJob& J, uint64_t q_begin, uint32_t qt, uint3
 // length sharing the 16-bit halves of the tables, gapblock_core.hpp) against chunks
 // [c_begin, c_end); the stage holds both rows of a chunk.
 template <uint32_t THREADS>
-__device__ void run_unit_twin(const BatchJob& J, uint64_t t_begin, uint32_t qt, uint32_t c_begin,
+__device__ __forceinline__ void run_unit_twin(const BatchJob& J, uint64_t t_begin, uint32_t qt, uint32_t c_begin,
                               uint32_t c_end, int32_t* lds, int32_t* stage) {
     constexpr uint32_t kSlots = kChunk / THREADS;
     const uint32_t tid = threadIdx.x;
     const uint64_t cap = twin_capacity(J.n_queries);
     const uint64_t t_end = t_begin + qt < cap ? t_begin + qt : cap;
     uint32_t* prof = reinterpret_cast<uint32_t*>(lds);
-    uint32_t* slice = prof + J.prof_rows * gb::kRowInts;
+    uint32_t* slice = prof + J.prof_rows * gb::kTwinRowInts;
     for (uint64_t p = t_begin; p < t_end; ++p) {
         // the twin (a, b) as one 8-byte global load: a in the low word
         const uint64_t tw = gb::global_ptr(reinterpret_cast<const uint64_t*>(J.twins))[p];
@@ -114,12 +114,23 @@ __device__ void run_unit_twin(const BatchJob& J, uint64_t t_begin, uint32_t qt, 
                         THREADS);
         gb::build_slice2(slice, J.slice_rows - 1, m, J.prior, J.w1, J.big, tid, THREADS);
         __syncthreads();
+        // the next slot's (index, length) load while the current slot is scored
+        uint32_t next_k = gb::global_ptr(J.refs.orig)[c_begin * kChunk + tid];
+        uint32_t next_n = gb::global_ptr(J.refs.len)[c_begin * kChunk + tid];
         for (uint32_t c = c_begin; c < c_end; ++c) {
 #pragma unroll 1  // one slot's twin scorer at a time: bounds the registers
             for (uint32_t it = 0; it < kSlots; ++it) {
                 const uint32_t slot = c * kChunk + it * THREADS + tid;
-                const uint32_t k = gb::global_ptr(J.refs.orig)[slot];
-                const uint32_t n = gb::global_ptr(J.refs.len)[slot];
+                const uint32_t k = next_k;
+                const uint32_t n = next_n;
+                {
+                    const uint32_t it2 = it + 1 < kSlots ? it + 1 : 0;
+                    const uint32_t c2 = it + 1 < kSlots ? c : c + 1;
+                    if (c2 < c_end) {
+                        next_k = gb::global_ptr(J.refs.orig)[c2 * kChunk + it2 * THREADS + tid];
+                        next_n = gb::global_ptr(J.refs.len)[c2 * kChunk + it2 * THREADS + tid];
+                    }
+                }
                 int32_t sa = 0, sb = 0;
                 gb::twin_score_short(
                     prof, slice, gb::buffer_words(J.refs.words, J.refs.npad, J.refs.nwords, slot),
@@ -144,8 +155,10 @@ __device__ void run_unit_twin(const BatchJob& J, uint64_t t_begin, uint32_t qt, 
     }
 }
 
-// Dynamic LDS: the chunk stage (stage_ints int32), then the unit's tables.
-template <uint32_t THREADS>
+// Dynamic LDS: the chunk stage (stage_ints int32), then the unit's tables. TWIN_ONLY: every
+// job has twin rows, so the kernel holds only the twin path (and needs far fewer registers
+// than the general one, whose other paths set its register count).
+template <uint32_t THREADS, bool TWIN_ONLY>
 __global__ __launch_bounds__(THREADS) void gapblock_batch_kernel(
     const BatchJob* __restrict__ jobs, uint32_t n_jobs, uint64_t total_units, uint32_t qt,
     uint32_t stage_ints, unsigned long long* __restrict__ next_unit) {
@@ -178,7 +191,7 @@ __global__ __launch_bounds__(THREADS) void gapblock_batch_kernel(
         const uint64_t q_begin = (local % J.qblocks) * qt;
         const uint32_t c0 = g * J.group;
         const uint32_t c1 = c0 + J.group < J.refs.nchunks ? c0 + J.group : J.refs.nchunks;
-        if (J.twin) {
+        if (TWIN_ONLY || J.twin) {
             run_unit_twin<THREADS>(J, q_begin, qt, c0, c1, lds, stage);
         } else if (J.short_path) {
             run_unit<true, false, true, THREADS>(J, q_begin, qt, c0, c1, lds, stage);
@@ -197,15 +210,20 @@ __global__ __launch_bounds__(THREADS) void gapblock_batch_kernel(
 
 void launch_gapblock_batch(const BatchJob* jobs, uint32_t n_jobs, uint64_t total_units,
                            const BatchTiles& tiles, uint32_t stage_ints, size_t lds_bytes,
-                           unsigned long long* next_unit, hipStream_t s) {
+                           bool twin_only, unsigned long long* next_unit, hipStream_t s) {
     if (total_units == 0) return;
     const dim3 grid(tiles.workgroups);
-    if (tiles.threads == 512)
-        hipLaunchKernelGGL(gapblock_batch_kernel<512>, grid, dim3(512), lds_bytes, s, jobs, n_jobs,
-                           total_units, tiles.queries_per_unit, stage_ints, next_unit);
-    else
-        hipLaunchKernelGGL(gapblock_batch_kernel<256>, grid, dim3(256), lds_bytes, s, jobs, n_jobs,
-                           total_units, tiles.queries_per_unit, stage_ints, next_unit);
+    const auto go = [&](auto kernel, uint32_t threads) {
+        hipLaunchKernelGGL(kernel, grid, dim3(threads), lds_bytes, s, jobs, n_jobs, total_units,
+                           tiles.queries_per_unit, stage_ints, next_unit);
+    };
+    if (tiles.threads == 512) {
+        if (twin_only) go(gapblock_batch_kernel<512, true>, 512);
+        else go(gapblock_batch_kernel<512, false>, 512);
+    } else {
+        if (twin_only) go(gapblock_batch_kernel<256, true>, 256);
+        else go(gapblock_batch_kernel<256, false>, 256);
+    }
 }
 
 }  // namespace st

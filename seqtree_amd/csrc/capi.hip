@@ -781,6 +781,7 @@ struct BatchPlanImpl {
     size_t lds_bytes = 0;        // stage + the largest job's tables
     uint32_t stage_ints = kChunk;  // 2 kChunk when some job has twin rows
     bool any_twin = false;
+    bool all_twin = false;  // every job has twin rows: the twin-only kernel
     BatchTiles tiles{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -819,7 +820,7 @@ struct BatchPlanImpl {
                              p.scale, p.max_len, p.tw, s);
         launch_fill_u32(reinterpret_cast<uint32_t*>(d_next), 2, 0u, s);
         launch_gapblock_batch(d_jobs, uint32_t(dev_jobs.size()), total_units, tiles, stage_ints,
-                              lds_bytes, d_next, s);
+                              lds_bytes, all_twin, d_next, s);
     }
 };
 
@@ -2018,6 +2019,47 @@ int32_t st_probe_hbm_write(uint64_t bytes, double* bytes_per_s, st_status* statu
     });
 }
 
+int32_t st_probe_lds(int32_t dwords, double* bytes_per_s, st_status* status) {
+    return guarded(status, [&] {
+        if (dwords != 1 && dwords != 2 && dwords != 4)
+            throw HipError{ST_INVALID_ARGUMENT, "seqtree_hip: the LDS probe reads 1, 2 or 4 dwords"};
+        require_device();
+        int dev = 0, cus = 0;
+        check(hipGetDevice(&dev), "hipGetDevice");
+        check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+              "hipDeviceGetAttribute");
+        const uint32_t blocks = uint32_t(cus) * 16, iters = 4096 / uint32_t(dwords);
+        uint32_t* out = nullptr;
+        check(hipMalloc(&out, size_t(blocks) * 256 * sizeof(uint32_t)), "hipMalloc");
+        hipEvent_t a = nullptr, b = nullptr;
+        double best = 0.0;
+        try {
+            check(hipEventCreate(&a), "hipEventCreate");
+            check(hipEventCreate(&b), "hipEventCreate");
+            for (int rep = 0; rep < 4; ++rep) {  // the first run is a warm-up
+                check(hipEventRecord(a, nullptr), "hipEventRecord");
+                const double bytes = launch_probe_lds(dwords, out, blocks, iters, nullptr);
+                check(hipGetLastError(), "probe launch");
+                check(hipEventRecord(b, nullptr), "hipEventRecord");
+                check(hipEventSynchronize(b), "hipEventSynchronize");
+                float ms = 0.f;
+                check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+                if (rep > 0) best = std::max(best, bytes / (double(ms) * 1e-3));
+            }
+        } catch (...) {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+            (void)hipFree(out);
+            throw;
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        check(hipFree(out), "hipFree");
+        *bytes_per_s = best;
+        return ok(status);
+    });
+}
+
 void st_timing_enable(int32_t on) {
     std::lock_guard<std::mutex> lk(g_timing_mu);
     g_timing = on != 0;
@@ -2493,9 +2535,10 @@ int32_t st_gapblock_batch_plan_create(const st_gapblock_job* jobs, uint32_t n_jo
                 d.big = big;
                 d.prof_rows = j.q_max_len + gb::kFastMaxL;
                 d.slice_rows = j.r_max_len + 1;
-                lds = std::max(lds, size_t(d.prof_rows) * gb::kRowBytes +
+                const bool twin = twin_launch(j.n_queries, j.q_max_len, pen_abs, prior_abs, &big);
+                lds = std::max(lds, size_t(d.prof_rows) * (twin ? gb::kTwinRowBytes : gb::kRowBytes) +
                                         size_t(d.slice_rows) * gb::kSliceStride * sizeof(int32_t));
-                if (twin_launch(j.n_queries, j.q_max_len, pen_abs, prior_abs, &big)) {
+                if (twin) {
                     d.twin = 1;
                     d.big = big;
                     uint2* twins = B.alloc<uint2>(twin_capacity(j.n_queries));
@@ -2507,6 +2550,9 @@ int32_t st_gapblock_batch_plan_create(const st_gapblock_job* jobs, uint32_t n_jo
             }
             B.dev_jobs.push_back(d);
         }
+        B.all_twin = !B.dev_jobs.empty() &&
+                     std::all_of(B.dev_jobs.begin(), B.dev_jobs.end(),
+                                 [](const BatchJob& d) { return d.twin != 0; });
         B.stage_ints = B.any_twin ? 2 * kChunk : kChunk;
         B.lds_bytes = size_t(B.stage_ints) * sizeof(int32_t) + lds;
         B.d_jobs = B.alloc<BatchJob>(std::max<size_t>(B.dev_jobs.size(), 1));

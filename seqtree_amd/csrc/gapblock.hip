@@ -206,7 +206,7 @@ __global__ __launch_bounds__(kThreads) void gapblock_twin_kernel(GapblockArgs a,
     const uint64_t qa0 = a.q_off[ia], qb0 = a.q_off[ib];
     const uint32_t m = uint32_t(a.q_off[ia + 1] - qa0);
     uint32_t* prof = reinterpret_cast<uint32_t*>(lds);
-    uint32_t* slice = prof + a.prof_rows * gb::kRowInts;
+    uint32_t* slice = prof + a.prof_rows * gb::kTwinRowInts;
     gb::build_rows2(prof, a.prof_rows, m, a.pen, a.A, a.q_codes + qa0, a.q_codes + qb0, tid,
                     kThreads);
     gb::build_slice2(slice, a.slice_rows - 1, m, a.prior, a.w1, a.big, tid, kThreads);
@@ -342,7 +342,7 @@ void launch_gapblock(const GapblockArgs& a, bool use_profile, uint32_t q_max_len
 void launch_gapblock_twin(const GapblockArgs& a, const uint2* twins, uint64_t n_twins,
                           hipStream_t s) {
     if (n_twins == 0 || a.refs.nchunks == 0) return;
-    const size_t lds = size_t(a.prof_rows) * gb::kRowBytes +
+    const size_t lds = size_t(a.prof_rows) * gb::kTwinRowBytes +
                        size_t(a.slice_rows) * gb::kSliceStride * sizeof(uint32_t);
     const uint64_t ngroups = (a.refs.nchunks + a.group - 1) / a.group;
     const uint64_t max_t = (uint64_t(1) << 30) / ngroups;  // launches under 2^30 workgroups

@@ -405,6 +405,9 @@ __device__ __forceinline__ int32_t pair_score_short(const int32_t* prof, const i
 }
 
 // ---- twin rows: two queries of one length in the 16-bit halves of every word ------------
+// Twin tables hold no negations: a twin row is 32 words (kTwinRowBytes), and the diagonal
+// penalty is subtracted with a packed 16-bit subtract (the same bits as adding the stored
+// 16-bit negation), which halves the profile's LDS and so raises the workgroups per CU.
 // Short launches can score two queries a and b of the same length m at once: every table
 // word holds query a's value in its low half and query b's in its high half (profile rows
 // pen[q_a[t]][c] | pen[q_b[t]][c] << 16 and the negations; prior-slice entries duplicated).
@@ -414,8 +417,21 @@ __device__ __forceinline__ int32_t pair_score_short(const int32_t* prof, const i
 // then |D| <= 2T, |nS| <= T, real candidates lie in [-2T - R, 2T + R], the sentinel
 // big = 2^15 - 1 - 2T keeps D + big in (2T + R, 2^15 - 1], and no half ever wraps, so the
 // signed 16-bit minimum and the final int32 combination equal the int32 path bit for bit.
+constexpr uint32_t kTwinRowInts = 32;
+constexpr uint32_t kTwinRowBytes = kTwinRowInts * 4;
+
 using u16x2 = uint16_t __attribute__((ext_vector_type(2)));
 using i16x2 = int16_t __attribute__((ext_vector_type(2)));
+
+// An empty asm statement that takes and returns v in a VGPR: the compiler must have v's value
+// at this point, so it cannot re-associate the arithmetic that produces v across it.
+template <class T>
+__device__ __forceinline__ void pin(T& v) {
+    static_assert(sizeof(T) == 4, "pin takes one 32-bit value");
+    uint32_t u = __builtin_bit_cast(uint32_t, v);
+    asm volatile("" : "+v"(u));
+    v = __builtin_bit_cast(T, u);
+}
 
 __device__ __forceinline__ uint32_t pack16(int32_t lo, int32_t hi) {
     return (uint32_t(lo) & 0xFFFFu) | (uint32_t(hi) << 16);
@@ -426,18 +442,19 @@ __device__ __forceinline__ u16x2 lds_at2(const uint32_t* base, uint32_t byte_off
         u16x2, *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(base) + byte_off));
 }
 
-// build_rows for two queries of length `valid` (codes qa, qb), packed as above.
+// Twin profile rows for two queries of length `valid` (codes qa, qb): row r entry b is
+// pen[qa[r]][b] | pen[qb[r]][b] << 16 for r < valid and b < A, zero elsewhere.
 __device__ __forceinline__ void build_rows2(uint32_t* lds, uint32_t rows, uint32_t valid,
                                             const int32_t* pen, uint32_t A, const uint8_t* qa,
                                             const uint8_t* qb, uint32_t tid, uint32_t threads) {
-    for (uint32_t e = tid; e < rows * kRowInts; e += threads) {
-        const uint32_t r = e / kRowInts, c = e % kRowInts, b = c % 32;
+    for (uint32_t e = tid; e < rows * kTwinRowInts; e += threads) {
+        const uint32_t r = e / kTwinRowInts, b = e % kTwinRowInts;
         int32_t va = 0, vb = 0;
         if (r < valid && b < A) {
             va = pen[uint32_t(qa[r]) * A + b];
             vb = pen[uint32_t(qb[r]) * A + b];
         }
-        lds[e] = c < 32 ? pack16(va, vb) : pack16(-va, -vb);
+        lds[e] = pack16(va, vb);
     }
 }
 
@@ -463,23 +480,19 @@ __device__ __forceinline__ int32_t twin_finish(int16_t ns, int16_t best, uint32_
                   : int32_t(Sd + uint32_t(int32_t(best)) + uint32_t(go) + (d - 1) * uint32_t(ge));
 }
 
-// pair_score_mixed on twin rows: per-lane n, one instruction stream of 4 * NW positions.
-template <uint32_t NW, class Words>
-__device__ __forceinline__ void twin_score_mixed(const uint32_t* prof, const uint32_t* slice,
-                                                 const Words& words, uint32_t m, uint32_t n,
-                                                 int32_t go, int32_t ge, int32_t& sa,
-                                                 int32_t& sb) {
+// pair_score_mixed on twin rows: per-lane n, one instruction stream of 4 * NW positions;
+// x: the reference words from position 0 (NW of them), z: the words from the lane's
+// diagonal word offset (NW + 1), both already in registers.
+template <uint32_t NW>
+__device__ __forceinline__ void twin_mixed_regs(const uint32_t* prof, const uint32_t* slice,
+                                                const uint32_t* x, const uint32_t* z,
+                                                uint32_t m, uint32_t n, int32_t go, int32_t ge,
+                                                int32_t& sa, int32_t& sb) {
     const uint32_t d = m > n ? m - n : n - m;
     const bool ql = m >= n;
-    const uint32_t dq = ql ? 0u : d >> 2, sh = ql ? 0u : d & 3u;
-    const uint32_t off1 = (ql ? d * kRowBytes : 0u) + kNegBytes;
+    const uint32_t sh = ql ? 0u : d & 3u;
+    const uint32_t off1 = ql ? d * kTwinRowBytes : 0u;
     const uint32_t* prow = slice + n * kSliceStride;
-    uint32_t x[NW], z[NW + 1];
-#pragma unroll
-    for (uint32_t w = 0; w < NW; ++w) x[w] = words(w);
-    const auto wz = words.shift(dq);
-#pragma unroll
-    for (uint32_t j = 0; j <= NW; ++j) z[j] = wz(j);
     u16x2 D = {0, 0}, nS = {0, 0};  // per half: D = P0 - Pd, nS = -Pd
     i16x2 best = __builtin_bit_cast(i16x2, prow[0]);
 #pragma unroll
@@ -490,14 +503,19 @@ __device__ __forceinline__ void twin_score_mixed(const uint32_t* prof, const uin
             const uint32_t t = 4 * w + b;
             const uint32_t rc4 = (x[w] >> (8 * b)) & 0xFFu;
             const uint32_t rd4 = (y >> (8 * b)) & 0xFFu;
-            const u16x2 p0 = lds_at2(prof, t * kRowBytes + rc4);
-            const u16x2 npd = lds_at2(prof, t * kRowBytes + (rd4 + off1));
+            const u16x2 p0 = lds_at2(prof, t * kTwinRowBytes + rc4);
+            const u16x2 pd = lds_at2(prof, t * kTwinRowBytes + (rd4 + off1));
             D += p0;
-            D += npd;
-            nS += npd;
+            D -= pd;
+            nS -= pd;
             const i16x2 cand = __builtin_bit_cast(i16x2, D + __builtin_bit_cast(u16x2, prow[t + 1]));
             best = __builtin_elementwise_min(best, cand);
         }
+        // the running sums and minimum are materialised once per word: no reassociation
+        // keeps a word's lookups live past it
+        pin(D);
+        pin(nS);
+        pin(best);
         __builtin_amdgcn_sched_barrier(0);
     }
     const i16x2 ns = __builtin_bit_cast(i16x2, nS);
@@ -505,22 +523,51 @@ __device__ __forceinline__ void twin_score_mixed(const uint32_t* prof, const uin
     sb = twin_finish(ns.y, best.y, d, go, ge);
 }
 
+// A lane's words for the scorers: x = words 0..NX-1, z = NZ words from its diagonal word
+// offset (d / 4 when the reference is longer, else 0).
+template <uint32_t NX, uint32_t NZ, class Words>
+__device__ __forceinline__ void load_words(const Words& words, uint32_t m, uint32_t n,
+                                           uint32_t* x, uint32_t* z) {
+#pragma unroll
+    for (uint32_t w = 0; w < NX; ++w) x[w] = words(w);
+    const auto wz = words.shift(m >= n ? 0u : (n - m) >> 2);
+#pragma unroll
+    for (uint32_t j = 0; j < NZ; ++j) z[j] = wz(j);
+}
+
+// pair_score_mixed on twin rows: per-lane n, one instruction stream of 4 * NW positions.
+template <uint32_t NW, class Words>
+__device__ __forceinline__ void twin_score_mixed(const uint32_t* prof, const uint32_t* slice,
+                                                 const Words& words, uint32_t m, uint32_t n,
+                                                 int32_t go, int32_t ge, int32_t& sa,
+                                                 int32_t& sb) {
+    uint32_t x[NW], z[NW + 1];
+    load_words<NW, NW + 1>(words, m, n, x, z);
+    twin_mixed_regs<NW>(prof, slice, x, z, m, n, go, ge, sa, sb);
+}
+
 // Every real lane has n == m (d == 0): the two sums only, over the m (wave-uniform) positions.
+template <uint32_t NW>
+__device__ __forceinline__ void twin_sum_regs(const uint32_t* prof, const uint32_t* x,
+                                              uint32_t m, int32_t& sa, int32_t& sb) {
+    u16x2 S = {0, 0};
+#pragma unroll
+    for (uint32_t t = 0; t < 4 * NW; ++t) {
+        if (t + 4 >= 4 * NW && t >= m) break;  // only the last word can be partial
+        S += lds_at2(prof, t * kTwinRowBytes + ((x[t / 4] >> (8 * (t % 4))) & 0xFFu));
+    }
+    const i16x2 s = __builtin_bit_cast(i16x2, S);
+    sa = s.x;
+    sb = s.y;
+}
+
 template <uint32_t NW, class Words>
 __device__ __forceinline__ void twin_score_sum(const uint32_t* prof, const Words& words,
                                                uint32_t m, int32_t& sa, int32_t& sb) {
     uint32_t x[NW];
 #pragma unroll
     for (uint32_t w = 0; w < NW; ++w) x[w] = words(w);
-    u16x2 S = {0, 0};
-#pragma unroll
-    for (uint32_t t = 0; t < 4 * NW; ++t) {
-        if (t + 4 >= 4 * NW && t >= m) break;  // only the last word can be partial
-        S += lds_at2(prof, t * kRowBytes + ((x[t / 4] >> (8 * (t % 4))) & 0xFFu));
-    }
-    const i16x2 s = __builtin_bit_cast(i16x2, S);
-    sa = s.x;
-    sb = s.y;
+    twin_sum_regs<NW>(prof, x, m, sa, sb);
 }
 
 // pair_score_short on twin rows: both scores of reference slot k against the twin queries

@@ -80,10 +80,61 @@ void launch_probe(int kind, uint32_t* out, uint32_t blocks, uint32_t iters, hipS
     hipLaunchKernelGGL(kProbes[kind], dim3(blocks), dim3(256), 0, s, out, iters, 12345u);
 }
 
-// HBM write rate: a grid-stride fill of `n` 16-byte vectors (the store pattern of the
-// write-bound distance kernels).
+// LDS read rate: each thread runs 16 reads of W dwords per iteration (ds_read_b32 / _b64 /
+// _b128 for W = 1, 2, 4). Read c of lane l takes W consecutive dwords at (c * 64 + l) * W,
+// so a wave's 64 lanes cover 64 * W consecutive dwords, the bank-conflict-free pattern. An
+// iteration's addresses carry an offset computed from the previous iteration's first sum
+// and a mask the host passes as zero, so the reads cannot be hoisted out of the loop, while
+// the 16 reads of one iteration share a base and issue back to back.
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
+template <int W>
+__global__ __launch_bounds__(256) void probe_lds_kernel(uint32_t* out, uint32_t iters,
+                                                        uint32_t mask) {
+    __shared__ uint32_t buf[16 * 64 * W];
+    for (uint32_t e = threadIdx.x; e < 16u * 64u * W; e += 256) buf[e] = e * 2654435761u;
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t acc[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) acc[c] = lane + c;
+    for (uint32_t i = 0; i < iters; ++i) {
+        const uint32_t* base = buf + lane * W + (acc[0] & mask);
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            const uint32_t* at = base + c * 64 * W;
+            if constexpr (W == 1) {
+                acc[c] += *at;
+            } else if constexpr (W == 2) {
+                const u32x2 v = *reinterpret_cast<const u32x2*>(at);
+                acc[c] += v.x + v.y;
+            } else {
+                const u32x4 v = *reinterpret_cast<const u32x4*>(at);
+                acc[c] += (v.x + v.y) + (v.z + v.w);
+            }
+        }
+    }
+    uint32_t s = 0;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) s += acc[c];
+    out[size_t(blockIdx.x) * 256 + threadIdx.x] = s;
+}
+
+double launch_probe_lds(int dwords, uint32_t* out, uint32_t blocks, uint32_t iters,
+                        hipStream_t s) {
+    if (dwords == 4)
+        hipLaunchKernelGGL(probe_lds_kernel<4>, dim3(blocks), dim3(256), 0, s, out, iters, 0u);
+    else if (dwords == 2)
+        hipLaunchKernelGGL(probe_lds_kernel<2>, dim3(blocks), dim3(256), 0, s, out, iters, 0u);
+    else
+        hipLaunchKernelGGL(probe_lds_kernel<1>, dim3(blocks), dim3(256), 0, s, out, iters, 0u);
+    const int w = dwords == 4 || dwords == 2 ? dwords : 1;
+    return double(blocks) * 256 * iters * 16 * w * 4;
+}
+
+// HBM write rate: a grid-stride fill of `n` 16-byte vectors (the store pattern of the
+// write-bound distance kernels).
 __global__ __launch_bounds__(256) void probe_fill_kernel(u32x4* __restrict__ p, uint64_t n,
                                                          uint32_t v) {
     const uint64_t stride = uint64_t(gridDim.x) * 256;

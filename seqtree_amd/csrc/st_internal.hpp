@@ -161,6 +161,9 @@ void launch_probe(int kind, uint32_t* out, uint32_t blocks, uint32_t iters, hipS
 // v_mfma_f32_32x32x16_bf16 chains (256-thread workgroups); returns the FLOPs launched.
 double launch_probe_mfma(float* out, uint32_t blocks, uint32_t iters, hipStream_t s);
 void launch_probe_fill(void* p, uint64_t bytes, uint32_t blocks, uint32_t v, hipStream_t s);
+// LDS read-rate probe with `dwords` (1, 2 or 4) per read; returns the bytes it reads.
+double launch_probe_lds(int dwords, uint32_t* out, uint32_t blocks, uint32_t iters,
+                        hipStream_t s);
 
 // ---- gapblock.hip ----------------------------------------------------------------------
 // Thresholded output: every pair with score <= cutoff is appended as (query, ref, score);
@@ -253,10 +256,11 @@ struct BatchTiles {
     uint32_t workgroups;
 };
 // lds_bytes: the chunk stage (stage_ints int32: kChunk, or 2 kChunk when a job has twin rows)
-// followed by the largest job's tables.
+// followed by the largest job's tables. twin_only: every job has twin rows (a kernel with
+// only that path).
 void launch_gapblock_batch(const BatchJob* jobs, uint32_t n_jobs, uint64_t total_units,
                            const BatchTiles& tiles, uint32_t stage_ints, size_t lds_bytes,
-                           unsigned long long* next_unit, hipStream_t s);
+                           bool twin_only, unsigned long long* next_unit, hipStream_t s);
 
 
 // ---- gotoh.hip -------------------------------------------------------------------------

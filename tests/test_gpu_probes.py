@@ -18,6 +18,7 @@ def _lib():
                                 ctypes.c_void_p]
     L.st_probe_mfma.argtypes = [dbl, ctypes.c_void_p]
     L.st_probe_hbm_write.argtypes = [ctypes.c_uint64, dbl, ctypes.c_void_p]
+    L.st_probe_lds.argtypes = [ctypes.c_int32, dbl, ctypes.c_void_p]
     return L, _device.Status
 
 
@@ -45,3 +46,17 @@ def test_mfma_and_write_probes(gpu):
     assert 1e11 < rate.value < 2e13
     assert L.st_probe_hbm_write(4096, ctypes.byref(rate), ctypes.byref(st)) == 1
     assert b"at least 1 MiB" in st.message
+
+
+def test_lds_probe(gpu):
+    L, Status = _lib()
+    rates = []
+    for dwords in (1, 2, 4):
+        rate, st = ctypes.c_double(), Status()
+        assert L.st_probe_lds(dwords, ctypes.byref(rate), ctypes.byref(st)) == 0, st.message
+        # above the HBM rate, below 256 CUs x 512 bytes per clock at 3 GHz
+        assert 1e13 < rate.value < 4e14
+        rates.append(rate.value)
+    rate, st = ctypes.c_double(), Status()
+    assert L.st_probe_lds(3, ctypes.byref(rate), ctypes.byref(st)) == 1
+    assert b"1, 2 or 4" in st.message
