@@ -2536,8 +2536,11 @@ int32_t st_gapblock_batch_plan_create(const st_gapblock_job* jobs, uint32_t n_jo
                 d.prof_rows = j.q_max_len + gb::kFastMaxL;
                 d.slice_rows = j.r_max_len + 1;
                 const bool twin = twin_launch(j.n_queries, j.q_max_len, pen_abs, prior_abs, &big);
-                lds = std::max(lds, size_t(d.prof_rows) * (twin ? gb::kTwinRowBytes : gb::kRowBytes) +
-                                        size_t(d.slice_rows) * gb::kSliceStride * sizeof(int32_t));
+                lds = std::max(lds, twin ? size_t(d.prof_rows) * gb::kTwinRowBytes +
+                                               size_t(d.slice_rows) * gb::kTwinSliceBytes
+                                         : size_t(d.prof_rows) * gb::kRowBytes +
+                                               size_t(d.slice_rows) * gb::kSliceStride *
+                                                   sizeof(int32_t));
                 if (twin) {
                     d.twin = 1;
                     d.big = big;

@@ -343,7 +343,7 @@ void launch_gapblock_twin(const GapblockArgs& a, const uint2* twins, uint64_t n_
                           hipStream_t s) {
     if (n_twins == 0 || a.refs.nchunks == 0) return;
     const size_t lds = size_t(a.prof_rows) * gb::kTwinRowBytes +
-                       size_t(a.slice_rows) * gb::kSliceStride * sizeof(uint32_t);
+                       size_t(a.slice_rows) * gb::kTwinSliceBytes;
     const uint64_t ngroups = (a.refs.nchunks + a.group - 1) / a.group;
     const uint64_t max_t = (uint64_t(1) << 30) / ngroups;  // launches under 2^30 workgroups
     for (uint64_t t0 = 0; t0 < n_twins; t0 += max_t) {

@@ -421,6 +421,7 @@ constexpr uint32_t kTwinRowInts = 32;
 constexpr uint32_t kTwinRowBytes = kTwinRowInts * 4;
 
 using u16x2 = uint16_t __attribute__((ext_vector_type(2)));
+using u32x4 = uint32_t __attribute__((ext_vector_type(4)));
 using i16x2 = int16_t __attribute__((ext_vector_type(2)));
 
 // An empty asm statement that takes and returns v in a VGPR: the compiler must have v's value
@@ -458,17 +459,34 @@ __device__ __forceinline__ void build_rows2(uint32_t* lds, uint32_t rows, uint32
     }
 }
 
-// build_slice with every entry duplicated into both halves (big: the 16-bit sentinel).
+// The twin prior slice: build_slice's entries as single 16-bit values (both queries of a twin
+// share them), kTwinSliceHalves to a row so that every row starts 16-byte aligned and a lane
+// reads eight entries with one 16-byte load. Entries past the candidates (i > min(m, n),
+// up to the row's end) hold big, the 16-bit sentinel.
+constexpr uint32_t kTwinSliceHalves = 40;
+constexpr uint32_t kTwinSliceBytes = kTwinSliceHalves * 2;
+
 __device__ __forceinline__ void build_slice2(uint32_t* slice, uint32_t r_max, uint32_t m,
                                              const int32_t* prior, uint32_t w1, int32_t big,
                                              uint32_t tid, uint32_t threads) {
-    for (uint32_t e = tid; e < (r_max + 1) * kSliceStride; e += threads) {
-        const uint32_t n = e / kSliceStride, i = e % kSliceStride;
+    uint16_t* s16 = reinterpret_cast<uint16_t*>(slice);
+    for (uint32_t e = tid; e < (r_max + 1) * kTwinSliceHalves; e += threads) {
+        const uint32_t n = e / kTwinSliceHalves, i = e % kTwinSliceHalves;
         const uint32_t L = m < n ? m : n, d = m > n ? m - n : n - m;
         int32_t v = big;
         if (i <= L) v = (prior && d) ? prior[(uint64_t(m > n ? m : n) * w1 + d) * w1 + i] : 0;
-        slice[e] = pack16(v, v);
+        s16[e] = uint16_t(uint32_t(v));
     }
+}
+
+// Entry i of a twin slice row held as 16-byte groups g[i / 8], in both halves of a word (i is
+// a constant once the caller's loops are unrolled).
+__device__ __forceinline__ u16x2 twin_prior(const u32x4* g, uint32_t i) {
+    // the element goes through a scalar first: __builtin_bit_cast applied to a vector
+    // element expression directly reads element 0 with this compiler
+    const uint32_t e = g[i / 8][(i % 8) / 2];
+    const u16x2 w = __builtin_bit_cast(u16x2, e);
+    return (i & 1u) ? __builtin_shufflevector(w, w, 1, 1) : __builtin_shufflevector(w, w, 0, 0);
 }
 
 // The int32 result of one half: Sd = -nS, then Sd + best + go + (d - 1) ge (d > 0), in the
@@ -492,11 +510,18 @@ __device__ __forceinline__ void twin_mixed_regs(const uint32_t* prof, const uint
     const bool ql = m >= n;
     const uint32_t sh = ql ? 0u : d & 3u;
     const uint32_t off1 = ql ? d * kTwinRowBytes : 0u;
-    const uint32_t* prow = slice + n * kSliceStride;
+    // the lane's prior row as 16-byte groups of eight entries; word w uses entries 4w + 1 ..
+    // 4w + 4, so group g is read at the first word that needs it (w = 0, then every odd w)
+    const u32x4* prow = reinterpret_cast<const u32x4*>(
+        reinterpret_cast<const char*>(slice) + n * kTwinSliceBytes);
+    constexpr uint32_t NG = (4 * NW + 1 + 7) / 8;
+    u32x4 g[NG];
+    g[0] = prow[0];
     u16x2 D = {0, 0}, nS = {0, 0};  // per half: D = P0 - Pd, nS = -Pd
-    i16x2 best = __builtin_bit_cast(i16x2, prow[0]);
+    i16x2 best = __builtin_bit_cast(i16x2, twin_prior(g, 0));
 #pragma unroll
     for (uint32_t w = 0; w < NW; ++w) {
+        if (w % 2 == 1 && (w + 1) / 2 < NG) g[(w + 1) / 2] = prow[(w + 1) / 2];
         const uint32_t y = __builtin_amdgcn_alignbyte(z[w + 1], z[w], sh);
 #pragma unroll
         for (uint32_t b = 0; b < 4; ++b) {
@@ -508,7 +533,7 @@ __device__ __forceinline__ void twin_mixed_regs(const uint32_t* prof, const uint
             D += p0;
             D -= pd;
             nS -= pd;
-            const i16x2 cand = __builtin_bit_cast(i16x2, D + __builtin_bit_cast(u16x2, prow[t + 1]));
+            const i16x2 cand = __builtin_bit_cast(i16x2, D + twin_prior(g, t + 1));
             best = __builtin_elementwise_min(best, cand);
         }
         // the running sums and minimum are materialised once per word: no reassociation
@@ -524,15 +549,22 @@ __device__ __forceinline__ void twin_mixed_regs(const uint32_t* prof, const uint
 }
 
 // A lane's words for the scorers: x = words 0..NX-1, z = NZ words from its diagonal word
-// offset (d / 4 when the reference is longer, else 0).
+// offset (d / 4 when the reference is longer, else 0). When no lane's reference is longer
+// than the query every offset is 0 and z repeats x, so only the words past x are loaded.
+// Every lane of the wave must be active.
 template <uint32_t NX, uint32_t NZ, class Words>
 __device__ __forceinline__ void load_words(const Words& words, uint32_t m, uint32_t n,
                                            uint32_t* x, uint32_t* z) {
 #pragma unroll
     for (uint32_t w = 0; w < NX; ++w) x[w] = words(w);
-    const auto wz = words.shift(m >= n ? 0u : (n - m) >> 2);
+    if (__all(m >= n)) {
 #pragma unroll
-    for (uint32_t j = 0; j < NZ; ++j) z[j] = wz(j);
+        for (uint32_t j = 0; j < NZ; ++j) z[j] = j < NX ? x[j] : words(j);
+    } else {
+        const auto wz = words.shift(m >= n ? 0u : (n - m) >> 2);
+#pragma unroll
+        for (uint32_t j = 0; j < NZ; ++j) z[j] = wz(j);
+    }
 }
 
 // pair_score_mixed on twin rows: per-lane n, one instruction stream of 4 * NW positions.
