@@ -349,6 +349,8 @@ class Lib:
         L.st_probe_mfma.argtypes = [ctypes.POINTER(ctypes.c_double), vp]
         L.st_probe_hbm_write.restype = i32
         L.st_probe_hbm_write.argtypes = [u64, ctypes.POINTER(ctypes.c_double), vp]
+        L.st_probe_lds.restype = i32
+        L.st_probe_lds.argtypes = [i32, ctypes.POINTER(ctypes.c_double), vp]
         L.st_probe_valu.restype = i32
         L.st_probe_valu.argtypes = [i32, ctypes.c_char_p, u64, ctypes.POINTER(ctypes.c_double),
                                     vp]
@@ -632,7 +634,9 @@ def run_gapblock(torch, dev, lib, stream, args) -> dict:
     ops = float(np.where(d > 0, 5 * L, L).sum())
     label, launches, avg_s = dominant(kernels)
     roofline = valu_roofline(label, launches, avg_s, ops, lookups_per_launch=lookups,
-                             ops_per_position=5) if label.startswith("gapblock") else None
+                             ops_per_position=5,
+                             lds_bytes_per_lookup=2 if "twin" in label else 4
+                             ) if label.startswith("gapblock") else None
     cpu = None
     if args.cpu_rows > 0:
         to_str = lambda v: "".join(AA_ORDER[c] for c in v)  # noqa: E731
@@ -713,8 +717,9 @@ def run_batch(torch, dev, lib, stream, args) -> dict:
 
     ms_step, kernels = time_steps(torch, lib, stream, step, args.steps, args.warmup)
     label, launches, avg_s = dominant(kernels)
+    all_twin = batch.twin_jobs() == len(jobs)
     roofline = valu_roofline(label, launches, avg_s, ops, lookups_per_launch=lookups,
-                             ops_per_position=5)
+                             ops_per_position=5, lds_bytes_per_lookup=2 if all_twin else 4)
     cpu = None
     if args.cpu_rows > 0:
         # the reference has no batch call: its gap-block score_matrix on rows of job 0
@@ -1174,7 +1179,8 @@ def compact(line: dict) -> dict:
     rf = line.get("roofline")
     if rf:
         out["roofline"] = {k: rf[k] for k in ("bound", "kernel", "achieved", "peak", "unit",
-                                              "frac", "traffic", "avg_launch_ms") if k in rf}
+                                              "frac", "traffic", "avg_launch_ms", "lds")
+                           if k in rf}
     cb = line.get("cpu_baseline")
     if cb:
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample",
@@ -1196,6 +1202,19 @@ def finish_roofline(line: dict, lib) -> None:
         return
     if rf["bound"] == "valu":
         rf["probe_tops"] = valu_probe(lib)
+    if "lds_bytes_per_lookup" in rf:
+        # the co-bound of the table-lookup kernels: one LDS word per lookup (4 bytes; with
+        # twin rows one word serves two pairs, 2 bytes per pair-lookup), prior reads not
+        # counted, against the measured conflict-free 4-byte LDS read rate
+        rate = ctypes.c_double()
+        st = Status()
+        if lib.lib.st_probe_lds(1, ctypes.byref(rate), ctypes.byref(st)) == 0:
+            peak = rate.value / 1e12
+            achieved = (rf["lookups_per_launch"] * rf["lds_bytes_per_lookup"]
+                        / (rf["avg_launch_ms"] / 1e3) / 1e12)
+            rf["lds"] = {"achieved": round(achieved, 2), "peak": round(peak, 2),
+                         "unit": "TB/s", "frac": round(achieved / peak, 4),
+                         "peak_basis": "st_probe_lds, 4-byte conflict-free reads"}
     elif rf["bound"] == "mfma":
         rate = ctypes.c_double()
         st = Status()

@@ -315,6 +315,9 @@ int32_t st_gapblock_batch_plan_run(st_batch_plan* plan, void* stream, st_status*
 int32_t st_gapblock_batch_plan_tiles(const st_batch_plan* plan, uint32_t* queries_per_unit,
                                      uint32_t* threads, uint32_t* workgroups,
                                      uint64_t* work_units);
+/* How many of the plan's jobs run twin rows (two queries of one length per 16-bit packed
+ * table, chosen when the job's sums provably fit 16 bits). */
+int32_t st_gapblock_batch_plan_twin_jobs(const st_batch_plan* plan, uint32_t* twin_jobs);
 void st_gapblock_batch_plan_destroy(st_batch_plan* plan);
 
 /* ---- library information ------------------------------------------------------------- */

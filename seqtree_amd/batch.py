@@ -56,6 +56,8 @@ def _lib():
     lib.st_gapblock_batch_plan_tiles.restype = ctypes.c_int32
     lib.st_gapblock_batch_plan_tiles.argtypes = [vp] + [ctypes.POINTER(ctypes.c_uint32)] * 3 + [
         ctypes.POINTER(ctypes.c_uint64)]
+    lib.st_gapblock_batch_plan_twin_jobs.restype = ctypes.c_int32
+    lib.st_gapblock_batch_plan_twin_jobs.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32)]
     lib.st_gapblock_batch_plan_destroy.argtypes = [vp]
     return lib
 
@@ -181,6 +183,12 @@ class GapblockBatch:
                                                ctypes.byref(wg), ctypes.byref(units))
         return {"queries_per_unit": q.value, "threads": th.value, "workgroups": wg.value,
                 "work_units": units.value}
+
+    def twin_jobs(self) -> int:
+        """How many jobs run twin rows (two queries per 16-bit packed table)."""
+        n = ctypes.c_uint32()
+        self._lib.st_gapblock_batch_plan_twin_jobs(self._plan, ctypes.byref(n))
+        return n.value
 
     def run(self, stream: int | None = None) -> None:
         st = _Status()

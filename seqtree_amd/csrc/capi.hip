@@ -2641,6 +2641,13 @@ int32_t st_gapblock_batch_plan_tiles(const st_batch_plan* plan, uint32_t* querie
     return ST_OK;
 }
 
+int32_t st_gapblock_batch_plan_twin_jobs(const st_batch_plan* plan, uint32_t* twin_jobs) {
+    if (!plan || !twin_jobs) return ST_INVALID_ARGUMENT;
+    *twin_jobs = uint32_t(std::count_if(plan->impl.dev_jobs.begin(), plan->impl.dev_jobs.end(),
+                                        [](const BatchJob& d) { return d.twin != 0; }));
+    return ST_OK;
+}
+
 void st_gapblock_batch_plan_destroy(st_batch_plan* plan) { delete plan; }
 
 int32_t st_align_matrix_device(const uint8_t* q_codes, const uint64_t* q_offsets,
