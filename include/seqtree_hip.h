@@ -328,6 +328,17 @@ const char* st_version(void);
 /* Number of visible HIP devices (0 when none); never faults on a machine without a GPU. */
 int32_t st_device_count(void);
 
+/* ---- pinned host buffers for matrix outputs (no counterpart in the reference) -----------
+ * Page-locked host memory owned by the library. A host-memory entry point whose output lies
+ * inside such a buffer transfers the matrix straight into it by DMA, with no staging copy;
+ * any other output goes through the library's pinned staging slots. The reference returns a
+ * std::vector<int32_t> that the Python ScoreMatrix owns (src/_bindings.cpp:203-209,
+ * 470-479); this library's ScoreMatrix takes its storage from here. st_host_alloc fails
+ * with ST_OUT_OF_MEMORY when the memory cannot be pinned (the caller then uses ordinary
+ * memory), st_host_free accepts only pointers st_host_alloc returned (and NULL). */
+int32_t st_host_alloc(uint64_t bytes, void** ptr, st_status* status);
+void st_host_free(void* ptr);
+
 /* ---- learned prefilter (no counterpart in the reference) -------------------------------
  * A 1D-conv encoder (embedding 32, three width-5 convolutions of 128 channels with ReLU,
  * mean pool, 128-d output; BF16 operands, FP32 accumulation) whose squared embedding

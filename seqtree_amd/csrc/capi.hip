@@ -269,6 +269,20 @@ private:
     uint64_t generation_ = 0;
 };
 
+// Host buffers pinned by st_host_alloc, by base address. A download whose destination lies in
+// one goes straight into it (one DMA, no staging and no host copy).
+std::mutex g_pin_mu;
+std::map<uintptr_t, size_t> g_pinned;
+
+bool library_pinned(const void* p, size_t bytes) {
+    const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    auto it = g_pinned.upper_bound(a);
+    if (it == g_pinned.begin()) return false;
+    --it;
+    return a >= it->first && a + bytes <= it->first + it->second;
+}
+
 // One transfer context: a stream, two pinned staging slots with an event each, and a small
 // device block for the scratch of small calls. A host-memory call holds one for its duration
 // (HostCall); the pool keeps them for reuse, so it grows to the number of concurrent calls.
@@ -339,14 +353,17 @@ public:
     // device -> host after everything enqueued so far on the call's stream: chunk c + 1
     // travels while chunk c is copied out of its slot. Returns when the data is in `host`.
     void download(void* host, const void* dev, size_t bytes) {
-        download_on(c_->stream, host, dev, bytes);
+        if (download_on(c_->stream, host, dev, bytes)) sync(c_->stream);
     }
     // The same on the copy stream once `after` (recorded on the call's stream after the
     // uploads) has completed, so the scoring enqueued after `after` overlaps the transfer.
+    // Into st_host_alloc memory the transfer is only enqueued: finish() waits for it.
     void download_after(hipEvent_t after, void* host, const void* dev, size_t bytes) {
         check(hipStreamWaitEvent(c_->copy, after, 0), "hipStreamWaitEvent");
         download_on(c_->copy, host, dev, bytes);
     }
+    // Waits for every transfer download_after has enqueued.
+    void finish() { sync(c_->copy); }
     // An event recorded on the call's stream now (one of a reusable set).
     hipEvent_t mark() {
         if (marks_used_ == c_marks_.size()) {
@@ -362,8 +379,14 @@ public:
     void get(T* host, const T* dev, size_t count) { download(host, dev, count * sizeof(T)); }
 
 private:
-    void download_on(hipStream_t st, void* host, const void* dev, size_t bytes) {
-        if (bytes == 0) return;
+    // Returns true when the transfer went straight into library-pinned memory and is only
+    // enqueued on `st`; otherwise the data is in `host` on return.
+    bool download_on(hipStream_t st, void* host, const void* dev, size_t bytes) {
+        if (bytes == 0) return false;
+        if (library_pinned(host, bytes)) {
+            check(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, st), "D2H");
+            return true;
+        }
         const char* src = static_cast<const char*>(dev);
         char* dst = static_cast<char*>(host);
         const size_t chunks = (bytes + kStageBytes - 1) / kStageBytes;
@@ -383,6 +406,7 @@ private:
             CopyPool::get().copy(dst + at, c_->slot[k], n);
         }
         turn_ += chunks;
+        return false;
     }
     static HostContext* create(int dev) {
         auto* c = new HostContext();
@@ -1178,6 +1202,7 @@ void run_rows(const HostSet& q, uint64_t K, int32_t* out, HostCall& call, Arena&
         }
         call.download_after(prev, out + (r0 + prev_b0) * K, dout + prev_b0 * K,
                             size_t(prev_nb) * K * 4);
+        call.finish();  // the next rows reuse dout
     }
 }
 
@@ -2104,6 +2129,36 @@ int32_t st_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
     return n;
+}
+
+int32_t st_host_alloc(uint64_t bytes, void** ptr, st_status* status) {
+    if (!ptr) {
+        set_status(status, ST_INVALID_ARGUMENT, "seqtree_hip: st_host_alloc needs a pointer");
+        return ST_INVALID_ARGUMENT;
+    }
+    *ptr = nullptr;
+    void* p = nullptr;
+    if (bytes == 0 || hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess || !p) {
+        (void)hipGetLastError();  // leave no error behind for the next call
+        set_status(status, ST_OUT_OF_MEMORY, "seqtree_hip: cannot pin %llu bytes of host memory",
+                   static_cast<unsigned long long>(bytes));
+        return ST_OUT_OF_MEMORY;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        g_pinned[reinterpret_cast<uintptr_t>(p)] = bytes;
+    }
+    *ptr = p;
+    return ok(status);
+}
+
+void st_host_free(void* ptr) {
+    if (!ptr) return;
+    {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        if (g_pinned.erase(reinterpret_cast<uintptr_t>(ptr)) == 0) return;
+    }
+    (void)hipHostFree(ptr);
 }
 
 int32_t st_gapblock_matrix_device(const uint8_t* q_codes, const uint64_t* q_offsets,

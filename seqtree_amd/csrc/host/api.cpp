@@ -17,6 +17,7 @@
 #include "seqtree_hip.h"
 
 #include <algorithm>
+#include <array>
 #include <climits>
 #include <stdexcept>
 #include <string>
@@ -33,48 +34,60 @@ struct Csr {
     uint32_t max_len = 0;
 };
 
-template <class Enc>
-Csr pack(const std::vector<std::string>& v, Enc&& encode_char) {
+// Codes of every character of `set` through `codec`, back to back, with the set's own offsets.
+// One table lookup per character; only when some character is outside the alphabet does a
+// second pass find the first such character in set order and call `bad(ch)`, which throws.
+// (Codes are below 32, so the top bit of the OR of all codes is set by kInvalid alone.)
+template <class Bad>
+Csr pack(const StringSet& set, const Codec& codec, Bad&& bad) {
     Csr c;
-    size_t total = 0;
-    for (const auto& s : v) total += s.size();
-    c.bytes.resize(total);
-    c.off.resize(v.size() + 1);
-    size_t p = 0;
-    for (size_t i = 0; i < v.size(); ++i) {
-        c.off[i] = p;
-        const std::string& s = v[i];
-        for (size_t t = 0; t < s.size(); ++t) c.bytes[p + t] = encode_char(s[t]);
-        p += s.size();
-        c.max_len = std::max<uint32_t>(c.max_len, uint32_t(s.size()));
+    const size_t n = set.total();
+    c.bytes.resize(std::max<size_t>(n, 1));
+    const char* ch = set.chars();
+    uint8_t* out = c.bytes.data();
+    uint8_t seen = 0;
+    for (size_t t = 0; t < n; ++t) {
+        const uint8_t code = codec.encode(ch[t]);
+        out[t] = code;
+        seen |= code;
     }
-    c.off[v.size()] = p;
+    if (seen & 0x80)
+        for (size_t t = 0; t < n; ++t)
+            if (out[t] == Codec::kInvalid) bad(ch[t]);
+    c.off.assign(set.offsets(), set.offsets() + set.size() + 1);
+    c.max_len = set.max_len();
     return c;
 }
 
-Csr pack_raw(const std::vector<std::string>& v) {
-    return pack(v, [](char ch) { return static_cast<uint8_t>(ch); });
-}
-
 // Encoding for gap-block: "symbol 'x' is not in the alphabet" (gapblock.cpp:12-22).
-Csr pack_gapblock(const std::vector<std::string>& v, const Codec& codec) {
-    return pack(v, [&](char ch) {
-        const uint8_t code = codec.encode(ch);
-        if (code == Codec::kInvalid)
-            throw std::invalid_argument(std::string("symbol '") + ch + "' is not in the alphabet");
-        return code;
+Csr pack_gapblock(const StringSet& v, const Codec& codec) {
+    return pack(v, codec, [](char ch) {
+        throw std::invalid_argument(std::string("symbol '") + ch + "' is not in the alphabet");
     });
 }
 
 // Encoding for the Gotoh paths: "invalid symbol 'x' in sequence" (pairwise.cpp:51-60).
-Csr pack_pairwise(const std::vector<std::string>& v, const Codec& codec) {
-    return pack(v, [&](char ch) {
-        const uint8_t code = codec.encode(ch);
-        if (code == Codec::kInvalid)
-            throw std::invalid_argument(std::string("invalid symbol '") + ch + "' in sequence");
-        return code;
+Csr pack_pairwise(const StringSet& v, const Codec& codec) {
+    return pack(v, codec, [](char ch) {
+        throw std::invalid_argument(std::string("invalid symbol '") + ch + "' in sequence");
     });
 }
+
+// A one-string set.
+StringSet one(std::string_view s) {
+    StringSet set;
+    set.push_back(s);
+    return set;
+}
+
+// The raw bytes of a set (the distance kernels compare bytes, not codes), and the offsets of a
+// single string.
+const uint8_t* raw(const StringSet& s) { return reinterpret_cast<const uint8_t*>(s.chars()); }
+const uint8_t* raw(std::string_view s) {
+    static const uint8_t kEmpty = 0;
+    return s.empty() ? &kEmpty : reinterpret_cast<const uint8_t*>(s.data());
+}
+std::array<uint64_t, 2> offsets(std::string_view s) { return {0, s.size()}; }
 
 void raise(int code, const st_status& st) {
     switch (code) {
@@ -95,8 +108,7 @@ void check_pairwise(const SubstitutionMatrix& mat, const Codec& codec, int32_t o
         throw std::invalid_argument("matrix size does not match the alphabet");
 }
 
-void pairwise_matrix(const std::vector<std::string>& queries,
-                     const std::vector<std::string>& refs, const SubstitutionMatrix& mat,
+void pairwise_matrix(const StringSet& queries, const StringSet& refs, const SubstitutionMatrix& mat,
                      Alphabet alphabet, AlignMode mode, int32_t go, int32_t ge, bool dist,
                      int32_t* out) {
     const Codec codec(alphabet);
@@ -118,8 +130,8 @@ int32_t align_score(std::string_view query, std::string_view ref, const Substitu
                     Alphabet alphabet, AlignMode mode, int32_t gap_open, int32_t gap_extend) {
     const Codec codec(alphabet);
     check_pairwise(mat, codec, gap_open, gap_extend);
-    const Csr q = pack_pairwise({std::string(query)}, codec);
-    const Csr r = pack_pairwise({std::string(ref)}, codec);
+    const Csr q = pack_pairwise(one(query), codec);
+    const Csr r = pack_pairwise(one(ref), codec);
     int32_t out = 0;
     st_status st{};
     const int rc = st_align_matrix(q.bytes.data(), q.off.data(), 1, r.bytes.data(), r.off.data(), 1,
@@ -173,8 +185,7 @@ Alignment build_alignment(std::string_view query, std::string_view ref, const Co
 
 }  // namespace
 
-std::vector<Alignment> align_pairs(const std::vector<std::string>& queries,
-                                   const std::vector<std::string>& refs,
+std::vector<Alignment> align_pairs(const StringSet& queries, const StringSet& refs,
                                    const std::vector<uint32_t>& pair_query,
                                    const std::vector<uint32_t>& pair_ref,
                                    const SubstitutionMatrix& mat, Alphabet alphabet,
@@ -210,19 +221,19 @@ std::vector<Alignment> align_pairs(const std::vector<std::string>& queries,
 
 Alignment align_pair(std::string_view query, std::string_view ref, const SubstitutionMatrix& mat,
                      Alphabet alphabet, AlignMode mode, int32_t gap_open, int32_t gap_extend) {
-    return align_pairs({std::string(query)}, {std::string(ref)}, {0u}, {0u}, mat, alphabet, mode,
+    return align_pairs(one(query), one(ref), {0u}, {0u}, mat, alphabet, mode,
                        gap_open, gap_extend)[0];
 }
 
-void align_score_matrix_into(const std::vector<std::string>& queries,
-                             const std::vector<std::string>& refs, const SubstitutionMatrix& mat,
+void align_score_matrix_into(const StringSet& queries, const StringSet& refs,
+                             const SubstitutionMatrix& mat,
                              Alphabet alphabet, AlignMode mode, int32_t gap_open,
                              int32_t gap_extend, int32_t* out) {
     pairwise_matrix(queries, refs, mat, alphabet, mode, gap_open, gap_extend, false, out);
 }
 
-void align_dist_matrix_into(const std::vector<std::string>& queries,
-                            const std::vector<std::string>& refs, const SubstitutionMatrix& mat,
+void align_dist_matrix_into(const StringSet& queries, const StringSet& refs,
+                            const SubstitutionMatrix& mat,
                             Alphabet alphabet, AlignMode mode, int32_t gap_open,
                             int32_t gap_extend, int32_t* out) {
     pairwise_matrix(queries, refs, mat, alphabet, mode, gap_open, gap_extend, true, out);
@@ -260,8 +271,7 @@ struct GapblockInput {
     bool empty = false;
 };
 
-GapblockInput gapblock_input(const std::vector<std::string>& queries,
-                             const std::vector<std::string>& refs, Alphabet alphabet,
+GapblockInput gapblock_input(const StringSet& queries, const StringSet& refs, Alphabet alphabet,
                              const SubstitutionMatrix* matrix, int32_t gap_open,
                              int32_t gap_extend, const std::vector<int32_t>& prior,
                              uint32_t prior_width) {
@@ -296,8 +306,7 @@ GapblockInput gapblock_input(const std::vector<std::string>& queries,
 
 }  // namespace
 
-GapblockHits gapblock_threshold(const std::vector<std::string>& queries,
-                                const std::vector<std::string>& refs, Alphabet alphabet,
+GapblockHits gapblock_threshold(const StringSet& queries, const StringSet& refs, Alphabet alphabet,
                                 const SubstitutionMatrix* matrix, int32_t gap_open,
                                 int32_t gap_extend, const std::vector<int32_t>& prior,
                                 uint32_t prior_width, int32_t cutoff) {
@@ -346,8 +355,8 @@ GapblockHits gapblock_threshold(const std::vector<std::string>& queries,
     return sorted;
 }
 
-std::vector<int32_t> gapblock_pairs(const std::vector<std::string>& queries,
-                                    const std::vector<std::string>& refs, Alphabet alphabet,
+std::vector<int32_t> gapblock_pairs(const StringSet& queries, const StringSet& refs,
+                                    Alphabet alphabet,
                                     const SubstitutionMatrix* matrix, int32_t gap_open,
                                     int32_t gap_extend, const std::vector<int32_t>& prior,
                                     uint32_t prior_width, const std::vector<uint32_t>& pair_query,
@@ -368,8 +377,8 @@ std::vector<int32_t> gapblock_pairs(const std::vector<std::string>& queries,
     return out;
 }
 
-std::vector<uint64_t> gapblock_histogram(const std::vector<std::string>& queries,
-                                         const std::vector<std::string>& refs, Alphabet alphabet,
+std::vector<uint64_t> gapblock_histogram(const StringSet& queries, const StringSet& refs,
+                                         Alphabet alphabet,
                                          const SubstitutionMatrix* matrix, int32_t gap_open,
                                          int32_t gap_extend, const std::vector<int32_t>& prior,
                                          uint32_t prior_width, uint32_t bins) {
@@ -399,8 +408,7 @@ std::vector<int32_t> gapblock_matrix(const std::vector<std::string>& queries,
     return out;
 }
 
-void gapblock_matrix_into(const std::vector<std::string>& queries,
-                          const std::vector<std::string>& refs, Alphabet alphabet,
+void gapblock_matrix_into(const StringSet& queries, const StringSet& refs, Alphabet alphabet,
                           const SubstitutionMatrix* matrix, int32_t gap_open, int32_t gap_extend,
                           const std::vector<int32_t>& prior, uint32_t prior_width,
                           int32_t* out) {
@@ -426,36 +434,27 @@ int hamming(std::string_view a, std::string_view b) {
         throw std::invalid_argument("seqtree: hamming distance is only defined for equal-length "
                                     "sequences (got " + std::to_string(a.size()) + " and " +
                                     std::to_string(b.size()) + ")");
-    const Csr x = pack_raw({std::string(a)});
-    const Csr y = pack_raw({std::string(b)});
     int32_t out = 0;
     st_status st{};
-    raise(st_hamming_matrix(x.bytes.data(), x.off.data(), 1, y.bytes.data(), y.off.data(), 1, &out,
-                            &st),
+    raise(st_hamming_matrix(raw(a), offsets(a).data(), 1, raw(b), offsets(b).data(), 1, &out, &st),
           st);
     return out;
 }
 
 int levenshtein(std::string_view a, std::string_view b) {
-    const Csr x = pack_raw({std::string(a)});
-    const Csr y = pack_raw({std::string(b)});
     int32_t out = 0;
     st_status st{};
-    raise(st_levenshtein_matrix(x.bytes.data(), x.off.data(), 1, y.bytes.data(), y.off.data(), 1,
-                                &out, &st),
+    raise(st_levenshtein_matrix(raw(a), offsets(a).data(), 1, raw(b), offsets(b).data(), 1, &out,
+                                &st),
           st);
     return out;
 }
 
-void hamming_matrix_into(const std::vector<std::string>& a, const std::vector<std::string>& b,
-                        int32_t* out) {
+void hamming_matrix_into(const StringSet& a, const StringSet& b, int32_t* out) {
     const size_t N = a.size(), K = b.size();
     if (N == 0 || K == 0) return;
-    const Csr x = pack_raw(a);
-    const Csr y = pack_raw(b);
     st_status st{};
-    raise(st_hamming_matrix(x.bytes.data(), x.off.data(), N, y.bytes.data(), y.off.data(), K, out, &st),
-          st);
+    raise(st_hamming_matrix(raw(a), a.offsets(), N, raw(b), b.offsets(), K, out, &st), st);
 }
 
 std::vector<int32_t> hamming_matrix(const std::vector<std::string>& a,
@@ -465,15 +464,11 @@ std::vector<int32_t> hamming_matrix(const std::vector<std::string>& a,
     return out;
 }
 
-void levenshtein_matrix_into(const std::vector<std::string>& a, const std::vector<std::string>& b,
-                        int32_t* out) {
+void levenshtein_matrix_into(const StringSet& a, const StringSet& b, int32_t* out) {
     const size_t N = a.size(), K = b.size();
     if (N == 0 || K == 0) return;
-    const Csr x = pack_raw(a);
-    const Csr y = pack_raw(b);
     st_status st{};
-    raise(st_levenshtein_matrix(x.bytes.data(), x.off.data(), N, y.bytes.data(), y.off.data(), K, out, &st),
-          st);
+    raise(st_levenshtein_matrix(raw(a), a.offsets(), N, raw(b), b.offsets(), K, out, &st), st);
 }
 
 std::vector<int32_t> levenshtein_matrix(const std::vector<std::string>& a,
@@ -513,20 +508,17 @@ std::string_view Index::ref_seq(uint32_t ref_id) const {
 }
 
 Alignment Index::align(std::string_view query, uint32_t ref_id, const SearchParams& p) const {
-    return align_batch({std::string(query)}, {ref_id}, p)[0];
+    return align_batch(one(query), {ref_id}, p)[0];
 }
 
-std::vector<Alignment> Index::align_batch(const std::vector<std::string>& queries,
+std::vector<Alignment> Index::align_batch(const StringSet& queries,
                                           const std::vector<uint32_t>& ref_ids,
                                           const SearchParams& p) const {
     if (queries.size() != ref_ids.size())
         throw std::invalid_argument("seqtree: queries and ref_ids differ in length");
     // the reference's order: query symbols, then gap costs (src/index.cpp:109-113)
-    const Csr q = pack(queries, [&](char ch) {
-        const uint8_t code = codec_.encode(ch);
-        if (code == Codec::kInvalid)
-            throw std::invalid_argument(std::string("invalid symbol '") + ch + "' in query");
-        return code;
+    const Csr q = pack(queries, codec_, [](char ch) {
+        throw std::invalid_argument(std::string("invalid symbol '") + ch + "' in query");
     });
     if (p.gap_open < 0 || p.gap_extend < 0)
         throw std::invalid_argument("seqtree: gap_open and gap_extend must be >= 0");
@@ -569,7 +561,7 @@ std::vector<Alignment> Index::align_batch(const std::vector<std::string>& querie
                                n_ops.data(), &st),
           st);
     for (size_t t = 0; t < P; ++t) {
-        const std::string& query = queries[t];
+        const std::string_view query = queries[t];
         const std::string_view ref = ref_seq(ref_ids[t]);
         Alignment& al = out[t];
         al.score = score[t];

@@ -23,6 +23,79 @@
 namespace py = pybind11;
 using namespace seqtree;
 
+// The gap prior cube as the matrix functions take it: a list of int as in the reference, or a
+// C-contiguous numpy int32 array (copied in one block; the list form converts per element).
+struct PriorCube {
+    std::vector<int32_t> v;
+};
+
+namespace pybind11::detail {
+
+// A list or tuple of str whose items are all compact ASCII is read straight from the str
+// objects into one StringSet (one copy of the characters, no std::string per item). Anything
+// else -- other sequences, bytes items, non-ASCII text -- takes pybind11's own conversion to
+// std::vector<std::string> first, so every input the reference's binding accepts (and every
+// TypeError it raises) is unchanged.
+template <>
+struct type_caster<StringSet> {
+    PYBIND11_TYPE_CASTER(StringSet, const_name("list[str]"));
+
+    bool load(handle src, bool convert) {
+        PyObject* o = src.ptr();
+        if (!o || PyUnicode_Check(o) || PyBytes_Check(o) || !PySequence_Check(o)) return false;
+        if (PyList_Check(o) || PyTuple_Check(o)) {
+            const Py_ssize_t n = PySequence_Fast_GET_SIZE(o);
+            PyObject** items = PySequence_Fast_ITEMS(o);
+            size_t total = 0;
+            bool ascii = true;
+            for (Py_ssize_t i = 0; i < n && ascii; ++i) {
+                PyObject* it = items[i];
+                ascii = PyUnicode_Check(it) && PyUnicode_IS_COMPACT_ASCII(it);
+                if (ascii) total += size_t(PyUnicode_GET_LENGTH(it));
+            }
+            if (ascii) {
+                value = StringSet();
+                value.reserve(size_t(n), total);
+                for (Py_ssize_t i = 0; i < n; ++i)
+                    value.push_back(std::string_view(
+                        static_cast<const char*>(PyUnicode_DATA(items[i])),
+                        size_t(PyUnicode_GET_LENGTH(items[i]))));
+                return true;
+            }
+        }
+        make_caster<std::vector<std::string>> generic;
+        if (!generic.load(src, convert)) return false;
+        value = StringSet(cast_op<std::vector<std::string>&>(generic));
+        return true;
+    }
+};
+
+template <>
+struct type_caster<PriorCube> {
+    PYBIND11_TYPE_CASTER(PriorCube, const_name("list[int]"));
+
+    bool load(handle src, bool convert) {
+        if (py::isinstance<py::array>(src)) {
+            auto a = py::reinterpret_borrow<py::array>(src);
+            if (a.dtype().is(py::dtype::of<int32_t>()) && a.ndim() == 1 &&
+                (a.flags() & py::array::c_style)) {
+                const auto* p = static_cast<const int32_t*>(a.data());
+                value.v.assign(p, p + a.size());
+                return true;
+            }
+        }
+        make_caster<std::vector<int32_t>> generic;
+        if (!generic.load(src, convert)) return false;
+        value.v = std::move(cast_op<std::vector<int32_t>&>(generic));
+        return true;
+    }
+    static handle cast(const PriorCube& c, return_value_policy p, handle parent) {
+        return make_caster<std::vector<int32_t>>::cast(c.v, p, parent);
+    }
+};
+
+}  // namespace pybind11::detail
+
 namespace {
 
 std::string lower(std::string s) {
@@ -52,44 +125,59 @@ std::string alphabet_symbols(Alphabet a) {
     return s;
 }
 
-// Uninitialised int32 storage for a ScoreMatrix: 2 MiB aligned with transparent huge pages
-// requested, so the library's parallel copy-out faults the memory in as large pages, and
-// never zero-filled, since the matrix functions write every element. Buffers of released
-// matrices are kept for reuse (at most kCacheBuffers, kCacheBytes in all): a fresh page costs
-// the operating system a zero-fill on first touch, which for a large matrix is as much as the
-// copy-out itself. A request takes a cached buffer of at least its size and at most twice it.
+// Uninitialised int32 storage for a ScoreMatrix, never zero-filled (the matrix functions write
+// every element). Up to kPinnedMax bytes it is page-locked memory from st_host_alloc, which
+// the library fills by DMA straight from the device; past that, or where pinning fails, it is
+// 2 MiB-aligned ordinary memory with transparent huge pages requested, filled through the
+// library's pinned staging. Buffers of released matrices are kept for reuse (at most
+// kCacheBuffers, kCacheBytes in all, the oldest released first): pinning or first-touching
+// fresh pages costs the operating system about as much as the copy-out itself. A request
+// takes a cached buffer of at least its size and at most twice it.
 class HostBuffer {
 public:
     HostBuffer() = default;
     explicit HostBuffer(size_t count) {
         const size_t align = size_t(2) << 20;
         bytes_ = (std::max<size_t>(count, 16) * sizeof(int32_t) + align - 1) / align * align;
-        p_ = take(bytes_);
-        if (p_) return;
+        if (take(bytes_, &p_, &pinned_)) return;
+        if (bytes_ <= kPinnedMax) {
+            void* v = nullptr;
+            st_status st{};
+            if (st_host_alloc(bytes_, &v, &st) == ST_OK) {
+                p_ = static_cast<int32_t*>(v);
+                pinned_ = true;
+                return;
+            }
+        }
         void* v = nullptr;
         if (posix_memalign(&v, align, bytes_) != 0) throw std::bad_alloc();
         (void)madvise(v, bytes_, MADV_HUGEPAGE);  // advice only; fine where unsupported
         p_ = static_cast<int32_t*>(v);
     }
-    HostBuffer(HostBuffer&& o) noexcept : p_(o.p_), bytes_(o.bytes_) { o.p_ = nullptr; }
+    HostBuffer(HostBuffer&& o) noexcept : p_(o.p_), bytes_(o.bytes_), pinned_(o.pinned_) {
+        o.p_ = nullptr;
+    }
     HostBuffer& operator=(HostBuffer&& o) noexcept {
         std::swap(p_, o.p_);
         std::swap(bytes_, o.bytes_);
+        std::swap(pinned_, o.pinned_);
         return *this;
     }
     HostBuffer(const HostBuffer&) = delete;
     HostBuffer& operator=(const HostBuffer&) = delete;
     ~HostBuffer() {
-        if (p_) give(p_, bytes_);
+        if (p_) give({p_, bytes_, pinned_});
     }
     int32_t* data() const { return p_; }
 
 private:
+    static constexpr size_t kPinnedMax = size_t(2) << 30;
     static constexpr size_t kCacheBuffers = 4;
     static constexpr size_t kCacheBytes = size_t(8) << 30;
     struct Cached {
         int32_t* p;
         size_t bytes;
+        bool pinned;
     };
     static std::mutex& mu() {
         static std::mutex m;
@@ -99,7 +187,11 @@ private:
         static auto* c = new std::vector<Cached>();  // outlives every ScoreMatrix
         return *c;
     }
-    static int32_t* take(size_t bytes) {
+    static void release(const Cached& e) {
+        if (e.pinned) st_host_free(e.p);
+        else std::free(e.p);
+    }
+    static bool take(size_t bytes, int32_t** p, bool* pinned) {
         std::lock_guard<std::mutex> lk(mu());
         auto& c = cache();
         size_t best = c.size();
@@ -107,27 +199,29 @@ private:
             if (c[i].bytes >= bytes && c[i].bytes <= 2 * bytes &&
                 (best == c.size() || c[i].bytes < c[best].bytes))
                 best = i;
-        if (best == c.size()) return nullptr;
-        int32_t* p = c[best].p;
+        if (best == c.size()) return false;
+        *p = c[best].p;
+        *pinned = c[best].pinned;
         c.erase(c.begin() + best);
-        return p;
+        return true;
     }
-    static void give(int32_t* p, size_t bytes) {
+    static void give(const Cached& e) {
         std::lock_guard<std::mutex> lk(mu());
         auto& c = cache();
-        size_t total = bytes;
-        for (const Cached& e : c) total += e.bytes;
+        size_t total = e.bytes;
+        for (const Cached& x : c) total += x.bytes;
         while (!c.empty() && (c.size() >= kCacheBuffers || total > kCacheBytes)) {
             total -= c.front().bytes;  // the oldest goes first
-            std::free(c.front().p);
+            release(c.front());
             c.erase(c.begin());
         }
-        if (total <= kCacheBytes) c.push_back({p, bytes});
-        else std::free(p);
+        if (total <= kCacheBytes) c.push_back(e);
+        else release(e);
     }
 
     int32_t* p_ = nullptr;
     size_t bytes_ = 0;
+    bool pinned_ = false;
 };
 
 // Dense (rows, cols) int32 block lent to Python through the buffer protocol.
@@ -375,7 +469,7 @@ PYBIND11_MODULE(_core, m) {
 
     m.def(
         "align_pairs",
-        [](const std::vector<std::string>& q, const std::vector<std::string>& r,
+        [](const StringSet& q, const StringSet& r,
            const std::vector<uint32_t>& pq, const std::vector<uint32_t>& pr,
            const SubstitutionMatrix& mat, const std::string& mode, int32_t gap_open,
            int32_t gap_extend, const std::string& alphabet) {
@@ -394,7 +488,7 @@ PYBIND11_MODULE(_core, m) {
 
     m.def(
         "align_score_matrix",
-        [](const std::vector<std::string>& q, const std::vector<std::string>& r,
+        [](const StringSet& q, const StringSet& r,
            const SubstitutionMatrix& mat, const std::string& mode, int32_t gap_open,
            int32_t gap_extend, const std::string& alphabet, int threads) {
             const Alphabet a = parse_alphabet(alphabet);
@@ -410,7 +504,7 @@ PYBIND11_MODULE(_core, m) {
 
     m.def(
         "align_dist_matrix",
-        [](const std::vector<std::string>& q, const std::vector<std::string>& r,
+        [](const StringSet& q, const StringSet& r,
            const SubstitutionMatrix& mat, const std::string& mode, int32_t gap_open,
            int32_t gap_extend, const std::string& alphabet, int threads) {
             const Alphabet a = parse_alphabet(alphabet);
@@ -427,35 +521,35 @@ PYBIND11_MODULE(_core, m) {
 
     m.def(
         "gapblock_matrix",
-        [](const std::vector<std::string>& queries, const std::vector<std::string>& refs,
+        [](const StringSet& queries, const StringSet& refs,
            const std::string& alphabet, const std::optional<SubstitutionMatrix>& matrix,
-           int32_t gap_open, int32_t gap_extend, const std::vector<int32_t>& prior,
+           int32_t gap_open, int32_t gap_extend, const PriorCube& prior,
            uint32_t prior_width, int threads) {
             const Alphabet a = parse_alphabet(alphabet);
             (void)threads;
             return make_matrix(queries.size(), refs.size(), [&](int32_t* out) {
                 gapblock_matrix_into(queries, refs, a, matrix ? &*matrix : nullptr, gap_open,
-                                     gap_extend, prior, prior_width, out);
+                                     gap_extend, prior.v, prior_width, out);
             });
         },
         py::arg("queries"), py::arg("refs"), py::arg("alphabet") = "aa",
         py::arg("matrix") = std::nullopt, py::arg("gap_open") = 1, py::arg("gap_extend") = 1,
-        py::arg("prior") = std::vector<int32_t>{}, py::arg("prior_width") = 0,
+        py::arg("prior") = PriorCube{}, py::arg("prior_width") = 0,
         py::arg("threads") = 0,
         "Exhaustive single-gap-block penalties for every (query, ref) pair, GIL released.");
 
     m.def(
         "gapblock_threshold",
-        [](const std::vector<std::string>& queries, const std::vector<std::string>& refs,
+        [](const StringSet& queries, const StringSet& refs,
            int32_t cutoff, const std::string& alphabet,
            const std::optional<SubstitutionMatrix>& matrix, int32_t gap_open, int32_t gap_extend,
-           const std::vector<int32_t>& prior, uint32_t prior_width) {
+           const PriorCube& prior, uint32_t prior_width) {
             const Alphabet a = parse_alphabet(alphabet);
             GapblockHits h;
             {
                 py::gil_scoped_release release;
                 h = gapblock_threshold(queries, refs, a, matrix ? &*matrix : nullptr, gap_open,
-                                       gap_extend, prior, prior_width, cutoff);
+                                       gap_extend, prior.v, prior_width, cutoff);
             }
             auto arr = [](const auto& v) {
                 using T = typename std::decay_t<decltype(v)>::value_type;
@@ -467,22 +561,22 @@ PYBIND11_MODULE(_core, m) {
         },
         py::arg("queries"), py::arg("refs"), py::arg("cutoff"), py::arg("alphabet") = "aa",
         py::arg("matrix") = std::nullopt, py::arg("gap_open") = 1, py::arg("gap_extend") = 1,
-        py::arg("prior") = std::vector<int32_t>{}, py::arg("prior_width") = 0,
+        py::arg("prior") = PriorCube{}, py::arg("prior_width") = 0,
         "Every (query, ref) pair with gap-block score <= cutoff as three arrays (query index, "
         "ref index, score), sorted by (query, ref). GIL released.");
 
     m.def(
         "gapblock_histogram",
-        [](const std::vector<std::string>& queries, const std::vector<std::string>& refs,
+        [](const StringSet& queries, const StringSet& refs,
            uint32_t bins, const std::string& alphabet,
            const std::optional<SubstitutionMatrix>& matrix, int32_t gap_open, int32_t gap_extend,
-           const std::vector<int32_t>& prior, uint32_t prior_width) {
+           const PriorCube& prior, uint32_t prior_width) {
             const Alphabet a = parse_alphabet(alphabet);
             std::vector<uint64_t> v;
             {
                 py::gil_scoped_release release;
                 v = gapblock_histogram(queries, refs, a, matrix ? &*matrix : nullptr, gap_open,
-                                       gap_extend, prior, prior_width, bins);
+                                       gap_extend, prior.v, prior_width, bins);
             }
             py::array_t<uint64_t> out({py::ssize_t(queries.size()), py::ssize_t(bins) + 2});
             std::copy(v.begin(), v.end(), out.mutable_data());
@@ -490,23 +584,23 @@ PYBIND11_MODULE(_core, m) {
         },
         py::arg("queries"), py::arg("refs"), py::arg("bins"), py::arg("alphabet") = "aa",
         py::arg("matrix") = std::nullopt, py::arg("gap_open") = 1, py::arg("gap_extend") = 1,
-        py::arg("prior") = std::vector<int32_t>{}, py::arg("prior_width") = 0,
+        py::arg("prior") = PriorCube{}, py::arg("prior_width") = 0,
         "Per-query histograms of the exact gap-block scores, uint64 [queries][bins + 2]: bin 0 "
         "scores < 0, bin 1 + s the score s < bins, the last bin scores >= bins. GIL released.");
 
     m.def(
         "gapblock_pairs",
-        [](const std::vector<std::string>& queries, const std::vector<std::string>& refs,
+        [](const StringSet& queries, const StringSet& refs,
            const std::vector<uint32_t>& pair_query, const std::vector<uint32_t>& pair_ref,
            const std::string& alphabet, const std::optional<SubstitutionMatrix>& matrix,
-           int32_t gap_open, int32_t gap_extend, const std::vector<int32_t>& prior,
+           int32_t gap_open, int32_t gap_extend, const PriorCube& prior,
            uint32_t prior_width) {
             const Alphabet a = parse_alphabet(alphabet);
             std::vector<int32_t> v;
             {
                 py::gil_scoped_release release;
                 v = gapblock_pairs(queries, refs, a, matrix ? &*matrix : nullptr, gap_open,
-                                   gap_extend, prior, prior_width, pair_query, pair_ref);
+                                   gap_extend, prior.v, prior_width, pair_query, pair_ref);
             }
             py::array_t<int32_t> out(v.size());
             std::copy(v.begin(), v.end(), out.mutable_data());
@@ -514,7 +608,7 @@ PYBIND11_MODULE(_core, m) {
         },
         py::arg("queries"), py::arg("refs"), py::arg("pair_query"), py::arg("pair_ref"),
         py::arg("alphabet") = "aa", py::arg("matrix") = std::nullopt, py::arg("gap_open") = 1,
-        py::arg("gap_extend") = 1, py::arg("prior") = std::vector<int32_t>{},
+        py::arg("gap_extend") = 1, py::arg("prior") = PriorCube{},
         py::arg("prior_width") = 0,
         "Gap-block scores of explicit (query index, ref index) pairs, as an int32 array.");
 
@@ -536,7 +630,7 @@ PYBIND11_MODULE(_core, m) {
 
     m.def(
         "hamming_matrix",
-        [](const std::vector<std::string>& a, const std::vector<std::string>& b, int threads) {
+        [](const StringSet& a, const StringSet& b, int threads) {
             (void)threads;
             return make_matrix(a.size(), b.size(),
                                [&](int32_t* out) { hamming_matrix_into(a, b, out); });
@@ -545,7 +639,7 @@ PYBIND11_MODULE(_core, m) {
         "Dense (len(a), len(b)) int32 Hamming matrix, GIL released.");
     m.def(
         "levenshtein_matrix",
-        [](const std::vector<std::string>& a, const std::vector<std::string>& b, int threads) {
+        [](const StringSet& a, const StringSet& b, int threads) {
             (void)threads;
             return make_matrix(a.size(), b.size(),
                                [&](int32_t* out) { levenshtein_matrix_into(a, b, out); });
@@ -636,7 +730,7 @@ PYBIND11_MODULE(_core, m) {
         .def(
             "align_batch",
             [](const Index& idx, const std::vector<uint32_t>& ref_ids,
-               const std::vector<std::string>& queries, const PyParams& pp) {
+               const StringSet& queries, const PyParams& pp) {
                 const auto mat = params_matrix(pp, idx.alphabet());
                 const SearchParams cp = to_cpp(pp, mat ? &*mat : nullptr);
                 py::gil_scoped_release release;
@@ -644,6 +738,30 @@ PYBIND11_MODULE(_core, m) {
             },
             py::arg("ref_ids"), py::arg("queries"), py::arg("params"),
             "align() of every (ref_ids[t], queries[t]) in one device batch.");
+
+    m.def(
+        "_longest",
+        [](const py::args& sets) {
+            Py_ssize_t best = 0;
+            auto take = [&](PyObject* it) {
+                const Py_ssize_t n = PyUnicode_Check(it) ? PyUnicode_GET_LENGTH(it)
+                                                         : PyObject_Length(it);
+                if (n < 0) throw py::error_already_set();
+                best = std::max(best, n);
+            };
+            for (const py::handle set : sets) {
+                PyObject* o = set.ptr();
+                if (PyList_Check(o) || PyTuple_Check(o)) {
+                    PyObject** items = PySequence_Fast_ITEMS(o);
+                    for (Py_ssize_t i = 0, n = PySequence_Fast_GET_SIZE(o); i < n; ++i)
+                        take(items[i]);
+                } else {
+                    for (const py::handle it : set) take(it.ptr());
+                }
+            }
+            return best;
+        },
+        "Longest len() over the items of the given sequences (0 when all are empty).");
 
     m.def("version", [] { return std::string(st_version()); }, "Native library version.");
     m.def("device_count", [] { return st_device_count(); }, "Visible HIP devices.");

@@ -66,6 +66,44 @@ private:
     std::vector<int32_t> sim_;
 };
 
+// A set of strings stored back to back: characters plus n + 1 offsets (no counterpart in the
+// reference, whose functions take std::vector<std::string>). The matrix functions encode from
+// this form in one pass; the Python binding fills it straight from a list of str, and a
+// std::vector<std::string> converts implicitly.
+class StringSet {
+public:
+    StringSet() = default;
+    StringSet(const std::vector<std::string>& v) {  // NOLINT: implicit on purpose
+        size_t total = 0;
+        for (const auto& s : v) total += s.size();
+        reserve(v.size(), total);
+        for (const auto& s : v) push_back(s);
+    }
+    void reserve(size_t n, size_t chars) {
+        chars_.reserve(chars);
+        off_.reserve(n + 1);
+    }
+    void push_back(std::string_view s) {
+        chars_.append(s.data(), s.size());
+        off_.push_back(chars_.size());
+        if (s.size() > max_len_) max_len_ = uint32_t(s.size());
+    }
+    size_t size() const { return off_.size() - 1; }
+    bool empty() const { return off_.size() == 1; }
+    std::string_view operator[](size_t i) const {
+        return std::string_view(chars_.data() + off_[i], off_[i + 1] - off_[i]);
+    }
+    const char* chars() const { return chars_.data(); }
+    const uint64_t* offsets() const { return off_.data(); }
+    size_t total() const { return chars_.size(); }
+    uint32_t max_len() const { return max_len_; }
+
+private:
+    std::string chars_;
+    std::vector<uint64_t> off_{0};
+    uint32_t max_len_ = 0;
+};
+
 enum class AlignMode { Global, Local };
 
 // Optimal Gotoh score of one pair (int64 DP semantics, narrowed to int32 or
@@ -87,8 +125,8 @@ Alignment align_pair(std::string_view query, std::string_view ref, const Substit
 
 // align_pair for every (queries[pair_query[p]], refs[pair_ref[p]]), in one device batch (no
 // reference counterpart: the traceback step for candidate pairs).
-std::vector<Alignment> align_pairs(const std::vector<std::string>& queries,
-                                   const std::vector<std::string>& refs,
+std::vector<Alignment> align_pairs(const StringSet& queries,
+                                   const StringSet& refs,
                                    const std::vector<uint32_t>& pair_query,
                                    const std::vector<uint32_t>& pair_ref,
                                    const SubstitutionMatrix&, Alphabet, AlignMode,
@@ -121,8 +159,8 @@ struct GapblockHits {
     std::vector<uint32_t> ref;
     std::vector<int32_t> score;
 };
-GapblockHits gapblock_threshold(const std::vector<std::string>& queries,
-                                const std::vector<std::string>& refs, Alphabet,
+GapblockHits gapblock_threshold(const StringSet& queries,
+                                const StringSet& refs, Alphabet,
                                 const SubstitutionMatrix* matrix, int32_t gap_open,
                                 int32_t gap_extend, const std::vector<int32_t>& prior,
                                 uint32_t prior_width, int32_t cutoff);
@@ -130,15 +168,15 @@ GapblockHits gapblock_threshold(const std::vector<std::string>& queries,
 // Per-query histograms of the exact gap-block scores against every reference, row-major
 // [queries.size()][bins + 2]: bin 0 counts scores below zero, bin 1 + s the score s in
 // [0, bins), bin bins + 1 scores >= bins (the counts behind control-calibrated E-values).
-std::vector<uint64_t> gapblock_histogram(const std::vector<std::string>& queries,
-                                         const std::vector<std::string>& refs, Alphabet,
+std::vector<uint64_t> gapblock_histogram(const StringSet& queries,
+                                         const StringSet& refs, Alphabet,
                                          const SubstitutionMatrix* matrix, int32_t gap_open,
                                          int32_t gap_extend, const std::vector<int32_t>& prior,
                                          uint32_t prior_width, uint32_t bins);
 
 // Gap-block scores of explicit (query index, ref index) pairs.
-std::vector<int32_t> gapblock_pairs(const std::vector<std::string>& queries,
-                                    const std::vector<std::string>& refs, Alphabet,
+std::vector<int32_t> gapblock_pairs(const StringSet& queries,
+                                    const StringSet& refs, Alphabet,
                                     const SubstitutionMatrix* matrix, int32_t gap_open,
                                     int32_t gap_extend, const std::vector<int32_t>& prior,
                                     uint32_t prior_width, const std::vector<uint32_t>& pair_query,
@@ -157,22 +195,22 @@ std::vector<int32_t> levenshtein_matrix(const std::vector<std::string>& a,
 // queries.size() * refs.size() int32 instead of returning a vector (no counterpart in the
 // reference): the same checks, order and exceptions. The pybind11 ScoreMatrix allocates its
 // buffer and calls these, which spares zero-filling a vector that the call overwrites.
-void gapblock_matrix_into(const std::vector<std::string>& queries,
-                          const std::vector<std::string>& refs, Alphabet,
+void gapblock_matrix_into(const StringSet& queries,
+                          const StringSet& refs, Alphabet,
                           const SubstitutionMatrix* matrix, int32_t gap_open, int32_t gap_extend,
                           const std::vector<int32_t>& prior, uint32_t prior_width, int32_t* out);
-void align_score_matrix_into(const std::vector<std::string>& queries,
-                             const std::vector<std::string>& refs, const SubstitutionMatrix&,
+void align_score_matrix_into(const StringSet& queries,
+                             const StringSet& refs, const SubstitutionMatrix&,
                              Alphabet, AlignMode, int32_t gap_open, int32_t gap_extend,
                              int32_t* out);
-void align_dist_matrix_into(const std::vector<std::string>& queries,
-                            const std::vector<std::string>& refs, const SubstitutionMatrix&,
+void align_dist_matrix_into(const StringSet& queries,
+                            const StringSet& refs, const SubstitutionMatrix&,
                             Alphabet, AlignMode, int32_t gap_open, int32_t gap_extend,
                             int32_t* out);
-void hamming_matrix_into(const std::vector<std::string>& a, const std::vector<std::string>& b,
+void hamming_matrix_into(const StringSet& a, const StringSet& b,
                          int32_t* out);
-void levenshtein_matrix_into(const std::vector<std::string>& a,
-                             const std::vector<std::string>& b, int32_t* out);
+void levenshtein_matrix_into(const StringSet& a,
+                             const StringSet& b, int32_t* out);
 
 // Search scope and budget (reference include/seqtree/types.hpp:23-54), kept field for field
 // so code that fills one compiles unchanged. Index::align reads the scoring fields (matrix,
@@ -215,7 +253,7 @@ public:
     // outside the alphabet ("invalid symbol 'c' in query"), then for negative gap costs, and
     // std::out_of_range for a ref_id past the set.
     Alignment align(std::string_view query, uint32_t ref_id, const SearchParams& p) const;
-    std::vector<Alignment> align_batch(const std::vector<std::string>& queries,
+    std::vector<Alignment> align_batch(const StringSet& queries,
                                        const std::vector<uint32_t>& ref_ids,
                                        const SearchParams& p) const;
 

@@ -121,3 +121,51 @@ def test_concurrent_host_calls(gpu, oracle_lib):
     for t in threads:
         t.join()
     assert not errors, errors
+
+
+def test_output_in_library_pinned_memory_equals_the_staged_path(gpu):
+    """st_host_alloc memory takes the matrix by direct DMA (no staging): an output placed at an
+    interior offset of such a buffer equals the same call into ordinary memory, the bytes
+    around it stay untouched, and st_host_free releases it (NULL and foreign pointers are
+    ignored)."""
+    st = gpu
+    from seqtree_amd import _device
+    from seqtree_amd._device import Status
+
+    L = _device.lib()
+    vp, u64, u32, i32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int32
+    L.st_host_alloc.restype = i32
+    L.st_host_alloc.argtypes = [u64, ctypes.POINTER(vp), ctypes.POINTER(Status)]
+    L.st_host_free.restype = None
+    L.st_host_free.argtypes = [vp]
+    L.st_hamming_matrix.restype = i32
+    L.st_hamming_matrix.argtypes = [vp, vp, u64, vp, vp, u64, vp, ctypes.POINTER(Status)]
+    rng = random.Random(12)
+    a = [rand_seq(rng, 12, 12, "ACGT") for _ in range(3000)]
+    b = [rand_seq(rng, 12, 12, "ACGT") for _ in range(12000)]  # 144 MB: two row blocks
+    raw = {c: ord(c) for c in "ACGT"}
+    (ac, ao), (bc, bo) = _csr(a, raw), _csr(b, raw)
+    n = len(a) * len(b)
+    stt = Status()
+    ptr = vp()
+    assert L.st_host_alloc(4 * (n + 64), ctypes.byref(ptr), ctypes.byref(stt)) == 0, stt.message
+    try:
+        buf = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_int32)),
+                                    shape=(n + 64,))
+        buf[:] = -7
+        out = buf[32:32 + n]
+        assert L.st_hamming_matrix(ac.ctypes.data, ao.ctypes.data, len(a), bc.ctypes.data,
+                                   bo.ctypes.data, len(b), out.ctypes.data,
+                                   ctypes.byref(stt)) == 0, stt.message
+        plain = np.empty(n, np.int32)
+        assert L.st_hamming_matrix(ac.ctypes.data, ao.ctypes.data, len(a), bc.ctypes.data,
+                                   bo.ctypes.data, len(b), plain.ctypes.data,
+                                   ctypes.byref(stt)) == 0, stt.message
+        assert np.array_equal(out, plain)
+        assert np.all(buf[:32] == -7) and np.all(buf[32 + n:] == -7)
+        want = np.asarray(st.distance.hamming_matrix(a[:5], b)).ravel()
+        assert np.array_equal(out[:want.size], want)
+    finally:
+        L.st_host_free(ptr)
+    L.st_host_free(None)
+    L.st_host_free(ctypes.c_void_p(plain.ctypes.data))  # not the library's: ignored
