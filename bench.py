@@ -760,7 +760,7 @@ PF_REFS, PF_FAMILIES = 1_000_000, (250_000, 4)
 PF_QUERIES = 50_000
 PF_CUTOFF = 48
 PF_EPOCHS = 8
-PF_QUANTILE = 0.999
+PF_QUANTILE = 0.997  # seqtree_amd.prefilter.DEFAULT_QUANTILE
 PF_AUDIT = 0.01  # queries searched exhaustively per search to calibrate the threshold
 
 
@@ -789,14 +789,15 @@ def pf_search_set(torch, lib, pf, model, gp, tau, sets, args):
     searcher = pf.Searcher(model, gp, PF_LEN[1])
     for _ in range(args.warmup):
         searcher.index(R)
-        res = searcher.query(Q, args.pf_cutoff, tau, audit=PF_AUDIT, quantile=PF_QUANTILE)
+        res = searcher.query(Q, args.pf_cutoff, tau, audit=PF_AUDIT,
+                             quantile=args.pf_quantile)
     torch.cuda.synchronize()
     build_times, search_times = [], []
     for step in range(args.steps):
         t0 = time.perf_counter()
         searcher.index(R)
         t1 = time.perf_counter()
-        res = searcher.query(Q, args.pf_cutoff, tau, audit=PF_AUDIT, quantile=PF_QUANTILE,
+        res = searcher.query(Q, args.pf_cutoff, tau, audit=PF_AUDIT, quantile=args.pf_quantile,
                              seed=step)
         search_times.append(time.perf_counter() - t1)
         build_times.append(t1 - t0)
@@ -827,9 +828,10 @@ def run_prefilter(torch, dev, lib, stream, args) -> dict:
                       seed=seed, max_len=PF_LEN[1],
                       log=lambda m: print(m, file=sys.stderr, flush=True))
     train_s = time.perf_counter() - t0
-    tau, n_cal = pf.calibrate_on_set(model, hc, ho, gp, args.pf_cutoff, PF_QUANTILE, PF_LEN[1])
+    tau, n_cal = pf.calibrate_on_set(model, hc, ho, gp, args.pf_cutoff, args.pf_quantile,
+                                      PF_LEN[1])
     calib = {"calibration_sequences": len(ho) - 1, "calibration_close_pairs": int(n_cal),
-             "quantile": PF_QUANTILE, "tau_calibrated": round(tau, 3)}
+             "quantile": args.pf_quantile, "tau_calibrated": round(tau, 3)}
 
     per_set = {}
     for name in ("iid", "planted"):
@@ -859,7 +861,8 @@ def run_prefilter(torch, dev, lib, stream, args) -> dict:
     lib.timing()
     for step in range(args.steps):
         searcher.index(R)
-        searcher.query(Q, args.pf_cutoff, tau, audit=PF_AUDIT, quantile=PF_QUANTILE, seed=step)
+        searcher.query(Q, args.pf_cutoff, tau, audit=PF_AUDIT, quantile=args.pf_quantile,
+                       seed=step)
     torch.cuda.synchronize()
     lib.lib.st_timing_enable(0)
     kernels = lib.timing()
@@ -1244,6 +1247,9 @@ def main() -> int:
                     help="skip the end-to-end (Python call, host output) timings")
     ap.add_argument("--pf-epochs", type=int, default=PF_EPOCHS)
     ap.add_argument("--pf-cutoff", type=int, default=PF_CUTOFF)
+    ap.add_argument("--pf-quantile", type=float, default=PF_QUANTILE,
+                    help="quantile of the calibration hits' model distances that sets the "
+                         "screening threshold")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scale for rocprofv3 --pmc "
                          "FETCH_SIZE / WRITE_SIZE readings)")
@@ -1266,6 +1272,8 @@ def main() -> int:
     stream = torch.cuda.Stream(device=dev)
     if args.workload != "all":
         line = RUNNERS[args.workload](torch, dev, lib, stream, args)
+        if args.workload in ARITHMETIC:
+            line["dtype"], line["arithmetic"] = ARITHMETIC[args.workload]
         finish_roofline(line, lib)
         print(json.dumps(line))
         return 0

@@ -69,10 +69,14 @@ class DeviceArray:
               "st_memcpy")
         return d
 
-    def numpy(self) -> np.ndarray:
-        out = np.empty(self.shape, self.dtype)
+    def numpy(self, count: int | None = None) -> np.ndarray:
+        """The buffer on the host; with `count`, only its first `count` elements (flat)."""
+        if count is None:
+            out = np.empty(self.shape, self.dtype)
+        else:
+            out = np.empty(min(int(count), self.nbytes // self.dtype.itemsize), self.dtype)
         st = Status()
-        check(lib().st_memcpy(out.ctypes.data, self.ptr, self.nbytes, 2, ctypes.byref(st)), st,
+        check(lib().st_memcpy(out.ctypes.data, self.ptr, out.nbytes, 2, ctypes.byref(st)), st,
               "st_memcpy")
         return out
 

@@ -38,6 +38,14 @@ from .synthetic import AA_ORDER
 EMB = 128
 # Distances are learned in units of this many gap-block score points.
 SCORE_UNIT = 100.0
+# Share of the calibration pairs' exact hits (score <= cutoff) whose model distance the
+# screening threshold keeps: 0.997 leaves margin over a 99.5 % recall target, since the
+# threshold is also raised past the cutoff and past the held-out calibration's quantile.
+DEFAULT_QUANTILE = 0.997
+# The audit of a search (Searcher.query) wants at least this many exact hits behind its
+# quantile, and searches at most this share of the queries exhaustively to get them.
+AUDIT_MIN_HITS = 1000
+AUDIT_MAX_SHARE = 0.1
 
 
 def _bind():
@@ -148,14 +156,22 @@ def exact_pairs(q: DeviceSet, r: DeviceSet, pa: DeviceArray, pb: DeviceArray, n_
 
 
 def exact_threshold(q: DeviceSet, r: DeviceSet, cutoff: int, gp: GapblockParams,
-                    capacity: int = 1 << 22):
+                    capacity: int = 1 << 22, buffers: dict | None = None):
     """Exhaustive exact search: every pair with score <= cutoff, as host arrays
-    (query, ref, score). Retries with more room when the first capacity is short."""
+    (query, ref, score). Retries with more room when the first capacity is short. `buffers`
+    (a dict, initially empty) keeps the device buffers for the next call."""
     L = _bind()
-    count = DeviceArray(1, np.uint64)  # the device entry point counts in device memory
+    bufs = buffers if buffers is not None else {}
+    if "count" not in bufs:
+        bufs["count"] = DeviceArray(1, np.uint64)  # the entry point counts in device memory
+    count = bufs["count"]
     while True:
-        hq, hr = DeviceArray(capacity, np.uint32), DeviceArray(capacity, np.uint32)
-        hs = DeviceArray(capacity, np.int32)
+        if bufs.get("cap", 0) < capacity:
+            bufs["hits"] = (DeviceArray(capacity, np.uint32), DeviceArray(capacity, np.uint32),
+                            DeviceArray(capacity, np.int32))
+            bufs["cap"] = capacity
+        capacity = bufs["cap"]
+        hq, hr, hs = bufs["hits"]
         st = Status()
         rc = L.st_gapblock_threshold_device(
             q.codes.ptr, q.offsets.ptr, q.n, q.max_len, r.codes.ptr, r.offsets.ptr, r.n,
@@ -165,7 +181,7 @@ def exact_threshold(q: DeviceSet, r: DeviceSet, cutoff: int, gp: GapblockParams,
         check(rc, st, "st_gapblock_threshold_device")
         k = int(count.numpy()[0])
         if k <= capacity:
-            return hq.numpy()[:k], hr.numpy()[:k], hs.numpy()[:k]
+            return hq.numpy(k), hr.numpy(k), hs.numpy(k)
         capacity = k
 
 
@@ -283,7 +299,7 @@ def select_le(values: DeviceArray, n: int, cutoff: int) -> np.ndarray:
         check(L.st_select_le(values.ptr, n, cutoff, cap, idx.ptr, ctypes.byref(count), None,
                              ctypes.byref(st)), st, "st_select_le")
         if count.value <= cap:
-            return np.sort(idx.numpy()[:count.value])
+            return np.sort(idx.numpy(count.value))
         cap = int(count.value)
 
 
@@ -425,7 +441,7 @@ def _bf16_to_f32(u16: np.ndarray) -> np.ndarray:
 
 
 def calibrate_tau(model_dist: np.ndarray, exact: np.ndarray, cutoff: int,
-                  quantile: float = 0.999) -> float:
+                  quantile: float = DEFAULT_QUANTILE) -> float:
     """Screening threshold (score units) that keeps `quantile` of the held-out pairs whose
     exact score is <= cutoff: the cutoff itself or the quantile of their model distance,
     whichever is larger."""
@@ -478,6 +494,7 @@ class Searcher:
         self._hit_cap = hit_capacity
         self.refs = None
         self._count = ctypes.c_uint64()
+        self._audit_buffers: dict = {}
 
     def index(self, r: DeviceSet) -> float:
         t0 = time.perf_counter()
@@ -487,12 +504,14 @@ class Searcher:
         return time.perf_counter() - t0
 
     def query(self, q: DeviceSet, cutoff: int, tau_score: float, audit: float = 0.0,
-              quantile: float = 0.999, seed: int = 0) -> SearchResult:
+              quantile: float = DEFAULT_QUANTILE, seed: int = 0) -> SearchResult:
         """With audit > 0, a random sample of that fraction of the queries (q needs its host
         copies) is first searched exhaustively with the exact kernel; the screening threshold
         becomes the larger of tau_score and the `quantile` of the model distances of the
         sample's exact hits, so it is calibrated on this query and reference distribution
-        rather than only on the training pairs'."""
+        rather than only on the training pairs'. A quantile near 1 is only as good as the
+        number of hits behind it, so a sample with fewer than AUDIT_MIN_HITS hits is grown
+        once, by the hit rate it saw, up to AUDIT_MAX_SHARE of the queries."""
         L = _bind()
         t = {}
         t0 = time.perf_counter()
@@ -504,9 +523,22 @@ class Searcher:
         if audit > 0:
             t0 = time.perf_counter()
             n_audit = max(1, min(q.n, int(round(audit * q.n))))
-            idx = np.sort(np.random.default_rng(seed).choice(q.n, n_audit, replace=False))
-            aq, ar, _ = exact_threshold(q.subset(idx), self.refs, cutoff, self.gp)
-            d = pair_distances(qe, qn, self.r_emb, self.r_norm, idx[aq], ar)
+            order = np.random.default_rng(seed).permutation(q.n)
+            idx = np.sort(order[:n_audit])
+            aq, ar, _ = exact_threshold(q.subset(idx), self.refs, cutoff, self.gp,
+                                        capacity=1 << 18, buffers=self._audit_buffers)
+            pq, pr = idx[aq], ar
+            cap = max(n_audit, int(AUDIT_MAX_SHARE * q.n))
+            if len(aq) < AUDIT_MIN_HITS and n_audit < cap:
+                # more queries of the same random order, sized by the hit rate seen so far
+                want = n_audit * AUDIT_MIN_HITS / max(len(aq), 1)
+                more = np.sort(order[n_audit:min(cap, int(np.ceil(want)))])
+                if len(more):
+                    bq, br, _ = exact_threshold(q.subset(more), self.refs, cutoff, self.gp,
+                                                capacity=1 << 18, buffers=self._audit_buffers)
+                    pq, pr = np.concatenate([pq, more[bq]]), np.concatenate([pr, br])
+                    n_audit += len(more)
+            d = pair_distances(qe, qn, self.r_emb, self.r_norm, pq, pr)
             audit_hits = len(d)
             if audit_hits:
                 # a little headroom for the screen's different FP32 summation order
@@ -537,13 +569,13 @@ class Searcher:
                 break
             self._hit_cap = k
         t["rescore"] = time.perf_counter() - t0
-        return SearchResult(hq.numpy()[:k], hr.numpy()[:k], hs.numpy()[:k], nc, t, tau,
-                            n_audit, audit_hits, within)
+        return SearchResult(hq.numpy(k), hr.numpy(k), hs.numpy(k), nc, t, tau, n_audit,
+                            audit_hits, within)
 
 
 def search(model: Prefilter, q: DeviceSet, r: DeviceSet, cutoff: int, tau_score: float,
            gp: GapblockParams, max_len: int, audit: float = 0.0,
-           quantile: float = 0.999) -> SearchResult:
+           quantile: float = DEFAULT_QUANTILE) -> SearchResult:
     """One-shot search: index the references, then query (see Searcher)."""
     s = Searcher(model, gp, max_len)
     t_index = s.index(r)

@@ -268,8 +268,9 @@ C5_LEN, C5_CUTOFF = (10, 20), 48
 @pytest.fixture(scope="module")
 def c5_model(gpu):
     from seqtree_amd import synthetic as syn
-    from seqtree_amd.prefilter import (GapblockParams, Prefilter, TrainingData, calibrate_on_set,
-                                       label_pairs, mine_pairs, train)
+    from seqtree_amd.prefilter import (DEFAULT_QUANTILE, GapblockParams, Prefilter,
+                                       TrainingData, calibrate_on_set, label_pairs, mine_pairs,
+                                       train)
 
     rng = np.random.default_rng(2025)
     # 180,000 training sequences and 20,000 held out for the calibration
@@ -283,7 +284,7 @@ def c5_model(gpu):
     losses = train(model, TrainingData(tc, to, pa, pb, score), epochs=8, max_len=C5_LEN[1],
                    seed=2025)
     assert losses[-1] < 0.5 * losses[0]
-    tau, n_cal = calibrate_on_set(model, hc, ho, gp, C5_CUTOFF, 0.999, C5_LEN[1])
+    tau, n_cal = calibrate_on_set(model, hc, ho, gp, C5_CUTOFF, DEFAULT_QUANTILE, C5_LEN[1])
     assert n_cal > 1000
     yield model, gp, tau
     model.close()
@@ -327,7 +328,7 @@ def test_scaled_c5_recall(c5_model, oracle_lib, kind):
         want = {(int(j), int(dense[k, j])) for j in np.flatnonzero(dense[k] <= C5_CUTOFF)}
         got = {(int(j), int(s)) for j, s in zip(er[eq == i], es[eq == i])}
         assert got == want, (kind, i)
-    res = search(model, Q, R, C5_CUTOFF, tau, gp, C5_LEN[1], audit=0.02, quantile=0.999)
+    res = search(model, Q, R, C5_CUTOFF, tau, gp, C5_LEN[1], audit=0.02)  # default quantile
     exact = dict(zip(((eq.astype(np.int64) << 32) | er).tolist(), es.tolist()))
     keys = ((res.query.astype(np.int64) << 32) | res.ref).tolist()
     assert len(set(keys)) == len(keys)
