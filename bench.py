@@ -759,7 +759,7 @@ PF_PAIRS, PF_NEAR = 2_000_000, 0.4
 PF_REFS, PF_FAMILIES = 1_000_000, (250_000, 4)
 PF_QUERIES = 50_000
 PF_CUTOFF = 48
-PF_EPOCHS = 8
+PF_EPOCHS = 16
 PF_QUANTILE = 0.997  # seqtree_amd.prefilter.DEFAULT_QUANTILE
 PF_AUDIT = 0.01  # queries searched exhaustively per search to calibrate the threshold
 

@@ -258,7 +258,7 @@ def test_small_search_reports_only_exact_hits(gpu):
 
 # ------------------------------------------------------------------- C5 at a scaled size
 # The C5 recipe (200,000 synthetic CDR3s -- independent draws and families of 4 -- of which
-# 180,000 train on 2M pairs mined and labelled by the exact kernel for 8 epochs and 20,000
+# 180,000 train on 2M pairs mined and labelled by the exact kernel for 16 epochs and 20,000
 # calibrate the margin) and its cutoff, searched with
 # 5,000 queries x 100,000 references of two kinds: independent CDR3s (the generator alone) and
 # planted families (each query a fresh mutant of one reference family's root).
@@ -281,7 +281,7 @@ def c5_model(gpu):
     assert n_close > 10_000
     score = label_pairs(tc, to, pa, pb, gp)
     model = Prefilter(seed=2025)
-    losses = train(model, TrainingData(tc, to, pa, pb, score), epochs=8, max_len=C5_LEN[1],
+    losses = train(model, TrainingData(tc, to, pa, pb, score), epochs=16, max_len=C5_LEN[1],
                    seed=2025)
     assert losses[-1] < 0.5 * losses[0]
     tau, n_cal = calibrate_on_set(model, hc, ho, gp, C5_CUTOFF, DEFAULT_QUANTILE, C5_LEN[1])
