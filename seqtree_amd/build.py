@@ -28,7 +28,7 @@ HOST = CSRC / "host"
 BUILD = HERE / "_build"
 
 HIP_SOURCES = ["prep.hip", "gapblock.hip", "batch.hip", "gotoh.hip", "traceback.hip", "distance.hip",
-               "probe.hip", "prefilter.hip", "capi.hip"]
+               "single.hip", "probe.hip", "prefilter.hip", "capi.hip"]
 HOST_SOURCES = ["codec.cpp", "substitution_matrix.cpp", "matrix_data.cpp", "api.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -1625,6 +1625,122 @@ uint64_t gapblock_scratch_bound(uint64_t N, uint64_t K, uint32_t r_max, uint32_t
 
 }  // namespace
 
+// ------------------------------------------------------------------------------------------
+// single pairs: the scalar entry points
+// ------------------------------------------------------------------------------------------
+// A host-memory call with one pair (align_score, align_pair, Index::align, hamming,
+// levenshtein) runs one single-pair kernel (single.hip) on a stream of its own and reads its
+// results from pinned host memory: one launch and one synchronisation instead of the batch
+// path's uploads, packing launches and downloads. The contexts (stream, pinned result) are
+// kept for reuse, one per concurrent caller.
+struct SingleContext {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    SingleOut* out = nullptr;
+};
+std::mutex g_single_mu;
+std::vector<SingleContext*> g_single_free;
+
+class SingleCall {
+public:
+    SingleCall() {
+        int dev = 0;
+        check(hipGetDevice(&dev), "hipGetDevice");
+        {
+            std::lock_guard<std::mutex> lk(g_single_mu);
+            for (size_t i = 0; i < g_single_free.size(); ++i)
+                if (g_single_free[i]->device == dev) {
+                    c_ = g_single_free[i];
+                    g_single_free.erase(g_single_free.begin() + i);
+                    break;
+                }
+        }
+        if (c_) return;
+        auto* c = new SingleContext();
+        c->device = dev;
+        check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream");
+        check(hipHostMalloc(reinterpret_cast<void**>(&c->out), sizeof(SingleOut),
+                            hipHostMallocDefault),
+              "hipHostMalloc");
+        c_ = c;
+    }
+    ~SingleCall() {
+        if (!done_) return;  // a failed call leaves its context behind
+        std::lock_guard<std::mutex> lk(g_single_mu);
+        g_single_free.push_back(c_);
+    }
+    SingleCall(const SingleCall&) = delete;
+    SingleCall& operator=(const SingleCall&) = delete;
+
+    hipStream_t stream() const { return c_->stream; }
+    // Launches the kernel and waits for it; the results are valid until the call ends.
+    const SingleOut& run(SingleKind kind, SingleArgs& a, const char* label) {
+        a.out = c_->out;
+        {
+            LaunchTimer t(label, c_->stream);
+            launch_single(kind, a, c_->stream);
+        }
+        launched(label);
+        sync(c_->stream);
+        done_ = true;
+        return *c_->out;
+    }
+
+private:
+    SingleContext* c_ = nullptr;
+    bool done_ = false;
+};
+
+// The pair as single-kernel arguments when it fits them: 1 <= m <= max_q, 1 <= n <=
+// kSingleMaxR (0 allowed for both when `empty_ok`), and every code below `codes_below` when
+// that is nonzero (the kernels index their tables by code).
+bool single_args(SingleArgs& a, const uint8_t* q, uint64_t m, const uint8_t* r, uint64_t n,
+                 uint32_t max_q, uint32_t codes_below, bool empty_ok = false) {
+    if (m > max_q || n > kSingleMaxR || (!empty_ok && (m == 0 || n == 0))) return false;
+    if (codes_below) {
+        for (uint64_t t = 0; t < m; ++t)
+            if (q[t] >= codes_below) return false;
+        for (uint64_t t = 0; t < n; ++t)
+            if (r[t] >= codes_below) return false;
+    }
+    a = SingleArgs{};
+    std::memcpy(a.q, q, m);
+    std::memcpy(a.r, r, n);
+    a.m = uint32_t(m);
+    a.n = uint32_t(n);
+    return true;
+}
+
+// Gotoh on int32 cells needs (m + n + 2) * max(|s|, go, ge) < 2^28 (the traceback kernels'
+// bound, sentinel -2^29).
+bool single_gotoh_range(const int32_t* table, uint32_t A, uint64_t m, uint64_t n, int32_t go,
+                        int32_t ge) {
+    if (go < 0 || ge < 0) return false;
+    int64_t mx = std::max<int64_t>(go, ge);
+    for (uint32_t e = 0; e < A * A; ++e) mx = std::max(mx, iabs64(table[e]));
+    return (int64_t(m) + int64_t(n) + 2) * mx < (int64_t(1) << 28);
+}
+
+// One Gotoh pair (the query's codes and the reference's, similarity table sim_h) through the
+// single-pair kernel: the score, or with `trace` the traceback too. False when it does not fit.
+bool single_gotoh(const uint8_t* q, uint64_t m, const uint8_t* r, uint64_t n,
+                  const int32_t* sim_h, uint32_t A, bool local, int32_t go, int32_t ge, bool trace,
+                  const char* label, SingleOut* res) {
+    SingleArgs a;
+    if (!single_args(a, q, m, r, n, kSingleMaxQ, A) || !single_gotoh_range(sim_h, A, m, n, go, ge))
+        return false;
+    SingleCall call;
+    Arena ar(call.stream());
+    a.sim = table_on_device(ar, sim_h, size_t(A) * A, 0, call.stream()).dev;
+    a.A = A;
+    a.go = go;
+    a.ge = ge;
+    const SingleKind kind = trace ? (local ? SingleKind::TracebackLocal : SingleKind::TracebackGlobal)
+                                  : (local ? SingleKind::GotohLocal : SingleKind::GotohGlobal);
+    *res = call.run(kind, a, label);
+    return true;
+}
+
 extern "C" {
 
 const char* st_version(void) { return kVersion; }
@@ -2427,6 +2543,24 @@ int32_t st_align_pairs(const uint8_t* q_codes, const uint64_t* q_offsets, uint64
         return int(ST_INVALID_ARGUMENT);
     return guarded(status, [&] {
         require_device();
+        if (n_pairs == 1) {  // align_pair: one pair
+            const uint64_t q0 = q_offsets[pair_query[0]], r0 = r_offsets[pair_ref[0]];
+            const uint64_t m = q_offsets[pair_query[0] + 1] - q0;
+            const uint64_t n = r_offsets[pair_ref[0] + 1] - r0;
+            SingleOut res;
+            if (ops_stride >= m + n &&
+                single_gotoh(q_codes + q0, m, r_codes + r0, n, sim, alphabet_size, local != 0,
+                             gap_open, gap_extend, true,
+                             local ? "single_traceback_local" : "single_traceback_global",
+                             &res)) {
+                std::memcpy(ops, res.ops, res.n_ops);
+                score[0] = res.score;
+                n_ops[0] = res.n_ops;
+                q_begin[0] = res.q_begin;
+                r_begin[0] = res.r_begin;
+                return ok(status);
+            }
+        }
         PairTraceback t(q_codes, q_offsets, n_queries, r_codes, r_offsets, n_refs, pair_query,
                         pair_ref, n_pairs, ops_stride, false);
         traceback_device(t.qd.bytes, t.qd.off, t.qh.max_len, t.rd.bytes, t.rd.off, t.rh.max_len,
@@ -2480,9 +2614,23 @@ int32_t st_index_align_pairs(const uint8_t* q_codes, const uint64_t* q_offsets,
         return int(ST_INVALID_ARGUMENT);
     return guarded(status, [&] {
         require_device();
+        const std::vector<int32_t> neg = negated_penalties(pen, alphabet_size);
+        if (n_pairs == 1) {  // Index::align: one pair
+            const uint64_t q0 = q_offsets[pair_query[0]], r0 = r_offsets[pair_ref[0]];
+            const uint64_t m = q_offsets[pair_query[0] + 1] - q0;
+            const uint64_t n = r_offsets[pair_ref[0] + 1] - r0;
+            SingleOut res;
+            if (ops_stride >= m + n &&
+                single_gotoh(q_codes + q0, m, r_codes + r0, n, neg.data(), alphabet_size, false,
+                             gap_open, gap_extend, true, "single_traceback_penalty", &res)) {
+                std::memcpy(ops, res.ops, res.n_ops);
+                score[0] = int32_t(-res.score);
+                n_ops[0] = res.n_ops;
+                return ok(status);
+            }
+        }
         PairTraceback t(q_codes, q_offsets, n_queries, r_codes, r_offsets, n_refs, pair_query,
                         pair_ref, n_pairs, ops_stride, true);
-        const std::vector<int32_t> neg = negated_penalties(pen, alphabet_size);
         traceback_device(t.qd.bytes, t.qd.off, t.qh.max_len, t.rd.bytes, t.rd.off, t.rh.max_len,
                          t.pq, t.pr, n_pairs, neg.data(), alphabet_size, 0, gap_open, gap_extend,
                          t.ops, ops_stride, nullptr, t.meta, t.meta + n_pairs,
@@ -2734,6 +2882,16 @@ int32_t st_align_matrix(const uint8_t* q_codes, const uint64_t* q_offsets, uint6
     if (n_queries == 0 || n_refs == 0) return ok(status);
     return guarded(status, [&] {
         require_device();
+        if (n_queries == 1 && n_refs == 1 && !dist) {  // align_score: one pair
+            SingleOut res;
+            if (single_gotoh(q_codes + q_offsets[0], q_offsets[1] - q_offsets[0],
+                             r_codes + r_offsets[0], r_offsets[1] - r_offsets[0], sim,
+                             alphabet_size, local != 0, gap_open, gap_extend, false,
+                             local ? "single_gotoh_local" : "single_gotoh_global", &res)) {
+                out[0] = int32_t(res.score);  // |score| < 2^28 under the kernel's range bound
+                return ok(status);
+            }
+        }
         const HostSet qh = host_set(q_codes, q_offsets, n_queries);
         const HostSet rh = host_set(r_codes, r_offsets, n_refs);
         HostCall call;
@@ -2801,6 +2959,14 @@ int32_t st_hamming_matrix(const uint8_t* a, const uint64_t* a_offsets, uint64_t 
     if (n_a == 0 || n_b == 0) return ok(status);
     return guarded(status, [&] {
         require_device();
+        SingleArgs sa;
+        if (n_a == 1 && n_b == 1 &&
+            single_args(sa, a + a_offsets[0], a_offsets[1] - a_offsets[0], b + b_offsets[0],
+                        b_offsets[1] - b_offsets[0], kSingleMaxR, 0, true)) {  // hamming
+            SingleCall one;
+            out[0] = int32_t(one.run(SingleKind::Hamming, sa, "single_hamming").score);
+            return ok(status);
+        }
         HostCall call;
         const hipStream_t s = call.stream();
         Arena ar(s);
@@ -2834,6 +3000,20 @@ int32_t st_levenshtein_matrix(const uint8_t* a, const uint64_t* a_offsets, uint6
     if (n_a == 0 || n_b == 0) return ok(status);
     return guarded(status, [&] {
         require_device();
+        if (n_a == 1 && n_b == 1) {  // levenshtein: rows over the shorter string
+            const uint8_t *x = a + a_offsets[0], *y = b + b_offsets[0];
+            uint64_t lx = a_offsets[1] - a_offsets[0], ly = b_offsets[1] - b_offsets[0];
+            if (ly < lx) {
+                std::swap(x, y);
+                std::swap(lx, ly);
+            }
+            SingleArgs sa;
+            if (single_args(sa, x, lx, y, ly, kSingleMaxQ, 0)) {
+                SingleCall one;
+                out[0] = int32_t(one.run(SingleKind::Levenshtein, sa, "single_levenshtein").score);
+                return ok(status);
+            }
+        }
         const HostSet ah = host_set(a, a_offsets, n_a);
         const HostSet bh = host_set(b, b_offsets, n_b);
         HostCall call;

@@ -430,4 +430,29 @@ void launch_levenshtein_generic(const DistanceArgs& a, const uint32_t* list, uin
                                 int32_t* scratch, uint32_t slots, uint64_t stride,
                                 hipStream_t s);
 
+// ---- single pairs (single.hip): the scalar entry points ---------------------------------
+// One wave per call, the pair in the kernel arguments, the results in pinned host memory.
+// Gotoh and Levenshtein rows are the query (one per lane, 1..kSingleMaxQ residues), columns
+// the reference (1..kSingleMaxR); Hamming takes up to kSingleMaxR positions of either.
+constexpr uint32_t kSingleMaxQ = 64;
+constexpr uint32_t kSingleMaxR = 256;
+struct SingleOut {
+    int64_t score;  // Gotoh similarity (or minus the penalty), distances
+    uint32_t n_ops, q_begin, r_begin, unused;
+    uint8_t ops[kSingleMaxQ + kSingleMaxR];  // traceback ops in forward order
+};
+struct SingleArgs {
+    uint8_t q[kSingleMaxR];
+    uint8_t r[kSingleMaxR];
+    uint32_t m, n;
+    const int32_t* sim;  // device A x A table [query code][reference code] (Gotoh)
+    uint32_t A;
+    int32_t go, ge;
+    SingleOut* out;  // pinned host memory
+};
+enum class SingleKind {
+    GotohGlobal, GotohLocal, TracebackGlobal, TracebackLocal, Levenshtein, Hamming
+};
+void launch_single(SingleKind kind, const SingleArgs& a, hipStream_t s);
+
 }  // namespace st
