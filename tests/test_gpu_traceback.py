@@ -78,7 +78,7 @@ def test_align_pairs_batch_equals_the_oracle(gpu, oracle_lib):
 def test_large_lists_run_in_shape_order(gpu, oracle_lib):
     """From 4,096 pairs on, the pairs are processed in shape order (sorted by clamped
     lengths); every output must still land at its own pair's index. Lengths past the clamp
-    (63) and repeated pairs included."""
+    (31) and repeated pairs included."""
     st = gpu
     m = st.SubstitutionMatrix.blosum62()
     sim = _sim(st, m)
