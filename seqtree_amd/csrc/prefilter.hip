@@ -37,6 +37,7 @@ namespace pf {
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ float bf2f(__bf16 v) { return float(v); }
@@ -197,7 +198,9 @@ __global__ __launch_bounds__(256) void conv_kernel(const __bf16* __restrict__ X,
 // rounding the unfused kernels store. Wave w computes output channels [32 w, 32 w + 32) of
 // every 32-row block (five for conv1 and conv2, whose rows past the needed ones are clamped and
 // discarded, four for conv3); its weight fragments come from global memory (L2) into a
-// register ring eight k-steps ahead, the activation fragments from LDS.
+// register ring eight k-steps ahead, the activation fragments from LDS. The MFMA takes the
+// weights as its first operand, so each lane's accumulators hold four consecutive channels of
+// one row per group and the layer's output goes to LDS as 8-byte stores.
 //
 // Frames: output frame row j of every layer reads its input frame rows j .. j + 4 (the input
 // frame starts two rows earlier). Frames start at global rows r0 - 6 (X0), r0 - 4 (H1),
@@ -242,34 +245,44 @@ __device__ __forceinline__ void enc_layer(const char* in, uint32_t stride, uint3
 #pragma unroll
         for (uint32_t b = 0; b < NBLK; ++b) {
             const bf16x8 a = *reinterpret_cast<const bf16x8*>(arow[b] + kk * stride + ci * 2);
-            acc[b] = mfma(a, bw, acc[b]);
+            acc[b] = mfma(bw, a, acc[b]);  // [channel][row]: channels in the accumulator rows
         }
         if (ks + P < KS) ring[ks % P] = next;
     }
 }
 
 // acc + bias, ReLU and the frame's valid mask (bytes, one per frame row), rounded to BF16 into
-// `out` (LDS, row stride kHStride); the last block stores its first LAST row groups of 8 only
-// (the rows past those are not needed).
+// `out` (LDS, row stride kHStride); the last block stores its first 8 LAST rows only (the rows
+// past those are not needed). Lane (li, lk) holds row 32 b + li of block b and, in group g,
+// the channels 32 w + 8 g + 4 lk + u, u < 4: one 8-byte store per group.
 template <uint32_t NBLK, uint32_t LAST>
-__device__ __forceinline__ void enc_store(const f32x16 (&acc)[NBLK], float bn,
-                                          const uint8_t* mask, char* out) {
+__device__ __forceinline__ void enc_store(const f32x16 (&acc)[NBLK],
+                                          const float* __restrict__ bias, const uint8_t* mask,
+                                          char* out) {
     const uint32_t lane = __lane_id(), li = lane & 31u, lk = lane >> 5, w = threadIdx.x >> 6;
-    const uint32_t n = 32 * w + li;
+    const uint32_t c0 = 32 * w + 4 * lk;
+    float bv[4][4];
 #pragma unroll
-    for (uint32_t b = 0; b < NBLK; ++b)
+    for (uint32_t g = 0; g < 4; ++g)
 #pragma unroll
-        for (uint32_t g = 0; g < (b + 1 < NBLK ? 4u : LAST); ++g) {
-            const uint32_t j0 = 32 * b + 8 * g + 4 * lk;  // rows j0 .. j0 + 3
-            const uint32_t m4 = *reinterpret_cast<const uint32_t*>(mask + j0);
+        for (uint32_t u = 0; u < 4; ++u) bv[g][u] = bias[c0 + 8 * g + u];
+#pragma unroll
+    for (uint32_t b = 0; b < NBLK; ++b) {
+        const uint32_t j = 32 * b + li;
+        if (b + 1 == NBLK && li >= 8 * LAST) continue;
+        const bool row_ok = mask[j] != 0;
+        char* dst = out + j * kHStride + c0 * 2;
+#pragma unroll
+        for (uint32_t g = 0; g < 4; ++g) {
+            bf16x4 pk;
 #pragma unroll
             for (uint32_t u = 0; u < 4; ++u) {
-                const float v = acc[b][4 * g + u] + bn;
-                const bool keep = v > 0.f && ((m4 >> (8 * u)) & 0xFFu);
-                *reinterpret_cast<__bf16*>(out + (j0 + u) * kHStride + n * 2) =
-                    f2bf(keep ? v : 0.f);
+                const float v = acc[b][4 * g + u] + bv[g][u];
+                pk[u] = f2bf(v > 0.f && row_ok ? v : 0.f);
             }
+            *reinterpret_cast<bf16x4*>(dst + 16 * g) = pk;
         }
+    }
 }
 
 // Pooling records of the fused encoder, per tile t: the FP32 channel sums of the sequence that
@@ -317,10 +330,9 @@ __global__ __launch_bounds__(256, 2) void encode_kernel(const __bf16* __restrict
     __shared__ uint64_t soff[kEncMaxSeqs + 1];  // offsets of the sequences meeting this tile
     __shared__ float red[2][2];
     __shared__ uint32_t head_s;
-    const uint32_t tid = threadIdx.x, n = 32 * (tid >> 6) + (tid & 31u);
+    const uint32_t tid = threadIdx.x;
     const uint32_t t = blockIdx.x;
     const int64_t r0 = int64_t(t) * kEncRows;
-    const float bn1 = P[kOffB1 + n], bn2 = P[kOffB2 + n], bn3 = P[kOffB3 + n];
     const uint32_t s0 = pool.tile_first[t];
     const uint64_t off0 = pool.off[0];
 
@@ -348,20 +360,20 @@ __global__ __launch_bounds__(256, 2) void encode_kernel(const __bf16* __restrict
     {
         f32x16 acc[5];
         enc_layer<kEmbDim, 5>(regB, kXStride, kH1Rows - 1, B + kOffWt1, acc);
-        enc_store<5, 1>(acc, bn1, mask[0], regA);
+        enc_store<5, 1>(acc, P + kOffB1, mask[0], regA);
     }
     __syncthreads();
     {
         f32x16 acc[5];
         enc_layer<kChannels, 5>(regA, kHStride, kH2Rows - 1, B + kOffWt2, acc);
         __syncthreads();  // every wave is done reading X0 before H2 overwrites it
-        enc_store<5, 1>(acc, bn2, mask[1], regB);
+        enc_store<5, 1>(acc, P + kOffB2, mask[1], regB);
     }
     __syncthreads();
     {
         f32x16 acc[4];
         enc_layer<kChannels, 4>(regB, kHStride, kEncRows - 1, B + kOffWt3, acc);
-        enc_store<4, 4>(acc, bn3, mask[2], regA);
+        enc_store<4, 4>(acc, P + kOffB3, mask[2], regA);
     }
     __syncthreads();
 
