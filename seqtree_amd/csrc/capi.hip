@@ -1274,15 +1274,15 @@ struct PfActs {
     __bf16* rows = nullptr;  // row 0
 };
 
-PfActs pf_acts(Arena& ar, uint64_t R, uint32_t C, hipStream_t s) {
+PfActs pf_acts(Arena& ar, uint64_t R, uint32_t C, hipStream_t s,
+               uint32_t guard = kGuardRows) {
     PfActs a;
-    const uint64_t total = (R + 2 * kGuardRows) * C;
+    const uint64_t total = (R + 2 * uint64_t(guard)) * C;
     a.base = ar.get<__bf16>(total);
-    a.rows = a.base + uint64_t(kGuardRows) * C;
+    a.rows = a.base + uint64_t(guard) * C;
     // zero the guard rows (bf16 zero is all-zero bits; two elements per 32-bit word)
-    launch_fill_u32(reinterpret_cast<uint32_t*>(a.base), uint64_t(kGuardRows) * C / 2, 0u, s);
-    launch_fill_u32(reinterpret_cast<uint32_t*>(a.rows + R * C), uint64_t(kGuardRows) * C / 2, 0u,
-                    s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(a.base), uint64_t(guard) * C / 2, 0u, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(a.rows + R * C), uint64_t(guard) * C / 2, 0u, s);
     return a;
 }
 
@@ -1334,17 +1334,25 @@ PfEncodeScratch pf_encode_scratch(Arena& ar, uint64_t R) {
             ar.get<uint32_t>(tiles)};
 }
 
+// Guard rows of the fused encoder's inputs: its tile windows read X0 rows -6 .. R + 5 and
+// valid bytes -4 .. R + 43, all zero outside the block's rows.
+constexpr uint32_t kEncX0Guard = 8;     // X0 rows before row 0 and past R
+constexpr uint32_t kEncValidHead = 16;  // valid bytes before row 0
+constexpr uint32_t kEncValidTail = 64;  // valid bytes past R
+
 // Inference forward and mean pool over one packed block: X0, `valid` and the tiles' first
 // sequences, then the fused kernel (H1 .. H3 never leave LDS) writing the BF16 embeddings eb
-// and their squared norms.
+// and their squared norms. x0 has kEncX0Guard guard rows on each side; valid points
+// kEncValidHead bytes into an allocation of R + kEncValidHead + kEncValidTail bytes.
 void pf_encode_forward(const __bf16* B, const float* P, const uint8_t* codes, const uint64_t* off,
                        uint32_t n, const PfRows& rows, const PfActs& x0, uint8_t* valid,
                        const PfEncodeScratch& sc, __bf16* eb, float* norm, hipStream_t s) {
     LaunchTimer t("prefilter_encode", s);
     const uint64_t R = rows.R;
-    launch_fill_u32(reinterpret_cast<uint32_t*>(valid), R / 4, 0u, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(valid - kEncValidHead),
+                    (R + kEncValidHead + kEncValidTail) / 4, 0u, s);
     launch_fill_u32(reinterpret_cast<uint32_t*>(x0.rows + rows.used * pf::kEmbDim),
-                    (R - rows.used) * pf::kEmbDim / 2, 0u, s);
+                    (R + kEncX0Guard - rows.used) * pf::kEmbDim / 2, 0u, s);
     launch_fill_u32(sc.tile_first, R / 128, n, s);
     pf::launch_embed(codes, off, n, B + pf::kOffEmbB, x0.rows, valid, sc.tile_first, s);
     pf::launch_encode(x0.rows, valid, B, P, R, off, n, sc.tile_first, eb, norm, sc.part,
@@ -2020,8 +2028,8 @@ int32_t st_prefilter_encode(const st_prefilter* model, const uint8_t* codes,
         require_device();
         const hipStream_t s = static_cast<hipStream_t>(stream);
         const PrefilterImpl& m = model->impl;
-        // blocks of about 2^20 rows (256 MB per 128-channel activation buffer)
-        const uint32_t blk = uint32_t(std::min<uint64_t>(n, std::max<uint64_t>(1024, (uint64_t(1) << 20) / (max_len + 2))));
+        // blocks of about 2^22 rows (256 MB of X0)
+        const uint32_t blk = uint32_t(std::min<uint64_t>(n, std::max<uint64_t>(1024, (uint64_t(1) << 22) / (max_len + 2))));
         const uint64_t Rb = (uint64_t(blk) * (max_len + 2) + 127) / 128 * 128;
         // the offsets at the block boundaries (blocks of blk sequences, then n)
         const uint32_t nb = (n + blk - 1) / blk;
@@ -2031,9 +2039,9 @@ int32_t st_prefilter_encode(const st_prefilter* model, const uint8_t* codes,
         std::vector<uint64_t> bound(nb + 1);
         d2h(bound.data(), dbound, nb + 1, s);
         sync(s);
-        const PfActs x0 = pf_acts(ar, Rb, pf::kEmbDim, s);
+        const PfActs x0 = pf_acts(ar, Rb, pf::kEmbDim, s, kEncX0Guard);
         const PfEncodeScratch sc = pf_encode_scratch(ar, Rb);
-        uint8_t* valid = ar.get<uint8_t>(Rb);
+        uint8_t* valid = ar.get<uint8_t>(Rb + kEncValidHead + kEncValidTail) + kEncValidHead;
         __bf16* out = static_cast<__bf16*>(emb);
         for (uint32_t s0 = 0; s0 < n; s0 += blk) {
             const uint32_t cnt = std::min(blk, n - s0);

@@ -43,6 +43,18 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 __device__ __forceinline__ float bf2f(__bf16 v) { return float(v); }
 __device__ __forceinline__ __bf16 f2bf(float v) { return __bf16(v); }
 
+// Buffer resource word 3 for raw (untyped) loads on gfx9-family parts.
+constexpr int kBufferFlags = 0x00020000;
+
+typedef __attribute__((address_space(3))) void* lds_void;
+
+// s_waitcnt vmcnt(N), leaving the other counters alone (gfx9 field layout).
+template <uint32_t N>
+__device__ __forceinline__ void wait_vmcnt() {
+    static_assert(N < 64, "vmcnt holds six bits");
+    __builtin_amdgcn_s_waitcnt((N & 0xFu) | ((N >> 4) << 14) | (0x7u << 4) | (0xFu << 8));
+}
+
 __device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
@@ -192,15 +204,17 @@ __global__ __launch_bounds__(256) void conv_kernel(const __bf16* __restrict__ X,
 }
 
 // ---- fused encoder (inference) -------------------------------------------------------------
-// One workgroup computes H3 for the 128 rows [r0, r0 + 128) straight from X0, without writing
-// H1 or H2 to global memory: conv1 over the 136 rows [r0 - 4, r0 + 132), conv2 over the 132
-// rows [r0 - 2, r0 + 130), conv3 over the 128 rows, each layer's output kept in LDS as the BF16
-// rounding the unfused kernels store. Wave w computes output channels [32 w, 32 w + 32) of
-// every 32-row block (five for conv1 and conv2, whose rows past the needed ones are clamped and
-// discarded, four for conv3); its weight fragments come from global memory (L2) into a
-// register ring eight k-steps ahead, the activation fragments from LDS. The MFMA takes the
-// weights as its first operand, so each lane's accumulators hold four consecutive channels of
-// one row per group and the layer's output goes to LDS as 8-byte stores.
+// One workgroup computes H3 for the 128 rows [r0, r0 + 128) straight from X0 and mean-pools it,
+// without writing H1, H2 or H3 to global memory: conv1 over the 136 rows [r0 - 4, r0 + 132),
+// conv2 over the 132 rows [r0 - 2, r0 + 130), conv3 over the 128 rows, each layer's output
+// rounded to BF16 as the unfused kernels store it. Wave w computes output channels
+// [32 w, 32 w + 32) of every 32-row block (five for conv1 and conv2, whose rows past the needed
+// ones are clamped and discarded, four for conv3); its weight fragments come from global memory
+// (L2) into a register ring eight k-steps ahead, the activation fragments from LDS one k-step
+// ahead. conv1 and conv2 take the weights as the MFMA's first operand, so each lane's
+// accumulators hold four consecutive channels of one row per group and the output goes to LDS
+// as 8-byte stores; conv3 takes the activations first, and its accumulators feed the pooling
+// MFMAs (enc_pool) without leaving registers.
 //
 // Frames: output frame row j of every layer reads its input frame rows j .. j + 4 (the input
 // frame starts two rows earlier). Frames start at global rows r0 - 6 (X0), r0 - 4 (H1),
@@ -209,21 +223,50 @@ constexpr uint32_t kEncRows = kEncTile;
 constexpr uint32_t kXStride = 80;   // bytes per X0 row in LDS: 64 + 16 (conflict-free reads)
 constexpr uint32_t kHStride = 272;  // bytes per H row in LDS: 256 + 16
 constexpr uint32_t kXRows = kEncRows + 12, kH1Rows = kEncRows + 8, kH2Rows = kEncRows + 4;
-// H1 (136 rows), later the H3 staging; X0, later H2 (132 rows plus 4 rows that the last block's
-// discarded rows land in)
+// H1 (136 rows), later the staging of the pooled embeddings; X0, later H2 (132 rows plus 4 rows
+// that the last block's discarded rows land in)
 constexpr uint32_t kEncRegionA = kH1Rows * kHStride;
 constexpr uint32_t kEncRegionB = (kH2Rows + 4) * kHStride;
 constexpr uint32_t kEncMaskRows = 160;  // per-layer valid masks, one byte per frame row
 static_assert(kXRows * kXStride <= kEncRegionB && kEncRows * kHStride <= kEncRegionA, "LDS");
 
-// acc[b] = sum_k A[32 b + li (clamped)][k] * W[k][32 w + li] over K = 5 CIN, A row j of the
-// input frame span j .. j + 4 in LDS (`in`, row stride `stride` bytes).
-template <uint32_t CIN, uint32_t NBLK>
+// The lane and wave indices of the fused encoder's per-tile code, recomputed at each use: the
+// empty asm makes the lane index a new value each time, so the addresses derived from it are not
+// hoisted out of the tile loop (where they would stay live across every layer and spill), and
+// the wave index is a scalar.
+__device__ __forceinline__ uint32_t enc_lane() {
+    uint32_t l = __lane_id();
+    asm volatile("" : "+v"(l));
+    return l;
+}
+__device__ __forceinline__ uint32_t enc_wave() {
+    return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+}
+
+// Weight fragments in flight per lane. The fused encoder streams the weight fragments of conv1
+// (10 k-steps), conv2 (40) and conv3 (40), then the next tile's conv1, through one register
+// ring: the fragment of stream position f sits in slot f % kRingDepth, and every layer's
+// k-step count is a multiple of the depth, so slot ks % kRingDepth holds k-step ks of the
+// current layer in each one.
+constexpr uint32_t kRingDepth = 5;
+
+// Block b of output channels [32 w, 32 w + 32) and rows 32 b + (0 .. 31) (rows past `clamp` read
+// row `clamp`): the sum over K = 5 CIN of A[row][k] W[k][channel], A row j of the input frame
+// span j .. j + 4 in LDS (`in`, row stride `stride` bytes). ROWS puts the rows in the
+// accumulators' rows (A as the MFMA's first operand), otherwise the channels. On entry the ring
+// holds this layer's first kRingDepth weight fragments (Wt); k-step ks consumes its slot and
+// refills it with the fragment kRingDepth k-steps further on in the stream, which past this
+// layer's last k-step is the next layer's (Wn, CIN_NEXT input channels).
+template <uint32_t CIN, uint32_t NBLK, bool ROWS, uint32_t CIN_NEXT>
 __device__ __forceinline__ void enc_layer(const char* in, uint32_t stride, uint32_t clamp,
-                                          const __bf16* __restrict__ Wt, f32x16 (&acc)[NBLK]) {
-    constexpr uint32_t K = 5 * CIN, KS = K / 16, P = 8;
-    const uint32_t lane = __lane_id(), li = lane & 31u, lk = lane >> 5, w = threadIdx.x >> 6;
+                                          const __bf16* __restrict__ Wt,
+                                          const __bf16* __restrict__ Wn,
+                                          bf16x8 (&ring)[kRingDepth], f32x16 (&acc)[NBLK]) {
+    constexpr uint32_t K = 5 * CIN, KS = K / 16, P = kRingDepth, KN = 5 * CIN_NEXT;
+    static_assert(KS % P == 0 && KN / 16 >= P, "ring slots follow every layer's k-steps");
+    const uint32_t lane = enc_lane(), li = lane & 31u, lk = lane >> 5, w = enc_wave();
     const __bf16* wrow = Wt + uint64_t(32 * w + li) * K + 8 * lk;
+    const __bf16* wnext = Wn + uint64_t(32 * w + li) * KN + 8 * lk;
     const char* arow[NBLK];
 #pragma unroll
     for (uint32_t b = 0; b < NBLK; ++b) {
@@ -231,23 +274,33 @@ __device__ __forceinline__ void enc_layer(const char* in, uint32_t stride, uint3
         arow[b] = in + (j < clamp ? j : clamp) * stride + 16 * lk;
         acc[b] = f32x16{0.f};
     }
-    bf16x8 ring[P];
+    // the activation fragments of k-step ks + 1 are read while the MFMAs of k-step ks run: the
+    // scheduling barriers keep the compiler from sinking each read to just before its MFMA,
+    // which would expose the LDS latency once per MFMA
+    const auto a_frag = [&](uint32_t ks, uint32_t b) {
+        const uint32_t kk = (16 * ks) / CIN, ci = (16 * ks) % CIN;
+        return *reinterpret_cast<const bf16x8*>(arow[b] + kk * stride + ci * 2);
+    };
+    bf16x8 cur[NBLK];
 #pragma unroll
-    for (uint32_t ks = 0; ks < P && ks < KS; ++ks)
-        ring[ks] = *reinterpret_cast<const bf16x8*>(wrow + 16 * ks);
+    for (uint32_t b = 0; b < NBLK; ++b) cur[b] = a_frag(0, b);
 #pragma unroll
     for (uint32_t ks = 0; ks < KS; ++ks) {
-        bf16x8 next{};
-        if (ks + P < KS) next = *reinterpret_cast<const bf16x8*>(wrow + 16 * (ks + P));
-        asm volatile("" ::: "memory");  // issue the weight load P k-steps ahead, here
-        const bf16x8 bw = ring[ks % P];
-        const uint32_t kk = (16 * ks) / CIN, ci = (16 * ks) % CIN;
+        const bf16x8 next = *reinterpret_cast<const bf16x8*>(
+            ks + P < KS ? wrow + 16 * (ks + P) : wnext + 16 * (ks + P - KS));
+        asm volatile("" ::: "memory");  // issue the weight load here
+        bf16x8 nxt[NBLK];
 #pragma unroll
-        for (uint32_t b = 0; b < NBLK; ++b) {
-            const bf16x8 a = *reinterpret_cast<const bf16x8*>(arow[b] + kk * stride + ci * 2);
-            acc[b] = mfma(bw, a, acc[b]);  // [channel][row]: channels in the accumulator rows
-        }
-        if (ks + P < KS) ring[ks % P] = next;
+        for (uint32_t b = 0; b < NBLK; ++b) nxt[b] = ks + 1 < KS ? a_frag(ks + 1, b) : cur[b];
+        __builtin_amdgcn_sched_barrier(0);
+        const bf16x8 bw = ring[ks % P];
+#pragma unroll
+        for (uint32_t b = 0; b < NBLK; ++b)
+            acc[b] = ROWS ? mfma(cur[b], bw, acc[b]) : mfma(bw, cur[b], acc[b]);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (uint32_t b = 0; b < NBLK; ++b) cur[b] = nxt[b];
+        ring[ks % P] = next;
     }
 }
 
@@ -256,10 +309,9 @@ __device__ __forceinline__ void enc_layer(const char* in, uint32_t stride, uint3
 // past those are not needed). Lane (li, lk) holds row 32 b + li of block b and, in group g,
 // the channels 32 w + 8 g + 4 lk + u, u < 4: one 8-byte store per group.
 template <uint32_t NBLK, uint32_t LAST>
-__device__ __forceinline__ void enc_store(const f32x16 (&acc)[NBLK],
-                                          const float* __restrict__ bias, const uint8_t* mask,
-                                          char* out) {
-    const uint32_t lane = __lane_id(), li = lane & 31u, lk = lane >> 5, w = threadIdx.x >> 6;
+__device__ __forceinline__ void enc_store(const f32x16 (&acc)[NBLK], const float* bias,
+                                          const uint8_t* mask, char* out) {
+    const uint32_t lane = enc_lane(), li = lane & 31u, lk = lane >> 5, w = enc_wave();
     const uint32_t c0 = 32 * w + 4 * lk;
     float bv[4][4];
 #pragma unroll
@@ -301,127 +353,264 @@ struct EncPool {
     uint32_t* head_seq;        // per tile
 };
 
-// Mean of `sum` over `len` rows, rounded to BF16, stored with its squared norm (a reduction over
-// the 128 threads c of one half of the workgroup; every thread of the workgroup calls this the
-// same number of times).
-__device__ __forceinline__ void enc_finish(float sum, uint32_t len, bool active, uint32_t s,
-                                           uint32_t c, uint32_t half, float (*red)[2],
-                                           const EncPool& p) {
-    const float v = len ? sum / float(len) : 0.f;
-    const __bf16 vb = f2bf(v);
-    if (active) p.eb[uint64_t(s) * kChannels + c] = vb;
-    float sq = active ? bf2f(vb) * bf2f(vb) : 0.f;
+// The sequences s0 + j, j < cnt, that meet a tile: each one's first residue row relative to the
+// tile's first row (negative for the one that started before it), its length and the
+// reciprocal of its length (0 for an empty one). Entries past cnt are stale.
+struct EncSeqs {
+    int32_t lo[96];
+    uint32_t len[96];
+    float inv[96];
+};
+static_assert(96 >= (kEncMaxSeqs + 31) / 32 * 32, "three blocks of 32 sequences");
+constexpr uint32_t kStageStride = 136;  // floats per staged row of sums (conflict-free stores)
+static_assert(32 * kStageStride * 4 <= kEncRegionB, "the sums' stage fits the H2 region");
+
+// Mean pool of the tile from conv3's accumulators, as MFMAs: sums[s][c] = sum over the tile's
+// rows r of M[s][r] H3[r][c], M the 0/1 membership of row r in sequence s's residue rows.
+// conv3 runs with the activations as the first MFMA operand, so lane (li, lk) of wave w holds
+// channel c = 32 w + li of the rows 32 b + 8 g + 4 lk + u (element 4 g + u of block b). After
+// bias, ReLU, the frame mask and the BF16 rounding (the values the unfused layers store),
+// elements 8 h .. 8 h + 7 of block b are the lane's second operand for k-step 2 b + h, k slot i
+// standing for row 32 b + 8 (2 h + i / 4) + 4 lk + i % 4; the membership operand uses the same
+// row for each k slot, so every row of the tile enters each sum once. Lane (li, lk) of the
+// result holds sequence 32 mb + 8 g + 4 lk + u (element 4 g + u) of channel c; the sums go to
+// an LDS stage (FP32, row stride kStageStride), from which 16 threads per sequence finish it:
+// a sequence that lies wholly in the tile gets its mean rounded to BF16 (16-byte stores) and
+// the squared norm of that rounding (each thread's 8 channels in order, then a fixed xor
+// tree), the one cut by the tile's start leaves its tail sums and the one cut by its end its
+// head sums, which encode_finish_kernel joins. Every thread of the workgroup calls this with
+// the same cnt.
+__device__ __forceinline__ void enc_pool(const f32x16 (&acc)[4], const float* bias,
+                                         const uint8_t* mask, const EncSeqs& seq, uint32_t cnt,
+                                         uint32_t s0, uint32_t t, float* stage,
+                                         const EncPool& p) {
+    const uint32_t lane = enc_lane(), tid = 64 * enc_wave() + lane, li = lane & 31u, lk = lane >> 5;
+    const uint32_t c = 32 * (tid >> 6) + li;
+    const float bc = bias[c];
+    bf16x8 hb[4][2];
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
-    if ((c & 63u) == 0) red[half][c >> 6] = sq;
-    __syncthreads();
-    if (active && c == 0) p.norm[s] = red[half][0] + red[half][1];
-    __syncthreads();
+    for (uint32_t b = 0; b < 4; ++b)
+#pragma unroll
+        for (uint32_t g = 0; g < 4; ++g) {
+            const uint32_t m4 = *reinterpret_cast<const uint32_t*>(mask + 32 * b + 8 * g + 4 * lk);
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) {
+                const float v = acc[b][4 * g + u] + bc;
+                hb[b][g / 2][4 * (g % 2) + u] = f2bf(v > 0.f && ((m4 >> (8 * u)) & 0xFFu) ? v : 0.f);
+            }
+        }
+    for (uint32_t mb = 0; mb < (cnt + 31) / 32; ++mb) {
+        const uint32_t jm = 32 * mb + li;
+        const int32_t lo = jm < cnt ? seq.lo[jm] : 0;
+        const int32_t hi = jm < cnt ? lo + int32_t(seq.len[jm]) : 0;
+        f32x16 ps = f32x16{0.f};
+#pragma unroll
+        for (uint32_t b = 0; b < 4; ++b)
+#pragma unroll
+            for (uint32_t h = 0; h < 2; ++h) {
+                bf16x8 pa;
+#pragma unroll
+                for (uint32_t i = 0; i < 8; ++i) {
+                    const int32_t row = int32_t(32 * b + 8 * (2 * h + i / 4) + 4 * lk + i % 4);
+                    pa[i] = f2bf(row >= lo && row < hi ? 1.f : 0.f);
+                }
+                ps = mfma(pa, hb[b][h], ps);
+            }
+#pragma unroll
+        for (uint32_t e = 0; e < 16; ++e)
+            stage[(8 * (e / 4) + 4 * lk + e % 4) * kStageStride + c] = ps[e];
+        __syncthreads();
+#pragma unroll
+        for (uint32_t half = 0; half < 2; ++half) {
+            const uint32_t idx = tid + 256 * half, jj = idx / 16, piece = idx % 16;
+            const uint32_t j = 32 * mb + jj;
+            const float4* src = reinterpret_cast<const float4*>(stage + jj * kStageStride + 8 * piece);
+            const float4 x0 = src[0], x1 = src[1];
+            const float sums[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+            const bool in = j < cnt;
+            const int32_t jlo = in ? seq.lo[j] : 0, jhi = in ? jlo + int32_t(seq.len[j]) : 0;
+            const float inv = in ? seq.inv[j] : 0.f;
+            bf16x8 v;
+            float sq = 0.f;
+#pragma unroll
+            for (uint32_t i = 0; i < 8; ++i) {
+                v[i] = f2bf(sums[i] * inv);
+                sq += bf2f(v[i]) * bf2f(v[i]);
+            }
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) sq += __shfl_xor(sq, o);
+            if (in && jlo >= 0 && jhi <= int32_t(kEncRows)) {
+                *reinterpret_cast<bf16x8*>(p.eb + uint64_t(s0 + j) * kChannels + 8 * piece) = v;
+                if (piece == 0) p.norm[s0 + j] = sq;
+            } else if (in && (jlo < 0 ? jhi > 0 : jhi > int32_t(kEncRows))) {
+                float4* dst = reinterpret_cast<float4*>(
+                    p.part + (uint64_t(t) * 2 + (jlo < 0 ? 1 : 0)) * kChannels + 8 * piece);
+                dst[0] = x0;
+                dst[1] = x1;
+            }
+        }
+        __syncthreads();  // the stage is read before the next block of sequences overwrites it
+    }
 }
 
+// The inputs of a tile land in the last kEncInBytes of regA by DMA (buffer loads into LDS, no
+// registers), issued during the previous tile's conv3 while regA is free: the X0 window in the
+// kXStride layout (kXSlots 16-byte slots, five per row, the fifth the row's padding), the valid
+// bytes of rows r0 - 4 .. r0 + 159 (the three frames' masks at offsets 0, 2 and 4) and the
+// offsets of the sequences s0 .. s0 + 95. Every X0 and valid read stays inside its buffer: the
+// caller's buffers hold zero guard rows (8 X0 rows and 16 valid bytes before row 0, 8 X0 rows and
+// 64 valid bytes past R), and padding slots and lanes past the window read a deliberately
+// out-of-range offset, whose slot content is never used; the offsets past n are never used.
+constexpr uint32_t kXSlots = 768;  // 12 per wave-instruction group: 4 waves x 3 x 64 lanes
+constexpr uint32_t kInX0 = 0, kInValid = kXSlots * 16, kInOff = kInValid + 256;
+constexpr uint32_t kEncInBytes = kInOff + 768;
+constexpr uint32_t kEncIn = kEncRegionA - kEncInBytes;  // byte offset of the inputs in regA
+static_assert(kXSlots >= kXRows * 5 && kEncIn % 16 == 0, "X0 window slots");
+constexpr uint32_t kValidWindow = kEncMaskRows + 4;  // bytes: rows r0 - 4 .. r0 + 159
+constexpr uint32_t kOutOfRange = 0xFFFFFFF0u;
+
+struct EncSrc {
+    __amdgpu_buffer_rsrc_t x0, valid, off;
+};
+
+__device__ __forceinline__ void enc_fetch(char* regA, uint32_t t, uint32_t s0,
+                                          const EncSrc& src) {
+    const uint32_t lane = enc_lane(), w = enc_wave();
+    const int32_t r0 = int32_t(t * kEncRows);
+    char* in = regA + kEncIn;
+#pragma unroll
+    for (uint32_t i = 0; i < 3; ++i) {
+        const uint32_t q = (3 * w + i) * 64, slot = q + lane, row = slot / 5, piece = slot % 5;
+        const uint32_t o = piece < 4 && row < kXRows
+                               ? uint32_t(r0 + 2 + int32_t(row)) * (kEmbDim * 2) + 16 * piece
+                               : kOutOfRange;  // window row r0 - 6 + row, 8 guard rows first
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(src.x0, (lds_void)(in + kInX0 + 16 * q), 16,
+                                                 int(o), 0, 0, 0);
+    }
+    if (w == 0) {
+        const uint32_t o =  // byte r0 - 4 + 4 lane, 16 guard bytes first
+            4 * lane < kValidWindow ? uint32_t(r0 + 12) + 4 * lane : kOutOfRange;
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(src.valid, (lds_void)(in + kInValid), 4, int(o),
+                                                 0, 0, 0);
+    } else {
+        const uint32_t q = 64 * (w - 1);  // dwords q .. q + 63 of the offsets from s0
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(src.off, (lds_void)(in + kInOff + 4 * q), 4,
+                                                 int(8 * s0 + 4 * (q + lane)), 0, 0, 0);
+    }
+}
+
+// After the tile's DMA has landed (every wave waited for its own, then a barrier): copies the
+// valid window out of regA (which conv1's output overwrites) and builds the sequence table,
+// returning the number of sequences meeting the tile (those from s0 on that start before its
+// end; every thread gets the count after a barrier) and recording the one that runs past its
+// end, if any.
+__device__ __forceinline__ uint32_t enc_put(const char* regA, uint32_t t, uint32_t s0,
+                                            uint64_t off0, uint8_t* mask, EncSeqs& seq,
+                                            const EncPool& pool) {
+    const uint32_t tid = threadIdx.x;
+    const char* in = regA + kEncIn;
+    if (tid < kValidWindow / 4)
+        reinterpret_cast<uint32_t*>(mask)[tid] =
+            reinterpret_cast<const uint32_t*>(in + kInValid)[tid];
+    bool meets = false;
+    if (tid < kEncMaxSeqs && s0 + tid < pool.n) {
+        const uint64_t* off = reinterpret_cast<const uint64_t*>(in + kInOff);
+        const uint64_t o0 = off[tid], o1 = off[tid + 1];
+        const uint32_t s = s0 + tid;
+        const int64_t lo = int64_t(o0 - off0 + 2ull * s) - int64_t(t) * kEncRows;
+        const uint32_t len = uint32_t(o1 - o0);
+        meets = lo < int64_t(kEncRows);
+        seq.lo[tid] = int32_t(lo);
+        seq.len[tid] = len;
+        seq.inv[tid] = len ? 1.f / float(len) : 0.f;
+    }
+    const uint32_t cnt = uint32_t(__syncthreads_count(meets));
+    if (tid == 0) {  // the last sequence meeting the tile, when it runs past the tile's end
+        const bool head = cnt && seq.lo[cnt - 1] >= 0 &&
+                          seq.lo[cnt - 1] + int32_t(seq.len[cnt - 1]) > int32_t(kEncRows);
+        pool.head_seq[t] = head ? s0 + cnt - 1 : kNoSeq;
+    }
+    return cnt;
+}
+
+// Persistent: workgroup b runs tiles b, b + gridDim.x, ...; each tile's inputs arrive by DMA
+// during the previous tile's conv3 and pooling, and the weight ring runs on from one layer and
+// tile into the next. X0 and `valid` have the guard rows enc_fetch reads.
 __global__ __launch_bounds__(256, 2) void encode_kernel(const __bf16* __restrict__ X0,
                                                         const uint8_t* __restrict__ valid,
                                                         const __bf16* __restrict__ B,
                                                         const float* __restrict__ P, uint64_t R,
-                                                        EncPool pool) {
+                                                        uint32_t tiles, EncPool pool) {
     __shared__ __attribute__((aligned(16))) char regA[kEncRegionA];
     __shared__ __attribute__((aligned(16))) char regB[kEncRegionB];
-    __shared__ __attribute__((aligned(4))) uint8_t mask[3][kEncMaskRows];  // conv1..3 frames
-    __shared__ uint64_t soff[kEncMaxSeqs + 1];  // offsets of the sequences meeting this tile
-    __shared__ float red[2][2];
-    __shared__ uint32_t head_s;
-    const uint32_t tid = threadIdx.x;
-    const uint32_t t = blockIdx.x;
-    const int64_t r0 = int64_t(t) * kEncRows;
-    const uint32_t s0 = pool.tile_first[t];
+    __shared__ __attribute__((aligned(16))) uint8_t mask[kValidWindow];  // frames at 0, 2, 4
+    __shared__ EncSeqs seq;
+    // the layers' biases (LDS reads in the epilogues: a global load there would wait for the
+    // weight ring's loads issued before it)
+    __shared__ float bias[3][kChannels];
+    uint32_t t = blockIdx.x;
+    if (t >= tiles) return;  // the whole workgroup
+    for (uint32_t e = threadIdx.x; e < 3 * kChannels; e += 256) {
+        const uint32_t layer = e / kChannels, c = e % kChannels;
+        bias[layer][c] = P[(layer == 0 ? kOffB1 : layer == 1 ? kOffB2 : kOffB3) + c];
+    }
     const uint64_t off0 = pool.off[0];
-
-    // X0 window (rows outside [0, R) are zero), the three frames' valid masks and the
-    // offsets of the tile's sequences
-    for (uint32_t e = tid; e < kXRows * 4; e += 256) {
-        const uint32_t m = e / 4, piece = e % 4;
-        const int64_t g = r0 - 6 + m;
-        bf16x8 v{};
-        if (g >= 0 && g < int64_t(R))
-            v = *reinterpret_cast<const bf16x8*>(X0 + uint64_t(g) * kEmbDim + piece * 8);
-        *reinterpret_cast<bf16x8*>(regB + m * kXStride + piece * 16) = v;
-    }
-    for (uint32_t e = tid; e < 3 * kEncMaskRows; e += 256) {
-        const uint32_t layer = e / kEncMaskRows, j = e % kEncMaskRows;
-        const int64_t g = r0 - 4 + 2 * int64_t(layer) + j;
-        mask[layer][j] = (g >= 0 && g < int64_t(R)) ? valid[g] : 0;
-    }
-    if (tid <= kEncMaxSeqs) {
-        const uint64_t si = uint64_t(s0) + tid;
-        soff[tid] = pool.off[si < pool.n ? si : pool.n];
-    }
-    if (tid == 0) head_s = kNoSeq;
-    __syncthreads();
+    // the resources reach the guard rows: X0 from row -8 to R + 8, valid from byte -16 to R + 64
+    const EncSrc src{
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<__bf16*>(X0 - 8 * kEmbDim), 0,
+                                          int((R + 16) * kEmbDim * 2), kBufferFlags),
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(valid - 16), 0, int(R + 80),
+                                          kBufferFlags),
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint64_t*>(pool.off), 0,
+                                          int((uint64_t(pool.n) + 1) * 8), kBufferFlags)};
+    uint32_t s0 = pool.tile_first[t];
+    enc_fetch(regA, t, s0, src);
+    bf16x8 ring[kRingDepth];
     {
-        f32x16 acc[5];
-        enc_layer<kEmbDim, 5>(regB, kXStride, kH1Rows - 1, B + kOffWt1, acc);
-        enc_store<5, 1>(acc, P + kOffB1, mask[0], regA);
+        const uint32_t lane = __lane_id();
+        const __bf16* w1 = B + kOffWt1 + uint64_t(32 * (threadIdx.x >> 6) + (lane & 31u)) *
+                                             (5 * kEmbDim) + 8 * (lane >> 5);
+#pragma unroll
+        for (uint32_t k = 0; k < kRingDepth; ++k)
+            ring[k] = *reinterpret_cast<const bf16x8*>(w1 + 16 * k);
     }
-    __syncthreads();
-    {
-        f32x16 acc[5];
-        enc_layer<kChannels, 5>(regA, kHStride, kH2Rows - 1, B + kOffWt2, acc);
-        __syncthreads();  // every wave is done reading X0 before H2 overwrites it
-        enc_store<5, 1>(acc, P + kOffB2, mask[1], regB);
-    }
-    __syncthreads();
-    {
-        f32x16 acc[4];
-        enc_layer<kChannels, 4>(regB, kHStride, kEncRows - 1, B + kOffWt3, acc);
-        enc_store<4, 4>(acc, P + kOffB3, mask[2], regA);
-    }
-    __syncthreads();
-
-    // Mean pool of H3 (rows [r0, r0 + 128) in regA): the sequences s0 + j, j < cnt, meet this
-    // tile; each half of the workgroup takes every other one, thread c its channel c. A
-    // sequence wholly inside the tile is finished here; the one running past the tile's end
-    // leaves its head sums, the one that started before it its tail sums, and
-    // encode_finish_kernel joins the two.
-    const uint64_t tile_end = uint64_t(r0) + kEncRows;
-    uint32_t cnt = 0;
-    while (cnt < kEncMaxSeqs && s0 + cnt < pool.n &&
-           (soff[cnt] - off0) + 2ull * (s0 + cnt) < tile_end)
-        ++cnt;
-    const uint32_t half = tid >> 7, c = tid & 127u;
-    for (uint32_t it = 0; it < (cnt + 1) / 2; ++it) {
-        const uint32_t j = 2 * it + half, s = s0 + j;
-        const bool in = j < cnt;
-        const uint64_t st = in ? (soff[j] - off0) + 2ull * s : 0;
-        const uint32_t len = in ? uint32_t(soff[j + 1] - soff[j]) : 0u;
-        const uint64_t lo = st > uint64_t(r0) ? st : uint64_t(r0);
-        const uint64_t hi = st + len < tile_end ? st + len : tile_end;
-        // four interleaved partial sums (independent LDS reads in flight), combined in a
-        // fixed order
-        const char* col = regA + c * 2;
-        const uint32_t a0 = uint32_t(lo - uint64_t(r0));
-        const uint32_t a1 = hi > lo ? uint32_t(hi - uint64_t(r0)) : a0;  // empty: rows ended
-        float q0 = 0.f, q1 = 0.f, q2 = 0.f, q3 = 0.f;
-        uint32_t o = a0;
-        for (; o + 4 <= a1; o += 4) {
-            q0 += bf2f(*reinterpret_cast<const __bf16*>(col + o * kHStride));
-            q1 += bf2f(*reinterpret_cast<const __bf16*>(col + (o + 1) * kHStride));
-            q2 += bf2f(*reinterpret_cast<const __bf16*>(col + (o + 2) * kHStride));
-            q3 += bf2f(*reinterpret_cast<const __bf16*>(col + (o + 3) * kHStride));
+    for (;;) {
+        wait_vmcnt<0>();  // this wave's DMA of the tile's inputs (and the ring) has landed
+        __syncthreads();
+        const uint32_t cnt = enc_put(regA, t, s0, off0, mask, seq, pool);
+        const uint32_t tn = t + gridDim.x;
+        const bool more = tn < tiles;
+        const uint32_t s0n = more ? pool.tile_first[tn] : 0;
+        {
+            f32x16 acc[5];
+            enc_layer<kEmbDim, 5, false, kChannels>(regA + kEncIn + kInX0, kXStride, kH1Rows - 1,
+                                                    B + kOffWt1, B + kOffWt2, ring, acc);
+            __syncthreads();  // every wave is done reading X0 before H1 overwrites it
+            enc_store<5, 1>(acc, bias[0], mask, regA);
         }
-        for (; o < a1; ++o) q0 += bf2f(*reinterpret_cast<const __bf16*>(col + o * kHStride));
-        const float sum = (q0 + q1) + (q2 + q3);
-        const bool before = st < uint64_t(r0);
-        const bool tail = in && before && st + len > uint64_t(r0);
-        const bool head = in && !before && st + len > tile_end;
-        const bool whole = in && !before && !head;
-        if (tail) pool.part[(uint64_t(t) * 2 + 1) * kChannels + c] = sum;
-        if (head) {
-            pool.part[uint64_t(t) * 2 * kChannels + c] = sum;
-            if (c == 0) head_s = s;
+        __syncthreads();
+        {
+            f32x16 acc[5];
+            enc_layer<kChannels, 5, false, kChannels>(regA, kHStride, kH2Rows - 1, B + kOffWt2,
+                                                      B + kOffWt3, ring, acc);
+            enc_store<5, 1>(acc, bias[1], mask + 2, regB);
         }
-        enc_finish(sum, len, whole, s, c, half, red, pool);
+        __syncthreads();  // H2 is complete and every wave is done reading H1
+        if (more) enc_fetch(regA, tn, s0n, src);
+        {
+            f32x16 acc[4];
+            enc_layer<kChannels, 4, true, kEmbDim>(regB, kHStride, kEncRows - 1, B + kOffWt3,
+                                                   B + kOffWt1, ring, acc);
+            __syncthreads();  // every wave is done reading H2 before the pooling stage overwrites it
+            enc_pool(acc, bias[2], mask + 4, seq, cnt, s0, t,
+                     reinterpret_cast<float*>(regB), pool);
+        }
+        if (!more) break;
+        t = tn;
+        s0 = s0n;
     }
-    if (tid == 0) pool.head_seq[t] = head_s;
+    // the ring's loads for a next tile that does not come land before the wave ends
+    wait_vmcnt<0>();
 }
 
 // Joins the head sums of tile t and the tail sums of tile t + 1 of the sequence cut by their
@@ -498,18 +687,8 @@ constexpr uint32_t kRing = 4;
 constexpr uint32_t kTilePieces = kTileR * kChannels / 8;  // 16-byte pieces per tile
 constexpr uint32_t kLoadsPerTile = 5;  // DMA loads per wave and tile: 4 x 1 KB + the norms
 constexpr uint32_t kNormSlot = 64;      // floats of a wave's norm DMA (16 rows, lanes 16+ repeat)
-// Buffer resource word 3 for raw (untyped) loads on gfx9-family parts.
-constexpr int kBufferFlags = 0x00020000;
 constexpr uint32_t kWaveQueue = 256;
 
-typedef __attribute__((address_space(3))) void* lds_void;
-
-// s_waitcnt vmcnt(N), leaving the other counters alone (gfx9 field layout).
-template <uint32_t N>
-__device__ __forceinline__ void wait_vmcnt() {
-    static_assert(N < 64, "vmcnt holds six bits");
-    __builtin_amdgcn_s_waitcnt((N & 0xFu) | ((N >> 4) << 14) | (0x7u << 4) | (0xFu << 8));
-}
 
 // Orders this wave's LDS writes before its later LDS reads by other lanes.
 __device__ __forceinline__ void wave_sync() {
@@ -1121,7 +1300,13 @@ void launch_encode(const __bf16* X0, const uint8_t* valid, const __bf16* B, cons
     if (R == 0) return;
     const EncPool pool{off, n, tile_first, eb, norm, part, head_seq};
     const uint32_t tiles = uint32_t(R / kEncRows);
-    hipLaunchKernelGGL(encode_kernel, dim3(tiles), dim3(256), 0, s, X0, valid, B, P, R, pool);
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 256;
+    const uint32_t grid = std::min<uint32_t>(tiles, 2u * uint32_t(cus));  // two per CU (LDS)
+    hipLaunchKernelGGL(encode_kernel, dim3(grid), dim3(256), 0, s, X0, valid, B, P, R, tiles,
+                       pool);
     hipLaunchKernelGGL(encode_finish_kernel, dim3(tiles), dim3(kChannels), 0, s, pool);
 }
 
