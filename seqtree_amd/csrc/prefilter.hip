@@ -667,8 +667,9 @@ __global__ __launch_bounds__(kPoolSeqs * kChannels) void pool_kernel(
 }
 
 // ---- Gram screen (MFMA) --------------------------------------------------------------------
-// Workgroup: 4 waves x 64 queries (two 32-row MFMA blocks per wave, operands kept in
-// registers for the whole K = 128) against the references [r_begin, r_end) in tiles of 64.
+// Workgroup: 4 waves x 64 queries (four 16-row MFMA blocks per wave, operands kept in
+// registers for the whole K = 128) against the references [r_begin, r_end) in tiles of 64,
+// each consumed in quarters of 16 references by 16 v_mfma_f32_16x16x32_bf16 per wave.
 // Reference tiles go from global memory straight into a four-buffer LDS ring by LDS-DMA loads
 // (global_load_lds_dwordx4, no staging registers), issued three tiles ahead of the MFMAs that
 // read them. A tile row of 128 BF16 (256 bytes) keeps its 16-byte pieces XOR-swizzled by the
@@ -678,9 +679,11 @@ __global__ __launch_bounds__(kPoolSeqs * kChannels) void pool_kernel(
 // Pair (q, r) is kept when dot(q, r) - hq[q] >= hr[r], i.e. |q|^2 + |r|^2 - 2 dot <= tau, with
 // hq = (|q|^2 - tau) / 2 and hr = |r|^2 / 2. Each MFMA chain starts from -hq held in registers
 // that stay unchanged, and all accumulators of a lane belong to that lane's reference column,
-// so a tile without hits costs a max tree and one compare per column. Hits are extracted with
-// one ballot per accumulator of the row groups that hold any, appended to the wave's LDS queue
-// and flushed to the global arrays with one atomic per queue-full.
+// so a quarter without hits costs a max tree and one compare per column. Hits are extracted
+// with one ballot per accumulator of the row groups that hold any, appended to the wave's LDS
+// queue and flushed to the global arrays with one atomic per queue-full. The quarters run as a
+// software pipeline inside each wave: the operands of quarter k + 1 are read from LDS and the
+// hits of quarter k - 1 extracted while the MFMAs of quarter k run.
 constexpr uint32_t kGramQ = 256;
 constexpr uint32_t kTileR = 64;
 constexpr uint32_t kRing = 4;
@@ -772,6 +775,62 @@ __device__ __forceinline__ void gram_dma(GramSlot* dst, __amdgpu_buffer_rsrc_t r
                                              l.norm, t * (kTileR * 4), 0, 0);
 }
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// One quarter of a tile (16 reference rows) in registers: lane l holds row 16 qt + l % 16 of
+// the slot, dims 32 ks + 8 (l / 16) .. + 7 in b[ks] (the second MFMA operand of k-step ks), and
+// half that row's squared norm as its threshold (infinite past r_end).
+struct GramQuarter {
+    bf16x8 b[4];
+    float hr;
+};
+
+// Per-lane byte offsets of the quarter reads in a slot: row 16 qt + n (n = l % 16), piece
+// 4 ks + l / 16 at index piece ^ n, i.e. n * 256 + ((4 ks + l / 16) ^ n) * 16 + 4096 qt (16
+// lanes of one piece column hit 16 distinct bank groups); the row's norm at
+// norms[qt * kNormSlot + n].
+struct GramReadLanes {
+    uint32_t b[4];
+    uint32_t norm;
+};
+
+__device__ __forceinline__ GramReadLanes gram_read_lanes() {
+    const uint32_t lane = __lane_id(), n = lane & 15u, g = lane >> 4;
+    GramReadLanes o;
+#pragma unroll
+    for (uint32_t ks = 0; ks < 4; ++ks) o.b[ks] = n * 256 + (((4 * ks + g) ^ n) * 16);
+    o.norm = n * 4;
+    return o;
+}
+
+// Quarter qt of a slot whose first row is reference `base`.
+__device__ __forceinline__ void gram_read(const GramSlot* __restrict__ slot, uint32_t qt,
+                                          const GramReadLanes& l, uint32_t base, uint32_t r_end,
+                                          GramQuarter& g) {
+    const char* rows = reinterpret_cast<const char*>(slot->rows) + 4096 * qt;
+#pragma unroll
+    for (uint32_t ks = 0; ks < 4; ++ks)
+        g.b[ks] = *reinterpret_cast<const bf16x8*>(rows + l.b[ks]);
+    const float norm = *reinterpret_cast<const float*>(
+        reinterpret_cast<const char*>(slot->norms) + qt * kNormSlot * 4 + l.norm);
+    g.hr = base + 16 * qt + (__lane_id() & 15u) < r_end ? 0.5f * norm : __builtin_inff();
+}
+
+// acc[qb] = -hq + dot over K = 128 of the wave's query block qb (16 queries) with the quarter's
+// 16 references: lane l's element e is query 16 qb + 4 (l / 16) + e, reference l % 16.
+__device__ __forceinline__ void gram_mfma(const bf16x8 (&a)[4][4], const f32x4 (&hneg)[4],
+                                          const GramQuarter& g, f32x4 (&acc)[4]) {
+#pragma unroll
+    for (uint32_t ks = 0; ks < 4; ++ks)
+#pragma unroll
+        for (uint32_t qb = 0; qb < 4; ++qb)
+            acc[qb] = mfma16(a[qb][ks], g.b[ks], ks ? acc[qb] : hneg[qb]);
+}
+
 struct GramHits {
     uint32_t* hit_q;
     uint32_t* hit_r;
@@ -779,99 +838,44 @@ struct GramHits {
     unsigned long long* count;
 };
 
-// The eight operand pieces (K = 128) of this lane's row in one reference half of a slot and the
-// row's squared norm, read by one asm statement that also waits for them. The screen's pipeline
-// has no LDS read the compiler can see, so its wait-count pass never drains the ring-slot DMA
-// in flight (whose ordering is the barrier waits); the LDS byte addresses come in registers.
-__device__ __forceinline__ void read_half(const uint32_t (&addr)[8], uint32_t norm_addr,
-                                          bf16x8 (&b)[8], float& norm) {
-    asm volatile(
-        "ds_read_b128 %0, %9\n\t"
-        "ds_read_b128 %1, %10\n\t"
-        "ds_read_b128 %2, %11\n\t"
-        "ds_read_b128 %3, %12\n\t"
-        "ds_read_b128 %4, %13\n\t"
-        "ds_read_b128 %5, %14\n\t"
-        "ds_read_b128 %6, %15\n\t"
-        "ds_read_b128 %7, %16\n\t"
-        "ds_read_b32 %8, %17\n\t"
-        "s_waitcnt lgkmcnt(0)"
-        : "=&v"(b[0]), "=&v"(b[1]), "=&v"(b[2]), "=&v"(b[3]), "=&v"(b[4]), "=&v"(b[5]),
-          "=&v"(b[6]), "=&v"(b[7]), "=&v"(norm)
-        : "v"(addr[0]), "v"(addr[1]), "v"(addr[2]), "v"(addr[3]), "v"(addr[4]), "v"(addr[5]),
-          "v"(addr[6]), "v"(addr[7]), "v"(norm_addr));
-}
-
-// MFMAs and hit extraction of the tile in `cur`, whose first reference is `base`.
-__device__ __forceinline__ void gram_tile(const GramSlot* cur, uint32_t* qq, uint32_t* qr,
-                                          uint32_t base, uint32_t r_end,
-                                          const bf16x8 (&a)[2][8], const f32x16 (&hneg)[2],
-                                          uint32_t q0, uint32_t& qn, const GramHits& out) {
-    const uint32_t lane = __lane_id(), li = lane & 31u, lk = lane >> 5;
-    const uint32_t slot = uint32_t(reinterpret_cast<uintptr_t>((lds_void)(const_cast<GramSlot*>(cur))));
+// The hits of a finished quarter, reference r in this lane: hr >= 0, and on non-negative floats
+// the float order is the order of the bit patterns as signed integers, below which every
+// negative float's pattern lies, so acc >= hr exactly when bits(acc) >= bits(hr) as int32. The
+// maxima are taken on the bit patterns (integer max, no NaN canonicalisation; the accumulators
+// hold no NaN). A quarter without hits costs the max tree and one compare.
+__device__ __forceinline__ void gram_hits(const f32x4 (&acc)[4], float hr, uint32_t r,
+                                          uint32_t q0, uint32_t* qq, uint32_t* qr, uint32_t& qn,
+                                          const GramHits& out) {
+    const uint32_t lane = __lane_id(), g4 = 4 * (lane >> 4);
+    const int32_t hr_i = __float_as_int(hr);
+    int32_t g[4];
 #pragma unroll
-    for (uint32_t h = 0; h < 2; ++h) {
-        f32x16 acc[2];  // [query block], reference half h
-        // piece 2 ks + lk of row h * 32 + li sits at byte rb XOR 32 ks of the slot's rows
-        const uint32_t row = h * 32 + li;
-        const uint32_t rb = row * 256 + (((lk ^ (li % 16)) & 1u) << 4) + ((li % 16) >> 1 << 5);
-        uint32_t addr[8];
+    for (uint32_t qb = 0; qb < 4; ++qb)
+        g[qb] = max(max(__float_as_int(acc[qb][0]), __float_as_int(acc[qb][1])),
+                    max(__float_as_int(acc[qb][2]), __float_as_int(acc[qb][3])));
+    const int32_t m = max(max(g[0], g[1]), max(g[2], g[3]));
+    if (!__any(m >= hr_i)) return;
 #pragma unroll
-        for (uint32_t ks = 0; ks < 8; ++ks) addr[ks] = slot + (rb ^ (ks << 5));
-        bf16x8 b[8];
-        float norm;
-        read_half(addr, slot + uint32_t(offsetof(GramSlot, norms)) +
-                            ((row / 16) * kNormSlot + row % 16) * 4,
-                  b, norm);
+    for (uint32_t qb = 0; qb < 4; ++qb) {
+        if (!__any(g[qb] >= hr_i)) continue;
 #pragma unroll
-        for (uint32_t ks = 0; ks < 8; ++ks) {
-            acc[0] = mfma(a[0][ks], b[ks], ks ? acc[0] : hneg[0]);
-            acc[1] = mfma(a[1][ks], b[ks], ks ? acc[1] : hneg[1]);
-        }
-        const uint32_t r = base + row;
-        const float hr = r < r_end ? 0.5f * norm : __builtin_inff();
-        // hr >= 0, and on non-negative floats the float order is the order of the bit patterns
-        // as signed integers, below which every negative float's pattern lies: acc >= hr
-        // exactly when bits(acc) >= bits(hr) as int32. The maxima are taken on the bit
-        // patterns (integer max, no NaN canonicalisation; the accumulators hold no NaN).
-        const int32_t hr_i = __float_as_int(hr);
-        int32_t g[2][4];  // maxima over the accumulators of rows 8k .. 8k + 7 of each block
-#pragma unroll
-        for (uint32_t b = 0; b < 2; ++b)
-#pragma unroll
-            for (uint32_t k = 0; k < 4; ++k)
-                g[b][k] = max(max(__float_as_int(acc[b][4 * k]), __float_as_int(acc[b][4 * k + 1])),
-                              max(__float_as_int(acc[b][4 * k + 2]),
-                                  __float_as_int(acc[b][4 * k + 3])));
-        const int32_t m = max(max(max(g[0][0], g[0][1]), max(g[0][2], g[0][3])),
-                              max(max(g[1][0], g[1][1]), max(g[1][2], g[1][3])));
-        if (!__any(m >= hr_i)) continue;
-#pragma unroll
-        for (uint32_t b = 0; b < 2; ++b) {
-#pragma unroll
-            for (uint32_t k = 0; k < 4; ++k) {
-                if (!__any(g[b][k] >= hr_i)) continue;
-#pragma unroll
-                for (uint32_t j = 0; j < 4; ++j) {
-                    const bool hit = acc[b][4 * k + j] >= hr;
-                    const uint64_t mask = __ballot(hit);
-                    if (!mask) continue;
-                    const uint32_t n = uint32_t(__popcll(mask));
-                    if (qn + n > kWaveQueue) {
-                        flush_queue(qq, qr, qn, out.hit_q, out.hit_r, out.cap, out.count);
-                        qn = 0;
-                    }
-                    if (hit) {
-                        const uint32_t pos =
-                            qn + __builtin_amdgcn_mbcnt_hi(
-                                     uint32_t(mask >> 32),
-                                     __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
-                        queue_store(qq + pos, q0 + b * 32 + k * 8 + lk * 4 + j);
-                        queue_store(qr + pos, r);
-                    }
-                    qn += n;
-                }
+        for (uint32_t e = 0; e < 4; ++e) {
+            const bool hit = acc[qb][e] >= hr;
+            const uint64_t mask = __ballot(hit);
+            if (!mask) continue;
+            const uint32_t n = uint32_t(__popcll(mask));
+            if (qn + n > kWaveQueue) {
+                flush_queue(qq, qr, qn, out.hit_q, out.hit_r, out.cap, out.count);
+                qn = 0;
             }
+            if (hit) {
+                const uint32_t pos =
+                    qn + __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+                queue_store(qq + pos, q0 + 16 * qb + g4 + e);
+                queue_store(qr + pos, r);
+            }
+            qn += n;
         }
     }
 }
@@ -883,14 +887,54 @@ __device__ __forceinline__ void gram_sync() {
     asm volatile("s_waitcnt vmcnt(10) lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
+// The running state of the quarter pipeline: the quarter whose MFMAs come next (its operands
+// read one quarter ahead) and the finished quarter whose hits are extracted after them, so
+// each wave's LDS reads and hit tests overlap its own MFMAs.
+struct GramRun {
+    GramQuarter cur;
+    f32x4 acc[4];  // the previous quarter's results
+    float hr;      // and its lane's threshold and reference
+    uint32_t r;
+};
+
+// Tile t (slot sl, quarter 0 already in run.cur) in four quarter steps; the last one first
+// waits for tile t + 1 (slot sn), refills slot sl with tile t + 4 once every wave is done
+// reading it, and reads tile t + 1's quarter 0.
+__device__ __forceinline__ void gram_tile(GramSlot* __restrict__ sl, GramSlot* __restrict__ sn,
+                                          uint32_t t, GramRun& run, const GramReadLanes& rl,
+                                          const GramDmaLanes& dl, __amdgpu_buffer_rsrc_t rows,
+                                          __amdgpu_buffer_rsrc_t norms, uint32_t r_begin,
+                                          uint32_t r_end, const bf16x8 (&a)[4][4],
+                                          const f32x4 (&hneg)[4], uint32_t q0, uint32_t* qq,
+                                          uint32_t* qr, uint32_t& qn, const GramHits& out) {
+    const uint32_t base = r_begin + t * kTileR, n16 = __lane_id() & 15u;
+#pragma unroll
+    for (uint32_t qt = 0; qt < 4; ++qt) {
+        GramQuarter nxt;
+        if (qt < 3) {
+            gram_read(sl, qt + 1, rl, base, r_end, nxt);
+        } else {
+            gram_sync();
+            gram_dma(sl, rows, norms, dl, t + 4);
+            gram_read(sn, 0, rl, base + kTileR, r_end, nxt);
+        }
+        f32x4 acc[4];
+        gram_mfma(a, hneg, run.cur, acc);
+        gram_hits(run.acc, run.hr, run.r, q0, qq, qr, qn, out);
+#pragma unroll
+        for (uint32_t qb = 0; qb < 4; ++qb) run.acc[qb] = acc[qb];
+        run.hr = run.cur.hr;
+        run.r = base + 16 * qt + n16;
+        run.cur = nxt;
+    }
+}
+
 // The pipeline over tiles [0, ntiles4) of the group (ntiles4 a multiple of 4; tiles past the
-// last real one hold no row before r_end and find no hits). Phase p of a round waits for tile
-// t + p (slot p), passes a barrier (every wave's share of that tile has landed, and every wave
-// is done with slot p - 1, the previous phase's), issues tile t + p + 3 into slot p - 1 and
-// computes slot p. The four slots are separate restrict pointers, which tells the compiler (and
-// its wait-count pass) that a DMA into one slot never aliases the reads of another, so the DMA
-// of later tiles stays in flight over the current tile's MFMAs; its ordering is the waits
-// above.
+// last real one hold no row before r_end and find no hits). Tile t sits in slot t % 4; the DMA
+// of tile t + 4 goes into it once every wave has read tile t's last quarter. The four slots are
+// separate restrict pointers, which tells the compiler (and its wait-count pass) that a DMA
+// into one slot never aliases the reads of another, so the DMA of later tiles stays in flight
+// over the current tile's MFMAs; its ordering is the waits in gram_sync.
 __device__ __forceinline__ void gram_pipeline(GramSlot* __restrict__ s0,
                                               GramSlot* __restrict__ s1,
                                               GramSlot* __restrict__ s2,
@@ -900,28 +944,33 @@ __device__ __forceinline__ void gram_pipeline(GramSlot* __restrict__ s0,
                                               __amdgpu_buffer_rsrc_t rows,
                                               __amdgpu_buffer_rsrc_t norms, uint32_t r_begin,
                                               uint32_t r_end, uint32_t ntiles4,
-                                              const bf16x8 (&a)[2][8], const f32x16 (&hneg)[2],
+                                              const bf16x8 (&a)[4][4], const f32x4 (&hneg)[4],
                                               uint32_t q0, uint32_t& qn, const GramHits& out) {
-    const GramDmaLanes l = gram_dma_lanes();
-    gram_dma(s0, rows, norms, l, 0);
-    gram_dma(s1, rows, norms, l, 1);
-    gram_dma(s2, rows, norms, l, 2);
+    const GramDmaLanes dl = gram_dma_lanes();
+    const GramReadLanes rl = gram_read_lanes();
+    gram_dma(s0, rows, norms, dl, 0);
+    gram_dma(s1, rows, norms, dl, 1);
+    gram_dma(s2, rows, norms, dl, 2);
+    gram_sync();  // tile 0 has landed
+    gram_dma(s3, rows, norms, dl, 3);
+    GramRun run;
+    gram_read(s0, 0, rl, r_begin, r_end, run.cur);
+#pragma unroll
+    for (uint32_t qb = 0; qb < 4; ++qb) run.acc[qb] = f32x4{-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+    run.hr = 0.f;
+    run.r = 0;
 #pragma unroll 1
     for (uint32_t t = 0; t < ntiles4; t += 4) {
-        const uint32_t b = r_begin + t * kTileR;
-        gram_sync();
-        gram_dma(s3, rows, norms, l, t + 3);
-        gram_tile(s0, qq, qr, b, r_end, a, hneg, q0, qn, out);
-        gram_sync();
-        gram_dma(s0, rows, norms, l, t + 4);
-        gram_tile(s1, qq, qr, b + kTileR, r_end, a, hneg, q0, qn, out);
-        gram_sync();
-        gram_dma(s1, rows, norms, l, t + 5);
-        gram_tile(s2, qq, qr, b + 2 * kTileR, r_end, a, hneg, q0, qn, out);
-        gram_sync();
-        gram_dma(s2, rows, norms, l, t + 6);
-        gram_tile(s3, qq, qr, b + 3 * kTileR, r_end, a, hneg, q0, qn, out);
+        gram_tile(s0, s1, t, run, rl, dl, rows, norms, r_begin, r_end, a, hneg, q0, qq, qr, qn,
+                  out);
+        gram_tile(s1, s2, t + 1, run, rl, dl, rows, norms, r_begin, r_end, a, hneg, q0, qq, qr,
+                  qn, out);
+        gram_tile(s2, s3, t + 2, run, rl, dl, rows, norms, r_begin, r_end, a, hneg, q0, qq, qr,
+                  qn, out);
+        gram_tile(s3, s0, t + 3, run, rl, dl, rows, norms, r_begin, r_end, a, hneg, q0, qq, qr,
+                  qn, out);
     }
+    gram_hits(run.acc, run.hr, run.r, q0, qq, qr, qn, out);  // the last quarter
 }
 
 // Grid: q_blocks x groups workgroups in group-major order, query block fastest, renumbered
@@ -945,10 +994,11 @@ __global__ __launch_bounds__(256, 2) void gram_kernel(const __bf16* __restrict__
                                                    uint32_t q_blocks, uint32_t* hit_q,
                                                    uint32_t* hit_r, uint64_t cap,
                                                    unsigned long long* count) {
-    __shared__ GramSlot ring[kRing];
+    // four separate objects, so that the compiler's alias analysis (and with it the wait-count
+    // pass) tells a DMA into one slot from the reads of another
+    __shared__ GramSlot slot0, slot1, slot2, slot3;
     __shared__ uint32_t queue_q[4][kWaveQueue], queue_r[4][kWaveQueue];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    const uint32_t li = lane & 31u, lk = lane >> 5;
     const uint32_t id = xcd_order(blockIdx.x, gridDim.x);
     const uint32_t qb = id % q_blocks, grp = id / q_blocks;
     const uint32_t q0 = qb * kGramQ + wave * 64;
@@ -959,20 +1009,26 @@ __global__ __launch_bounds__(256, 2) void gram_kernel(const __bf16* __restrict__
     uint32_t qn = 0;  // wave-uniform queue fill (waves past the last query find no hits)
     const GramHits out{hit_q, hit_r, cap, count};
 
-    bf16x8 a[2][8];
-    f32x16 hneg[2];
+    // query block b (16 queries): lane l holds query q0 + 16 b + l % 16, dims
+    // 32 ks + 8 (l / 16) .. + 7 in a[b][ks]; element e of its accumulator is query
+    // q0 + 16 b + 4 (l / 16) + e, whose -hq starts the chain
+    bf16x8 a[4][4];
+    f32x4 hneg[4];
+    // (every load unconditional, from a clamped row, so that none waits under a branch)
 #pragma unroll
-    for (uint32_t b = 0; b < 2; ++b) {
-        const uint32_t q = q0 + b * 32 + li;
+    for (uint32_t b = 0; b < 4; ++b) {
+        const uint32_t q = q0 + 16 * b + (lane & 15u), qc = q < nq ? q : nq - 1;
 #pragma unroll
-        for (uint32_t ks = 0; ks < 8; ++ks)
-            a[b][ks] = q < nq ? *reinterpret_cast<const bf16x8*>(Q + uint64_t(q) * kChannels +
-                                                                 ks * 16 + 8 * lk)
-                              : bf16x8{};
+        for (uint32_t ks = 0; ks < 4; ++ks) {
+            const bf16x8 v = *reinterpret_cast<const bf16x8*>(Q + uint64_t(qc) * kChannels +
+                                                              ks * 32 + 8 * (lane >> 4));
+            a[b][ks] = q < nq ? v : bf16x8{};
+        }
 #pragma unroll
-        for (uint32_t e = 0; e < 16; ++e) {
-            const uint32_t qe = q0 + b * 32 + (e / 4) * 8 + lk * 4 + (e % 4);
-            hneg[b][e] = qe < nq ? -0.5f * (q_norm[qe] - tau) : -__builtin_inff();
+        for (uint32_t e = 0; e < 4; ++e) {
+            const uint32_t qe = q0 + 16 * b + 4 * (lane >> 4) + e, qec = qe < nq ? qe : nq - 1;
+            const float v = q_norm[qec];
+            hneg[b][e] = qe < nq ? -0.5f * (v - tau) : -__builtin_inff();
         }
     }
     // the query operands are complete before the pipeline, whose waits count tile DMA only
@@ -984,7 +1040,7 @@ __global__ __launch_bounds__(256, 2) void gram_kernel(const __bf16* __restrict__
         int((r_end - r_begin) * kChannels * 2), kBufferFlags);
     const __amdgpu_buffer_rsrc_t norms = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<float*>(r_norm) + r_begin, 0, int((r_end - r_begin) * 4), kBufferFlags);
-    gram_pipeline(&ring[0], &ring[1], &ring[2], &ring[3], queue_q[wave], queue_r[wave], rows,
+    gram_pipeline(&slot0, &slot1, &slot2, &slot3, queue_q[wave], queue_r[wave], rows,
                   norms, r_begin, r_end, (ntiles + 3) / 4 * 4, a, hneg, q0, qn, out);
     if (qn) flush_queue(queue_q[wave], queue_r[wave], qn, hit_q, hit_r, cap, count);
     // no DMA may still be writing this workgroup's LDS when it ends
