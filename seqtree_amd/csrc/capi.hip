@@ -106,6 +106,24 @@ bool pools_supported() {
     return yes;
 }
 
+// Freed stream-ordered memory stays in the current device's default pool up to this many
+// bytes instead of going back to the driver at the next synchronisation (the pool's default
+// threshold is zero), so a call's scratch is mapped once and reused by the calls after it.
+constexpr uint64_t kPoolKeepBytes = uint64_t(4) << 30;
+
+void keep_pool_memory() {
+    constexpr int kMaxDevices = 64;
+    static std::once_flag once[kMaxDevices];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return;
+    std::call_once(once[dev], [dev] {
+        hipMemPool_t pool = nullptr;
+        uint64_t keep = kPoolKeepBytes;
+        if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess && pool)
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    });
+}
+
 // Stream-ordered scratch for one call. Everything allocated here is released on the same
 // stream when the arena goes out of scope (after a stream sync when pools are unavailable).
 // Requests of at most kSmall bytes are carved (256-byte aligned) out of shared blocks of
@@ -169,10 +187,12 @@ private:
 
     void* raw(size_t bytes) {
         void* p = nullptr;
-        if (pools_supported())
+        if (pools_supported()) {
+            keep_pool_memory();
             check(hipMallocAsync(&p, bytes, s_), "hipMallocAsync");
-        else
+        } else {
             check(hipMalloc(&p, bytes), "hipMalloc");
+        }
         bufs_.push_back(p);
         return p;
     }
@@ -1399,10 +1419,13 @@ void traceback_device(const uint8_t* q, const uint64_t* q_off, uint32_t q_max, c
     const bool wide = (int64_t(q_max) + r_max + 2) * mx >= (int64_t(1) << 28);
     // registers for the rows of short pairs, LDS rows up to 64 reference residues, else scratch
     const bool reg = !wide && r_max <= kTbRegCols && q_max <= kTbRegMaxQ;
+    // four bits per cell reproduce every back-pointer when gap_open >= gap_extend
+    const bool nib = reg && go >= ge;
     const bool lds_rows = !wide && r_max <= kTracebackLdsMaxLen;
     const uint64_t W = uint64_t(r_max) + 1;
-    const uint64_t dir_bytes = reg ? (uint64_t(q_max) + 1) * kTbRegGroups * 64 * 4
-                                   : (uint64_t(q_max) + 1) * W * 64;  // per workgroup
+    const uint64_t dir_bytes = nib   ? std::max<uint64_t>(q_max, 1) * kTbNibDwords * 64 * 4
+                               : reg ? (uint64_t(q_max) + 1) * kTbRegGroups * 64 * 4
+                                     : (uint64_t(q_max) + 1) * W * 64;  // per workgroup
     const uint64_t row_bytes = lds_rows ? 0 : 3 * W * 64 * (wide ? 8 : 4);
     const uint64_t need = (n_pairs + 63) / 64;
     const uint64_t fit = std::max<uint64_t>(1, (uint64_t(1) << 30) / (dir_bytes + row_bytes));
@@ -1447,10 +1470,12 @@ void traceback_device(const uint8_t* q, const uint64_t* q_off, uint32_t q_max, c
         a.order = order;
     }
     {
-        LaunchTimer t(std::string("traceback_") + (reg ? "reg" : wide ? "i64" : "i32") +
+        LaunchTimer t(std::string("traceback_") +
+                          (nib ? "nib" : reg ? "reg" : wide ? "i64" : "i32") +
                           (penalty ? "_penalty" : local ? "_local" : "_global"),
                       s);
-        if (reg) launch_traceback_reg(a, wgs, reinterpret_cast<uint32_t*>(dirs), s);
+        if (nib) launch_traceback_nib(a, wgs, reinterpret_cast<uint32_t*>(dirs), s);
+        else if (reg) launch_traceback_reg(a, wgs, reinterpret_cast<uint32_t*>(dirs), s);
         else launch_traceback(a, wide, lds_rows, wgs, rows, dirs, s);
     }
     launched("traceback");

@@ -394,6 +394,11 @@ constexpr uint32_t kTbRegGroups = (kTbRegCols + 1 + 3) / 4;
 constexpr uint32_t kTbRegMaxQ = 64;
 void launch_traceback_reg(const TracebackArgs& a, uint32_t workgroups, uint32_t* dirs,
                           hipStream_t s);
+// Nibble variant of the register kernel (gap_open >= gap_extend): dirs holds
+// q_max * kTbNibDwords * 64 dwords per workgroup.
+constexpr uint32_t kTbNibDwords = (kTbRegCols + 7) / 8;
+void launch_traceback_nib(const TracebackArgs& a, uint32_t workgroups, uint32_t* dirs,
+                          hipStream_t s);
 // Shape order for the traceback (n_pairs < 2^32): counting sort of the pairs by their
 // clamped (query, reference) lengths into order[n_pairs], over traceback_order_slices(n)
 // contiguous slices; table holds (slices + 1) * kTracebackShapeBins counters, keys n_pairs.

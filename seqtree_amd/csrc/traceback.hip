@@ -422,6 +422,199 @@ __global__ __launch_bounds__(kTbLanes) __attribute__((amdgpu_waves_per_eu(3, 8))
     }
 }
 
+// Nibble variant of the register kernel, for gap_open >= gap_extend: four bits per interior
+// cell (1 <= i <= m, 1 <= j <= n) instead of a direction byte, eight cells to a dword
+// ([q_max][kTbNibDwords][64 lanes] dwords per workgroup; row 0 and column 0 are not stored).
+// Bits 0-1 hold the state that is the cell's own maximum in the argmax order (pick(M, X, Y)),
+// or 3 for M when M = 0 in local mode (the local stop); bit 2 is set when X extends X
+// (pick(M_up - go, X_up - ge, Y_up - go) picked X), bit 3 when Y extends Y. That is all the
+// reference's traceback reads when gap_open >= gap_extend:
+//  * M at (i, j) came from pick(M, X, Y) at (i - 1, j - 1): that cell's own bits 0-1;
+//  * X at (i, j) that did not extend came from M or Y at (i - 1, j); of the two, pick takes Y
+//    exactly when Y_up > M_up. The up cell's own maximum is M when M_up >= X_up, Y_up and Y
+//    when Y_up > M_up, X_up, and it cannot be X: X_up > M_up, X_up >= Y_up gives
+//    X_up - ge > M_up - go and X_up - ge >= Y_up - go, so X would have extended. So the
+//    origin is the up cell's bits 0-1;
+//  * Y likewise with the left cell (pick order M, X, then Y: X over M exactly when
+//    X_left > M_left; a left cell whose maximum is Y makes Y extend).
+// The walk enters state M at a cell only through one of these pointers, which then name M as
+// that cell's maximum, so the local stop (M = 0) can share bits 0-1 with it; the walk's first
+// cell in local mode holds the best M > 0. Row 0 and column 0 take their maxima and extension
+// bits from the boundary values (global: Y along row 0 and X down column 0, each opening at
+// its first step; local: M). Same recurrences, results, ops and ties as traceback_reg_kernel,
+// with half its direction stores and none for row 0.
+template <bool LOCAL>
+__global__ __launch_bounds__(kTbLanes) __attribute__((amdgpu_waves_per_eu(3, 8))) void traceback_nib_kernel(TracebackArgs a,
+                                                                 uint32_t* __restrict__ dirs_g) {
+    constexpr uint32_t NC = kTbRegCols;
+    constexpr uint32_t RW = kTbNibDwords;
+    extern __shared__ int32_t tb_lds[];
+    const uint32_t lane = threadIdx.x;
+    int32_t* sim_l = tb_lds;
+    for (uint32_t e = lane; e < a.A * a.A; e += kTbLanes) sim_l[e] = a.sim[e];
+    __syncthreads();  // the only barrier: the pair loop below has none
+    uint8_t* ops_l = reinterpret_cast<uint8_t*>(tb_lds + kMaxAlphabet * kMaxAlphabet);
+    uint32_t* dirs = dirs_g + uint64_t(blockIdx.x) * a.q_max * RW * kTbLanes + lane;
+    // the four bits of interior cell (i, j), i, j >= 1
+    auto nib = [&](uint32_t i, uint32_t j) -> uint32_t {
+        const uint32_t w = dirs[(uint64_t(i - 1) * RW + (j - 1) / 8) * kTbLanes];
+        return (w >> (4 * ((j - 1) % 8))) & 0xFu;
+    };
+    // the four bits of cell (i, j) when it is interior, else 0
+    auto cell = [&](uint32_t i, uint32_t j) -> uint32_t { return i && j ? nib(i, j) : 0u; };
+    // the state that is cell (i, j)'s maximum from its bits h, boundary cells included (3 reads
+    // as M)
+    auto top = [&](uint32_t i, uint32_t j, uint32_t h) -> uint8_t {
+        if (i == 0 || j == 0) return LOCAL || (i == 0 && j == 0) ? kStM : i == 0 ? kStY : kStX;
+        h &= 3u;
+        return h == 3u ? kStM : uint8_t(h);
+    };
+    constexpr int32_t NEG = NegInf<int32_t>::v;
+    const int32_t go = a.gap_open, ge = a.gap_extend;
+    const uint64_t stride = uint64_t(gridDim.x) * kTbLanes;
+
+    for (uint64_t t = uint64_t(blockIdx.x) * kTbLanes + lane; t < a.n_pairs; t += stride) {
+        const uint64_t p = a.order ? a.order[t] : t;
+        const uint32_t qi = a.pair_q[p], ri = a.pair_r[p];
+        const uint64_t q0 = a.q_off[qi], r0 = a.r_off[ri];
+        const uint64_t m64 = a.q_off[qi + 1] - q0, n64 = a.r_off[ri + 1] - r0;
+        if (m64 > a.q_max || n64 > a.r_max) {  // r_max <= kTbRegCols, q_max <= kTbRegMaxQ
+            skip_pair(a, p);
+            continue;
+        }
+        const uint32_t m = uint32_t(m64), n = uint32_t(n64);
+        const uint8_t* q = a.q_codes + q0;
+        const uint8_t* r = a.r_codes + r0;
+        uint8_t* ops = a.ops + p * a.ops_stride;
+        uint32_t cnt = 0, i = 0, j = 0;
+        int32_t score;
+        if (m == 0 || n == 0) {
+            score = LOCAL ? 0 : -gap_run<int32_t>(m ? m : n, go, ge);
+            if (!LOCAL) {
+                for (uint32_t t2 = 0; t2 < m; ++t2) ops[cnt++] = 'D';
+                for (uint32_t t2 = 0; t2 < n; ++t2) ops[cnt++] = 'I';
+            }
+        } else {
+            uint32_t rcw[NC / 4];
+#pragma unroll
+            for (uint32_t w = 0; w < NC / 4; ++w) rcw[w] = 0;
+#pragma unroll
+            for (uint32_t c = 0; c < NC; ++c)
+                if (c < n) rcw[c / 4] |= uint32_t(r[c]) << (8 * (c % 4));
+            int32_t rm[NC + 1], rx[NC + 1], ry[NC + 1];  // the row above, updated in place
+            rm[0] = 0;
+            rx[0] = NEG;
+            ry[0] = NEG;
+#pragma unroll
+            for (uint32_t c = 1; c <= NC; ++c) {
+                rm[c] = LOCAL ? 0 : NEG;
+                rx[c] = NEG;
+                ry[c] = LOCAL ? NEG : -gap_run<int32_t>(c, go, ge);
+            }
+            int32_t best = LOCAL ? 0 : NEG;
+            uint32_t bi = 0, bj = 0;
+            uint32_t qc_next = q[0];
+            for (uint32_t row = 1; row <= m; ++row) {
+                const int32_t* srow = sim_l + qc_next * a.A;
+                if (row < m) qc_next = q[row];
+                uint32_t* drow = dirs + uint64_t(row - 1) * RW * kTbLanes;
+                int32_t dm = rm[0], dx = rx[0], dy = ry[0];
+                int32_t lm = LOCAL ? 0 : NEG;
+                int32_t lx = LOCAL ? NEG : -gap_run<int32_t>(row, go, ge);
+                int32_t ly = NEG;
+                rm[0] = lm;
+                rx[0] = lx;
+                ry[0] = ly;
+                uint32_t acc = 0, done = 0;
+#pragma unroll
+                for (uint32_t c = 1; c <= NC; ++c) {
+                    if (c % 4 == 1 && c > 1 && !__any(c <= n)) break;  // uniform
+                    const uint32_t rc = (rcw[(c - 1) / 4] >> (8 * ((c - 1) % 4))) & 0xFFu;
+                    const int32_t um = rm[c], ux = rx[c], uy = ry[c];
+                    int32_t mv = max(max(dm, dx), dy) + srow[rc];
+                    if (LOCAL) mv = max(mv, 0);
+                    const int32_t xm = um - go, xx = ux - ge, xy = uy - go;
+                    const int32_t xv = max(max(xm, xx), xy);
+                    const uint32_t ext_x = (xx > xm) & (xx >= xy);
+                    const int32_t ym = lm - go, yx = lx - go, yy = ly - ge;
+                    const int32_t yv = max(max(ym, yx), yy);
+                    const uint32_t ext_y = (yy > ym) & (yy >= yx);
+                    // the cell's own maximum in pick's order (M, X, Y; strictly larger wins)
+                    uint32_t hd = xv > mv ? (yv > xv ? kStY : kStX) : (yv > mv ? kStY : kStM);
+                    if (LOCAL && hd == kStM && mv == 0) hd = 3u;
+                    if (LOCAL && c <= n && mv > best) {
+                        best = mv;
+                        bi = row;
+                        bj = c;
+                    }
+                    rm[c] = mv;
+                    rx[c] = xv;
+                    ry[c] = yv;
+                    acc |= (hd | (ext_x << 2) | (ext_y << 3)) << (4 * ((c - 1) % 8));
+                    if (c % 8 == 0) {
+                        drow[uint64_t(c / 8 - 1) * kTbLanes] = acc;
+                        acc = 0;
+                    }
+                    done = c;
+                    dm = um;
+                    dx = ux;
+                    dy = uy;
+                    lm = mv;
+                    lx = xv;
+                    ly = yv;
+                }
+                if (done % 8 != 0) drow[uint64_t(done / 8) * kTbLanes] = acc;  // partial dword
+            }
+            uint8_t st;
+            bool run = true;
+            if (LOCAL) {
+                score = best;
+                i = bi;
+                j = bj;
+                st = kStM;
+                run = best != 0;
+            } else {
+                int32_t fm = rm[0], fx = rx[0], fy = ry[0];
+#pragma unroll
+                for (uint32_t c = 1; c <= NC; ++c)
+                    if (c == n) {
+                        fm = rm[c];
+                        fx = rx[c];
+                        fy = ry[c];
+                    }
+                score = pick(fm, fx, fy, st);
+                i = m;
+                j = n;
+            }
+            // h: the bits of the walk's current cell, loaded once when the walk enters it. Each
+            // step is written without branches on the state (the loads from clamped indices,
+            // which m, n >= 1 keep inside the pair), so that the lanes of a wave, whose walks
+            // take different turns, stay on one path.
+            uint32_t h = run ? cell(i, j) : 0u;
+            while (run) {
+                const bool sm = st == kStM, sx = st == kStX, sy = !sm && !sx;
+                if ((i == 0 && j == 0) || (sm && (i == 0 || j == 0)) || (sx && i == 0) ||
+                    (sy && j == 0) || (LOCAL && sm && (h & 3u) == 3u))
+                    break;  // the corner, a move off the matrix, or M = 0: the local stop
+                const uint8_t qc = q[i ? i - 1 : 0], rc = r[j ? j - 1 : 0];
+                const uint8_t op = sm ? (qc == rc ? 'M' : 'S') : sx ? 'D' : 'I';
+                const bool ext = sx   ? (j == 0 ? i > 1 : ((h >> 2) & 1u) != 0)
+                                 : sy ? (i == 0 ? j > 1 : ((h >> 3) & 1u) != 0)
+                                      : false;
+                i -= uint32_t(sm || sx);
+                j -= uint32_t(sm || sy);
+                const uint32_t w = nib(i ? i : 1u, j ? j : 1u);
+                h = i && j ? w : 0u;
+                st = ext ? st : top(i, j, h);
+                ops_l[cnt * kTbLanes + lane] = op;
+                ++cnt;
+            }
+            for (uint32_t t2 = 0; t2 < cnt; ++t2) ops[t2] = ops_l[(cnt - 1 - t2) * kTbLanes + lane];
+        }
+        put_result(a, p, int64_t(score), cnt, i, j);
+    }
+}
+
 // Shape order of the pairs: a counting sort by (min(m, 31), min(n, 31)), so that the lanes of
 // a wave get pairs of one shape and run the same number of DP cells and traceback steps.
 // Workgroup b owns a contiguous slice of the list. Count: an LDS histogram of the slice's
@@ -565,6 +758,19 @@ void launch_traceback_reg(const TracebackArgs& a, uint32_t workgroups, uint32_t*
                            a, dirs);
     else
         hipLaunchKernelGGL(traceback_reg_kernel<false>, dim3(workgroups), dim3(kTbLanes), lds, s,
+                           a, dirs);
+}
+
+void launch_traceback_nib(const TracebackArgs& a, uint32_t workgroups, uint32_t* dirs,
+                          hipStream_t s) {
+    if (a.n_pairs == 0 || workgroups == 0) return;
+    const size_t lds = kMaxAlphabet * kMaxAlphabet * sizeof(int32_t) +
+                       size_t(a.q_max + a.r_max) * kTbLanes;
+    if (a.local)
+        hipLaunchKernelGGL(traceback_nib_kernel<true>, dim3(workgroups), dim3(kTbLanes), lds, s,
+                           a, dirs);
+    else
+        hipLaunchKernelGGL(traceback_nib_kernel<false>, dim3(workgroups), dim3(kTbLanes), lds, s,
                            a, dirs);
 }
 

@@ -92,7 +92,8 @@ def test_align_matches_the_reference(gpu, reference, matrix, go, ge):
 
 
 def test_batch_kernels_equal_the_oracle(gpu, oracle_lib):
-    """One batch per kernel: CDR3-like pairs (register kernel), references of 33-64 residues
+    """One batch per kernel: CDR3-like pairs (register kernels: four bits per cell when
+    gap_open >= gap_extend, direction bytes otherwise), references of 33-64 residues
     (LDS rows), longer ones (rows in scratch), absurd gap costs (int64 cells) and a list of
     more than 4,096 pairs (shape order); every output at its own index."""
     st = gpu
@@ -100,15 +101,17 @@ def test_batch_kernels_equal_the_oracle(gpu, oracle_lib):
     pen = _pen(st, m)
     rng = random.Random(77)
     cases = [
-        ("traceback_reg_penalty", cdr3_like(rng, 200, 6, 30), cdr3_like(rng, 300, 6, 30),
+        ("traceback_nib_penalty", cdr3_like(rng, 200, 6, 30), cdr3_like(rng, 300, 6, 30),
          11, 1, 3000),
+        ("traceback_reg_penalty", cdr3_like(rng, 200, 6, 30), cdr3_like(rng, 300, 6, 30),
+         2, 5, 3000),
         ("traceback_i32_penalty", cdr3_like(rng, 60, 20, 70), [rand_seq(rng, 33, 64)
                                                                for _ in range(50)], 11, 1, 700),
         ("traceback_i32_penalty", [rand_seq(rng, 1, 90) for _ in range(40)],
          [rand_seq(rng, 65, 120) for _ in range(30)], 5, 2, 300),
         ("traceback_i64_penalty", cdr3_like(rng, 40, 6, 30), cdr3_like(rng, 40, 6, 30),
          20_000_000, 3_000_000, 400),
-        ("traceback_reg_penalty", cdr3_like(rng, 500, 6, 30), cdr3_like(rng, 800, 6, 30),
+        ("traceback_nib_penalty", cdr3_like(rng, 500, 6, 30), cdr3_like(rng, 800, 6, 30),
          14, 1, 6000),
     ]
     for label, qs, refs, go, ge, n in cases:

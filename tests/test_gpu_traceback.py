@@ -101,11 +101,13 @@ def test_large_lists_run_in_shape_order(gpu, oracle_lib):
             assert (g.aligned_query, g.aligned_ref) == _strings(a, b, ops, qb, rb), t
 
 
-@pytest.mark.parametrize("go,ge", [(11, 1), (1, 3), (0, 0)])
+@pytest.mark.parametrize("go,ge", [(11, 1), (5, 5), (2, 1), (1, 3), (0, 0)])
 def test_register_rows_kernel(gpu, oracle_lib, go, ge):
     """References of at most 32 residues and queries of at most 64 take the register-row
-    kernel: every reference length 1..32 (the 4-column groups and the last partial one),
-    queries up to 64, lists below and above the shape-order threshold."""
+    kernels: every reference length 1..32 (the 4- and 8-column groups and the last partial
+    ones), queries up to 64, lists below and above the shape-order threshold. gap_open >=
+    gap_extend takes the four-bit kernel (go = ge included), gap_open < gap_extend the
+    direction-byte one."""
     st = gpu
     m = st.SubstitutionMatrix.blosum62()
     sim = _sim(st, m)
@@ -120,7 +122,8 @@ def test_register_rows_kernel(gpu, oracle_lib, go, ge):
         for mode in ("global", "local"):
             got, labels = launch_labels(lambda: st.pairwise.align_pairs(
                 q, r, pq, pr, m, mode=mode, gap_open=go, gap_extend=ge))
-            assert f"traceback_reg_{mode}" in labels, labels
+            kernel = "nib" if go >= ge else "reg"
+            assert f"traceback_{kernel}_{mode}" in labels, labels
             for t in range(0, n, 1 if n < 1000 else 3):
                 a, b = q[pq[t]], r[pr[t]]
                 s_, ops, qb, rb = oracle_lib.align_pair(a, b, sim, mode == "local", go, ge)
