@@ -184,6 +184,37 @@ def test_encode_empty_and_longest_sequences(gpu):
     model.close()
 
 
+def test_encode_across_blocks(gpu):
+    """450,000 sequences of length 0 .. 20 take three encoder blocks of up to 2^22 rows (the
+    persistent kernel's tiles, with their inputs fetched ahead, run many per workgroup): the
+    sequences at and beside the block boundaries and a random sample equal the numpy model;
+    every norm is its embedding's."""
+    from seqtree_amd.prefilter import DeviceSet, Prefilter, _bf16_to_f32
+
+    rng = np.random.default_rng(31)
+    model = Prefilter(seed=6)
+    params = rand_params(rng, model.n_params)
+    model.load(params)
+    n = 450_000
+    seqs, codes, off = rand_set(rng, n, 0, 20)
+    emb, norm = model.encode(DeviceSet.upload(codes, off), 20)
+    got_all = emb.numpy()
+    norm_all = norm.numpy()
+    blk = (1 << 22) // 22  # sequences per block at max_len 20
+    edges = [0, blk, 2 * blk, n]
+    idx = sorted({i for e in edges for i in range(e - 4, e + 4) if 0 <= i < n} |
+                 set(rng.integers(0, n, 400).tolist()))
+    got = _bf16_to_f32(got_all[idx])
+    want, _ = forward(params, [seqs[i] for i in idx])
+    assert np.allclose(got, want, rtol=3e-2, atol=3e-2 * np.abs(want).max())
+    sample = np.unique(np.concatenate([idx, rng.integers(0, n, 20_000)]))
+    g = _bf16_to_f32(got_all[sample]).astype(np.float64)
+    assert np.allclose(norm_all[sample], (g ** 2).sum(1), rtol=1e-4, atol=1e-6)
+    empty = np.array([len(seqs[i]) == 0 for i in sample])
+    assert empty.any() and not g[empty].any()
+    model.close()
+
+
 def test_training_gradients_match_numpy_model(gpu):
     from seqtree_amd._device import DeviceArray
     from seqtree_amd.prefilter import DeviceSet, Prefilter
