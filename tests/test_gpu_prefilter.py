@@ -147,6 +147,31 @@ def test_screen_matches_numpy(gpu):
     assert n == len(want) and np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("nq,nr", [(1, 1), (1, 15), (3, 16), (17, 63), (65, 64), (64, 65),
+                                   (2, 257)])
+def test_screen_small_shapes(gpu, nq, nr):
+    """Query and reference counts around the screen's blocks (16 queries, 64 per wave, 16-row
+    quarters, 64-row tiles, four-tile ring rounds): every pair within tau, exactly."""
+    from seqtree_amd._device import DeviceArray
+    from seqtree_amd.prefilter import screen
+
+    rng = np.random.default_rng(nq * 1000 + nr)
+    Q = (rng.integers(-8, 9, size=(nq, C)) / 8.0).astype(np.float32)
+    R = (rng.integers(-8, 9, size=(nr, C)) / 8.0).astype(np.float32)
+    qn = (Q.astype(np.float64) ** 2).sum(1)
+    rn = (R.astype(np.float64) ** 2).sum(1)
+    d = qn[:, None] + rn[None, :] - 2 * (Q.astype(np.float64) @ R.astype(np.float64).T)
+    tau = float(np.median(d) // (1 / 64) * (1 / 64))
+    want = np.argwhere(d <= tau)
+    hq, hr, n = screen(DeviceArray.from_numpy(to_bf16_bits(Q)),
+                       DeviceArray.from_numpy(qn.astype(np.float32)), nq,
+                       DeviceArray.from_numpy(to_bf16_bits(R)),
+                       DeviceArray.from_numpy(rn.astype(np.float32)), nr, tau, 8)
+    got = np.stack([hq.numpy()[:n], hr.numpy()[:n]], 1).astype(np.int64)
+    got = got[np.lexsort((got[:, 1], got[:, 0]))]
+    assert n == len(want) and np.array_equal(got, want)
+
+
 def test_encode_matches_numpy_model(gpu):
     from seqtree_amd.prefilter import DeviceSet, Prefilter, _bf16_to_f32
 
@@ -181,6 +206,28 @@ def test_encode_empty_and_longest_sequences(gpu):
     empty = np.array([len(q) == 0 for q in seqs])
     assert empty.any() and not got[empty].any()
     assert np.allclose(norm.numpy(), (got.astype(np.float64) ** 2).sum(1), rtol=1e-4)
+    model.close()
+
+
+@pytest.mark.parametrize("lens", [[7], [0], [0, 0, 3], [64, 0, 64], [20] * 7])
+def test_encode_tiny_sets(gpu, lens):
+    """Sets of one to seven sequences: a single tile holding every sequence, empty ones
+    included, and sequences that run past the first tile's end."""
+    from seqtree_amd.prefilter import DeviceSet, Prefilter, _bf16_to_f32
+
+    rng = np.random.default_rng(sum(lens) + len(lens))
+    model = Prefilter(seed=3)
+    params = rand_params(rng, model.n_params)
+    model.load(params)
+    seqs = [rng.integers(0, 20, size=L).astype(np.uint8) for L in lens]
+    off = np.zeros(len(lens) + 1, np.uint64)
+    off[1:] = np.cumsum(lens)
+    codes = np.concatenate(seqs) if sum(lens) else np.zeros(0, np.uint8)
+    emb, norm = model.encode(DeviceSet.upload(codes, off), max(max(lens), 1))
+    got = _bf16_to_f32(emb.numpy())
+    want, _ = forward(params, seqs)
+    assert np.allclose(got, want, rtol=3e-2, atol=3e-2 * max(np.abs(want).max(), 1e-6))
+    assert np.allclose(norm.numpy(), (got.astype(np.float64) ** 2).sum(1), rtol=1e-4, atol=1e-6)
     model.close()
 
 
