@@ -39,6 +39,7 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(__bf16 v) { return float(v); }
 __device__ __forceinline__ __bf16 f2bf(float v) { return __bf16(v); }
@@ -256,11 +257,12 @@ constexpr uint32_t kRingDepth = 5;
 // accumulators' rows (A as the MFMA's first operand), otherwise the channels. On entry the ring
 // holds this layer's first kRingDepth weight fragments (Wt); k-step ks consumes its slot and
 // refills it with the fragment kRingDepth k-steps further on in the stream, which past this
-// layer's last k-step is the next layer's (Wn, CIN_NEXT input channels).
+// layer's last k-step is the next layer's (Wn, CIN_NEXT input channels). The accumulators start
+// from `init` (the layer's bias in the accumulator layout), taken as the first MFMA's addend.
 template <uint32_t CIN, uint32_t NBLK, bool ROWS, uint32_t CIN_NEXT>
 __device__ __forceinline__ void enc_layer(const char* in, uint32_t stride, uint32_t clamp,
                                           const __bf16* __restrict__ Wt,
-                                          const __bf16* __restrict__ Wn,
+                                          const __bf16* __restrict__ Wn, const f32x16& init,
                                           bf16x8 (&ring)[kRingDepth], f32x16 (&acc)[NBLK]) {
     constexpr uint32_t K = 5 * CIN, KS = K / 16, P = kRingDepth, KN = 5 * CIN_NEXT;
     static_assert(KS % P == 0 && KN / 16 >= P, "ring slots follow every layer's k-steps");
@@ -272,7 +274,6 @@ __device__ __forceinline__ void enc_layer(const char* in, uint32_t stride, uint3
     for (uint32_t b = 0; b < NBLK; ++b) {
         const uint32_t j = 32 * b + li;
         arow[b] = in + (j < clamp ? j : clamp) * stride + 16 * lk;
-        acc[b] = f32x16{0.f};
     }
     // the activation fragments of k-step ks + 1 are read while the MFMAs of k-step ks run: the
     // scheduling barriers keep the compiler from sinking each read to just before its MFMA,
@@ -295,8 +296,10 @@ __device__ __forceinline__ void enc_layer(const char* in, uint32_t stride, uint3
         __builtin_amdgcn_sched_barrier(0);
         const bf16x8 bw = ring[ks % P];
 #pragma unroll
-        for (uint32_t b = 0; b < NBLK; ++b)
-            acc[b] = ROWS ? mfma(cur[b], bw, acc[b]) : mfma(bw, cur[b], acc[b]);
+        for (uint32_t b = 0; b < NBLK; ++b) {
+            const f32x16& c = ks == 0 ? init : acc[b];
+            acc[b] = ROWS ? mfma(cur[b], bw, c) : mfma(bw, cur[b], c);
+        }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (uint32_t b = 0; b < NBLK; ++b) cur[b] = nxt[b];
@@ -304,34 +307,56 @@ __device__ __forceinline__ void enc_layer(const char* in, uint32_t stride, uint3
     }
 }
 
-// acc + bias, ReLU and the frame's valid mask (bytes, one per frame row), rounded to BF16 into
-// `out` (LDS, row stride kHStride); the last block stores its first 8 LAST rows only (the rows
-// past those are not needed). Lane (li, lk) holds row 32 b + li of block b and, in group g,
-// the channels 32 w + 8 g + 4 lk + u, u < 4: one 8-byte store per group.
+// The bias of a layer whose accumulators hold channels in their rows (lane (li, lk), element
+// 4 g + u: channel 32 w + 8 g + 4 lk + u), or one channel per lane (32 w + li).
+__device__ __forceinline__ f32x16 enc_bias_channels(const float* bias) {
+    const uint32_t lane = enc_lane(), lk = lane >> 5, w = enc_wave();
+    f32x16 v;
+#pragma unroll
+    for (uint32_t g = 0; g < 4; ++g) {
+        const float4 b4 = *reinterpret_cast<const float4*>(bias + 32 * w + 8 * g + 4 * lk);
+        v[4 * g] = b4.x;
+        v[4 * g + 1] = b4.y;
+        v[4 * g + 2] = b4.z;
+        v[4 * g + 3] = b4.w;
+    }
+    return v;
+}
+__device__ __forceinline__ f32x16 enc_bias_lane(const float* bias) {
+    const float b = bias[32 * enc_wave() + (enc_lane() & 31u)];
+    f32x16 v;
+#pragma unroll
+    for (uint32_t e = 0; e < 16; ++e) v[e] = b;
+    return v;
+}
+
+// ReLU and the frame's valid mask in one step: `cap` is +inf on a valid row and 0 on any
+// other, so med3(v, 0, cap) is max(v, 0) on valid rows and 0 elsewhere.
+__device__ __forceinline__ float enc_relu(float v, float cap) {
+    return __builtin_amdgcn_fmed3f(v, 0.f, cap);
+}
+
+// The accumulators (bias included), after ReLU and the frame's valid mask (`cap`, one float
+// per frame row), rounded to BF16 into `out` (LDS, row stride kHStride); the last block
+// stores its first 8 LAST rows only (the rows past those are not needed). Lane (li, lk) holds
+// row 32 b + li of block b and, in group g, the channels 32 w + 8 g + 4 lk + u, u < 4: one
+// 8-byte store per group.
 template <uint32_t NBLK, uint32_t LAST>
-__device__ __forceinline__ void enc_store(const f32x16 (&acc)[NBLK], const float* bias,
-                                          const uint8_t* mask, char* out) {
+__device__ __forceinline__ void enc_store(const f32x16 (&acc)[NBLK], const float* cap,
+                                          char* out) {
     const uint32_t lane = enc_lane(), li = lane & 31u, lk = lane >> 5, w = enc_wave();
     const uint32_t c0 = 32 * w + 4 * lk;
-    float bv[4][4];
-#pragma unroll
-    for (uint32_t g = 0; g < 4; ++g)
-#pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) bv[g][u] = bias[c0 + 8 * g + u];
 #pragma unroll
     for (uint32_t b = 0; b < NBLK; ++b) {
         const uint32_t j = 32 * b + li;
         if (b + 1 == NBLK && li >= 8 * LAST) continue;
-        const bool row_ok = mask[j] != 0;
+        const float hi = cap[j];
         char* dst = out + j * kHStride + c0 * 2;
 #pragma unroll
         for (uint32_t g = 0; g < 4; ++g) {
             bf16x4 pk;
 #pragma unroll
-            for (uint32_t u = 0; u < 4; ++u) {
-                const float v = acc[b][4 * g + u] + bv[g][u];
-                pk[u] = f2bf(v > 0.f && row_ok ? v : 0.f);
-            }
+            for (uint32_t u = 0; u < 4; ++u) pk[u] = f2bf(enc_relu(acc[b][4 * g + u], hi));
             *reinterpret_cast<bf16x4*>(dst + 16 * g) = pk;
         }
     }
@@ -368,8 +393,9 @@ static_assert(32 * kStageStride * 4 <= kEncRegionB, "the sums' stage fits the H2
 // Mean pool of the tile from conv3's accumulators, as MFMAs: sums[s][c] = sum over the tile's
 // rows r of M[s][r] H3[r][c], M the 0/1 membership of row r in sequence s's residue rows.
 // conv3 runs with the activations as the first MFMA operand, so lane (li, lk) of wave w holds
-// channel c = 32 w + li of the rows 32 b + 8 g + 4 lk + u (element 4 g + u of block b). After
-// bias, ReLU, the frame mask and the BF16 rounding (the values the unfused layers store),
+// channel c = 32 w + li of the rows 32 b + 8 g + 4 lk + u (element 4 g + u of block b), bias
+// included. After ReLU, the frame mask (`cap`) and the BF16 rounding (the values the unfused
+// layers store),
 // elements 8 h .. 8 h + 7 of block b are the lane's second operand for k-step 2 b + h, k slot i
 // standing for row 32 b + 8 (2 h + i / 4) + 4 lk + i % 4; the membership operand uses the same
 // row for each k slot, so every row of the tile enters each sum once. Lane (li, lk) of the
@@ -380,42 +406,48 @@ static_assert(32 * kStageStride * 4 <= kEncRegionB, "the sums' stage fits the H2
 // tree), the one cut by the tile's start leaves its tail sums and the one cut by its end its
 // head sums, which encode_finish_kernel joins. Every thread of the workgroup calls this with
 // the same cnt.
-__device__ __forceinline__ void enc_pool(const f32x16 (&acc)[4], const float* bias,
-                                         const uint8_t* mask, const EncSeqs& seq, uint32_t cnt,
-                                         uint32_t s0, uint32_t t, float* stage,
-                                         const EncPool& p) {
+__device__ __forceinline__ void enc_pool(const f32x16 (&acc)[4], const float* cap,
+                                         const EncSeqs& seq, uint32_t cnt, uint32_t s0,
+                                         uint32_t t, float* stage, const EncPool& p) {
     const uint32_t lane = enc_lane(), tid = 64 * enc_wave() + lane, li = lane & 31u, lk = lane >> 5;
     const uint32_t c = 32 * (tid >> 6) + li;
-    const float bc = bias[c];
     bf16x8 hb[4][2];
 #pragma unroll
     for (uint32_t b = 0; b < 4; ++b)
 #pragma unroll
         for (uint32_t g = 0; g < 4; ++g) {
-            const uint32_t m4 = *reinterpret_cast<const uint32_t*>(mask + 32 * b + 8 * g + 4 * lk);
+            const float4 c4 = *reinterpret_cast<const float4*>(cap + 32 * b + 8 * g + 4 * lk);
+            const float cu[4] = {c4.x, c4.y, c4.z, c4.w};
 #pragma unroll
-            for (uint32_t u = 0; u < 4; ++u) {
-                const float v = acc[b][4 * g + u] + bc;
-                hb[b][g / 2][4 * (g % 2) + u] = f2bf(v > 0.f && ((m4 >> (8 * u)) & 0xFFu) ? v : 0.f);
-            }
+            for (uint32_t u = 0; u < 4; ++u)
+                hb[b][g / 2][4 * (g % 2) + u] = f2bf(enc_relu(acc[b][4 * g + u], cu[u]));
         }
     for (uint32_t mb = 0; mb < (cnt + 31) / 32; ++mb) {
         const uint32_t jm = 32 * mb + li;
         const int32_t lo = jm < cnt ? seq.lo[jm] : 0;
         const int32_t hi = jm < cnt ? lo + int32_t(seq.len[jm]) : 0;
+        // the sequence's rows [lo, hi) of the tile as a 128-bit mask, 32 rows per word
+        const auto below = [](int32_t k) -> uint32_t {  // bits [0, k), k clamped to [0, 32]
+            return k <= 0 ? 0u : k >= 32 ? 0xFFFFFFFFu : (1u << k) - 1u;
+        };
         f32x16 ps = f32x16{0.f};
 #pragma unroll
-        for (uint32_t b = 0; b < 4; ++b)
+        for (uint32_t b = 0; b < 4; ++b) {
+            const uint32_t word = below(hi - int32_t(32 * b)) & ~below(lo - int32_t(32 * b));
 #pragma unroll
             for (uint32_t h = 0; h < 2; ++h) {
-                bf16x8 pa;
+                // k slots 4 q + u stand for row 32 b + 16 h + 8 q + 4 lk + u: two slots per
+                // dword, BF16 1.0 (0x3F80) where the row's bit is set
+                const uint32_t x = word >> (16 * h + 4 * lk);
+                u32x4 pk;
 #pragma unroll
-                for (uint32_t i = 0; i < 8; ++i) {
-                    const int32_t row = int32_t(32 * b + 8 * (2 * h + i / 4) + 4 * lk + i % 4);
-                    pa[i] = f2bf(row >= lo && row < hi ? 1.f : 0.f);
+                for (uint32_t d = 0; d < 4; ++d) {
+                    const uint32_t two = (x >> (8 * (d / 2) + 2 * (d % 2))) & 3u;
+                    pk[d] = ((two * 0x8001u) & 0x10001u) * 0x3F80u;
                 }
-                ps = mfma(pa, hb[b][h], ps);
+                ps = mfma(__builtin_bit_cast(bf16x8, pk), hb[b][h], ps);
             }
+        }
 #pragma unroll
         for (uint32_t e = 0; e < 16; ++e)
             stage[(8 * (e / 4) + 4 * lk + e % 4) * kStageStride + c] = ps[e];
@@ -499,19 +531,19 @@ __device__ __forceinline__ void enc_fetch(char* regA, uint32_t t, uint32_t s0,
     }
 }
 
-// After the tile's DMA has landed (every wave waited for its own, then a barrier): copies the
-// valid window out of regA (which conv1's output overwrites) and builds the sequence table,
+// After the tile's DMA has landed (every wave waited for its own, then a barrier): turns the
+// valid window in regA (which conv1's output overwrites) into the layers' ReLU caps (+inf on a
+// valid row, 0 on any other; enc_relu) and builds the sequence table,
 // returning the number of sequences meeting the tile (those from s0 on that start before its
 // end; every thread gets the count after a barrier) and recording the one that runs past its
 // end, if any.
 __device__ __forceinline__ uint32_t enc_put(const char* regA, uint32_t t, uint32_t s0,
-                                            uint64_t off0, uint8_t* mask, EncSeqs& seq,
+                                            uint64_t off0, float* cap, EncSeqs& seq,
                                             const EncPool& pool) {
     const uint32_t tid = threadIdx.x;
     const char* in = regA + kEncIn;
-    if (tid < kValidWindow / 4)
-        reinterpret_cast<uint32_t*>(mask)[tid] =
-            reinterpret_cast<const uint32_t*>(in + kInValid)[tid];
+    if (tid < kValidWindow)
+        cap[tid] = reinterpret_cast<const uint8_t*>(in + kInValid)[tid] ? __builtin_inff() : 0.f;
     bool meets = false;
     if (tid < kEncMaxSeqs && s0 + tid < pool.n) {
         const uint64_t* off = reinterpret_cast<const uint64_t*>(in + kInOff);
@@ -543,7 +575,7 @@ __global__ __launch_bounds__(256, 2) void encode_kernel(const __bf16* __restrict
                                                         uint32_t tiles, EncPool pool) {
     __shared__ __attribute__((aligned(16))) char regA[kEncRegionA];
     __shared__ __attribute__((aligned(16))) char regB[kEncRegionB];
-    __shared__ __attribute__((aligned(16))) uint8_t mask[kValidWindow];  // frames at 0, 2, 4
+    __shared__ __attribute__((aligned(16))) float cap[kValidWindow];  // frames at 0, 2, 4
     __shared__ EncSeqs seq;
     // the layers' biases (LDS reads in the epilogues: a global load there would wait for the
     // weight ring's loads issued before it)
@@ -577,33 +609,35 @@ __global__ __launch_bounds__(256, 2) void encode_kernel(const __bf16* __restrict
     for (;;) {
         wait_vmcnt<0>();  // this wave's DMA of the tile's inputs (and the ring) has landed
         __syncthreads();
-        const uint32_t cnt = enc_put(regA, t, s0, off0, mask, seq, pool);
+        const uint32_t cnt = enc_put(regA, t, s0, off0, cap, seq, pool);
         const uint32_t tn = t + gridDim.x;
         const bool more = tn < tiles;
         const uint32_t s0n = more ? pool.tile_first[tn] : 0;
         {
             f32x16 acc[5];
             enc_layer<kEmbDim, 5, false, kChannels>(regA + kEncIn + kInX0, kXStride, kH1Rows - 1,
-                                                    B + kOffWt1, B + kOffWt2, ring, acc);
+                                                    B + kOffWt1, B + kOffWt2,
+                                                    enc_bias_channels(bias[0]), ring, acc);
             __syncthreads();  // every wave is done reading X0 before H1 overwrites it
-            enc_store<5, 1>(acc, bias[0], mask, regA);
+            enc_store<5, 1>(acc, cap, regA);
         }
         __syncthreads();
         {
             f32x16 acc[5];
             enc_layer<kChannels, 5, false, kChannels>(regA, kHStride, kH2Rows - 1, B + kOffWt2,
-                                                      B + kOffWt3, ring, acc);
-            enc_store<5, 1>(acc, bias[1], mask + 2, regB);
+                                                      B + kOffWt3, enc_bias_channels(bias[1]),
+                                                      ring, acc);
+            enc_store<5, 1>(acc, cap + 2, regB);
         }
         __syncthreads();  // H2 is complete and every wave is done reading H1
         if (more) enc_fetch(regA, tn, s0n, src);
         {
             f32x16 acc[4];
             enc_layer<kChannels, 4, true, kEmbDim>(regB, kHStride, kEncRows - 1, B + kOffWt3,
-                                                   B + kOffWt1, ring, acc);
+                                                   B + kOffWt1, enc_bias_lane(bias[2]), ring,
+                                                   acc);
             __syncthreads();  // every wave is done reading H2 before the pooling stage overwrites it
-            enc_pool(acc, bias[2], mask + 4, seq, cnt, s0, t,
-                     reinterpret_cast<float*>(regB), pool);
+            enc_pool(acc, cap + 4, seq, cnt, s0, t, reinterpret_cast<float*>(regB), pool);
         }
         if (!more) break;
         t = tn;

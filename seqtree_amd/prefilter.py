@@ -237,10 +237,14 @@ class Prefilter:
         check(rc, st, "st_prefilter_train_step")
         return float(loss.value)
 
-    def encode(self, seqs: DeviceSet, max_len: int | None = None):
-        """(embeddings n x 128 BF16 as a uint16 device array, squared norms float32)."""
-        emb = DeviceArray((seqs.n, EMB), np.uint16)
-        norm = DeviceArray(seqs.n, np.float32)
+    def encode(self, seqs: DeviceSet, max_len: int | None = None, out: tuple | None = None):
+        """(embeddings n x 128 BF16 as a uint16 device array, squared norms float32). `out`
+        may pass (embeddings, norms) buffers of at least n rows to write into."""
+        if out is not None and out[0].shape[0] >= seqs.n and out[1].shape[0] >= seqs.n:
+            emb, norm = out
+        else:
+            emb = DeviceArray((seqs.n, EMB), np.uint16)
+            norm = DeviceArray(seqs.n, np.float32)
         st = Status()
         rc = self._L.st_prefilter_encode(self._h, seqs.codes.ptr, seqs.offsets.ptr, seqs.n,
                                          max_len or seqs.max_len, emb.ptr, norm.ptr, None,
@@ -493,13 +497,18 @@ class Searcher:
         self._hits = None
         self._hit_cap = hit_capacity
         self.refs = None
+        self.refs_built = False
+        self._q_buffers = None
         self._count = ctypes.c_uint64()
         self._audit_buffers: dict = {}
 
     def index(self, r: DeviceSet) -> float:
         t0 = time.perf_counter()
         self.refs = r
-        self.r_emb, self.r_norm = self.model.encode(r, self.max_len)
+        # the buffers of the previous index are written over when they hold r
+        prev = (self.r_emb, self.r_norm) if self.refs_built else None
+        self.r_emb, self.r_norm = self.model.encode(r, self.max_len, out=prev)
+        self.refs_built = True
         _device.synchronize()
         return time.perf_counter() - t0
 
@@ -515,7 +524,8 @@ class Searcher:
         L = _bind()
         t = {}
         t0 = time.perf_counter()
-        qe, qn = self.model.encode(q, self.max_len)
+        qe, qn = self.model.encode(q, self.max_len, out=self._q_buffers)
+        self._q_buffers = (qe, qn)
         _device.synchronize()
         t["encode_queries"] = time.perf_counter() - t0
         tau = float(tau_score)
@@ -523,16 +533,20 @@ class Searcher:
         if audit > 0:
             t0 = time.perf_counter()
             n_audit = max(1, min(q.n, int(round(audit * q.n))))
-            order = np.random.default_rng(seed).permutation(q.n)
-            idx = np.sort(order[:n_audit])
+            rng = np.random.default_rng(seed)
+            # a sample without replacement drawn directly (a permutation of all the queries
+            # costs more host time than the audit's exact pass saves on the device)
+            idx = np.sort(rng.choice(q.n, n_audit, replace=False))
             aq, ar, _ = exact_threshold(q.subset(idx), self.refs, cutoff, self.gp,
                                         capacity=1 << 18, buffers=self._audit_buffers)
             pq, pr = idx[aq], ar
             cap = max(n_audit, int(AUDIT_MAX_SHARE * q.n))
             if len(aq) < AUDIT_MIN_HITS and n_audit < cap:
-                # more queries of the same random order, sized by the hit rate seen so far
+                # more queries not yet sampled, sized by the hit rate seen so far
                 want = n_audit * AUDIT_MIN_HITS / max(len(aq), 1)
-                more = np.sort(order[n_audit:min(cap, int(np.ceil(want)))])
+                rest = np.setdiff1d(np.arange(q.n), idx, assume_unique=True)
+                k = min(cap, int(np.ceil(want))) - n_audit
+                more = np.sort(rng.choice(rest, min(k, len(rest)), replace=False))
                 if len(more):
                     bq, br, _ = exact_threshold(q.subset(more), self.refs, cutoff, self.gp,
                                                 capacity=1 << 18, buffers=self._audit_buffers)
