@@ -66,31 +66,40 @@ __device__ __forceinline__ uint64_t start_row(const uint64_t* off, uint32_t s) {
     return (off[s] - off[0]) + 2ull * s;
 }
 
-// One thread per (sequence, channel): the sequence's rows and its two zero rows. With
-// tile_first, also records for every 128-row tile whose first row falls in this sequence's
-// span (its rows and two zero rows; spans partition the used rows) the sequence index.
+// Four threads per sequence, one per 16-byte piece (8 channels) of the 64-byte rows: the
+// sequence's rows and its two zero rows, with the table in LDS. With tile_first, also records
+// for every 128-row tile whose first row falls in this sequence's span (its rows and two zero
+// rows; spans partition the used rows) the sequence index.
+constexpr uint32_t kEmbPieces = kEmbDim / 8;
 __global__ __launch_bounds__(256) void embed_kernel(const uint8_t* __restrict__ codes,
                                                     const uint64_t* __restrict__ off, uint32_t n,
                                                     const __bf16* __restrict__ emb,
                                                     __bf16* __restrict__ X0,
                                                     uint8_t* __restrict__ valid,
                                                     uint32_t* __restrict__ tile_first) {
+    __shared__ bf16x8 table[kEmbRows * kEmbPieces];
+    for (uint32_t e = threadIdx.x; e < kEmbRows * kEmbPieces; e += 256)
+        table[e] = reinterpret_cast<const bf16x8*>(emb)[e];
+    __syncthreads();
     const uint64_t idx = uint64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (idx >= uint64_t(n) * kEmbDim) return;
-    const uint32_t s = uint32_t(idx / kEmbDim), c = uint32_t(idx % kEmbDim);
+    if (idx >= uint64_t(n) * kEmbPieces) return;
+    const uint32_t s = uint32_t(idx / kEmbPieces), piece = uint32_t(idx % kEmbPieces);
     const uint64_t o = off[s];
     const uint32_t len = uint32_t(off[s + 1] - o);
     const uint64_t r0 = start_row(off, s);
-    if (tile_first && c == 0) {
+    if (tile_first && piece == 0) {
         const uint64_t t = (r0 + kEncTile - 1) / kEncTile;  // spans are shorter than a tile
         if (t * kEncTile < r0 + len + 2) tile_first[t] = s;
     }
-    for (uint32_t p = 0; p < len + 2; ++p) {
-        const bool ok = p < len;
-        X0[(r0 + p) * kEmbDim + c] =
-            ok ? emb[(uint32_t(codes[o + p]) % kEmbRows) * kEmbDim + c] : f2bf(0.f);
-        if (c == 0) valid[r0 + p] = ok ? 1 : 0;
-    }
+    bf16x8* dst = reinterpret_cast<bf16x8*>(X0) + r0 * kEmbPieces + piece;
+    const bf16x8* tp = table + piece;
+#pragma unroll 4
+    for (uint32_t p = 0; p < len; ++p)
+        dst[uint64_t(p) * kEmbPieces] = tp[(uint32_t(codes[o + p]) % kEmbRows) * kEmbPieces];
+    dst[uint64_t(len) * kEmbPieces] = bf16x8{};
+    dst[uint64_t(len + 1) * kEmbPieces] = bf16x8{};
+    // the valid bytes: each of the four threads takes every fourth row
+    for (uint32_t p = piece; p < len + 2; p += kEmbPieces) valid[r0 + p] = p < len ? 1 : 0;
 }
 
 // ---- convolution GEMM (MFMA) -----------------------------------------------------------
@@ -1360,7 +1369,7 @@ void launch_pair_distance(const __bf16* Q, const float* q_norm, const __bf16* Rm
 
 void launch_embed(const uint8_t* codes, const uint64_t* off, uint32_t n, const __bf16* emb,
                   __bf16* X0, uint8_t* valid, uint32_t* tile_first, hipStream_t s) {
-    const uint64_t total = uint64_t(n) * kEmbDim;
+    const uint64_t total = uint64_t(n) * kEmbPieces;
     if (total == 0) return;
     hipLaunchKernelGGL(embed_kernel, dim3(uint32_t((total + 255) / 256)), dim3(256), 0, s, codes,
                        off, n, emb, X0, valid, tile_first);
