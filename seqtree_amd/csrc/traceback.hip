@@ -89,6 +89,29 @@ __device__ __forceinline__ void skip_pair(const TracebackArgs& a, uint64_t p) {
 // [W][64] and the three DP rows [3][W][64]; otherwise rows in `rows_g` (3 * W * 64 T per
 // workgroup) and codes read from the CSR set. dirs: (q_max + 1) * W * 64 bytes per workgroup;
 // W = r_max + 1.
+// The residues r[0 .. n), 1 <= n <= NC, as little-endian bytes of NC / 4 words, zero past n:
+// NC / 4 + 1 aligned dword loads, none behind a branch (each dword holds at least one of the
+// n bytes, the ones past the last clamped to it, so no read leaves the pages the residues lie
+// in), all in flight at once, then realigned with v_alignbyte.
+template <uint32_t NC>
+__device__ __forceinline__ void load_codes(const uint8_t* r, uint32_t n, uint32_t (&rcw)[NC / 4]) {
+    const uint64_t addr = reinterpret_cast<uint64_t>(r);
+    const uint64_t a0 = addr & ~uint64_t(3), last = (addr + n - 1) & ~uint64_t(3);
+    const uint32_t sh = uint32_t(addr & 3u);
+    uint32_t d[NC / 4 + 1];
+#pragma unroll
+    for (uint32_t w = 0; w <= NC / 4; ++w) {
+        const uint64_t aw = a0 + 4 * w;
+        d[w] = *reinterpret_cast<const uint32_t*>(aw <= last ? aw : last);
+    }
+#pragma unroll
+    for (uint32_t w = 0; w < NC / 4; ++w) {
+        const uint32_t have = n > 4 * w ? n - 4 * w : 0u;  // residues in word w
+        const uint32_t keep = have >= 4 ? 0xFFFFFFFFu : (1u << (8 * have)) - 1u;
+        rcw[w] = __builtin_amdgcn_alignbyte(d[w + 1], d[w], sh) & keep;
+    }
+}
+
 template <class T, bool LOCAL, bool LDS_ROWS>
 __global__ __launch_bounds__(kTbLanes) void traceback_kernel(
     TracebackArgs a, T* __restrict__ rows_g, uint8_t* __restrict__ dirs_g) {
@@ -288,11 +311,7 @@ __global__ __launch_bounds__(kTbLanes) __attribute__((amdgpu_waves_per_eu(3, 8))
             }
         } else {
             uint32_t rcw[NC / 4];
-#pragma unroll
-            for (uint32_t w = 0; w < NC / 4; ++w) rcw[w] = 0;
-#pragma unroll
-            for (uint32_t c = 0; c < NC; ++c)
-                if (c < n) rcw[c / 4] |= uint32_t(r[c]) << (8 * (c % 4));
+            load_codes<NC>(r, n, rcw);
             int32_t rm[NC + 1], rx[NC + 1], ry[NC + 1];  // the row above, updated in place
             rm[0] = 0;
             rx[0] = NEG;
@@ -496,11 +515,7 @@ __global__ __launch_bounds__(kTbLanes) __attribute__((amdgpu_waves_per_eu(3, 8))
             }
         } else {
             uint32_t rcw[NC / 4];
-#pragma unroll
-            for (uint32_t w = 0; w < NC / 4; ++w) rcw[w] = 0;
-#pragma unroll
-            for (uint32_t c = 0; c < NC; ++c)
-                if (c < n) rcw[c / 4] |= uint32_t(r[c]) << (8 * (c % 4));
+            load_codes<NC>(r, n, rcw);
             int32_t rm[NC + 1], rx[NC + 1], ry[NC + 1];  // the row above, updated in place
             rm[0] = 0;
             rx[0] = NEG;
