@@ -463,7 +463,7 @@ __global__ __launch_bounds__(kTbLanes) __attribute__((amdgpu_waves_per_eu(3, 8))
 // its first step; local: M). Same recurrences, results, ops and ties as traceback_reg_kernel,
 // with half its direction stores and none for row 0.
 template <bool LOCAL>
-__global__ __launch_bounds__(kTbLanes) __attribute__((amdgpu_waves_per_eu(3, 8))) void traceback_nib_kernel(TracebackArgs a,
+__global__ __launch_bounds__(kTbLanes) __attribute__((amdgpu_waves_per_eu(LOCAL ? 3 : 2, 8))) void traceback_nib_kernel(TracebackArgs a,
                                                                  uint32_t* __restrict__ dirs_g) {
     constexpr uint32_t NC = kTbRegCols;
     constexpr uint32_t RW = kTbNibDwords;
