@@ -75,6 +75,32 @@ def test_align_pairs_batch_equals_the_oracle(gpu, oracle_lib):
             assert (g.score, g.ops, g.aligned_query, g.aligned_ref) == (s, ops, aq, ar), t
 
 
+@pytest.mark.parametrize("go,ge", [(11, 1), (1, 3)])
+def test_reference_residues_at_every_alignment(gpu, oracle_lib, go, ge):
+    """The register-row kernels read each reference's residues as aligned dwords: references of
+    1 to 12 residues start at every byte offset modulo 4, the set ends with references of 1 to
+    3 residues (the last dword holds the set's last byte), and every (query, reference) pair
+    must equal the oracle (four-bit kernel for gap_open >= gap_extend, byte kernel otherwise)."""
+    st = gpu
+    m = st.SubstitutionMatrix.blosum62()
+    sim = _sim(st, m)
+    rng = random.Random(seed_of("align", go, ge))
+    r = [rand_seq(rng, n, n) for n in list(range(1, 13)) * 3] + ["W", "CA", "KKQ"]
+    q = [rand_seq(rng, n, n) for n in (1, 2, 5, 9, 16, 25)]
+    pq = np.repeat(np.arange(len(q)), len(r)).astype(np.uint32)
+    pr = np.tile(np.arange(len(r)), len(q)).astype(np.uint32)
+    for mode in ("global", "local"):
+        got, labels = launch_labels(lambda: st.pairwise.align_pairs(
+            q, r, pq, pr, m, mode=mode, gap_open=go, gap_extend=ge))
+        assert f"traceback_{'nib' if go >= ge else 'reg'}_{mode}" in labels, labels
+        for t in range(len(pq)):
+            a, b = q[pq[t]], r[pr[t]]
+            s, ops, qb, rb = oracle_lib.align_pair(a, b, sim, mode == "local", go, ge)
+            aq, ar = _strings(a, b, ops, qb, rb)
+            g = got[t]
+            assert (g.score, g.ops, g.aligned_query, g.aligned_ref) == (s, ops, aq, ar), (mode, a, b)
+
+
 def test_large_lists_run_in_shape_order(gpu, oracle_lib):
     """From 4,096 pairs on, the pairs are processed in shape order (sorted by clamped
     lengths); every output must still land at its own pair's index. Lengths past the clamp
